@@ -1,0 +1,124 @@
+"""Benchmark: brute-force SAT gate-evaluation rate on one MI355X.
+
+A step is one complete search of the 36-input, 20,000-gate SAT workload (seed 2, 40-bit
+output word, planted 36-bit key): all 2^36 assignments evaluated bit-sliced on the GPU,
+satisfying count and lowest assignment reduced on the device. value = gate-evals/s =
+2^36 x (gates in the output cone) / seconds per step.
+
+Also reported on the same line:
+  roofline      32-bit VALU op rate of the search kernel (one op per gate per 32
+                assignments) against the gfx950 VALU peak, from HIP events around the
+                launches on the engine's stream;
+  cpu_baseline  the CPU oracle (C, AVX2, pthreads over the host cores) on a bounded slice
+                of the same search;
+  extra         the SATOSTOP table-pass HBM rate (30-bit Feistel, 2^30-entry table) and the
+                QSAT search time (32 inputs, 10,000 gates).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-extra]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "gate-evals/s vs VALU peak; batched solves/s + p99 latency vs one-at-a-time"
+VALU_PEAK_OPS = 256 * 64 * 2.4e9  # 256 CUs x 64 lanes/clk x 2.4 GHz, 32-bit VALU ops/s
+HBM_PEAK_GBS = 8000.0
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(the search shards by assignment range, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import naivesat_amd as ns
+
+    ctx = ns.Context(0)
+    c, key = ns.Circuit.gen_sat(2, 36, 20000, 40)
+    cone, peak_live = c.cone()
+    n = 36
+    for _ in range(args.warmup):
+        r = ctx.sat_solve(c)
+    t0 = time.perf_counter()
+    kernel_ms, launches = 0.0, 0
+    for _ in range(args.steps):
+        r = ctx.sat_solve(c)
+        kernel_ms += r["kernel_ms"]
+        launches += r["launches"]
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1000.0 / args.steps
+    gate_evals = (1 << n) * cone
+    value = gate_evals / (ms_per_step / 1000.0)
+    # roofline of the search kernel: ops per launch / average launch duration
+    avg_launch_ms = kernel_ms / launches
+    words_per_launch = (1 << (n - 5)) / (launches / args.steps)
+    ops_per_launch = words_per_launch * cone
+    achieved_tops = ops_per_launch / (avg_launch_ms / 1000.0) / 1e12
+    peak_tops = VALU_PEAK_OPS / 1e12
+    line = {
+        "metric": METRIC, "value": value, "unit": "gate-evals/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u32", "data": "synthetic",
+        "config": {"workload": "SAT n=36 inputs, 20000 random AND/OR/XOR gates (seed 2), output "
+                               "[W(x)==W(k)] over a 40-gate word, planted 36-bit key",
+                   "cone_gates": cone, "peak_live_values": peak_live, "assignments_per_step": 1 << n,
+                   "launches_per_step": launches // args.steps, "count": r["count"], "lowest": r["lowest"],
+                   "key": key},
+        "roofline": {"bound": "valu", "achieved": achieved_tops, "peak": peak_tops, "unit": "TOP/s",
+                     "frac": achieved_tops / peak_tops, "traffic": None,
+                     "kernel": "ns_sat", "avg_launch_ms": avg_launch_ms,
+                     "ops_per_launch": ops_per_launch},
+    }
+    if not args.no_cpu:
+        import oracle
+
+        threads = args.cpu_threads or min(os.cpu_count() or 1, 64)
+        oc = oracle.OracleCircuit.from_dict(c.arrays())
+        words = 1 << 25  # 2^31 assignments (64 per word)
+        tc0 = time.perf_counter()
+        oc.sat_range(0, words, threads=threads)
+        tc = time.perf_counter() - tc0
+        line["cpu_baseline"] = {"value": (words * 64) * cone / tc, "unit": "gate-evals/s", "cores": threads,
+                                "kind": "port", "sample": f"first 2^31 of the 2^36 assignments, {tc:.1f} s"}
+    if not args.no_extra:
+        extra = {}
+        cs, target = ns.Circuit.gen_satostop(4, 15, 6, 300, 14)
+        rs = ctx.satostop_solve(cs, 14, arrays=False)
+        passes = ctx.pass_times()
+        jump = passes[1:]  # u64 passes (the first reads the u32 table)
+        entries = 1 << 30
+        bytes_pass = entries * 24  # read entry + gathered entry + write, 8 B each
+        avg = sum(jump) / len(jump)
+        extra["satostop_table"] = {
+            "workload": "SATOSTOP n=30 Feistel (6 rounds x 300 gates, seed 4), 14 unknowns, 2^30-entry table",
+            "build_ms": rs["build_ms"], "jump_ms": rs["jump_ms"], "passes": rs["passes"],
+            "n_stopped": rs["n_stopped"], "avg_pass_ms": avg,
+            "roofline": {"bound": "hbm", "achieved": bytes_pass / (avg / 1000.0) / 1e9, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": bytes_pass / (avg / 1000.0) / 1e9 / HBM_PEAK_GBS,
+                         "traffic": None, "kernel": "k_jump_pass", "bytes_per_entry": 24}}
+        cq = ns.Circuit.gen_qsat(3, 32, 10000, 200)
+        q = [0] * 8 + [1] * 8 + [0] * 8 + [1] * 8
+        ctx.qsat_solve(cq, q)
+        rq = ctx.qsat_solve(cq, q)
+        extra["qsat"] = {"workload": "QSAT n=32, 10000 gates (seed 3), E8 A8 E8 A8", "kernel_ms": rq["kernel_ms"],
+                         "truth": rq["truth"], "result": ns.qsat_result_string(rq), "cone_gates": rq["cone_gates"],
+                         "gate_evals_per_s": (1 << 32) * rq["cone_gates"] / (rq["kernel_ms"] / 1000.0)}
+        line["extra"] = extra
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

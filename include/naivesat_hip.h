@@ -1,0 +1,189 @@
+/*
+ * naivesat_hip — C ABI of the MI355X (gfx950) brute-force SAT / QSAT / SATOSTOP engine.
+ *
+ * Every entry point returns an int status (NS_OK on success) and never throws or aborts;
+ * the message of the last failure on the calling thread is ns_last_error(). Handles are
+ * opaque. A context owns one HIP stream and its device buffers; use a context from one
+ * thread at a time. Result structs are caller-allocated.
+ *
+ * The reference (matszpk/naivesat 0.1.3, Rust) reaches its hot path through the external
+ * `gatenative` builder/executor API and `gatesim` circuits; each entry point below names
+ * the reference interface it replaces.
+ */
+#ifndef NAIVESAT_HIP_H
+#define NAIVESAT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NS_OK 0
+#define NS_ERR_INVALID 1     /* bad argument or malformed circuit */
+#define NS_ERR_HIP 2         /* HIP runtime failure */
+#define NS_ERR_COMPILE 3     /* run-time kernel compilation failed */
+#define NS_ERR_UNSUPPORTED 4 /* size or device outside what the engine handles */
+#define NS_ERR_NOMEM 5       /* device or host allocation failed */
+#define NS_ERR_INTERNAL 6
+
+/* gate functions (gatesim's gate set) */
+#define NS_FUNC_AND 0   /* a & b    */
+#define NS_FUNC_NOR 1   /* ~(a | b) */
+#define NS_FUNC_NIMPL 2 /* a & ~b   */
+#define NS_FUNC_XOR 3   /* a ^ b    */
+
+typedef struct ns_circuit ns_circuit;
+typedef struct ns_ctx ns_ctx;
+
+const char* ns_last_error(void);
+const char* ns_version(void);
+
+/* ---- circuits (replace gatesim `Circuit<usize>`: src/main.rs:6,391; src/qsat.rs:6277-6285) ---- */
+
+/* Parses the circuit text form "{0 1 2 and(0,1) xor(2,3):0n}" (replaces
+ * `Circuit::<usize>::from_str`, src/main.rs:391). */
+int ns_circuit_parse(const char* text, size_t len, ns_circuit** out);
+/* Parses "eaa {circuit}" (one quantifier letter per input, input 0 first) into the
+ * circuit plus quantifiers (quants_all[k] = 1 for 'a'); *is_quant = 0 and a plain parse
+ * when the text has no quantifier prefix (replaces `QuantCircuit::from_str` with the
+ * all-exists fallback, src/qsat.rs:6277-6285). quants_all needs room for the input count. */
+int ns_quant_circuit_parse(const char* text, size_t len, ns_circuit** out, uint8_t* quants_all,
+                           size_t quants_cap, int* is_quant);
+/* Builds a circuit from arrays: gate j drives wire n_inputs + j (replaces
+ * `Circuit::new`, src/satostop.rs:1793). */
+int ns_circuit_from_arrays(uint32_t n_inputs, uint32_t n_gates, const uint32_t* in0,
+                           const uint32_t* in1, const uint8_t* func, uint32_t n_outputs,
+                           const uint32_t* out_wire, const uint8_t* out_neg, ns_circuit** out);
+int ns_circuit_info(const ns_circuit* c, uint32_t* n_inputs, uint32_t* n_gates, uint32_t* n_outputs);
+/* Copies the circuit into caller arrays sized from ns_circuit_info (any may be NULL). */
+int ns_circuit_to_arrays(const ns_circuit* c, uint32_t* in0, uint32_t* in1, uint8_t* func,
+                         uint32_t* out_wire, uint8_t* out_neg);
+/* Text form; *needed gets the length without the terminating NUL. */
+int ns_circuit_format(const ns_circuit* c, char* buf, size_t cap, size_t* needed);
+/* Scalar evaluation on the host (replaces `Circuit::eval`, src/main.rs:374), used to verify
+ * answers the way the reference's CLIs do. inputs[k], outputs[i] are 0/1. */
+int ns_circuit_eval(const ns_circuit* c, const uint8_t* inputs, uint8_t* outputs);
+/* Gates in the output cone and the peak live values of the kernel's evaluation order. */
+int ns_circuit_cone(const ns_circuit* c, uint32_t* cone_gates, uint32_t* peak_live);
+void ns_circuit_free(ns_circuit* c);
+
+/* ---- synthetic workloads (deterministic in the seed; SplitMix64 stream) ---- */
+int ns_gen_sat(uint64_t seed, uint32_t n, uint32_t gates, uint32_t word_bits, ns_circuit** out,
+               uint64_t* key);
+int ns_gen_qsat(uint64_t seed, uint32_t n, uint32_t gates, uint32_t last, ns_circuit** out);
+int ns_gen_satostop(uint64_t seed, uint32_t half, uint32_t rounds, uint32_t round_gates,
+                    uint32_t unknowns, ns_circuit** out, uint64_t* target);
+
+/* ---- device context (replaces OpenCLBuilder::new(&device, cfg) + its command queue,
+ *      src/main.rs:292-301) ---- */
+int ns_ctx_create(int device, ns_ctx** out);
+void ns_ctx_destroy(ns_ctx* ctx);
+
+/* ---- SAT (replaces the naivesat mapper run: BasicMapperBuilder + OpenCLBuilder with
+ *      AGGR_OUTPUT_OPENCL_CODE and the per-arg callback, src/main.rs:91-104,157-201) ----
+ * Input k of the circuit is bit k of the assignment x; the circuit must have one output.
+ * Finds the number of satisfying assignments and the lowest one. */
+#define NS_SAT_STOP_AT_FIRST 1u /* stop after the first launch range holding a solution */
+typedef struct {
+  uint32_t elem_bits;      /* log2 assignments per launch (0 = engine default) */
+  uint32_t group_len;      /* threads per block, multiple of 64 (0 = default) */
+  uint32_t words_per_lane; /* 32-assignment words per lane, power of two (0 = default) */
+  uint32_t flags;
+} ns_sat_opts;
+typedef struct {
+  uint64_t count;     /* satisfying assignments in the searched range */
+  uint64_t lowest;    /* lowest satisfying assignment (valid when found) */
+  uint32_t found;
+  uint32_t complete;  /* 1 when the whole 2^n space was searched */
+  uint32_t launches;
+  uint32_t cone_gates;
+  double kernel_ms;   /* device time of the search launches (HIP events) */
+  double compile_ms;  /* run-time compilation (0 when cached) */
+  double wall_ms;
+} ns_sat_result;
+int ns_sat_solve(ns_ctx* ctx, const ns_circuit* c, const ns_sat_opts* opts, ns_sat_result* res);
+
+/* ---- QSAT (replaces the naiveqsat run: AGGR_OUTPUT_OPENCL_CODE tree + OpenCLQuantReducer +
+ *      QuantReducer, src/qsat.rs:172-291,405-720,942-1321,1658-1720) ----
+ * quants_all[k] = 1 when input k is universally quantified; input 0 is outermost. The
+ * outermost block is the maximal prefix of equal quantifiers (first_bits inputs). For each
+ * value u of that block (u's most significant bit = input 0) the formula below it is
+ * reduced; truth = OR (exists) / AND (all) over u. The witness is the lowest u whose value
+ * decides the truth (1 for exists, 0 for all) — the solution the reference prints. */
+typedef struct {
+  uint32_t elem_bits;
+  uint32_t group_len;
+  uint32_t words_per_lane;
+  uint32_t flags;
+} ns_qsat_opts;
+typedef struct {
+  uint32_t truth;
+  uint32_t first_bits;  /* size of the outermost block */
+  uint32_t first_all;   /* outermost quantifier is 'all' */
+  uint32_t has_witness;
+  uint64_t witness;       /* lowest witness u (index order, input 0 = MSB) */
+  uint64_t witness_count; /* number of u values that are witnesses */
+  uint32_t passes;        /* reduction passes after the circuit kernel */
+  uint32_t cone_gates;
+  double kernel_ms;
+  double compile_ms;
+  double wall_ms;
+} ns_qsat_result;
+/* bitmap (optional): 2^first_bits bits, bit u = value for block value u; bitmap_words
+ * u32 words of room (needs ceil(2^first_bits / 32)). */
+int ns_qsat_solve(ns_ctx* ctx, const ns_circuit* c, const uint8_t* quants_all,
+                  const ns_qsat_opts* opts, ns_qsat_result* res, uint32_t* bitmap,
+                  uint64_t bitmap_words);
+
+/* ---- SATOSTOP on a device-resident next-state table (replaces greedysatostop's
+ *      do_solve_with_opencl_builder: transpose epilogue + check_stop / join_nexts /
+ *      find_solution, src/greedysatostop.rs:841-1099; and the answer contract of
+ *      naivesatostop, src/satostop.rs:322-327,1650-1782) ----
+ * The circuit has n inputs (state) and n+1 outputs: outputs 0..n-1 are the next state,
+ * output n is STOP. Unknowns are the top `unknowns` state bits; start state of unknown u
+ * is u << (n - unknowns). For every u the engine finds whether iterating the circuit from
+ * its start reaches STOP and after how many evaluations (the stopping one included). */
+#define NS_STOP_FIRST_PASS 1u /* end after the first jump pass where some start stopped */
+typedef struct {
+  uint32_t flags;
+  uint32_t max_passes;     /* 0 = n (enough for every path) */
+  uint32_t group_len;      /* table build block size (0 = default) */
+  uint32_t words_per_lane; /* table build words per lane (0 = default) */
+} ns_satostop_opts;
+typedef struct {
+  uint32_t any_stop;   /* some state's STOP output is 1 */
+  uint32_t passes;     /* jump passes run */
+  uint64_t n_stopped;  /* unknown starts that stop */
+  uint64_t lowest_u;   /* lowest stopping unknown (valid when n_stopped > 0) */
+  uint64_t start;      /* its start state */
+  uint64_t end;        /* state produced by the stopping evaluation */
+  uint64_t steps;      /* evaluations until STOP, the stopping one included */
+  uint32_t cone_gates;
+  uint32_t reserved;
+  double build_ms;     /* table build kernel time */
+  double jump_ms;      /* total jump-pass kernel time */
+  double compile_ms;
+  double wall_ms;
+} ns_satostop_result;
+/* steps_out / ends_out (optional): 2^unknowns entries each; steps 0 = never stops. */
+int ns_satostop_solve(ns_ctx* ctx, const ns_circuit* c, uint32_t unknowns,
+                      const ns_satostop_opts* opts, ns_satostop_result* res, uint32_t* steps_out,
+                      uint32_t* ends_out);
+
+/* Table build alone (replaces greedysatostop's first-nexts launch with the transpose
+ * epilogue, src/greedysatostop.rs:1042-1072; identity layout pinned by
+ * src/satostop.rs:1788-1915): table_out gets 2^n u32, entry s = sum_i output_i(s) << i.
+ * At most 32 outputs and n <= 31. */
+int ns_table_build(ns_ctx* ctx, const ns_circuit* c, uint32_t* table_out, uint64_t table_len,
+                   double* kernel_ms);
+
+/* Per-pass timing of the last ns_satostop_solve on this context: pass_ms[i] for jump pass i
+ * (HIP events on the context stream). Returns the number of passes written. */
+int ns_satostop_pass_times(ns_ctx* ctx, double* pass_ms, uint32_t cap, uint32_t* written);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NAIVESAT_HIP_H */

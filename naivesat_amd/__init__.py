@@ -1,0 +1,352 @@
+"""MI355X (gfx950) brute-force SAT / QSAT / SATOSTOP engine — Python host binding.
+
+Thin ctypes layer over ``libnaivesat_hip.so`` (the C ABI declared in
+``include/naivesat_hip.h``). There is no CPU fallback: every solve runs the HIP
+kernels, and a missing library or device raises.
+
+The solver entry points mirror the reference's search runs:
+  * ``Context.sat_solve``      — naivesat (src/main.rs:157-201, 266-382)
+  * ``Context.qsat_solve``     — naiveqsat (src/qsat.rs:1658-1720, 1838-2000)
+  * ``Context.satostop_solve`` — greedysatostop's table + pointer jumping
+    (src/greedysatostop.rs:1042-1099) and naivesatostop's answer
+    (src/satostop.rs:322-327, 1650-1782)
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional, Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libnaivesat_hip.so")
+BIN_DIR = os.path.join(_HERE, "bin")
+
+NS_SAT_STOP_AT_FIRST = 1
+NS_STOP_FIRST_PASS = 1
+FUNC_NAMES = ("and", "nor", "nimpl", "xor")
+
+
+class NaivesatError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"naivesat_hip status {status}: {msg}")
+        self.status = status
+
+
+class _SatOpts(C.Structure):
+    _fields_ = [("elem_bits", C.c_uint32), ("group_len", C.c_uint32),
+                ("words_per_lane", C.c_uint32), ("flags", C.c_uint32)]
+
+
+class _SatResult(C.Structure):
+    _fields_ = [("count", C.c_uint64), ("lowest", C.c_uint64), ("found", C.c_uint32),
+                ("complete", C.c_uint32), ("launches", C.c_uint32), ("cone_gates", C.c_uint32),
+                ("kernel_ms", C.c_double), ("compile_ms", C.c_double), ("wall_ms", C.c_double)]
+
+
+_QsatOpts = _SatOpts
+
+
+class _QsatResult(C.Structure):
+    _fields_ = [("truth", C.c_uint32), ("first_bits", C.c_uint32), ("first_all", C.c_uint32),
+                ("has_witness", C.c_uint32), ("witness", C.c_uint64), ("witness_count", C.c_uint64),
+                ("passes", C.c_uint32), ("cone_gates", C.c_uint32), ("kernel_ms", C.c_double),
+                ("compile_ms", C.c_double), ("wall_ms", C.c_double)]
+
+
+class _StopOpts(C.Structure):
+    _fields_ = [("flags", C.c_uint32), ("max_passes", C.c_uint32), ("group_len", C.c_uint32),
+                ("words_per_lane", C.c_uint32)]
+
+
+class _StopResult(C.Structure):
+    _fields_ = [("any_stop", C.c_uint32), ("passes", C.c_uint32), ("n_stopped", C.c_uint64),
+                ("lowest_u", C.c_uint64), ("start", C.c_uint64), ("end", C.c_uint64),
+                ("steps", C.c_uint64), ("cone_gates", C.c_uint32), ("reserved", C.c_uint32),
+                ("build_ms", C.c_double), ("jump_ms", C.c_double), ("compile_ms", C.c_double),
+                ("wall_ms", C.c_double)]
+
+
+_lib = None
+
+
+def load_library() -> C.CDLL:
+    """Loads libnaivesat_hip.so (raises when it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+    lib = C.CDLL(LIB_PATH)
+    P, U32, U64, I = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int
+    sig = {
+        "ns_last_error": (C.c_char_p, []),
+        "ns_version": (C.c_char_p, []),
+        "ns_circuit_parse": (I, [C.c_char_p, C.c_size_t, C.POINTER(P)]),
+        "ns_quant_circuit_parse": (I, [C.c_char_p, C.c_size_t, C.POINTER(P), P, C.c_size_t, C.POINTER(I)]),
+        "ns_circuit_from_arrays": (I, [U32, U32, P, P, P, U32, P, P, C.POINTER(P)]),
+        "ns_circuit_info": (I, [P, C.POINTER(U32), C.POINTER(U32), C.POINTER(U32)]),
+        "ns_circuit_to_arrays": (I, [P, P, P, P, P, P]),
+        "ns_circuit_format": (I, [P, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
+        "ns_circuit_eval": (I, [P, P, P]),
+        "ns_circuit_cone": (I, [P, C.POINTER(U32), C.POINTER(U32)]),
+        "ns_circuit_free": (None, [P]),
+        "ns_gen_sat": (I, [U64, U32, U32, U32, C.POINTER(P), C.POINTER(U64)]),
+        "ns_gen_qsat": (I, [U64, U32, U32, U32, C.POINTER(P)]),
+        "ns_gen_satostop": (I, [U64, U32, U32, U32, U32, C.POINTER(P), C.POINTER(U64)]),
+        "ns_ctx_create": (I, [I, C.POINTER(P)]),
+        "ns_ctx_destroy": (None, [P]),
+        "ns_sat_solve": (I, [P, P, C.POINTER(_SatOpts), C.POINTER(_SatResult)]),
+        "ns_qsat_solve": (I, [P, P, P, C.POINTER(_QsatOpts), C.POINTER(_QsatResult), P, U64]),
+        "ns_satostop_solve": (I, [P, P, U32, C.POINTER(_StopOpts), C.POINTER(_StopResult), P, P]),
+        "ns_table_build": (I, [P, P, P, U64, C.POINTER(C.c_double)]),
+        "ns_satostop_pass_times": (I, [P, P, U32, C.POINTER(U32)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def exported_symbols() -> list:
+    """Names of the C-ABI entry points this binding declares."""
+    return ["ns_last_error", "ns_version", "ns_circuit_parse", "ns_quant_circuit_parse",
+            "ns_circuit_from_arrays", "ns_circuit_info", "ns_circuit_to_arrays", "ns_circuit_format",
+            "ns_circuit_eval", "ns_circuit_cone", "ns_circuit_free", "ns_gen_sat", "ns_gen_qsat",
+            "ns_gen_satostop", "ns_ctx_create", "ns_ctx_destroy", "ns_sat_solve", "ns_qsat_solve",
+            "ns_satostop_solve", "ns_table_build", "ns_satostop_pass_times"]
+
+
+def _check(status: int) -> None:
+    if status != 0:
+        msg = load_library().ns_last_error().decode(errors="replace")
+        raise NaivesatError(status, msg)
+
+
+def _ptr(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class Circuit:
+    """Gate circuit (gatesim model: inputs 0..n-1, gate j is wire n+j, outputs (wire, neg))."""
+
+    def __init__(self, handle: int):
+        self._h = C.c_void_p(handle)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and _lib is not None:
+            _lib.ns_circuit_free(h)
+            self._h = None
+
+    # ---- constructors ----
+    @classmethod
+    def parse(cls, text: str) -> "Circuit":
+        lib = load_library()
+        h = C.c_void_p()
+        b = text.encode()
+        _check(lib.ns_circuit_parse(b, len(b), C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def parse_quant(cls, text: str):
+        """Returns (circuit, quants_all or None) for "eaa {..}" / plain text."""
+        lib = load_library()
+        h = C.c_void_p()
+        b = text.encode()
+        q = np.zeros(4096, dtype=np.uint8)
+        isq = C.c_int(0)
+        _check(lib.ns_quant_circuit_parse(b, len(b), C.byref(h), _ptr(q), q.size, C.byref(isq)))
+        c = cls(h.value)
+        return c, (q[: c.n_inputs].copy() if isq.value else None)
+
+    @classmethod
+    def from_arrays(cls, n: int, in0: Sequence[int], in1: Sequence[int], func: Sequence[int],
+                    out_wire: Sequence[int], out_neg: Optional[Sequence[int]] = None) -> "Circuit":
+        lib = load_library()
+        a0 = np.ascontiguousarray(in0, dtype=np.uint32)
+        a1 = np.ascontiguousarray(in1, dtype=np.uint32)
+        f = np.ascontiguousarray(func, dtype=np.uint8)
+        ow = np.ascontiguousarray(out_wire, dtype=np.uint32)
+        on = np.ascontiguousarray(out_neg if out_neg is not None else np.zeros(len(ow)), dtype=np.uint8)
+        if not (len(a0) == len(a1) == len(f)) or len(on) != len(ow):
+            raise ValueError("array lengths differ")
+        h = C.c_void_p()
+        _check(lib.ns_circuit_from_arrays(n, len(a0), _ptr(a0), _ptr(a1), _ptr(f), len(ow), _ptr(ow),
+                                          _ptr(on), C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def gen_sat(cls, seed: int, n: int, gates: int, word_bits: int):
+        """SAT workload: output = [W(x) == W(key)], W = last word_bits gates. Returns (c, key)."""
+        lib = load_library()
+        h = C.c_void_p()
+        key = C.c_uint64()
+        _check(lib.ns_gen_sat(seed, n, gates, word_bits, C.byref(h), C.byref(key)))
+        return cls(h.value), key.value
+
+    @classmethod
+    def gen_qsat(cls, seed: int, n: int, gates: int, last: int) -> "Circuit":
+        lib = load_library()
+        h = C.c_void_p()
+        _check(lib.ns_gen_qsat(seed, n, gates, last, C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def gen_satostop(cls, seed: int, half: int, rounds: int, round_gates: int, unknowns: int):
+        """Feistel next-state circuit with STOP = [next == target]. Returns (c, target)."""
+        lib = load_library()
+        h = C.c_void_p()
+        t = C.c_uint64()
+        _check(lib.ns_gen_satostop(seed, half, rounds, round_gates, unknowns, C.byref(h), C.byref(t)))
+        return cls(h.value), t.value
+
+    # ---- accessors ----
+    def _info(self):
+        n, g, o = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        _check(load_library().ns_circuit_info(self._h, C.byref(n), C.byref(g), C.byref(o)))
+        return n.value, g.value, o.value
+
+    @property
+    def n_inputs(self) -> int:
+        return self._info()[0]
+
+    @property
+    def n_gates(self) -> int:
+        return self._info()[1]
+
+    @property
+    def n_outputs(self) -> int:
+        return self._info()[2]
+
+    def arrays(self) -> dict:
+        n, g, o = self._info()
+        in0 = np.zeros(g, np.uint32)
+        in1 = np.zeros(g, np.uint32)
+        func = np.zeros(g, np.uint8)
+        ow = np.zeros(o, np.uint32)
+        on = np.zeros(o, np.uint8)
+        _check(load_library().ns_circuit_to_arrays(self._h, _ptr(in0), _ptr(in1), _ptr(func), _ptr(ow),
+                                                   _ptr(on)))
+        return {"n": n, "in0": in0, "in1": in1, "func": func, "out_wire": ow, "out_neg": on}
+
+    def text(self) -> str:
+        lib = load_library()
+        need = C.c_size_t()
+        _check(lib.ns_circuit_format(self._h, None, 0, C.byref(need)))
+        buf = C.create_string_buffer(need.value + 1)
+        _check(lib.ns_circuit_format(self._h, buf, need.value + 1, C.byref(need)))
+        return buf.value.decode()
+
+    def eval(self, inputs: Sequence[int]) -> list:
+        n, _, o = self._info()
+        a = np.ascontiguousarray(inputs, dtype=np.uint8)
+        if a.size != n:
+            raise ValueError("one value per input is required")
+        out = np.zeros(o, np.uint8)
+        _check(load_library().ns_circuit_eval(self._h, _ptr(a), _ptr(out)))
+        return [int(v) for v in out]
+
+    def cone(self):
+        """(gates in the output cone, peak live values of the kernel's order)."""
+        g, p = C.c_uint32(), C.c_uint32()
+        _check(load_library().ns_circuit_cone(self._h, C.byref(g), C.byref(p)))
+        return g.value, p.value
+
+
+class Context:
+    """One HIP device context (stream + device buffers + compiled-kernel cache)."""
+
+    def __init__(self, device: int = 0):
+        lib = load_library()
+        h = C.c_void_p()
+        _check(lib.ns_ctx_create(device, C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value and _lib is not None:
+            _lib.ns_ctx_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def sat_solve(self, c: Circuit, elem_bits: int = 0, group_len: int = 0, words_per_lane: int = 0,
+                  stop_at_first: bool = False) -> dict:
+        o = _SatOpts(elem_bits, group_len, words_per_lane, NS_SAT_STOP_AT_FIRST if stop_at_first else 0)
+        r = _SatResult()
+        _check(load_library().ns_sat_solve(self._h, c._h, C.byref(o), C.byref(r)))
+        return {k: getattr(r, k) for k, _ in _SatResult._fields_}
+
+    def qsat_solve(self, c: Circuit, quants_all: Sequence[int], elem_bits: int = 0, group_len: int = 0,
+                   words_per_lane: int = 0, bitmap: bool = False) -> dict:
+        q = np.ascontiguousarray(quants_all, dtype=np.uint8)
+        if q.size != c.n_inputs:
+            raise ValueError("one quantifier per input is required")
+        o = _QsatOpts(elem_bits, group_len, words_per_lane, 0)
+        r = _QsatResult()
+        fb = 0
+        while fb < q.size and q[fb] == q[0]:
+            fb += 1
+        bm = np.zeros(max(1, (1 << fb) // 32), np.uint32) if bitmap else None
+        _check(load_library().ns_qsat_solve(self._h, c._h, _ptr(q), C.byref(o), C.byref(r), _ptr(bm),
+                                            0 if bm is None else bm.size))
+        d = {k: getattr(r, k) for k, _ in _QsatResult._fields_}
+        if bm is not None:
+            bits = np.unpackbits(bm.view(np.uint8), bitorder="little")[: 1 << fb]
+            d["bitmap"] = bits
+        return d
+
+    def satostop_solve(self, c: Circuit, unknowns: int, first_pass: bool = False, max_passes: int = 0,
+                       group_len: int = 0, words_per_lane: int = 0, arrays: bool = True) -> dict:
+        o = _StopOpts(NS_STOP_FIRST_PASS if first_pass else 0, max_passes, group_len, words_per_lane)
+        r = _StopResult()
+        steps = np.zeros(1 << unknowns, np.uint32) if arrays else None
+        ends = np.zeros(1 << unknowns, np.uint32) if arrays else None
+        _check(load_library().ns_satostop_solve(self._h, c._h, unknowns, C.byref(o), C.byref(r), _ptr(steps),
+                                                _ptr(ends)))
+        d = {k: getattr(r, k) for k, _ in _StopResult._fields_}
+        if arrays:
+            d["steps_arr"] = steps
+            d["ends_arr"] = ends
+        return d
+
+    def table_build(self, c: Circuit) -> np.ndarray:
+        n = c.n_inputs
+        t = np.zeros(1 << n, np.uint32)
+        ms = C.c_double()
+        _check(load_library().ns_table_build(self._h, c._h, _ptr(t), t.size, C.byref(ms)))
+        return t
+
+    def pass_times(self) -> list:
+        buf = np.zeros(64, np.float64)
+        w = C.c_uint32()
+        _check(load_library().ns_satostop_pass_times(self._h, _ptr(buf), 64, C.byref(w)))
+        return [float(x) for x in buf[: w.value]]
+
+
+def quants_from_string(s: str) -> np.ndarray:
+    """'eeaa' -> [0,0,1,1] (1 = all); other characters are ignored."""
+    return np.array([1 if ch == "a" else 0 for ch in s.lower() if ch in "ae"], dtype=np.uint8)
+
+
+def qsat_result_string(r: dict) -> str:
+    """The reference's printed FinalResult (src/qsat.rs:156-170): the witness is printed
+    with bit b = input b, most significant character first."""
+    fb = r["first_bits"]
+    sol = None
+    if r["has_witness"]:
+        u = r["witness"]
+        sol = int(format(u, f"0{fb}b")[::-1], 2) if fb else 0
+    if r["first_all"]:
+        return f"(Unsatisfiable: {sol:0{fb}b})" if sol is not None else "(Satisfiable)"
+    return f"(Satisfiable: {sol:0{fb}b})" if sol is not None else "(Unsatisfiable)"

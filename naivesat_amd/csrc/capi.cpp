@@ -1,0 +1,254 @@
+// extern "C" boundary: converts exceptions to status codes and keeps the last error
+// message per thread.
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "../../include/naivesat_hip.h"
+#include "circuit.h"
+#include "runtime.h"
+#include "solvers.h"
+
+struct ns_circuit {
+  ns::Circuit c;
+};
+struct ns_ctx {
+  ns::Context ctx;
+  explicit ns_ctx(int d) : ctx(d) {}
+};
+
+namespace {
+thread_local std::string g_err;
+
+template <class F>
+int guard(F&& f) {
+  try {
+    f();
+    g_err.clear();
+    return NS_OK;
+  } catch (const ns::Error& e) {
+    g_err = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_err = "out of host memory";
+    return NS_ERR_NOMEM;
+  } catch (const std::out_of_range& e) {
+    g_err = e.what();
+    return NS_ERR_INVALID;
+  } catch (const std::runtime_error& e) {
+    g_err = e.what();
+    return NS_ERR_INVALID;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return NS_ERR_INTERNAL;
+  } catch (...) {
+    g_err = "unknown error";
+    return NS_ERR_INTERNAL;
+  }
+}
+
+void need(bool ok, const char* what) {
+  if (!ok) throw ns::Error(NS_ERR_INVALID, what);
+}
+}  // namespace
+
+extern "C" {
+
+const char* ns_last_error(void) { return g_err.c_str(); }
+const char* ns_version(void) { return "naivesat_hip 0.1.0 (gfx950)"; }
+
+int ns_circuit_parse(const char* text, size_t len, ns_circuit** out) {
+  return guard([&] {
+    need(text && out, "null argument");
+    auto* h = new ns_circuit{ns::parse_circuit(std::string(text, len))};
+    *out = h;
+  });
+}
+
+int ns_quant_circuit_parse(const char* text, size_t len, ns_circuit** out, uint8_t* quants_all,
+                           size_t quants_cap, int* is_quant) {
+  return guard([&] {
+    need(text && out && is_quant, "null argument");
+    std::string s(text, len);
+    std::vector<uint8_t> q;
+    ns::Circuit c;
+    if (ns::parse_quant_circuit(s, q, c)) {
+      need(quants_all && quants_cap >= q.size(), "quantifier buffer too small");
+      std::memcpy(quants_all, q.data(), q.size());
+      *is_quant = 1;
+    } else {
+      c = ns::parse_circuit(s);
+      if (quants_all) {
+        need(quants_cap >= c.n, "quantifier buffer too small");
+        std::memset(quants_all, 0, c.n);
+      }
+      *is_quant = 0;
+    }
+    *out = new ns_circuit{std::move(c)};
+  });
+}
+
+int ns_circuit_from_arrays(uint32_t n_inputs, uint32_t n_gates, const uint32_t* in0, const uint32_t* in1,
+                           const uint8_t* func, uint32_t n_outputs, const uint32_t* out_wire,
+                           const uint8_t* out_neg, ns_circuit** out) {
+  return guard([&] {
+    need(out, "null argument");
+    need(n_gates == 0 || (in0 && in1 && func), "null gate arrays");
+    need(n_outputs == 0 || out_wire, "null output array");
+    ns::Circuit c;
+    c.n = n_inputs;
+    c.gates.resize(n_gates);
+    for (uint32_t j = 0; j < n_gates; j++) c.gates[j] = ns::Gate{in0[j], in1[j], func[j]};
+    c.outputs.resize(n_outputs);
+    for (uint32_t i = 0; i < n_outputs; i++) c.outputs[i] = ns::Output{out_wire[i], out_neg ? out_neg[i] != 0 : false};
+    c.validate();
+    *out = new ns_circuit{std::move(c)};
+  });
+}
+
+int ns_circuit_info(const ns_circuit* c, uint32_t* n_inputs, uint32_t* n_gates, uint32_t* n_outputs) {
+  return guard([&] {
+    need(c, "null circuit");
+    if (n_inputs) *n_inputs = c->c.n;
+    if (n_gates) *n_gates = (uint32_t)c->c.gates.size();
+    if (n_outputs) *n_outputs = (uint32_t)c->c.outputs.size();
+  });
+}
+
+int ns_circuit_to_arrays(const ns_circuit* c, uint32_t* in0, uint32_t* in1, uint8_t* func,
+                         uint32_t* out_wire, uint8_t* out_neg) {
+  return guard([&] {
+    need(c, "null circuit");
+    for (size_t j = 0; j < c->c.gates.size(); j++) {
+      if (in0) in0[j] = c->c.gates[j].i0;
+      if (in1) in1[j] = c->c.gates[j].i1;
+      if (func) func[j] = c->c.gates[j].func;
+    }
+    for (size_t i = 0; i < c->c.outputs.size(); i++) {
+      if (out_wire) out_wire[i] = c->c.outputs[i].wire;
+      if (out_neg) out_neg[i] = c->c.outputs[i].neg ? 1 : 0;
+    }
+  });
+}
+
+int ns_circuit_format(const ns_circuit* c, char* buf, size_t cap, size_t* needed) {
+  return guard([&] {
+    need(c, "null circuit");
+    std::string s = ns::format_circuit(c->c);
+    if (needed) *needed = s.size();
+    if (buf && cap) {
+      need(cap > s.size(), "text buffer too small");
+      std::memcpy(buf, s.c_str(), s.size() + 1);
+    }
+  });
+}
+
+int ns_circuit_eval(const ns_circuit* c, const uint8_t* inputs, uint8_t* outputs) {
+  return guard([&] {
+    need(c && inputs && outputs, "null argument");
+    std::vector<bool> in(c->c.n);
+    for (uint32_t k = 0; k < c->c.n; k++) in[k] = inputs[k] != 0;
+    auto o = c->c.eval(in);
+    for (size_t i = 0; i < o.size(); i++) outputs[i] = o[i] ? 1 : 0;
+  });
+}
+
+int ns_circuit_cone(const ns_circuit* c, uint32_t* cone_gates, uint32_t* peak) {
+  return guard([&] {
+    need(c, "null circuit");
+    auto cone = ns::output_cone(c->c);
+    auto order = ns::schedule_gates(c->c, cone);
+    if (cone_gates) *cone_gates = (uint32_t)order.size();
+    if (peak) *peak = ns::peak_live(c->c, order);
+  });
+}
+
+void ns_circuit_free(ns_circuit* c) { delete c; }
+
+int ns_gen_sat(uint64_t seed, uint32_t n, uint32_t gates, uint32_t word_bits, ns_circuit** out, uint64_t* key) {
+  return guard([&] {
+    need(out, "null argument");
+    *out = new ns_circuit{ns::gen_sat_circuit(seed, n, gates, word_bits, key)};
+  });
+}
+
+int ns_gen_qsat(uint64_t seed, uint32_t n, uint32_t gates, uint32_t last, ns_circuit** out) {
+  return guard([&] {
+    need(out, "null argument");
+    *out = new ns_circuit{ns::gen_qsat_circuit(seed, n, gates, last)};
+  });
+}
+
+int ns_gen_satostop(uint64_t seed, uint32_t half, uint32_t rounds, uint32_t round_gates, uint32_t unknowns,
+                    ns_circuit** out, uint64_t* target) {
+  return guard([&] {
+    need(out, "null argument");
+    *out = new ns_circuit{ns::gen_satostop_circuit(seed, half, rounds, round_gates, unknowns, target)};
+  });
+}
+
+int ns_ctx_create(int device, ns_ctx** out) {
+  return guard([&] {
+    need(out, "null argument");
+    *out = new ns_ctx(device);
+  });
+}
+
+void ns_ctx_destroy(ns_ctx* ctx) {
+  try {
+    delete ctx;
+  } catch (...) {
+  }
+}
+
+int ns_sat_solve(ns_ctx* ctx, const ns_circuit* c, const ns_sat_opts* opts, ns_sat_result* res) {
+  return guard([&] {
+    need(ctx && c && res, "null argument");
+    ns_sat_opts o{};
+    if (opts) o = *opts;
+    ns::sat_solve(ctx->ctx, c->c, o, *res);
+  });
+}
+
+int ns_qsat_solve(ns_ctx* ctx, const ns_circuit* c, const uint8_t* quants_all, const ns_qsat_opts* opts,
+                  ns_qsat_result* res, uint32_t* bitmap, uint64_t bitmap_words) {
+  return guard([&] {
+    need(ctx && c && res && quants_all, "null argument");
+    ns_qsat_opts o{};
+    if (opts) o = *opts;
+    std::vector<uint8_t> q(quants_all, quants_all + c->c.n);
+    for (auto& v : q) v = v ? 1 : 0;
+    ns::qsat_solve(ctx->ctx, c->c, q, o, *res, bitmap, bitmap_words);
+  });
+}
+
+int ns_satostop_solve(ns_ctx* ctx, const ns_circuit* c, uint32_t unknowns, const ns_satostop_opts* opts,
+                      ns_satostop_result* res, uint32_t* steps_out, uint32_t* ends_out) {
+  return guard([&] {
+    need(ctx && c && res, "null argument");
+    ns_satostop_opts o{};
+    if (opts) o = *opts;
+    ns::satostop_solve(ctx->ctx, c->c, unknowns, o, *res, steps_out, ends_out);
+  });
+}
+
+int ns_table_build(ns_ctx* ctx, const ns_circuit* c, uint32_t* table_out, uint64_t table_len, double* kernel_ms) {
+  return guard([&] {
+    need(ctx && c, "null argument");
+    double ms = ns::table_build(ctx->ctx, c->c, table_out, table_len);
+    if (kernel_ms) *kernel_ms = ms;
+  });
+}
+
+int ns_satostop_pass_times(ns_ctx* ctx, double* pass_ms, uint32_t cap, uint32_t* written) {
+  return guard([&] {
+    need(ctx && written, "null argument");
+    const auto& v = ns::last_pass_times(ctx->ctx);
+    uint32_t k = 0;
+    for (; k < v.size() && k < cap; k++)
+      if (pass_ms) pass_ms[k] = v[k];
+    *written = k;
+  });
+}
+
+}  // extern "C"

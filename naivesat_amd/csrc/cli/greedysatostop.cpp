@@ -1,0 +1,69 @@
+// greedysatostop: SATOSTOP by a device-resident next-state table and pointer jumping.
+// Command line and printed lines follow the reference's greedysatostop
+// (src/greedysatostop.rs:28-49,1042-1138,1356-1369).
+#include "cli_common.h"
+
+static const char* kUsage =
+    "Usage: greedysatostop [OPTIONS] <CIRCUIT> <UNKNOWNS>\n"
+    "  -C, --opencl <N>                  HIP device index [default: 0]\n"
+    "  -n, --no-optimize-negs            accepted for compatibility\n"
+    "  -p, --partitions <P>              accepted; the table always lives in HBM\n"
+    "  -x, --file-image-prefix <PREFIX>  accepted; the table always lives in HBM\n"
+    "  -v, --verify                      verify the solution by walking the circuit\n"
+    "  -a, --all                         run every pass and report all stopping unknowns\n";
+
+int main(int argc, char** argv) {
+  auto a = cli::parse_args(argc, argv,
+                           {{'C', "opencl", true},
+                            {'n', "no-optimize-negs", false},
+                            {'p', "partitions", true},
+                            {'x', "file-image-prefix", true},
+                            {'v', "verify", false},
+                            {'a', "all", false}},
+                           kUsage);
+  if (a.positional.size() != 2) cli::die(std::string("expected <CIRCUIT> <UNKNOWNS>\n") + kUsage);
+  cli::print_devices();
+  std::string text = cli::read_file(a.positional[0]);
+  ns_circuit* c = nullptr;
+  cli::check(ns_circuit_parse(text.data(), text.size(), &c), "circuit");
+  uint32_t n = 0, g = 0, no = 0;
+  cli::check(ns_circuit_info(c, &n, &g, &no), "circuit");
+  if (no != n + 1) cli::die("circuit must have one output more than inputs");
+  long long U = std::atoll(a.positional[1].c_str());
+  if (U < 0 || U > (long long)n) cli::die("unknowns must not exceed the inputs");
+  if (a.has("partitions") || a.has("file-image-prefix"))
+    std::printf("Note: the state table lives in device memory; partitions are not used\n");
+  int dev = (int)a.num("opencl", 0);
+  ns_ctx* ctx = nullptr;
+  cli::check(ns_ctx_create(dev, &ctx), "device");
+  std::printf("Calculate first nexts\n");
+  ns_satostop_opts o{};
+  o.flags = a.has("all") ? 0u : NS_STOP_FIRST_PASS;
+  ns_satostop_result r{};
+  cli::check(ns_satostop_solve(ctx, c, (uint32_t)U, &o, &r, nullptr, nullptr), "search");
+  for (uint32_t i = 0; i < r.passes; i++) std::printf("Joining nexts: Stage: %u / %u\n", i, n);
+  std::printf("Time: %g\n", (r.build_ms + r.jump_ms) / 1000.0);
+  if (a.has("all")) std::printf("Stopping unknowns: %llu\n", (unsigned long long)r.n_stopped);
+  if (r.n_stopped) {
+    std::printf("Solution: Solution { start: %llu, end: %llu }\n", (unsigned long long)r.start,
+                (unsigned long long)r.end);
+    if (a.has("verify")) {
+      unsigned long long s = r.start;
+      std::vector<uint8_t> in(n), out(n + 1);
+      for (unsigned long long step = 0; step <= (1ull << n); step++) {
+        for (uint32_t k = 0; k < n; k++) in[k] = (uint8_t)((s >> k) & 1ull);
+        cli::check(ns_circuit_eval(c, in.data(), out.data()), "verify");
+        s = 0;
+        for (uint32_t k = 0; k < n; k++) s |= (unsigned long long)out[k] << k;
+        if (out[n]) break;
+      }
+      std::printf("Verified: end=%llu\n", s);
+      if (s != r.end) std::printf("INCORRECT\n");
+    }
+  } else {
+    std::printf("Unsatisfiable!\n");
+  }
+  ns_ctx_destroy(ctx);
+  ns_circuit_free(c);
+  return 0;
+}

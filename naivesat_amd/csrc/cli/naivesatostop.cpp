@@ -1,0 +1,92 @@
+// naivesatostop: SATOSTOP answer (start, end, steps) on one HIP device.
+// Command line and printed lines follow the reference's naivesatostop
+// (src/satostop.rs:62-99,1650-1782,4443-4467). The search runs on the device-resident
+// next-state table (at most 31 state bits); the hash-map options are accepted.
+#include "cli_common.h"
+
+static const char* kUsage =
+    "Usage: naivesatostop [OPTIONS] --exec-type <EXEC_TYPE> <CIRCUIT> <UNKNOWNS>\n"
+    "  -S, --simple                          lowest stopping unknown, printed as 'Stopped at'\n"
+    "  -e, --elem-inputs <N>                 accepted [default: 24]\n"
+    "  -H, --hashmap-len-bits <N>            accepted [default: 24]\n"
+    "  -U, --unknown-fill-bits <N>           accepted\n"
+    "  -P, --max-predecessors <N>            accepted [default: 0]\n"
+    "  -C, --iters-per-clear-predecessors <N> accepted [default: 20]\n"
+    "  -t, --exec-type <TYPE>                cpu|opencl:N|hip:N (all run on one HIP device)\n"
+    "  -n, --no-optimize-negs                accepted for compatibility\n"
+    "  -G, --opencl-group-len <G>            threads per block of the table build\n"
+    "  -v, --verify                          verify the solution by walking the circuit\n";
+
+int main(int argc, char** argv) {
+  auto a = cli::parse_args(argc, argv,
+                           {{'S', "simple", false},
+                            {'e', "elem-inputs", true},
+                            {'H', "hashmap-len-bits", true},
+                            {'U', "unknown-fill-bits", true},
+                            {'P', "max-predecessors", true},
+                            {'C', "iters-per-clear-predecessors", true},
+                            {'t', "exec-type", true},
+                            {'n', "no-optimize-negs", false},
+                            {'G', "opencl-group-len", true},
+                            {'v', "verify", false}},
+                           kUsage);
+  if (a.positional.size() != 2) cli::die(std::string("expected <CIRCUIT> <UNKNOWNS>\n") + kUsage);
+  if (!a.has("exec-type")) cli::die(std::string("--exec-type is required\n") + kUsage);
+  cli::print_devices();
+  std::string text = cli::read_file(a.positional[0]);
+  ns_circuit* c = nullptr;
+  cli::check(ns_circuit_parse(text.data(), text.size(), &c), "circuit");
+  uint32_t n = 0, g = 0, no = 0;
+  cli::check(ns_circuit_info(c, &n, &g, &no), "circuit");
+  if (no != n + 1) cli::die("circuit must have one output more than inputs");
+  long long U = std::atoll(a.positional[1].c_str());
+  if (U < 0 || U > (long long)n) cli::die("unknowns must not exceed the inputs");
+  const bool simple = a.has("simple");
+  int dev = cli::device_from_exec_type(a.get("exec-type"), !simple);
+  if (!simple) {
+    long long e = a.num("elem-inputs", 24);
+    if (e >= (long long)n) e = n - 1;
+    std::printf("Elem inputs: %lld\n", e);
+    std::printf("Unknowns: %lld\n", U);
+    std::printf("Execute in HIP device=%d\n", dev);
+  }
+  ns_ctx* ctx = nullptr;
+  cli::check(ns_ctx_create(dev, &ctx), "device");
+  ns_satostop_opts o{};
+  o.group_len = (uint32_t)a.num("opencl-group-len", 0);
+  ns_satostop_result r{};
+  cli::check(ns_satostop_solve(ctx, c, (uint32_t)U, &o, &r, nullptr, nullptr), "search");
+  if (simple) {
+    // src/satostop.rs:1776-1781: decimal, zero-padded to the unknown / state widths
+    if (r.n_stopped)
+      std::printf("Stopped at %0*llu %0*llu\n", (int)U, (unsigned long long)r.lowest_u, (int)n,
+                  (unsigned long long)r.end);
+    else
+      std::printf("Unsatisfiable!\n");
+  } else {
+    std::printf("Time: %g\n", (r.build_ms + r.jump_ms) / 1000.0);
+    if (r.n_stopped) {
+      std::printf("Solution: Solution { start: %llu, end: %llu, steps: %llu }\n", (unsigned long long)r.start,
+                  (unsigned long long)r.end, (unsigned long long)r.steps);
+      if (a.has("verify")) {
+        unsigned long long s = r.start, steps = 0;
+        std::vector<uint8_t> in(n), out(n + 1);
+        for (unsigned long long step = 0; step <= (1ull << n); step++) {
+          for (uint32_t k = 0; k < n; k++) in[k] = (uint8_t)((s >> k) & 1ull);
+          cli::check(ns_circuit_eval(c, in.data(), out.data()), "verify");
+          s = 0;
+          for (uint32_t k = 0; k < n; k++) s |= (unsigned long long)out[k] << k;
+          steps++;
+          if (out[n]) break;
+        }
+        std::printf("Verified: end=%llu steps=%llu\n", s, steps);
+        if (s != r.end || steps != r.steps) std::printf("INCORRECT\n");
+      }
+    } else {
+      std::printf("Unsatisfiable!\n");
+    }
+  }
+  ns_ctx_destroy(ctx);
+  ns_circuit_free(c);
+  return 0;
+}

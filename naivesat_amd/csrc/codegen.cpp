@@ -1,0 +1,337 @@
+// Circuit -> straight-line HIP kernel source.
+//
+// Every generated kernel evaluates the circuit bit-sliced: one 32-bit word holds 32
+// consecutive assignments. The five lowest index bits are the constant masks
+// 0xAAAAAAAA .. 0xFFFF0000 (the in-word enumeration the reference's generated kernels
+// use, src/qsat.rs:1299-1301), six index bits come from the lane id, the j-loop bits
+// and everything above are wave-uniform and become scalar (SGPR) operands.
+#include "codegen.h"
+
+#include <sstream>
+#include <stdexcept>
+
+namespace ns {
+
+namespace {
+
+const uint32_t kWordMasks[5] = {0xAAAAAAAAu, 0xCCCCCCCCu, 0xF0F0F0F0u, 0xFF00FF00u, 0xFFFF0000u};
+
+uint32_t log2u(uint32_t x) {
+  uint32_t r = 0;
+  while ((1u << r) < x) r++;
+  if ((1u << r) != x) throw std::runtime_error("value must be a power of two");
+  return r;
+}
+
+// name of the value on wire w
+std::string wname(const Circuit& c, uint32_t w) {
+  return (w < c.n ? "x" : "g") + std::to_string(w);
+}
+
+// Emits the common kernel head: lane/wave ids, the wave-uniform base U, lane masks.
+void emit_head(std::ostringstream& os, const Geometry& g) {
+  os << "  const unsigned int lane = threadIdx.x & 63u;\n"
+        "  const unsigned int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);\n"
+        "  const unsigned long long U = w0 + ((unsigned long long)blockIdx.x * "
+     << (g.block / 64) << "ull + wave) * " << (64ull * g.wpl) << "ull;\n";
+  for (int i = 0; i < 6; i++)
+    os << "  const unsigned int LM" << i << " = 0u - ((lane >> " << i << ") & 1u);\n";
+}
+
+// Emits the definitions of the inputs for one word. vbit_of_input[k] is the position
+// of input k inside the assignment index.
+void emit_inputs(std::ostringstream& os, const Circuit& c, const std::vector<uint32_t>& vbit_of_input,
+                 const std::vector<uint8_t>& used, const Geometry& g) {
+  uint32_t jb = log2u(g.wpl);
+  for (uint32_t k = 0; k < c.n; k++) {
+    if (!used[k]) continue;
+    uint32_t vb = vbit_of_input[k];
+    os << "    const unsigned int x" << k << " = ";
+    if (vb < 5) {
+      os << "0x" << std::hex << kWordMasks[vb] << std::dec << "u;\n";
+      continue;
+    }
+    uint32_t wb = vb - 5;
+    if (wb >= g.lane_shift && wb < g.lane_shift + 6)
+      os << "LM" << (wb - g.lane_shift) << ";\n";
+    else if (wb >= g.j_shift && wb < g.j_shift + jb)
+      os << "0u - ((j >> " << (wb - g.j_shift) << ") & 1u);\n";
+    else
+      os << "0u - (unsigned int)((U >> " << wb << ") & 1ull);\n";
+  }
+}
+
+// Emits the gates of the given order.
+void emit_gates(std::ostringstream& os, const Circuit& c, const std::vector<uint32_t>& order) {
+  for (uint32_t w : order) {
+    const Gate& gt = c.gates[w - c.n];
+    std::string a = wname(c, gt.i0), b = wname(c, gt.i1);
+    os << "    const unsigned int g" << w << " = ";
+    switch (gt.func) {
+      case FUNC_AND: os << a << " & " << b; break;
+      case FUNC_NOR: os << "~(" << a << " | " << b << ")"; break;
+      case FUNC_NIMPL: os << a << " & ~" << b; break;
+      default: os << a << " ^ " << b; break;
+    }
+    os << ";\n";
+  }
+}
+
+std::string out_expr(const Circuit& c, uint32_t k) {
+  const Output& o = c.outputs.at(k);
+  std::string s = wname(c, o.wire);
+  return o.neg ? "(~" + s + ")" : s;
+}
+
+// inputs referenced by the ordered gates or directly by the selected outputs
+std::vector<uint8_t> used_inputs(const Circuit& c, const std::vector<uint32_t>& order,
+                                 const std::vector<uint32_t>& outs) {
+  std::vector<uint8_t> u(c.n, 0);
+  for (uint32_t w : order) {
+    const Gate& gt = c.gates[w - c.n];
+    if (gt.i0 < c.n) u[gt.i0] = 1;
+    if (gt.i1 < c.n) u[gt.i1] = 1;
+  }
+  for (uint32_t k : outs)
+    if (c.outputs[k].wire < c.n) u[c.outputs[k].wire] = 1;
+  return u;
+}
+
+void prelude(std::ostringstream& os) {
+  os << "// generated: bit-sliced circuit kernel\n";
+}
+
+// valid bits of a word when the whole space has fewer than 32 assignments
+uint32_t word_valid_bits(uint32_t n_bits) {
+  return n_bits >= 5 ? 0xffffffffu : (uint32_t)((1ull << (1u << n_bits)) - 1);
+}
+
+}  // namespace
+
+std::string gen_sat_kernel(const Circuit& c, const Geometry& g, GenInfo* info) {
+  if (c.outputs.size() != 1) throw std::runtime_error("SAT needs exactly one output");
+  std::vector<uint32_t> outs = {0};
+  auto cone = output_cone(c, outs);
+  auto order = schedule_gates(c, cone, outs);
+  if (info) {
+    info->cone_gates = (uint32_t)order.size();
+    info->peak_live = peak_live(c, order, outs);
+  }
+  std::vector<uint32_t> vb(c.n);
+  for (uint32_t k = 0; k < c.n; k++) vb[k] = k;
+  std::ostringstream os;
+  prelude(os);
+  os << "extern \"C\" __global__ void __launch_bounds__(" << g.block
+     << ") ns_sat(unsigned long long w0, unsigned long long wend, unsigned long long* res) {\n";
+  emit_head(os, g);
+  os << "  unsigned int cnt = 0u;\n  unsigned long long best = ~0ull;\n";
+  os << "#pragma unroll 1\n  for (unsigned int j = 0; j < " << g.wpl << "u; j++) {\n";
+  os << "    const unsigned long long w = U + ((unsigned long long)lane << " << g.lane_shift
+     << ") + ((unsigned long long)j << " << g.j_shift << ");\n";
+  emit_inputs(os, c, vb, used_inputs(c, order, outs), g);
+  emit_gates(os, c, order);
+  os << "    unsigned int o = " << out_expr(c, 0) << " & 0x" << std::hex << word_valid_bits(c.n)
+     << std::dec << "u;\n";
+  os << "    if (w >= wend) o = 0u;\n";
+  os << "    cnt += __popc(o);\n"
+        "    if (o != 0u && best == ~0ull) best = (w << 5) | (unsigned long long)__builtin_ctz(o);\n"
+        "  }\n";
+  os << "  if (__ballot(cnt != 0u) != 0ull) {\n"
+        "    unsigned long long c64 = cnt;\n"
+        "    for (int off = 32; off > 0; off >>= 1) {\n"
+        "      unsigned long long ob = __shfl_xor(best, off);\n"
+        "      best = ob < best ? ob : best;\n"
+        "      c64 += __shfl_xor(c64, off);\n"
+        "    }\n"
+        "    if (lane == 0u) {\n"
+        "      atomicAdd(&res[0], c64);\n"
+        "      atomicMin(&res[1], best);\n"
+        "    }\n"
+        "  }\n}\n";
+  return os.str();
+}
+
+std::string gen_qsat_kernel(const Circuit& c, const std::vector<uint8_t>& q, const Geometry& g,
+                            GenInfo* info) {
+  if (c.outputs.size() != 1) throw std::runtime_error("QSAT needs exactly one output");
+  if (q.size() != c.n) throw std::runtime_error("one quantifier per input is required");
+  const uint32_t n = c.n;
+  uint32_t fb = 0;
+  while (fb < n && q[fb] == q[0]) fb++;
+  const uint32_t L = n - fb;
+  const uint32_t jb = log2u(g.wpl);
+  const bool q0all = q[0] != 0;
+  // op of level l (level 0 = index bit 0 = input n-1): 1 = AND (all), 0 = OR (exists)
+  auto op = [&](uint32_t l) { return q[n - 1 - l] ? "&" : "|"; };
+  if (g.lane_shift != jb || g.j_shift != 0)
+    throw std::runtime_error("QSAT kernels need the j bits below the lane bits");
+  std::vector<uint32_t> outs = {0};
+  auto cone = output_cone(c, outs);
+  auto order = schedule_gates(c, cone, outs);
+  if (info) {
+    info->cone_gates = (uint32_t)order.size();
+    info->peak_live = peak_live(c, order, outs);
+  }
+  std::vector<uint32_t> vb(n);
+  for (uint32_t k = 0; k < n; k++) vb[k] = n - 1 - k;
+  const uint32_t valid_bits = word_valid_bits(n);
+  const bool case_a = L >= 5 + jb;
+  std::ostringstream os;
+  prelude(os);
+  if (case_a)
+    os << "extern \"C\" __global__ void __launch_bounds__(" << g.block
+       << ") ns_qsat(unsigned long long w0, unsigned long long wend, unsigned long long* bits) {\n";
+  else
+    os << "extern \"C\" __global__ void __launch_bounds__(" << g.block
+       << ") ns_qsat(unsigned long long w0, unsigned long long wend, unsigned long long* stats, "
+          "unsigned int* bitmap) {\n";
+  emit_head(os, g);
+  os << "  const unsigned long long wl = U + ((unsigned long long)lane << " << jb << ");\n";
+  if (!case_a && L <= 5)
+    os << "  unsigned int cnt = 0u;\n  unsigned long long best = ~0ull;\n";
+  else
+    os << "  unsigned int R = 0u;\n";
+  os << "#pragma unroll 1\n  for (unsigned int j = 0; j < " << g.wpl << "u; j++) {\n";
+  os << "    const unsigned long long w = wl + j;\n";
+  emit_inputs(os, c, vb, used_inputs(c, order, outs), g);
+  emit_gates(os, c, order);
+  os << "    unsigned int o = " << out_expr(c, 0) << ";\n";
+  uint32_t inword = L < 5 ? L : 5;
+  for (uint32_t l = 0; l < inword; l++)
+    os << "    o = o " << op(l) << " (o >> " << (1u << l) << ");\n";
+  if (!case_a && L <= 5) {
+    // positions p * 2^L of the word hold outermost-block values u = (32w + p 2^L) >> L
+    uint32_t pos = 0;
+    for (uint32_t p = 0; p < (32u >> L); p++) pos |= 1u << (p << L);
+    pos &= valid_bits;
+    os << "    unsigned int wit = (" << (q0all ? "~o" : "o") << ") & 0x" << std::hex << pos << std::dec
+       << "u;\n";
+    os << "    if (w >= wend) wit = 0u;\n";
+    os << "    cnt += __popc(wit);\n";
+    os << "    if (wit != 0u && best == ~0ull) best = (w << " << (5 - L)
+       << ") + (unsigned long long)(__builtin_ctz(wit) >> " << L << ");\n";
+    os << "    if (bitmap != 0 && w < wend) {\n";
+    if (L == 0) {
+      os << "      bitmap[w] = o & 0x" << std::hex << valid_bits << std::dec << "u;\n";
+    } else {
+      os << "      unsigned int r = 0u;\n";
+      for (uint32_t p = 0; p < (32u >> L); p++)
+        if ((pos >> (p << L)) & 1u)
+          os << "      r |= ((o >> " << (p << L) << ") & 1u) << " << p << ";\n";
+      os << "      const unsigned long long ub = w << " << (5 - L) << ";\n";
+      os << "      atomicOr(&bitmap[ub >> 5], r << (unsigned int)(ub & 31ull));\n";
+    }
+    os << "    }\n";
+    os << "  }\n";
+  } else {
+    os << "    R |= (o & 1u) << j;\n  }\n";
+    // j levels (levels 5 .. 5+jb-1)
+    uint32_t jl = (L - 5) < jb ? (L - 5) : jb;
+    for (uint32_t l = 0; l < jl; l++)
+      os << "  R = R " << op(5 + l) << " (R >> " << (1u << l) << ");\n";
+    os << "  const unsigned int nvalid = wend > wl ? (wend - wl >= " << g.wpl << "ull ? " << g.wpl
+       << "u : (unsigned int)(wend - wl)) : 0u;\n";
+    if (case_a) {
+      os << "  const unsigned long long ball = __ballot((R & 1u) != 0u && nvalid != 0u);\n";
+      os << "  if (lane == 0u) bits[U >> " << (jb + 6) << "] = ball;\n}\n";
+    } else {
+      // positions p * 2^(L-5) of R hold u = (wl >> (L-5)) + p
+      uint32_t s = L - 5;
+      uint32_t pos = 0;
+      for (uint32_t p = 0; p < (g.wpl >> s); p++) pos |= 1u << (p << s);
+      os << "  const unsigned int jmask = nvalid >= 32u ? 0xffffffffu : ((1u << nvalid) - 1u);\n";
+      os << "  unsigned int wit = (" << (q0all ? "~R" : "R") << ") & 0x" << std::hex << pos << std::dec
+         << "u & jmask;\n";
+      os << "  unsigned int cnt = __popc(wit);\n";
+      os << "  unsigned long long best = wit != 0u ? (wl >> " << s
+         << ") + (unsigned long long)(__builtin_ctz(wit) >> " << s << ") : ~0ull;\n";
+      os << "  if (bitmap != 0 && nvalid != 0u) {\n"
+            "    unsigned int r = 0u;\n";
+      for (uint32_t p = 0; p < (g.wpl >> s); p++)
+        os << "    r |= ((R >> " << (p << s) << ") & 1u) << " << p << ";\n";
+      os << "    r &= (nvalid >= " << g.wpl << "u) ? 0xffffffffu : ((1u << ((nvalid + "
+         << ((1u << s) - 1) << "u) >> " << s << ")) - 1u);\n";
+      os << "    const unsigned long long ub = wl >> " << s << ";\n";
+      os << "    atomicOr(&bitmap[ub >> 5], r << (unsigned int)(ub & 31ull));\n";
+      os << "  }\n";
+    }
+  }
+  if (!case_a) {
+    os << "  if (__ballot(cnt != 0u) != 0ull) {\n"
+          "    unsigned long long c64 = cnt;\n"
+          "    for (int off = 32; off > 0; off >>= 1) {\n"
+          "      unsigned long long ob = __shfl_xor(best, off);\n"
+          "      best = ob < best ? ob : best;\n"
+          "      c64 += __shfl_xor(c64, off);\n"
+          "    }\n"
+          "    if (lane == 0u) {\n"
+          "      atomicAdd(&stats[0], c64);\n"
+          "      atomicMin(&stats[1], best);\n"
+          "    }\n"
+          "  }\n}\n";
+  }
+  return os.str();
+}
+
+std::string gen_table_kernel(const Circuit& c, const Geometry& g, GenInfo* info) {
+  const uint32_t m = (uint32_t)c.outputs.size();
+  if (m == 0 || m > 32) throw std::runtime_error("table build supports 1..32 outputs");
+  std::vector<uint32_t> outs;
+  for (uint32_t k = 0; k < m; k++) outs.push_back(k);
+  auto cone = output_cone(c, outs);
+  auto order = schedule_gates(c, cone, outs);
+  if (info) {
+    info->cone_gates = (uint32_t)order.size();
+    info->peak_live = peak_live(c, order, outs);
+  }
+  std::vector<uint32_t> vb(c.n);
+  for (uint32_t k = 0; k < c.n; k++) vb[k] = k;
+  std::ostringstream os;
+  prelude(os);
+  os << "extern \"C\" __global__ void __launch_bounds__(" << g.block
+     << ") ns_table(unsigned long long w0, unsigned long long wend, unsigned int* table, "
+        "unsigned int* anylast) {\n";
+  emit_head(os, g);
+  os << "  unsigned int last_any = 0u;\n";
+  os << "#pragma unroll 1\n  for (unsigned int j = 0; j < " << g.wpl << "u; j++) {\n";
+  os << "    const unsigned long long w = U + ((unsigned long long)lane << " << g.lane_shift
+     << ") + ((unsigned long long)j << " << g.j_shift << ");\n";
+  emit_inputs(os, c, vb, used_inputs(c, order, outs), g);
+  emit_gates(os, c, order);
+  os << "    unsigned int A[32];\n";
+  for (uint32_t k = 0; k < 32; k++) {
+    if (k < m) os << "    A[" << k << "] = " << out_expr(c, k) << ";\n";
+    else os << "    A[" << k << "] = 0u;\n";
+  }
+  os << "    if (w < wend) last_any |= A[" << (m - 1) << "] & 0x" << std::hex << word_valid_bits(c.n)
+     << std::dec << "u;\n";
+  // 32x32 bit transpose: afterwards bit i of A[b] is bit b of output i
+  os << "    {\n"
+        "      unsigned int msk = 0x0000FFFFu;\n"
+        "#pragma unroll\n"
+        "      for (int s = 16; s != 0; s >>= 1, msk ^= (msk << s)) {\n"
+        "#pragma unroll\n"
+        "        for (int k = 0; k < 32; k = ((k | s) + 1) & ~s) {\n"
+        "          const unsigned int t = ((A[k] >> s) ^ A[k | s]) & msk;\n"
+        "          A[k] ^= t << s;\n"
+        "          A[k | s] ^= t;\n"
+        "        }\n"
+        "      }\n"
+        "    }\n";
+  if (c.n >= 5) {
+    os << "    if (w < wend) {\n"
+          "      uint4* dst = (uint4*)(table + (w << 5));\n"
+          "#pragma unroll\n"
+          "      for (int q = 0; q < 8; q++) dst[q] = make_uint4(A[4*q], A[4*q+1], A[4*q+2], A[4*q+3]);\n"
+          "    }\n";
+  } else {
+    os << "    if (w < wend) {\n";
+    for (uint32_t b = 0; b < (1u << c.n); b++) os << "      table[" << b << "] = A[" << b << "];\n";
+    os << "    }\n";
+  }
+  os << "  }\n";
+  os << "  if (last_any != 0u) atomicOr(anylast, 1u);\n}\n";
+  return os.str();
+}
+
+}  // namespace ns

@@ -1,0 +1,77 @@
+// HIP runtime plumbing: error checks, run-time compilation of generated kernels
+// (hiprtc, gfx950 code objects) and the per-context stream/buffers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+namespace ns {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+// status codes shared with the C ABI
+enum : int {
+  ST_OK = 0,
+  ST_INVALID = 1,
+  ST_HIP = 2,
+  ST_COMPILE = 3,
+  ST_UNSUPPORTED = 4,
+  ST_NOMEM = 5,
+  ST_INTERNAL = 6,
+};
+
+void check_hip(hipError_t e, const char* what);
+#define NS_HIP(x) ::ns::check_hip((x), #x)
+
+struct JitModule {
+  hipModule_t mod = nullptr;
+  hipFunction_t fn = nullptr;
+  double compile_ms = 0.0;  // 0 when served from the cache
+};
+
+// Compiles generated sources once per context; keyed by the full source text.
+class Jit {
+ public:
+  std::shared_ptr<JitModule> get(const std::string& src, const char* kernel, double* compile_ms);
+  ~Jit();
+
+ private:
+  std::mutex mu_;
+  std::unordered_map<std::string, std::shared_ptr<JitModule>> cache_;
+};
+
+// Growable device allocation (contents are not preserved when it grows).
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  void ensure(size_t bytes);
+  void release();
+  ~DevBuf() { release(); }
+  template <class T>
+  T* as() { return static_cast<T*>(p); }
+};
+
+struct Context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  Jit jit;
+  DevBuf stats;             // small result block
+  DevBuf buf_a, buf_b, buf_c, buf_d;
+  std::vector<double> pass_ms;  // jump-pass kernel times of the last SATOSTOP solve
+  explicit Context(int dev);
+  ~Context();
+  // elapsed ms between ev0 and ev1 (both recorded on `stream`)
+  float elapsed_ms();
+};
+
+}  // namespace ns

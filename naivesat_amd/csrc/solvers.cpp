@@ -1,0 +1,379 @@
+// Solver drivers. The search space is covered by launches over contiguous ranges of
+// 32-assignment words (the reference's per-`arg` launches, src/main.rs:178-196, but
+// with results accumulated on the device and read back once).
+#include "solvers.h"
+
+#include <chrono>
+#include <cstring>
+
+#include "codegen.h"
+#include "static_kernels.h"
+
+namespace ns {
+
+namespace {
+
+using clk = std::chrono::steady_clock;
+
+double ms_since(clk::time_point t0) {
+  return std::chrono::duration<double, std::milli>(clk::now() - t0).count();
+}
+
+uint32_t ilog2(uint64_t x) {
+  uint32_t r = 0;
+  while ((1ull << r) < x) r++;
+  return r;
+}
+
+bool is_pow2(uint64_t x) { return x && !(x & (x - 1)); }
+
+// Geometry for a search over `total_words` words.
+Geometry pick_geometry(uint64_t total_words, uint32_t group_len, uint32_t wpl, bool j_low) {
+  Geometry g;
+  g.block = group_len ? group_len : 256;
+  g.wpl = wpl ? wpl : 4;
+  if (g.block % 64 || g.block > 1024) throw Error(ST_INVALID, "group_len must be a multiple of 64, at most 1024");
+  if (!is_pow2(g.wpl) || g.wpl > 32) throw Error(ST_INVALID, "words_per_lane must be a power of two <= 32");
+  // small spaces: one wave, one word per lane
+  while ((uint64_t)g.block * g.wpl > total_words && (g.block > 64 || g.wpl > 1)) {
+    if (g.wpl > 1) g.wpl >>= 1;
+    else g.block >>= 1;
+    if (g.block % 64) g.block = 64;
+  }
+  uint32_t jb = ilog2(g.wpl);
+  if (j_low) {
+    g.lane_shift = jb;
+    g.j_shift = 0;
+  } else {
+    g.lane_shift = 0;
+    g.j_shift = 6;
+  }
+  return g;
+}
+
+// Words per launch: 2^(elem_bits-5) rounded up to whole blocks.
+uint64_t chunk_words(uint32_t elem_bits, const Geometry& g, uint64_t total_words) {
+  uint32_t eb = elem_bits ? elem_bits : 32;
+  if (eb < 5) eb = 5;
+  if (eb > 40) eb = 40;
+  uint64_t cw = 1ull << (eb - 5);
+  uint64_t wpb = g.words_per_block();
+  cw = (cw + wpb - 1) / wpb * wpb;
+  uint64_t tot = (total_words + wpb - 1) / wpb * wpb;
+  return cw < tot ? cw : tot;
+}
+
+void launch_range(Context& ctx, hipFunction_t fn, const Geometry& g, uint64_t w0, uint64_t words,
+                  void** extra_args, int n_extra, uint64_t wend) {
+  uint64_t blocks = (words + g.words_per_block() - 1) / g.words_per_block();
+  if (blocks > 0x7fffffffull) throw Error(ST_UNSUPPORTED, "launch range too large");
+  void* args[8];
+  args[0] = &w0;
+  args[1] = &wend;
+  for (int i = 0; i < n_extra; i++) args[2 + i] = extra_args[i];
+  NS_HIP(hipModuleLaunchKernel(fn, (unsigned)blocks, 1, 1, g.block, 1, 1, 0, ctx.stream, args, nullptr));
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// SAT
+// ---------------------------------------------------------------------------
+void sat_solve(Context& ctx, const Circuit& c, const ns_sat_opts& o, ns_sat_result& r) {
+  auto t0 = clk::now();
+  std::memset(&r, 0, sizeof(r));
+  if (c.outputs.size() != 1) throw Error(ST_INVALID, "SAT needs a circuit with exactly one output");
+  if (c.n == 0 || c.n > 64) throw Error(ST_UNSUPPORTED, "SAT supports 1..64 inputs");
+  NS_HIP(hipSetDevice(ctx.device));
+  const uint64_t total_words = c.n >= 5 ? (c.n == 64 ? (1ull << 59) : (1ull << (c.n - 5))) : 1ull;
+  Geometry g = pick_geometry(total_words, o.group_len, o.words_per_lane, false);
+  GenInfo info;
+  std::string src = gen_sat_kernel(c, g, &info);
+  auto mod = ctx.jit.get(src, "ns_sat", &r.compile_ms);
+  r.cone_gates = info.cone_gates;
+  unsigned long long* res = ctx.stats.as<unsigned long long>();
+  NS_HIP(launch_fill_u64(ctx.stream, res, 0ull, 1));
+  NS_HIP(launch_fill_u64(ctx.stream, res + 1, ~0ull, 1));
+  const uint64_t cw = chunk_words(o.elem_bits, g, total_words);
+  const bool stop_first = (o.flags & NS_SAT_STOP_AT_FIRST) != 0;
+  unsigned long long host[2] = {0, ~0ull};
+  double kms = 0;
+  uint64_t w0 = 0;
+  r.complete = 1;
+  NS_HIP(hipEventRecord(ctx.ev0, ctx.stream));
+  while (w0 < total_words) {
+    uint64_t words = total_words - w0 < cw ? total_words - w0 : cw;
+    void* extra[1] = {&res};
+    launch_range(ctx, mod->fn, g, w0, words, extra, 1, total_words);
+    r.launches++;
+    w0 += cw;
+    if (stop_first) {
+      NS_HIP(hipMemcpyAsync(host, res, sizeof(host), hipMemcpyDeviceToHost, ctx.stream));
+      NS_HIP(hipStreamSynchronize(ctx.stream));
+      if (host[0] != 0) {
+        r.complete = w0 >= total_words;
+        break;
+      }
+    }
+  }
+  NS_HIP(hipEventRecord(ctx.ev1, ctx.stream));
+  NS_HIP(hipMemcpyAsync(host, res, sizeof(host), hipMemcpyDeviceToHost, ctx.stream));
+  NS_HIP(hipStreamSynchronize(ctx.stream));
+  kms = ctx.elapsed_ms();
+  r.count = host[0];
+  r.found = host[0] != 0;
+  r.lowest = r.found ? host[1] : 0;
+  r.kernel_ms = kms;
+  r.wall_ms = ms_since(t0);
+}
+
+// ---------------------------------------------------------------------------
+// QSAT
+// ---------------------------------------------------------------------------
+void qsat_solve(Context& ctx, const Circuit& c, const std::vector<uint8_t>& q, const ns_qsat_opts& o,
+                ns_qsat_result& r, uint32_t* bitmap, uint64_t bitmap_words) {
+  auto t0 = clk::now();
+  std::memset(&r, 0, sizeof(r));
+  if (c.outputs.size() != 1) throw Error(ST_INVALID, "QSAT needs a circuit with exactly one output");
+  if (c.n == 0 || c.n > 60) throw Error(ST_UNSUPPORTED, "QSAT supports 1..60 inputs");
+  if (q.size() != c.n) throw Error(ST_INVALID, "one quantifier per input is required");
+  NS_HIP(hipSetDevice(ctx.device));
+  const uint32_t n = c.n;
+  uint32_t fb = 0;
+  while (fb < n && q[fb] == q[0]) fb++;
+  const uint32_t L = n - fb;
+  const bool q0all = q[0] != 0;
+  r.first_bits = fb;
+  r.first_all = q0all;
+  if (bitmap) {
+    if (fb > 36) throw Error(ST_UNSUPPORTED, "bitmap of the outermost block limited to 2^36 bits");
+    uint64_t need = fb >= 5 ? (1ull << (fb - 5)) : 1ull;
+    if (bitmap_words < need) throw Error(ST_INVALID, "bitmap buffer too small");
+  }
+  const uint64_t total_words = n >= 5 ? (1ull << (n - 5)) : 1ull;
+  Geometry g = pick_geometry(total_words, o.group_len, o.words_per_lane, true);
+  const uint32_t jb = ilog2(g.wpl);
+  const bool case_a = L >= 5 + jb;
+  GenInfo info;
+  std::string src = gen_qsat_kernel(c, q, g, &info);
+  auto mod = ctx.jit.get(src, "ns_qsat", &r.compile_ms);
+  r.cone_gates = info.cone_gates;
+  unsigned long long* stats = ctx.stats.as<unsigned long long>();
+  NS_HIP(launch_fill_u64(ctx.stream, stats, 0ull, 1));
+  NS_HIP(launch_fill_u64(ctx.stream, stats + 1, ~0ull, 1));
+  const uint64_t cw = chunk_words(o.elem_bits, g, total_words);
+  unsigned int* dev_bitmap = nullptr;  // final 2^fb-bit array on the device
+  NS_HIP(hipEventRecord(ctx.ev0, ctx.stream));
+  if (case_a) {
+    // phase 1: one bit per thread
+    const uint32_t m0 = n - 5 - jb;
+    const uint64_t xbytes = ((1ull << m0) + 63) / 64 * 8;
+    ctx.buf_a.ensure(xbytes);
+    unsigned long long* X = ctx.buf_a.as<unsigned long long>();
+    for (uint64_t w0 = 0; w0 < total_words; w0 += cw) {
+      uint64_t words = total_words - w0 < cw ? total_words - w0 : cw;
+      void* extra[1] = {&X};
+      launch_range(ctx, mod->fn, g, w0, words, extra, 1, total_words);
+    }
+    // phase 2: reduction passes over the bit array
+    uint32_t m = m0, remaining = L - 5 - jb, level = 5 + jb;
+    const unsigned int* in = reinterpret_cast<unsigned int*>(X);
+    DevBuf* outs[2] = {&ctx.buf_b, &ctx.buf_c};
+    int which = 0;
+    while (true) {
+      uint32_t k = remaining < 11 ? remaining : 11;
+      bool final_pass = k == remaining;
+      uint32_t mo = m - k;
+      uint64_t owords = mo >= 5 ? (1ull << (mo - 5)) : 1ull;
+      outs[which]->ensure((owords + 1) * 4);
+      unsigned int* out = outs[which]->as<unsigned int>();
+      NS_HIP(launch_fill_u32(ctx.stream, out, 0u, owords + 1));
+      uint32_t opmask = 0;
+      for (uint32_t l = 0; l < k; l++)
+        if (q[n - 1 - (level + l)]) opmask |= 1u << l;
+      NS_HIP(launch_qsat_reduce(ctx.stream, in, m, k, opmask, out, final_pass, q0all, stats));
+      r.passes++;
+      if (final_pass) {
+        dev_bitmap = out;
+        break;
+      }
+      in = out;
+      m = mo;
+      remaining -= k;
+      level += k;
+      which ^= 1;
+    }
+  } else {
+    if (bitmap) {
+      uint64_t bw = fb >= 5 ? (1ull << (fb - 5)) : 1ull;
+      ctx.buf_b.ensure(bw * 4);
+      dev_bitmap = ctx.buf_b.as<unsigned int>();
+      NS_HIP(launch_fill_u32(ctx.stream, dev_bitmap, 0u, bw));
+    }
+    for (uint64_t w0 = 0; w0 < total_words; w0 += cw) {
+      uint64_t words = total_words - w0 < cw ? total_words - w0 : cw;
+      void* extra[2] = {&stats, &dev_bitmap};
+      launch_range(ctx, mod->fn, g, w0, words, extra, 2, total_words);
+    }
+  }
+  NS_HIP(hipEventRecord(ctx.ev1, ctx.stream));
+  unsigned long long host[2];
+  NS_HIP(hipMemcpyAsync(host, stats, sizeof(host), hipMemcpyDeviceToHost, ctx.stream));
+  if (bitmap && dev_bitmap) {
+    uint64_t need = fb >= 5 ? (1ull << (fb - 5)) : 1ull;
+    NS_HIP(hipMemcpyAsync(bitmap, dev_bitmap, need * 4, hipMemcpyDeviceToHost, ctx.stream));
+  }
+  NS_HIP(hipStreamSynchronize(ctx.stream));
+  if (bitmap && fb < 5) bitmap[0] &= (uint32_t)((1ull << (1u << fb)) - 1);
+  r.kernel_ms = ctx.elapsed_ms();
+  r.witness_count = host[0];
+  r.has_witness = host[0] != 0;
+  r.witness = r.has_witness ? host[1] : 0;
+  r.truth = (r.has_witness ? 1u : 0u) ^ (q0all ? 1u : 0u);
+  r.wall_ms = ms_since(t0);
+}
+
+// ---------------------------------------------------------------------------
+// Next-state table and SATOSTOP
+// ---------------------------------------------------------------------------
+namespace {
+
+// Builds the u32 table into ctx.buf_a; returns kernel ms; sets *anylast.
+double build_table_dev(Context& ctx, const Circuit& c, uint32_t group_len, uint32_t wpl,
+                       bool* anylast, double* compile_ms, uint32_t* cone) {
+  const uint32_t n = c.n;
+  if (n == 0 || n > 31) throw Error(ST_UNSUPPORTED, "next-state tables support 1..31 state bits");
+  if (c.outputs.empty() || c.outputs.size() > 32) throw Error(ST_UNSUPPORTED, "table entries hold at most 32 outputs");
+  const uint64_t states = 1ull << n;
+  const uint64_t total_words = n >= 5 ? (1ull << (n - 5)) : 1ull;
+  Geometry g = pick_geometry(total_words, group_len, wpl ? wpl : 2, false);
+  GenInfo info;
+  std::string src = gen_table_kernel(c, g, &info);
+  auto mod = ctx.jit.get(src, "ns_table", compile_ms);
+  if (cone) *cone = info.cone_gates;
+  ctx.buf_a.ensure((states < 32 ? 32 : states) * 4);
+  unsigned int* T = ctx.buf_a.as<unsigned int>();
+  unsigned int* flag = reinterpret_cast<unsigned int*>(ctx.stats.as<char>() + 64);
+  NS_HIP(launch_fill_u32(ctx.stream, flag, 0u, 1));
+  const uint64_t cw = chunk_words(30, g, total_words);
+  NS_HIP(hipEventRecord(ctx.ev0, ctx.stream));
+  for (uint64_t w0 = 0; w0 < total_words; w0 += cw) {
+    uint64_t words = total_words - w0 < cw ? total_words - w0 : cw;
+    void* extra[2] = {&T, &flag};
+    launch_range(ctx, mod->fn, g, w0, words, extra, 2, total_words);
+  }
+  NS_HIP(hipEventRecord(ctx.ev1, ctx.stream));
+  unsigned int hf = 0;
+  NS_HIP(hipMemcpyAsync(&hf, flag, 4, hipMemcpyDeviceToHost, ctx.stream));
+  NS_HIP(hipStreamSynchronize(ctx.stream));
+  *anylast = hf != 0;
+  return ctx.elapsed_ms();
+}
+
+}  // namespace
+
+double table_build(Context& ctx, const Circuit& c, uint32_t* table_out, uint64_t table_len) {
+  NS_HIP(hipSetDevice(ctx.device));
+  bool any = false;
+  double cms = 0;
+  double kms = build_table_dev(ctx, c, 0, 0, &any, &cms, nullptr);
+  const uint64_t states = 1ull << c.n;
+  if (table_out) {
+    if (table_len < states) throw Error(ST_INVALID, "table buffer too small");
+    NS_HIP(hipMemcpy(table_out, ctx.buf_a.p, states * 4, hipMemcpyDeviceToHost));
+  }
+  return kms;
+}
+
+void satostop_solve(Context& ctx, const Circuit& c, uint32_t unknowns, const ns_satostop_opts& o,
+                    ns_satostop_result& r, uint32_t* steps_out, uint32_t* ends_out) {
+  auto t0 = clk::now();
+  std::memset(&r, 0, sizeof(r));
+  const uint32_t n = c.n;
+  if (c.outputs.size() != (size_t)n + 1) throw Error(ST_INVALID, "SATOSTOP needs n inputs and n+1 outputs");
+  if (n == 0 || n > 31) throw Error(ST_UNSUPPORTED, "SATOSTOP tables support 1..31 state bits");
+  if (unknowns > n) throw Error(ST_INVALID, "more unknowns than state bits");
+  NS_HIP(hipSetDevice(ctx.device));
+  ctx.pass_ms.clear();
+  bool any = false;
+  r.build_ms = build_table_dev(ctx, c, o.group_len, o.words_per_lane, &any, &r.compile_ms, &r.cone_gates);
+  r.any_stop = any;
+  const uint64_t states = 1ull << n;
+  const uint64_t starts = 1ull << unknowns;
+  ctx.buf_d.ensure(starts * 8 + 64);
+  unsigned int* d_steps = ctx.buf_d.as<unsigned int>();
+  unsigned int* d_ends = d_steps + starts;
+  unsigned long long* stats = ctx.stats.as<unsigned long long>();
+  uint32_t max_passes = o.max_passes ? o.max_passes : n;
+  if (max_passes > n) max_passes = n;
+  const bool first_pass_mode = (o.flags & NS_STOP_FIRST_PASS) != 0;
+  unsigned long long host[2] = {0, ~0ull};
+  const unsigned long long* final_tab = nullptr;
+  if (any) {
+    // even entry count for the two-entries-per-thread pass
+    const uint64_t cnt = states < 2 ? 2 : states;
+    ctx.buf_b.ensure(cnt * 8);
+    ctx.buf_c.ensure(cnt * 8);
+    unsigned long long* cur = ctx.buf_b.as<unsigned long long>();
+    unsigned long long* nxt = ctx.buf_c.as<unsigned long long>();
+    std::vector<hipEvent_t> evs(2 * max_passes);
+    for (auto& e : evs) NS_HIP(hipEventCreate(&e));
+    uint32_t p = 0;
+    for (; p < max_passes; p++) {
+      NS_HIP(hipEventRecord(evs[2 * p], ctx.stream));
+      if (p == 0) {
+        NS_HIP(launch_jump_first(ctx.stream, ctx.buf_a.as<unsigned int>(), cur, states, n));
+      } else {
+        NS_HIP(launch_jump_pass(ctx.stream, cur, nxt, states < 2 ? 2 : states, n));
+        std::swap(cur, nxt);
+      }
+      NS_HIP(hipEventRecord(evs[2 * p + 1], ctx.stream));
+      if (first_pass_mode) {
+        NS_HIP(launch_fill_u64(ctx.stream, stats, 0ull, 1));
+        NS_HIP(launch_fill_u64(ctx.stream, stats + 1, ~0ull, 1));
+        NS_HIP(launch_starts(ctx.stream, cur, n, unknowns, nullptr, nullptr, stats));
+        NS_HIP(hipMemcpyAsync(host, stats, sizeof(host), hipMemcpyDeviceToHost, ctx.stream));
+        NS_HIP(hipStreamSynchronize(ctx.stream));
+        if (host[0] != 0) {
+          p++;
+          break;
+        }
+      }
+    }
+    NS_HIP(hipStreamSynchronize(ctx.stream));
+    r.passes = p;
+    for (uint32_t i = 0; i < p; i++) {
+      float ms = 0;
+      NS_HIP(hipEventElapsedTime(&ms, evs[2 * i], evs[2 * i + 1]));
+      ctx.pass_ms.push_back(ms);
+      r.jump_ms += ms;
+    }
+    for (auto& e : evs) (void)hipEventDestroy(e);
+    final_tab = cur;
+    NS_HIP(launch_fill_u64(ctx.stream, stats, 0ull, 1));
+    NS_HIP(launch_fill_u64(ctx.stream, stats + 1, ~0ull, 1));
+    NS_HIP(launch_starts(ctx.stream, final_tab, n, unknowns, d_steps, d_ends, stats));
+    NS_HIP(hipMemcpyAsync(host, stats, sizeof(host), hipMemcpyDeviceToHost, ctx.stream));
+  } else {
+    NS_HIP(launch_fill_u32(ctx.stream, d_steps, 0u, 2 * starts));
+    host[0] = 0;
+  }
+  NS_HIP(hipStreamSynchronize(ctx.stream));
+  r.n_stopped = host[0];
+  if (r.n_stopped) {
+    r.lowest_u = host[1];
+    r.start = r.lowest_u << (n - unknowns);
+    unsigned int se[2];
+    NS_HIP(hipMemcpy(&se[0], d_steps + r.lowest_u, 4, hipMemcpyDeviceToHost));
+    NS_HIP(hipMemcpy(&se[1], d_ends + r.lowest_u, 4, hipMemcpyDeviceToHost));
+    r.steps = se[0];
+    r.end = se[1];
+  }
+  if (steps_out) NS_HIP(hipMemcpy(steps_out, d_steps, starts * 4, hipMemcpyDeviceToHost));
+  if (ends_out) NS_HIP(hipMemcpy(ends_out, d_ends, starts * 4, hipMemcpyDeviceToHost));
+  r.wall_ms = ms_since(t0);
+}
+
+const std::vector<double>& last_pass_times(Context& ctx) { return ctx.pass_ms; }
+
+}  // namespace ns

@@ -1,0 +1,113 @@
+"""SAT parity: HIP engine vs the CPU oracle (count and lowest assignment, bit-exact)."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import naivesat_amd as ns
+import oracle
+from helpers import random_circuit
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "configs.json")
+
+
+def check(ctx, d, **kw):
+    c = ns.Circuit.from_arrays(d["n"], d["in0"], d["in1"], d["func"], d["out_wire"], d["out_neg"])
+    r = ctx.sat_solve(c, **kw)
+    cnt, low = oracle.OracleCircuit.from_dict(d).sat(threads=8)
+    assert r["count"] == cnt, (d["n"], kw)
+    assert bool(r["found"]) == (cnt > 0)
+    if cnt:
+        assert r["lowest"] == low, (d["n"], kw)
+    return r
+
+
+@pytest.mark.parametrize("n", list(range(1, 21)))
+def test_random_circuits_all_sizes(ctx, n):
+    rng = np.random.default_rng(1000 + n)
+    d = random_circuit(rng, n, int(rng.integers(1, 120)), recent=int(rng.integers(4, 40)))
+    check(ctx, d)
+
+
+@pytest.mark.parametrize("group_len,wpl", [(64, 1), (128, 2), (256, 4), (512, 8), (1024, 1), (64, 16), (256, 32)])
+def test_launch_geometries(ctx, group_len, wpl):
+    rng = np.random.default_rng(77)
+    d = random_circuit(rng, 22, 150, recent=30)
+    check(ctx, d, group_len=group_len, words_per_lane=wpl)
+
+
+def test_many_launch_ranges(ctx):
+    rng = np.random.default_rng(5)
+    d = random_circuit(rng, 21, 90, recent=25)
+    r = check(ctx, d, elem_bits=14)
+    assert r["launches"] > 1
+
+
+def test_stop_at_first_returns_global_lowest(ctx):
+    c, key = ns.Circuit.gen_sat(21, 26, 600, 20)
+    d = c.arrays()
+    r = ctx.sat_solve(c, elem_bits=20, stop_at_first=True)
+    full = ctx.sat_solve(c)
+    assert r["found"] and r["lowest"] == full["lowest"] <= key
+    assert r["count"] <= full["count"]
+    lo = full["lowest"]
+    # the oracle checks the words up to the lowest hit only
+    cnt, low = oracle.OracleCircuit.from_dict(d).sat_range(0, (lo >> 6) + 1, threads=8)
+    assert low == lo
+
+
+def test_unsatisfiable_and_tautology(ctx):
+    # x0 & ~x0 (never) and x0 | ~x0 (always) on 20 inputs
+    never = ns.Circuit.from_arrays(20, [0], [0], [2], [20], [0])
+    always = ns.Circuit.from_arrays(20, [0], [0], [2], [20], [1])
+    r = ctx.sat_solve(never)
+    assert r["count"] == 0 and not r["found"]
+    r = ctx.sat_solve(always)
+    assert r["count"] == 1 << 20 and r["lowest"] == 0
+
+
+def test_output_is_an_input(ctx):
+    c = ns.Circuit.from_arrays(17, [], [], [], [16], [0])
+    r = ctx.sat_solve(c)
+    assert r["count"] == 1 << 16 and r["lowest"] == 1 << 16
+
+
+def test_planted_key_circuits(ctx):
+    for seed in range(3):
+        c, key = ns.Circuit.gen_sat(50 + seed, 24, 1200, 24)
+        r = ctx.sat_solve(c)
+        assert r["found"] and r["lowest"] <= key
+        cnt, low = oracle.OracleCircuit.from_dict(c.arrays()).sat(threads=8)
+        assert (r["count"], r["lowest"]) == (cnt, low)
+
+
+def golden(name):
+    if not os.path.exists(GOLDEN):
+        pytest.skip("configs.json not generated")
+    g = json.load(open(GOLDEN))
+    if name not in g:
+        pytest.skip(f"{name} golden not generated")
+    return g[name]
+
+
+def test_config0_golden_and_oracle(ctx):
+    g = golden("cfg0")
+    c, key = ns.Circuit.gen_sat(1, 24, 2000, 32)
+    assert key == g["key"]
+    r = ctx.sat_solve(c)
+    assert (r["count"], r["lowest"]) == (g["count"], g["lowest"])
+    cnt, low = oracle.OracleCircuit.from_dict(c.arrays()).sat(threads=8)
+    assert (cnt, low) == (g["count"], g["lowest"])
+
+
+def test_config1_full_size_golden(ctx):
+    g = golden("cfg1")
+    c, key = ns.Circuit.gen_sat(2, 36, 20000, 40)
+    assert key == g["key"]
+    r = ctx.sat_solve(c)
+    assert r["complete"] == 1
+    assert (r["count"], r["lowest"]) == (g["count"], g["lowest"])
+    # the planted key satisfies, so it bounds the lowest answer
+    assert r["lowest"] <= key

@@ -1,0 +1,106 @@
+"""SATOSTOP parity: device table + pointer jumping vs the CPU oracle (per unknown: stops
+or not, evaluations until STOP and the end state, bit-exact)."""
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+import naivesat_amd as ns
+import oracle
+from helpers import identity_circuit, random_circuit
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "configs.json")
+
+
+def mk(d):
+    return ns.Circuit.from_arrays(d["n"], d["in0"], d["in1"], d["func"], d["out_wire"], d["out_neg"])
+
+
+def check(ctx, d, U, **kw):
+    r = ctx.satostop_solve(mk(d), U, **kw)
+    o = oracle.OracleCircuit.from_dict(d).satostop(U, threads=8)
+    assert np.array_equal(r["steps_arr"], o["steps"]), (d["n"], U)
+    assert np.array_equal(r["ends_arr"], o["ends"]), (d["n"], U)
+    assert r["n_stopped"] == o["n_stopped"]
+    if o["n_stopped"]:
+        lo = o["lowest_u"]
+        assert r["lowest_u"] == lo
+        assert r["start"] == lo << (d["n"] - U)
+        assert (r["steps"], r["end"]) == (o["steps"][lo], o["ends"][lo])
+    return r, o
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 6, 8, 11, 14, 17, 20])
+def test_random_next_state_circuits(ctx, n):
+    rng = np.random.default_rng(5000 + n)
+    d = random_circuit(rng, n, int(rng.integers(n + 1, 4 * n + 20)), nout=n + 1, recent=3 * n + 4)
+    # make STOP rarer: AND the last output with a few more wires
+    check(ctx, d, int(rng.integers(0, n + 1)))
+
+
+@pytest.mark.parametrize("half", [4, 6, 8, 10])
+def test_feistel_workloads(ctx, half):
+    c, target = ns.Circuit.gen_satostop(70 + half, half, 6, 12 * half, min(2 * half, 12))
+    d = c.arrays()
+    r, o = check(ctx, d, min(2 * half, 12))
+    assert r["any_stop"] == 1
+
+
+def test_first_pass_mode_matches_reference_rule(ctx):
+    # greedysatostop ends after the first jump pass p where some start has stopped; the
+    # entries then cover 2^p evaluations, so the candidates are starts with steps <= 2^p
+    c, _ = ns.Circuit.gen_satostop(91, 9, 6, 100, 12)
+    d = c.arrays()
+    o = oracle.OracleCircuit.from_dict(d).satostop(12, threads=8)
+    r = ctx.satostop_solve(c, 12, first_pass=True)
+    steps = o["steps"]
+    assert o["n_stopped"] > 0
+    p = max(1, int(np.ceil(np.log2(steps[steps > 0].min()))))
+    cand = np.nonzero((steps > 0) & (steps <= (1 << p)))[0]
+    assert r["passes"] == p
+    assert r["lowest_u"] == cand[0]
+    assert r["steps"] == steps[cand[0]]
+
+
+def test_no_stop_anywhere(ctx):
+    # STOP output tied to 0: no start stops, no passes run
+    n = 12
+    d = random_circuit(np.random.default_rng(1), n, 40, nout=n, recent=20)
+    d["in0"] = np.append(d["in0"], 0).astype(np.uint32)
+    d["in1"] = np.append(d["in1"], 0).astype(np.uint32)
+    d["func"] = np.append(d["func"], 2).astype(np.uint8)  # x0 & ~x0 = 0
+    d["out_wire"] = np.append(d["out_wire"], n + len(d["func"]) - 1).astype(np.uint32)
+    d["out_neg"] = np.append(d["out_neg"], 0).astype(np.uint8)
+    r, o = check(ctx, d, 6)
+    assert r["any_stop"] == 0 and r["n_stopped"] == 0 and r["passes"] == 0
+
+
+@pytest.mark.parametrize("m", [5, 20, 24])
+def test_table_identity_layout(ctx, m):
+    t = ctx.table_build(mk(identity_circuit(m)))
+    assert np.array_equal(t, np.arange(1 << m, dtype=np.uint32))
+
+
+def test_table_matches_oracle(ctx):
+    rng = np.random.default_rng(9)
+    d = random_circuit(rng, 18, 200, nout=19, recent=40)
+    t = ctx.table_build(mk(d))
+    assert np.array_equal(t, oracle.OracleCircuit.from_dict(d).table_u32())
+
+
+def test_config3_full_size_golden(ctx):
+    if not os.path.exists(GOLDEN) or "cfg3" not in json.load(open(GOLDEN)):
+        pytest.skip("cfg3 golden not generated")
+    g = json.load(open(GOLDEN))["cfg3"]
+    c, target = ns.Circuit.gen_satostop(4, 15, 6, 300, 14)
+    assert target == g["target"]
+    r = ctx.satostop_solve(c, 14)
+    assert r["passes"] == 30
+    assert r["n_stopped"] == g["n_stopped"]
+    assert hashlib.sha256(r["steps_arr"].tobytes()).hexdigest() == g["steps_sha256"]
+    assert hashlib.sha256(r["ends_arr"].tobytes()).hexdigest() == g["ends_sha256"]
+    if g["lowest_u"] is not None:
+        assert (r["lowest_u"], r["steps"], r["end"]) == (g["lowest_u"], g["steps"], g["end"])

@@ -1,0 +1,175 @@
+"""Pins the CPU oracle: reference golden vectors, closed forms and brute force."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from helpers import FUNC_TT, block_values, identity_circuit, py_eval, qsat_recursive, random_circuit
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def quants(s):
+    return [1 if ch == "a" else 0 for ch in s.lower() if ch in "ae"]
+
+
+def test_gate_truth_tables():
+    # closed forms: and, nor, nimpl (a & ~b), xor
+    expect = {0: lambda a, b: a & b, 1: lambda a, b: 1 - (a | b), 2: lambda a, b: a & (1 - b),
+              3: lambda a, b: a ^ b}
+    for f in range(4):
+        for neg in (0, 1):
+            d = {"n": 2, "in0": [0], "in1": [1], "func": [f], "out_wire": [2], "out_neg": [neg]}
+            oc = oracle.OracleCircuit.from_dict(d)
+            for x in range(4):
+                a, b = x & 1, (x >> 1) & 1
+                assert oc.eval(x) == expect[f](a, b) ^ neg
+                assert FUNC_TT[f][(a << 1) | b] == expect[f](a, b)
+
+
+def test_quant_reducer_reference_vectors():
+    cases = json.load(open(os.path.join(GOLDEN, "quant_reducer_cases.json")))["cases"]
+    assert len(cases) == 30
+    for i, c in enumerate(cases):
+        qr = oracle.QuantReducer(quants(c["quants"]))
+        items = [ch == "1" for ch in c["items"]]
+        for j, idx in enumerate(c["order"]):
+            qr.push(idx, items[idx])
+            if c["final_from"] is not None and j >= c["final_from"]:
+                assert qr.final_result() == c["expected"], (i, j)
+        assert qr.final_result() == c["expected"], i
+
+
+def test_quant_reducer_rejects_push_after_end():
+    qr = oracle.QuantReducer([0, 0])
+    for i in range(4):
+        qr.push(i, False)
+    with pytest.raises(AssertionError):
+        qr.push(0, True)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_oracle_sat_matches_brute_force(seed):
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(3, 11))
+    d = random_circuit(rng, n, int(rng.integers(5, 60)))
+    oc = oracle.OracleCircuit.from_dict(d)
+    sat = [x for x in range(1 << n) if py_eval(d, x)[0]]
+    cnt, low = oc.sat(threads=3)
+    assert cnt == len(sat)
+    assert low == (sat[0] if sat else None)
+    for x in range(1 << n):
+        assert oc.eval(x) & 1 == py_eval(d, x)[0]
+
+
+def test_oracle_sat_multiword_threads():
+    rng = np.random.default_rng(42)
+    d = random_circuit(rng, 14, 80, recent=20)
+    oc = oracle.OracleCircuit.from_dict(d)
+    sat = [x for x in range(1 << 14) if py_eval(d, x)[0]]
+    assert oc.sat(threads=7) == (len(sat), sat[0] if sat else None)
+    # range form: words [8, 100) of 64 assignments
+    part = [x for x in sat if 8 * 64 <= x < 100 * 64]
+    assert oc.sat_range(8, 100, threads=2) == (len(part), part[0] if part else None)
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_oracle_qsat_matches_recursion_and_quant_reducer(seed):
+    rng = np.random.default_rng(100 + seed)
+    n = int(rng.integers(2, 10))
+    d = random_circuit(rng, n, int(rng.integers(4, 40)))
+    q = [int(v) for v in rng.integers(0, 2, n)]
+    oc = oracle.OracleCircuit.from_dict(d)
+    r = oc.qsat(q, threads=2)
+    assert r["truth"] == qsat_recursive(d, q)
+    vals = block_values(d, q)
+    assert list(r["bitmap"]) == vals
+    wit = [u for u, v in enumerate(vals) if v != q[0]]
+    assert r["witness_count"] == len(wit)
+    assert r["witness"] == (wit[0] if wit else None)
+    # the reference's small-n path: QuantReducer over the truth table with input 0 = MSB
+    qr = oracle.QuantReducer(q)
+    for v in range(1 << n):
+        if qr.final_result() is not None:
+            break
+        x = sum(((v >> (n - 1 - k)) & 1) << k for k in range(n))
+        qr.push(v, py_eval(d, x)[0])
+    fr = qr.final_result()
+    fb = r["first_bits"]
+    assert fr["solution_bits"] == fb
+    assert fr["reversed"] == bool(q[0])
+    if wit:
+        assert fr["solution"] == int(format(wit[0], f"0{fb}b")[::-1], 2)
+    else:
+        assert fr["solution"] is None
+
+
+def test_oracle_qsat_multichunk():
+    # n > 20 exercises the chunked evaluation path against a slower direct evaluation
+    rng = np.random.default_rng(7)
+    n = 21
+    d = random_circuit(rng, n, 30, recent=12)
+    q = [0] * 3 + [1] * 6 + [0] * 6 + [1] * 6
+    oc = oracle.OracleCircuit.from_dict(d)
+    r = oc.qsat(q, threads=4)
+    # independent recomputation: numpy truth table over index v (input k = bit n-1-k of v),
+    # then pairwise reduction level by level from the innermost input outwards
+    v = np.arange(1 << n, dtype=np.uint32)
+    w = [((v >> (n - 1 - k)) & 1).astype(np.uint8) for k in range(n)]
+    for a, b, f in zip(d["in0"], d["in1"], d["func"]):
+        tt = np.array(FUNC_TT[int(f)], np.uint8)
+        w.append(tt[(w[int(a)] << 1) | w[int(b)]])
+    vals = w[int(d["out_wire"][0])] ^ np.uint8(d["out_neg"][0])
+    for lvl in range(n - 3):
+        pairs = vals.reshape(-1, 2)
+        vals = (pairs[:, 0] & pairs[:, 1]) if q[n - 1 - lvl] else (pairs[:, 0] | pairs[:, 1])
+    assert r["first_bits"] == 3
+    assert list(r["bitmap"]) == list(vals)
+    assert r["truth"] == int(vals.any())
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_oracle_satostop_matches_walk(seed):
+    rng = np.random.default_rng(200 + seed)
+    n = int(rng.integers(3, 9))
+    d = random_circuit(rng, n, int(rng.integers(n + 1, 50)), nout=n + 1)
+    oc = oracle.OracleCircuit.from_dict(d)
+    U = int(rng.integers(0, n + 1))
+    r = oc.satostop(U, threads=2)
+    lowest = None
+    for u in range(1 << U):
+        steps, end = oc.walk(u << (n - U), 1 << n)
+        assert r["steps"][u] == steps
+        assert r["ends"][u] == (end if steps else 0)
+        if steps and lowest is None:
+            lowest = u
+    assert r["lowest_u"] == lowest
+    assert r["n_stopped"] == int((r["steps"] > 0).sum())
+
+
+def test_walk_matches_python_semantics():
+    rng = np.random.default_rng(5)
+    n = 6
+    d = random_circuit(rng, n, 40, nout=n + 1)
+    oc = oracle.OracleCircuit.from_dict(d)
+    for start in range(0, 1 << n, 7):
+        s, steps, end = start, 0, 0
+        for step in range(1, (1 << n) + 1):
+            o = py_eval(d, s)
+            s = sum(b << i for i, b in enumerate(o[:n]))
+            if o[n]:
+                steps, end = step, s
+                break
+        assert oc.walk(start, 1 << n) == (steps, end)
+
+
+@pytest.mark.parametrize("m", [5, 12, 24])
+def test_table_identity_layout(m):
+    # identity circuit: entry s packs output i (= input i) at bit i, so entry s == s
+    # (layout of src/satostop.rs:1788-1915)
+    if m > 20:
+        pytest.skip("identity table above 2^20 entries is covered on the GPU")
+    t = oracle.OracleCircuit.from_dict(identity_circuit(m)).table_u32()
+    assert np.array_equal(t, np.arange(1 << m, dtype=np.uint32))
