@@ -61,22 +61,153 @@ void emit_inputs(std::ostringstream& os, const Circuit& c, const std::vector<uin
   }
 }
 
-// Emits the gates of the given order.
-void emit_gates(std::ostringstream& os, const Circuit& c, const std::vector<uint32_t>& order) {
-  for (uint32_t w : order) {
-    const Gate& gt = c.gates[w - c.n];
-    std::string a = wname(c, gt.i0), b = wname(c, gt.i1);
-    os << "    const unsigned int g" << w << " = ";
-    switch (gt.func) {
-      case FUNC_AND: os << a << " & " << b; break;
-      case FUNC_NOR: os << "~(" << a << " | " << b << ")"; break;
-      case FUNC_NIMPL: os << a << " & ~" << b; break;
-      default: os << a << " ^ " << b; break;
-    }
-    os << ";\n";
+// ---------------------------------------------------------------------------
+// 3-input LUT mapping.
+//
+// Every emitted operation is one v_bitop3_b32: an arbitrary boolean function of up to
+// three 32-bit operands given by an 8-bit truth table (bit (s0<<2 | s1<<1 | s2)). The
+// mapper walks the gates in evaluation order and folds a gate into its consumer when the
+// gate has a single use and the combined function still has at most three distinct
+// leaves; negations are absorbed into the truth tables. The LUTs are emitted as inline
+// assembly so that each one is exactly one VALU instruction and the compiler does not
+// re-combine them (gfx950's compiler-formed bitop3 combinations have been observed to
+// compute wrong functions for some gate patterns).
+// ---------------------------------------------------------------------------
+struct Expr {
+  uint8_t nl = 0;
+  uint32_t leaf[3] = {0, 0, 0};
+  uint8_t tt = 0;  // bit (v0<<2 | v1<<1 | v2) where vi is the value of leaf i
+};
+
+inline Expr leaf_expr(uint32_t w) {
+  Expr e;
+  e.nl = 1;
+  e.leaf[0] = e.leaf[1] = e.leaf[2] = w;
+  e.tt = 0xF0;  // value = bit of s0
+  return e;
+}
+
+// value of expression e for leaf assignment given as (wire -> bit) lookup
+inline uint32_t eval_expr(const Expr& e, const uint32_t* ul, uint32_t un, uint32_t idx) {
+  uint32_t bits[3] = {0, 0, 0};
+  for (uint32_t i = 0; i < e.nl; i++) {
+    for (uint32_t k = 0; k < un; k++)
+      if (ul[k] == e.leaf[i]) {
+        bits[i] = (idx >> (2 - k)) & 1u;
+        break;
+      }
+  }
+  // leaves beyond nl repeat leaf 0 (their table bits are don't-care but consistent)
+  for (uint32_t i = e.nl; i < 3; i++) bits[i] = bits[0];
+  return (e.tt >> ((bits[0] << 2) | (bits[1] << 1) | bits[2])) & 1u;
+}
+
+inline uint32_t gate_fn(uint8_t f, uint32_t a, uint32_t b) {
+  switch (f) {
+    case FUNC_AND: return a & b;
+    case FUNC_NOR: return (~(a | b)) & 1u;
+    case FUNC_NIMPL: return a & (~b & 1u);
+    default: return a ^ b;
   }
 }
 
+class LutMapper {
+ public:
+  LutMapper(const Circuit& c, std::ostringstream& os) : c_(c), os_(os), state_(c.wires(), 0), pend_(c.wires()) {}
+
+  // maps the gates in `order`; outputs in `out_wires` are materialised
+  void run(const std::vector<uint32_t>& order, const std::vector<uint32_t>& out_wires) {
+    std::vector<uint32_t> fan(c_.wires(), 0);
+    for (uint32_t w : order) {
+      const Gate& g = c_.gates[w - c_.n];
+      fan[g.i0]++;
+      if (g.i1 != g.i0) fan[g.i1]++;
+    }
+    for (uint32_t w : out_wires) fan[w] += 2;
+    for (uint32_t w : order) {
+      const Gate& g = c_.gates[w - c_.n];
+      Expr ea = get(g.i0), eb = get(g.i1);
+      uint32_t ul[6], un;
+      while (true) {
+        un = 0;
+        for (const Expr* e : {&ea, &eb})
+          for (uint32_t i = 0; i < e->nl; i++) {
+            bool dup = false;
+            for (uint32_t k = 0; k < un; k++) dup |= ul[k] == e->leaf[i];
+            if (!dup) ul[un++] = e->leaf[i];
+          }
+        if (un <= 3) break;
+        // materialise the pending operand with more leaves
+        uint32_t x = (ea.nl >= eb.nl && state_[g.i0] == 1) ? g.i0 : (state_[g.i1] == 1 ? g.i1 : g.i0);
+        materialize(x);
+        ea = get(g.i0);
+        eb = get(g.i1);
+      }
+      Expr r;
+      r.nl = (uint8_t)un;
+      for (uint32_t k = 0; k < 3; k++) r.leaf[k] = ul[k < un ? k : 0];
+      uint8_t tt = 0;
+      for (uint32_t idx = 0; idx < 8; idx++) {
+        uint32_t a = eval_expr(ea, ul, un, idx), b = eval_expr(eb, ul, un, idx);
+        if (gate_fn(g.func, a, b)) tt |= (uint8_t)(1u << idx);
+      }
+      // leaves beyond un repeat leaf 0: make the table consistent for those positions
+      r.tt = normalize(tt, un);
+      pend_[w] = r;
+      state_[w] = 1;
+      if (fan[w] != 1) materialize(w);
+    }
+  }
+
+  uint32_t ops() const { return ops_; }
+
+ private:
+  // table over (s0,s1,s2) where s_k for k >= un equals s0
+  static uint8_t normalize(uint8_t tt, uint32_t un) {
+    uint8_t r = 0;
+    for (uint32_t idx = 0; idx < 8; idx++) {
+      uint32_t b0 = (idx >> 2) & 1, b1 = (idx >> 1) & 1, b2 = idx & 1;
+      if (un < 2) b1 = b0;
+      if (un < 3) b2 = b0;
+      uint32_t src = (b0 << 2) | (b1 << 1) | b2;
+      if ((tt >> src) & 1) r |= (uint8_t)(1u << idx);
+    }
+    return r;
+  }
+
+  Expr get(uint32_t w) {
+    if (w < c_.n || state_[w] == 2) return leaf_expr(w);
+    return pend_[w];
+  }
+
+  void materialize(uint32_t w) {
+    if (w < c_.n || state_[w] == 2) return;
+    const Expr& e = pend_[w];
+    os_ << "    unsigned int g" << w << ";\n    asm(\"v_bitop3_b32 %0, %1, %2, %3 bitop3:0x" << std::hex
+        << (unsigned)e.tt << std::dec << "\" : \"=v\"(g" << w << ") : \"v\"(" << wname(c_, e.leaf[0]) << "), \"v\"("
+        << wname(c_, e.leaf[1]) << "), \"v\"(" << wname(c_, e.leaf[2]) << "));\n";
+    state_[w] = 2;
+    ops_++;
+  }
+
+  const Circuit& c_;
+  std::ostringstream& os_;
+  std::vector<uint8_t> state_;  // 0 not reached, 1 pending expression, 2 materialised
+  std::vector<Expr> pend_;
+  uint32_t ops_ = 0;
+};
+
+// Emits the gates of the given order as mapped LUTs; returns the number of VALU ops.
+uint32_t emit_gates(std::ostringstream& os, const Circuit& c, const std::vector<uint32_t>& order,
+                    const std::vector<uint32_t>& outs) {
+  std::vector<uint32_t> out_wires;
+  for (uint32_t k : outs) out_wires.push_back(c.outputs[k].wire);
+  LutMapper m(c, os);
+  m.run(order, out_wires);
+  return m.ops();
+}
+
+// Output k as an expression of materialised values (negation as one more VALU op).
 std::string out_expr(const Circuit& c, uint32_t k) {
   const Output& o = c.outputs.at(k);
   std::string s = wname(c, o.wire);
@@ -129,7 +260,8 @@ std::string gen_sat_kernel(const Circuit& c, const Geometry& g, GenInfo* info) {
   os << "    const unsigned long long w = U + ((unsigned long long)lane << " << g.lane_shift
      << ") + ((unsigned long long)j << " << g.j_shift << ");\n";
   emit_inputs(os, c, vb, used_inputs(c, order, outs), g);
-  emit_gates(os, c, order);
+  uint32_t ops = emit_gates(os, c, order, outs);
+  if (info) info->valu_ops = ops;
   os << "    unsigned int o = " << out_expr(c, 0) << " & 0x" << std::hex << word_valid_bits(c.n)
      << std::dec << "u;\n";
   os << "    if (w >= wend) o = 0u;\n";
@@ -194,7 +326,8 @@ std::string gen_qsat_kernel(const Circuit& c, const std::vector<uint8_t>& q, con
   os << "#pragma unroll 1\n  for (unsigned int j = 0; j < " << g.wpl << "u; j++) {\n";
   os << "    const unsigned long long w = wl + j;\n";
   emit_inputs(os, c, vb, used_inputs(c, order, outs), g);
-  emit_gates(os, c, order);
+  uint32_t ops = emit_gates(os, c, order, outs);
+  if (info) info->valu_ops = ops;
   os << "    unsigned int o = " << out_expr(c, 0) << ";\n";
   uint32_t inword = L < 5 ? L : 5;
   for (uint32_t l = 0; l < inword; l++)
@@ -297,7 +430,8 @@ std::string gen_table_kernel(const Circuit& c, const Geometry& g, GenInfo* info)
   os << "    const unsigned long long w = U + ((unsigned long long)lane << " << g.lane_shift
      << ") + ((unsigned long long)j << " << g.j_shift << ");\n";
   emit_inputs(os, c, vb, used_inputs(c, order, outs), g);
-  emit_gates(os, c, order);
+  uint32_t ops = emit_gates(os, c, order, outs);
+  if (info) info->valu_ops = ops;
   os << "    unsigned int A[32];\n";
   for (uint32_t k = 0; k < 32; k++) {
     if (k < m) os << "    A[" << k << "] = " << out_expr(c, k) << ";\n";

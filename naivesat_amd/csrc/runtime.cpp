@@ -3,6 +3,7 @@
 #include <hip/hiprtc.h>
 
 #include <chrono>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -30,9 +31,24 @@ std::shared_ptr<JitModule> Jit::get(const std::string& src, const char* kernel, 
   hiprtcProgram prog;
   check_rtc(hiprtcCreateProgram(&prog, src.c_str(), "ns_kernel.hip", 0, nullptr, nullptr),
             "hiprtcCreateProgram");
-  const char* opts[] = {"--offload-arch=gfx950", "-O3", "-Xclang", "-target-feature", "-Xclang",
-                        "-packed-fp32-ops"};
-  hiprtcResult r = hiprtcCompileProgram(prog, (int)(sizeof(opts) / sizeof(opts[0])), opts);
+  // NS_HIPRTC_OPT overrides the optimisation level (e.g. "-O1") for debugging
+  const char* opt_env = std::getenv("NS_HIPRTC_OPT");
+  std::vector<const char*> opts = {"--offload-arch=gfx950", opt_env && *opt_env ? opt_env : "-O3",
+                                   "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+  const char* extra_env = std::getenv("NS_HIPRTC_EXTRA");
+  std::vector<std::string> extra;
+  if (extra_env && *extra_env) {
+    std::string s(extra_env);
+    size_t p = 0;
+    while (p < s.size()) {
+      size_t q = s.find(' ', p);
+      if (q == std::string::npos) q = s.size();
+      if (q > p) extra.push_back(s.substr(p, q - p));
+      p = q + 1;
+    }
+    for (auto& e : extra) opts.push_back(e.c_str());
+  }
+  hiprtcResult r = hiprtcCompileProgram(prog, (int)opts.size(), opts.data());
   if (r != HIPRTC_SUCCESS) {
     size_t ls = 0;
     hiprtcGetProgramLogSize(prog, &ls);

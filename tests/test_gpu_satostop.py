@@ -65,6 +65,18 @@ def test_first_pass_mode_matches_reference_rule(ctx):
     assert r["steps"] == steps[cand[0]]
 
 
+def test_fused_gate_pattern_regression(ctx):
+    # nine gates over lane-mask inputs and a constant mask whose compiler-fused form
+    # computed a wrong table; each LUT is now one explicit v_bitop3
+    c = ns.Circuit.parse("{0 1 2 3 4 5 6 7 8 9 10 11 xor(9,10) nimpl(6,7) xor(13,12) nimpl(12,13) "
+                         "nor(8,3) xor(15,14) nimpl(16,15) nimpl(17,16) xor(19,18):0}")
+    d = c.arrays()
+    t = ctx.table_build(c)
+    assert np.array_equal(t, oracle.OracleCircuit.from_dict(d).table_u32())
+    r = ctx.sat_solve(c)
+    assert (r["count"], r["lowest"]) == oracle.OracleCircuit.from_dict(d).sat(threads=4)
+
+
 def test_no_stop_anywhere(ctx):
     # STOP output tied to 0: no start stops, no passes run
     n = 12
