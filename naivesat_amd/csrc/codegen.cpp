@@ -113,7 +113,13 @@ inline uint32_t gate_fn(uint8_t f, uint32_t a, uint32_t b) {
 
 class LutMapper {
  public:
-  LutMapper(const Circuit& c, std::ostringstream& os) : c_(c), os_(os), state_(c.wires(), 0), pend_(c.wires()) {}
+  // uni_input[k] = 1 when input k is the same in every lane of a wave (constant word
+  // masks, loop and block bits); gates over uniform values only are uniform too and are
+  // computed once per wave with scalar instructions.
+  LutMapper(const Circuit& c, std::ostringstream& os, const std::vector<uint8_t>& uni_input)
+      : c_(c), os_(os), state_(c.wires(), 0), pend_(c.wires()), uni_(c.wires(), 0) {
+    for (uint32_t k = 0; k < c.n; k++) uni_[k] = uni_input[k];
+  }
 
   // maps the gates in `order`; outputs in `out_wires` are materialised
   void run(const std::vector<uint32_t>& order, const std::vector<uint32_t>& out_wires) {
@@ -126,6 +132,22 @@ class LutMapper {
     for (uint32_t w : out_wires) fan[w] += 2;
     for (uint32_t w : order) {
       const Gate& g = c_.gates[w - c_.n];
+      if (uni_[g.i0] && uni_[g.i1]) {
+        // wave-uniform gate: one scalar operation, always materialised
+        uni_[w] = 1;
+        std::string a = wname(c_, g.i0), b = wname(c_, g.i1);
+        os_ << "    const unsigned int g" << w << " = ";
+        switch (g.func) {
+          case FUNC_AND: os_ << a << " & " << b; break;
+          case FUNC_NOR: os_ << "~(" << a << " | " << b << ")"; break;
+          case FUNC_NIMPL: os_ << a << " & ~" << b; break;
+          default: os_ << a << " ^ " << b; break;
+        }
+        os_ << ";\n";
+        state_[w] = 2;
+        salu_++;
+        continue;
+      }
       Expr ea = get(g.i0), eb = get(g.i1);
       uint32_t ul[6], un;
       while (true) {
@@ -160,6 +182,7 @@ class LutMapper {
   }
 
   uint32_t ops() const { return ops_; }
+  uint32_t scalar_ops() const { return salu_; }
 
  private:
   // table over (s0,s1,s2) where s_k for k >= un equals s0
@@ -183,9 +206,25 @@ class LutMapper {
   void materialize(uint32_t w) {
     if (w < c_.n || state_[w] == 2) return;
     const Expr& e = pend_[w];
+    // one wave-uniform leaf may be read from a scalar register; the others must be VGPRs
+    bool sgpr_used = false;
+    std::string cons[3];
+    for (int i = 0; i < 3; i++) {
+      bool first_use = true;
+      for (int k = 0; k < i; k++) first_use &= e.leaf[k] != e.leaf[i];
+      if (uni_[e.leaf[i]] && !sgpr_used && first_use) {
+        cons[i] = "s";
+        sgpr_used = true;
+        for (int k = i + 1; k < 3; k++)
+          if (e.leaf[k] == e.leaf[i]) cons[k] = "s";
+      } else if (cons[i].empty()) {
+        cons[i] = "v";
+      }
+    }
     os_ << "    unsigned int g" << w << ";\n    asm(\"v_bitop3_b32 %0, %1, %2, %3 bitop3:0x" << std::hex
-        << (unsigned)e.tt << std::dec << "\" : \"=v\"(g" << w << ") : \"v\"(" << wname(c_, e.leaf[0]) << "), \"v\"("
-        << wname(c_, e.leaf[1]) << "), \"v\"(" << wname(c_, e.leaf[2]) << "));\n";
+        << (unsigned)e.tt << std::dec << "\" : \"=v\"(g" << w << ") : \"" << cons[0] << "\"(" << wname(c_, e.leaf[0])
+        << "), \"" << cons[1] << "\"(" << wname(c_, e.leaf[1]) << "), \"" << cons[2] << "\"("
+        << wname(c_, e.leaf[2]) << "));\n";
     state_[w] = 2;
     ops_++;
   }
@@ -194,17 +233,33 @@ class LutMapper {
   std::ostringstream& os_;
   std::vector<uint8_t> state_;  // 0 not reached, 1 pending expression, 2 materialised
   std::vector<Expr> pend_;
+  std::vector<uint8_t> uni_;    // wave-uniform value
   uint32_t ops_ = 0;
+  uint32_t salu_ = 0;
 };
 
-// Emits the gates of the given order as mapped LUTs; returns the number of VALU ops.
+// Emits the gates of the given order as mapped LUTs; returns the number of VALU ops per
+// word and stores the number of scalar (per-wave) ops in *scalar_ops.
 uint32_t emit_gates(std::ostringstream& os, const Circuit& c, const std::vector<uint32_t>& order,
-                    const std::vector<uint32_t>& outs) {
+                    const std::vector<uint32_t>& outs, const std::vector<uint8_t>& uni_input,
+                    uint32_t* scalar_ops) {
   std::vector<uint32_t> out_wires;
   for (uint32_t k : outs) out_wires.push_back(c.outputs[k].wire);
-  LutMapper m(c, os);
+  LutMapper m(c, os, uni_input);
   m.run(order, out_wires);
+  if (scalar_ops) *scalar_ops = m.scalar_ops();
   return m.ops();
+}
+
+// wave-uniform inputs for an index mapping: everything but the six lane bits
+std::vector<uint8_t> uniform_inputs(const Circuit& c, const std::vector<uint32_t>& vbit_of_input,
+                                    const Geometry& g) {
+  std::vector<uint8_t> u(c.n, 1);
+  for (uint32_t k = 0; k < c.n; k++) {
+    uint32_t vb = vbit_of_input[k];
+    if (vb >= 5 && vb - 5 >= g.lane_shift && vb - 5 < g.lane_shift + 6) u[k] = 0;
+  }
+  return u;
 }
 
 // Output k as an expression of materialised values (negation as one more VALU op).
@@ -260,8 +315,12 @@ std::string gen_sat_kernel(const Circuit& c, const Geometry& g, GenInfo* info) {
   os << "    const unsigned long long w = U + ((unsigned long long)lane << " << g.lane_shift
      << ") + ((unsigned long long)j << " << g.j_shift << ");\n";
   emit_inputs(os, c, vb, used_inputs(c, order, outs), g);
-  uint32_t ops = emit_gates(os, c, order, outs);
-  if (info) info->valu_ops = ops;
+  uint32_t sops = 0;
+  uint32_t ops = emit_gates(os, c, order, outs, uniform_inputs(c, vb, g), &sops);
+  if (info) {
+    info->valu_ops = ops;
+    info->scalar_ops = sops;
+  }
   os << "    unsigned int o = " << out_expr(c, 0) << " & 0x" << std::hex << word_valid_bits(c.n)
      << std::dec << "u;\n";
   os << "    if (w >= wend) o = 0u;\n";
@@ -326,8 +385,12 @@ std::string gen_qsat_kernel(const Circuit& c, const std::vector<uint8_t>& q, con
   os << "#pragma unroll 1\n  for (unsigned int j = 0; j < " << g.wpl << "u; j++) {\n";
   os << "    const unsigned long long w = wl + j;\n";
   emit_inputs(os, c, vb, used_inputs(c, order, outs), g);
-  uint32_t ops = emit_gates(os, c, order, outs);
-  if (info) info->valu_ops = ops;
+  uint32_t sops = 0;
+  uint32_t ops = emit_gates(os, c, order, outs, uniform_inputs(c, vb, g), &sops);
+  if (info) {
+    info->valu_ops = ops;
+    info->scalar_ops = sops;
+  }
   os << "    unsigned int o = " << out_expr(c, 0) << ";\n";
   uint32_t inword = L < 5 ? L : 5;
   for (uint32_t l = 0; l < inword; l++)
@@ -430,8 +493,12 @@ std::string gen_table_kernel(const Circuit& c, const Geometry& g, GenInfo* info)
   os << "    const unsigned long long w = U + ((unsigned long long)lane << " << g.lane_shift
      << ") + ((unsigned long long)j << " << g.j_shift << ");\n";
   emit_inputs(os, c, vb, used_inputs(c, order, outs), g);
-  uint32_t ops = emit_gates(os, c, order, outs);
-  if (info) info->valu_ops = ops;
+  uint32_t sops = 0;
+  uint32_t ops = emit_gates(os, c, order, outs, uniform_inputs(c, vb, g), &sops);
+  if (info) {
+    info->valu_ops = ops;
+    info->scalar_ops = sops;
+  }
   os << "    unsigned int A[32];\n";
   for (uint32_t k = 0; k < 32; k++) {
     if (k < m) os << "    A[" << k << "] = " << out_expr(c, k) << ";\n";

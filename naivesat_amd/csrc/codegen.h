@@ -25,6 +25,7 @@ struct GenInfo {
   uint32_t cone_gates = 0;  // gates actually emitted
   uint32_t peak_live = 0;   // peak live values of the emitted order
   uint32_t valu_ops = 0;    // v_bitop3 LUT operations emitted per 32-assignment word
+  uint32_t scalar_ops = 0;  // wave-uniform gates (scalar instructions, once per wave and word)
 };
 
 // SAT: count of satisfying assignments and lowest one.
