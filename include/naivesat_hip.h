@@ -99,6 +99,10 @@ typedef struct {
   uint32_t complete;  /* 1 when the whole 2^n space was searched */
   uint32_t launches;
   uint32_t cone_gates;
+  uint32_t valu_ops;   /* v_bitop3 LUTs per 32-assignment word */
+  uint32_t scalar_ops; /* wave-uniform gates per word (scalar unit) */
+  uint32_t backend;    /* 1 = native assembly path, 0 = compiler path */
+  uint32_t vgpr_class; /* VGPR budget of the kernel (native path) */
   double kernel_ms;   /* device time of the search launches (HIP events) */
   double compile_ms;  /* run-time compilation (0 when cached) */
   double wall_ms;
@@ -127,6 +131,10 @@ typedef struct {
   uint64_t witness_count; /* number of u values that are witnesses */
   uint32_t passes;        /* reduction passes after the circuit kernel */
   uint32_t cone_gates;
+  uint32_t valu_ops;
+  uint32_t scalar_ops;
+  uint32_t backend;
+  uint32_t vgpr_class;
   double kernel_ms;
   double compile_ms;
   double wall_ms;
@@ -161,7 +169,9 @@ typedef struct {
   uint64_t end;        /* state produced by the stopping evaluation */
   uint64_t steps;      /* evaluations until STOP, the stopping one included */
   uint32_t cone_gates;
-  uint32_t reserved;
+  uint32_t valu_ops;
+  uint32_t backend;
+  uint32_t vgpr_class;
   double build_ms;     /* table build kernel time */
   double jump_ms;      /* total jump-pass kernel time */
   double compile_ms;
@@ -178,6 +188,15 @@ int ns_satostop_solve(ns_ctx* ctx, const ns_circuit* c, uint32_t unknowns,
  * At most 32 outputs and n <= 31. */
 int ns_table_build(ns_ctx* ctx, const ns_circuit* c, uint32_t* table_out, uint64_t table_len,
                    double* kernel_ms);
+
+/* Diagnostic (no counterpart in the reference; no device needed): builds the native-path
+ * kernel for a circuit, assembles and links it, and returns the circuit body's assembly.
+ * kind: 0 SAT (output 0), 1 QSAT one-bit-per-thread, 2 QSAT outer-block-in-thread,
+ * 3 next-state table (all outputs). Input k sits at index bit k (SAT, table) or n-1-k
+ * (QSAT); lane_shift places the six lane bits at index bits 5+lane_shift..10+lane_shift.
+ * Returns NS_ERR_UNSUPPORTED when the circuit exceeds the native register budget. */
+int ns_native_body(const ns_circuit* c, uint32_t kind, uint32_t lane_shift, char* body, size_t cap,
+                   size_t* needed, uint32_t* vgpr_class, uint64_t* code_bytes);
 
 /* Per-pass timing of the last ns_satostop_solve on this context: pass_ms[i] for jump pass i
  * (HIP events on the context stream). Returns the number of passes written. */

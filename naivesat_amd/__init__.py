@@ -42,7 +42,8 @@ class _SatOpts(C.Structure):
 class _SatResult(C.Structure):
     _fields_ = [("count", C.c_uint64), ("lowest", C.c_uint64), ("found", C.c_uint32),
                 ("complete", C.c_uint32), ("launches", C.c_uint32), ("cone_gates", C.c_uint32),
-                ("kernel_ms", C.c_double), ("compile_ms", C.c_double), ("wall_ms", C.c_double)]
+                ("valu_ops", C.c_uint32), ("scalar_ops", C.c_uint32), ("backend", C.c_uint32),
+                ("vgpr_class", C.c_uint32), ("kernel_ms", C.c_double), ("compile_ms", C.c_double), ("wall_ms", C.c_double)]
 
 
 _QsatOpts = _SatOpts
@@ -51,7 +52,9 @@ _QsatOpts = _SatOpts
 class _QsatResult(C.Structure):
     _fields_ = [("truth", C.c_uint32), ("first_bits", C.c_uint32), ("first_all", C.c_uint32),
                 ("has_witness", C.c_uint32), ("witness", C.c_uint64), ("witness_count", C.c_uint64),
-                ("passes", C.c_uint32), ("cone_gates", C.c_uint32), ("kernel_ms", C.c_double),
+                ("passes", C.c_uint32), ("cone_gates", C.c_uint32), ("valu_ops", C.c_uint32),
+                ("scalar_ops", C.c_uint32), ("backend", C.c_uint32), ("vgpr_class", C.c_uint32),
+                ("kernel_ms", C.c_double),
                 ("compile_ms", C.c_double), ("wall_ms", C.c_double)]
 
 
@@ -63,7 +66,8 @@ class _StopOpts(C.Structure):
 class _StopResult(C.Structure):
     _fields_ = [("any_stop", C.c_uint32), ("passes", C.c_uint32), ("n_stopped", C.c_uint64),
                 ("lowest_u", C.c_uint64), ("start", C.c_uint64), ("end", C.c_uint64),
-                ("steps", C.c_uint64), ("cone_gates", C.c_uint32), ("reserved", C.c_uint32),
+                ("steps", C.c_uint64), ("cone_gates", C.c_uint32), ("valu_ops", C.c_uint32),
+                ("backend", C.c_uint32), ("vgpr_class", C.c_uint32),
                 ("build_ms", C.c_double), ("jump_ms", C.c_double), ("compile_ms", C.c_double),
                 ("wall_ms", C.c_double)]
 
@@ -102,6 +106,8 @@ def load_library() -> C.CDLL:
         "ns_satostop_solve": (I, [P, P, U32, C.POINTER(_StopOpts), C.POINTER(_StopResult), P, P]),
         "ns_table_build": (I, [P, P, P, U64, C.POINTER(C.c_double)]),
         "ns_satostop_pass_times": (I, [P, P, U32, C.POINTER(U32)]),
+        "ns_native_body": (I, [P, U32, U32, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t), C.POINTER(U32),
+                               C.POINTER(U64)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -117,7 +123,7 @@ def exported_symbols() -> list:
             "ns_circuit_from_arrays", "ns_circuit_info", "ns_circuit_to_arrays", "ns_circuit_format",
             "ns_circuit_eval", "ns_circuit_cone", "ns_circuit_free", "ns_gen_sat", "ns_gen_qsat",
             "ns_gen_satostop", "ns_ctx_create", "ns_ctx_destroy", "ns_sat_solve", "ns_qsat_solve",
-            "ns_satostop_solve", "ns_table_build", "ns_satostop_pass_times"]
+            "ns_satostop_solve", "ns_table_build", "ns_satostop_pass_times", "ns_native_body"]
 
 
 def _check(status: int) -> None:
@@ -249,6 +255,19 @@ class Circuit:
         out = np.zeros(o, np.uint8)
         _check(load_library().ns_circuit_eval(self._h, _ptr(a), _ptr(out)))
         return [int(v) for v in out]
+
+    def native_body(self, kind: int = 0, lane_shift: int = 0):
+        """Assembly of the circuit body on the native path and its register class
+        (kind 0 SAT, 1/2 QSAT, 3 table); assembled and linked on the host as a check."""
+        lib = load_library()
+        need = C.c_size_t()
+        cls = C.c_uint32()
+        size = C.c_uint64()
+        _check(lib.ns_native_body(self._h, kind, lane_shift, None, 0, C.byref(need), C.byref(cls), C.byref(size)))
+        buf = C.create_string_buffer(need.value + 1)
+        _check(lib.ns_native_body(self._h, kind, lane_shift, buf, need.value + 1, C.byref(need), C.byref(cls),
+                                  C.byref(size)))
+        return buf.value.decode(), cls.value, size.value
 
     def cone(self):
         """(gates in the output cone, peak live values of the kernel's order)."""

@@ -1,0 +1,38 @@
+// Native code path: circuit bodies written directly as gfx950 assembly into precompiled
+// kernel templates (templates/gen_templates.py) and assembled in-process.
+#pragma once
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "circuit.h"
+#include "codegen.h"
+
+namespace ns {
+
+enum class NativeKind { SAT, QSAT_A, QSAT_B, TABLE };
+
+// Thrown when a circuit does not fit the register budget of any template (the caller then
+// uses the compiler path).
+struct NativeUnfit : std::runtime_error {
+  explicit NativeUnfit(const std::string& m) : std::runtime_error(m) {}
+};
+
+struct NativeKernel {
+  std::string asm_text;     // complete assembly module
+  std::string kernel_name;  // symbol to launch
+  uint32_t vgpr_class = 0;  // 64 / 128 / 256
+  GenInfo info;
+};
+
+// Builds the assembly module for `kind`. vbit_of_input[k] = position of input k in the
+// assignment index; lane bits are index bits 5+lane_shift .. 10+lane_shift; outs = the
+// outputs evaluated (their values land in v0.. in order).
+NativeKernel build_native(const Circuit& c, NativeKind kind, const std::vector<uint32_t>& vbit_of_input,
+                          uint32_t lane_shift, const std::vector<uint32_t>& outs);
+
+// Number of embedded templates and lookup (templates_data.cpp, generated at build time).
+const char* native_template(const std::string& kernel_name);
+
+}  // namespace ns

@@ -5,6 +5,7 @@
 #include <string>
 
 #include "../../include/naivesat_hip.h"
+#include "asmgen.h"
 #include "circuit.h"
 #include "runtime.h"
 #include "solvers.h"
@@ -237,6 +238,40 @@ int ns_table_build(ns_ctx* ctx, const ns_circuit* c, uint32_t* table_out, uint64
     need(ctx && c, "null argument");
     double ms = ns::table_build(ctx->ctx, c->c, table_out, table_len);
     if (kernel_ms) *kernel_ms = ms;
+  });
+}
+
+int ns_native_body(const ns_circuit* c, uint32_t kind, uint32_t lane_shift, char* body, size_t cap,
+                   size_t* needed, uint32_t* vgpr_class, uint64_t* code_bytes) {
+  return guard([&] {
+    need(c && kind <= 3, "bad argument");
+    const ns::Circuit& cc = c->c;
+    std::vector<uint32_t> vb(cc.n), outs;
+    for (uint32_t k = 0; k < cc.n; k++) vb[k] = (kind == 1 || kind == 2) ? cc.n - 1 - k : k;
+    if (kind == 3)
+      for (uint32_t k = 0; k < cc.outputs.size(); k++) outs.push_back(k);
+    else
+      outs.push_back(0);
+    static const ns::NativeKind kinds[4] = {ns::NativeKind::SAT, ns::NativeKind::QSAT_A, ns::NativeKind::QSAT_B,
+                                            ns::NativeKind::TABLE};
+    ns::NativeKernel nk;
+    try {
+      nk = ns::build_native(cc, kinds[kind], vb, lane_shift, outs);
+    } catch (const ns::NativeUnfit& e) {
+      throw ns::Error(NS_ERR_UNSUPPORTED, e.what());
+    }
+    std::vector<char> code = ns::assemble_native(nk.asm_text);
+    // the body sits between the marker comment and the template's ";;#ASMEND"
+    size_t a = nk.asm_text.find("; circuit body\n");
+    size_t b = nk.asm_text.find(";;#ASMEND", a);
+    std::string text = nk.asm_text.substr(a, b - a);
+    if (needed) *needed = text.size();
+    if (vgpr_class) *vgpr_class = nk.vgpr_class;
+    if (code_bytes) *code_bytes = code.size();
+    if (body && cap) {
+      need(cap > text.size(), "body buffer too small");
+      std::memcpy(body, text.c_str(), text.size() + 1);
+    }
   });
 }
 

@@ -116,9 +116,10 @@ class LutMapper {
   // uni_input[k] = 1 when input k is the same in every lane of a wave (constant word
   // masks, loop and block bits); gates over uniform values only are uniform too and are
   // computed once per wave with scalar instructions.
-  LutMapper(const Circuit& c, std::ostringstream& os, const std::vector<uint8_t>& uni_input)
-      : c_(c), os_(os), state_(c.wires(), 0), pend_(c.wires()), uni_(c.wires(), 0) {
-    for (uint32_t k = 0; k < c.n; k++) uni_[k] = uni_input[k];
+  LutMapper(const Circuit& c, const std::vector<uint8_t>& uni_input)
+      : c_(c), state_(c.wires(), 0), pend_(c.wires()) {
+    prog_.uniform.assign(c.wires(), 0);
+    for (uint32_t k = 0; k < c.n; k++) prog_.uniform[k] = uni_input[k];
   }
 
   // maps the gates in `order`; outputs in `out_wires` are materialised
@@ -130,22 +131,22 @@ class LutMapper {
       if (g.i1 != g.i0) fan[g.i1]++;
     }
     for (uint32_t w : out_wires) fan[w] += 2;
+    std::vector<uint8_t>& uni = prog_.uniform;
     for (uint32_t w : order) {
       const Gate& g = c_.gates[w - c_.n];
-      if (uni_[g.i0] && uni_[g.i1]) {
+      if (uni[g.i0] && uni[g.i1]) {
         // wave-uniform gate: one scalar operation, always materialised
-        uni_[w] = 1;
-        std::string a = wname(c_, g.i0), b = wname(c_, g.i1);
-        os_ << "    const unsigned int g" << w << " = ";
-        switch (g.func) {
-          case FUNC_AND: os_ << a << " & " << b; break;
-          case FUNC_NOR: os_ << "~(" << a << " | " << b << ")"; break;
-          case FUNC_NIMPL: os_ << a << " & ~" << b; break;
-          default: os_ << a << " ^ " << b; break;
-        }
-        os_ << ";\n";
+        uni[w] = 1;
+        LutOp op;
+        op.lut = 0;
+        op.fn = g.func;
+        op.dst = w;
+        op.src[0] = g.i0;
+        op.src[1] = g.i1;
+        op.src[2] = g.i0;
+        prog_.ops.push_back(op);
         state_[w] = 2;
-        salu_++;
+        prog_.scalar_ops++;
         continue;
       }
       Expr ea = get(g.i0), eb = get(g.i1);
@@ -181,8 +182,7 @@ class LutMapper {
     }
   }
 
-  uint32_t ops() const { return ops_; }
-  uint32_t scalar_ops() const { return salu_; }
+  LutProgram take() { return std::move(prog_); }
 
  private:
   // table over (s0,s1,s2) where s_k for k >= un equals s0
@@ -206,37 +206,69 @@ class LutMapper {
   void materialize(uint32_t w) {
     if (w < c_.n || state_[w] == 2) return;
     const Expr& e = pend_[w];
-    // one wave-uniform leaf may be read from a scalar register; the others must be VGPRs
+    LutOp op;
+    op.lut = 1;
+    op.fn = e.tt;
+    op.dst = w;
+    for (int i = 0; i < 3; i++) op.src[i] = e.leaf[i];
+    prog_.ops.push_back(op);
+    state_[w] = 2;
+    prog_.valu_ops++;
+  }
+
+  const Circuit& c_;
+  std::vector<uint8_t> state_;  // 0 not reached, 1 pending expression, 2 materialised
+  std::vector<Expr> pend_;
+  LutProgram prog_;
+};
+
+}  // namespace
+
+LutProgram map_luts(const Circuit& c, const std::vector<uint32_t>& order, const std::vector<uint32_t>& out_wires,
+                    const std::vector<uint8_t>& uni_input) {
+  LutMapper m(c, uni_input);
+  m.run(order, out_wires);
+  return m.take();
+}
+
+namespace {
+
+// Emits the mapped program as HIP source: scalar gates as C++ expressions, LUTs as one
+// inline-assembly v_bitop3_b32 each (at most one wave-uniform operand, read from an SGPR).
+void emit_program(std::ostringstream& os, const Circuit& c, const LutProgram& p) {
+  for (const LutOp& op : p.ops) {
+    if (!op.lut) {
+      std::string a = wname(c, op.src[0]), b = wname(c, op.src[1]);
+      os << "    const unsigned int g" << op.dst << " = ";
+      switch (op.fn) {
+        case FUNC_AND: os << a << " & " << b; break;
+        case FUNC_NOR: os << "~(" << a << " | " << b << ")"; break;
+        case FUNC_NIMPL: os << a << " & ~" << b; break;
+        default: os << a << " ^ " << b; break;
+      }
+      os << ";\n";
+      continue;
+    }
     bool sgpr_used = false;
     std::string cons[3];
     for (int i = 0; i < 3; i++) {
       bool first_use = true;
-      for (int k = 0; k < i; k++) first_use &= e.leaf[k] != e.leaf[i];
-      if (uni_[e.leaf[i]] && !sgpr_used && first_use) {
+      for (int k = 0; k < i; k++) first_use &= op.src[k] != op.src[i];
+      if (p.uniform[op.src[i]] && !sgpr_used && first_use) {
         cons[i] = "s";
         sgpr_used = true;
         for (int k = i + 1; k < 3; k++)
-          if (e.leaf[k] == e.leaf[i]) cons[k] = "s";
+          if (op.src[k] == op.src[i]) cons[k] = "s";
       } else if (cons[i].empty()) {
         cons[i] = "v";
       }
     }
-    os_ << "    unsigned int g" << w << ";\n    asm(\"v_bitop3_b32 %0, %1, %2, %3 bitop3:0x" << std::hex
-        << (unsigned)e.tt << std::dec << "\" : \"=v\"(g" << w << ") : \"" << cons[0] << "\"(" << wname(c_, e.leaf[0])
-        << "), \"" << cons[1] << "\"(" << wname(c_, e.leaf[1]) << "), \"" << cons[2] << "\"("
-        << wname(c_, e.leaf[2]) << "));\n";
-    state_[w] = 2;
-    ops_++;
+    os << "    unsigned int g" << op.dst << ";\n    asm(\"v_bitop3_b32 %0, %1, %2, %3 bitop3:0x" << std::hex
+       << (unsigned)op.fn << std::dec << "\" : \"=v\"(g" << op.dst << ") : \"" << cons[0] << "\"("
+       << wname(c, op.src[0]) << "), \"" << cons[1] << "\"(" << wname(c, op.src[1]) << "), \"" << cons[2]
+       << "\"(" << wname(c, op.src[2]) << "));\n";
   }
-
-  const Circuit& c_;
-  std::ostringstream& os_;
-  std::vector<uint8_t> state_;  // 0 not reached, 1 pending expression, 2 materialised
-  std::vector<Expr> pend_;
-  std::vector<uint8_t> uni_;    // wave-uniform value
-  uint32_t ops_ = 0;
-  uint32_t salu_ = 0;
-};
+}
 
 // Emits the gates of the given order as mapped LUTs; returns the number of VALU ops per
 // word and stores the number of scalar (per-wave) ops in *scalar_ops.
@@ -245,10 +277,10 @@ uint32_t emit_gates(std::ostringstream& os, const Circuit& c, const std::vector<
                     uint32_t* scalar_ops) {
   std::vector<uint32_t> out_wires;
   for (uint32_t k : outs) out_wires.push_back(c.outputs[k].wire);
-  LutMapper m(c, os, uni_input);
-  m.run(order, out_wires);
-  if (scalar_ops) *scalar_ops = m.scalar_ops();
-  return m.ops();
+  LutProgram p = map_luts(c, order, out_wires, uni_input);
+  emit_program(os, c, p);
+  if (scalar_ops) *scalar_ops = p.scalar_ops;
+  return p.valu_ops;
 }
 
 // wave-uniform inputs for an index mapping: everything but the six lane bits

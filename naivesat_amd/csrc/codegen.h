@@ -28,6 +28,29 @@ struct GenInfo {
   uint32_t scalar_ops = 0;  // wave-uniform gates (scalar instructions, once per wave and word)
 };
 
+// One operation of a mapped circuit. lut = 1: dst = v_bitop3(src0, src1, src2) with
+// truth table fn (bit (s0<<2 | s1<<1 | s2)); lut = 0: wave-uniform two-input gate
+// dst = func fn (FUNC_*) of (src0, src1), evaluated with scalar instructions.
+struct LutOp {
+  uint8_t lut = 0;
+  uint8_t fn = 0;
+  uint32_t dst = 0;
+  uint32_t src[3] = {0, 0, 0};
+};
+
+struct LutProgram {
+  std::vector<LutOp> ops;        // evaluation order
+  std::vector<uint8_t> uniform;  // per wire: value is the same in all lanes of a wave
+  uint32_t valu_ops = 0;
+  uint32_t scalar_ops = 0;
+};
+
+// Maps the gates of `order` onto 3-input LUTs (single-use gates folded into their consumer
+// while the function keeps at most three leaves). uni_input[k] marks wave-uniform inputs;
+// gates over uniform values only become scalar ops. out_wires are always materialised.
+LutProgram map_luts(const Circuit& c, const std::vector<uint32_t>& order, const std::vector<uint32_t>& out_wires,
+                    const std::vector<uint8_t>& uni_input);
+
 // SAT: count of satisfying assignments and lowest one.
 // kernel ns_sat(u64 w0, u64 wend, unsigned long long* res): res[0] += count,
 // res[1] = min(res[1], lowest).

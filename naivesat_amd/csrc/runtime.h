@@ -41,13 +41,19 @@ struct JitModule {
 // Compiles generated sources once per context; keyed by the full source text.
 class Jit {
  public:
+  // HIP source compiled with hiprtc
   std::shared_ptr<JitModule> get(const std::string& src, const char* kernel, double* compile_ms);
+  // gfx950 assembly module assembled and linked in-process (amd_comgr)
+  std::shared_ptr<JitModule> get_asm(const std::string& asm_text, const std::string& kernel, double* compile_ms);
   ~Jit();
 
  private:
   std::mutex mu_;
   std::unordered_map<std::string, std::shared_ptr<JitModule>> cache_;
 };
+
+// Assembles and links a gfx950 assembly module in-process; returns the code object.
+std::vector<char> assemble_native(const std::string& asm_text);
 
 // Growable device allocation (contents are not preserved when it grows).
 struct DevBuf {

@@ -4,8 +4,10 @@
 #include "solvers.h"
 
 #include <chrono>
+#include <cstdlib>
 #include <cstring>
 
+#include "asmgen.h"
 #include "codegen.h"
 #include "static_kernels.h"
 
@@ -67,12 +69,31 @@ void launch_range(Context& ctx, hipFunction_t fn, const Geometry& g, uint64_t w0
                   void** extra_args, int n_extra, uint64_t wend) {
   uint64_t blocks = (words + g.words_per_block() - 1) / g.words_per_block();
   if (blocks > 0x7fffffffull) throw Error(ST_UNSUPPORTED, "launch range too large");
-  void* args[8];
+  if (n_extra > 10) throw Error(ST_INTERNAL, "too many kernel arguments");
+  void* args[12];
   args[0] = &w0;
   args[1] = &wend;
   for (int i = 0; i < n_extra; i++) args[2 + i] = extra_args[i];
   NS_HIP(hipModuleLaunchKernel(fn, (unsigned)blocks, 1, 1, g.block, 1, 1, 0, ctx.stream, args, nullptr));
 }
+
+// The native path (assembly bodies in precompiled templates) is used unless the
+// environment selects the compiler path (NS_BACKEND=hiprtc) or the launch needs more than
+// the templates' 256 threads per block.
+bool native_wanted(const Geometry& g) {
+  const char* b = std::getenv("NS_BACKEND");
+  if (b && std::strcmp(b, "hiprtc") == 0) return false;
+  return g.block <= 256;
+}
+
+uint32_t word_valid_mask(uint32_t n) { return n >= 5 ? 0xffffffffu : (uint32_t)((1ull << (1u << n)) - 1); }
+
+struct Chosen {
+  std::shared_ptr<JitModule> mod;
+  GenInfo info;
+  bool native = false;
+  uint32_t vgpr_class = 0;
+};
 
 }  // namespace
 
@@ -87,10 +108,30 @@ void sat_solve(Context& ctx, const Circuit& c, const ns_sat_opts& o, ns_sat_resu
   NS_HIP(hipSetDevice(ctx.device));
   const uint64_t total_words = c.n >= 5 ? (c.n == 64 ? (1ull << 59) : (1ull << (c.n - 5))) : 1ull;
   Geometry g = pick_geometry(total_words, o.group_len, o.words_per_lane, false);
-  GenInfo info;
-  std::string src = gen_sat_kernel(c, g, &info);
-  auto mod = ctx.jit.get(src, "ns_sat", &r.compile_ms);
-  r.cone_gates = info.cone_gates;
+  Chosen ch;
+  if (native_wanted(g)) {
+    try {
+      std::vector<uint32_t> vb(c.n);
+      for (uint32_t k = 0; k < c.n; k++) vb[k] = k;
+      NativeKernel nk = build_native(c, NativeKind::SAT, vb, 0, {0});
+      ch.mod = ctx.jit.get_asm(nk.asm_text, nk.kernel_name, &r.compile_ms);
+      ch.info = nk.info;
+      ch.native = true;
+      ch.vgpr_class = nk.vgpr_class;
+    } catch (const NativeUnfit&) {
+    }
+  }
+  if (!ch.native) {
+    std::string src = gen_sat_kernel(c, g, &ch.info);
+    ch.mod = ctx.jit.get(src, "ns_sat", &r.compile_ms);
+  }
+  auto mod = ch.mod;
+  r.cone_gates = ch.info.cone_gates;
+  r.valu_ops = ch.info.valu_ops;
+  r.scalar_ops = ch.info.scalar_ops;
+  r.backend = ch.native ? 1 : 0;
+  r.vgpr_class = ch.vgpr_class;
+  uint32_t wpl = g.wpl, valid = word_valid_mask(c.n);
   unsigned long long* res = ctx.stats.as<unsigned long long>();
   NS_HIP(launch_fill_u64(ctx.stream, res, 0ull, 1));
   NS_HIP(launch_fill_u64(ctx.stream, res + 1, ~0ull, 1));
@@ -103,8 +144,8 @@ void sat_solve(Context& ctx, const Circuit& c, const ns_sat_opts& o, ns_sat_resu
   NS_HIP(hipEventRecord(ctx.ev0, ctx.stream));
   while (w0 < total_words) {
     uint64_t words = total_words - w0 < cw ? total_words - w0 : cw;
-    void* extra[1] = {&res};
-    launch_range(ctx, mod->fn, g, w0, words, extra, 1, total_words);
+    void* extra[3] = {&res, &wpl, &valid};
+    launch_range(ctx, mod->fn, g, w0, words, extra, ch.native ? 3 : 1, total_words);
     r.launches++;
     w0 += cw;
     if (stop_first) {
@@ -154,10 +195,33 @@ void qsat_solve(Context& ctx, const Circuit& c, const std::vector<uint8_t>& q, c
   Geometry g = pick_geometry(total_words, o.group_len, o.words_per_lane, true);
   const uint32_t jb = ilog2(g.wpl);
   const bool case_a = L >= 5 + jb;
-  GenInfo info;
-  std::string src = gen_qsat_kernel(c, q, g, &info);
-  auto mod = ctx.jit.get(src, "ns_qsat", &r.compile_ms);
-  r.cone_gates = info.cone_gates;
+  Chosen ch;
+  if (native_wanted(g)) {
+    try {
+      std::vector<uint32_t> vb(n);
+      for (uint32_t k = 0; k < n; k++) vb[k] = n - 1 - k;
+      NativeKernel nk = build_native(c, case_a ? NativeKind::QSAT_A : NativeKind::QSAT_B, vb, jb, {0});
+      ch.mod = ctx.jit.get_asm(nk.asm_text, nk.kernel_name, &r.compile_ms);
+      ch.info = nk.info;
+      ch.native = true;
+      ch.vgpr_class = nk.vgpr_class;
+    } catch (const NativeUnfit&) {
+    }
+  }
+  if (!ch.native) {
+    std::string src = gen_qsat_kernel(c, q, g, &ch.info);
+    ch.mod = ctx.jit.get(src, "ns_qsat", &r.compile_ms);
+  }
+  auto mod = ch.mod;
+  r.cone_gates = ch.info.cone_gates;
+  r.valu_ops = ch.info.valu_ops;
+  r.scalar_ops = ch.info.scalar_ops;
+  r.backend = ch.native ? 1 : 0;
+  r.vgpr_class = ch.vgpr_class;
+  // runtime parameters of the native templates: level ops (bit l = AND for level l)
+  uint32_t wpl = g.wpl, jbv = jb, opm = 0, Lv = L, q0v = q0all ? 1u : 0u, valid = word_valid_mask(n);
+  for (uint32_t l = 0; l < 32 && l < n; l++)
+    if (q[n - 1 - l]) opm |= 1u << l;
   unsigned long long* stats = ctx.stats.as<unsigned long long>();
   NS_HIP(launch_fill_u64(ctx.stream, stats, 0ull, 1));
   NS_HIP(launch_fill_u64(ctx.stream, stats + 1, ~0ull, 1));
@@ -172,8 +236,8 @@ void qsat_solve(Context& ctx, const Circuit& c, const std::vector<uint8_t>& q, c
     unsigned long long* X = ctx.buf_a.as<unsigned long long>();
     for (uint64_t w0 = 0; w0 < total_words; w0 += cw) {
       uint64_t words = total_words - w0 < cw ? total_words - w0 : cw;
-      void* extra[1] = {&X};
-      launch_range(ctx, mod->fn, g, w0, words, extra, 1, total_words);
+      void* extra[4] = {&X, &wpl, &jbv, &opm};
+      launch_range(ctx, mod->fn, g, w0, words, extra, ch.native ? 4 : 1, total_words);
     }
     // phase 2: reduction passes over the bit array
     uint32_t m = m0, remaining = L - 5 - jb, level = 5 + jb;
@@ -212,8 +276,8 @@ void qsat_solve(Context& ctx, const Circuit& c, const std::vector<uint8_t>& q, c
     }
     for (uint64_t w0 = 0; w0 < total_words; w0 += cw) {
       uint64_t words = total_words - w0 < cw ? total_words - w0 : cw;
-      void* extra[2] = {&stats, &dev_bitmap};
-      launch_range(ctx, mod->fn, g, w0, words, extra, 2, total_words);
+      void* extra[8] = {&stats, &dev_bitmap, &wpl, &jbv, &opm, &Lv, &q0v, &valid};
+      launch_range(ctx, mod->fn, g, w0, words, extra, ch.native ? 8 : 2, total_words);
     }
   }
   NS_HIP(hipEventRecord(ctx.ev1, ctx.stream));
@@ -238,19 +302,48 @@ void qsat_solve(Context& ctx, const Circuit& c, const std::vector<uint8_t>& q, c
 // ---------------------------------------------------------------------------
 namespace {
 
+struct TableKernelInfo {
+  GenInfo info;
+  bool native = false;
+  uint32_t vgpr_class = 0;
+};
+
 // Builds the u32 table into ctx.buf_a; returns kernel ms; sets *anylast.
 double build_table_dev(Context& ctx, const Circuit& c, uint32_t group_len, uint32_t wpl,
-                       bool* anylast, double* compile_ms, uint32_t* cone) {
+                       bool* anylast, double* compile_ms, uint32_t* cone, TableKernelInfo* info_out = nullptr) {
   const uint32_t n = c.n;
   if (n == 0 || n > 31) throw Error(ST_UNSUPPORTED, "next-state tables support 1..31 state bits");
   if (c.outputs.empty() || c.outputs.size() > 32) throw Error(ST_UNSUPPORTED, "table entries hold at most 32 outputs");
   const uint64_t states = 1ull << n;
   const uint64_t total_words = n >= 5 ? (1ull << (n - 5)) : 1ull;
   Geometry g = pick_geometry(total_words, group_len, wpl ? wpl : 2, false);
-  GenInfo info;
-  std::string src = gen_table_kernel(c, g, &info);
-  auto mod = ctx.jit.get(src, "ns_table", compile_ms);
-  if (cone) *cone = info.cone_gates;
+  Chosen ch;
+  if (native_wanted(g)) {
+    try {
+      std::vector<uint32_t> vb(n), outs;
+      for (uint32_t k = 0; k < n; k++) vb[k] = k;
+      for (uint32_t k = 0; k < c.outputs.size(); k++) outs.push_back(k);
+      NativeKernel nk = build_native(c, NativeKind::TABLE, vb, 0, outs);
+      ch.mod = ctx.jit.get_asm(nk.asm_text, nk.kernel_name, compile_ms);
+      ch.info = nk.info;
+      ch.native = true;
+      ch.vgpr_class = nk.vgpr_class;
+    } catch (const NativeUnfit&) {
+    }
+  }
+  if (!ch.native) {
+    std::string src = gen_table_kernel(c, g, &ch.info);
+    ch.mod = ctx.jit.get(src, "ns_table", compile_ms);
+  }
+  auto mod = ch.mod;
+  if (cone) *cone = ch.info.cone_gates;
+  if (info_out) {
+    info_out->info = ch.info;
+    info_out->native = ch.native;
+    info_out->vgpr_class = ch.vgpr_class;
+  }
+  uint32_t wpl_v = g.wpl, last = (uint32_t)c.outputs.size() - 1, valid = word_valid_mask(n),
+           small = n < 5 ? (1u << n) : 0u;
   ctx.buf_a.ensure((states < 32 ? 32 : states) * 4);
   unsigned int* T = ctx.buf_a.as<unsigned int>();
   unsigned int* flag = reinterpret_cast<unsigned int*>(ctx.stats.as<char>() + 64);
@@ -259,8 +352,8 @@ double build_table_dev(Context& ctx, const Circuit& c, uint32_t group_len, uint3
   NS_HIP(hipEventRecord(ctx.ev0, ctx.stream));
   for (uint64_t w0 = 0; w0 < total_words; w0 += cw) {
     uint64_t words = total_words - w0 < cw ? total_words - w0 : cw;
-    void* extra[2] = {&T, &flag};
-    launch_range(ctx, mod->fn, g, w0, words, extra, 2, total_words);
+    void* extra[6] = {&T, &flag, &wpl_v, &last, &valid, &small};
+    launch_range(ctx, mod->fn, g, w0, words, extra, ch.native ? 6 : 2, total_words);
   }
   NS_HIP(hipEventRecord(ctx.ev1, ctx.stream));
   unsigned int hf = 0;
@@ -296,7 +389,11 @@ void satostop_solve(Context& ctx, const Circuit& c, uint32_t unknowns, const ns_
   NS_HIP(hipSetDevice(ctx.device));
   ctx.pass_ms.clear();
   bool any = false;
-  r.build_ms = build_table_dev(ctx, c, o.group_len, o.words_per_lane, &any, &r.compile_ms, &r.cone_gates);
+  TableKernelInfo tki;
+  r.build_ms = build_table_dev(ctx, c, o.group_len, o.words_per_lane, &any, &r.compile_ms, &r.cone_gates, &tki);
+  r.valu_ops = tki.info.valu_ops;
+  r.backend = tki.native ? 1 : 0;
+  r.vgpr_class = tki.vgpr_class;
   r.any_stop = any;
   const uint64_t states = 1ull << n;
   const uint64_t starts = 1ull << unknowns;

@@ -1,0 +1,20 @@
+"""Embeds the compiled template assembly files into a C++ source (templates_data.cpp).
+
+Usage: python embed.py OUT.cpp tpl_<kernel>.s ...
+"""
+import os
+import sys
+
+out = ["// generated by embed.py: assembly of the native-path kernel templates", "#include <cstring>",
+       "#include <string>", "namespace ns {", "namespace {", "struct Tpl { const char* name; const char* text; };",
+       "const Tpl kTemplates[] = {"]
+for path in sys.argv[2:]:
+    name = os.path.basename(path)[len("tpl_"):-len(".s")]
+    text = open(path).read()
+    if ")NSTPL" in text or text.count("; NS_BODY " + name) != 1:
+        sys.exit(f"{path}: placeholder missing or text unusable")
+    out.append(f'  {{"{name}", R"NSTPL({text})NSTPL"}},')
+out += ["};", "}  // namespace", "const char* native_template(const std::string& name) {",
+        "  for (const Tpl& t : kTemplates)", "    if (name == t.name) return t.text;", "  return nullptr;", "}",
+        "}  // namespace ns", ""]
+open(sys.argv[1], "w").write("\n".join(out))

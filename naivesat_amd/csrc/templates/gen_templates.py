@@ -1,0 +1,241 @@
+"""Writes templates.hip: the circuit-independent kernels of the native code path.
+
+Each kernel evaluates a circuit through an inline-assembly placeholder whose register
+contract is fixed; at run time the engine writes the circuit's register-allocated
+instructions into the placeholder of the compiled assembly and assembles it.
+
+Register contract of every placeholder ("; NS_BODY <name>"):
+  inputs   v[LM..LM+5]   lane masks: v[LM+i] = 0 - ((lane >> i) & 1)
+           s20, s21      wave-uniform part of the word index (lo, hi): every index bit except
+                         the six lane bits
+  outputs  v0 .. v[OUT-1] circuit outputs for the current word
+  scratch  v[POOL_LO..POOL_HI], s[40..95], scc (clobbered)
+Register classes bound the VGPR budget (and with it the occupancy).
+
+Usage: python gen_templates.py OUT.hip
+"""
+import sys
+
+CLASSES = {  # name: (total VGPR budget, pool high register)
+    "r64": (64, 55),
+    "r128": (128, 119),
+    "r256": (256, 239),
+}
+SGPR_POOL = (40, 95)
+
+
+def clobbers(lo, hi):
+    regs = [f'"v{i}"' for i in range(lo, hi + 1)]
+    regs += [f'"s{i}"' for i in range(SGPR_POOL[0], SGPR_POOL[1] + 1)]
+    regs.append('"scc"')
+    return ", ".join(regs)
+
+
+HEAD = r'''// generated by gen_templates.py - circuit-independent kernels of the native code path
+#include <hip/hip_runtime.h>
+typedef unsigned int u32;
+typedef unsigned long long u64;
+
+// lane masks and wave-uniform word base, as the circuit bodies expect them
+#define NS_PROLOGUE                                                              \
+  const u32 lane = threadIdx.x & 63u;                                             \
+  const u32 wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);              \
+  const u64 U = w0 + ((u64)blockIdx.x * (blockDim.x >> 6) + wave) * (64ull * wpl); \
+  const u32 LM0 = 0u - (lane & 1u), LM1 = 0u - ((lane >> 1) & 1u);                \
+  const u32 LM2 = 0u - ((lane >> 2) & 1u), LM3 = 0u - ((lane >> 3) & 1u);         \
+  const u32 LM4 = 0u - ((lane >> 4) & 1u), LM5 = 0u - ((lane >> 5) & 1u);
+
+__device__ inline void ns_wave_stats(u64 cnt, u64 best, u64* stats) {
+  if (__ballot(cnt != 0ull) == 0ull) return;
+  for (int off = 32; off > 0; off >>= 1) {
+    u64 ob = __shfl_xor(best, off);
+    best = ob < best ? ob : best;
+    cnt += __shfl_xor(cnt, off);
+  }
+  if ((threadIdx.x & 63u) == 0u) {
+    atomicAdd(&stats[0], cnt);
+    atomicMin(&stats[1], best);
+  }
+}
+
+__device__ inline u32 ns_red(u32 x, u32 sh, u32 all) { return all ? (x & (x >> sh)) : (x | (x >> sh)); }
+'''
+
+
+def body1(name, lm, lo, hi):
+    """placeholder with one output in v0"""
+    ins = ", ".join([f'"{{v{lm + i}}}"(LM{i})' for i in range(6)] + ['"{s20}"(wlo)', '"{s21}"(whi)'])
+    return (f'    u32 o;\n'
+            f'    asm volatile("; NS_BODY {name}" : "={{v0}}"(o) : {ins} : {clobbers(lo, hi)});\n')
+
+
+def sat_kernel(cls, vb, pool_hi):
+    name = f"ns_nat_sat_{cls}"
+    return f'''
+// SAT: res[0] += satisfying assignments, res[1] = min(lowest); w = U + lane + (j << 6)
+extern "C" __global__ void __launch_bounds__(256) {name}(u64 w0, u64 wend, u64* res, u32 wpl, u32 valid) {{
+  NS_PROLOGUE
+  u32 cnt = 0u;
+  u64 best = ~0ull;
+#pragma unroll 1
+  for (u32 j = 0; j < wpl; j++) {{
+    const u64 wu = U + ((u64)j << 6);
+    const u64 w = wu + lane;
+    const u32 wlo = (u32)wu, whi = (u32)(wu >> 32);
+{body1(name, 1, 8, pool_hi)}    o &= valid;
+    if (w >= wend) o = 0u;
+    cnt += __popc(o);
+    if (o != 0u && best == ~0ull) best = (w << 5) | (u64)__builtin_ctz(o);
+  }}
+  ns_wave_stats(cnt, best, res);
+}}
+'''
+
+
+def qsat_a_kernel(cls, pool_hi):
+    name = f"ns_nat_qsata_{cls}"
+    return f'''
+// QSAT, one reduced bit per thread (all in-thread levels below the outermost block):
+// w = U + (lane << jb) + j; opm bit l = AND for level l (level 0 = index bit 0).
+extern "C" __global__ void __launch_bounds__(256) {name}(u64 w0, u64 wend, u64* bits, u32 wpl, u32 jb, u32 opm) {{
+  NS_PROLOGUE
+  const u64 wl = U + ((u64)lane << jb);
+  u32 R = 0u;
+#pragma unroll 1
+  for (u32 j = 0; j < wpl; j++) {{
+    const u64 wu = U + j;
+    const u32 wlo = (u32)wu, whi = (u32)(wu >> 32);
+{body1(name, 1, 8, pool_hi)}#pragma unroll
+    for (u32 l = 0; l < 5; l++) o = ns_red(o, 1u << l, (opm >> l) & 1u);
+    R |= (o & 1u) << j;
+  }}
+  for (u32 l = 0; l < jb; l++) R = ns_red(R, 1u << l, (opm >> (5 + l)) & 1u);
+  const u32 nvalid = wend > wl ? 1u : 0u;
+  const u64 ball = __ballot((R & 1u) != 0u && nvalid != 0u);
+  if (lane == 0u) bits[U >> (jb + 6)] = ball;
+}}
+'''
+
+
+def qsat_b_kernel(cls, pool_hi):
+    name = f"ns_nat_qsatb_{cls}"
+    return f'''
+// QSAT, outermost block reached inside the thread (L < 5 + jb levels below it):
+// stats[0] += witnesses, stats[1] = min(lowest witness u), bitmap bit u = block value.
+extern "C" __global__ void __launch_bounds__(256) {name}(u64 w0, u64 wend, u64* stats, u32* bitmap, u32 wpl,
+                                                      u32 jb, u32 opm, u32 L, u32 q0all, u32 valid) {{
+  NS_PROLOGUE
+  const u64 wl = U + ((u64)lane << jb);
+  u32 cnt = 0u;
+  u64 best = ~0ull;
+  u32 R = 0u;
+  const u32 inword = L < 5u ? L : 5u;
+#pragma unroll 1
+  for (u32 j = 0; j < wpl; j++) {{
+    const u64 wu = U + j;
+    const u64 w = wl + j;
+    const u32 wlo = (u32)wu, whi = (u32)(wu >> 32);
+{body1(name, 1, 8, pool_hi)}    for (u32 l = 0; l < inword; l++) o = ns_red(o, 1u << l, (opm >> l) & 1u);
+    if (L <= 5u) {{
+      u32 pos = 0u;
+      for (u32 p = 0; p < (32u >> L); p++) pos |= 1u << (p << L);
+      pos &= valid;
+      u32 wit = (q0all ? ~o : o) & pos;
+      if (w >= wend) wit = 0u;
+      cnt += __popc(wit);
+      if (wit != 0u && best == ~0ull) best = (w << (5u - L)) + (u64)(__builtin_ctz(wit) >> L);
+      if (bitmap != 0 && w < wend) {{
+        u32 r = 0u;
+        for (u32 p = 0; p < (32u >> L); p++) r |= ((o >> (p << L)) & 1u) << p;
+        r &= (L == 0u) ? valid : 0xffffffffu;
+        const u64 ub = w << (5u - L);
+        atomicOr(&bitmap[ub >> 5], r << (u32)(ub & 31ull));
+      }}
+    }} else {{
+      R |= (o & 1u) << j;
+    }}
+  }}
+  if (L > 5u) {{
+    const u32 s = L - 5u;
+    for (u32 l = 0; l < s; l++) R = ns_red(R, 1u << l, (opm >> (5u + l)) & 1u);
+    const u32 nvalid = wend > wl ? (wend - wl >= wpl ? wpl : (u32)(wend - wl)) : 0u;
+    u32 pos = 0u;
+    for (u32 p = 0; p < (wpl >> s); p++) pos |= 1u << (p << s);
+    const u32 jmask = nvalid >= 32u ? 0xffffffffu : ((1u << nvalid) - 1u);
+    const u32 wit = (q0all ? ~R : R) & pos & jmask;
+    cnt = __popc(wit);
+    best = wit != 0u ? (wl >> s) + (u64)(__builtin_ctz(wit) >> s) : ~0ull;
+    if (bitmap != 0 && nvalid != 0u) {{
+      u32 r = 0u;
+      for (u32 p = 0; p < (wpl >> s); p++) r |= ((R >> (p << s)) & 1u) << p;
+      r &= (nvalid >= wpl) ? 0xffffffffu : ((1u << ((nvalid + (1u << s) - 1u) >> s)) - 1u);
+      const u64 ub = wl >> s;
+      atomicOr(&bitmap[ub >> 5], r << (u32)(ub & 31ull));
+    }}
+  }}
+  ns_wave_stats(cnt, best, stats);
+}}
+'''
+
+
+def table_kernel(cls, pool_hi):
+    name = f"ns_nat_table_{cls}"
+    outs = ", ".join(f'"={{v{k}}}"(A[{k}])' for k in range(32))
+    ins = ", ".join([f'"{{v{32 + i}}}"(LM{i})' for i in range(6)] + ['"{s20}"(wlo)', '"{s21}"(whi)'])
+    return f'''
+// Next-state table: table[32w + b] = outputs of state 32w + b packed (output i at bit i);
+// *anylast |= any state whose last output (STOP) is 1. w = U + lane + (j << 6).
+extern "C" __global__ void __launch_bounds__(256) {name}(u64 w0, u64 wend, u32* table, u32* anylast, u32 wpl,
+                                                      u32 last, u32 valid, u32 small) {{
+  NS_PROLOGUE
+  u32 last_any = 0u;
+#pragma unroll 1
+  for (u32 j = 0; j < wpl; j++) {{
+    const u64 wu = U + ((u64)j << 6);
+    const u64 w = wu + lane;
+    const u32 wlo = (u32)wu, whi = (u32)(wu >> 32);
+    u32 A[32];
+    asm volatile("; NS_BODY {name}" : {outs} : {ins} : {clobbers(40, pool_hi)});
+    if (w < wend) last_any |= A[last] & valid;
+    {{
+      u32 msk = 0x0000FFFFu;
+#pragma unroll
+      for (int s = 16; s != 0; s >>= 1, msk ^= (msk << s)) {{
+#pragma unroll
+        for (int k = 0; k < 32; k = ((k | s) + 1) & ~s) {{
+          const u32 t = ((A[k] >> s) ^ A[k | s]) & msk;
+          A[k] ^= t << s;
+          A[k | s] ^= t;
+        }}
+      }}
+    }}
+    if (w < wend) {{
+      if (small == 0u) {{
+        uint4* dst = (uint4*)(table + (w << 5));
+#pragma unroll
+        for (int q = 0; q < 8; q++) dst[q] = make_uint4(A[4 * q], A[4 * q + 1], A[4 * q + 2], A[4 * q + 3]);
+      }} else {{
+        for (u32 b = 0; b < small; b++) table[b] = A[b];
+      }}
+    }}
+  }}
+  if (last_any != 0u) atomicOr(anylast, 1u);
+}}
+'''
+
+
+def main():
+    out = [HEAD]
+    for cls, (budget, hi) in CLASSES.items():
+        kernels = [("sat", sat_kernel(cls, budget, hi)), ("qsata", qsat_a_kernel(cls, hi)),
+                   ("qsatb", qsat_b_kernel(cls, hi))]
+        if budget >= 128:
+            kernels.append(("table", table_kernel(cls, hi)))
+        for kind, src in kernels:
+            # one kernel per compilation: NS_T_<name> selects it
+            out.append(f"#if defined(NS_T_ns_nat_{kind}_{cls})\n{src}#endif\n")
+    open(sys.argv[1], "w").write("".join(out))
+
+
+if __name__ == "__main__":
+    main()

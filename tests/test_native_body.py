@@ -1,0 +1,154 @@
+"""Native code path, checked on the host: the generated circuit bodies (register-allocated
+gfx950 assembly) are executed instruction by instruction by a small interpreter of the
+instruction subset they use, and their outputs compared with the oracle's evaluation."""
+import numpy as np
+import pytest
+
+import naivesat_amd as ns
+import oracle
+from helpers import random_circuit
+
+M32 = 0xFFFFFFFF
+LANES = np.arange(64, dtype=np.uint32)
+
+
+def run_body(body, lm_base, wu):
+    v, s = {}, {20: wu & M32, 21: (wu >> 32) & M32}
+    for i in range(6):
+        v[lm_base + i] = np.where((LANES >> i) & 1, np.uint32(M32), np.uint32(0)).astype(np.uint32)
+
+    def val(tok):
+        if tok.startswith("v"):
+            return v[int(tok[1:])]
+        if tok.startswith("s"):
+            return np.full(64, s[int(tok[1:])], np.uint32)
+        return np.full(64, int(tok, 0) & M32, np.uint32)
+
+    def sval(tok):
+        return s[int(tok[1:])] if tok.startswith("s") else int(tok, 0) & M32
+
+    for line in body.splitlines():
+        line = line.strip()
+        if not line or line.startswith(";"):
+            continue
+        op, rest = line.split(None, 1)
+        args = [a.strip() for a in rest.split(",")]
+        if op == "s_mov_b32":
+            s[int(args[0][1:])] = sval(args[1])
+        elif op == "s_bfe_i32":
+            src, ctl = sval(args[1]), sval(args[2])
+            off, width = ctl & 31, (ctl >> 16) & 0x7F
+            assert width == 1
+            s[int(args[0][1:])] = M32 if (src >> off) & 1 else 0
+        elif op in ("s_and_b32", "s_nor_b32", "s_andn2_b32", "s_xor_b32"):
+            a, b = sval(args[1]), sval(args[2])
+            r = {"s_and_b32": a & b, "s_nor_b32": ~(a | b), "s_andn2_b32": a & ~b, "s_xor_b32": a ^ b}[op]
+            s[int(args[0][1:])] = r & M32
+        elif op == "v_bitop3_b32":
+            src2, tt = args[3].split()
+            tt = int(tt.split(":")[1], 0)
+            A, B, Cc = val(args[1]), val(args[2]), val(src2)
+            r = np.zeros(64, np.uint32)
+            for idx in range(8):
+                if (tt >> idx) & 1:
+                    r |= (A if idx & 4 else ~A) & (B if idx & 2 else ~B) & (Cc if idx & 1 else ~Cc)
+            v[int(args[0][1:])] = r
+        elif op == "v_mov_b32":
+            v[int(args[0][1:])] = val(args[1]).copy()
+        elif op == "v_not_b32":
+            v[int(args[0][1:])] = ~val(args[1])
+        else:
+            raise AssertionError("unexpected instruction " + op)
+    return v
+
+
+def expected_words(oc, n, outs, vbit, lane_shift, wu):
+    """outputs as 32-bit words per lane: bit b of lane l = output for index 32*w + b,
+    w = wu + (l << lane_shift)"""
+    res = np.zeros((len(outs), 64), np.uint32)
+    for lane in range(64):
+        w = wu + (lane << lane_shift)
+        for b in range(32):
+            idx = 32 * w + b
+            if n < 64 and idx >> n:
+                continue
+            x = 0
+            for k in range(n):
+                if (idx >> vbit[k]) & 1:
+                    x |= 1 << k
+            o = oc.eval(x)
+            for i, out in enumerate(outs):
+                if (o >> out) & 1:
+                    res[i, lane] |= np.uint32(1 << b)
+    return res
+
+
+def check(c, kind, lane_shift, wus):
+    body, cls, size = c.native_body(kind, lane_shift)
+    assert size > 0 and cls in (64, 128, 256)
+    d = c.arrays()
+    n = d["n"]
+    oc = oracle.OracleCircuit.from_dict(d)
+    outs = list(range(len(d["out_wire"]))) if kind == 3 else [0]
+    vbit = [n - 1 - k for k in range(n)] if kind in (1, 2) else list(range(n))
+    lm = 32 if kind == 3 else 1
+    for wu in wus:
+        v = run_body(body, lm, wu)
+        exp = expected_words(oc, n, outs, vbit, lane_shift, wu)
+        for i in range(len(outs)):
+            got = v[i]
+            if n < 5:
+                got = got & np.uint32((1 << (1 << n)) - 1)
+            if n < 11:  # lanes beyond the space carry no assignments
+                nl = max(1, 1 << max(0, n - 5)) if lane_shift == 0 else 64
+                got, e = got[:nl], exp[i][:nl]
+            else:
+                e = exp[i]
+            assert np.array_equal(got, e), (kind, lane_shift, wu, i)
+
+
+@pytest.mark.parametrize("seed", range(5))
+def test_sat_bodies(seed):
+    rng = np.random.default_rng(900 + seed)
+    n = int(rng.integers(12, 22))
+    d = random_circuit(rng, n, int(rng.integers(30, 200)), recent=int(rng.integers(6, 40)))
+    c = ns.Circuit.from_arrays(d["n"], d["in0"], d["in1"], d["func"], d["out_wire"], d["out_neg"])
+    wus = [0, 64 * 3, (1 << (n - 5)) - 64] if n > 11 else [0]
+    check(c, 0, 0, wus)
+
+
+def test_config0_body():
+    c, _ = ns.Circuit.gen_sat(1, 24, 2000, 32)
+    check(c, 0, 0, [0, 64 * 1000 + 128])
+
+
+@pytest.mark.parametrize("lane_shift", [0, 2, 3])
+def test_qsat_bodies(lane_shift):
+    c = ns.Circuit.gen_qsat(17, 20, 300, 60)
+    n = 20
+    base = [0, 1 << (lane_shift + 6), (1 << (n - 5)) - (1 << (lane_shift + 6))]
+    check(c, 1, lane_shift, [b | j for b in base for j in range(min(2, 1 << lane_shift))])
+
+
+def test_table_bodies():
+    c, _ = ns.Circuit.gen_satostop(33, 8, 6, 60, 8)
+    check(c, 3, 0, [0, 64 * 5, (1 << 11) - 64])
+    rng = np.random.default_rng(4)
+    d = random_circuit(rng, 14, 120, nout=15, recent=30)
+    c2 = ns.Circuit.from_arrays(d["n"], d["in0"], d["in1"], d["func"], d["out_wire"], d["out_neg"])
+    check(c2, 3, 0, [0, 64 * 2])
+
+
+def test_small_spaces():
+    rng = np.random.default_rng(12)
+    for n in (1, 3, 6, 9):
+        d = random_circuit(rng, n, 20)
+        c = ns.Circuit.from_arrays(d["n"], d["in0"], d["in1"], d["func"], d["out_wire"], d["out_neg"])
+        check(c, 0, 0, [0])
+
+
+def test_register_budget_is_reported():
+    c, _ = ns.Circuit.gen_sat(2, 36, 20000, 40)
+    with pytest.raises(ns.NaivesatError) as e:
+        c.native_body(0)
+    assert e.value.status == 4
