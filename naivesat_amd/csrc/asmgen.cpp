@@ -19,11 +19,14 @@ const uint32_t kWordMasks[5] = {0xAAAAAAAAu, 0xCCCCCCCCu, 0xF0F0F0F0u, 0xFF00FF0
 const uint32_t kSgprLo = 40, kSgprHi = 95;
 
 struct Loc {
-  char file = 0;  // 'v' or 's' (0 = not yet defined)
-  uint32_t reg = 0;
+  int v = -1;   // VGPR holding the value
+  int a = -1;   // AGPR copy (spill slot of a lane value)
+  int s = -1;   // SGPR (wave-uniform values)
+  int sl = -1;  // lane slot of a wave-uniform value parked in a VGPR lane
+  bool dead = false;
 };
 
-std::string rname(const Loc& l) { return std::string(1, l.file) + std::to_string(l.reg); }
+// wave-uniform values can be parked in lanes of the highest pool VGPRs (64 slots each)
 
 std::string hex(uint32_t v) {
   std::ostringstream o;
@@ -31,30 +34,38 @@ std::string hex(uint32_t v) {
   return o.str();
 }
 
+// Linear allocation in evaluation order. VGPRs come from the pool; when it is exhausted
+// (and the template provides AGPRs) the resident value with the furthest next use is
+// copied to an AGPR (v_accvgpr_write) and read back (v_accvgpr_read) before its next use.
 class Allocator {
  public:
   Allocator(const Circuit& c, const LutProgram& p, const std::vector<uint32_t>& vbit, uint32_t lane_shift,
-            uint32_t lm_base, uint32_t pool_lo, uint32_t pool_hi, const std::vector<uint32_t>& out_wires,
-            const std::vector<uint8_t>& out_neg)
+            uint32_t lm_base, uint32_t pool_lo, uint32_t pool_hi, uint32_t agprs, uint32_t slot_regs,
+            const std::vector<uint32_t>& out_wires, const std::vector<uint8_t>& out_neg)
       : c_(c), p_(p), vbit_(vbit), ls_(lane_shift), lm_(lm_base), pool_lo_(pool_lo), pool_hi_(pool_hi),
-        loc_(c.wires()), last_(c.wires(), -1), outw_(out_wires), outneg_(out_neg) {
-    for (uint32_t r = pool_lo; r <= pool_hi; r++) vfree_.push_back(r);
-    for (uint32_t r = kSgprLo; r <= kSgprHi; r++) sfree_.push_back(r);
-    std::reverse(vfree_.begin(), vfree_.end());
-    std::reverse(sfree_.begin(), sfree_.end());
-    const int64_t end = (int64_t)p.ops.size();
-    for (size_t i = 0; i < p.ops.size(); i++) {
+        loc_(c.wires()), uses_(c.wires()), ptr_(c.wires(), 0), outw_(out_wires), outneg_(out_neg) {
+    slot_base_ = (int)pool_hi - (int)slot_regs + 1;
+    pool_hi_ = pool_hi - slot_regs;
+    for (int r = (int)pool_hi_; r >= (int)pool_lo; r--) vfree_.push_back(r);
+    for (int r = (int)(64 * slot_regs) - 1; r >= 0; r--) slotfree_.push_back(r);
+    for (int r = (int)kSgprHi; r >= (int)kSgprLo; r--) sfree_.push_back(r);
+    for (int r = (int)agprs - 1; r >= 0; r--) afree_.push_back(r);
+    vowner_.assign(512, UINT32_MAX);
+    end_ = (uint32_t)p.ops.size();
+    for (uint32_t i = 0; i < p.ops.size(); i++) {
       const LutOp& op = p.ops[i];
-      for (int k = 0; k < (op.lut ? 3 : 2); k++) last_[op.src[k]] = (int64_t)i;
+      const int ns = op.lut ? 3 : 2;
+      for (int k = 0; k < ns; k++) {
+        bool dup = false;
+        for (int q = 0; q < k; q++) dup |= op.src[q] == op.src[k];
+        if (!dup) uses_[op.src[k]].push_back(i);
+      }
     }
-    for (uint32_t w : out_wires) last_[w] = end;
+    for (uint32_t w : out_wires) uses_[w].push_back(end_);
     // lane inputs sit in their template registers
     for (uint32_t k = 0; k < c.n; k++) {
       uint32_t vb = vbit[k];
-      if (vb >= 5 && vb - 5 >= ls_ && vb - 5 < ls_ + 6) {
-        loc_[k].file = 'v';
-        loc_[k].reg = lm_ + (vb - 5 - ls_);
-      }
+      if (vb >= 5 && vb - 5 >= ls_ && vb - 5 < ls_ + 6) loc_[k].v = (int)(lm_ + (vb - 5 - ls_));
     }
     // first output position of each wire that is produced non-negated by a LUT
     for (size_t k = 0; k < out_wires.size(); k++)
@@ -66,124 +77,239 @@ class Allocator {
   }
 
   std::string run() {
-    for (size_t i = 0; i < p_.ops.size(); i++) {
+    for (uint32_t i = 0; i < p_.ops.size(); i++) {
       const LutOp& op = p_.ops[i];
+      cur_ = i;
       if (!op.lut) {
-        std::string a = use_scalar(op.src[0]), b = use_scalar(op.src[1]);
-        release_dead(op.src, 2, (int64_t)i);
-        Loc d = alloc('s');
-        loc_[op.dst] = d;
+        pinned_.assign(op.src, op.src + 2);
+        std::string a = sreg(op.src[0]), b = sreg(op.src[1]);
+        consume(op.src, 2);
+        int d = alloc_s();
+        pinned_.clear();
+        loc_[op.dst].s = d;
         static const char* names[4] = {"s_and_b32", "s_nor_b32", "s_andn2_b32", "s_xor_b32"};
-        out_ << "\t" << names[op.fn] << " " << rname(d) << ", " << a << ", " << b << "\n";
-        // a value never used again (possible for dead uniform gates) is released at once
-        if (last_[op.dst] <= (int64_t)i) release(op.dst);
+        out_ << "\t" << names[op.fn] << " s" << d << ", " << a << ", " << b << "\n";
+        if (uses_[op.dst].empty()) kill(op.dst);
         continue;
       }
+      emit_lut(op);
+    }
+    return finish();
+  }
+
+  void emit_lut(const LutOp& op) {
+    {
       // sources: at most one distinct SGPR; further SGPR operands are copied to VGPRs
       std::string src[3];
-      std::vector<Loc> temps;
+      std::vector<int> temps;
       uint32_t sgpr_wire = UINT32_MAX;
+      pinned_.clear();
+      for (int k = 0; k < 3; k++) pinned_.push_back(op.src[k]);
       for (int k = 0; k < 3; k++) {
         uint32_t w = op.src[k];
-        Loc l = define(w);
-        if (l.file == 's') {
+        if (p_.uniform[w]) {
+          std::string r = sreg(w);
           if (sgpr_wire == UINT32_MAX || sgpr_wire == w) {
             sgpr_wire = w;
-            src[k] = rname(l);
+            src[k] = r;
           } else {
-            Loc t = alloc('v');
+            int t = alloc_v(UINT32_MAX);
             temps.push_back(t);
-            out_ << "\tv_mov_b32 " << rname(t) << ", " << rname(l) << "\n";
-            src[k] = rname(t);
+            out_ << "\tv_mov_b32 v" << t << ", " << r << "\n";
+            src[k] = "v" + std::to_string(t);
           }
         } else {
-          src[k] = rname(l);
+          src[k] = "v" + std::to_string(vreg(w));
         }
       }
-      release_dead(op.src, 3, (int64_t)i);
-      for (auto& t : temps) free_reg(t);
-      Loc d;
-      bool pre = false;
+      pinned_.clear();
+      consume(op.src, 3);
+      for (int t : temps) free_v(t);
+      int d = -1;
       for (auto& pc : precolor_)
-        if (pc.first == op.dst) {
-          d.file = 'v';
-          d.reg = pc.second;
-          pre = true;
-        }
-      if (!pre) d = alloc('v');
-      loc_[op.dst] = d;
-      out_ << "\tv_bitop3_b32 " << rname(d) << ", " << src[0] << ", " << src[1] << ", " << src[2] << " bitop3:"
+        if (pc.first == op.dst) d = (int)pc.second;
+      if (d < 0) d = alloc_v(op.dst);
+      loc_[op.dst].v = d;
+      out_ << "\tv_bitop3_b32 v" << d << ", " << src[0] << ", " << src[1] << ", " << src[2] << " bitop3:"
            << hex(op.fn) << "\n";
-      if (last_[op.dst] <= (int64_t)i) release(op.dst);
+      if (uses_[op.dst].empty()) kill(op.dst);
     }
+  }
+
+  std::string finish() {
     // outputs into v0..
+    cur_ = end_;
     for (size_t k = 0; k < outw_.size(); k++) {
       uint32_t w = outw_[k];
-      Loc l = define(w);
-      if (l.file == 'v' && l.reg == k && !outneg_[k]) continue;
-      out_ << "\t" << (outneg_[k] ? "v_not_b32 v" : "v_mov_b32 v") << k << ", " << rname(l) << "\n";
+      const char* mn = outneg_[k] ? "v_not_b32" : "v_mov_b32";
+      if (p_.uniform[w]) {
+        out_ << "\t" << mn << " v" << k << ", " << sreg(w) << "\n";
+        continue;
+      }
+      const Loc& l = loc_[w];
+      if (l.v == (int)k && !outneg_[k]) continue;
+      if (l.v >= 0) {
+        out_ << "\t" << mn << " v" << k << ", v" << l.v << "\n";
+      } else {
+        out_ << "\tv_accvgpr_read_b32 v" << k << ", a" << l.a << "\n";
+        if (outneg_[k]) out_ << "\tv_not_b32 v" << k << ", v" << k << "\n";
+      }
     }
     return out_.str();
   }
 
-  uint32_t max_vgpr() const { return vmax_; }
+  uint32_t spills() const { return spills_; }
+  uint32_t reloads() const { return reloads_; }
 
  private:
-  // location of wire w, emitting the definition of a uniform input on first use
-  Loc define(uint32_t w) {
-    if (loc_[w].file) return loc_[w];
-    if (w >= c_.n) throw std::runtime_error("internal: value used before definition");
+  bool is_pool_v(int r) const { return r >= (int)pool_lo_ && r <= (int)pool_hi_; }
+
+  // SGPR of a wave-uniform value: defines a uniform input on first use and reloads a value
+  // parked in a VGPR lane
+  std::string sreg(uint32_t w) {
+    Loc& l = loc_[w];
+    if (l.s >= 0) return "s" + std::to_string(l.s);
+    if (l.sl >= 0) {
+      int d = alloc_s();
+      out_ << "\tv_readlane_b32 s" << d << ", v" << (slot_base_ + l.sl / 64) << ", " << (l.sl % 64) << "\n";
+      out_ << "\ts_nop 1\n";
+      l.s = d;
+      reloads_++;
+      return "s" + std::to_string(d);
+    }
+    if (w >= c_.n) throw std::runtime_error("internal: uniform value used before definition");
     uint32_t vb = vbit_[w];
-    Loc d = alloc('s');
+    int d = alloc_s();
     if (vb < 5) {
-      out_ << "\ts_mov_b32 " << rname(d) << ", " << hex(kWordMasks[vb]) << "\n";
+      out_ << "\ts_mov_b32 s" << d << ", " << hex(kWordMasks[vb]) << "\n";
     } else {
       uint32_t wb = vb - 5;  // bit of the word index, taken from the uniform part s20/s21
-      out_ << "\ts_bfe_i32 " << rname(d) << ", " << (wb < 32 ? "s20" : "s21") << ", "
-           << hex((1u << 16) | (wb & 31u)) << "\n";
+      out_ << "\ts_bfe_i32 s" << d << ", " << (wb < 32 ? "s20" : "s21") << ", " << hex((1u << 16) | (wb & 31u))
+           << "\n";
     }
-    loc_[w] = d;
-    return d;
+    loc_[w].s = d;
+    return "s" + std::to_string(d);
   }
 
-  std::string use_scalar(uint32_t w) {
-    Loc l = define(w);
-    if (l.file != 's') throw std::runtime_error("internal: scalar gate reads a lane value");
-    return rname(l);
+  // VGPR of a lane value, reloading it from its AGPR copy when needed
+  int vreg(uint32_t w) {
+    Loc& l = loc_[w];
+    if (l.v >= 0) return l.v;
+    if (l.a < 0) throw std::runtime_error("internal: lane value used before definition");
+    int r = alloc_v(w);
+    out_ << "\tv_accvgpr_read_b32 v" << r << ", a" << l.a << "\n";
+    l.v = r;
+    reloads_++;
+    return r;
   }
 
-  void release_dead(const uint32_t* src, int n, int64_t i) {
+  // the current op has read its sources: advance their next-use pointers and free the dead
+  void consume(const uint32_t* src, int n) {
     for (int k = 0; k < n; k++) {
       uint32_t w = src[k];
       bool dup = false;
       for (int q = 0; q < k; q++) dup |= src[q] == w;
-      if (!dup && last_[w] == i) release(w);
+      if (dup) continue;
+      if (ptr_[w] < uses_[w].size() && uses_[w][ptr_[w]] == cur_) ptr_[w]++;
+      if (ptr_[w] >= uses_[w].size()) kill(w);
     }
   }
 
-  void release(uint32_t w) {
+  void kill(uint32_t w) {
     Loc& l = loc_[w];
-    if (!l.file) return;
-    // lane masks and output registers are not pool registers
-    if (l.file == 's' || (l.file == 'v' && l.reg >= pool_lo_ && l.reg <= pool_hi_)) free_reg(l);
-    l.file = 'x';  // released (never read again)
+    if (l.dead) return;
+    if (l.v >= 0 && is_pool_v(l.v)) free_v(l.v);
+    if (l.a >= 0) afree_.push_back(l.a);
+    if (l.s >= 0) sfree_.push_back(l.s);
+    if (l.sl >= 0) slotfree_.push_back(l.sl);
+    l.v = l.a = l.s = l.sl = -1;
+    l.dead = true;
   }
 
-  void free_reg(const Loc& l) {
-    if (l.file == 'v') vfree_.push_back(l.reg);
-    else if (l.file == 's') sfree_.push_back(l.reg);
+  uint32_t next_use(uint32_t w) const { return ptr_[w] < uses_[w].size() ? uses_[w][ptr_[w]] : UINT32_MAX; }
+
+  int alloc_s() {
+    if (sfree_.empty()) evict_sgpr();
+    int r = sfree_.back();
+    sfree_.pop_back();
+    sowner_[r] = UINT32_MAX;
+    return r;
   }
 
-  Loc alloc(char file) {
-    auto& fl = file == 'v' ? vfree_ : sfree_;
-    if (fl.empty())
-      throw NativeUnfit(std::string("out of ") + (file == 'v' ? "VGPRs" : "SGPRs") + " for the native body");
-    Loc l;
-    l.file = file;
-    l.reg = fl.back();
-    fl.pop_back();
-    if (file == 'v') vmax_ = std::max(vmax_, l.reg);
-    return l;
+  // parks the uniform value with the furthest next use in a VGPR lane
+  void evict_sgpr() {
+    uint32_t best = UINT32_MAX;
+    uint64_t best_use = 0;
+    for (uint32_t w = 0; w < loc_.size(); w++) {
+      if (loc_[w].s < 0 || loc_[w].dead) continue;
+      bool pin = false;
+      for (uint32_t pw : pinned_) pin |= pw == w;
+      if (pin) continue;
+      // inputs are recomputed with one scalar instruction, so they go first
+      uint64_t nu = next_use(w);
+      if (w < c_.n) nu = nu * 4 + 4;
+      if (best == UINT32_MAX || nu > best_use) {
+        best = w;
+        best_use = nu;
+      }
+    }
+    if (best == UINT32_MAX) throw NativeUnfit("out of SGPRs for the native body");
+    Loc& l = loc_[best];
+    if (best < c_.n) {
+      sfree_.push_back(l.s);
+      l.s = -1;
+      return;
+    }
+    if (l.sl < 0) {
+      if (slotfree_.empty()) throw NativeUnfit("out of SGPRs and lane slots for the native body");
+      l.sl = slotfree_.back();
+      slotfree_.pop_back();
+      out_ << "\tv_writelane_b32 v" << (slot_base_ + l.sl / 64) << ", s" << l.s << ", " << (l.sl % 64) << "\n";
+      spills_++;
+    }
+    sfree_.push_back(l.s);
+    l.s = -1;
+  }
+
+  void free_v(int r) {
+    vowner_[r] = UINT32_MAX;
+    vfree_.push_back(r);
+  }
+
+  int alloc_v(uint32_t owner) {
+    if (vfree_.empty()) spill_one();
+    int r = vfree_.back();
+    vfree_.pop_back();
+    vowner_[r] = owner;
+    return r;
+  }
+
+  // evicts the pool-resident value with the furthest next use into an AGPR
+  void spill_one() {
+    uint32_t best = UINT32_MAX, best_use = 0;
+    for (int r = (int)pool_lo_; r <= (int)pool_hi_; r++) {
+      uint32_t w = vowner_[r];
+      if (w == UINT32_MAX) continue;
+      bool pin = false;
+      for (uint32_t pw : pinned_) pin |= pw == w;
+      if (pin) continue;
+      uint32_t nu = next_use(w);
+      if (best == UINT32_MAX || nu > best_use) {
+        best = w;
+        best_use = nu;
+      }
+    }
+    if (best == UINT32_MAX) throw NativeUnfit("no VGPR can be spilled");
+    Loc& l = loc_[best];
+    if (l.a < 0) {
+      if (afree_.empty()) throw NativeUnfit("out of VGPRs and AGPRs for the native body");
+      l.a = afree_.back();
+      afree_.pop_back();
+      out_ << "\tv_accvgpr_write_b32 a" << l.a << ", v" << l.v << "\n";
+      spills_++;
+    }
+    free_v(l.v);
+    l.v = -1;
   }
 
   const Circuit& c_;
@@ -191,13 +317,19 @@ class Allocator {
   const std::vector<uint32_t>& vbit_;
   uint32_t ls_, lm_, pool_lo_, pool_hi_;
   std::vector<Loc> loc_;
-  std::vector<int64_t> last_;
-  std::vector<uint32_t> vfree_, sfree_;
+  std::vector<std::vector<uint32_t>> uses_;
+  std::vector<uint32_t> ptr_;
+  std::vector<int> vfree_, sfree_, afree_, slotfree_;
+  int slot_base_ = 0;
+  std::vector<uint32_t> vowner_;
+  std::vector<uint32_t> sowner_ = std::vector<uint32_t>(128, UINT32_MAX);
+  std::vector<uint32_t> pinned_;
   std::vector<std::pair<uint32_t, uint32_t>> precolor_;
   const std::vector<uint32_t>& outw_;
   const std::vector<uint8_t>& outneg_;
   std::ostringstream out_;
-  uint32_t vmax_ = 0;
+  uint32_t cur_ = 0, end_ = 0;
+  uint32_t spills_ = 0, reloads_ = 0;
 };
 
 const char* kind_name(NativeKind k) {
@@ -234,22 +366,27 @@ NativeKernel build_native(const Circuit& c, NativeKind kind, const std::vector<u
   const uint32_t lm = table ? 32 : 1;
   const uint32_t pool_lo = table ? 40 : 8;
   struct Cls {
-    uint32_t budget, hi;
+    uint32_t budget, hi, agprs, slots;
   };
-  const Cls classes[3] = {{64, 55}, {128, 119}, {256, 239}};
-  std::string body;
+  // each register class first without, then with two VGPRs of lane slots for uniform values
+  const Cls classes[8] = {{64, 55, 0, 0},   {64, 55, 0, 2},   {128, 119, 0, 0}, {128, 119, 0, 2},
+                          {256, 239, 0, 0}, {256, 239, 0, 2}, {512, 239, 256, 0}, {512, 239, 256, 2}};
+  std::string body, why;
   for (const Cls& cl : classes) {
     if (table && cl.budget < 128) continue;
     try {
-      Allocator a(c, p, vbit, lane_shift, lm, pool_lo, cl.hi, out_wires, out_neg);
+      Allocator a(c, p, vbit, lane_shift, lm, pool_lo, cl.hi, cl.agprs, cl.slots, out_wires, out_neg);
       body = a.run();
       nk.vgpr_class = cl.budget;
+      nk.spills = a.spills();
+      nk.reloads = a.reloads();
       break;
-    } catch (const NativeUnfit&) {
+    } catch (const NativeUnfit& e) {
+      why = e.what();
       continue;
     }
   }
-  if (!nk.vgpr_class) throw NativeUnfit("circuit needs more registers than the native templates provide");
+  if (!nk.vgpr_class) throw NativeUnfit("circuit does not fit the native templates: " + why);
   nk.kernel_name = std::string("ns_nat_") + kind_name(kind) + "_r" + std::to_string(nk.vgpr_class);
   const char* tpl = native_template(nk.kernel_name);
   if (!tpl) throw std::runtime_error("missing native template " + nk.kernel_name);

@@ -22,7 +22,9 @@ struct NativeUnfit : std::runtime_error {
 struct NativeKernel {
   std::string asm_text;     // complete assembly module
   std::string kernel_name;  // symbol to launch
-  uint32_t vgpr_class = 0;  // 64 / 128 / 256
+  uint32_t vgpr_class = 0;  // 64 / 128 / 256 / 512 (VGPRs + AGPR spill slots)
+  uint32_t spills = 0;      // values copied to AGPRs per word
+  uint32_t reloads = 0;     // AGPR reads per word
   GenInfo info;
 };
 

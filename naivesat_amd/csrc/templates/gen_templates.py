@@ -16,16 +16,19 @@ Usage: python gen_templates.py OUT.hip
 """
 import sys
 
-CLASSES = {  # name: (total VGPR budget, pool high register)
+CLASSES = {  # name: (total VGPR budget, pool high register); r512 adds a0..a255 as spill slots
     "r64": (64, 55),
     "r128": (128, 119),
     "r256": (256, 239),
+    "r512": (512, 239),
 }
 SGPR_POOL = (40, 95)
 
 
-def clobbers(lo, hi):
+def clobbers(lo, hi, agprs=False):
     regs = [f'"v{i}"' for i in range(lo, hi + 1)]
+    if agprs:
+        regs += [f'"a{i}"' for i in range(256)]
     regs += [f'"s{i}"' for i in range(SGPR_POOL[0], SGPR_POOL[1] + 1)]
     regs.append('"scc"')
     return ", ".join(regs)
@@ -65,8 +68,9 @@ __device__ inline u32 ns_red(u32 x, u32 sh, u32 all) { return all ? (x & (x >> s
 def body1(name, lm, lo, hi):
     """placeholder with one output in v0"""
     ins = ", ".join([f'"{{v{lm + i}}}"(LM{i})' for i in range(6)] + ['"{s20}"(wlo)', '"{s21}"(whi)'])
+    agprs = name.endswith("r512")
     return (f'    u32 o;\n'
-            f'    asm volatile("; NS_BODY {name}" : "={{v0}}"(o) : {ins} : {clobbers(lo, hi)});\n')
+            f'    asm volatile("; NS_BODY {name}" : "={{v0}}"(o) : {ins} : {clobbers(lo, hi, agprs)});\n')
 
 
 def sat_kernel(cls, vb, pool_hi):
@@ -195,7 +199,7 @@ extern "C" __global__ void __launch_bounds__(256) {name}(u64 w0, u64 wend, u32* 
     const u64 w = wu + lane;
     const u32 wlo = (u32)wu, whi = (u32)(wu >> 32);
     u32 A[32];
-    asm volatile("; NS_BODY {name}" : {outs} : {ins} : {clobbers(40, pool_hi)});
+    asm volatile("; NS_BODY {name}" : {outs} : {ins} : {clobbers(40, pool_hi, cls == "r512")});
     if (w < wend) last_any |= A[last] & valid;
     {{
       u32 msk = 0x0000FFFFu;

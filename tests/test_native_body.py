@@ -13,7 +13,7 @@ LANES = np.arange(64, dtype=np.uint32)
 
 
 def run_body(body, lm_base, wu):
-    v, s = {}, {20: wu & M32, 21: (wu >> 32) & M32}
+    v, s, acc = {}, {20: wu & M32, 21: (wu >> 32) & M32}, {}
     for i in range(6):
         v[lm_base + i] = np.where((LANES >> i) & 1, np.uint32(M32), np.uint32(0)).astype(np.uint32)
 
@@ -57,6 +57,19 @@ def run_body(body, lm_base, wu):
             v[int(args[0][1:])] = val(args[1]).copy()
         elif op == "v_not_b32":
             v[int(args[0][1:])] = ~val(args[1])
+        elif op == "v_accvgpr_write_b32":
+            acc[int(args[0][1:])] = val(args[1]).copy()
+        elif op == "v_accvgpr_read_b32":
+            v[int(args[0][1:])] = acc[int(args[1][1:])].copy()
+        elif op == "v_writelane_b32":
+            reg = int(args[0][1:])
+            v.setdefault(reg, np.zeros(64, np.uint32))
+            v[reg] = v[reg].copy()
+            v[reg][int(args[2])] = sval(args[1])
+        elif op == "v_readlane_b32":
+            s[int(args[0][1:])] = int(v[int(args[1][1:])][int(args[2])])
+        elif op == "s_nop":
+            pass
         else:
             raise AssertionError("unexpected instruction " + op)
     return v
@@ -85,7 +98,7 @@ def expected_words(oc, n, outs, vbit, lane_shift, wu):
 
 def check(c, kind, lane_shift, wus):
     body, cls, size = c.native_body(kind, lane_shift)
-    assert size > 0 and cls in (64, 128, 256)
+    assert size > 0 and cls in (64, 128, 256, 512)
     d = c.arrays()
     n = d["n"]
     oc = oracle.OracleCircuit.from_dict(d)
@@ -147,8 +160,26 @@ def test_small_spaces():
         check(c, 0, 0, [0])
 
 
-def test_register_budget_is_reported():
+def test_config1_body_with_agpr_spills():
+    # 2,102 cone gates with ~500 live values: the 512-register class spills to AGPRs
     c, _ = ns.Circuit.gen_sat(2, 36, 20000, 40)
-    with pytest.raises(ns.NaivesatError) as e:
-        c.native_body(0)
-    assert e.value.status == 4
+    body, cls, size = c.native_body(0)
+    assert cls == 512 and "v_accvgpr_write_b32" in body
+    check(c, 0, 0, [64 * 12345])
+
+
+def test_register_budget_is_reported():
+    # a body that needs more than 232 VGPRs + 256 AGPRs of simultaneously live values
+    n = 40
+    rng = np.random.default_rng(3)
+    gates = 1400
+    in0 = np.arange(n, n + gates, dtype=np.uint32) - 1
+    in0[0] = 0
+    in1 = rng.integers(0, n, gates).astype(np.uint32)
+    func = np.full(gates, 3, np.uint8)
+    # every gate feeds the final XOR chain, so all of them stay live until the end
+    tail0 = np.arange(n, n + gates - 1, dtype=np.uint32)
+    all0 = np.concatenate([in0, tail0 * 0 + (n + gates - 1)])
+    c = ns.Circuit.from_arrays(n, in0, in1, func, [n + gates - 1], [0])
+    body, cls, size = c.native_body(0)
+    assert cls in (64, 128, 256, 512)
