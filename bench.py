@@ -11,10 +11,13 @@ Also reported on the same line:
                 launches on the engine's stream;
   cpu_baseline  the CPU oracle (C, AVX2, pthreads over the host cores) on a bounded slice
                 of the same search;
-  extra         the SATOSTOP table-pass HBM rate (30-bit Feistel, 2^30-entry table) and the
-                QSAT search time (32 inputs, 10,000 gates).
+  extra         the SATOSTOP table-pass HBM rate (30-bit Feistel, 2^30-entry table), the
+                QSAT search time (32 inputs, 10,000 gates) and the served request stream
+                (4,096 SAT/QSAT/SATOSTOP requests, Poisson 200/s, seed 5): solves/s with every
+                request queued at once and p50/p99 latency with requests arriving in real
+                time, each for 8 concurrent worker lanes and for one-at-a-time solving.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-extra]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-extra] [--no-serve]
 """
 import argparse
 import json
@@ -38,6 +41,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=4096)
+    ap.add_argument("--serve-workers", type=int, default=8)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -50,8 +56,11 @@ def main():
     c, key = ns.Circuit.gen_sat(2, 36, 20000, 40)
     cone, peak_live = c.cone()
     n = 36
+    first_compile_ms = None
     for _ in range(args.warmup):
         r = ctx.sat_solve(c)
+        if first_compile_ms is None:
+            first_compile_ms = r["compile_ms"]
     t0 = time.perf_counter()
     kernel_ms, launches = 0.0, 0
     for _ in range(args.steps):
@@ -76,10 +85,13 @@ def main():
                                "[W(x)==W(k)] over a 40-gate word, planted 36-bit key",
                    "cone_gates": cone, "peak_live_values": peak_live, "assignments_per_step": 1 << n,
                    "launches_per_step": launches // args.steps, "count": r["count"], "lowest": r["lowest"],
-                   "key": key},
+                   "key": key, "backend": "native" if r["backend"] else "hiprtc",
+                   "luts_per_word": r["valu_ops"], "scalar_ops_per_word": r["scalar_ops"],
+                   "vgpr_class": r["vgpr_class"], "compile_ms_first": first_compile_ms},
         "roofline": {"bound": "valu", "achieved": achieved_tops, "peak": peak_tops, "unit": "TOP/s",
                      "frac": achieved_tops / peak_tops, "traffic": None,
-                     "kernel": "ns_sat", "avg_launch_ms": avg_launch_ms,
+                     "kernel": f"ns_nat_sat_r{r['vgpr_class']}" if r["backend"] else "ns_sat",
+                     "avg_launch_ms": avg_launch_ms,
                      "ops_per_launch": ops_per_launch},
     }
     if not args.no_cpu:
@@ -116,8 +128,35 @@ def main():
         extra["qsat"] = {"workload": "QSAT n=32, 10000 gates (seed 3), E8 A8 E8 A8", "kernel_ms": rq["kernel_ms"],
                          "truth": rq["truth"], "result": ns.qsat_result_string(rq), "cone_gates": rq["cone_gates"],
                          "gate_evals_per_s": (1 << 32) * rq["cone_gates"] / (rq["kernel_ms"] / 1000.0)}
+        if not args.no_serve:
+            extra["serve"] = serve_bench(ns, args.serve_requests, args.serve_workers)
         line["extra"] = extra
     print(json.dumps(line))
+
+
+def serve_bench(ns, n_requests, workers):
+    """Served stream vs one-at-a-time; every run starts with an empty kernel cache."""
+    from naivesat_amd import trace
+
+    reqs = trace.make_trace(seed=5, n_requests=n_requests, rate=200.0)
+    out = {"workload": f"{n_requests} requests (60% SAT / 25% QSAT / 15% SATOSTOP, n 16..32, gates "
+                       "200..20000 log-uniform), Poisson 200/s, seed 5"}
+    for name, w, realtime in (("batched", workers, False), ("batched_one_at_a_time", 1, False),
+                              ("served", workers, True), ("served_one_at_a_time", 1, True)):
+        ctx = ns.Context(0)
+        res, total = ctx.serve(reqs, workers=w, realtime=realtime)
+        ctx.close()
+        bad = sum(1 for r in res if r["status"] != 0)
+        d = {"workers": w, "seconds": total, "solves_per_s": len(reqs) / total, "failed": bad,
+             "compile_ms_total": sum(r["compile_ms"] for r in res),
+             "kernel_ms_total": sum(r["kernel_ms"] for r in res)}
+        if realtime:
+            d.update(trace.latency_stats(reqs, res))
+        out[name] = d
+        print(f"serve {name}: {json.dumps(d)}", file=sys.stderr, flush=True)
+    out["batched_speedup"] = out["batched"]["solves_per_s"] / out["batched_one_at_a_time"]["solves_per_s"]
+    out["p99_speedup"] = out["served_one_at_a_time"]["p99_s"] / out["served"]["p99_s"]
+    return out
 
 
 if __name__ == "__main__":

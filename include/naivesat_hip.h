@@ -189,6 +189,39 @@ int ns_satostop_solve(ns_ctx* ctx, const ns_circuit* c, uint32_t unknowns,
 int ns_table_build(ns_ctx* ctx, const ns_circuit* c, uint32_t* table_out, uint64_t table_len,
                    double* kernel_ms);
 
+/* ---- Serving (no counterpart in the reference, which solves one circuit per process):
+ *      a stream of requests solved by `workers` lanes (a HIP stream and buffers each) that
+ *      run concurrently on the device. A free worker takes the next request that has
+ *      arrived, so requests that arrive while others run are admitted as soon as a lane and
+ *      device memory are free. With realtime = 1 arrivals are honoured (a request waits
+ *      until arrival_s after the start); otherwise all requests are available at once. ---- */
+#define NS_REQ_SAT 0u
+#define NS_REQ_QSAT 1u
+#define NS_REQ_SATOSTOP 2u
+typedef struct {
+  uint32_t kind;
+  uint32_t unknowns;          /* SATOSTOP: unknown state bits */
+  const ns_circuit* circuit;
+  const uint8_t* quants_all;  /* QSAT: one quantifier per input */
+  double arrival_s;           /* arrival time after the start of the run */
+} ns_request;
+typedef struct {
+  int32_t status;             /* NS_OK or the error of this request */
+  uint32_t worker;
+  /* SAT: count, lowest, found; QSAT: truth, witness, witness_count, has_witness;
+   * SATOSTOP: n_stopped, lowest_u, steps, end */
+  uint64_t v0, v1, v2, v3;
+  double start_s, finish_s;   /* service start and completion after the start of the run */
+  double compile_ms, kernel_ms;
+} ns_request_result;
+typedef struct {
+  uint32_t workers;           /* concurrent lanes (0 = 8) */
+  uint32_t realtime;
+  uint64_t hbm_budget_bytes;  /* admission limit for request tables (0 = 64 GiB) */
+} ns_serve_opts;
+int ns_serve(ns_ctx* ctx, const ns_request* reqs, uint32_t n, const ns_serve_opts* opts,
+             ns_request_result* results, double* total_s);
+
 /* Diagnostic (no counterpart in the reference; no device needed): builds the native-path
  * kernel for a circuit, assembles and links it, and returns the circuit body's assembly.
  * kind: 0 SAT (output 0), 1 QSAT one-bit-per-thread, 2 QSAT outer-block-in-thread,

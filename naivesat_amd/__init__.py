@@ -72,6 +72,23 @@ class _StopResult(C.Structure):
                 ("wall_ms", C.c_double)]
 
 
+class _Request(C.Structure):
+    _fields_ = [("kind", C.c_uint32), ("unknowns", C.c_uint32), ("circuit", C.c_void_p),
+                ("quants_all", C.c_void_p), ("arrival_s", C.c_double)]
+
+
+class _RequestResult(C.Structure):
+    _fields_ = [("status", C.c_int32), ("worker", C.c_uint32), ("v0", C.c_uint64), ("v1", C.c_uint64),
+                ("v2", C.c_uint64), ("v3", C.c_uint64), ("start_s", C.c_double), ("finish_s", C.c_double),
+                ("compile_ms", C.c_double), ("kernel_ms", C.c_double)]
+
+
+class _ServeOpts(C.Structure):
+    _fields_ = [("workers", C.c_uint32), ("realtime", C.c_uint32), ("hbm_budget_bytes", C.c_uint64)]
+
+
+REQ_SAT, REQ_QSAT, REQ_SATOSTOP = 0, 1, 2
+
 _lib = None
 
 
@@ -108,6 +125,8 @@ def load_library() -> C.CDLL:
         "ns_satostop_pass_times": (I, [P, P, U32, C.POINTER(U32)]),
         "ns_native_body": (I, [P, U32, U32, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t), C.POINTER(U32),
                                C.POINTER(U64)]),
+        "ns_serve": (I, [P, C.POINTER(_Request), U32, C.POINTER(_ServeOpts), C.POINTER(_RequestResult),
+                         C.POINTER(C.c_double)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -123,7 +142,7 @@ def exported_symbols() -> list:
             "ns_circuit_from_arrays", "ns_circuit_info", "ns_circuit_to_arrays", "ns_circuit_format",
             "ns_circuit_eval", "ns_circuit_cone", "ns_circuit_free", "ns_gen_sat", "ns_gen_qsat",
             "ns_gen_satostop", "ns_ctx_create", "ns_ctx_destroy", "ns_sat_solve", "ns_qsat_solve",
-            "ns_satostop_solve", "ns_table_build", "ns_satostop_pass_times", "ns_native_body"]
+            "ns_satostop_solve", "ns_table_build", "ns_satostop_pass_times", "ns_native_body", "ns_serve"]
 
 
 def _check(status: int) -> None:
@@ -345,6 +364,26 @@ class Context:
         ms = C.c_double()
         _check(load_library().ns_table_build(self._h, c._h, _ptr(t), t.size, C.byref(ms)))
         return t
+
+    def serve(self, requests, workers: int = 8, realtime: bool = True, hbm_budget_bytes: int = 0):
+        """Solves a stream of requests on concurrent lanes. Each request is a dict with
+        kind (REQ_SAT / REQ_QSAT / REQ_SATOSTOP), circuit, quants (QSAT), unknowns
+        (SATOSTOP) and arrival_s. Returns (list of result dicts, total seconds)."""
+        n = len(requests)
+        arr = (_Request * max(1, n))()
+        keep = []
+        for i, rq in enumerate(requests):
+            q = None
+            if rq["kind"] == REQ_QSAT:
+                q = np.ascontiguousarray(rq["quants"], dtype=np.uint8)
+                keep.append(q)
+            arr[i] = _Request(rq["kind"], rq.get("unknowns", 0), rq["circuit"]._h, _ptr(q), rq.get("arrival_s", 0.0))
+        res = (_RequestResult * max(1, n))()
+        total = C.c_double()
+        o = _ServeOpts(workers, 1 if realtime else 0, hbm_budget_bytes)
+        _check(load_library().ns_serve(self._h, arr, n, C.byref(o), res, C.byref(total)))
+        out = [{k: getattr(res[i], k) for k, _ in _RequestResult._fields_} for i in range(n)]
+        return out, total.value
 
     def pass_times(self) -> list:
         buf = np.zeros(64, np.float64)

@@ -143,7 +143,8 @@ class Allocator {
       uint32_t w = outw_[k];
       const char* mn = outneg_[k] ? "v_not_b32" : "v_mov_b32";
       if (p_.uniform[w]) {
-        out_ << "\t" << mn << " v" << k << ", " << sreg(w) << "\n";
+        std::string r = sreg(w);  // may emit its own definition first
+        out_ << "\t" << mn << " v" << k << ", " << r << "\n";
         continue;
       }
       const Loc& l = loc_[w];
@@ -387,6 +388,9 @@ NativeKernel build_native(const Circuit& c, NativeKind kind, const std::vector<u
     }
   }
   if (!nk.vgpr_class) throw NativeUnfit("circuit does not fit the native templates: " + why);
+  // the table kernel packs all 32 output registers: those without an output are zero
+  if (table)
+    for (size_t k = outs.size(); k < 32; k++) body += "\tv_mov_b32 v" + std::to_string(k) + ", 0\n";
   nk.kernel_name = std::string("ns_nat_") + kind_name(kind) + "_r" + std::to_string(nk.vgpr_class);
   const char* tpl = native_template(nk.kernel_name);
   if (!tpl) throw std::runtime_error("missing native template " + nk.kernel_name);

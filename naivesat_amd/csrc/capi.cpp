@@ -8,6 +8,7 @@
 #include "asmgen.h"
 #include "circuit.h"
 #include "runtime.h"
+#include "serve.h"
 #include "solvers.h"
 
 struct ns_circuit {
@@ -207,7 +208,7 @@ int ns_sat_solve(ns_ctx* ctx, const ns_circuit* c, const ns_sat_opts* opts, ns_s
     need(ctx && c && res, "null argument");
     ns_sat_opts o{};
     if (opts) o = *opts;
-    ns::sat_solve(ctx->ctx, c->c, o, *res);
+    ns::sat_solve(*ctx->ctx.main, c->c, o, *res);
   });
 }
 
@@ -219,7 +220,7 @@ int ns_qsat_solve(ns_ctx* ctx, const ns_circuit* c, const uint8_t* quants_all, c
     if (opts) o = *opts;
     std::vector<uint8_t> q(quants_all, quants_all + c->c.n);
     for (auto& v : q) v = v ? 1 : 0;
-    ns::qsat_solve(ctx->ctx, c->c, q, o, *res, bitmap, bitmap_words);
+    ns::qsat_solve(*ctx->ctx.main, c->c, q, o, *res, bitmap, bitmap_words);
   });
 }
 
@@ -229,15 +230,41 @@ int ns_satostop_solve(ns_ctx* ctx, const ns_circuit* c, uint32_t unknowns, const
     need(ctx && c && res, "null argument");
     ns_satostop_opts o{};
     if (opts) o = *opts;
-    ns::satostop_solve(ctx->ctx, c->c, unknowns, o, *res, steps_out, ends_out);
+    ns::satostop_solve(*ctx->ctx.main, c->c, unknowns, o, *res, steps_out, ends_out);
   });
 }
 
 int ns_table_build(ns_ctx* ctx, const ns_circuit* c, uint32_t* table_out, uint64_t table_len, double* kernel_ms) {
   return guard([&] {
     need(ctx && c, "null argument");
-    double ms = ns::table_build(ctx->ctx, c->c, table_out, table_len);
+    double ms = ns::table_build(*ctx->ctx.main, c->c, table_out, table_len);
     if (kernel_ms) *kernel_ms = ms;
+  });
+}
+
+int ns_serve(ns_ctx* ctx, const ns_request* reqs, uint32_t n, const ns_serve_opts* opts,
+             ns_request_result* results, double* total_s) {
+  return guard([&] {
+    need(ctx && (n == 0 || (reqs && results)), "null argument");
+    std::vector<ns::ServeRequest> rq(n);
+    for (uint32_t i = 0; i < n; i++) {
+      need(reqs[i].circuit != nullptr, "request without a circuit");
+      need(reqs[i].kind <= NS_REQ_SATOSTOP, "bad request kind");
+      rq[i].kind = reqs[i].kind;
+      rq[i].unknowns = reqs[i].unknowns;
+      rq[i].circuit = &reqs[i].circuit->c;
+      rq[i].arrival_s = reqs[i].arrival_s;
+      if (reqs[i].kind == NS_REQ_QSAT) {
+        need(reqs[i].quants_all != nullptr, "QSAT request without quantifiers");
+        rq[i].quants.assign(reqs[i].quants_all, reqs[i].quants_all + rq[i].circuit->n);
+        for (auto& v : rq[i].quants) v = v ? 1 : 0;
+      }
+    }
+    ns_serve_opts o{};
+    if (opts) o = *opts;
+    std::vector<ns_request_result> out;
+    ns::serve(ctx->ctx, rq, o, out, total_s);
+    if (n) std::memcpy(results, out.data(), sizeof(ns_request_result) * n);
   });
 }
 
@@ -278,7 +305,7 @@ int ns_native_body(const ns_circuit* c, uint32_t kind, uint32_t lane_shift, char
 int ns_satostop_pass_times(ns_ctx* ctx, double* pass_ms, uint32_t cap, uint32_t* written) {
   return guard([&] {
     need(ctx && written, "null argument");
-    const auto& v = ns::last_pass_times(ctx->ctx);
+    const auto& v = ns::last_pass_times(*ctx->ctx.main);
     uint32_t k = 0;
     for (; k < v.size() && k < cap; k++)
       if (pass_ms) pass_ms[k] = v[k];

@@ -20,14 +20,22 @@ static void check_rtc(hiprtcResult r, const char* what) {
   if (r != HIPRTC_SUCCESS) throw Error(ST_COMPILE, std::string(what) + ": " + hiprtcGetErrorString(r));
 }
 
-std::shared_ptr<JitModule> Jit::get(const std::string& src, const char* kernel, double* compile_ms) {
+JitModule::~JitModule() {
+  if (mod) (void)hipModuleUnload(mod);
+}
+
+std::shared_ptr<JitModule> Jit::lookup(const std::string& key, double* compile_ms) {
   std::lock_guard<std::mutex> lk(mu_);
-  std::string key = std::string(kernel) + '\n' + src;
   auto it = cache_.find(key);
-  if (it != cache_.end()) {
-    if (compile_ms) *compile_ms = 0.0;
-    return it->second;
-  }
+  if (it == cache_.end()) return nullptr;
+  it->second->last_use = ++clock_;
+  if (compile_ms) *compile_ms = 0.0;
+  return it->second;
+}
+
+std::shared_ptr<JitModule> Jit::get(const std::string& src, const char* kernel, double* compile_ms) {
+  std::string key = std::string(kernel) + '\n' + src;
+  if (auto hit = lookup(key, compile_ms)) return hit;
   auto t0 = std::chrono::steady_clock::now();
   hiprtcProgram prog;
   check_rtc(hiprtcCreateProgram(&prog, src.c_str(), "ns_kernel.hip", 0, nullptr, nullptr),
@@ -69,6 +77,20 @@ std::shared_ptr<JitModule> Jit::get(const std::string& src, const char* kernel, 
   auto t1 = std::chrono::steady_clock::now();
   m->compile_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
   if (compile_ms) *compile_ms = m->compile_ms;
+  return insert(key, m);
+}
+
+std::shared_ptr<JitModule> Jit::insert(const std::string& key, std::shared_ptr<JitModule> m) {
+  std::lock_guard<std::mutex> lk(mu_);
+  auto it = cache_.find(key);
+  if (it != cache_.end()) return it->second;  // compiled concurrently by another thread: keep the first
+  if (cache_.size() >= kCapacity) {
+    auto victim = cache_.begin();
+    for (auto jt = cache_.begin(); jt != cache_.end(); ++jt)
+      if (jt->second->last_use < victim->second->last_use) victim = jt;
+    cache_.erase(victim);
+  }
+  m->last_use = ++clock_;
   cache_[key] = m;
   return m;
 }
@@ -143,13 +165,8 @@ std::vector<char> assemble_native(const std::string& asm_text) {
 }
 
 std::shared_ptr<JitModule> Jit::get_asm(const std::string& asm_text, const std::string& kernel, double* compile_ms) {
-  std::lock_guard<std::mutex> lk(mu_);
   std::string key = "asm:" + kernel + '\n' + asm_text;
-  auto it = cache_.find(key);
-  if (it != cache_.end()) {
-    if (compile_ms) *compile_ms = 0.0;
-    return it->second;
-  }
+  if (auto hit = lookup(key, compile_ms)) return hit;
   auto t0 = std::chrono::steady_clock::now();
   std::vector<char> code = assemble_native(asm_text);
   auto m = std::make_shared<JitModule>();
@@ -157,14 +174,10 @@ std::shared_ptr<JitModule> Jit::get_asm(const std::string& asm_text, const std::
   NS_HIP(hipModuleGetFunction(&m->fn, m->mod, kernel.c_str()));
   m->compile_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   if (compile_ms) *compile_ms = m->compile_ms;
-  cache_[key] = m;
-  return m;
+  return insert(key, m);
 }
 
-Jit::~Jit() {
-  for (auto& kv : cache_)
-    if (kv.second->mod) (void)hipModuleUnload(kv.second->mod);
-}
+Jit::~Jit() = default;
 
 void DevBuf::ensure(size_t bytes) {
   if (bytes <= cap && p) return;
@@ -190,13 +203,18 @@ Context::Context(int dev) : device(dev) {
   if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
     throw Error(ST_UNSUPPORTED, std::string("device arch ") + prop.gcnArchName +
                                     " is not gfx950 (kernels are built for gfx950 only)");
+  main = std::make_unique<Lane>(dev, jit);
+}
+
+Lane::Lane(int dev, Jit& j) : device(dev), jit(j) {
+  NS_HIP(hipSetDevice(dev));
   NS_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   NS_HIP(hipEventCreate(&ev0));
   NS_HIP(hipEventCreate(&ev1));
   stats.ensure(4096);
 }
 
-Context::~Context() {
+Lane::~Lane() {
   (void)hipSetDevice(device);
   if (stream) (void)hipStreamSynchronize(stream);
   buf_a.release();
@@ -209,7 +227,7 @@ Context::~Context() {
   if (stream) (void)hipStreamDestroy(stream);
 }
 
-float Context::elapsed_ms() {
+float Lane::elapsed_ms() {
   float ms = 0.f;
   NS_HIP(hipEventElapsedTime(&ms, ev0, ev1));
   return ms;

@@ -36,9 +36,13 @@ struct JitModule {
   hipModule_t mod = nullptr;
   hipFunction_t fn = nullptr;
   double compile_ms = 0.0;  // 0 when served from the cache
+  uint64_t last_use = 0;    // cache clock of the last lookup
+  ~JitModule();
 };
 
-// Compiles generated sources once per context; keyed by the full source text.
+// Compiles generated sources once per context; keyed by the full source text. Holds at
+// most kCapacity modules: the least recently used is dropped (a solve still running it
+// keeps it alive through its reference).
 class Jit {
  public:
   // HIP source compiled with hiprtc
@@ -48,7 +52,11 @@ class Jit {
   ~Jit();
 
  private:
+  std::shared_ptr<JitModule> lookup(const std::string& key, double* compile_ms);
+  std::shared_ptr<JitModule> insert(const std::string& key, std::shared_ptr<JitModule> m);
+  static constexpr size_t kCapacity = 512;
   std::mutex mu_;
+  uint64_t clock_ = 0;
   std::unordered_map<std::string, std::shared_ptr<JitModule>> cache_;
 };
 
@@ -66,18 +74,38 @@ struct DevBuf {
   T* as() { return static_cast<T*>(p); }
 };
 
-struct Context {
+// One in-flight solve: a HIP stream with its own result and scratch buffers. Several lanes
+// run concurrently on the device (the serving path); the plain entry points use the
+// context's main lane.
+struct Lane {
   int device = 0;
+  Jit& jit;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  Jit jit;
   DevBuf stats;             // small result block
   DevBuf buf_a, buf_b, buf_c, buf_d;
   std::vector<double> pass_ms;  // jump-pass kernel times of the last SATOSTOP solve
-  explicit Context(int dev);
-  ~Context();
+  Lane(int dev, Jit& j);
+  ~Lane();
+  Lane(const Lane&) = delete;
+  Lane& operator=(const Lane&) = delete;
   // elapsed ms between ev0 and ev1 (both recorded on `stream`)
   float elapsed_ms();
+  // device bytes held by the scratch buffers
+  size_t held_bytes() const { return buf_a.cap + buf_b.cap + buf_c.cap + buf_d.cap; }
+  void trim() {
+    buf_a.release();
+    buf_b.release();
+    buf_c.release();
+    buf_d.release();
+  }
+};
+
+struct Context {
+  int device = 0;
+  Jit jit;
+  std::unique_ptr<Lane> main;
+  explicit Context(int dev);
 };
 
 }  // namespace ns

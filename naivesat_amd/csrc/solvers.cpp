@@ -65,7 +65,7 @@ uint64_t chunk_words(uint32_t elem_bits, const Geometry& g, uint64_t total_words
   return cw < tot ? cw : tot;
 }
 
-void launch_range(Context& ctx, hipFunction_t fn, const Geometry& g, uint64_t w0, uint64_t words,
+void launch_range(Lane& ctx, hipFunction_t fn, const Geometry& g, uint64_t w0, uint64_t words,
                   void** extra_args, int n_extra, uint64_t wend) {
   uint64_t blocks = (words + g.words_per_block() - 1) / g.words_per_block();
   if (blocks > 0x7fffffffull) throw Error(ST_UNSUPPORTED, "launch range too large");
@@ -100,7 +100,7 @@ struct Chosen {
 // ---------------------------------------------------------------------------
 // SAT
 // ---------------------------------------------------------------------------
-void sat_solve(Context& ctx, const Circuit& c, const ns_sat_opts& o, ns_sat_result& r) {
+void sat_solve(Lane& ctx, const Circuit& c, const ns_sat_opts& o, ns_sat_result& r) {
   auto t0 = clk::now();
   std::memset(&r, 0, sizeof(r));
   if (c.outputs.size() != 1) throw Error(ST_INVALID, "SAT needs a circuit with exactly one output");
@@ -171,7 +171,7 @@ void sat_solve(Context& ctx, const Circuit& c, const ns_sat_opts& o, ns_sat_resu
 // ---------------------------------------------------------------------------
 // QSAT
 // ---------------------------------------------------------------------------
-void qsat_solve(Context& ctx, const Circuit& c, const std::vector<uint8_t>& q, const ns_qsat_opts& o,
+void qsat_solve(Lane& ctx, const Circuit& c, const std::vector<uint8_t>& q, const ns_qsat_opts& o,
                 ns_qsat_result& r, uint32_t* bitmap, uint64_t bitmap_words) {
   auto t0 = clk::now();
   std::memset(&r, 0, sizeof(r));
@@ -309,7 +309,7 @@ struct TableKernelInfo {
 };
 
 // Builds the u32 table into ctx.buf_a; returns kernel ms; sets *anylast.
-double build_table_dev(Context& ctx, const Circuit& c, uint32_t group_len, uint32_t wpl,
+double build_table_dev(Lane& ctx, const Circuit& c, uint32_t group_len, uint32_t wpl,
                        bool* anylast, double* compile_ms, uint32_t* cone, TableKernelInfo* info_out = nullptr) {
   const uint32_t n = c.n;
   if (n == 0 || n > 31) throw Error(ST_UNSUPPORTED, "next-state tables support 1..31 state bits");
@@ -365,7 +365,7 @@ double build_table_dev(Context& ctx, const Circuit& c, uint32_t group_len, uint3
 
 }  // namespace
 
-double table_build(Context& ctx, const Circuit& c, uint32_t* table_out, uint64_t table_len) {
+double table_build(Lane& ctx, const Circuit& c, uint32_t* table_out, uint64_t table_len) {
   NS_HIP(hipSetDevice(ctx.device));
   bool any = false;
   double cms = 0;
@@ -373,12 +373,13 @@ double table_build(Context& ctx, const Circuit& c, uint32_t* table_out, uint64_t
   const uint64_t states = 1ull << c.n;
   if (table_out) {
     if (table_len < states) throw Error(ST_INVALID, "table buffer too small");
-    NS_HIP(hipMemcpy(table_out, ctx.buf_a.p, states * 4, hipMemcpyDeviceToHost));
+    NS_HIP(hipMemcpyAsync(table_out, ctx.buf_a.p, states * 4, hipMemcpyDeviceToHost, ctx.stream));
+    NS_HIP(hipStreamSynchronize(ctx.stream));
   }
   return kms;
 }
 
-void satostop_solve(Context& ctx, const Circuit& c, uint32_t unknowns, const ns_satostop_opts& o,
+void satostop_solve(Lane& ctx, const Circuit& c, uint32_t unknowns, const ns_satostop_opts& o,
                     ns_satostop_result& r, uint32_t* steps_out, uint32_t* ends_out) {
   auto t0 = clk::now();
   std::memset(&r, 0, sizeof(r));
@@ -460,17 +461,22 @@ void satostop_solve(Context& ctx, const Circuit& c, uint32_t unknowns, const ns_
   if (r.n_stopped) {
     r.lowest_u = host[1];
     r.start = r.lowest_u << (n - unknowns);
-    unsigned int se[2];
-    NS_HIP(hipMemcpy(&se[0], d_steps + r.lowest_u, 4, hipMemcpyDeviceToHost));
-    NS_HIP(hipMemcpy(&se[1], d_ends + r.lowest_u, 4, hipMemcpyDeviceToHost));
+  }
+  unsigned int se[2] = {0, 0};
+  if (r.n_stopped) {
+    NS_HIP(hipMemcpyAsync(&se[0], d_steps + r.lowest_u, 4, hipMemcpyDeviceToHost, ctx.stream));
+    NS_HIP(hipMemcpyAsync(&se[1], d_ends + r.lowest_u, 4, hipMemcpyDeviceToHost, ctx.stream));
+  }
+  if (steps_out) NS_HIP(hipMemcpyAsync(steps_out, d_steps, starts * 4, hipMemcpyDeviceToHost, ctx.stream));
+  if (ends_out) NS_HIP(hipMemcpyAsync(ends_out, d_ends, starts * 4, hipMemcpyDeviceToHost, ctx.stream));
+  NS_HIP(hipStreamSynchronize(ctx.stream));
+  if (r.n_stopped) {
     r.steps = se[0];
     r.end = se[1];
   }
-  if (steps_out) NS_HIP(hipMemcpy(steps_out, d_steps, starts * 4, hipMemcpyDeviceToHost));
-  if (ends_out) NS_HIP(hipMemcpy(ends_out, d_ends, starts * 4, hipMemcpyDeviceToHost));
   r.wall_ms = ms_since(t0);
 }
 
-const std::vector<double>& last_pass_times(Context& ctx) { return ctx.pass_ms; }
+const std::vector<double>& last_pass_times(Lane& ctx) { return ctx.pass_ms; }
 
 }  // namespace ns

@@ -8,14 +8,14 @@
 
 namespace ns {
 
-void sat_solve(Context& ctx, const Circuit& c, const ns_sat_opts& o, ns_sat_result& r);
-void qsat_solve(Context& ctx, const Circuit& c, const std::vector<uint8_t>& quants_all,
+void sat_solve(Lane& ctx, const Circuit& c, const ns_sat_opts& o, ns_sat_result& r);
+void qsat_solve(Lane& ctx, const Circuit& c, const std::vector<uint8_t>& quants_all,
                 const ns_qsat_opts& o, ns_qsat_result& r, uint32_t* bitmap, uint64_t bitmap_words);
-void satostop_solve(Context& ctx, const Circuit& c, uint32_t unknowns, const ns_satostop_opts& o,
+void satostop_solve(Lane& ctx, const Circuit& c, uint32_t unknowns, const ns_satostop_opts& o,
                     ns_satostop_result& r, uint32_t* steps_out, uint32_t* ends_out);
-double table_build(Context& ctx, const Circuit& c, uint32_t* table_out, uint64_t table_len);
+double table_build(Lane& ctx, const Circuit& c, uint32_t* table_out, uint64_t table_len);
 
 // per-pass jump times of the last satostop_solve on a context
-const std::vector<double>& last_pass_times(Context& ctx);
+const std::vector<double>& last_pass_times(Lane& ctx);
 
 }  // namespace ns

@@ -118,6 +118,9 @@ def check(c, kind, lane_shift, wus):
             else:
                 e = exp[i]
             assert np.array_equal(got, e), (kind, lane_shift, wu, i)
+        if kind == 3:  # table entries pack 32 registers: the ones without an output are zero
+            for k in range(len(outs), 32):
+                assert not v[k].any(), (wu, k)
 
 
 @pytest.mark.parametrize("seed", range(5))
@@ -150,6 +153,20 @@ def test_table_bodies():
     d = random_circuit(rng, 14, 120, nout=15, recent=30)
     c2 = ns.Circuit.from_arrays(d["n"], d["in0"], d["in1"], d["func"], d["out_wire"], d["out_neg"])
     check(c2, 3, 0, [0, 64 * 2])
+
+
+def test_table_outputs_are_inputs():
+    # identity next-state function: every output is an input, most of them wave-uniform
+    for m in (5, 12, 20):
+        c = ns.Circuit.from_arrays(m, [], [], [], list(range(m)), [0] * m)
+        check(c, 3, 0, [0] if m < 12 else [0, 64])
+
+
+def test_uniform_and_negated_outputs():
+    n = 17
+    c = ns.Circuit.from_arrays(n, [0], [16], [3], [16, 17, 3, 17], [0, 1, 1, 0])
+    check(c, 3, 0, [0, 64, 128 + 64])
+    check(c, 0, 0, [0, 64])
 
 
 def test_small_spaces():
