@@ -6,9 +6,12 @@ satisfying count and lowest assignment reduced on the device. value = gate-evals
 2^36 x (gates in the output cone) / seconds per step.
 
 Also reported on the same line:
-  roofline      32-bit VALU op rate of the search kernel (one op per gate per 32
-                assignments) against the gfx950 VALU peak, from HIP events around the
-                launches on the engine's stream;
+  roofline      the search kernel's algorithmic 32-bit op rate (one op per cone gate per
+                32 assignments) against the gfx950 VALU peak (256 CUs x 128 lanes/clk x
+                2.4 GHz = 78.6e12 32-bit ops/s, the published 157.3 TFLOP/s FP32 vector rate
+                counted as one op per FMA), from HIP events around the launches on the
+                engine's stream; `traffic` = HBM bytes per launch measured with rocprofv3
+                FETCH_SIZE / WRITE_SIZE (profiles/traffic.txt), when recorded;
   cpu_baseline  the CPU oracle (C, AVX2, pthreads over the host cores) on a bounded slice
                 of the same search;
   extra         the SATOSTOP table-pass HBM rate (30-bit Feistel, 2^30-entry table), the
@@ -18,6 +21,8 @@ Also reported on the same line:
                 time, each for 8 concurrent worker lanes and for one-at-a-time solving.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-extra] [--no-serve]
+       python bench.py --only {sat,table,qsat,serve,copy}   (one part, for profiling runs;
+       `copy` times a plain 1 GiB device copy used to calibrate the HBM byte counters)
 """
 import argparse
 import json
@@ -29,112 +34,109 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "gate-evals/s vs VALU peak; batched solves/s + p99 latency vs one-at-a-time"
-VALU_PEAK_OPS = 256 * 64 * 2.4e9  # 256 CUs x 64 lanes/clk x 2.4 GHz, 32-bit VALU ops/s
+VALU_PEAK_OPS = 256 * 128 * 2.4e9  # 32-bit VALU lane-ops/s (FP32 vector peak / 2)
 HBM_PEAK_GBS = 8000.0
+SAT_WORKLOAD = ("SAT n=36 inputs, 20000 random AND/OR/XOR gates (seed 2), output [W(x)==W(k)] over a "
+                "40-gate word, planted 36-bit key")
+TABLE_WORKLOAD = "SATOSTOP n=30 Feistel (6 rounds x 300 gates, seed 4), 14 unknowns, 2^30-entry table"
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-extra", action="store_true")
-    ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--serve-requests", type=int, default=4096)
-    ap.add_argument("--serve-workers", type=int, default=8)
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
-              "(the search shards by assignment range, see DESIGN.md)", file=sys.stderr)
-        sys.exit(2)
+def measured_traffic():
+    """kernel -> HBM bytes per launch, from profiles/traffic.txt (lines: kernel workload bytes)"""
+    path = os.path.join(ROOT, "profiles", "traffic.txt")
+    out = {}
+    if os.path.exists(path):
+        for line in open(path):
+            t = line.split()
+            if len(t) == 3 and not line.startswith("#"):
+                out[(t[0], t[1])] = float(t[2])
+    return out
 
-    import naivesat_amd as ns
 
-    ctx = ns.Context(0)
+def sat_part(ns, ctx, steps, warmup):
     c, key = ns.Circuit.gen_sat(2, 36, 20000, 40)
     cone, peak_live = c.cone()
     n = 36
     first_compile_ms = None
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         r = ctx.sat_solve(c)
         if first_compile_ms is None:
             first_compile_ms = r["compile_ms"]
     t0 = time.perf_counter()
     kernel_ms, launches = 0.0, 0
-    for _ in range(args.steps):
+    for _ in range(steps):
         r = ctx.sat_solve(c)
         kernel_ms += r["kernel_ms"]
         launches += r["launches"]
     t1 = time.perf_counter()
-    ms_per_step = (t1 - t0) * 1000.0 / args.steps
-    gate_evals = (1 << n) * cone
-    value = gate_evals / (ms_per_step / 1000.0)
-    # roofline of the search kernel: ops per launch / average launch duration
+    if first_compile_ms is None:
+        first_compile_ms = r["compile_ms"]
+    ms_per_step = (t1 - t0) * 1000.0 / steps
+    value = (1 << n) * cone / (ms_per_step / 1000.0)
+    # roofline of the search kernel: algorithmic ops per launch / average launch duration
     avg_launch_ms = kernel_ms / launches
-    words_per_launch = (1 << (n - 5)) / (launches / args.steps)
+    words_per_launch = (1 << (n - 5)) / (launches / steps)
     ops_per_launch = words_per_launch * cone
-    achieved_tops = ops_per_launch / (avg_launch_ms / 1000.0) / 1e12
-    peak_tops = VALU_PEAK_OPS / 1e12
-    line = {
-        "metric": METRIC, "value": value, "unit": "gate-evals/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "u32", "data": "synthetic",
-        "config": {"workload": "SAT n=36 inputs, 20000 random AND/OR/XOR gates (seed 2), output "
-                               "[W(x)==W(k)] over a 40-gate word, planted 36-bit key",
-                   "cone_gates": cone, "peak_live_values": peak_live, "assignments_per_step": 1 << n,
-                   "launches_per_step": launches // args.steps, "count": r["count"], "lowest": r["lowest"],
+    achieved = ops_per_launch / (avg_launch_ms / 1000.0) / 1e12
+    peak = VALU_PEAK_OPS / 1e12
+    kernel = f"ns_nat_sat_r{r['vgpr_class']}" if r["backend"] else "ns_sat"
+    traffic = measured_traffic().get((kernel, "sat_n36"))
+    return c, cone, {
+        "ms_per_step": ms_per_step, "value": value,
+        "config": {"workload": SAT_WORKLOAD, "cone_gates": cone, "gates_nominal": 20000,
+                   "peak_live_values": peak_live, "assignments_per_step": 1 << n,
+                   "launches_per_step": launches // steps, "count": r["count"], "lowest": r["lowest"],
                    "key": key, "backend": "native" if r["backend"] else "hiprtc",
                    "luts_per_word": r["valu_ops"], "scalar_ops_per_word": r["scalar_ops"],
                    "vgpr_class": r["vgpr_class"], "compile_ms_first": first_compile_ms},
-        "roofline": {"bound": "valu", "achieved": achieved_tops, "peak": peak_tops, "unit": "TOP/s",
-                     "frac": achieved_tops / peak_tops, "traffic": None,
-                     "kernel": f"ns_nat_sat_r{r['vgpr_class']}" if r["backend"] else "ns_sat",
-                     "avg_launch_ms": avg_launch_ms,
-                     "ops_per_launch": ops_per_launch},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TOP/s",
+                     "frac": achieved / peak, "traffic": traffic, "kernel": kernel,
+                     "avg_launch_ms": avg_launch_ms, "ops_per_launch": ops_per_launch,
+                     "lut_ops_per_s": words_per_launch * r["valu_ops"] / (avg_launch_ms / 1000.0)},
     }
-    if not args.no_cpu:
-        import oracle
-
-        threads = args.cpu_threads or min(os.cpu_count() or 1, 64)
-        oc = oracle.OracleCircuit.from_dict(c.arrays())
-        words = 1 << 25  # 2^31 assignments (64 per word)
-        tc0 = time.perf_counter()
-        oc.sat_range(0, words, threads=threads)
-        tc = time.perf_counter() - tc0
-        line["cpu_baseline"] = {"value": (words * 64) * cone / tc, "unit": "gate-evals/s", "cores": threads,
-                                "kind": "port", "sample": f"first 2^31 of the 2^36 assignments, {tc:.1f} s"}
-    if not args.no_extra:
-        extra = {}
-        cs, target = ns.Circuit.gen_satostop(4, 15, 6, 300, 14)
-        rs = ctx.satostop_solve(cs, 14, arrays=False)
-        passes = ctx.pass_times()
-        jump = passes[1:]  # u64 passes (the first reads the u32 table)
-        entries = 1 << 30
-        bytes_pass = entries * 24  # read entry + gathered entry + write, 8 B each
-        avg = sum(jump) / len(jump)
-        extra["satostop_table"] = {
-            "workload": "SATOSTOP n=30 Feistel (6 rounds x 300 gates, seed 4), 14 unknowns, 2^30-entry table",
-            "build_ms": rs["build_ms"], "jump_ms": rs["jump_ms"], "passes": rs["passes"],
-            "n_stopped": rs["n_stopped"], "avg_pass_ms": avg,
-            "roofline": {"bound": "hbm", "achieved": bytes_pass / (avg / 1000.0) / 1e9, "peak": HBM_PEAK_GBS,
-                         "unit": "GB/s", "frac": bytes_pass / (avg / 1000.0) / 1e9 / HBM_PEAK_GBS,
-                         "traffic": None, "kernel": "k_jump_pass", "bytes_per_entry": 24}}
-        cq = ns.Circuit.gen_qsat(3, 32, 10000, 200)
-        q = [0] * 8 + [1] * 8 + [0] * 8 + [1] * 8
-        ctx.qsat_solve(cq, q)
-        rq = ctx.qsat_solve(cq, q)
-        extra["qsat"] = {"workload": "QSAT n=32, 10000 gates (seed 3), E8 A8 E8 A8", "kernel_ms": rq["kernel_ms"],
-                         "truth": rq["truth"], "result": ns.qsat_result_string(rq), "cone_gates": rq["cone_gates"],
-                         "gate_evals_per_s": (1 << 32) * rq["cone_gates"] / (rq["kernel_ms"] / 1000.0)}
-        if not args.no_serve:
-            extra["serve"] = serve_bench(ns, args.serve_requests, args.serve_workers)
-        line["extra"] = extra
-    print(json.dumps(line))
 
 
-def serve_bench(ns, n_requests, workers):
+def cpu_part(c, cone, threads):
+    import oracle
+
+    oc = oracle.OracleCircuit.from_dict(c.arrays())
+    words = 1 << 25  # 2^31 assignments (64 per oracle word)
+    tc0 = time.perf_counter()
+    oc.sat_range(0, words, threads=threads)
+    tc = time.perf_counter() - tc0
+    return {"value": (words * 64) * cone / tc, "unit": "gate-evals/s", "cores": threads, "kind": "port",
+            "sample": f"first 2^31 of the 2^36 assignments, {tc:.1f} s"}
+
+
+def table_part(ns, ctx):
+    cs, target = ns.Circuit.gen_satostop(4, 15, 6, 300, 14)
+    rs = ctx.satostop_solve(cs, 14, arrays=False)
+    passes = ctx.pass_times()
+    jump = passes[1:]  # u64 passes (the first reads the u32 table)
+    entries = 1 << 30
+    bytes_pass = entries * 24  # read entry + gathered entry + write, 8 B each
+    avg = sum(jump) / len(jump)
+    gbs = bytes_pass / (avg / 1000.0) / 1e9
+    return {"workload": TABLE_WORKLOAD, "build_ms": rs["build_ms"], "jump_ms": rs["jump_ms"],
+            "passes": rs["passes"], "n_stopped": rs["n_stopped"], "lowest_u": rs["lowest_u"],
+            "steps": rs["steps"], "end": rs["end"], "avg_pass_ms": avg,
+            "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": gbs / HBM_PEAK_GBS,
+                         "traffic": measured_traffic().get(("k_jump_pass", "satostop_n30")),
+                         "kernel": "k_jump_pass", "bytes_per_entry": 24}}
+
+
+def qsat_part(ns, ctx):
+    cq = ns.Circuit.gen_qsat(3, 32, 10000, 200)
+    q = [0] * 8 + [1] * 8 + [0] * 8 + [1] * 8
+    ctx.qsat_solve(cq, q)
+    rq = ctx.qsat_solve(cq, q)
+    return {"workload": "QSAT n=32, 10000 gates (seed 3), E8 A8 E8 A8", "kernel_ms": rq["kernel_ms"],
+            "truth": rq["truth"], "result": ns.qsat_result_string(rq), "cone_gates": rq["cone_gates"],
+            "gate_evals_per_s": (1 << 32) * rq["cone_gates"] / (rq["kernel_ms"] / 1000.0)}
+
+
+def serve_part(ns, n_requests, workers):
     """Served stream vs one-at-a-time; every run starts with an empty kernel cache."""
     from naivesat_amd import trace
 
@@ -157,6 +159,78 @@ def serve_bench(ns, n_requests, workers):
     out["batched_speedup"] = out["batched"]["solves_per_s"] / out["batched_one_at_a_time"]["solves_per_s"]
     out["p99_speedup"] = out["served_one_at_a_time"]["p99_s"] / out["served"]["p99_s"]
     return out
+
+
+def copy_part():
+    """A plain device copy of 1 GiB (torch), the reference point for the byte counters."""
+    import torch
+
+    x = torch.empty(1 << 27, dtype=torch.int64, device="cuda")
+    y = torch.empty_like(x)
+    x.fill_(1)
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    y.copy_(x)
+    ev1.record()
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1)
+    return {"copy_bytes_read": 1 << 30, "copy_bytes_written": 1 << 30, "ms": ms,
+            "gbs": 2 * (1 << 30) / (ms / 1000.0) / 1e9}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--serve-requests", type=int, default=4096)
+    ap.add_argument("--serve-workers", type=int, default=8)
+    ap.add_argument("--only", choices=["sat", "table", "qsat", "serve", "copy"])
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(the search shards by assignment range, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import naivesat_amd as ns
+
+    if args.only:
+        if args.only == "copy":
+            part = copy_part()
+        elif args.only == "serve":
+            part = serve_part(ns, args.serve_requests, args.serve_workers)
+        else:
+            ctx = ns.Context(0)
+            if args.only == "sat":
+                part = sat_part(ns, ctx, args.steps, args.warmup)[2]
+            elif args.only == "table":
+                part = table_part(ns, ctx)
+            else:
+                part = qsat_part(ns, ctx)
+        print(json.dumps({"only": args.only, **part}))
+        return
+
+    ctx = ns.Context(0)
+    c, cone, sat = sat_part(ns, ctx, args.steps, args.warmup)
+    line = {
+        "metric": METRIC, "value": sat["value"], "unit": "gate-evals/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": sat["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u32", "data": "synthetic", "config": sat["config"],
+        "roofline": sat["roofline"],
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_part(c, cone, args.cpu_threads or min(os.cpu_count() or 1, 64))
+    if not args.no_extra:
+        extra = {"satostop_table": table_part(ns, ctx), "qsat": qsat_part(ns, ctx)}
+        if not args.no_serve:
+            extra["serve"] = serve_part(ns, args.serve_requests, args.serve_workers)
+        line["extra"] = extra
+    print(json.dumps(line))
 
 
 if __name__ == "__main__":

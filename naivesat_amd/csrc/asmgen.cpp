@@ -46,7 +46,7 @@ class Allocator {
         loc_(c.wires()), uses_(c.wires()), ptr_(c.wires(), 0), outw_(out_wires), outneg_(out_neg) {
     slot_base_ = (int)pool_hi - (int)slot_regs + 1;
     pool_hi_ = pool_hi - slot_regs;
-    for (int r = (int)pool_hi_; r >= (int)pool_lo; r--) vfree_.push_back(r);
+    for (int r = (int)pool_hi_; r >= (int)pool_lo; r--) vfree_[r & 3].push_back(r);
     for (int r = (int)(64 * slot_regs) - 1; r >= 0; r--) slotfree_.push_back(r);
     for (int r = (int)kSgprHi; r >= (int)kSgprLo; r--) sfree_.push_back(r);
     for (int r = (int)agprs - 1; r >= 0; r--) afree_.push_back(r);
@@ -80,6 +80,7 @@ class Allocator {
     for (uint32_t i = 0; i < p_.ops.size(); i++) {
       const LutOp& op = p_.ops[i];
       cur_ = i;
+      prefetch_uniform(i + 2);
       if (!op.lut) {
         pinned_.assign(op.src, op.src + 2);
         std::string a = sreg(op.src[0]), b = sreg(op.src[1]);
@@ -172,10 +173,8 @@ class Allocator {
     if (l.s >= 0) return "s" + std::to_string(l.s);
     if (l.sl >= 0) {
       int d = alloc_s();
-      out_ << "\tv_readlane_b32 s" << d << ", v" << (slot_base_ + l.sl / 64) << ", " << (l.sl % 64) << "\n";
-      out_ << "\ts_nop 1\n";
-      l.s = d;
-      reloads_++;
+      reload_slot(w, d);
+      out_ << "\ts_nop 1\n";  // two wait states between the SGPR write and its read
       return "s" + std::to_string(d);
     }
     if (w >= c_.n) throw std::runtime_error("internal: uniform value used before definition");
@@ -190,6 +189,29 @@ class Allocator {
     }
     loc_[w].s = d;
     return "s" + std::to_string(d);
+  }
+
+  void reload_slot(uint32_t w, int d) {
+    Loc& l = loc_[w];
+    out_ << "\tv_readlane_b32 s" << d << ", v" << (slot_base_ + l.sl / 64) << ", " << (l.sl % 64) << "\n";
+    l.s = d;
+    reloads_++;
+  }
+
+  // reads the parked uniform sources of op j into free SGPRs two ops ahead, so that the
+  // v_readlane and the read of its result are separated without an s_nop
+  void prefetch_uniform(uint32_t j) {
+    if (j >= end_) return;
+    const LutOp& op = p_.ops[j];
+    for (int k = 0; k < (op.lut ? 3 : 2); k++) {
+      uint32_t w = op.src[k];
+      Loc& l = loc_[w];
+      if (!p_.uniform[w] || l.s >= 0 || l.sl < 0 || sfree_.empty()) continue;
+      int d = sfree_.back();
+      sfree_.pop_back();
+      sowner_[d] = UINT32_MAX;
+      reload_slot(w, d);
+    }
   }
 
   // VGPR of a lane value, reloading it from its AGPR copy when needed
@@ -274,13 +296,42 @@ class Allocator {
 
   void free_v(int r) {
     vowner_[r] = UINT32_MAX;
-    vfree_.push_back(r);
+    vfree_[r & 3].push_back(r);
   }
 
+  // bank (register index mod 4) of a lane value's VGPR, or -1
+  int bank_of(uint32_t w) const {
+    if (p_.uniform[w]) return -1;
+    return loc_[w].v >= 0 ? (loc_[w].v & 3) : -1;
+  }
+
+  // A VALU op that reads two different VGPRs of one bank stalls. The new register of
+  // `owner` avoids the banks of the current op's other VGPR sources and of the sources
+  // of owner's next uses that already sit in VGPRs.
   int alloc_v(uint32_t owner) {
-    if (vfree_.empty()) spill_one();
-    int r = vfree_.back();
-    vfree_.pop_back();
+    if (vfree_[0].empty() && vfree_[1].empty() && vfree_[2].empty() && vfree_[3].empty()) spill_one();
+    int pen[4] = {0, 0, 0, 0};
+    if (cur_ < end_ && p_.ops[cur_].lut)
+      for (uint32_t s : p_.ops[cur_].src)
+        if (s != owner && bank_of(s) >= 0) pen[bank_of(s)] += 8;
+    if (owner != UINT32_MAX) {
+      const auto& u = uses_[owner];
+      for (size_t k = ptr_[owner], m = 0; k < u.size() && m < 3; k++, m++) {
+        if (u[k] >= end_ || u[k] == cur_) continue;
+        const LutOp& op = p_.ops[u[k]];
+        if (!op.lut) continue;
+        for (uint32_t s : op.src)
+          if (s != owner && bank_of(s) >= 0) pen[bank_of(s)] += m == 0 ? 4 : 2;
+      }
+    }
+    int best = -1;
+    for (int b = 0; b < 4; b++) {
+      if (vfree_[b].empty()) continue;
+      if (best < 0 || pen[b] < pen[best] || (pen[b] == pen[best] && vfree_[b].size() > vfree_[best].size()))
+        best = b;
+    }
+    int r = vfree_[best].back();
+    vfree_[best].pop_back();
     vowner_[r] = owner;
     return r;
   }
@@ -320,7 +371,7 @@ class Allocator {
   std::vector<Loc> loc_;
   std::vector<std::vector<uint32_t>> uses_;
   std::vector<uint32_t> ptr_;
-  std::vector<int> vfree_, sfree_, afree_, slotfree_;
+  std::vector<int> vfree_[4], sfree_, afree_, slotfree_;
   int slot_base_ = 0;
   std::vector<uint32_t> vowner_;
   std::vector<uint32_t> sowner_ = std::vector<uint32_t>(128, UINT32_MAX);
@@ -332,6 +383,99 @@ class Allocator {
   uint32_t cur_ = 0, end_ = 0;
   uint32_t spills_ = 0, reloads_ = 0;
 };
+
+// Reorders a LUT program to lower the number of simultaneously live lane values: greedy
+// list scheduling over the lane-varying ops (an op is ready when its lane-varying sources
+// exist), preferring the op that ends the most lifetimes, then program order. Wave-uniform
+// ops are emitted just before their first consumer, so scalar values live briefly.
+LutProgram pressure_order(const LutProgram& p, const std::vector<uint32_t>& out_wires, uint32_t wires) {
+  const uint32_t m = (uint32_t)p.ops.size();
+  std::vector<int32_t> prod(wires, -1);
+  for (uint32_t i = 0; i < m; i++) prod[p.ops[i].dst] = (int32_t)i;
+  auto nsrc = [&](const LutOp& op) { return op.lut ? 3 : 2; };
+  std::vector<uint32_t> rem(wires, 0), npred(m, 0);
+  std::vector<std::vector<uint32_t>> succ(m);
+  for (uint32_t i = 0; i < m; i++) {
+    const LutOp& op = p.ops[i];
+    for (int k = 0; k < nsrc(op); k++) {
+      uint32_t s = op.src[k];
+      bool dup = false;
+      for (int q = 0; q < k; q++) dup |= op.src[q] == s;
+      if (dup) continue;
+      rem[s]++;
+      if (prod[s] >= 0 && !p.uniform[s]) {
+        npred[i]++;
+        succ[(uint32_t)prod[s]].push_back(i);
+      }
+    }
+  }
+  for (uint32_t w : out_wires) rem[w]++;
+  LutProgram q;
+  q.uniform = p.uniform;
+  q.valu_ops = p.valu_ops;
+  q.scalar_ops = p.scalar_ops;
+  q.ops.reserve(m);
+  std::vector<uint8_t> done(m, 0);
+  std::vector<uint32_t> stack;
+  auto emit = [&](uint32_t root) {
+    // uniform producers first (depth-first), then the op itself
+    stack.assign(1, root);
+    while (!stack.empty()) {
+      uint32_t i = stack.back();
+      const LutOp& op = p.ops[i];
+      bool pending = false;
+      for (int k = 0; k < nsrc(op); k++) {
+        int32_t pr = prod[op.src[k]];
+        if (pr >= 0 && p.uniform[op.src[k]] && !done[(uint32_t)pr]) {
+          stack.push_back((uint32_t)pr);
+          pending = true;
+          break;
+        }
+      }
+      if (pending) continue;
+      stack.pop_back();
+      if (done[i]) continue;
+      done[i] = 1;
+      q.ops.push_back(op);
+      for (int k = 0; k < nsrc(op); k++) {
+        bool dup = false;
+        for (int r = 0; r < k; r++) dup |= op.src[r] == op.src[k];
+        if (!dup) rem[op.src[k]]--;
+      }
+    }
+  };
+  std::vector<uint32_t> ready;
+  for (uint32_t i = 0; i < m; i++)
+    if (!p.uniform[p.ops[i].dst] && npred[i] == 0) ready.push_back(i);
+  while (!ready.empty()) {
+    size_t bi = 0;
+    int bcost = 0;
+    for (size_t r = 0; r < ready.size(); r++) {
+      const LutOp& op = p.ops[ready[r]];
+      int freed = 0;
+      for (int k = 0; k < nsrc(op); k++) {
+        uint32_t s = op.src[k];
+        bool dup = false;
+        for (int t = 0; t < k; t++) dup |= op.src[t] == s;
+        if (!dup && prod[s] >= 0 && !p.uniform[s] && rem[s] == 1) freed++;
+      }
+      int cost = 1 - freed;
+      if (r == 0 || cost < bcost || (cost == bcost && ready[r] < ready[bi])) {
+        bi = r;
+        bcost = cost;
+      }
+    }
+    uint32_t i = ready[bi];
+    ready[bi] = ready.back();
+    ready.pop_back();
+    emit(i);
+    for (uint32_t j : succ[i])
+      if (--npred[j] == 0) ready.push_back(j);
+  }
+  for (uint32_t i = 0; i < m; i++)
+    if (!done[i]) emit(i);
+  return q;
+}
 
 const char* kind_name(NativeKind k) {
   switch (k) {
@@ -358,7 +502,7 @@ NativeKernel build_native(const Circuit& c, NativeKind kind, const std::vector<u
   std::vector<uint8_t> uni(c.n, 1);
   for (uint32_t k = 0; k < c.n; k++)
     if (vbit[k] >= 5 && vbit[k] - 5 >= lane_shift && vbit[k] - 5 < lane_shift + 6) uni[k] = 0;
-  LutProgram p = map_luts(c, order, out_wires, uni);
+  LutProgram p = pressure_order(map_luts(c, order, out_wires, uni), out_wires, c.wires());
   nk.info.cone_gates = (uint32_t)order.size();
   nk.info.peak_live = peak_live(c, order, outs);
   nk.info.valu_ops = p.valu_ops;

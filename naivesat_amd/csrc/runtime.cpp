@@ -1,5 +1,7 @@
 #include "runtime.h"
 
+#include <cstdio>
+
 #include <amd_comgr/amd_comgr.h>
 #include <hip/hiprtc.h>
 
@@ -18,6 +20,15 @@ void check_hip(hipError_t e, const char* what) {
 
 static void check_rtc(hiprtcResult r, const char* what) {
   if (r != HIPRTC_SUCCESS) throw Error(ST_COMPILE, std::string(what) + ": " + hiprtcGetErrorString(r));
+}
+
+// NS_TRACE_JIT=1 prints the assembly and module-load time of every native kernel
+static bool trace_jit() {
+  static const bool on = [] {
+    const char* e = std::getenv("NS_TRACE_JIT");
+    return e && *e && *e != '0';
+  }();
+  return on;
 }
 
 JitModule::~JitModule() {
@@ -169,10 +180,16 @@ std::shared_ptr<JitModule> Jit::get_asm(const std::string& asm_text, const std::
   if (auto hit = lookup(key, compile_ms)) return hit;
   auto t0 = std::chrono::steady_clock::now();
   std::vector<char> code = assemble_native(asm_text);
+  auto t1 = std::chrono::steady_clock::now();
   auto m = std::make_shared<JitModule>();
   NS_HIP(hipModuleLoadData(&m->mod, code.data()));
   NS_HIP(hipModuleGetFunction(&m->fn, m->mod, kernel.c_str()));
-  m->compile_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  auto t2 = std::chrono::steady_clock::now();
+  m->compile_ms = std::chrono::duration<double, std::milli>(t2 - t0).count();
+  if (trace_jit())
+    std::fprintf(stderr, "[ns jit] %s assemble_ms=%.3f load_ms=%.3f bytes=%zu\n", kernel.c_str(),
+                 std::chrono::duration<double, std::milli>(t1 - t0).count(),
+                 std::chrono::duration<double, std::milli>(t2 - t1).count(), code.size());
   if (compile_ms) *compile_ms = m->compile_ms;
   return insert(key, m);
 }
@@ -181,6 +198,7 @@ Jit::~Jit() = default;
 
 void DevBuf::ensure(size_t bytes) {
   if (bytes <= cap && p) return;
+  if (!owned) throw Error(ST_INTERNAL, "a lent device buffer is smaller than the solve needs");
   release();
   if (bytes == 0) bytes = 256;
   NS_HIP(hipMalloc(&p, bytes));
@@ -188,9 +206,10 @@ void DevBuf::ensure(size_t bytes) {
 }
 
 void DevBuf::release() {
-  if (p) (void)hipFree(p);
+  if (p && owned) (void)hipFree(p);
   p = nullptr;
   cap = 0;
+  owned = true;
 }
 
 Context::Context(int dev) : device(dev) {
@@ -220,7 +239,7 @@ Lane::~Lane() {
   buf_a.release();
   buf_b.release();
   buf_c.release();
-  buf_d.release();
+  trim_tables();
   stats.release();
   if (ev0) (void)hipEventDestroy(ev0);
   if (ev1) (void)hipEventDestroy(ev1);

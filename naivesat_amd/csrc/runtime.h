@@ -63,12 +63,21 @@ class Jit {
 // Assembles and links a gfx950 assembly module in-process; returns the code object.
 std::vector<char> assemble_native(const std::string& asm_text);
 
-// Growable device allocation (contents are not preserved when it grows).
+// Growable device allocation (contents are not preserved when it grows). A buffer can
+// also be lent memory it does not own (a slice of the serving arena): it then never grows
+// and release() only forgets it.
 struct DevBuf {
   void* p = nullptr;
   size_t cap = 0;
+  bool owned = true;
   void ensure(size_t bytes);
   void release();
+  void borrow(void* q, size_t bytes) {
+    release();
+    p = q;
+    cap = bytes;
+    owned = false;
+  }
   ~DevBuf() { release(); }
   template <class T>
   T* as() { return static_cast<T*>(p); }
@@ -83,7 +92,8 @@ struct Lane {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   DevBuf stats;             // small result block
-  DevBuf buf_a, buf_b, buf_c, buf_d;
+  DevBuf buf_a, buf_b, buf_c;         // QSAT scratch (bit arrays)
+  DevBuf tab_a, tab_b, tab_c, tab_d;  // SATOSTOP: next-state table, jump tables, starts
   std::vector<double> pass_ms;  // jump-pass kernel times of the last SATOSTOP solve
   Lane(int dev, Jit& j);
   ~Lane();
@@ -92,12 +102,12 @@ struct Lane {
   // elapsed ms between ev0 and ev1 (both recorded on `stream`)
   float elapsed_ms();
   // device bytes held by the scratch buffers
-  size_t held_bytes() const { return buf_a.cap + buf_b.cap + buf_c.cap + buf_d.cap; }
-  void trim() {
-    buf_a.release();
-    buf_b.release();
-    buf_c.release();
-    buf_d.release();
+  size_t held_bytes() const { return buf_a.cap + buf_b.cap + buf_c.cap + tab_a.cap + tab_b.cap + tab_c.cap + tab_d.cap; }
+  void trim_tables() {
+    tab_a.release();
+    tab_b.release();
+    tab_c.release();
+    tab_d.release();
   }
 };
 

@@ -9,6 +9,7 @@
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <thread>
 
@@ -16,22 +17,84 @@
 
 namespace ns {
 
+namespace {
+
+uint64_t align256(uint64_t x) { return (x + 255) & ~255ull; }
+
+// SATOSTOP buffers, in the order the solver uses them: u32 next-state table, two u64
+// jump tables, per-start results
+void stop_sizes(const ServeRequest& r, uint64_t sz[4]) {
+  const uint64_t states = 1ull << r.circuit->n;
+  sz[0] = align256((states < 32 ? 32 : states) * 4);
+  sz[1] = align256((states < 2 ? 2 : states) * 8);
+  sz[2] = sz[1];
+  sz[3] = align256((1ull << r.unknowns) * 8 + 64);
+}
+
+}  // namespace
+
 uint64_t request_hbm_bytes(const ServeRequest& r) {
-  const uint32_t n = r.circuit->n;
-  if (r.kind == NS_REQ_SATOSTOP) return (1ull << n) * 20ull + (1ull << r.unknowns) * 8ull;  // u32 + 2 x u64 tables
-  if (r.kind == NS_REQ_QSAT) return n > 10 ? (1ull << (n - 8)) : 64;  // reduction bit arrays
-  return 64;
+  if (r.kind != NS_REQ_SATOSTOP) return 0;  // SAT / QSAT use the lane's small buffers
+  uint64_t sz[4];
+  stop_sizes(r, sz);
+  return sz[0] + sz[1] + sz[2] + sz[3];
 }
 
 namespace {
 
 using clk = std::chrono::steady_clock;
 
+// First-fit sub-allocator over one device allocation: SATOSTOP tables are carved out of
+// it, so serving makes no hipMalloc / hipFree calls per request.
+class Arena {
+ public:
+  explicit Arena(uint64_t bytes) : size_(bytes) {
+    if (bytes) {
+      NS_HIP(hipMalloc(&base_, bytes));
+      holes_[0] = bytes;
+    }
+  }
+  ~Arena() {
+    if (base_) (void)hipFree(base_);
+  }
+  char* base() const { return static_cast<char*>(base_); }
+  bool alloc(uint64_t bytes, uint64_t* off) {
+    for (auto it = holes_.begin(); it != holes_.end(); ++it) {
+      if (it->second < bytes) continue;
+      *off = it->first;
+      uint64_t rest = it->second - bytes, at = it->first + bytes;
+      holes_.erase(it);
+      if (rest) holes_[at] = rest;
+      return true;
+    }
+    return false;
+  }
+  void free(uint64_t off, uint64_t bytes) {
+    auto it = holes_.emplace(off, bytes).first;
+    auto nx = std::next(it);
+    if (nx != holes_.end() && it->first + it->second == nx->first) {
+      it->second += nx->second;
+      holes_.erase(nx);
+    }
+    if (it != holes_.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) {
+        pv->second += it->second;
+        holes_.erase(it);
+      }
+    }
+  }
+
+ private:
+  void* base_ = nullptr;
+  uint64_t size_ = 0;
+  std::map<uint64_t, uint64_t> holes_;  // offset -> size
+};
+
 struct Shared {
   std::mutex mu;
   std::condition_variable cv;
-  size_t next = 0;        // next request to hand out (arrival order)
-  uint64_t hbm_used = 0;  // admitted request bytes
+  size_t next = 0;  // next request to hand out (arrival order)
 };
 
 }  // namespace
@@ -39,18 +102,23 @@ struct Shared {
 void serve(Context& ctx, const std::vector<ServeRequest>& reqs, const ns_serve_opts& o,
            std::vector<ns_request_result>& out, double* total_s) {
   const uint32_t workers = o.workers ? o.workers : 8;
-  const uint64_t budget = o.hbm_budget_bytes ? o.hbm_budget_bytes : (64ull << 30);
+  uint64_t biggest = 0;
+  for (const auto& r : reqs) {
+    if (!r.circuit) throw Error(ST_INVALID, "request without a circuit");
+    biggest = std::max(biggest, request_hbm_bytes(r));
+  }
+  // default budget: room for every worker's largest table, at most 16 GiB
+  const uint64_t budget =
+      o.hbm_budget_bytes ? o.hbm_budget_bytes : std::max(biggest, std::min<uint64_t>(biggest * workers, 16ull << 30));
   out.assign(reqs.size(), ns_request_result{});
   // requests are served in arrival order
   std::vector<size_t> order(reqs.size());
   for (size_t i = 0; i < order.size(); i++) order[i] = i;
   std::stable_sort(order.begin(), order.end(),
                    [&](size_t a, size_t b) { return reqs[a].arrival_s < reqs[b].arrival_s; });
-  for (const auto& r : reqs) {
-    if (!r.circuit) throw Error(ST_INVALID, "request without a circuit");
-    if (request_hbm_bytes(r) > budget) throw Error(ST_UNSUPPORTED, "a request needs more memory than the budget");
-  }
+  if (biggest > budget) throw Error(ST_UNSUPPORTED, "a request needs more memory than the budget");
   NS_HIP(hipSetDevice(ctx.device));
+  Arena arena(biggest ? budget : 0);
   std::vector<std::unique_ptr<Lane>> lanes;
   for (uint32_t w = 0; w < workers; w++) lanes.push_back(std::make_unique<Lane>(ctx.device, ctx.jit));
   Shared sh;
@@ -61,22 +129,31 @@ void serve(Context& ctx, const std::vector<ServeRequest>& reqs, const ns_serve_o
     (void)hipSetDevice(ctx.device);
     while (true) {
       size_t idx;
-      uint64_t need;
+      uint64_t need, off = 0;
       {
         std::unique_lock<std::mutex> lk(sh.mu);
         if (sh.next >= order.size()) return;
         idx = order[sh.next];
         need = request_hbm_bytes(reqs[idx]);
-        // admission: wait until the table fits the budget (requests keep arrival order)
+        // admission: the head request waits until its tables fit in the arena; requests
+        // behind it wait too, so they keep arrival order
         sh.cv.wait(lk, [&] {
-          return sh.next >= order.size() || order[sh.next] != idx || sh.hbm_used + need <= budget;
+          return sh.next >= order.size() || order[sh.next] != idx || need == 0 || arena.alloc(need, &off);
         });
         if (sh.next >= order.size()) return;
-        if (order[sh.next] != idx) continue;  // taken meanwhile
+        if (order[sh.next] != idx) continue;  // taken by another worker meanwhile
         sh.next++;
-        sh.hbm_used += need;
       }
       const ServeRequest& r = reqs[idx];
+      if (need) {
+        uint64_t sz[4], at = off;
+        stop_sizes(r, sz);
+        DevBuf* bufs[4] = {&lane.tab_a, &lane.tab_b, &lane.tab_c, &lane.tab_d};
+        for (int k = 0; k < 4; k++) {
+          bufs[k]->borrow(arena.base() + at, sz[k]);
+          at += sz[k];
+        }
+      }
       if (o.realtime) {
         double wait = r.arrival_s - now_s();
         if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(wait));
@@ -114,7 +191,6 @@ void serve(Context& ctx, const std::vector<ServeRequest>& reqs, const ns_serve_o
           res.v3 = tr.end;
           res.compile_ms = tr.compile_ms;
           res.kernel_ms = tr.build_ms + tr.jump_ms;
-          lane.trim();  // table memory goes back to the budget
         }
         res.status = ST_OK;
       } catch (const Error& e) {
@@ -123,9 +199,10 @@ void serve(Context& ctx, const std::vector<ServeRequest>& reqs, const ns_serve_o
         res.status = ST_INTERNAL;
       }
       res.finish_s = now_s();
-      {
+      if (need) {
+        lane.trim_tables();  // forgets the lent slices
         std::lock_guard<std::mutex> lk(sh.mu);
-        sh.hbm_used -= need;
+        arena.free(off, need);
       }
       sh.cv.notify_all();
     }

@@ -308,7 +308,7 @@ struct TableKernelInfo {
   uint32_t vgpr_class = 0;
 };
 
-// Builds the u32 table into ctx.buf_a; returns kernel ms; sets *anylast.
+// Builds the u32 table into ctx.tab_a; returns kernel ms; sets *anylast.
 double build_table_dev(Lane& ctx, const Circuit& c, uint32_t group_len, uint32_t wpl,
                        bool* anylast, double* compile_ms, uint32_t* cone, TableKernelInfo* info_out = nullptr) {
   const uint32_t n = c.n;
@@ -344,8 +344,8 @@ double build_table_dev(Lane& ctx, const Circuit& c, uint32_t group_len, uint32_t
   }
   uint32_t wpl_v = g.wpl, last = (uint32_t)c.outputs.size() - 1, valid = word_valid_mask(n),
            small = n < 5 ? (1u << n) : 0u;
-  ctx.buf_a.ensure((states < 32 ? 32 : states) * 4);
-  unsigned int* T = ctx.buf_a.as<unsigned int>();
+  ctx.tab_a.ensure((states < 32 ? 32 : states) * 4);
+  unsigned int* T = ctx.tab_a.as<unsigned int>();
   unsigned int* flag = reinterpret_cast<unsigned int*>(ctx.stats.as<char>() + 64);
   NS_HIP(launch_fill_u32(ctx.stream, flag, 0u, 1));
   const uint64_t cw = chunk_words(30, g, total_words);
@@ -373,7 +373,7 @@ double table_build(Lane& ctx, const Circuit& c, uint32_t* table_out, uint64_t ta
   const uint64_t states = 1ull << c.n;
   if (table_out) {
     if (table_len < states) throw Error(ST_INVALID, "table buffer too small");
-    NS_HIP(hipMemcpyAsync(table_out, ctx.buf_a.p, states * 4, hipMemcpyDeviceToHost, ctx.stream));
+    NS_HIP(hipMemcpyAsync(table_out, ctx.tab_a.p, states * 4, hipMemcpyDeviceToHost, ctx.stream));
     NS_HIP(hipStreamSynchronize(ctx.stream));
   }
   return kms;
@@ -398,8 +398,8 @@ void satostop_solve(Lane& ctx, const Circuit& c, uint32_t unknowns, const ns_sat
   r.any_stop = any;
   const uint64_t states = 1ull << n;
   const uint64_t starts = 1ull << unknowns;
-  ctx.buf_d.ensure(starts * 8 + 64);
-  unsigned int* d_steps = ctx.buf_d.as<unsigned int>();
+  ctx.tab_d.ensure(starts * 8 + 64);
+  unsigned int* d_steps = ctx.tab_d.as<unsigned int>();
   unsigned int* d_ends = d_steps + starts;
   unsigned long long* stats = ctx.stats.as<unsigned long long>();
   uint32_t max_passes = o.max_passes ? o.max_passes : n;
@@ -410,17 +410,17 @@ void satostop_solve(Lane& ctx, const Circuit& c, uint32_t unknowns, const ns_sat
   if (any) {
     // even entry count for the two-entries-per-thread pass
     const uint64_t cnt = states < 2 ? 2 : states;
-    ctx.buf_b.ensure(cnt * 8);
-    ctx.buf_c.ensure(cnt * 8);
-    unsigned long long* cur = ctx.buf_b.as<unsigned long long>();
-    unsigned long long* nxt = ctx.buf_c.as<unsigned long long>();
+    ctx.tab_b.ensure(cnt * 8);
+    ctx.tab_c.ensure(cnt * 8);
+    unsigned long long* cur = ctx.tab_b.as<unsigned long long>();
+    unsigned long long* nxt = ctx.tab_c.as<unsigned long long>();
     std::vector<hipEvent_t> evs(2 * max_passes);
     for (auto& e : evs) NS_HIP(hipEventCreate(&e));
     uint32_t p = 0;
     for (; p < max_passes; p++) {
       NS_HIP(hipEventRecord(evs[2 * p], ctx.stream));
       if (p == 0) {
-        NS_HIP(launch_jump_first(ctx.stream, ctx.buf_a.as<unsigned int>(), cur, states, n));
+        NS_HIP(launch_jump_first(ctx.stream, ctx.tab_a.as<unsigned int>(), cur, states, n));
       } else {
         NS_HIP(launch_jump_pass(ctx.stream, cur, nxt, states < 2 ? 2 : states, n));
         std::swap(cur, nxt);

@@ -1,0 +1,171 @@
+// VALU issue micro-benchmark for gfx950: time per instruction of the patterns the native
+// circuit bodies use (v_bitop3_b32 chains, SGPR operands, AGPR copies, v_readlane /
+// v_writelane), at 1, 2 and 4 waves per SIMD (256, 512, 1024 blocks of 4 waves on 256 CUs).
+// Prints per-wave clock64 cycles per iteration and the wall-clock lane-op rate.
+#include <hip/hip_runtime.h>
+#include <cstdio>
+#include <cstdlib>
+typedef unsigned int u32;
+typedef unsigned long long u64;
+
+#define R8(x) x x x x x x x x
+#define R64(x) R8(R8(x))
+
+__global__ void __launch_bounds__(256) k_dep(u32* out, int iters, u64* cyc) {
+  u32 a = threadIdx.x, b = a * 3, c = a * 7;
+  u64 t0 = clock64();
+  for (int i = 0; i < iters; i++) {
+    asm volatile(R64("v_bitop3_b32 %0, %0, %1, %2 bitop3:0x96\n") : "+v"(a) : "v"(b), "v"(c));
+  }
+  u64 t1 = clock64();
+  out[blockIdx.x * 256 + threadIdx.x] = a;
+  if (threadIdx.x == 0) cyc[blockIdx.x] = t1 - t0;
+}
+
+// 64 rotating-independent VALU ops per iteration (baseline for the next kernels)
+__global__ void __launch_bounds__(256) k_rot(u32* out, int iters, u64* cyc) {
+  u32 a0 = threadIdx.x, a1 = a0 + 1, a2 = a0 + 2, a3 = a0 + 3, b = a0 * 3, c = a0 * 7;
+  u64 t0 = clock64();
+  for (int i = 0; i < iters; i++) {
+    asm volatile(R8(R8("v_bitop3_b32 %0, %0, %4, %5 bitop3:0x96\nv_bitop3_b32 %1, %1, %4, %5 bitop3:0x96\n"
+                       "v_bitop3_b32 %2, %2, %4, %5 bitop3:0x96\nv_bitop3_b32 %3, %3, %4, %5 bitop3:0x96\n"))
+                 : "+v"(a0), "+v"(a1), "+v"(a2), "+v"(a3) : "v"(b), "v"(c));
+  }
+  u64 t1 = clock64();
+  out[blockIdx.x * 256 + threadIdx.x] = a0 ^ a1 ^ a2 ^ a3;
+  if (threadIdx.x == 0) cyc[blockIdx.x] = t1 - t0;
+}
+
+// 256 rotating VALU ops + 64 SALU ops per iteration (one SALU after every 4 VALU)
+__global__ void __launch_bounds__(256) k_salu(u32* out, int iters, u64* cyc) {
+  u32 a0 = threadIdx.x, a1 = a0 + 1, a2 = a0 + 2, a3 = a0 + 3, b = a0 * 3, c = a0 * 7;
+  u32 s0 = __builtin_amdgcn_readfirstlane(blockIdx.x), s1 = s0 * 5;
+  u64 t0 = clock64();
+  for (int i = 0; i < iters; i++) {
+    asm volatile(R8(R8("v_bitop3_b32 %0, %0, %4, %5 bitop3:0x96\nv_bitop3_b32 %1, %1, %4, %5 bitop3:0x96\n"
+                       "v_bitop3_b32 %2, %2, %4, %5 bitop3:0x96\nv_bitop3_b32 %3, %3, %4, %5 bitop3:0x96\n"
+                       "s_xor_b32 %6, %6, %7\n"))
+                 : "+v"(a0), "+v"(a1), "+v"(a2), "+v"(a3), "+v"(b), "+v"(c), "+s"(s0) : "s"(s1) : "scc");
+  }
+  u64 t1 = clock64();
+  out[blockIdx.x * 256 + threadIdx.x] = a0 ^ a1 ^ a2 ^ a3 ^ s0;
+  if (threadIdx.x == 0) cyc[blockIdx.x] = t1 - t0;
+}
+
+// 256 rotating VALU ops + 64 AGPR write/read pairs per iteration
+__global__ void __launch_bounds__(256) k_acc(u32* out, int iters, u64* cyc) {
+  u32 a0 = threadIdx.x, a1 = a0 + 1, a2 = a0 + 2, a3 = a0 + 3, b = a0 * 3, c = a0 * 7, t = 0;
+  u64 t0 = clock64();
+  for (int i = 0; i < iters; i++) {
+    asm volatile(R8(R8("v_bitop3_b32 %0, %0, %4, %5 bitop3:0x96\nv_accvgpr_write_b32 a7, %0\n"
+                       "v_bitop3_b32 %1, %1, %4, %5 bitop3:0x96\nv_bitop3_b32 %2, %2, %4, %5 bitop3:0x96\n"
+                       "v_bitop3_b32 %3, %3, %4, %5 bitop3:0x96\nv_accvgpr_read_b32 %6, a7\n"))
+                 : "+v"(a0), "+v"(a1), "+v"(a2), "+v"(a3), "+v"(b), "+v"(c), "=v"(t) : : "a7");
+    a0 ^= t;
+  }
+  u64 t1 = clock64();
+  out[blockIdx.x * 256 + threadIdx.x] = a0 ^ a1 ^ a2 ^ a3;
+  if (threadIdx.x == 0) cyc[blockIdx.x] = t1 - t0;
+}
+
+// 256 rotating VALU ops + 64 (readlane, s_nop 1, VALU use of the SGPR) per iteration
+__global__ void __launch_bounds__(256) k_rl(u32* out, int iters, u64* cyc) {
+  u32 a0 = threadIdx.x, a1 = a0 + 1, a2 = a0 + 2, a3 = a0 + 3, b = a0 * 3, c = a0 * 7;
+  u64 t0 = clock64();
+  for (int i = 0; i < iters; i++) {
+    asm volatile(R8(R8("v_readlane_b32 s60, %4, 5\ns_nop 1\nv_bitop3_b32 %0, %0, s60, %5 bitop3:0x96\n"
+                       "v_bitop3_b32 %1, %1, %4, %5 bitop3:0x96\n"
+                       "v_bitop3_b32 %2, %2, %4, %5 bitop3:0x96\nv_bitop3_b32 %3, %3, %4, %5 bitop3:0x96\n"))
+                 : "+v"(a0), "+v"(a1), "+v"(a2), "+v"(a3), "+v"(b), "+v"(c) : : "s60");
+  }
+  u64 t1 = clock64();
+  out[blockIdx.x * 256 + threadIdx.x] = a0 ^ a1 ^ a2 ^ a3;
+  if (threadIdx.x == 0) cyc[blockIdx.x] = t1 - t0;
+}
+
+// like k_rl but the readlane is issued 3 VALU ops before its use and no s_nop
+__global__ void __launch_bounds__(256) k_rl_early(u32* out, int iters, u64* cyc) {
+  u32 a0 = threadIdx.x, a1 = a0 + 1, a2 = a0 + 2, a3 = a0 + 3, b = a0 * 3, c = a0 * 7;
+  u64 t0 = clock64();
+  for (int i = 0; i < iters; i++) {
+    asm volatile(R8(R8("v_readlane_b32 s60, %4, 5\n"
+                       "v_bitop3_b32 %1, %1, %4, %5 bitop3:0x96\n"
+                       "v_bitop3_b32 %2, %2, %4, %5 bitop3:0x96\nv_bitop3_b32 %3, %3, %4, %5 bitop3:0x96\n"
+                       "v_bitop3_b32 %0, %0, s60, %5 bitop3:0x96\n"))
+                 : "+v"(a0), "+v"(a1), "+v"(a2), "+v"(a3), "+v"(b), "+v"(c) : : "s60");
+  }
+  u64 t1 = clock64();
+  out[blockIdx.x * 256 + threadIdx.x] = a0 ^ a1 ^ a2 ^ a3;
+  if (threadIdx.x == 0) cyc[blockIdx.x] = t1 - t0;
+}
+
+// 256 VALU + 64 writelane per iteration
+__global__ void __launch_bounds__(256) k_wl(u32* out, int iters, u64* cyc) {
+  u32 a0 = threadIdx.x, a1 = a0 + 1, a2 = a0 + 2, a3 = a0 + 3, b = a0 * 3, c = a0 * 7, t = 0;
+  u32 s0 = __builtin_amdgcn_readfirstlane(blockIdx.x);
+  u64 t0 = clock64();
+  for (int i = 0; i < iters; i++) {
+    asm volatile(R8(R8("v_writelane_b32 %6, %7, 3\nv_bitop3_b32 %0, %0, %4, %5 bitop3:0x96\n"
+                       "v_bitop3_b32 %1, %1, %4, %5 bitop3:0x96\n"
+                       "v_bitop3_b32 %2, %2, %4, %5 bitop3:0x96\nv_bitop3_b32 %3, %3, %4, %5 bitop3:0x96\n"))
+                 : "+v"(a0), "+v"(a1), "+v"(a2), "+v"(a3), "+v"(b), "+v"(c), "+v"(t) : "s"(s0));
+  }
+  u64 t1 = clock64();
+  out[blockIdx.x * 256 + threadIdx.x] = a0 ^ a1 ^ a2 ^ a3 ^ t;
+  if (threadIdx.x == 0) cyc[blockIdx.x] = t1 - t0;
+}
+
+// dependent chain through an SGPR operand: SALU result feeds the next VALU at once
+__global__ void __launch_bounds__(256) k_salu_dep(u32* out, int iters, u64* cyc) {
+  u32 a0 = threadIdx.x, a1 = a0 + 1, a2 = a0 + 2, a3 = a0 + 3, b = a0 * 3, c = a0 * 7;
+  u32 s0 = __builtin_amdgcn_readfirstlane(blockIdx.x), s1 = s0 * 5;
+  u64 t0 = clock64();
+  for (int i = 0; i < iters; i++) {
+    asm volatile(R8(R8("s_xor_b32 %6, %6, %7\nv_bitop3_b32 %0, %0, %6, %5 bitop3:0x96\n"
+                       "v_bitop3_b32 %1, %1, %4, %5 bitop3:0x96\n"
+                       "v_bitop3_b32 %2, %2, %4, %5 bitop3:0x96\nv_bitop3_b32 %3, %3, %4, %5 bitop3:0x96\n"))
+                 : "+v"(a0), "+v"(a1), "+v"(a2), "+v"(a3), "+v"(b), "+v"(c), "+s"(s0) : "s"(s1) : "scc");
+  }
+  u64 t1 = clock64();
+  out[blockIdx.x * 256 + threadIdx.x] = a0 ^ a1 ^ a2 ^ a3 ^ s0;
+  if (threadIdx.x == 0) cyc[blockIdx.x] = t1 - t0;
+}
+
+typedef void (*kfn)(u32*, int, u64*);
+int main() {
+  struct K { const char* name; kfn f; int valu; int other; } ks[] = {
+      {"dep64", k_dep, 64, 0},         {"rot256", k_rot, 256, 0},       {"salu256+64", k_salu, 256, 64},
+      {"acc256+128", k_acc, 256, 128}, {"rl256+64+nop", k_rl, 256, 64}, {"rlearly256+64", k_rl_early, 256, 64},
+      {"wl256+64", k_wl, 256, 64},     {"saludep256+64", k_salu_dep, 256, 64}};
+  const int maxb = 256 * 8;
+  u32* out;
+  u64* cyc;
+  if (hipMalloc(&out, sizeof(u32) * 256 * maxb) != hipSuccess) return 1;
+  if (hipMalloc(&cyc, sizeof(u64) * maxb) != hipSuccess) return 1;
+  u64* hc = (u64*)malloc(sizeof(u64) * maxb);
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  const int iters = 2000;
+  for (auto& k : ks) {
+    for (int wps = 1; wps <= 4; wps *= 2) {
+      int blocks = 256 * wps;  // 4 waves per block: wps waves per SIMD if spread evenly
+      hipLaunchKernelGGL(k.f, dim3(blocks), dim3(256), 0, 0, out, 10, cyc);
+      (void)hipEventRecord(e0, 0);
+      hipLaunchKernelGGL(k.f, dim3(blocks), dim3(256), 0, 0, out, iters, cyc);
+      (void)hipEventRecord(e1, 0);
+      if (hipEventSynchronize(e1) != hipSuccess) return 2;
+      float ms;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      (void)hipMemcpy(hc, cyc, sizeof(u64) * blocks, hipMemcpyDeviceToHost);
+      double avg = 0;
+      for (int i = 0; i < blocks; i++) avg += hc[i];
+      avg /= blocks;
+      double per_iter = avg / iters;
+      const double lane_ops = (double)blocks * 256.0 * iters * (k.valu + k.other);
+      printf("%-16s waves/SIMD=%d  ms=%.3f  clk/iter=%.1f  clk per VALU=%.2f  lane-ops/s=%.3e\n", k.name, wps, ms,
+             per_iter, per_iter / k.valu, lane_ops / (ms * 1e-3));
+    }
+  }
+  return 0;
+}

@@ -3,6 +3,8 @@
 Reads the Rust test tables under /root/reference (data only — quantifier strings, truth
 tables, push orders and expected results) and writes:
   quant_reducer_cases.json  <- test_quant_reducer, src/qsat.rs:2035-2466 (30 cases)
+  mem_image_find_solution.json <- test_mem_image_find_solution,
+                               src/greedysatostop.rs:1203-1268 (10 cases)
 
 Run once in a container that has the reference mounted:
     python tests/golden/extract_reference_vectors.py
@@ -82,7 +84,52 @@ def quant_reducer_cases():
     return cases
 
 
+def int_expr(text):
+    """value of an integer literal expression of the test tables (<<, *, + and parentheses)"""
+    t = text.strip()
+    if not re.fullmatch(r"[0-9_<>*+() ]+", t):
+        raise ValueError("unsupported expression: " + t)
+    return int(eval(t.replace("_", "")))  # digits and integer operators only (checked above)
+
+
+def mem_image_cases():
+    src = open(os.path.join(REF, "src/greedysatostop.rs")).read()
+    a = src.index("fn test_mem_image_find_solution()")
+    b = src.index(".into_iter()", a)
+    body = src[src.index("[", a) + 1: b]
+    body = body[: body.rindex("]")]
+    cases = []
+    i = 0
+    while True:
+        i = body.find("(", i)
+        if i < 0:
+            break
+        j = matching_paren(body, i)
+        tup = body[i + 1: j]
+        i = j + 1
+        head, _, sol = tup.partition("Some(")
+        parts = [x for x in head.split(",")]
+        vals = [int_expr(x) for x in parts[:6]]
+        if sol:
+            ms = re.search(r"start:\s*([^,]+),\s*end:\s*([^,}]+)", sol)
+            exp = {"start": int_expr(ms.group(1)), "end": int_expr(ms.group(2))}
+        else:
+            if parts[6].strip() != "None":
+                raise ValueError("unexpected solution field: " + parts[6])
+            exp = None
+        cases.append({"state_len": vals[0], "start": vals[1], "len": vals[2], "unknowns": vals[3],
+                      "pos_sol": vals[4], "next": vals[5], "expected": exp})
+    return cases
+
+
 def main():
+    mi = mem_image_cases()
+    if len(mi) != 10:
+        sys.exit(f"expected 10 mem-image cases, found {len(mi)}")
+    with open(os.path.join(OUT, "mem_image_find_solution.json"), "w") as f:
+        json.dump({"source": "src/greedysatostop.rs:1203-1268 (test_mem_image_find_solution)", "cases": mi}, f,
+                  indent=1)
+    print("mem_image_find_solution.json:", len(mi), "cases")
     cases = quant_reducer_cases()
     if len(cases) != 30:
         sys.exit(f"expected 30 QuantReducer cases, found {len(cases)}")

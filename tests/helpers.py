@@ -73,3 +73,45 @@ def identity_circuit(m):
     return {"n": m, "in0": np.zeros(0, np.uint32), "in1": np.zeros(0, np.uint32),
             "func": np.zeros(0, np.uint8), "out_wire": np.arange(m, dtype=np.uint32),
             "out_neg": np.zeros(m, np.uint8)}
+
+
+def point_stop_circuit(n, p, nxt):
+    """Next-state circuit whose table is the identity except at state p, whose entry is
+    STOP with next state nxt: outputs 0..n-1 = s XOR ([s == p] & (p XOR nxt)), output n =
+    [s == p]. Built from And / Nor / Nimpl / Xor gates only."""
+    in0, in1, func = [], [], []
+
+    def gate(f, a, b):
+        in0.append(a)
+        in1.append(b)
+        func.append(f)
+        return n + len(func) - 1
+
+    # literals (wire, positive) of s_i == p_i, folded into one positive AND
+    lits = [(i, (p >> i) & 1 == 1) for i in range(n)]
+    while len(lits) > 1:
+        nxt_lits = []
+        for k in range(0, len(lits) - 1, 2):
+            (a, pa), (b, pb) = lits[k], lits[k + 1]
+            if pa and pb:
+                w = gate(0, a, b)
+            elif pa:
+                w = gate(2, a, b)
+            elif pb:
+                w = gate(2, b, a)
+            else:
+                w = gate(1, a, b)
+            nxt_lits.append((w, True))
+        if len(lits) % 2:
+            nxt_lits.append(lits[-1])
+        lits = nxt_lits
+    eq, pos = lits[0]
+    if not pos:  # n == 1 and p_0 == 0: eq = NOR(s0, s0)
+        eq = gate(1, eq, eq)
+    outs = []
+    for i in range(n):
+        outs.append(gate(3, i, eq) if ((p ^ nxt) >> i) & 1 else i)
+    outs.append(eq)
+    return {"n": n, "in0": np.array(in0, np.uint32), "in1": np.array(in1, np.uint32),
+            "func": np.array(func, np.uint8), "out_wire": np.array(outs, np.uint32),
+            "out_neg": np.zeros(n + 1, np.uint8)}

@@ -9,7 +9,7 @@ import pytest
 
 import naivesat_amd as ns
 import oracle
-from helpers import identity_circuit, random_circuit
+from helpers import identity_circuit, point_stop_circuit, random_circuit
 
 pytestmark = pytest.mark.gpu
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "configs.json")
@@ -116,3 +116,25 @@ def test_config3_full_size_golden(ctx):
     assert hashlib.sha256(r["ends_arr"].tobytes()).hexdigest() == g["ends_sha256"]
     if g["lowest_u"] is not None:
         assert (r["lowest_u"], r["steps"], r["end"]) == (g["lowest_u"], g["steps"], g["end"])
+
+
+def test_mem_image_find_solution_vectors(ctx):
+    # the reference's test_mem_image_find_solution cases (src/greedysatostop.rs:1203-1268)
+    # as full next-state tables: identity except one STOP entry at start + pos_sol
+    with open(os.path.join(os.path.dirname(__file__), "golden", "mem_image_find_solution.json")) as f:
+        cases = json.load(f)["cases"]
+    for case in cases:
+        n, U = case["state_len"], case["unknowns"]
+        p = case["start"] + case["pos_sol"]
+        r = ctx.satostop_solve(mk(point_stop_circuit(n, p, case["next"])), U, arrays=False)
+        exp = case["expected"]
+        if exp is None:
+            assert r["n_stopped"] == 0, case
+        else:
+            assert r["n_stopped"] == 1, case
+            assert (r["start"], r["end"], r["steps"]) == (exp["start"], exp["end"], 1), case
+        # greedysatostop's rule: stop after the first pass in which a start has stopped
+        g = ctx.satostop_solve(mk(point_stop_circuit(n, p, case["next"])), U, first_pass=True, arrays=False)
+        assert (g["n_stopped"] > 0) == (exp is not None), case
+        if exp is not None:
+            assert (g["start"], g["end"], g["passes"]) == (exp["start"], exp["end"], 1), case

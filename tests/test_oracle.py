@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 import oracle
-from helpers import FUNC_TT, block_values, identity_circuit, py_eval, qsat_recursive, random_circuit
+from helpers import FUNC_TT, block_values, identity_circuit, point_stop_circuit, py_eval, qsat_recursive, random_circuit
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 
@@ -173,3 +173,33 @@ def test_table_identity_layout(m):
         pytest.skip("identity table above 2^20 entries is covered on the GPU")
     t = oracle.OracleCircuit.from_dict(identity_circuit(m)).table_u32()
     assert np.array_equal(t, np.arange(1 << m, dtype=np.uint32))
+
+
+def mem_image_cases():
+    with open(os.path.join(GOLDEN, "mem_image_find_solution.json")) as f:
+        return json.load(f)["cases"]
+
+
+def test_mem_image_find_solution_vectors():
+    # the reference's test_mem_image_find_solution cases (src/greedysatostop.rs:1203-1268):
+    # one STOP entry at image position start + pos_sol. A full table that is the identity
+    # elsewhere has the same answers: start states are u << (n - U), the identity entries
+    # never stop, and every case's STOP entry lies inside its image.
+    for case in mem_image_cases():
+        n, U = case["state_len"], case["unknowns"]
+        p = case["start"] + case["pos_sol"]
+        assert case["pos_sol"] < case["len"]
+        d = point_stop_circuit(n, p, case["next"])
+        oc = oracle.OracleCircuit.from_dict(d)
+        t = oc.table_u32()
+        want = np.arange(1 << n, dtype=np.uint32)
+        want[p] = (1 << n) | (case["next"] & ((1 << n) - 1))
+        assert np.array_equal(t, want)
+        r = oc.satostop(U, threads=8)
+        exp = case["expected"]
+        if exp is None:
+            assert r["n_stopped"] == 0, case
+        else:
+            lo = r["lowest_u"]
+            assert r["n_stopped"] == 1 and lo << (n - U) == exp["start"], case
+            assert (r["ends"][lo], r["steps"][lo]) == (exp["end"], 1), case
