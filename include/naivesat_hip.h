@@ -217,7 +217,8 @@ typedef struct {
 typedef struct {
   uint32_t workers;           /* concurrent lanes (0 = 8) */
   uint32_t realtime;
-  uint64_t hbm_budget_bytes;  /* admission limit for request tables (0 = 64 GiB) */
+  uint64_t hbm_budget_bytes;  /* device arena for SATOSTOP tables (0 = workers x the largest
+                                 request's tables, at most 16 GiB, at least the largest) */
 } ns_serve_opts;
 int ns_serve(ns_ctx* ctx, const ns_request* reqs, uint32_t n, const ns_serve_opts* opts,
              ns_request_result* results, double* total_s);
@@ -230,6 +231,19 @@ int ns_serve(ns_ctx* ctx, const ns_request* reqs, uint32_t n, const ns_serve_opt
  * Returns NS_ERR_UNSUPPORTED when the circuit exceeds the native register budget. */
 int ns_native_body(const ns_circuit* c, uint32_t kind, uint32_t lane_shift, char* body, size_t cap,
                    size_t* needed, uint32_t* vgpr_class, uint64_t* code_bytes);
+
+/* Diagnostic (no counterpart in the reference; no device needed): encodes native body text
+ * (the instruction subset ns_native_body returns) into gfx950 machine-code words.
+ * Returns NS_ERR_UNSUPPORTED for an instruction outside that subset. */
+int ns_native_encode(const char* body, size_t len, uint32_t* words, size_t cap, size_t* n_words);
+
+/* Diagnostic (no counterpart in the reference; no device needed): the code object the
+ * engine loads for a circuit's native kernel (kind and lane_shift as for ns_native_body):
+ * the template's cached code object with the encoded body copied into its body slot at
+ * byte *slot_offset (*patched = 1), or the fully assembled module (*patched = 0) when the
+ * body does not fit the largest slot. */
+int ns_native_code(const ns_circuit* c, uint32_t kind, uint32_t lane_shift, void* code, size_t cap,
+                   size_t* needed, uint32_t* patched, uint64_t* slot_offset);
 
 /* Per-pass timing of the last ns_satostop_solve on this context: pass_ms[i] for jump pass i
  * (HIP events on the context stream). Returns the number of passes written. */

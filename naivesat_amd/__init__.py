@@ -127,6 +127,9 @@ def load_library() -> C.CDLL:
                                C.POINTER(U64)]),
         "ns_serve": (I, [P, C.POINTER(_Request), U32, C.POINTER(_ServeOpts), C.POINTER(_RequestResult),
                          C.POINTER(C.c_double)]),
+        "ns_native_encode": (I, [C.c_char_p, C.c_size_t, P, C.c_size_t, C.POINTER(C.c_size_t)]),
+        "ns_native_code": (I, [P, U32, U32, P, C.c_size_t, C.POINTER(C.c_size_t), C.POINTER(U32),
+                               C.POINTER(U64)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -142,7 +145,8 @@ def exported_symbols() -> list:
             "ns_circuit_from_arrays", "ns_circuit_info", "ns_circuit_to_arrays", "ns_circuit_format",
             "ns_circuit_eval", "ns_circuit_cone", "ns_circuit_free", "ns_gen_sat", "ns_gen_qsat",
             "ns_gen_satostop", "ns_ctx_create", "ns_ctx_destroy", "ns_sat_solve", "ns_qsat_solve",
-            "ns_satostop_solve", "ns_table_build", "ns_satostop_pass_times", "ns_native_body", "ns_serve"]
+            "ns_satostop_solve", "ns_table_build", "ns_satostop_pass_times", "ns_native_body", "ns_serve", "ns_native_encode",
+            "ns_native_code"]
 
 
 def _check(status: int) -> None:
@@ -288,11 +292,33 @@ class Circuit:
                                   C.byref(size)))
         return buf.value.decode(), cls.value, size.value
 
+    def native_code(self, kind: int = 0, lane_shift: int = 0):
+        """Code object the engine loads for this circuit's native kernel:
+        (bytes, patched into the template's body slot, slot byte offset)."""
+        lib = load_library()
+        need, patched, off = C.c_size_t(), C.c_uint32(), C.c_uint64()
+        _check(lib.ns_native_code(self._h, kind, lane_shift, None, 0, C.byref(need), C.byref(patched), C.byref(off)))
+        buf = (C.c_char * need.value)()
+        _check(lib.ns_native_code(self._h, kind, lane_shift, buf, need.value, C.byref(need), C.byref(patched),
+                                  C.byref(off)))
+        return bytes(buf), bool(patched.value), off.value
+
     def cone(self):
         """(gates in the output cone, peak live values of the kernel's order)."""
         g, p = C.c_uint32(), C.c_uint32()
         _check(load_library().ns_circuit_cone(self._h, C.byref(g), C.byref(p)))
         return g.value, p.value
+
+
+def native_encode(body: str) -> np.ndarray:
+    """gfx950 machine-code words of native body text (the engine's own encoder)."""
+    lib = load_library()
+    raw = body.encode()
+    n = C.c_size_t()
+    _check(lib.ns_native_encode(raw, len(raw), None, 0, C.byref(n)))
+    w = np.zeros(max(1, n.value), np.uint32)
+    _check(lib.ns_native_encode(raw, len(raw), _ptr(w), w.size, C.byref(n)))
+    return w[: n.value]
 
 
 class Context:

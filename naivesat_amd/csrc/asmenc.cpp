@@ -1,0 +1,277 @@
+// gfx950 machine-code encoding of native circuit bodies and their placement in templates
+// (see asmenc.h). Encodings follow the GFX9 formats: VOP1, VOP3 (with v_bitop3_b32's
+// truth table spread over the ABS, OMOD and NEG fields), VOP3P (AGPR copies), SOP1, SOP2,
+// SOPP.
+#include "asmenc.h"
+
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <sstream>
+
+#include "asmgen.h"
+#include "runtime.h"
+
+namespace ns {
+
+namespace {
+
+constexpr uint32_t kLiteral = 255;
+
+struct Operand {
+  enum Kind { SGPR, VGPR, AGPR, IMM } kind;
+  int64_t value;
+};
+
+bool parse_int(const std::string& t, int64_t* v) {
+  if (t.empty()) return false;
+  char* end = nullptr;
+  long long x = std::strtoll(t.c_str(), &end, 0);
+  if (*end != '\0') return false;
+  *v = x;
+  return true;
+}
+
+bool parse_operand(const std::string& t, Operand* op) {
+  if (t.size() >= 2 && (t[0] == 's' || t[0] == 'v' || t[0] == 'a') && t[1] >= '0' && t[1] <= '9') {
+    int64_t r;
+    if (!parse_int(t.substr(1), &r)) return false;
+    op->kind = t[0] == 's' ? Operand::SGPR : (t[0] == 'v' ? Operand::VGPR : Operand::AGPR);
+    op->value = r;
+    if (op->kind == Operand::SGPR && r > 101) return false;
+    if (op->kind != Operand::SGPR && r > 255) return false;
+    return true;
+  }
+  op->kind = Operand::IMM;
+  return parse_int(t, &op->value);
+}
+
+// inline constant code for an integer, or kLiteral
+uint32_t inline_code(int64_t v) {
+  if (v >= 0 && v <= 64) return 128 + (uint32_t)v;
+  if (v >= -16 && v <= -1) return 192 + (uint32_t)(-v);
+  return kLiteral;
+}
+
+// 9-bit source field of a VALU operand; *lit receives a literal when one is needed
+bool vsrc(const Operand& o, uint32_t* code, bool* has_lit, uint32_t* lit) {
+  switch (o.kind) {
+    case Operand::SGPR: *code = (uint32_t)o.value; return true;
+    case Operand::VGPR: *code = 256 + (uint32_t)o.value; return true;
+    case Operand::IMM:
+      *code = inline_code(o.value);
+      if (*code == kLiteral) {
+        *has_lit = true;
+        *lit = (uint32_t)o.value;
+      }
+      return true;
+    default: return false;
+  }
+}
+
+// 8-bit source field of a SALU operand
+bool ssrc(const Operand& o, uint32_t* code, bool* has_lit, uint32_t* lit) {
+  if (o.kind == Operand::SGPR) {
+    *code = (uint32_t)o.value;
+    return true;
+  }
+  if (o.kind != Operand::IMM) return false;
+  *code = inline_code(o.value);
+  if (*code == kLiteral) {
+    *has_lit = true;
+    *lit = (uint32_t)o.value;
+  }
+  return true;
+}
+
+uint32_t vop3(uint32_t op, uint32_t vdst, uint32_t s0, uint32_t s1, uint32_t s2, uint32_t* w1, uint32_t tt = 0) {
+  // v_bitop3 truth table: bits 2..0 -> NEG, 5..3 -> ABS, 7..6 -> OMOD
+  const uint32_t neg = tt & 7u, abs = (tt >> 3) & 7u, omod = (tt >> 6) & 3u;
+  *w1 = s0 | (s1 << 9) | (s2 << 18) | (omod << 27) | (neg << 29);
+  return (0x34u << 26) | (op << 16) | (abs << 8) | vdst;
+}
+
+}  // namespace
+
+bool encode_body(const std::string& body, std::vector<uint32_t>& w, std::string* err) {
+  std::istringstream in(body);
+  std::string line;
+  auto fail = [&](const std::string& m) {
+    if (err) *err = m + ": " + line;
+    return false;
+  };
+  while (std::getline(in, line)) {
+    size_t a = line.find_first_not_of(" \t");
+    if (a == std::string::npos || line[a] == ';') continue;
+    std::string t = line.substr(a);
+    size_t sp = t.find_first_of(" \t");
+    std::string mn = t.substr(0, sp);
+    std::vector<std::string> args;
+    std::string modifier;
+    if (sp != std::string::npos) {
+      std::string rest = t.substr(sp + 1);
+      size_t bp = rest.find("bitop3:");
+      if (bp != std::string::npos) {
+        modifier = rest.substr(bp + 7);
+        rest = rest.substr(0, bp);
+      }
+      std::string cur;
+      for (char ch : rest) {
+        if (ch == ',') {
+          args.push_back(cur);
+          cur.clear();
+        } else if (ch != ' ' && ch != '\t') {
+          cur += ch;
+        }
+      }
+      if (!cur.empty()) args.push_back(cur);
+    }
+    std::vector<Operand> ops(args.size());
+    for (size_t k = 0; k < args.size(); k++)
+      if (!parse_operand(args[k], &ops[k])) return fail("bad operand '" + args[k] + "'");
+    auto is = [&](size_t k, Operand::Kind kd) { return k < ops.size() && ops[k].kind == kd; };
+    if (mn == "v_bitop3_b32") {
+      int64_t tt;
+      if (ops.size() != 4 || !is(0, Operand::VGPR) || !parse_int(modifier, &tt) || tt < 0 || tt > 255)
+        return fail("bad v_bitop3_b32");
+      uint32_t s[3];
+      for (int k = 0; k < 3; k++) {
+        bool lit = false;
+        uint32_t lv = 0;
+        if (ops[1 + k].kind == Operand::IMM || !vsrc(ops[1 + k], &s[k], &lit, &lv)) return fail("bad source");
+      }
+      uint32_t w1;
+      w.push_back(vop3(0x234, (uint32_t)ops[0].value, s[0], s[1], s[2], &w1, (uint32_t)tt));
+      w.push_back(w1);
+    } else if (mn == "v_mov_b32" || mn == "v_not_b32") {
+      if (ops.size() != 2 || !is(0, Operand::VGPR)) return fail("bad VOP1");
+      uint32_t s0, lv = 0;
+      bool lit = false;
+      if (!vsrc(ops[1], &s0, &lit, &lv)) return fail("bad source");
+      const uint32_t op = mn == "v_mov_b32" ? 0x01 : 0x2b;
+      w.push_back(0x7E000000u | ((uint32_t)ops[0].value << 17) | (op << 9) | s0);
+      if (lit) w.push_back(lv);
+    } else if (mn == "v_accvgpr_write_b32") {
+      if (ops.size() != 2 || !is(0, Operand::AGPR) || !is(1, Operand::VGPR)) return fail("bad accvgpr write");
+      w.push_back(0xD3D94000u | (uint32_t)ops[0].value);
+      w.push_back(0x18000100u | (uint32_t)ops[1].value);
+    } else if (mn == "v_accvgpr_read_b32") {
+      if (ops.size() != 2 || !is(0, Operand::VGPR) || !is(1, Operand::AGPR)) return fail("bad accvgpr read");
+      w.push_back(0xD3D84000u | (uint32_t)ops[0].value);
+      w.push_back(0x18000100u | (uint32_t)ops[1].value);
+    } else if (mn == "v_writelane_b32") {
+      if (ops.size() != 3 || !is(0, Operand::VGPR) || !is(1, Operand::SGPR) || !is(2, Operand::IMM) ||
+          ops[2].value < 0 || ops[2].value > 63)
+        return fail("bad v_writelane_b32");
+      uint32_t w1;
+      w.push_back(vop3(0x28a, (uint32_t)ops[0].value, (uint32_t)ops[1].value, 128 + (uint32_t)ops[2].value, 0, &w1));
+      w.push_back(w1);
+    } else if (mn == "v_readlane_b32") {
+      if (ops.size() != 3 || !is(0, Operand::SGPR) || !is(1, Operand::VGPR) || !is(2, Operand::IMM) ||
+          ops[2].value < 0 || ops[2].value > 63)
+        return fail("bad v_readlane_b32");
+      uint32_t w1;
+      w.push_back(
+          vop3(0x289, (uint32_t)ops[0].value, 256 + (uint32_t)ops[1].value, 128 + (uint32_t)ops[2].value, 0, &w1));
+      w.push_back(w1);
+    } else if (mn == "s_mov_b32") {
+      if (ops.size() != 2 || !is(0, Operand::SGPR)) return fail("bad s_mov_b32");
+      uint32_t s0, lv = 0;
+      bool lit = false;
+      if (!ssrc(ops[1], &s0, &lit, &lv)) return fail("bad source");
+      w.push_back(0xBE800000u | ((uint32_t)ops[0].value << 16) | s0);
+      if (lit) w.push_back(lv);
+    } else if (mn == "s_bfe_i32" || mn == "s_and_b32" || mn == "s_nor_b32" || mn == "s_andn2_b32" ||
+               mn == "s_xor_b32") {
+      if (ops.size() != 3 || !is(0, Operand::SGPR)) return fail("bad SOP2");
+      uint32_t op = mn == "s_bfe_i32" ? 0x26 : mn == "s_and_b32" ? 0x0c : mn == "s_nor_b32" ? 0x18
+                                                : mn == "s_andn2_b32" ? 0x12 : 0x10;
+      uint32_t s0, s1, l0 = 0, l1 = 0;
+      bool lit0 = false, lit1 = false;
+      if (!ssrc(ops[1], &s0, &lit0, &l0) || !ssrc(ops[2], &s1, &lit1, &l1)) return fail("bad source");
+      if (lit0 && lit1 && l0 != l1) return fail("two different literals");
+      w.push_back(0x80000000u | (op << 23) | ((uint32_t)ops[0].value << 16) | (s1 << 8) | s0);
+      if (lit0 || lit1) w.push_back(lit0 ? l0 : l1);
+    } else if (mn == "s_nop") {
+      if (ops.size() != 1 || !is(0, Operand::IMM) || ops[0].value < 0 || ops[0].value > 15) return fail("bad s_nop");
+      w.push_back(0xBF800000u | (uint32_t)ops[0].value);
+    } else {
+      return fail("instruction outside the encoded subset");
+    }
+  }
+  return true;
+}
+
+namespace {
+
+// slot marker: the first 8 bytes of every reserved body region
+const unsigned char kMarker[8] = {0x4d, 0x42, 0x53, 0x4e, 0x42, 0x4f, 0x44, 0x59};
+// slot sizes; the largest stays within reach of one s_branch (+-32767 dwords)
+const size_t kSledBytes[3] = {16384, 65536, 131008};
+
+struct SledTemplate {
+  std::vector<char> elf;
+  size_t offset = 0;  // byte offset of the slot in the code object
+  size_t bytes = 0;
+};
+
+std::mutex g_sled_mu;
+std::map<std::string, std::shared_ptr<SledTemplate>> g_sleds;
+
+std::shared_ptr<SledTemplate> sled_template(const std::string& kernel_name, size_t bytes) {
+  const std::string key = kernel_name + "/" + std::to_string(bytes);
+  std::lock_guard<std::mutex> lk(g_sled_mu);
+  auto it = g_sleds.find(key);
+  if (it != g_sleds.end()) return it->second;
+  const char* tpl = native_template(kernel_name);
+  if (!tpl) throw std::runtime_error("missing native template " + kernel_name);
+  std::string text(tpl);
+  const std::string marker = "; NS_BODY " + kernel_name;
+  size_t pos = text.find(marker);
+  if (pos == std::string::npos) throw std::runtime_error("template placeholder not found");
+  size_t eol = text.find('\n', pos);
+  std::ostringstream sled;
+  sled << "; body slot\n\t.long 0x4e53424d\n\t.long 0x59444f42\n\t.fill " << (bytes / 4 - 2) << ", 4, 0xbf800000\n";
+  text.replace(pos, eol - pos, sled.str());
+  auto st = std::make_shared<SledTemplate>();
+  st->elf = assemble_native(text);
+  st->bytes = bytes;
+  size_t found = 0, at = 0;
+  for (size_t i = 0; i + 8 <= st->elf.size(); i++)
+    if (std::memcmp(&st->elf[i], kMarker, 8) == 0) {
+      found++;
+      at = i;
+    }
+  if (found != 1) throw std::runtime_error("body slot marker not unique in the template code object");
+  st->offset = at;
+  g_sleds[key] = st;
+  return st;
+}
+
+}  // namespace
+
+std::vector<char> patch_native(const std::string& kernel_name, const std::string& body, size_t* slot_offset) {
+  std::vector<uint32_t> words;
+  std::string err;
+  if (!encode_body(body, words, &err)) return {};
+  const size_t need = words.size() * 4 + 4;  // body + branch to the end of the slot
+  size_t bytes = 0;
+  for (size_t b : kSledBytes)
+    if (need <= b) {
+      bytes = b;
+      break;
+    }
+  if (!bytes) return {};
+  std::shared_ptr<SledTemplate> st = sled_template(kernel_name, bytes);
+  std::vector<char> elf = st->elf;
+  const size_t body_bytes = words.size() * 4;
+  const uint32_t skip = (uint32_t)((st->bytes - body_bytes - 4) / 4);  // dwords after the branch
+  words.push_back(0xBF820000u | skip);                                  // s_branch to the slot end
+  std::memcpy(&elf[st->offset], words.data(), words.size() * 4);
+  if (slot_offset) *slot_offset = st->offset;
+  return elf;
+}
+
+}  // namespace ns

@@ -87,7 +87,7 @@ class Allocator {
         consume(op.src, 2);
         int d = alloc_s();
         pinned_.clear();
-        loc_[op.dst].s = d;
+        set_s(op.dst, d);
         static const char* names[4] = {"s_and_b32", "s_nor_b32", "s_andn2_b32", "s_xor_b32"};
         out_ << "\t" << names[op.fn] << " s" << d << ", " << a << ", " << b << "\n";
         if (uses_[op.dst].empty()) kill(op.dst);
@@ -187,14 +187,14 @@ class Allocator {
       out_ << "\ts_bfe_i32 s" << d << ", " << (wb < 32 ? "s20" : "s21") << ", " << hex((1u << 16) | (wb & 31u))
            << "\n";
     }
-    loc_[w].s = d;
+    set_s(w, d);
     return "s" + std::to_string(d);
   }
 
   void reload_slot(uint32_t w, int d) {
     Loc& l = loc_[w];
     out_ << "\tv_readlane_b32 s" << d << ", v" << (slot_base_ + l.sl / 64) << ", " << (l.sl % 64) << "\n";
-    l.s = d;
+    set_s(w, d);
     reloads_++;
   }
 
@@ -243,13 +243,24 @@ class Allocator {
     if (l.dead) return;
     if (l.v >= 0 && is_pool_v(l.v)) free_v(l.v);
     if (l.a >= 0) afree_.push_back(l.a);
-    if (l.s >= 0) sfree_.push_back(l.s);
+    if (l.s >= 0) drop_s(w);
     if (l.sl >= 0) slotfree_.push_back(l.sl);
     l.v = l.a = l.s = l.sl = -1;
     l.dead = true;
   }
 
   uint32_t next_use(uint32_t w) const { return ptr_[w] < uses_[w].size() ? uses_[w][ptr_[w]] : UINT32_MAX; }
+
+  void set_s(uint32_t w, int d) {
+    loc_[w].s = d;
+    sowner_[d] = w;
+  }
+
+  void drop_s(uint32_t w) {
+    sowner_[loc_[w].s] = UINT32_MAX;
+    sfree_.push_back(loc_[w].s);
+    loc_[w].s = -1;
+  }
 
   int alloc_s() {
     if (sfree_.empty()) evict_sgpr();
@@ -263,8 +274,9 @@ class Allocator {
   void evict_sgpr() {
     uint32_t best = UINT32_MAX;
     uint64_t best_use = 0;
-    for (uint32_t w = 0; w < loc_.size(); w++) {
-      if (loc_[w].s < 0 || loc_[w].dead) continue;
+    for (uint32_t r = kSgprLo; r <= kSgprHi; r++) {
+      const uint32_t w = sowner_[r];
+      if (w == UINT32_MAX || loc_[w].dead) continue;
       bool pin = false;
       for (uint32_t pw : pinned_) pin |= pw == w;
       if (pin) continue;
@@ -279,8 +291,7 @@ class Allocator {
     if (best == UINT32_MAX) throw NativeUnfit("out of SGPRs for the native body");
     Loc& l = loc_[best];
     if (best < c_.n) {
-      sfree_.push_back(l.s);
-      l.s = -1;
+      drop_s(best);
       return;
     }
     if (l.sl < 0) {
@@ -290,8 +301,7 @@ class Allocator {
       out_ << "\tv_writelane_b32 v" << (slot_base_ + l.sl / 64) << ", s" << l.s << ", " << (l.sl % 64) << "\n";
       spills_++;
     }
-    sfree_.push_back(l.s);
-    l.s = -1;
+    drop_s(best);
   }
 
   void free_v(int r) {
@@ -516,9 +526,34 @@ NativeKernel build_native(const Circuit& c, NativeKind kind, const std::vector<u
   // each register class first without, then with two VGPRs of lane slots for uniform values
   const Cls classes[8] = {{64, 55, 0, 0},   {64, 55, 0, 2},   {128, 119, 0, 0}, {128, 119, 0, 2},
                           {256, 239, 0, 0}, {256, 239, 0, 2}, {512, 239, 256, 0}, {512, 239, 256, 2}};
-  std::string body, why;
+  // lane values live at once in this order: classes whose pool cannot hold them and that
+  // have no AGPRs to spill to are skipped
+  uint32_t peak = 0;
+  {
+    std::vector<uint32_t> rem(c.wires(), 0);
+    for (const LutOp& op : p.ops)
+      for (int k = 0; k < (op.lut ? 3 : 2); k++) {
+        bool dup = false;
+        for (int q = 0; q < k; q++) dup |= op.src[q] == op.src[k];
+        if (!dup) rem[op.src[k]]++;
+      }
+    for (uint32_t w : out_wires) rem[w]++;
+    uint32_t live = 0;
+    for (const LutOp& op : p.ops) {
+      for (int k = 0; k < (op.lut ? 3 : 2); k++) {
+        uint32_t w = op.src[k];
+        bool dup = false;
+        for (int q = 0; q < k; q++) dup |= op.src[q] == w;
+        if (!dup && --rem[w] == 0 && w >= c.n && !p.uniform[w]) live--;
+      }
+      if (rem[op.dst] > 0 && !p.uniform[op.dst]) live++;
+      peak = std::max(peak, live);
+    }
+  }
+  std::string body, why = "no register class tried";
   for (const Cls& cl : classes) {
     if (table && cl.budget < 128) continue;
+    if (!cl.agprs && peak + cl.slots > cl.hi - pool_lo + 1) continue;
     try {
       Allocator a(c, p, vbit, lane_shift, lm, pool_lo, cl.hi, cl.agprs, cl.slots, out_wires, out_neg);
       body = a.run();
@@ -545,6 +580,7 @@ NativeKernel build_native(const Circuit& c, NativeKind kind, const std::vector<u
   size_t eol = text.find('\n', pos);
   text.replace(pos, eol - pos, "; circuit body\n" + body);
   nk.asm_text = std::move(text);
+  nk.body = std::move(body);
   return nk;
 }
 

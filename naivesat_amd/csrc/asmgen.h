@@ -21,6 +21,7 @@ struct NativeUnfit : std::runtime_error {
 
 struct NativeKernel {
   std::string asm_text;     // complete assembly module
+  std::string body;         // the circuit body alone (what fills the template's slot)
   std::string kernel_name;  // symbol to launch
   uint32_t vgpr_class = 0;  // 64 / 128 / 256 / 512 (VGPRs + AGPR spill slots)
   uint32_t spills = 0;      // values copied to AGPRs per word

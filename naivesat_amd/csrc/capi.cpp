@@ -5,6 +5,7 @@
 #include <string>
 
 #include "../../include/naivesat_hip.h"
+#include "asmenc.h"
 #include "asmgen.h"
 #include "circuit.h"
 #include "runtime.h"
@@ -298,6 +299,53 @@ int ns_native_body(const ns_circuit* c, uint32_t kind, uint32_t lane_shift, char
     if (body && cap) {
       need(cap > text.size(), "body buffer too small");
       std::memcpy(body, text.c_str(), text.size() + 1);
+    }
+  });
+}
+
+int ns_native_encode(const char* body, size_t len, uint32_t* words, size_t cap, size_t* n_words) {
+  return guard([&] {
+    need(body != nullptr || len == 0, "bad argument");
+    std::vector<uint32_t> w;
+    std::string err;
+    if (!ns::encode_body(std::string(body ? body : "", len), w, &err)) throw ns::Error(NS_ERR_UNSUPPORTED, err);
+    if (n_words) *n_words = w.size();
+    if (words && cap) {
+      need(cap >= w.size(), "word buffer too small");
+      std::memcpy(words, w.data(), w.size() * 4);
+    }
+  });
+}
+
+int ns_native_code(const ns_circuit* c, uint32_t kind, uint32_t lane_shift, void* code, size_t cap,
+                   size_t* needed, uint32_t* patched, uint64_t* slot_offset) {
+  return guard([&] {
+    need(c && kind <= 3, "bad argument");
+    const ns::Circuit& cc = c->c;
+    std::vector<uint32_t> vb(cc.n), outs;
+    for (uint32_t k = 0; k < cc.n; k++) vb[k] = (kind == 1 || kind == 2) ? cc.n - 1 - k : k;
+    if (kind == 3)
+      for (uint32_t k = 0; k < cc.outputs.size(); k++) outs.push_back(k);
+    else
+      outs.push_back(0);
+    static const ns::NativeKind kinds[4] = {ns::NativeKind::SAT, ns::NativeKind::QSAT_A, ns::NativeKind::QSAT_B,
+                                            ns::NativeKind::TABLE};
+    ns::NativeKernel nk;
+    try {
+      nk = ns::build_native(cc, kinds[kind], vb, lane_shift, outs);
+    } catch (const ns::NativeUnfit& e) {
+      throw ns::Error(NS_ERR_UNSUPPORTED, e.what());
+    }
+    size_t off = 0;
+    std::vector<char> obj = ns::patch_native(nk.kernel_name, nk.body, &off);
+    const bool p = !obj.empty();
+    if (!p) obj = ns::assemble_native(nk.asm_text);
+    if (needed) *needed = obj.size();
+    if (patched) *patched = p ? 1u : 0u;
+    if (slot_offset) *slot_offset = p ? off : 0;
+    if (code && cap) {
+      need(cap >= obj.size(), "code buffer too small");
+      std::memcpy(code, obj.data(), obj.size());
     }
   });
 }

@@ -1,5 +1,7 @@
 #include "runtime.h"
 
+#include "asmenc.h"
+
 #include <cstdio>
 
 #include <amd_comgr/amd_comgr.h>
@@ -21,6 +23,11 @@ void check_hip(hipError_t e, const char* what) {
 static void check_rtc(hiprtcResult r, const char* what) {
   if (r != HIPRTC_SUCCESS) throw Error(ST_COMPILE, std::string(what) + ": " + hiprtcGetErrorString(r));
 }
+
+// Every generated kernel is unique and the engine keeps its own in-memory cache, so comgr's
+// on-disk compilation cache only adds file writes (and slows down as it fills). It is
+// turned off when the library loads unless the caller has set AMD_COMGR_CACHE.
+static const int kComgrCacheOff = setenv("AMD_COMGR_CACHE", "0", 0);
 
 // NS_TRACE_JIT=1 prints the assembly and module-load time of every native kernel
 static bool trace_jit() {
@@ -191,6 +198,48 @@ std::shared_ptr<JitModule> Jit::get_asm(const std::string& asm_text, const std::
                  std::chrono::duration<double, std::milli>(t1 - t0).count(),
                  std::chrono::duration<double, std::milli>(t2 - t1).count(), code.size());
   if (compile_ms) *compile_ms = m->compile_ms;
+  return insert(key, m);
+}
+
+bool Jit::find_built(const std::string& fp, BuiltKernel* out) {
+  std::lock_guard<std::mutex> lk(mu_);
+  auto it = built_.find(fp);
+  if (it == built_.end()) return false;
+  it->second.second = ++clock_;
+  *out = it->second.first;
+  return true;
+}
+
+void Jit::put_built(const std::string& fp, const BuiltKernel& k) {
+  std::lock_guard<std::mutex> lk(mu_);
+  if (built_.size() >= kCapacity && !built_.count(fp)) {
+    auto victim = built_.begin();
+    for (auto it = built_.begin(); it != built_.end(); ++it)
+      if (it->second.second < victim->second.second) victim = it;
+    built_.erase(victim);
+  }
+  built_[fp] = {k, ++clock_};
+}
+
+std::shared_ptr<JitModule> Jit::get_native(const std::string& kernel, const std::string& body,
+                                           const std::string& asm_text, double* compile_ms) {
+  std::string key = "nat:" + kernel + '\n' + body;
+  if (auto hit = lookup(key, compile_ms)) return hit;
+  auto t0 = std::chrono::steady_clock::now();
+  std::vector<char> code = patch_native(kernel, body);
+  const bool patched = !code.empty();
+  if (!patched) code = assemble_native(asm_text);
+  auto t1 = std::chrono::steady_clock::now();
+  auto m = std::make_shared<JitModule>();
+  NS_HIP(hipModuleLoadData(&m->mod, code.data()));
+  NS_HIP(hipModuleGetFunction(&m->fn, m->mod, kernel.c_str()));
+  auto t2 = std::chrono::steady_clock::now();
+  m->compile_ms = std::chrono::duration<double, std::milli>(t2 - t0).count();
+  if (compile_ms) *compile_ms = m->compile_ms;
+  if (trace_jit())
+    std::fprintf(stderr, "[ns jit] %s %s_ms=%.3f load_ms=%.3f bytes=%zu\n", kernel.c_str(),
+                 patched ? "patch" : "assemble", std::chrono::duration<double, std::milli>(t1 - t0).count(),
+                 std::chrono::duration<double, std::milli>(t2 - t1).count(), code.size());
   return insert(key, m);
 }
 

@@ -11,6 +11,8 @@
 #include <unordered_map>
 #include <vector>
 
+#include "codegen.h"
+
 namespace ns {
 
 struct Error : std::runtime_error {
@@ -40,6 +42,15 @@ struct JitModule {
   ~JitModule();
 };
 
+// A kernel built for one circuit and its code-generation parameters.
+struct BuiltKernel {
+  std::shared_ptr<JitModule> mod;
+  GenInfo info;
+  bool native = false;      // register-allocated assembly in a precompiled template
+  uint32_t vgpr_class = 0;  // template register class (native only)
+  uint32_t tuned_wpl = 0;   // words per lane chosen by timing (0: not tuned yet)
+};
+
 // Compiles generated sources once per context; keyed by the full source text. Holds at
 // most kCapacity modules: the least recently used is dropped (a solve still running it
 // keeps it alive through its reference).
@@ -49,6 +60,14 @@ class Jit {
   std::shared_ptr<JitModule> get(const std::string& src, const char* kernel, double* compile_ms);
   // gfx950 assembly module assembled and linked in-process (amd_comgr)
   std::shared_ptr<JitModule> get_asm(const std::string& asm_text, const std::string& kernel, double* compile_ms);
+  // native kernel: the body's machine code copied into the template's cached code object;
+  // bodies outside the encoded subset or too large for the slot are assembled (asm_text)
+  std::shared_ptr<JitModule> get_native(const std::string& kernel, const std::string& body,
+                                        const std::string& asm_text, double* compile_ms);
+  // kernels by fingerprint (circuit content, kernel kind, parameters): a repeated solve of
+  // the same circuit skips code generation as well as compilation
+  bool find_built(const std::string& fp, BuiltKernel* out);
+  void put_built(const std::string& fp, const BuiltKernel& k);
   ~Jit();
 
  private:
@@ -58,6 +77,7 @@ class Jit {
   std::mutex mu_;
   uint64_t clock_ = 0;
   std::unordered_map<std::string, std::shared_ptr<JitModule>> cache_;
+  std::unordered_map<std::string, std::pair<BuiltKernel, uint64_t>> built_;  // value, last use
 };
 
 // Assembles and links a gfx950 assembly module in-process; returns the code object.

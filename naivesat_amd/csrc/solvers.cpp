@@ -4,6 +4,8 @@
 #include "solvers.h"
 
 #include <chrono>
+#include <cstdio>
+#include <functional>
 #include <cstdlib>
 #include <cstring>
 
@@ -88,12 +90,65 @@ bool native_wanted(const Geometry& g) {
 
 uint32_t word_valid_mask(uint32_t n) { return n >= 5 ? 0xffffffffu : (uint32_t)((1ull << (1u << n)) - 1); }
 
-struct Chosen {
-  std::shared_ptr<JitModule> mod;
-  GenInfo info;
-  bool native = false;
-  uint32_t vgpr_class = 0;
-};
+using Chosen = BuiltKernel;
+
+// 128-bit fingerprint of a circuit's content (two independent 64-bit hashes)
+std::string circuit_fingerprint(const Circuit& c) {
+  uint64_t h1 = 0xcbf29ce484222325ull, h2 = 0x9e3779b97f4a7c15ull;
+  auto mix = [&](uint64_t v) {
+    h1 = (h1 ^ v) * 0x100000001b3ull;
+    h2 ^= v + 0x9e3779b97f4a7c15ull + (h2 << 6) + (h2 >> 2);
+    h2 *= 0xff51afd7ed558ccdull;
+  };
+  mix(c.n);
+  mix(c.gates.size());
+  for (const Gate& g : c.gates) mix(((uint64_t)g.i0 << 34) | ((uint64_t)g.i1 << 2) | g.func);
+  mix(c.outputs.size());
+  for (const Output& o : c.outputs) mix(((uint64_t)o.wire << 1) | (o.neg ? 1u : 0u));
+  char buf[40];
+  std::snprintf(buf, sizeof(buf), "%016llx%016llx", (unsigned long long)h1, (unsigned long long)h2);
+  return buf;
+}
+
+// Kernel for a circuit: register-allocated assembly in a precompiled template when the body
+// fits one, else generated HIP compiled by hiprtc. Cached per context under the circuit's
+// fingerprint and every parameter the code depends on; *compile_ms counts code generation
+// and compilation (0 on a cache hit).
+Chosen choose_kernel(Lane& ctx, const Circuit& c, NativeKind kind, const std::vector<uint32_t>& vb,
+                     uint32_t lane_shift, const std::vector<uint32_t>& outs, const Geometry& g,
+                     const std::string& extra_key, const std::function<std::string(GenInfo*)>& hiprtc_src,
+                     const char* hiprtc_kernel, double* compile_ms, std::string* fp_out = nullptr) {
+  const bool native_ok = native_wanted(g);
+  std::string fp = circuit_fingerprint(c) + "/" + std::to_string((int)kind) + "/" + std::to_string(lane_shift) +
+                   "/" + std::to_string(g.block) + "/" + std::to_string(g.wpl) + "/" + std::to_string(g.j_shift) +
+                   (native_ok ? "/n/" : "/r/") + extra_key;
+  for (uint32_t k : outs) fp += "," + std::to_string(k);
+  if (fp_out) *fp_out = fp;
+  Chosen ch;
+  if (ctx.jit.find_built(fp, &ch)) {
+    if (compile_ms) *compile_ms = 0.0;
+    return ch;
+  }
+  auto t0 = clk::now();
+  double cms = 0.0;
+  if (native_ok) {
+    try {
+      NativeKernel nk = build_native(c, kind, vb, lane_shift, outs);
+      ch.mod = ctx.jit.get_native(nk.kernel_name, nk.body, nk.asm_text, &cms);
+      ch.info = nk.info;
+      ch.native = true;
+      ch.vgpr_class = nk.vgpr_class;
+    } catch (const NativeUnfit&) {
+    }
+  }
+  if (!ch.native) {
+    std::string src = hiprtc_src(&ch.info);
+    ch.mod = ctx.jit.get(src, hiprtc_kernel, &cms);
+  }
+  if (compile_ms) *compile_ms = ms_since(t0);
+  ctx.jit.put_built(fp, ch);
+  return ch;
+}
 
 }  // namespace
 
@@ -108,23 +163,20 @@ void sat_solve(Lane& ctx, const Circuit& c, const ns_sat_opts& o, ns_sat_result&
   NS_HIP(hipSetDevice(ctx.device));
   const uint64_t total_words = c.n >= 5 ? (c.n == 64 ? (1ull << 59) : (1ull << (c.n - 5))) : 1ull;
   Geometry g = pick_geometry(total_words, o.group_len, o.words_per_lane, false);
-  Chosen ch;
-  if (native_wanted(g)) {
-    try {
-      std::vector<uint32_t> vb(c.n);
-      for (uint32_t k = 0; k < c.n; k++) vb[k] = k;
-      NativeKernel nk = build_native(c, NativeKind::SAT, vb, 0, {0});
-      ch.mod = ctx.jit.get_asm(nk.asm_text, nk.kernel_name, &r.compile_ms);
-      ch.info = nk.info;
-      ch.native = true;
-      ch.vgpr_class = nk.vgpr_class;
-    } catch (const NativeUnfit&) {
-    }
-  }
-  if (!ch.native) {
-    std::string src = gen_sat_kernel(c, g, &ch.info);
-    ch.mod = ctx.jit.get(src, "ns_sat", &r.compile_ms);
-  }
+  std::vector<uint32_t> vb(c.n);
+  for (uint32_t k = 0; k < c.n; k++) vb[k] = k;
+  std::string fp;
+  Chosen ch = choose_kernel(
+      ctx, c, NativeKind::SAT, vb, 0, {0}, g, "",
+      [&](GenInfo* info) { return gen_sat_kernel(c, g, info); }, "ns_sat", &r.compile_ms, &fp);
+  // Words per lane is a run-time argument of the native kernel: a long search times its
+  // first launches with 1..16 words per lane and keeps the fastest (cached with the kernel).
+  // Every word is still evaluated exactly once, so the answers do not depend on it.
+  const uint64_t cw_tune = chunk_words(o.elem_bits, g, total_words);
+  const bool tune = ch.native && o.words_per_lane == 0 && (o.flags & NS_SAT_STOP_AT_FIRST) == 0 &&
+                    total_words >= 16 * cw_tune && cw_tune % ((uint64_t)g.block * 16) == 0;
+  if (ch.native && ch.tuned_wpl && o.words_per_lane == 0 && cw_tune % ((uint64_t)g.block * ch.tuned_wpl) == 0)
+    g.wpl = ch.tuned_wpl;
   auto mod = ch.mod;
   r.cone_gates = ch.info.cone_gates;
   r.valu_ops = ch.info.valu_ops;
@@ -142,6 +194,36 @@ void sat_solve(Lane& ctx, const Circuit& c, const ns_sat_opts& o, ns_sat_result&
   uint64_t w0 = 0;
   r.complete = 1;
   NS_HIP(hipEventRecord(ctx.ev0, ctx.stream));
+  if (tune && !ch.tuned_wpl) {
+    static const uint32_t cand[5] = {1, 2, 4, 8, 16};
+    hipEvent_t ev[6];
+    for (auto& e : ev) NS_HIP(hipEventCreate(&e));
+    NS_HIP(hipEventRecord(ev[0], ctx.stream));
+    for (int k = 0; k < 5; k++) {
+      Geometry gk = g;
+      gk.wpl = cand[k];
+      uint32_t wk = cand[k];
+      void* extra[3] = {&res, &wk, &valid};
+      launch_range(ctx, mod->fn, gk, w0, cw, extra, 3, total_words);
+      NS_HIP(hipEventRecord(ev[k + 1], ctx.stream));
+      r.launches++;
+      w0 += cw;
+    }
+    NS_HIP(hipEventSynchronize(ev[5]));
+    float best = 0.f;
+    for (int k = 0; k < 5; k++) {
+      float ms = 0.f;
+      NS_HIP(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
+      if (k == 0 || ms < best) {
+        best = ms;
+        g.wpl = cand[k];
+      }
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    wpl = g.wpl;
+    ch.tuned_wpl = g.wpl;
+    ctx.jit.put_built(fp, ch);
+  }
   while (w0 < total_words) {
     uint64_t words = total_words - w0 < cw ? total_words - w0 : cw;
     void* extra[3] = {&res, &wpl, &valid};
@@ -195,23 +277,13 @@ void qsat_solve(Lane& ctx, const Circuit& c, const std::vector<uint8_t>& q, cons
   Geometry g = pick_geometry(total_words, o.group_len, o.words_per_lane, true);
   const uint32_t jb = ilog2(g.wpl);
   const bool case_a = L >= 5 + jb;
-  Chosen ch;
-  if (native_wanted(g)) {
-    try {
-      std::vector<uint32_t> vb(n);
-      for (uint32_t k = 0; k < n; k++) vb[k] = n - 1 - k;
-      NativeKernel nk = build_native(c, case_a ? NativeKind::QSAT_A : NativeKind::QSAT_B, vb, jb, {0});
-      ch.mod = ctx.jit.get_asm(nk.asm_text, nk.kernel_name, &r.compile_ms);
-      ch.info = nk.info;
-      ch.native = true;
-      ch.vgpr_class = nk.vgpr_class;
-    } catch (const NativeUnfit&) {
-    }
-  }
-  if (!ch.native) {
-    std::string src = gen_qsat_kernel(c, q, g, &ch.info);
-    ch.mod = ctx.jit.get(src, "ns_qsat", &r.compile_ms);
-  }
+  std::vector<uint32_t> vb(n);
+  for (uint32_t k = 0; k < n; k++) vb[k] = n - 1 - k;
+  std::string qkey(q.size(), '0');  // the generated HIP source depends on the prefix
+  for (size_t k = 0; k < q.size(); k++) qkey[k] = q[k] ? '1' : '0';
+  Chosen ch = choose_kernel(
+      ctx, c, case_a ? NativeKind::QSAT_A : NativeKind::QSAT_B, vb, jb, {0}, g, qkey,
+      [&](GenInfo* info) { return gen_qsat_kernel(c, q, g, info); }, "ns_qsat", &r.compile_ms);
   auto mod = ch.mod;
   r.cone_gates = ch.info.cone_gates;
   r.valu_ops = ch.info.valu_ops;
@@ -317,24 +389,12 @@ double build_table_dev(Lane& ctx, const Circuit& c, uint32_t group_len, uint32_t
   const uint64_t states = 1ull << n;
   const uint64_t total_words = n >= 5 ? (1ull << (n - 5)) : 1ull;
   Geometry g = pick_geometry(total_words, group_len, wpl ? wpl : 2, false);
-  Chosen ch;
-  if (native_wanted(g)) {
-    try {
-      std::vector<uint32_t> vb(n), outs;
-      for (uint32_t k = 0; k < n; k++) vb[k] = k;
-      for (uint32_t k = 0; k < c.outputs.size(); k++) outs.push_back(k);
-      NativeKernel nk = build_native(c, NativeKind::TABLE, vb, 0, outs);
-      ch.mod = ctx.jit.get_asm(nk.asm_text, nk.kernel_name, compile_ms);
-      ch.info = nk.info;
-      ch.native = true;
-      ch.vgpr_class = nk.vgpr_class;
-    } catch (const NativeUnfit&) {
-    }
-  }
-  if (!ch.native) {
-    std::string src = gen_table_kernel(c, g, &ch.info);
-    ch.mod = ctx.jit.get(src, "ns_table", compile_ms);
-  }
+  std::vector<uint32_t> vb(n), outs;
+  for (uint32_t k = 0; k < n; k++) vb[k] = k;
+  for (uint32_t k = 0; k < c.outputs.size(); k++) outs.push_back(k);
+  Chosen ch = choose_kernel(
+      ctx, c, NativeKind::TABLE, vb, 0, outs, g, "",
+      [&](GenInfo* info) { return gen_table_kernel(c, g, info); }, "ns_table", compile_ms);
   auto mod = ch.mod;
   if (cone) *cone = ch.info.cone_gates;
   if (info_out) {

@@ -131,12 +131,65 @@ __global__ void __launch_bounds__(256) k_salu_dep(u32* out, int iters, u64* cyc)
   if (threadIdx.x == 0) cyc[blockIdx.x] = t1 - t0;
 }
 
+// 256 VALU + 64 LDS spill/reload pairs per iteration: per group of 4 VALU ops a value is
+// written to LDS and read back; the read is consumed after 3 VALU ops (short) or the
+// reads of 4 groups are issued 16 VALU ops ahead of their uses (long)
+__global__ void __launch_bounds__(256) k_lds_short(u32* out, int iters, u64* cyc) {
+  __shared__ u32 slots[4 * 256];
+  u32 a0 = threadIdx.x, a1 = a0 + 1, a2 = a0 + 2, a3 = a0 + 3, b = a0 * 3, c = a0 * 7, t = 0;
+  const u32 addr = (u32)(size_t)&slots[threadIdx.x];
+  for (int k = 0; k < 4; k++) slots[threadIdx.x + 256 * k] = 0u;  // keeps the allocation
+  __syncthreads();
+  u64 t0 = clock64();
+  for (int i = 0; i < iters; i++) {
+    asm volatile(R8(R8("ds_write_b32 %7, %0\nds_read_b32 %6, %7 offset:1024\n"
+                       "v_bitop3_b32 %1, %1, %4, %5 bitop3:0x96\nv_bitop3_b32 %2, %2, %4, %5 bitop3:0x96\n"
+                       "v_bitop3_b32 %3, %3, %4, %5 bitop3:0x96\ns_waitcnt lgkmcnt(0)\n"
+                       "v_bitop3_b32 %0, %0, %6, %5 bitop3:0x96\n"))
+                 : "+v"(a0), "+v"(a1), "+v"(a2), "+v"(a3), "+v"(b), "+v"(c), "=&v"(t) : "v"(addr) : "memory");
+  }
+  u64 t1 = clock64();
+  __syncthreads();
+  out[blockIdx.x * 256 + threadIdx.x] = a0 ^ a1 ^ a2 ^ a3 ^ t ^ slots[(threadIdx.x + 1) & 255];
+  if (threadIdx.x == 0) cyc[blockIdx.x] = t1 - t0;
+}
+
+__global__ void __launch_bounds__(256) k_lds_long(u32* out, int iters, u64* cyc) {
+  __shared__ u32 slots[4 * 256];
+  u32 a0 = threadIdx.x, a1 = a0 + 1, a2 = a0 + 2, a3 = a0 + 3, b = a0 * 3, c = a0 * 7;
+  u32 t0r = 0, t1r = 0, t2r = 0, t3r = 0;
+  const u32 addr = (u32)(size_t)&slots[threadIdx.x];
+  for (int k = 0; k < 4; k++) slots[threadIdx.x + 256 * k] = 0u;  // keeps the allocation
+  __syncthreads();
+#define ROT4 "v_bitop3_b32 %1, %1, %4, %5 bitop3:0x96\nv_bitop3_b32 %2, %2, %4, %5 bitop3:0x96\n" \
+             "v_bitop3_b32 %3, %3, %4, %5 bitop3:0x96\nv_bitop3_b32 %0, %0, %4, %5 bitop3:0x96\n"
+  u64 t0 = clock64();
+  for (int i = 0; i < iters; i++) {
+    asm volatile(R8(R8("ds_write_b32 %10, %0\nds_write_b32 %10, %1 offset:1024\n"
+                       "ds_write_b32 %10, %2 offset:2048\nds_write_b32 %10, %3 offset:3072\n"
+                       "ds_read_b32 %6, %10\nds_read_b32 %7, %10 offset:1024\n"
+                       "ds_read_b32 %8, %10 offset:2048\nds_read_b32 %9, %10 offset:3072\n"
+                       ROT4 ROT4 ROT4 "s_waitcnt lgkmcnt(0)\n"
+                       "v_bitop3_b32 %0, %0, %6, %5 bitop3:0x96\nv_bitop3_b32 %1, %1, %7, %5 bitop3:0x96\n"
+                       "v_bitop3_b32 %2, %2, %8, %5 bitop3:0x96\nv_bitop3_b32 %3, %3, %9, %5 bitop3:0x96\n"))
+                 : "+v"(a0), "+v"(a1), "+v"(a2), "+v"(a3), "+v"(b), "+v"(c), "=&v"(t0r), "=&v"(t1r), "=&v"(t2r),
+                   "=&v"(t3r)
+                 : "v"(addr) : "memory");
+  }
+#undef ROT4
+  u64 t1 = clock64();
+  __syncthreads();
+  out[blockIdx.x * 256 + threadIdx.x] = a0 ^ a1 ^ a2 ^ a3 ^ t0r ^ t1r ^ t2r ^ t3r ^ slots[(threadIdx.x + 1) & 255];
+  if (threadIdx.x == 0) cyc[blockIdx.x] = t1 - t0;
+}
+
 typedef void (*kfn)(u32*, int, u64*);
 int main() {
   struct K { const char* name; kfn f; int valu; int other; } ks[] = {
       {"dep64", k_dep, 64, 0},         {"rot256", k_rot, 256, 0},       {"salu256+64", k_salu, 256, 64},
       {"acc256+128", k_acc, 256, 128}, {"rl256+64+nop", k_rl, 256, 64}, {"rlearly256+64", k_rl_early, 256, 64},
-      {"wl256+64", k_wl, 256, 64},     {"saludep256+64", k_salu_dep, 256, 64}};
+      {"wl256+64", k_wl, 256, 64},     {"saludep256+64", k_salu_dep, 256, 64},
+      {"ldsshort256+128", k_lds_short, 256, 128}, {"ldslong256+512", k_lds_long, 256 + 256, 512}};
   const int maxb = 256 * 8;
   u32* out;
   u64* cyc;

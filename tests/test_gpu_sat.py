@@ -111,3 +111,14 @@ def test_config1_full_size_golden(ctx):
     assert (r["count"], r["lowest"]) == (g["count"], g["lowest"])
     # the planted key satisfies, so it bounds the lowest answer
     assert r["lowest"] <= key
+
+
+def test_words_per_lane_tuning_keeps_answers(ctx):
+    # 64 launches of 2^15 words: the first solve times words-per-lane 1..16 on its first
+    # launches, later solves reuse the choice; every setting must give the same answers
+    c, key = ns.Circuit.gen_sat(21, 26, 800, 26)
+    count, lowest = oracle.OracleCircuit.from_dict(c.arrays()).sat(threads=8)
+    for kw in ({}, {}, {"words_per_lane": 2}, {"words_per_lane": 16}):
+        r = ctx.sat_solve(c, elem_bits=20, **kw)
+        assert (r["count"], r["lowest"]) == (count, lowest), kw
+        assert r["launches"] == 64
