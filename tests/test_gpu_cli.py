@@ -1,0 +1,98 @@
+"""The four command-line tools (the process boundary the reference's users script against)
+print the reference's lines with the oracle's answers."""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import naivesat_amd as ns
+import oracle
+from helpers import point_stop_circuit
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "configs.json")
+
+
+def run(tool, *args):
+    r = subprocess.run([os.path.join(ns.BIN_DIR, tool), *map(str, args)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    return r.stdout.splitlines()
+
+
+def mk(d):
+    return ns.Circuit.from_arrays(d["n"], d["in0"], d["in1"], d["func"], d["out_wire"], d["out_neg"])
+
+
+def test_naivesat_config0(tmp_path):
+    c, key = ns.Circuit.gen_sat(1, 24, 2000, 32)
+    f = tmp_path / "cfg0.circuit"
+    f.write_text(c.text())
+    with open(GOLDEN) as g:
+        lowest = json.load(g)["cfg0"]["lowest"]
+    out = run("naivesat", "-t", "opencl:0", f)
+    assert f"Found Input: {lowest:024b}" in out
+    assert "Start execution" in out and any(x.startswith("Time: ") for x in out)
+
+
+def test_naivesat_small_and_unsatisfiable(tmp_path):
+    f = tmp_path / "small.circuit"
+    # x0 & ~x1 & x2: lowest satisfying assignment is 0b101
+    f.write_text(ns.Circuit.from_arrays(3, [0, 3], [1, 2], [2, 0], [4], [0]).text())
+    assert "Found Input: 101" in run("naivesat", "-t", "opencl:0", f)
+    u = tmp_path / "unsat.circuit"
+    u.write_text(ns.Circuit.from_arrays(2, [0, 0, 2], [1, 1, 3], [0, 1, 0], [4], [0]).text())
+    assert "Unsatisfiable!" in run("naivesat", "-t", "opencl:0", u)
+
+
+def test_naiveqsat_prefix_and_plain_circuit(tmp_path):
+    c = ns.Circuit.gen_qsat(12, 20, 400, 60)
+    q = [0] * 5 + [1] * 5 + [0] * 5 + [1] * 5
+    f = tmp_path / "q.circuit"
+    f.write_text("".join("ea"[x] for x in q) + " " + c.text())
+    o = oracle.OracleCircuit.from_dict(c.arrays()).qsat(q)
+    want = ns.qsat_result_string({"first_bits": o["first_bits"], "first_all": o["first_all"],
+                                  "has_witness": int(o["witness"] is not None), "witness": o["witness"] or 0})
+    assert f"Result: {want}" in run("naiveqsat", "-t", "opencl:0", f)
+    # a circuit without a prefix is read with every input existential
+    p = tmp_path / "plain.circuit"
+    p.write_text(c.text())
+    out = run("naiveqsat", "-t", "opencl:0", p)
+    assert "Load as circuit with quantifiers" in out
+    count, lowest = oracle.OracleCircuit.from_dict(c.arrays()).sat(threads=8)
+    assert (count > 0) == any(x.startswith("Result: (Satisfiable: ") for x in out)
+
+
+def test_satostop_tools(tmp_path):
+    # the reference's mem-image case 2 (src/greedysatostop.rs:1203-1268) as a full table
+    n, U, p, nxt = 24, 11, (3 << 20) + 79 * 8192, 58591
+    f = tmp_path / "stop.circuit"
+    f.write_text(mk(point_stop_circuit(n, p, nxt)).text())
+    out = run("naivesatostop", "-t", "opencl:0", f, U)
+    assert f"Solution: Solution {{ start: {p}, end: {nxt}, steps: 1 }}" in out
+    out = run("naivesatostop", "-t", "cpu", "-S", f, U)
+    assert f"Stopped at {p >> (n - U):0{U}d} {nxt:0{n}d}" in out
+    out = run("greedysatostop", f, U)
+    assert f"Solution: Solution {{ start: {p}, end: {nxt} }}" in out
+    # a start state that never stops
+    g = tmp_path / "nostop.circuit"
+    g.write_text(mk(point_stop_circuit(n, p + 1, nxt)).text())
+    assert "Unsatisfiable!" in run("naivesatostop", "-t", "opencl:0", g, U)
+    assert "Unsatisfiable!" in run("greedysatostop", g, U)
+
+
+def test_naivesatostop_verify_walk(tmp_path):
+    c, target = ns.Circuit.gen_satostop(6, 8, 6, 40, 8)
+    f = tmp_path / "f.circuit"
+    f.write_text(c.text())
+    out = run("naivesatostop", "-t", "opencl:0", "-v", f, 8)
+    r = oracle.OracleCircuit.from_dict(c.arrays()).satostop(8)
+    if r["n_stopped"]:
+        lo = r["lowest_u"]
+        assert (f"Solution: Solution {{ start: {lo << 8}, end: {r['ends'][lo]}, steps: {r['steps'][lo]} }}"
+                in out)
+        assert f"Verified: end={r['ends'][lo]} steps={r['steps'][lo]}" in out and "INCORRECT" not in out
+    else:
+        assert "Unsatisfiable!" in out
