@@ -118,7 +118,9 @@ def table_part(ns, ctx):
     avg = sum(jump) / len(jump)
     gbs = bytes_pass / (avg / 1000.0) / 1e9
     return {"workload": TABLE_WORKLOAD, "build_ms": rs["build_ms"], "jump_ms": rs["jump_ms"],
-            "passes": rs["passes"], "n_stopped": rs["n_stopped"], "lowest_u": rs["lowest_u"],
+            "passes": rs["passes"], "walk_jumps": rs["walk_jumps"], "walk_ms": rs["walk_ms"],
+            "device_ms": rs["build_ms"] + rs["jump_ms"] + rs["walk_ms"], "wall_ms": rs["wall_ms"],
+            "n_stopped": rs["n_stopped"], "lowest_u": rs["lowest_u"],
             "steps": rs["steps"], "end": rs["end"], "avg_pass_ms": avg,
             "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": gbs / HBM_PEAK_GBS,

@@ -162,7 +162,7 @@ typedef struct {
 } ns_satostop_opts;
 typedef struct {
   uint32_t any_stop;   /* some state's STOP output is 1 */
-  uint32_t passes;     /* jump passes run */
+  uint32_t passes;     /* jump passes run over the whole table (P) */
   uint64_t n_stopped;  /* unknown starts that stop */
   uint64_t lowest_u;   /* lowest stopping unknown (valid when n_stopped > 0) */
   uint64_t start;      /* its start state */
@@ -176,6 +176,8 @@ typedef struct {
   double jump_ms;      /* total jump-pass kernel time */
   double compile_ms;
   double wall_ms;
+  uint64_t walk_jumps; /* per start, up to this many jumps of 2^P evaluations follow the passes */
+  double walk_ms;      /* kernel time of those walks */
 } ns_satostop_result;
 /* steps_out / ends_out (optional): 2^unknowns entries each; steps 0 = never stops. */
 int ns_satostop_solve(ns_ctx* ctx, const ns_circuit* c, uint32_t unknowns,

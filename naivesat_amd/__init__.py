@@ -69,7 +69,7 @@ class _StopResult(C.Structure):
                 ("steps", C.c_uint64), ("cone_gates", C.c_uint32), ("valu_ops", C.c_uint32),
                 ("backend", C.c_uint32), ("vgpr_class", C.c_uint32),
                 ("build_ms", C.c_double), ("jump_ms", C.c_double), ("compile_ms", C.c_double),
-                ("wall_ms", C.c_double)]
+                ("wall_ms", C.c_double), ("walk_jumps", C.c_uint64), ("walk_ms", C.c_double)]
 
 
 class _Request(C.Structure):

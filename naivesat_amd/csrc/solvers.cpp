@@ -3,7 +3,9 @@
 // with results accumulated on the device and read back once).
 #include "solvers.h"
 
+#include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <functional>
 #include <cstdlib>
@@ -439,6 +441,33 @@ double table_build(Lane& ctx, const Circuit& c, uint32_t* table_out, uint64_t ta
   return kms;
 }
 
+// Passes over the whole table, then walks from the starts. After P passes every entry that
+// has not stopped jumps 2^P evaluations, so 2^(C-P) jumps per start cover the same 2^C
+// evaluations as C passes, with the same answers. A pass costs 2^n random gathers
+// (throughput-bound, ~27 ps each); a walk costs 2^(C-P) dependent gathers per start
+// (latency-bound, ~0.7 us each, while the starts are few). P minimises the modelled time;
+// NS_JUMP_PASSES overrides it.
+uint32_t choose_passes(uint32_t n, uint32_t unknowns, uint32_t cover) {
+  if (cover == 0) return 0;
+  if (const char* e = std::getenv("NS_JUMP_PASSES")) {
+    long v = std::atol(e);
+    if (v > 0) return v > (long)cover ? cover : (uint32_t)v;
+  }
+  const double per_gather = 27e-12, per_jump = 0.7e-6;
+  uint32_t best = cover;
+  double best_t = 0;
+  for (uint32_t p = 1; p <= cover; p++) {
+    const double jumps = std::ldexp(1.0, (int)(cover - p));
+    const double t = p * std::ldexp(per_gather, (int)n) +
+                     std::max(jumps * per_jump, std::ldexp(jumps * per_gather, (int)unknowns));
+    if (p == 1 || t < best_t) {
+      best = p;
+      best_t = t;
+    }
+  }
+  return best;
+}
+
 void satostop_solve(Lane& ctx, const Circuit& c, uint32_t unknowns, const ns_satostop_opts& o,
                     ns_satostop_result& r, uint32_t* steps_out, uint32_t* ends_out) {
   auto t0 = clk::now();
@@ -462,9 +491,12 @@ void satostop_solve(Lane& ctx, const Circuit& c, uint32_t unknowns, const ns_sat
   unsigned int* d_steps = ctx.tab_d.as<unsigned int>();
   unsigned int* d_ends = d_steps + starts;
   unsigned long long* stats = ctx.stats.as<unsigned long long>();
-  uint32_t max_passes = o.max_passes ? o.max_passes : n;
-  if (max_passes > n) max_passes = n;
+  // C = log2 of the evaluations every walk covers
+  uint32_t cover = o.max_passes ? o.max_passes : n;
+  if (cover > n) cover = n;
   const bool first_pass_mode = (o.flags & NS_STOP_FIRST_PASS) != 0;
+  const uint32_t max_passes = first_pass_mode ? cover : choose_passes(n, unknowns, cover);
+  const uint64_t walk_jumps = 1ull << (cover - max_passes);
   unsigned long long host[2] = {0, ~0ull};
   const unsigned long long* final_tab = nullptr;
   if (any) {
@@ -489,7 +521,7 @@ void satostop_solve(Lane& ctx, const Circuit& c, uint32_t unknowns, const ns_sat
       if (first_pass_mode) {
         NS_HIP(launch_fill_u64(ctx.stream, stats, 0ull, 1));
         NS_HIP(launch_fill_u64(ctx.stream, stats + 1, ~0ull, 1));
-        NS_HIP(launch_starts(ctx.stream, cur, n, unknowns, nullptr, nullptr, stats));
+        NS_HIP(launch_starts(ctx.stream, cur, n, unknowns, 1, nullptr, nullptr, stats));
         NS_HIP(hipMemcpyAsync(host, stats, sizeof(host), hipMemcpyDeviceToHost, ctx.stream));
         NS_HIP(hipStreamSynchronize(ctx.stream));
         if (host[0] != 0) {
@@ -508,10 +540,23 @@ void satostop_solve(Lane& ctx, const Circuit& c, uint32_t unknowns, const ns_sat
     }
     for (auto& e : evs) (void)hipEventDestroy(e);
     final_tab = cur;
+    const uint64_t jumps = p == max_passes ? walk_jumps : 1;  // first-pass mode may end early
+    r.walk_jumps = jumps;
     NS_HIP(launch_fill_u64(ctx.stream, stats, 0ull, 1));
     NS_HIP(launch_fill_u64(ctx.stream, stats + 1, ~0ull, 1));
-    NS_HIP(launch_starts(ctx.stream, final_tab, n, unknowns, d_steps, d_ends, stats));
+    hipEvent_t w0, w1;
+    NS_HIP(hipEventCreate(&w0));
+    NS_HIP(hipEventCreate(&w1));
+    NS_HIP(hipEventRecord(w0, ctx.stream));
+    NS_HIP(launch_starts(ctx.stream, final_tab, n, unknowns, jumps, d_steps, d_ends, stats));
+    NS_HIP(hipEventRecord(w1, ctx.stream));
     NS_HIP(hipMemcpyAsync(host, stats, sizeof(host), hipMemcpyDeviceToHost, ctx.stream));
+    NS_HIP(hipStreamSynchronize(ctx.stream));
+    float wms = 0.f;
+    NS_HIP(hipEventElapsedTime(&wms, w0, w1));
+    r.walk_ms = wms;
+    (void)hipEventDestroy(w0);
+    (void)hipEventDestroy(w1);
   } else {
     NS_HIP(launch_fill_u32(ctx.stream, d_steps, 0u, 2 * starts));
     host[0] = 0;

@@ -139,21 +139,37 @@ __global__ void __launch_bounds__(256) k_jump_pass(const unsigned long long* __r
   *reinterpret_cast<ulonglong2*>(dst + base) = make_ulonglong2(e[0], e[1]);
 }
 
-// Per unknown start u (state u << (n - unknowns)): steps_out[u] = evaluations until
-// STOP (0 when it has not stopped), ends_out[u] = state produced by the stopping
-// evaluation. stats[0] += stopped starts, stats[1] = min(lowest stopped u).
+// Per unknown start u (state u << (n - unknowns)), with tab = the table after P passes
+// (a non-stopped entry covers 2^P evaluations; a stopped one holds the evaluations until
+// STOP and the end state): follows up to `jumps` entries from the start, so the walk covers
+// jumps x 2^P evaluations. steps_out[u] = evaluations until STOP (0 when it does not stop
+// within them), ends_out[u] = state produced by the stopping evaluation; stats[0] +=
+// stopped starts, stats[1] = min(lowest stopped u).
 __global__ void __launch_bounds__(256) k_starts(const unsigned long long* __restrict__ tab,
                                                 unsigned int n, unsigned int unknowns,
-                                                unsigned int* steps_out, unsigned int* ends_out,
-                                                unsigned long long* stats) {
+                                                unsigned long long jumps, unsigned int* steps_out,
+                                                unsigned int* ends_out, unsigned long long* stats) {
   const unsigned long long u = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
   unsigned long long cnt = 0, best = ~0ull;
   if (u < (1ull << unknowns)) {
-    const unsigned long long e = tab[u << (n - unknowns)];
-    const unsigned int lo = (unsigned int)e;
-    const bool stop = (lo >> n) & 1u;
-    if (steps_out) steps_out[u] = stop ? (unsigned int)(e >> 32) : 0u;
-    if (ends_out) ends_out[u] = stop ? (lo & ((1u << n) - 1u)) : 0u;
+    const unsigned int mask = (1u << n) - 1u;
+    unsigned int x = (unsigned int)(u << (n - unknowns));
+    unsigned long long total = 0;
+    bool stop = false;
+    unsigned int end = 0;
+    for (unsigned long long j = 0; j < jumps; j++) {
+      const unsigned long long e = tab[x];
+      const unsigned int lo = (unsigned int)e;
+      total += e >> 32;
+      if ((lo >> n) & 1u) {
+        stop = true;
+        end = lo & mask;
+        break;
+      }
+      x = lo & mask;
+    }
+    if (steps_out) steps_out[u] = stop ? (unsigned int)total : 0u;
+    if (ends_out) ends_out[u] = stop ? end : 0u;
     if (stop) {
       cnt = 1;
       best = u;
@@ -202,11 +218,11 @@ hipError_t launch_jump_pass(hipStream_t s, const unsigned long long* src, unsign
 }
 
 hipError_t launch_starts(hipStream_t s, const unsigned long long* tab, unsigned int n,
-                         unsigned int unknowns, unsigned int* steps_out, unsigned int* ends_out,
-                         unsigned long long* stats) {
+                         unsigned int unknowns, unsigned long long jumps, unsigned int* steps_out,
+                         unsigned int* ends_out, unsigned long long* stats) {
   unsigned long long cnt = 1ull << unknowns;
   unsigned long long threads = (cnt + 63ull) & ~63ull;
-  hipLaunchKernelGGL(k_starts, dim3(blocks_for(threads, 256)), dim3(256), 0, s, tab, n, unknowns,
+  hipLaunchKernelGGL(k_starts, dim3(blocks_for(threads, 256)), dim3(256), 0, s, tab, n, unknowns, jumps,
                      steps_out, ends_out, stats);
   return hipGetLastError();
 }

@@ -110,7 +110,8 @@ def test_config3_full_size_golden(ctx):
     c, target = ns.Circuit.gen_satostop(4, 15, 6, 300, 14)
     assert target == g["target"]
     r = ctx.satostop_solve(c, 14)
-    assert r["passes"] == 30
+    # P passes over the table plus walks of 2^(30-P) jumps cover 2^30 evaluations
+    assert r["walk_jumps"] << r["passes"] == 1 << 30
     assert r["n_stopped"] == g["n_stopped"]
     assert hashlib.sha256(r["steps_arr"].tobytes()).hexdigest() == g["steps_sha256"]
     assert hashlib.sha256(r["ends_arr"].tobytes()).hexdigest() == g["ends_sha256"]
@@ -138,3 +139,30 @@ def test_mem_image_find_solution_vectors(ctx):
         assert (g["n_stopped"] > 0) == (exp is not None), case
         if exp is not None:
             assert (g["start"], g["end"], g["passes"]) == (exp["start"], exp["end"], 1), case
+
+
+@pytest.mark.parametrize("passes", [1, 2, 5, 9])
+def test_walks_after_fewer_passes(ctx, monkeypatch, passes):
+    # P passes over the table and 2^(n-P) jumps per start give the n-pass answers
+    monkeypatch.setenv("NS_JUMP_PASSES", str(passes))
+    for half, U in ((6, 10), (8, 9)):
+        c, _ = ns.Circuit.gen_satostop(120 + half, half, 6, 10 * half, U)
+        r, o = check(ctx, c.arrays(), U)
+        assert r["passes"] == passes and r["walk_jumps"] == 1 << (2 * half - passes)
+    rng = np.random.default_rng(3100 + passes)
+    d = random_circuit(rng, 15, 70, nout=16, recent=40)
+    r, o = check(ctx, d, 11)
+    assert r["walk_jumps"] == 1 << (15 - passes)
+
+
+def test_walk_coverage_with_max_passes(ctx, monkeypatch):
+    # max_passes = C limits every walk to 2^C evaluations, with or without walks
+    c, _ = ns.Circuit.gen_satostop(131, 8, 6, 80, 10)
+    full = oracle.OracleCircuit.from_dict(c.arrays()).satostop(10, threads=8)
+    for C in (4, 9):
+        for P in ("", "2"):
+            monkeypatch.setenv("NS_JUMP_PASSES", P)
+            r = ctx.satostop_solve(c, 10, max_passes=C)
+            within = (full["steps"] > 0) & (full["steps"] <= (1 << C))
+            assert np.array_equal(r["steps_arr"], np.where(within, full["steps"], 0)), (C, P)
+            assert np.array_equal(r["ends_arr"], np.where(within, full["ends"], 0)), (C, P)
