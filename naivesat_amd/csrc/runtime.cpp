@@ -282,9 +282,22 @@ Lane::Lane(int dev, Jit& j) : device(dev), jit(j) {
   stats.ensure(4096);
 }
 
+void Lane::put_graph(const std::string& key, hipGraphExec_t exec, std::shared_ptr<JitModule> mod) {
+  if (graphs.size() >= 32) drop_graphs();
+  graphs[key] = CapturedGraph{exec, std::move(mod)};
+}
+
+void Lane::drop_graphs() {
+  if (stream) (void)hipStreamSynchronize(stream);
+  for (auto& kv : graphs)
+    if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
+  graphs.clear();
+}
+
 Lane::~Lane() {
   (void)hipSetDevice(device);
   if (stream) (void)hipStreamSynchronize(stream);
+  drop_graphs();
   buf_a.release();
   buf_b.release();
   buf_c.release();

@@ -115,6 +115,19 @@ struct Lane {
   DevBuf buf_a, buf_b, buf_c;         // QSAT scratch (bit arrays)
   DevBuf tab_a, tab_b, tab_c, tab_d;  // SATOSTOP: next-state table, jump tables, starts
   std::vector<double> pass_ms;  // jump-pass kernel times of the last SATOSTOP solve
+  // launch sequences of repeated searches, captured once and replayed as HIP graphs; each
+  // holds its kernel's module so the module outlives the graph
+  struct CapturedGraph {
+    hipGraphExec_t exec = nullptr;
+    std::shared_ptr<JitModule> mod;
+  };
+  std::unordered_map<std::string, CapturedGraph> graphs;
+  hipGraphExec_t find_graph(const std::string& key) const {
+    auto it = graphs.find(key);
+    return it == graphs.end() ? nullptr : it->second.exec;
+  }
+  void put_graph(const std::string& key, hipGraphExec_t exec, std::shared_ptr<JitModule> mod);
+  void drop_graphs();
   Lane(int dev, Jit& j);
   ~Lane();
   Lane(const Lane&) = delete;

@@ -187,14 +187,56 @@ void sat_solve(Lane& ctx, const Circuit& c, const ns_sat_opts& o, ns_sat_result&
   r.vgpr_class = ch.vgpr_class;
   uint32_t wpl = g.wpl, valid = word_valid_mask(c.n);
   unsigned long long* res = ctx.stats.as<unsigned long long>();
-  NS_HIP(launch_fill_u64(ctx.stream, res, 0ull, 1));
-  NS_HIP(launch_fill_u64(ctx.stream, res + 1, ~0ull, 1));
   const uint64_t cw = chunk_words(o.elem_bits, g, total_words);
   const bool stop_first = (o.flags & NS_SAT_STOP_AT_FIRST) != 0;
   unsigned long long host[2] = {0, ~0ull};
   double kms = 0;
   uint64_t w0 = 0;
   r.complete = 1;
+  // A repeated complete search (kernel from the cache, geometry settled) is captured once as
+  // a HIP graph (result reset + every word-range launch) and replayed afterwards.
+  const bool repeated = r.compile_ms == 0.0;
+  if (repeated && !stop_first && !(tune && !ch.tuned_wpl)) {
+    const std::string key = fp + "/graph/" + std::to_string(g.block) + "/" + std::to_string(g.wpl) + "/" +
+                            std::to_string(cw) + "/" + std::to_string(total_words);
+    hipGraphExec_t exec = ctx.find_graph(key);
+    if (!exec) {
+      NS_HIP(hipStreamBeginCapture(ctx.stream, hipStreamCaptureModeThreadLocal));
+      hipGraph_t graph = nullptr;
+      try {
+        NS_HIP(launch_fill_u64(ctx.stream, res, 0ull, 1));
+        NS_HIP(launch_fill_u64(ctx.stream, res + 1, ~0ull, 1));
+        for (uint64_t a = 0; a < total_words; a += cw) {
+          uint64_t words = total_words - a < cw ? total_words - a : cw;
+          void* extra[3] = {&res, &wpl, &valid};
+          launch_range(ctx, mod->fn, g, a, words, extra, ch.native ? 3 : 1, total_words);
+        }
+      } catch (...) {
+        (void)hipStreamEndCapture(ctx.stream, &graph);
+        if (graph) (void)hipGraphDestroy(graph);
+        throw;
+      }
+      NS_HIP(hipStreamEndCapture(ctx.stream, &graph));
+      hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      NS_HIP(ie);
+      ctx.put_graph(key, exec, mod);
+    }
+    NS_HIP(hipEventRecord(ctx.ev0, ctx.stream));
+    NS_HIP(hipGraphLaunch(exec, ctx.stream));
+    NS_HIP(hipEventRecord(ctx.ev1, ctx.stream));
+    NS_HIP(hipMemcpyAsync(host, res, sizeof(host), hipMemcpyDeviceToHost, ctx.stream));
+    NS_HIP(hipStreamSynchronize(ctx.stream));
+    r.launches = (uint32_t)((total_words + cw - 1) / cw);
+    r.count = host[0];
+    r.found = host[0] != 0;
+    r.lowest = r.found ? host[1] : 0;
+    r.kernel_ms = ctx.elapsed_ms();
+    r.wall_ms = ms_since(t0);
+    return;
+  }
+  NS_HIP(launch_fill_u64(ctx.stream, res, 0ull, 1));
+  NS_HIP(launch_fill_u64(ctx.stream, res + 1, ~0ull, 1));
   NS_HIP(hipEventRecord(ctx.ev0, ctx.stream));
   if (tune && !ch.tuned_wpl) {
     static const uint32_t cand[5] = {1, 2, 4, 8, 16};

@@ -183,13 +183,39 @@ __global__ void __launch_bounds__(256) k_lds_long(u32* out, int iters, u64* cyc)
   if (threadIdx.x == 0) cyc[blockIdx.x] = t1 - t0;
 }
 
+// 64 dependent 4-byte VOP2 ops (v_xor_b32) per iteration
+__global__ void __launch_bounds__(256) k_vop2(u32* out, int iters, u64* cyc) {
+  u32 a = threadIdx.x, b = a * 3;
+  u64 t0 = clock64();
+  for (int i = 0; i < iters; i++) {
+    asm volatile(R64("v_xor_b32 %0, %1, %0\n") : "+v"(a) : "v"(b));
+  }
+  u64 t1 = clock64();
+  out[blockIdx.x * 256 + threadIdx.x] = a;
+  if (threadIdx.x == 0) cyc[blockIdx.x] = t1 - t0;
+}
+
+// 64 independent 4-byte VOP2 ops over 4 chains
+__global__ void __launch_bounds__(256) k_vop2_rot(u32* out, int iters, u64* cyc) {
+  u32 a0 = threadIdx.x, a1 = a0 + 1, a2 = a0 + 2, a3 = a0 + 3, b = a0 * 3;
+  u64 t0 = clock64();
+  for (int i = 0; i < iters; i++) {
+    asm volatile(R8(R8("v_xor_b32 %0, %4, %0\nv_xor_b32 %1, %4, %1\nv_xor_b32 %2, %4, %2\nv_xor_b32 %3, %4, %3\n"))
+                 : "+v"(a0), "+v"(a1), "+v"(a2), "+v"(a3) : "v"(b));
+  }
+  u64 t1 = clock64();
+  out[blockIdx.x * 256 + threadIdx.x] = a0 ^ a1 ^ a2 ^ a3;
+  if (threadIdx.x == 0) cyc[blockIdx.x] = t1 - t0;
+}
+
 typedef void (*kfn)(u32*, int, u64*);
 int main() {
   struct K { const char* name; kfn f; int valu; int other; } ks[] = {
       {"dep64", k_dep, 64, 0},         {"rot256", k_rot, 256, 0},       {"salu256+64", k_salu, 256, 64},
       {"acc256+128", k_acc, 256, 128}, {"rl256+64+nop", k_rl, 256, 64}, {"rlearly256+64", k_rl_early, 256, 64},
       {"wl256+64", k_wl, 256, 64},     {"saludep256+64", k_salu_dep, 256, 64},
-      {"ldsshort256+128", k_lds_short, 256, 128}, {"ldslong256+512", k_lds_long, 256 + 256, 512}};
+      {"ldsshort256+128", k_lds_short, 256, 128}, {"ldslong256+512", k_lds_long, 256 + 256, 512},
+      {"vop2dep64", k_vop2, 64, 0},    {"vop2rot256", k_vop2_rot, 256, 0}};
   const int maxb = 256 * 8;
   u32* out;
   u64* cyc;

@@ -122,3 +122,18 @@ def test_words_per_lane_tuning_keeps_answers(ctx):
         r = ctx.sat_solve(c, elem_bits=20, **kw)
         assert (r["count"], r["lowest"]) == (count, lowest), kw
         assert r["launches"] == 64
+
+
+def test_graph_replay_matches_eager(ctx):
+    # the first solve runs eagerly, the second captures a HIP graph, later ones replay it;
+    # other work between the replays must not disturb them
+    c, key = ns.Circuit.gen_sat(31, 27, 700, 27)
+    other, _ = ns.Circuit.gen_sat(32, 22, 300, 22)
+    count, lowest = oracle.OracleCircuit.from_dict(c.arrays()).sat(threads=8)
+    ocount, olowest = oracle.OracleCircuit.from_dict(other.arrays()).sat(threads=8)
+    for i in range(5):
+        r = ctx.sat_solve(c, elem_bits=24)
+        assert (r["count"], r["lowest"]) == (count, lowest), i
+        assert r["launches"] == 8  # 2^22 words in launches of 2^19
+        ro = ctx.sat_solve(other)
+        assert (ro["count"], ro["lowest"]) == (ocount, olowest), i
