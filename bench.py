@@ -196,7 +196,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
-              "(the search shards by assignment range, see DESIGN.md)", file=sys.stderr)
+              "(one GPU per process; the searches shard by assignment range across processes)",
+              file=sys.stderr)
         sys.exit(2)
 
     import naivesat_amd as ns
