@@ -179,10 +179,19 @@ typedef struct {
   uint64_t walk_jumps; /* per start, up to this many jumps of 2^P evaluations follow the passes */
   double walk_ms;      /* kernel time of those walks */
 } ns_satostop_result;
-/* steps_out / ends_out (optional): 2^unknowns entries each; steps 0 = never stops. */
+/* steps_out / ends_out (optional): 2^unknowns entries each; steps 0 = never stops.
+ * These 32-bit arrays need n <= 31; ns_satostop_solve_wide takes up to 33 state bits. */
 int ns_satostop_solve(ns_ctx* ctx, const ns_circuit* c, uint32_t unknowns,
                       const ns_satostop_opts* opts, ns_satostop_result* res, uint32_t* steps_out,
                       uint32_t* ends_out);
+
+/* Same contract for state spaces of up to 33 bits (the reference handles states wider
+ * than 32 bits with u64 entries: join_nexts_64, src/greedysatostop.rs:169-194, and
+ * HashEntry, src/satostop.rs:168-174): the next-state table (2^n entries) and its jump
+ * tables stay in device memory (n = 33: 192 GiB), and the per-start results are 64-bit. */
+int ns_satostop_solve_wide(ns_ctx* ctx, const ns_circuit* c, uint32_t unknowns,
+                           const ns_satostop_opts* opts, ns_satostop_result* res, uint64_t* steps_out,
+                           uint64_t* ends_out);
 
 /* Table build alone (replaces greedysatostop's first-nexts launch with the transpose
  * epilogue, src/greedysatostop.rs:1042-1072; identity layout pinned by

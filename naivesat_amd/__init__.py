@@ -121,6 +121,7 @@ def load_library() -> C.CDLL:
         "ns_sat_solve": (I, [P, P, C.POINTER(_SatOpts), C.POINTER(_SatResult)]),
         "ns_qsat_solve": (I, [P, P, P, C.POINTER(_QsatOpts), C.POINTER(_QsatResult), P, U64]),
         "ns_satostop_solve": (I, [P, P, U32, C.POINTER(_StopOpts), C.POINTER(_StopResult), P, P]),
+        "ns_satostop_solve_wide": (I, [P, P, U32, C.POINTER(_StopOpts), C.POINTER(_StopResult), P, P]),
         "ns_table_build": (I, [P, P, P, U64, C.POINTER(C.c_double)]),
         "ns_satostop_pass_times": (I, [P, P, U32, C.POINTER(U32)]),
         "ns_native_body": (I, [P, U32, U32, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t), C.POINTER(U32),
@@ -146,7 +147,7 @@ def exported_symbols() -> list:
             "ns_circuit_eval", "ns_circuit_cone", "ns_circuit_free", "ns_gen_sat", "ns_gen_qsat",
             "ns_gen_satostop", "ns_ctx_create", "ns_ctx_destroy", "ns_sat_solve", "ns_qsat_solve",
             "ns_satostop_solve", "ns_table_build", "ns_satostop_pass_times", "ns_native_body", "ns_serve", "ns_native_encode",
-            "ns_native_code"]
+            "ns_native_code", "ns_satostop_solve_wide"]
 
 
 def _check(status: int) -> None:
@@ -371,13 +372,17 @@ class Context:
         return d
 
     def satostop_solve(self, c: Circuit, unknowns: int, first_pass: bool = False, max_passes: int = 0,
-                       group_len: int = 0, words_per_lane: int = 0, arrays: bool = True) -> dict:
+                       group_len: int = 0, words_per_lane: int = 0, arrays: bool = True,
+                       wide: bool = True) -> dict:
+        """Per-start steps/ends arrays are uint64 (ns_satostop_solve_wide, up to 33 state
+        bits) or, with wide=False, uint32 (ns_satostop_solve, up to 31)."""
         o = _StopOpts(NS_STOP_FIRST_PASS if first_pass else 0, max_passes, group_len, words_per_lane)
         r = _StopResult()
-        steps = np.zeros(1 << unknowns, np.uint32) if arrays else None
-        ends = np.zeros(1 << unknowns, np.uint32) if arrays else None
-        _check(load_library().ns_satostop_solve(self._h, c._h, unknowns, C.byref(o), C.byref(r), _ptr(steps),
-                                                _ptr(ends)))
+        dt = np.uint64 if wide else np.uint32
+        steps = np.zeros(1 << unknowns, dt) if arrays else None
+        ends = np.zeros(1 << unknowns, dt) if arrays else None
+        fn = load_library().ns_satostop_solve_wide if wide else load_library().ns_satostop_solve
+        _check(fn(self._h, c._h, unknowns, C.byref(o), C.byref(r), _ptr(steps), _ptr(ends)))
         d = {k: getattr(r, k) for k, _ in _StopResult._fields_}
         if arrays:
             d["steps_arr"] = steps

@@ -3,6 +3,7 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "../../include/naivesat_hip.h"
 #include "asmenc.h"
@@ -227,6 +228,24 @@ int ns_qsat_solve(ns_ctx* ctx, const ns_circuit* c, const uint8_t* quants_all, c
 
 int ns_satostop_solve(ns_ctx* ctx, const ns_circuit* c, uint32_t unknowns, const ns_satostop_opts* opts,
                       ns_satostop_result* res, uint32_t* steps_out, uint32_t* ends_out) {
+  return guard([&] {
+    need(ctx && c && res, "null argument");
+    need(c->c.n <= 31 || (!steps_out && !ends_out),
+         "32-bit arrays hold states of at most 31 bits: use ns_satostop_solve_wide");
+    ns_satostop_opts o{};
+    if (opts) o = *opts;
+    const size_t starts = unknowns <= c->c.n && unknowns < 40 ? (size_t)1 << unknowns : 0;
+    std::vector<uint64_t> st(steps_out ? starts : 0), en(ends_out ? starts : 0);
+    ns::satostop_solve(*ctx->ctx.main, c->c, unknowns, o, *res, steps_out ? st.data() : nullptr,
+                       ends_out ? en.data() : nullptr);
+    // n <= 31: steps <= 2^31 and states < 2^31 fit 32 bits
+    for (size_t i = 0; i < st.size(); i++) steps_out[i] = (uint32_t)st[i];
+    for (size_t i = 0; i < en.size(); i++) ends_out[i] = (uint32_t)en[i];
+  });
+}
+
+int ns_satostop_solve_wide(ns_ctx* ctx, const ns_circuit* c, uint32_t unknowns, const ns_satostop_opts* opts,
+                           ns_satostop_result* res, uint64_t* steps_out, uint64_t* ends_out) {
   return guard([&] {
     need(ctx && c && res, "null argument");
     ns_satostop_opts o{};

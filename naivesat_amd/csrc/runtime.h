@@ -114,6 +114,7 @@ struct Lane {
   DevBuf stats;             // small result block
   DevBuf buf_a, buf_b, buf_c;         // QSAT scratch (bit arrays)
   DevBuf tab_a, tab_b, tab_c, tab_d;  // SATOSTOP: next-state table, jump tables, starts
+  DevBuf tab_e;                       // SATOSTOP over 31 state bits: outputs 32..n of the table
   std::vector<double> pass_ms;  // jump-pass kernel times of the last SATOSTOP solve
   // launch sequences of repeated searches, captured once and replayed as HIP graphs; each
   // holds its kernel's module so the module outlives the graph
@@ -135,12 +136,15 @@ struct Lane {
   // elapsed ms between ev0 and ev1 (both recorded on `stream`)
   float elapsed_ms();
   // device bytes held by the scratch buffers
-  size_t held_bytes() const { return buf_a.cap + buf_b.cap + buf_c.cap + tab_a.cap + tab_b.cap + tab_c.cap + tab_d.cap; }
+  size_t held_bytes() const {
+    return buf_a.cap + buf_b.cap + buf_c.cap + tab_a.cap + tab_b.cap + tab_c.cap + tab_d.cap + tab_e.cap;
+  }
   void trim_tables() {
     tab_a.release();
     tab_b.release();
     tab_c.release();
     tab_d.release();
+    tab_e.release();
   }
 };
 

@@ -22,22 +22,27 @@ namespace {
 uint64_t align256(uint64_t x) { return (x + 255) & ~255ull; }
 
 // SATOSTOP buffers, in the order the solver uses them: u32 next-state table, two u64
-// jump tables, per-start results
-void stop_sizes(const ServeRequest& r, uint64_t sz[4]) {
+// jump tables, per-start results (u64 steps and ends), and for more than 31 state bits
+// the table of outputs 32..n
+constexpr int kStopBufs = 5;
+void stop_sizes(const ServeRequest& r, uint64_t sz[kStopBufs]) {
   const uint64_t states = 1ull << r.circuit->n;
   sz[0] = align256((states < 32 ? 32 : states) * 4);
   sz[1] = align256((states < 2 ? 2 : states) * 8);
   sz[2] = sz[1];
-  sz[3] = align256((1ull << r.unknowns) * 8 + 64);
+  sz[3] = align256((1ull << r.unknowns) * 16 + 64);
+  sz[4] = r.circuit->n > 31 ? sz[0] : 0;
 }
 
 }  // namespace
 
 uint64_t request_hbm_bytes(const ServeRequest& r) {
   if (r.kind != NS_REQ_SATOSTOP) return 0;  // SAT / QSAT use the lane's small buffers
-  uint64_t sz[4];
+  uint64_t sz[kStopBufs];
   stop_sizes(r, sz);
-  return sz[0] + sz[1] + sz[2] + sz[3];
+  uint64_t total = 0;
+  for (uint64_t b : sz) total += b;
+  return total;
 }
 
 namespace {
@@ -146,10 +151,11 @@ void serve(Context& ctx, const std::vector<ServeRequest>& reqs, const ns_serve_o
       }
       const ServeRequest& r = reqs[idx];
       if (need) {
-        uint64_t sz[4], at = off;
+        uint64_t sz[kStopBufs], at = off;
         stop_sizes(r, sz);
-        DevBuf* bufs[4] = {&lane.tab_a, &lane.tab_b, &lane.tab_c, &lane.tab_d};
-        for (int k = 0; k < 4; k++) {
+        DevBuf* bufs[kStopBufs] = {&lane.tab_a, &lane.tab_b, &lane.tab_c, &lane.tab_d, &lane.tab_e};
+        for (int k = 0; k < kStopBufs; k++) {
+          if (!sz[k]) continue;
           bufs[k]->borrow(arena.base() + at, sz[k]);
           at += sz[k];
         }

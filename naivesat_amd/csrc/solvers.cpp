@@ -424,11 +424,12 @@ struct TableKernelInfo {
   uint32_t vgpr_class = 0;
 };
 
-// Builds the u32 table into ctx.tab_a; returns kernel ms; sets *anylast.
-double build_table_dev(Lane& ctx, const Circuit& c, uint32_t group_len, uint32_t wpl,
+// Builds the u32 table of c's outputs (at most 32; output i at bit i) into dst; returns
+// kernel ms; sets *anylast when the last output is 1 for some state.
+double build_table_dev(Lane& ctx, const Circuit& c, uint32_t group_len, uint32_t wpl, DevBuf& dst,
                        bool* anylast, double* compile_ms, uint32_t* cone, TableKernelInfo* info_out = nullptr) {
   const uint32_t n = c.n;
-  if (n == 0 || n > 31) throw Error(ST_UNSUPPORTED, "next-state tables support 1..31 state bits");
+  if (n == 0 || n > 33) throw Error(ST_UNSUPPORTED, "next-state tables support 1..33 state bits");
   if (c.outputs.empty() || c.outputs.size() > 32) throw Error(ST_UNSUPPORTED, "table entries hold at most 32 outputs");
   const uint64_t states = 1ull << n;
   const uint64_t total_words = n >= 5 ? (1ull << (n - 5)) : 1ull;
@@ -448,8 +449,8 @@ double build_table_dev(Lane& ctx, const Circuit& c, uint32_t group_len, uint32_t
   }
   uint32_t wpl_v = g.wpl, last = (uint32_t)c.outputs.size() - 1, valid = word_valid_mask(n),
            small = n < 5 ? (1u << n) : 0u;
-  ctx.tab_a.ensure((states < 32 ? 32 : states) * 4);
-  unsigned int* T = ctx.tab_a.as<unsigned int>();
+  dst.ensure((states < 32 ? 32 : states) * 4);
+  unsigned int* T = dst.as<unsigned int>();
   unsigned int* flag = reinterpret_cast<unsigned int*>(ctx.stats.as<char>() + 64);
   NS_HIP(launch_fill_u32(ctx.stream, flag, 0u, 1));
   const uint64_t cw = chunk_words(30, g, total_words);
@@ -473,7 +474,8 @@ double table_build(Lane& ctx, const Circuit& c, uint32_t* table_out, uint64_t ta
   NS_HIP(hipSetDevice(ctx.device));
   bool any = false;
   double cms = 0;
-  double kms = build_table_dev(ctx, c, 0, 0, &any, &cms, nullptr);
+  if (c.n > 31) throw Error(ST_UNSUPPORTED, "u32 next-state tables support 1..31 state bits");
+  double kms = build_table_dev(ctx, c, 0, 0, ctx.tab_a, &any, &cms, nullptr);
   const uint64_t states = 1ull << c.n;
   if (table_out) {
     if (table_len < states) throw Error(ST_INVALID, "table buffer too small");
@@ -510,34 +512,70 @@ uint32_t choose_passes(uint32_t n, uint32_t unknowns, uint32_t cover) {
   return best;
 }
 
+namespace {
+
+// the circuit with only the outputs [from, to)
+Circuit with_outputs(const Circuit& c, uint32_t from, uint32_t to) {
+  Circuit s = c;
+  s.outputs.assign(c.outputs.begin() + from, c.outputs.begin() + to);
+  return s;
+}
+
+}  // namespace
+
 void satostop_solve(Lane& ctx, const Circuit& c, uint32_t unknowns, const ns_satostop_opts& o,
-                    ns_satostop_result& r, uint32_t* steps_out, uint32_t* ends_out) {
+                    ns_satostop_result& r, uint64_t* steps_out, uint64_t* ends_out) {
   auto t0 = clk::now();
   std::memset(&r, 0, sizeof(r));
   const uint32_t n = c.n;
   if (c.outputs.size() != (size_t)n + 1) throw Error(ST_INVALID, "SATOSTOP needs n inputs and n+1 outputs");
-  if (n == 0 || n > 31) throw Error(ST_UNSUPPORTED, "SATOSTOP tables support 1..31 state bits");
+  if (n == 0 || n > 33) throw Error(ST_UNSUPPORTED, "SATOSTOP tables support 1..33 state bits");
   if (unknowns > n) throw Error(ST_INVALID, "more unknowns than state bits");
+  // More than 31 state bits: the next-state table is split into outputs 0..31 and 32..n
+  // and the jump entries use the wide layout. NS_WIDE_TABLE=1 takes that path for any n.
+  const char* we = std::getenv("NS_WIDE_TABLE");
+  const bool wide = n > 31 || (we && *we == '1');
   NS_HIP(hipSetDevice(ctx.device));
   ctx.pass_ms.clear();
   bool any = false;
   TableKernelInfo tki;
-  r.build_ms = build_table_dev(ctx, c, o.group_len, o.words_per_lane, &any, &r.compile_ms, &r.cone_gates, &tki);
+  uint32_t split = 0;
+  if (!wide) {
+    r.build_ms = build_table_dev(ctx, c, o.group_len, o.words_per_lane, ctx.tab_a, &any, &r.compile_ms,
+                                 &r.cone_gates, &tki);
+  } else {
+    split = n + 1 > 32 ? 32 : n;  // outputs 0..split-1 in the low table
+    bool any_lo = false;
+    double cms_lo = 0, cms_hi = 0;
+    uint32_t cone_lo = 0, cone_hi = 0;
+    TableKernelInfo lo_info;
+    r.build_ms = build_table_dev(ctx, with_outputs(c, 0, split), o.group_len, o.words_per_lane, ctx.tab_a,
+                                 &any_lo, &cms_lo, &cone_lo, &lo_info);
+    r.build_ms += build_table_dev(ctx, with_outputs(c, split, n + 1), o.group_len, o.words_per_lane, ctx.tab_e,
+                                  &any, &cms_hi, &cone_hi, &tki);
+    r.compile_ms = cms_lo + cms_hi;
+    r.cone_gates = cone_lo + cone_hi;
+    tki.info.valu_ops += lo_info.info.valu_ops;
+    if (!lo_info.native) tki.native = false;
+    tki.vgpr_class = std::max(tki.vgpr_class, lo_info.vgpr_class);
+  }
   r.valu_ops = tki.info.valu_ops;
   r.backend = tki.native ? 1 : 0;
   r.vgpr_class = tki.vgpr_class;
   r.any_stop = any;
   const uint64_t states = 1ull << n;
   const uint64_t starts = 1ull << unknowns;
-  ctx.tab_d.ensure(starts * 8 + 64);
-  unsigned int* d_steps = ctx.tab_d.as<unsigned int>();
-  unsigned int* d_ends = d_steps + starts;
+  ctx.tab_d.ensure(starts * 16 + 64);
+  unsigned long long* d_steps = ctx.tab_d.as<unsigned long long>();
+  unsigned long long* d_ends = d_steps + starts;
   unsigned long long* stats = ctx.stats.as<unsigned long long>();
-  // C = log2 of the evaluations every walk covers
+  // C = log2 of the evaluations every walk covers; the wide layout counts up to 2^30 - 1
   uint32_t cover = o.max_passes ? o.max_passes : n;
   if (cover > n) cover = n;
   const bool first_pass_mode = (o.flags & NS_STOP_FIRST_PASS) != 0;
-  const uint32_t max_passes = first_pass_mode ? cover : choose_passes(n, unknowns, cover);
+  uint32_t max_passes = first_pass_mode ? cover : choose_passes(n, unknowns, cover);
+  if (wide && max_passes > 29) max_passes = 29;
+  if (first_pass_mode && wide && cover > 29) throw Error(ST_UNSUPPORTED, "first-pass mode covers at most 2^29 evaluations for more than 31 state bits");
   const uint64_t walk_jumps = 1ull << (cover - max_passes);
   unsigned long long host[2] = {0, ~0ull};
   const unsigned long long* final_tab = nullptr;
@@ -554,16 +592,20 @@ void satostop_solve(Lane& ctx, const Circuit& c, uint32_t unknowns, const ns_sat
     for (; p < max_passes; p++) {
       NS_HIP(hipEventRecord(evs[2 * p], ctx.stream));
       if (p == 0) {
-        NS_HIP(launch_jump_first(ctx.stream, ctx.tab_a.as<unsigned int>(), cur, states, n));
+        if (wide)
+          NS_HIP(launch_jump_first_wide(ctx.stream, ctx.tab_a.as<unsigned int>(), ctx.tab_e.as<unsigned int>(), cur,
+                                        states, n, split));
+        else
+          NS_HIP(launch_jump_first(ctx.stream, ctx.tab_a.as<unsigned int>(), cur, states, n));
       } else {
-        NS_HIP(launch_jump_pass(ctx.stream, cur, nxt, states < 2 ? 2 : states, n));
+        NS_HIP(launch_jump_pass(ctx.stream, cur, nxt, states < 2 ? 2 : states, n, wide));
         std::swap(cur, nxt);
       }
       NS_HIP(hipEventRecord(evs[2 * p + 1], ctx.stream));
       if (first_pass_mode) {
         NS_HIP(launch_fill_u64(ctx.stream, stats, 0ull, 1));
         NS_HIP(launch_fill_u64(ctx.stream, stats + 1, ~0ull, 1));
-        NS_HIP(launch_starts(ctx.stream, cur, n, unknowns, 1, nullptr, nullptr, stats));
+        NS_HIP(launch_starts(ctx.stream, cur, n, unknowns, 1, wide, nullptr, nullptr, stats));
         NS_HIP(hipMemcpyAsync(host, stats, sizeof(host), hipMemcpyDeviceToHost, ctx.stream));
         NS_HIP(hipStreamSynchronize(ctx.stream));
         if (host[0] != 0) {
@@ -590,7 +632,7 @@ void satostop_solve(Lane& ctx, const Circuit& c, uint32_t unknowns, const ns_sat
     NS_HIP(hipEventCreate(&w0));
     NS_HIP(hipEventCreate(&w1));
     NS_HIP(hipEventRecord(w0, ctx.stream));
-    NS_HIP(launch_starts(ctx.stream, final_tab, n, unknowns, jumps, d_steps, d_ends, stats));
+    NS_HIP(launch_starts(ctx.stream, final_tab, n, unknowns, jumps, wide, d_steps, d_ends, stats));
     NS_HIP(hipEventRecord(w1, ctx.stream));
     NS_HIP(hipMemcpyAsync(host, stats, sizeof(host), hipMemcpyDeviceToHost, ctx.stream));
     NS_HIP(hipStreamSynchronize(ctx.stream));
@@ -600,7 +642,7 @@ void satostop_solve(Lane& ctx, const Circuit& c, uint32_t unknowns, const ns_sat
     (void)hipEventDestroy(w0);
     (void)hipEventDestroy(w1);
   } else {
-    NS_HIP(launch_fill_u32(ctx.stream, d_steps, 0u, 2 * starts));
+    NS_HIP(launch_fill_u64(ctx.stream, d_steps, 0ull, 2 * starts));
     host[0] = 0;
   }
   NS_HIP(hipStreamSynchronize(ctx.stream));
@@ -609,13 +651,13 @@ void satostop_solve(Lane& ctx, const Circuit& c, uint32_t unknowns, const ns_sat
     r.lowest_u = host[1];
     r.start = r.lowest_u << (n - unknowns);
   }
-  unsigned int se[2] = {0, 0};
+  unsigned long long se[2] = {0, 0};
   if (r.n_stopped) {
-    NS_HIP(hipMemcpyAsync(&se[0], d_steps + r.lowest_u, 4, hipMemcpyDeviceToHost, ctx.stream));
-    NS_HIP(hipMemcpyAsync(&se[1], d_ends + r.lowest_u, 4, hipMemcpyDeviceToHost, ctx.stream));
+    NS_HIP(hipMemcpyAsync(&se[0], d_steps + r.lowest_u, 8, hipMemcpyDeviceToHost, ctx.stream));
+    NS_HIP(hipMemcpyAsync(&se[1], d_ends + r.lowest_u, 8, hipMemcpyDeviceToHost, ctx.stream));
   }
-  if (steps_out) NS_HIP(hipMemcpyAsync(steps_out, d_steps, starts * 4, hipMemcpyDeviceToHost, ctx.stream));
-  if (ends_out) NS_HIP(hipMemcpyAsync(ends_out, d_ends, starts * 4, hipMemcpyDeviceToHost, ctx.stream));
+  if (steps_out) NS_HIP(hipMemcpyAsync(steps_out, d_steps, starts * 8, hipMemcpyDeviceToHost, ctx.stream));
+  if (ends_out) NS_HIP(hipMemcpyAsync(ends_out, d_ends, starts * 8, hipMemcpyDeviceToHost, ctx.stream));
   NS_HIP(hipStreamSynchronize(ctx.stream));
   if (r.n_stopped) {
     r.steps = se[0];

@@ -9,11 +9,14 @@ hipError_t launch_qsat_reduce(hipStream_t s, const unsigned int* in, unsigned in
                               unsigned long long* stats);
 hipError_t launch_jump_first(hipStream_t s, const unsigned int* t0, unsigned long long* dst,
                              unsigned long long count, unsigned int n);
+hipError_t launch_jump_first_wide(hipStream_t s, const unsigned int* lo, const unsigned int* hi,
+                                  unsigned long long* dst, unsigned long long count, unsigned int n,
+                                  unsigned int split);
 hipError_t launch_jump_pass(hipStream_t s, const unsigned long long* src, unsigned long long* dst,
-                            unsigned long long count, unsigned int n);
-hipError_t launch_starts(hipStream_t s, const unsigned long long* tab, unsigned int n,
-                         unsigned int unknowns, unsigned long long jumps, unsigned int* steps_out,
-                         unsigned int* ends_out, unsigned long long* stats);
+                            unsigned long long count, unsigned int n, bool wide);
+hipError_t launch_starts(hipStream_t s, const unsigned long long* tab, unsigned int n, unsigned int unknowns,
+                         unsigned long long jumps, bool wide, unsigned long long* steps_out,
+                         unsigned long long* ends_out, unsigned long long* stats);
 hipError_t launch_fill_u64(hipStream_t s, unsigned long long* p, unsigned long long v,
                            unsigned long long count);
 hipError_t launch_fill_u32(hipStream_t s, unsigned int* p, unsigned int v, unsigned long long count);

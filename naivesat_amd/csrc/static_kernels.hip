@@ -93,50 +93,80 @@ __global__ void __launch_bounds__(256) k_qsat_reduce(const unsigned int* __restr
 // ---------------------------------------------------------------------------
 // Pointer jumping over the next-state table.
 //
-// Entry layout (u64): bits 0..n-1 next state, bit n STOP, bits 32..63 number of
-// circuit evaluations the entry stands for. A stopped entry is final; any other
-// entry i is replaced by T[next(i)] with the step counts added, so after pass p
-// every entry stands for exactly 2^p evaluations or for the path up to its STOP.
+// Entry layouts (u64). Narrow (n <= 31): bits 0..n-1 next state, bit n STOP, bits
+// 32..63 number of circuit evaluations the entry stands for. Wide (n <= 33): the same
+// with the evaluation count in bits 34..63 (at most 2^30 - 1; the wide path runs at most
+// 29 passes). A stopped entry is final; any other entry i is replaced by T[next(i)] with
+// the counts added, so after pass p every entry stands for exactly 2^p evaluations or for
+// the path up to its STOP.
 // ---------------------------------------------------------------------------
+template <bool W>
+struct Layout {
+  static constexpr unsigned int shift = W ? 34u : 32u;  // first bit of the count
+  static constexpr unsigned long long low = (1ull << shift) - 1ull;
+};
+
+// (the table kernels loop over their entries: a launch holds fewer than 2^32 threads)
 __global__ void __launch_bounds__(256) k_jump_first(const unsigned int* __restrict__ t0,
                                                     unsigned long long* __restrict__ dst,
                                                     unsigned long long count, unsigned int n) {
-  const unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= count) return;
-  const unsigned int e = t0[i];
+  const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
   const unsigned int stop_bit = 1u << n;
-  unsigned long long r;
-  if (e & stop_bit) {
-    r = (unsigned long long)e | (1ull << 32);
-  } else {
-    const unsigned int f = t0[e & (stop_bit - 1u)];
-    r = (unsigned long long)f | (2ull << 32);
+  for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
+    const unsigned int e = t0[i];
+    unsigned long long r;
+    if (e & stop_bit) {
+      r = (unsigned long long)e | (1ull << 32);
+    } else {
+      const unsigned int f = t0[e & (stop_bit - 1u)];
+      r = (unsigned long long)f | (2ull << 32);
+    }
+    dst[i] = r;
   }
-  dst[i] = r;
 }
 
+// wide first pass: entry s of T0 = lo[s] | hi[s] << split (outputs 0..split-1 and split..n)
+__global__ void __launch_bounds__(256) k_jump_first_wide(const unsigned int* __restrict__ lo,
+                                                         const unsigned int* __restrict__ hi,
+                                                         unsigned long long* __restrict__ dst,
+                                                         unsigned long long count, unsigned int n,
+                                                         unsigned int split) {
+  const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
+  const unsigned long long mask = (1ull << n) - 1ull;
+  for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
+    const unsigned long long e = (unsigned long long)lo[i] | ((unsigned long long)hi[i] << split);
+    unsigned long long r;
+    if ((e >> n) & 1ull) {
+      r = (e & Layout<true>::low) | (1ull << 34);
+    } else {
+      const unsigned long long x = e & mask;
+      const unsigned long long f = (unsigned long long)lo[x] | ((unsigned long long)hi[x] << split);
+      r = (f & Layout<true>::low) | (2ull << 34);
+    }
+    dst[i] = r;
+  }
+}
+
+template <bool W>
 __global__ void __launch_bounds__(256) k_jump_pass(const unsigned long long* __restrict__ src,
                                                    unsigned long long* __restrict__ dst,
                                                    unsigned long long count, unsigned int n) {
-  const unsigned long long base = ((unsigned long long)blockIdx.x * blockDim.x + threadIdx.x) * 2ull;
-  if (base >= count) return;
-  const unsigned int stop_bit = 1u << n;
-  // two entries per thread: two independent gathers in flight
-  const ulonglong2 e2 = *reinterpret_cast<const ulonglong2*>(src + base);
-  unsigned long long e[2] = {e2.x, e2.y};
-  unsigned long long f[2];
+  const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x * 2ull;
+  const unsigned long long stop_bit = 1ull << n;
+  constexpr unsigned int sh = Layout<W>::shift;
+  for (unsigned long long base = ((unsigned long long)blockIdx.x * blockDim.x + threadIdx.x) * 2ull; base < count;
+       base += stride) {
+    // two entries per thread: two independent gathers in flight
+    const ulonglong2 e2 = *reinterpret_cast<const ulonglong2*>(src + base);
+    unsigned long long e[2] = {e2.x, e2.y};
+    unsigned long long f[2];
 #pragma unroll
-  for (int q = 0; q < 2; q++) {
-    const unsigned int lo = (unsigned int)e[q];
-    f[q] = (lo & stop_bit) ? 0ull : src[lo & (stop_bit - 1u)];
-  }
+    for (int q = 0; q < 2; q++) f[q] = (e[q] & stop_bit) ? 0ull : src[e[q] & (stop_bit - 1ull)];
 #pragma unroll
-  for (int q = 0; q < 2; q++) {
-    const unsigned int lo = (unsigned int)e[q];
-    if (!(lo & stop_bit))
-      e[q] = (f[q] & 0xffffffffull) | (((e[q] >> 32) + (f[q] >> 32)) << 32);
+    for (int q = 0; q < 2; q++)
+      if (!(e[q] & stop_bit)) e[q] = (f[q] & Layout<W>::low) | (((e[q] >> sh) + (f[q] >> sh)) << sh);
+    *reinterpret_cast<ulonglong2*>(dst + base) = make_ulonglong2(e[0], e[1]);
   }
-  *reinterpret_cast<ulonglong2*>(dst + base) = make_ulonglong2(e[0], e[1]);
 }
 
 // Per unknown start u (state u << (n - unknowns)), with tab = the table after P passes
@@ -145,47 +175,51 @@ __global__ void __launch_bounds__(256) k_jump_pass(const unsigned long long* __r
 // jumps x 2^P evaluations. steps_out[u] = evaluations until STOP (0 when it does not stop
 // within them), ends_out[u] = state produced by the stopping evaluation; stats[0] +=
 // stopped starts, stats[1] = min(lowest stopped u).
+template <bool W>
 __global__ void __launch_bounds__(256) k_starts(const unsigned long long* __restrict__ tab,
                                                 unsigned int n, unsigned int unknowns,
-                                                unsigned long long jumps, unsigned int* steps_out,
-                                                unsigned int* ends_out, unsigned long long* stats) {
-  const unsigned long long u = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+                                                unsigned long long jumps, unsigned long long* steps_out,
+                                                unsigned long long* ends_out, unsigned long long* stats) {
+  const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
+  const unsigned long long limit = ((1ull << unknowns) + 63ull) & ~63ull;  // whole waves take part
   unsigned long long cnt = 0, best = ~0ull;
-  if (u < (1ull << unknowns)) {
-    const unsigned int mask = (1u << n) - 1u;
-    unsigned int x = (unsigned int)(u << (n - unknowns));
-    unsigned long long total = 0;
+  for (unsigned long long u = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; u < limit; u += stride) {
+    if (u >= (1ull << unknowns)) continue;
+    constexpr unsigned int sh = Layout<W>::shift;
+    const unsigned long long mask = (1ull << n) - 1ull;
+    unsigned long long x = u << (n - unknowns);
+    unsigned long long total = 0, end = 0;
     bool stop = false;
-    unsigned int end = 0;
     for (unsigned long long j = 0; j < jumps; j++) {
       const unsigned long long e = tab[x];
-      const unsigned int lo = (unsigned int)e;
-      total += e >> 32;
-      if ((lo >> n) & 1u) {
+      total += e >> sh;
+      if ((e >> n) & 1ull) {
         stop = true;
-        end = lo & mask;
+        end = e & mask;
         break;
       }
-      x = lo & mask;
+      x = e & mask;
     }
-    if (steps_out) steps_out[u] = stop ? (unsigned int)total : 0u;
-    if (ends_out) ends_out[u] = stop ? end : 0u;
+    if (steps_out) steps_out[u] = stop ? total : 0ull;
+    if (ends_out) ends_out[u] = stop ? end : 0ull;
     if (stop) {
-      cnt = 1;
-      best = u;
+      cnt++;
+      best = u < best ? u : best;
     }
   }
   wave_stats(cnt, best, stats);
 }
 
 __global__ void k_fill_u64(unsigned long long* p, unsigned long long v, unsigned long long count) {
-  const unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < count) p[i] = v;
+  const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
+  for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride)
+    p[i] = v;
 }
 
 __global__ void k_fill_u32(unsigned int* p, unsigned int v, unsigned long long count) {
-  const unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < count) p[i] = v;
+  const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
+  for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride)
+    p[i] = v;
 }
 
 // ---------------------------------------------------------------------------
@@ -193,6 +227,13 @@ __global__ void k_fill_u32(unsigned int* p, unsigned int v, unsigned long long c
 // ---------------------------------------------------------------------------
 static inline unsigned int blocks_for(unsigned long long threads, unsigned int bs) {
   return (unsigned int)((threads + bs - 1) / bs);
+}
+
+// grid of a kernel that loops over its items: one item per thread up to 2^31 threads
+static inline unsigned int looped_blocks(unsigned long long items, unsigned int bs) {
+  const unsigned long long cap = (1ull << 31) / bs;
+  const unsigned long long b = (items + bs - 1) / bs;
+  return (unsigned int)(b < cap ? b : cap);
 }
 
 hipError_t launch_qsat_reduce(hipStream_t s, const unsigned int* in, unsigned int m, unsigned int k,
@@ -207,36 +248,51 @@ hipError_t launch_qsat_reduce(hipStream_t s, const unsigned int* in, unsigned in
 
 hipError_t launch_jump_first(hipStream_t s, const unsigned int* t0, unsigned long long* dst,
                              unsigned long long count, unsigned int n) {
-  hipLaunchKernelGGL(k_jump_first, dim3(blocks_for(count, 256)), dim3(256), 0, s, t0, dst, count, n);
+  hipLaunchKernelGGL(k_jump_first, dim3(looped_blocks(count, 256)), dim3(256), 0, s, t0, dst, count, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_jump_first_wide(hipStream_t s, const unsigned int* lo, const unsigned int* hi,
+                                  unsigned long long* dst, unsigned long long count, unsigned int n,
+                                  unsigned int split) {
+  hipLaunchKernelGGL(k_jump_first_wide, dim3(looped_blocks(count, 256)), dim3(256), 0, s, lo, hi, dst, count, n,
+                     split);
   return hipGetLastError();
 }
 
 hipError_t launch_jump_pass(hipStream_t s, const unsigned long long* src, unsigned long long* dst,
-                            unsigned long long count, unsigned int n) {
-  hipLaunchKernelGGL(k_jump_pass, dim3(blocks_for(count / 2, 256)), dim3(256), 0, s, src, dst, count, n);
+                            unsigned long long count, unsigned int n, bool wide) {
+  if (wide)
+    hipLaunchKernelGGL(k_jump_pass<true>, dim3(looped_blocks(count / 2, 256)), dim3(256), 0, s, src, dst, count, n);
+  else
+    hipLaunchKernelGGL(k_jump_pass<false>, dim3(looped_blocks(count / 2, 256)), dim3(256), 0, s, src, dst, count, n);
   return hipGetLastError();
 }
 
-hipError_t launch_starts(hipStream_t s, const unsigned long long* tab, unsigned int n,
-                         unsigned int unknowns, unsigned long long jumps, unsigned int* steps_out,
-                         unsigned int* ends_out, unsigned long long* stats) {
+hipError_t launch_starts(hipStream_t s, const unsigned long long* tab, unsigned int n, unsigned int unknowns,
+                         unsigned long long jumps, bool wide, unsigned long long* steps_out,
+                         unsigned long long* ends_out, unsigned long long* stats) {
   unsigned long long cnt = 1ull << unknowns;
   unsigned long long threads = (cnt + 63ull) & ~63ull;
-  hipLaunchKernelGGL(k_starts, dim3(blocks_for(threads, 256)), dim3(256), 0, s, tab, n, unknowns, jumps,
-                     steps_out, ends_out, stats);
+  if (wide)
+    hipLaunchKernelGGL(k_starts<true>, dim3(looped_blocks(threads, 256)), dim3(256), 0, s, tab, n, unknowns, jumps,
+                       steps_out, ends_out, stats);
+  else
+    hipLaunchKernelGGL(k_starts<false>, dim3(looped_blocks(threads, 256)), dim3(256), 0, s, tab, n, unknowns, jumps,
+                       steps_out, ends_out, stats);
   return hipGetLastError();
 }
 
 hipError_t launch_fill_u64(hipStream_t s, unsigned long long* p, unsigned long long v,
                            unsigned long long count) {
   if (!count) return hipSuccess;
-  hipLaunchKernelGGL(k_fill_u64, dim3(blocks_for(count, 256)), dim3(256), 0, s, p, v, count);
+  hipLaunchKernelGGL(k_fill_u64, dim3(looped_blocks(count, 256)), dim3(256), 0, s, p, v, count);
   return hipGetLastError();
 }
 
 hipError_t launch_fill_u32(hipStream_t s, unsigned int* p, unsigned int v, unsigned long long count) {
   if (!count) return hipSuccess;
-  hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(count, 256)), dim3(256), 0, s, p, v, count);
+  hipLaunchKernelGGL(k_fill_u32, dim3(looped_blocks(count, 256)), dim3(256), 0, s, p, v, count);
   return hipGetLastError();
 }
 

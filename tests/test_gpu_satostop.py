@@ -109,7 +109,7 @@ def test_config3_full_size_golden(ctx):
     g = json.load(open(GOLDEN))["cfg3"]
     c, target = ns.Circuit.gen_satostop(4, 15, 6, 300, 14)
     assert target == g["target"]
-    r = ctx.satostop_solve(c, 14)
+    r = ctx.satostop_solve(c, 14, wide=False)  # the golden hashes are of uint32 arrays
     # P passes over the table plus walks of 2^(30-P) jumps cover 2^30 evaluations
     assert r["walk_jumps"] << r["passes"] == 1 << 30
     assert r["n_stopped"] == g["n_stopped"]
@@ -166,3 +166,65 @@ def test_walk_coverage_with_max_passes(ctx, monkeypatch):
             within = (full["steps"] > 0) & (full["steps"] <= (1 << C))
             assert np.array_equal(r["steps_arr"], np.where(within, full["steps"], 0)), (C, P)
             assert np.array_equal(r["ends_arr"], np.where(within, full["ends"], 0)), (C, P)
+
+
+def test_wide_layout_matches_oracle(ctx, monkeypatch):
+    # the code path of state spaces over 31 bits (split table, wide jump entries, 64-bit
+    # results) taken for small n, against the oracle
+    monkeypatch.setenv("NS_WIDE_TABLE", "1")
+    rng = np.random.default_rng(7100)
+    for n in (3, 8, 14, 19):
+        d = random_circuit(rng, n, int(rng.integers(n + 1, 4 * n + 20)), nout=n + 1, recent=3 * n + 4)
+        check(ctx, d, int(rng.integers(0, n + 1)))
+    for half in (6, 8):
+        c, _ = ns.Circuit.gen_satostop(7200 + half, half, 6, 10 * half, 12)
+        check(ctx, c.arrays(), 12)
+        monkeypatch.setenv("NS_JUMP_PASSES", "3")
+        check(ctx, c.arrays(), 12)
+        monkeypatch.delenv("NS_JUMP_PASSES")
+
+
+def test_u32_and_u64_entry_points_agree(ctx):
+    c, _ = ns.Circuit.gen_satostop(7301, 8, 6, 70, 11)
+    a = ctx.satostop_solve(c, 11, wide=False)
+    b = ctx.satostop_solve(c, 11)
+    assert a["steps_arr"].dtype == np.uint32 and b["steps_arr"].dtype == np.uint64
+    assert np.array_equal(a["steps_arr"], b["steps_arr"]) and np.array_equal(a["ends_arr"], b["ends_arr"])
+    assert (a["n_stopped"], a["lowest_u"], a["steps"], a["end"]) == (b["n_stopped"], b["lowest_u"], b["steps"],
+                                                                     b["end"])
+
+
+def test_32_bit_state_space(ctx, monkeypatch):
+    # n = 32: a 2^32-entry table (16 GiB per output half, 32 GiB per jump table). The
+    # Feistel map is a permutation with one STOP transition (into the planted target), so
+    # every stopping start ends at the target after a distinct number of evaluations; the
+    # starts that stop within 64 evaluations are found independently by walking on the host
+    n, U = 32, 14
+    c, target = ns.Circuit.gen_satostop(7401, 16, 6, 100, U)
+    r = ctx.satostop_solve(c, U)
+    steps, ends = r["steps_arr"], r["ends_arr"]
+    stopped = steps > 0
+    assert r["n_stopped"] == int(stopped.sum()) > 0
+    assert np.all(ends[stopped] == target)
+    assert len(set(steps[stopped].tolist())) == int(stopped.sum())
+    lo = int(np.nonzero(stopped)[0][0])
+    assert (r["lowest_u"], r["start"], r["steps"], r["end"]) == (lo, lo << (n - U), int(steps[lo]), target)
+    oc = oracle.OracleCircuit.from_dict(c.arrays())
+    short = 0
+    for u in range(1 << U):
+        s, e = oc.walk(u << (n - U), 64)
+        if s:
+            short += 1
+            assert (int(steps[u]), int(ends[u])) == (s, e), u
+        else:
+            assert steps[u] == 0 or steps[u] > 64, u
+    assert short >= 1  # the planted start
+    # the few stopping starts with the smallest counts above 64, walked on the host
+    for u in sorted(np.nonzero(steps > 64)[0], key=lambda u: steps[u])[:3]:
+        if steps[u] <= 1 << 21:
+            assert oc.walk(int(u) << (n - U), int(steps[u])) == (int(steps[u]), target)
+    # another split between passes and walks gives the same arrays
+    monkeypatch.setenv("NS_JUMP_PASSES", "12")
+    r2 = ctx.satostop_solve(c, U)
+    assert r2["passes"] == 12
+    assert np.array_equal(r2["steps_arr"], steps) and np.array_equal(r2["ends_arr"], ends)

@@ -66,8 +66,8 @@ def test_small_requests_match_oracle(ctx, mixed):
 def test_hbm_budget_admission(ctx, mixed):
     # a budget that admits one SATOSTOP table at a time still serves every request
     stop = [rq for rq in mixed if rq["kind"] == ns.REQ_SATOSTOP][:6]
-    # u32 table + two u64 jump tables + per-start results, each padded to 256 bytes
-    biggest = max((1 << rq["n"]) * 20 + (1 << rq["unknowns"]) * 8 + 2048 for rq in stop)
+    # u32 table + two u64 jump tables + per-start u64 steps and ends, each padded to 256 bytes
+    biggest = max((1 << rq["n"]) * 20 + (1 << rq["unknowns"]) * 16 + 2048 for rq in stop)
     res, _ = ctx.serve(stop, workers=4, realtime=False, hbm_budget_bytes=biggest)
     for rq, r in zip(stop, res):
         assert r["status"] == 0
