@@ -193,6 +193,43 @@ int ns_satostop_solve_wide(ns_ctx* ctx, const ns_circuit* c, uint32_t unknowns,
                            const ns_satostop_opts* opts, ns_satostop_result* res, uint64_t* steps_out,
                            uint64_t* ends_out);
 
+/* naivesatostop's hash-map search (src/satostop.rs:1152-1486, 1585-1648) for state spaces
+ * beyond the device table (n up to 63): the circuit's outputs for one block of 2^e states
+ * at a time are joined into a hash map of 2^H entries that jump along the paths, until some
+ * unknown start reaches STOP (a solution) or every unknown start is known to stop or loop
+ * (no solution). Which solution is reported depends on the hash map's contents, as in the
+ * reference; it is verified by walking the circuit. */
+typedef struct {
+  uint32_t hashmap_len_bits;             /* H: 2^H entries of 32 bytes (0 = 24, -H) */
+  uint32_t elem_bits;                    /* e: 2^e states per block (0 = 24, -e; at most n - 1) */
+  int32_t unknown_fill_bits;             /* -U (-1 = max(0, unknowns - (H - 1))) */
+  uint32_t max_predecessors;             /* -P (default 0) */
+  uint32_t iters_per_clear_predecessors; /* -C (0 = 20) */
+  uint32_t max_sweeps;                   /* sweeps over all blocks before giving up (0 = no limit) */
+} ns_hashmap_opts;
+typedef struct {
+  uint32_t found;      /* 1 solution, 0 no solution, 2 max_sweeps reached without either */
+  uint32_t sweeps;     /* sweeps over the blocks started */
+  uint64_t start, end, steps;
+  uint64_t resolved_unknowns; /* unknown-fill groups whose starts all stop or loop */
+  uint64_t blocks;     /* blocks processed */
+  double compile_ms, kernel_ms, wall_ms;
+} ns_hashmap_result;
+int ns_satostop_hashmap_solve(ns_ctx* ctx, const ns_circuit* c, uint32_t unknowns, const ns_hashmap_opts* opts,
+                              ns_hashmap_result* res);
+
+/* Diagnostic (one device step of the hash-map search on caller arrays, for checking it
+ * against the reference's data sets): which = 0 join_to_hashmap (src/satostop.rs:186-266),
+ * 1 join_hashmap_itself_and_check_solution (391-594, from hm into hm_out), 2
+ * add_to_hashmap_and_check_solution (713-942, `test` skips states whose next state is 0 as
+ * the reference's test routine does). hm / hm_out: hm_len entries of 32 bytes (HashEntry);
+ * outputs: the block's output words, two per state when state_len + 1 > 32; fills: the
+ * unknown-fill counters; sol: {resolved_unknowns, start, end, steps: u64; defined: u32}. */
+int ns_hashmap_step(ns_ctx* ctx, uint32_t which, uint32_t state_len, uint32_t arg_bit_place, uint64_t arg,
+                    uint32_t unknown_bits, uint32_t unknown_fill_bits, uint32_t max_predecessors, uint32_t test,
+                    const uint32_t* outputs, uint64_t output_words, void* hm, void* hm_out, uint64_t hm_len,
+                    uint32_t* fills, uint64_t fills_len, void* sol);
+
 /* Table build alone (replaces greedysatostop's first-nexts launch with the transpose
  * epilogue, src/greedysatostop.rs:1042-1072; identity layout pinned by
  * src/satostop.rs:1788-1915): table_out gets 2^n u32, entry s = sum_i output_i(s) << i.

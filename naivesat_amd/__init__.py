@@ -72,6 +72,18 @@ class _StopResult(C.Structure):
                 ("wall_ms", C.c_double), ("walk_jumps", C.c_uint64), ("walk_ms", C.c_double)]
 
 
+class _HmOpts(C.Structure):
+    _fields_ = [("hashmap_len_bits", C.c_uint32), ("elem_bits", C.c_uint32), ("unknown_fill_bits", C.c_int32),
+                ("max_predecessors", C.c_uint32), ("iters_per_clear_predecessors", C.c_uint32),
+                ("max_sweeps", C.c_uint32)]
+
+
+class _HmResult(C.Structure):
+    _fields_ = [("found", C.c_uint32), ("sweeps", C.c_uint32), ("start", C.c_uint64), ("end", C.c_uint64),
+                ("steps", C.c_uint64), ("resolved_unknowns", C.c_uint64), ("blocks", C.c_uint64),
+                ("compile_ms", C.c_double), ("kernel_ms", C.c_double), ("wall_ms", C.c_double)]
+
+
 class _Request(C.Structure):
     _fields_ = [("kind", C.c_uint32), ("unknowns", C.c_uint32), ("circuit", C.c_void_p),
                 ("quants_all", C.c_void_p), ("arrival_s", C.c_double)]
@@ -122,6 +134,8 @@ def load_library() -> C.CDLL:
         "ns_qsat_solve": (I, [P, P, P, C.POINTER(_QsatOpts), C.POINTER(_QsatResult), P, U64]),
         "ns_satostop_solve": (I, [P, P, U32, C.POINTER(_StopOpts), C.POINTER(_StopResult), P, P]),
         "ns_satostop_solve_wide": (I, [P, P, U32, C.POINTER(_StopOpts), C.POINTER(_StopResult), P, P]),
+        "ns_satostop_hashmap_solve": (I, [P, P, U32, C.POINTER(_HmOpts), C.POINTER(_HmResult)]),
+        "ns_hashmap_step": (I, [P, U32, U32, U32, U64, U32, U32, U32, U32, P, U64, P, P, U64, P, U64, P]),
         "ns_table_build": (I, [P, P, P, U64, C.POINTER(C.c_double)]),
         "ns_satostop_pass_times": (I, [P, P, U32, C.POINTER(U32)]),
         "ns_native_body": (I, [P, U32, U32, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t), C.POINTER(U32),
@@ -147,7 +161,7 @@ def exported_symbols() -> list:
             "ns_circuit_eval", "ns_circuit_cone", "ns_circuit_free", "ns_gen_sat", "ns_gen_qsat",
             "ns_gen_satostop", "ns_ctx_create", "ns_ctx_destroy", "ns_sat_solve", "ns_qsat_solve",
             "ns_satostop_solve", "ns_table_build", "ns_satostop_pass_times", "ns_native_body", "ns_serve", "ns_native_encode",
-            "ns_native_code", "ns_satostop_solve_wide"]
+            "ns_native_code", "ns_satostop_solve_wide", "ns_satostop_hashmap_solve", "ns_hashmap_step"]
 
 
 def _check(status: int) -> None:
@@ -388,6 +402,25 @@ class Context:
             d["steps_arr"] = steps
             d["ends_arr"] = ends
         return d
+
+    def satostop_hashmap_solve(self, c: Circuit, unknowns: int, hashmap_len_bits: int = 0, elem_bits: int = 0,
+                               unknown_fill_bits: int = -1, max_predecessors: int = 0,
+                               iters_per_clear_predecessors: int = 0, max_sweeps: int = 0) -> dict:
+        o = _HmOpts(hashmap_len_bits, elem_bits, unknown_fill_bits, max_predecessors,
+                    iters_per_clear_predecessors, max_sweeps)
+        r = _HmResult()
+        _check(load_library().ns_satostop_hashmap_solve(self._h, c._h, unknowns, C.byref(o), C.byref(r)))
+        return {k: getattr(r, k) for k, _ in _HmResult._fields_}
+
+    def hashmap_step(self, which: int, state_len: int, arg_bit_place: int, arg: int, unknown_bits: int,
+                     unknown_fill_bits: int, max_predecessors: int, test: bool, outputs, hm, hm_out, fills, sol):
+        """One device step of the hash-map search on numpy arrays (hm: 32-byte entries,
+        sol: 40-byte {resolved_unknowns, start, end, steps, defined}); updates them in place."""
+        out_ptr = _ptr(outputs) if outputs is not None else None
+        _check(load_library().ns_hashmap_step(
+            self._h, which, state_len, arg_bit_place, arg, unknown_bits, unknown_fill_bits, max_predecessors,
+            1 if test else 0, out_ptr, 0 if outputs is None else outputs.size, _ptr(hm),
+            _ptr(hm_out) if hm_out is not None else None, hm.size, _ptr(fills), fills.size, _ptr(sol)))
 
     def table_build(self, c: Circuit) -> np.ndarray:
         n = c.n_inputs

@@ -254,6 +254,28 @@ int ns_satostop_solve_wide(ns_ctx* ctx, const ns_circuit* c, uint32_t unknowns, 
   });
 }
 
+int ns_satostop_hashmap_solve(ns_ctx* ctx, const ns_circuit* c, uint32_t unknowns, const ns_hashmap_opts* opts,
+                              ns_hashmap_result* res) {
+  return guard([&] {
+    need(ctx && c && res, "null argument");
+    ns_hashmap_opts o{};
+    o.unknown_fill_bits = -1;
+    if (opts) o = *opts;
+    ns::satostop_hashmap_solve(*ctx->ctx.main, c->c, unknowns, o, *res);
+  });
+}
+
+int ns_hashmap_step(ns_ctx* ctx, uint32_t which, uint32_t state_len, uint32_t arg_bit_place, uint64_t arg,
+                    uint32_t unknown_bits, uint32_t unknown_fill_bits, uint32_t max_predecessors, uint32_t test,
+                    const uint32_t* outputs, uint64_t output_words, void* hm, void* hm_out, uint64_t hm_len,
+                    uint32_t* fills, uint64_t fills_len, void* sol) {
+  return guard([&] {
+    need(ctx && hm && fills && sol && (which != 1 || hm_out) && (which == 1 || outputs), "null argument");
+    ns::hashmap_step(*ctx->ctx.main, which, state_len, arg_bit_place, arg, unknown_bits, unknown_fill_bits,
+                     max_predecessors, test != 0, outputs, output_words, hm, hm_out, hm_len, fills, fills_len, sol);
+  });
+}
+
 int ns_table_build(ns_ctx* ctx, const ns_circuit* c, uint32_t* table_out, uint64_t table_len, double* kernel_ms) {
   return guard([&] {
     need(ctx && c, "null argument");

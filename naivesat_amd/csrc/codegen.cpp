@@ -553,7 +553,7 @@ std::string gen_table_kernel(const Circuit& c, const Geometry& g, GenInfo* info)
         "    }\n";
   if (c.n >= 5) {
     os << "    if (w < wend) {\n"
-          "      uint4* dst = (uint4*)(table + (w << 5));\n"
+          "      uint4* dst = (uint4*)(table + ((w - w0) << 5));\n"
           "#pragma unroll\n"
           "      for (int q = 0; q < 8; q++) dst[q] = make_uint4(A[4*q], A[4*q+1], A[4*q+2], A[4*q+3]);\n"
           "    }\n";

@@ -13,6 +13,7 @@
 
 #include "asmgen.h"
 #include "codegen.h"
+#include "hashmap_kernels.h"
 #include "static_kernels.h"
 
 namespace ns {
@@ -457,7 +458,8 @@ double build_table_dev(Lane& ctx, const Circuit& c, uint32_t group_len, uint32_t
   NS_HIP(hipEventRecord(ctx.ev0, ctx.stream));
   for (uint64_t w0 = 0; w0 < total_words; w0 += cw) {
     uint64_t words = total_words - w0 < cw ? total_words - w0 : cw;
-    void* extra[6] = {&T, &flag, &wpl_v, &last, &valid, &small};
+    unsigned int* Tc = T + (w0 << 5);  // the kernel writes relative to its first word
+    void* extra[6] = {&Tc, &flag, &wpl_v, &last, &valid, &small};
     launch_range(ctx, mod->fn, g, w0, words, extra, ch.native ? 6 : 2, total_words);
   }
   NS_HIP(hipEventRecord(ctx.ev1, ctx.stream));
@@ -663,6 +665,176 @@ void satostop_solve(Lane& ctx, const Circuit& c, uint32_t unknowns, const ns_sat
     r.steps = se[0];
     r.end = se[1];
   }
+  r.wall_ms = ms_since(t0);
+}
+
+// ---------------------------------------------------------------------------
+// SATOSTOP through naivesatostop's hash map
+// ---------------------------------------------------------------------------
+void hashmap_step(Lane& ctx, uint32_t which, uint32_t state_len, uint32_t arg_bit_place, uint64_t arg,
+                  uint32_t unknown_bits, uint32_t unknown_fill_bits, uint32_t max_predecessors, bool test,
+                  const uint32_t* outputs, uint64_t output_words, void* hm, void* hm_out, uint64_t hm_len,
+                  uint32_t* fills, uint64_t fills_len, void* sol) {
+  if (which > 2 || state_len == 0 || state_len > 63 || hm_len == 0 || (hm_len & (hm_len - 1)))
+    throw Error(ST_INVALID, "bad hash-map step arguments");
+  if (unknown_fill_bits > unknown_bits || unknown_bits > state_len || arg_bit_place >= state_len)
+    throw Error(ST_INVALID, "bad unknown / block widths");
+  uint32_t hbits = 0;
+  while ((1ull << hbits) < hm_len) hbits++;
+  if (hbits > state_len) throw Error(ST_INVALID, "hash map larger than the state space");
+  const uint64_t wpe = (state_len + 1 + 31) >> 5;
+  if (which != 1 && output_words < (wpe << arg_bit_place)) throw Error(ST_INVALID, "outputs array too small");
+  if (fills_len < (1ull << (unknown_bits - unknown_fill_bits))) throw Error(ST_INVALID, "fills array too small");
+  if (which != 1 && (arg >> (state_len - arg_bit_place)) != 0) throw Error(ST_INVALID, "block index out of range");
+  // the kernels index the map and the fill counters by the entries' states
+  for (uint64_t i = 0; i < hm_len; i++) {
+    const HmEntry& he = static_cast<const HmEntry*>(hm)[i];
+    if (he.state > 3) throw Error(ST_INVALID, "hash entry with an unknown state");
+    if (he.state != 0 && ((he.current >> state_len) != 0 || (he.next >> state_len) != 0))
+      throw Error(ST_INVALID, "hash entry state wider than state_len");
+  }
+  NS_HIP(hipSetDevice(ctx.device));
+  DevBuf d_out, d_hm, d_hm2, d_fills, d_sol;
+  const uint64_t hm_bytes = hm_len * sizeof(HmEntry);
+  d_hm.ensure(hm_bytes);
+  d_fills.ensure(fills_len * 4);
+  d_sol.ensure(sizeof(HmSolution));
+  NS_HIP(hipMemcpyAsync(d_hm.p, hm, hm_bytes, hipMemcpyHostToDevice, ctx.stream));
+  NS_HIP(hipMemcpyAsync(d_fills.p, fills, fills_len * 4, hipMemcpyHostToDevice, ctx.stream));
+  NS_HIP(hipMemcpyAsync(d_sol.p, sol, sizeof(HmSolution), hipMemcpyHostToDevice, ctx.stream));
+  HmParams hp{state_len, arg_bit_place, hbits, unknown_bits, unknown_fill_bits};
+  HmOutputs ho{nullptr, nullptr, wpe};
+  if (which != 1) {
+    d_out.ensure(output_words * 4);
+    NS_HIP(hipMemcpyAsync(d_out.p, outputs, output_words * 4, hipMemcpyHostToDevice, ctx.stream));
+    ho.lo = d_out.as<unsigned int>();
+    ho.hi = wpe == 2 ? ho.lo + 1 : nullptr;
+  }
+  if (which == 0) {
+    NS_HIP(hm_join_to(ctx.stream, hp, arg, ho, d_hm.as<HmEntry>(), hm_len));
+  } else if (which == 1) {
+    d_hm2.ensure(hm_bytes);
+    NS_HIP(hm_join_itself(ctx.stream, hp, d_hm.as<HmEntry>(), d_hm2.as<HmEntry>(), hm_len,
+                          d_fills.as<unsigned int>(), d_sol.as<HmSolution>()));
+  } else {
+    NS_HIP(hm_add(ctx.stream, hp, arg, ho, d_hm.as<HmEntry>(), d_fills.as<unsigned int>(), d_sol.as<HmSolution>(),
+                  max_predecessors, test));
+  }
+  NS_HIP(hipMemcpyAsync(which == 1 ? hm_out : hm, which == 1 ? d_hm2.p : d_hm.p, hm_bytes, hipMemcpyDeviceToHost,
+                        ctx.stream));
+  NS_HIP(hipMemcpyAsync(fills, d_fills.p, fills_len * 4, hipMemcpyDeviceToHost, ctx.stream));
+  NS_HIP(hipMemcpyAsync(sol, d_sol.p, sizeof(HmSolution), hipMemcpyDeviceToHost, ctx.stream));
+  NS_HIP(hipStreamSynchronize(ctx.stream));
+}
+
+void satostop_hashmap_solve(Lane& ctx, const Circuit& c, uint32_t unknowns, const ns_hashmap_opts& o,
+                            ns_hashmap_result& r) {
+  auto t0 = clk::now();
+  std::memset(&r, 0, sizeof(r));
+  const uint32_t n = c.n;
+  if (c.outputs.size() != (size_t)n + 1) throw Error(ST_INVALID, "SATOSTOP needs n inputs and n+1 outputs");
+  if (n < 10 || n > 63) throw Error(ST_UNSUPPORTED, "the hash-map search takes 10..63 state bits");
+  if (unknowns == 0 || unknowns > n) throw Error(ST_INVALID, "unknowns must be in 1..n");
+  const uint32_t H = o.hashmap_len_bits ? o.hashmap_len_bits : 24;
+  if (H > n || H > 34) throw Error(ST_INVALID, "hash map bits must not exceed the state bits (and 34)");
+  uint32_t e = o.elem_bits ? o.elem_bits : 24;
+  if (e >= n) e = n - 1;
+  if (e < 5) throw Error(ST_INVALID, "blocks need at least 32 states");
+  // calculate_default_unknown_fill_bits, src/satostop.rs:101-107
+  const uint32_t ufb = o.unknown_fill_bits >= 0 ? (uint32_t)o.unknown_fill_bits
+                                                : (H - 1 < unknowns ? unknowns - (H - 1) : 0);
+  if (ufb > unknowns) throw Error(ST_INVALID, "unknown fill bits exceed the unknowns");
+  const uint32_t clear_every = o.iters_per_clear_predecessors ? o.iters_per_clear_predecessors : 20;
+  NS_HIP(hipSetDevice(ctx.device));
+  // the block's outputs: outputs 0..31 (and 32..n) as u32 tables of 2^e entries
+  const bool two = n + 1 > 32;
+  const uint32_t split = two ? 32 : n + 1;
+  const uint64_t block = 1ull << e, block_words = block >> 5;
+  Geometry g = pick_geometry(block_words, 0, 2, false);
+  std::vector<uint32_t> vb(n);
+  for (uint32_t k = 0; k < n; k++) vb[k] = k;
+  Chosen kern[2];
+  const Circuit parts[2] = {with_outputs(c, 0, split), two ? with_outputs(c, split, n + 1) : Circuit()};
+  for (int h = 0; h < (two ? 2 : 1); h++) {
+    std::vector<uint32_t> outs;
+    for (uint32_t k = 0; k < parts[h].outputs.size(); k++) outs.push_back(k);
+    double cms = 0;
+    kern[h] = choose_kernel(
+        ctx, parts[h], NativeKind::TABLE, vb, 0, outs, g, "",
+        [&](GenInfo* info) { return gen_table_kernel(parts[h], g, info); }, "ns_table", &cms);
+    r.compile_ms += cms;
+  }
+  // a launch covers whole thread blocks: small blocks leave slack at the end of the tables
+  const uint64_t wpb = g.words_per_block();
+  const uint64_t tab_bytes = (block_words + wpb - 1) / wpb * wpb * 32 * 4;
+  ctx.tab_a.ensure(tab_bytes);
+  if (two) ctx.tab_e.ensure(tab_bytes);
+  const uint64_t hm_len = 1ull << H;
+  ctx.tab_b.ensure(hm_len * sizeof(HmEntry));
+  ctx.tab_c.ensure(hm_len * sizeof(HmEntry));
+  const uint64_t nfills = 1ull << (unknowns - ufb);
+  ctx.tab_d.ensure(nfills * 4 + 256);
+  HmEntry* hm1 = ctx.tab_b.as<HmEntry>();
+  HmEntry* hm2 = ctx.tab_c.as<HmEntry>();
+  unsigned int* fills = ctx.tab_d.as<unsigned int>();
+  HmSolution* sol = reinterpret_cast<HmSolution*>(ctx.stats.as<char>() + 128);
+  NS_HIP(launch_fill_u64(ctx.stream, reinterpret_cast<unsigned long long*>(hm1), 0ull, hm_len * 4));
+  NS_HIP(launch_fill_u64(ctx.stream, reinterpret_cast<unsigned long long*>(hm2), 0ull, hm_len * 4));
+  NS_HIP(launch_fill_u32(ctx.stream, fills, 0u, nfills));
+  NS_HIP(launch_fill_u64(ctx.stream, reinterpret_cast<unsigned long long*>(sol), 0ull, 5));
+  HmParams hp{n, e, H, unknowns, ufb};
+  HmOutputs ho{ctx.tab_a.as<unsigned int>(), two ? ctx.tab_e.as<unsigned int>() : nullptr, 1};
+  unsigned int* flag = reinterpret_cast<unsigned int*>(ctx.stats.as<char>() + 64);
+  const uint64_t args = 1ull << (n - e);
+  uint32_t iter = 0;
+  HmSolution host{};
+  float kms_total = 0;
+  NS_HIP(hipEventRecord(ctx.ev0, ctx.stream));
+  for (r.sweeps = 1;; r.sweeps++) {
+    bool done = false;
+    for (uint64_t arg = 0; arg < args && !done; arg++) {
+      // outputs of states [arg << e, (arg + 1) << e)
+      const uint64_t w0 = arg * block_words;
+      for (int h = 0; h < (two ? 2 : 1); h++) {
+        unsigned int* T = h ? ctx.tab_e.as<unsigned int>() : ctx.tab_a.as<unsigned int>();
+        uint32_t wpl_v = g.wpl, last = (uint32_t)parts[h].outputs.size() - 1, valid = 0xffffffffu, small = 0;
+        void* extra[6] = {&T, &flag, &wpl_v, &last, &valid, &small};
+        launch_range(ctx, kern[h].mod->fn, g, w0, block_words, extra, kern[h].native ? 6 : 2, w0 + block_words);
+      }
+      // the reference's process(): join_to, join_itself, add, clear, swap
+      NS_HIP(hm_join_to(ctx.stream, hp, arg, ho, hm1, hm_len));
+      NS_HIP(hm_join_itself(ctx.stream, hp, hm1, hm2, hm_len, fills, sol));
+      NS_HIP(hm_add(ctx.stream, hp, arg, ho, hm2, fills, sol, o.max_predecessors, false));
+      if (++iter >= clear_every) {
+        NS_HIP(hm_clear_preds(ctx.stream, hm2, hm_len));
+        iter = 0;
+      }
+      std::swap(hm1, hm2);
+      r.blocks++;
+      NS_HIP(hipMemcpyAsync(&host, sol, sizeof(host), hipMemcpyDeviceToHost, ctx.stream));
+      NS_HIP(hipStreamSynchronize(ctx.stream));
+      if (host.defined) {
+        r.found = 1;
+        r.start = host.start;
+        r.end = host.end;
+        r.steps = host.steps;
+        done = true;
+      } else if (host.resolved_unknowns == nfills) {
+        r.found = 0;
+        done = true;
+      }
+    }
+    if (done) break;
+    if (o.max_sweeps && r.sweeps >= o.max_sweeps) {
+      r.found = 2;
+      break;
+    }
+  }
+  NS_HIP(hipEventRecord(ctx.ev1, ctx.stream));
+  NS_HIP(hipEventSynchronize(ctx.ev1));
+  kms_total = ctx.elapsed_ms();
+  r.resolved_unknowns = host.resolved_unknowns;
+  r.kernel_ms = kms_total;
   r.wall_ms = ms_since(t0);
 }
 

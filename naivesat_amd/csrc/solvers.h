@@ -16,6 +16,15 @@ void satostop_solve(Lane& ctx, const Circuit& c, uint32_t unknowns, const ns_sat
                     ns_satostop_result& r, uint64_t* steps_out, uint64_t* ends_out);
 double table_build(Lane& ctx, const Circuit& c, uint32_t* table_out, uint64_t table_len);
 
+// naivesatostop's hash-map search (state spaces beyond the device table)
+void satostop_hashmap_solve(Lane& ctx, const Circuit& c, uint32_t unknowns, const ns_hashmap_opts& o,
+                            ns_hashmap_result& r);
+// one device step of it on host arrays (diagnostic; see ns_hashmap_step)
+void hashmap_step(Lane& ctx, uint32_t which, uint32_t state_len, uint32_t arg_bit_place, uint64_t arg,
+                  uint32_t unknown_bits, uint32_t unknown_fill_bits, uint32_t max_predecessors, bool test,
+                  const uint32_t* outputs, uint64_t output_words, void* hm, void* hm_out, uint64_t hm_len,
+                  uint32_t* fills, uint64_t fills_len, void* sol);
+
 // per-pass jump times of the last satostop_solve on a context
 const std::vector<double>& last_pass_times(Lane& ctx);
 

@@ -187,7 +187,8 @@ def table_kernel(cls, pool_hi):
     outs = ", ".join(f'"={{v{k}}}"(A[{k}])' for k in range(32))
     ins = ", ".join([f'"{{v{32 + i}}}"(LM{i})' for i in range(6)] + ['"{s20}"(wlo)', '"{s21}"(whi)'])
     return f'''
-// Next-state table: table[32w + b] = outputs of state 32w + b packed (output i at bit i);
+// Next-state table: table[32(w - w0) + b] = outputs of state 32w + b packed (output i at
+// bit i), so a launch over words [w0, wend) fills its own slice;
 // *anylast |= any state whose last output (STOP) is 1. w = U + lane + (j << 6).
 extern "C" __global__ void __launch_bounds__(256) {name}(u64 w0, u64 wend, u32* table, u32* anylast, u32 wpl,
                                                       u32 last, u32 valid, u32 small) {{
@@ -215,7 +216,7 @@ extern "C" __global__ void __launch_bounds__(256) {name}(u64 w0, u64 wend, u32* 
     }}
     if (w < wend) {{
       if (small == 0u) {{
-        uint4* dst = (uint4*)(table + (w << 5));
+        uint4* dst = (uint4*)(table + ((w - w0) << 5));
 #pragma unroll
         for (int q = 0; q < 8; q++) dst[q] = make_uint4(A[4 * q], A[4 * q + 1], A[4 * q + 2], A[4 * q + 3]);
       }} else {{

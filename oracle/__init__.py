@@ -21,6 +21,14 @@ class _Circ(C.Structure):
                 ("out_wire", C.c_void_p), ("out_neg", C.c_void_p)]
 
 
+class _HmSolution(C.Structure):
+    _fields_ = [("start", C.c_uint64), ("end", C.c_uint64), ("steps", C.c_uint64), ("defined", C.c_uint32)]
+
+
+# naivesatostop hash entry (src/satostop.rs:168-174): 32 bytes
+HASH_ENTRY = np.dtype([("current", "<u8"), ("next", "<u8"), ("steps", "<u8"), ("predecessors", "<u4"),
+                       ("state", "<u4")])
+
 _lib = None
 
 
@@ -57,6 +65,15 @@ def lib() -> C.CDLL:
         L.or_satostop_table.argtypes = [C.POINTER(_Circ), U32, I, P, P, C.POINTER(U64), C.POINTER(U64)]
         L.or_table_u32.restype = I
         L.or_table_u32.argtypes = [C.POINTER(_Circ), P]
+        L.or_hash64.restype = U64
+        L.or_hash64.argtypes = [U32, U64]
+        L.or_hm_join_to.restype = None
+        L.or_hm_join_to.argtypes = [U32, U32, U64, P, P, U64]
+        L.or_hm_join_itself.restype = None
+        L.or_hm_join_itself.argtypes = [U32, P, P, U64, U32, U32, P, C.POINTER(U64), C.POINTER(_HmSolution)]
+        L.or_hm_add.restype = None
+        L.or_hm_add.argtypes = [U32, U32, U64, P, P, U64, U32, U32, P, C.POINTER(U64), C.POINTER(_HmSolution),
+                                U32, I]
         _lib = L
     return _lib
 
@@ -158,3 +175,39 @@ class QuantReducer:
         if getattr(self, "_h", None) and _lib is not None:
             _lib.or_qr_free(self._h)
             self._h = None
+
+
+def hash64(bits: int, value: int) -> int:
+    return int(lib().or_hash64(bits, value))
+
+
+class HashMapState:
+    """Unknown-fill counters, resolved-unknown count and solution of the hash-map path."""
+
+    def __init__(self, unknown_fills: np.ndarray, resolved: int = 0):
+        self.unknown_fills = np.ascontiguousarray(unknown_fills, np.uint32)
+        self.resolved = C.c_uint64(resolved)
+        self.sol = _HmSolution()
+
+    def solution(self):
+        s = self.sol
+        return {"start": s.start, "end": s.end, "steps": s.steps} if s.defined else None
+
+
+def hm_join_to(state_len, arg_bit_place, arg, outputs, hm):
+    lib().or_hm_join_to(state_len, arg_bit_place, arg, outputs.ctypes.data_as(C.c_void_p),
+                        hm.ctypes.data_as(C.c_void_p), hm.size)
+
+
+def hm_join_itself(state_len, hm_in, hm_out, unknown_bits, unknown_fill_bits, st: HashMapState):
+    lib().or_hm_join_itself(state_len, hm_in.ctypes.data_as(C.c_void_p), hm_out.ctypes.data_as(C.c_void_p),
+                            hm_in.size, unknown_bits, unknown_fill_bits,
+                            st.unknown_fills.ctypes.data_as(C.c_void_p), C.byref(st.resolved), C.byref(st.sol))
+
+
+def hm_add(state_len, arg_bit_place, arg, outputs, hm, unknown_bits, unknown_fill_bits, st: HashMapState,
+           max_predecessors, test):
+    lib().or_hm_add(state_len, arg_bit_place, arg, outputs.ctypes.data_as(C.c_void_p),
+                    hm.ctypes.data_as(C.c_void_p), hm.size, unknown_bits, unknown_fill_bits,
+                    st.unknown_fills.ctypes.data_as(C.c_void_p), C.byref(st.resolved), C.byref(st.sol),
+                    max_predecessors, 1 if test else 0)

@@ -587,3 +587,152 @@ int or_table_u32(const or_circuit* c, uint32_t* table) {
   for (uint64_t s = 0; s < (1ull << n); s++) table[s] = (uint32_t)or_eval_u64(c, s);
   return 0;
 }
+
+/* ------------------------------------------------------------------------------------
+ * naivesatostop hash map (src/satostop.rs:127-1150), restated sequentially: entries are
+ * processed in index order, so where the reference's parallel loops race (which of two
+ * starts becomes `solution` first, the order of unknown-fill increments), this is one of
+ * the orders the reference may take.
+ * ---------------------------------------------------------------------------------- */
+
+/* hash_function_64, src/satostop.rs:127-136 (u64 wrapping arithmetic) */
+uint64_t or_hash64(uint32_t bits, uint64_t value) {
+  const uint64_t mask = bits >= 64 ? ~0ull : ((1ull << bits) - 1ull);
+  const uint32_t half = bits >> 1;
+  uint64_t temp = value * 9615409803190489167ull;
+  temp = (temp << half) | (bits - half >= 64 ? 0 : (temp >> (bits - half)));
+  const uint64_t hash = (value * 6171710485021949031ull) ^ temp ^ 0xb89d2ecda078ca1full;
+  return hash & mask;
+}
+
+enum { HS_UNUSED = 0, HS_USED = 1, HS_STOPPED = 2, HS_LOOPED = 3, HS_RESERVED = 4 };
+
+static uint64_t output_at(const uint32_t* outputs, uint32_t state_len, uint64_t i) {
+  const uint32_t wpe = (state_len + 1 + 31) >> 5;
+  return wpe == 2 ? ((uint64_t)outputs[2 * i] | ((uint64_t)outputs[2 * i + 1] << 32)) : (uint64_t)outputs[i];
+}
+
+/* join_to_hashmap_cpu, src/satostop.rs:186-235 */
+void or_hm_join_to(uint32_t state_len, uint32_t arg_bit_place, uint64_t arg, const uint32_t* outputs,
+                   or_hash_entry* hm, uint64_t len) {
+  const uint64_t arg_start = arg << arg_bit_place, arg_end = arg_start + (1ull << arg_bit_place);
+  const uint64_t state_mask = (1ull << state_len) - 1ull;
+  for (uint64_t i = 0; i < len; i++) {
+    or_hash_entry* he = &hm[i];
+    if (he->state == HS_USED && arg_start <= he->next && he->next < arg_end) {
+      const uint64_t output = output_at(outputs, state_len, he->next - arg_start);
+      const uint64_t old_next = he->next;
+      he->next = output & state_mask;
+      if ((output >> state_len) & 1ull)
+        he->state = HS_STOPPED;
+      else if (state_mask <= he->steps || he->next == he->current || he->next == old_next)
+        he->state = HS_LOOPED;
+      else
+        he->state = HS_USED;
+      he->steps += 1;
+    }
+  }
+}
+
+/* resolve_unknowns, src/satostop.rs:330-381 */
+static void resolve_unknowns(uint32_t state_len, uint32_t unknown_bits, uint32_t unknown_fill_bits, uint64_t current,
+                             uint64_t next, uint64_t steps, uint32_t entry_state, uint32_t* unknown_fills,
+                             uint64_t* resolved, or_hm_solution* sol) {
+  if ((current & ((1ull << (state_len - unknown_bits)) - 1ull)) == 0 &&
+      (entry_state == HS_LOOPED || entry_state == HS_STOPPED)) {
+    if (entry_state == HS_STOPPED && !sol->defined) {
+      sol->defined = 1;
+      sol->start = current;
+      sol->end = next;
+      sol->steps = steps;
+    }
+    const uint64_t idx = current >> (state_len - unknown_bits + unknown_fill_bits);
+    const uint32_t fill_mask = (1u << unknown_fill_bits) - 1u;
+    const uint32_t value = (uint32_t)((current >> (state_len - unknown_bits)) & fill_mask);
+    if (unknown_fills[idx] == value) {
+      unknown_fills[idx] = value + 1;
+      if (value == fill_mask) (*resolved)++;
+    }
+  }
+}
+
+/* join_hashmap_itself_and_check_solution, the device kernel of src/satostop.rs:540-594
+ * (its CPU twin, 391-479, bounds the summed steps by 2^(state_len-1) - 1 instead of
+ * 2^state_len - 1; the reference's data set gives the same result under both) */
+void or_hm_join_itself(uint32_t state_len, const or_hash_entry* in, or_hash_entry* out, uint64_t len,
+                       uint32_t unknown_bits, uint32_t unknown_fill_bits, uint32_t* unknown_fills,
+                       uint64_t* resolved, or_hm_solution* sol) {
+  uint32_t hbits = 0;
+  while ((1ull << hbits) < len) hbits++;
+  const uint32_t shift = state_len - hbits;
+  const uint64_t state_mask = state_len >= 64 ? ~0ull : (1ull << state_len) - 1ull;
+  uint32_t* preds = (uint32_t*)calloc(len, sizeof(uint32_t));
+  for (uint64_t i = 0; i < len; i++) {
+    const or_hash_entry* inhe = &in[i];
+    or_hash_entry* outhe = &out[i];
+    if (inhe->state == HS_USED) {
+      const uint64_t nh = or_hash64(state_len, inhe->next) >> shift;
+      const or_hash_entry* nexthe = &in[nh];
+      if (nexthe->state != HS_UNUSED && nexthe->current == inhe->next) {
+        outhe->current = inhe->current;
+        outhe->next = nexthe->next;
+        const uint64_t res = inhe->steps + nexthe->steps;
+        const int ov = res < inhe->steps;
+        outhe->steps = res;
+        outhe->state = nexthe->state;
+        if (outhe->state == HS_USED && (inhe->current == nexthe->next || ov || res > state_mask))
+          outhe->state = HS_LOOPED;
+        outhe->predecessors = inhe->predecessors;
+        preds[nh]++;
+      } else {
+        *outhe = *inhe;
+      }
+    } else {
+      *outhe = *inhe;
+    }
+    resolve_unknowns(state_len, unknown_bits, unknown_fill_bits, outhe->current, outhe->next, outhe->steps,
+                     outhe->state, unknown_fills, resolved, sol);
+  }
+  for (uint64_t i = 0; i < len; i++) out[i].predecessors += preds[i];
+  free(preds);
+}
+
+/* add_to_hashmap_and_check_solution_cpu, src/satostop.rs:713-860 (test: skip elements
+ * whose next state is 0, as the reference's test routine does) */
+void or_hm_add(uint32_t state_len, uint32_t arg_bit_place, uint64_t arg, const uint32_t* outputs,
+               or_hash_entry* hm, uint64_t len, uint32_t unknown_bits, uint32_t unknown_fill_bits,
+               uint32_t* unknown_fills, uint64_t* resolved, or_hm_solution* sol, uint32_t max_predecessors,
+               int test) {
+  uint32_t hbits = 0;
+  while ((1ull << hbits) < len) hbits++;
+  const uint32_t shift = state_len - hbits;
+  const uint64_t arg_start = arg << arg_bit_place;
+  const uint64_t state_mask = (1ull << state_len) - 1ull;
+  const uint64_t fill_mask = (1ull << unknown_fill_bits) - 1ull;
+  const uint64_t before_unknowns = (1ull << (state_len - unknown_bits)) - 1ull;
+  const uint32_t fill_shift = state_len - unknown_bits + unknown_fill_bits;
+  for (uint64_t i = 0; i < (1ull << arg_bit_place); i++) {
+    const uint64_t output = output_at(outputs, state_len, i);
+    const uint64_t current = i + arg_start;
+    const uint64_t next = output & state_mask;
+    const uint32_t state = ((output >> state_len) & 1ull) ? HS_STOPPED : (next == current ? HS_LOOPED : HS_USED);
+    if (test && next == 0) continue;
+    or_hash_entry* cur = &hm[or_hash64(state_len, current) >> shift];
+    const int cur_solved = (current & before_unknowns) == 0 &&
+                           unknown_fills[current >> fill_shift] ==
+                               (uint32_t)((current >> (state_len - unknown_bits)) & fill_mask);
+    const int old_solved = cur->state != HS_UNUSED && (cur->current & before_unknowns) == 0 &&
+                           unknown_fills[cur->current >> fill_shift] ==
+                               (uint32_t)((cur->current >> (state_len - unknown_bits)) & fill_mask);
+    /* one thread at a time: the reservation flag is always free, so the first trial decides */
+    if ((cur_solved && !old_solved) || (!old_solved && cur->predecessors <= max_predecessors)) {
+      cur->current = current;
+      cur->next = next;
+      cur->steps = 1;
+      cur->predecessors = 0;
+      cur->state = state;
+    }
+    resolve_unknowns(state_len, unknown_bits, unknown_fill_bits, current, next, 1, state, unknown_fills, resolved,
+                     sol);
+  }
+}

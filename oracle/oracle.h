@@ -17,6 +17,10 @@
  *                      unknown; start-state rule of test_mem_image_find_solution
  *                      (src/greedysatostop.rs:1203-1268, starts are u << (n-U)).
  *   or_table_u32       identity circuit layout of src/satostop.rs:1788-1915.
+ *   or_hash64, or_hm_* the hash-map data sets of src/satostop.rs:1917-4144 (join_to_hashmap
+ *                      x3, join_hashmap_itself x1, add_to_hashmap x2), transcribed to
+ *                      tests/golden/hashmap_cases.json; entries are placed and looked up with
+ *                      or_hash64, so the add_to_hashmap collisions pin the hash too.
  */
 #ifndef NS_ORACLE_H
 #define NS_ORACLE_H
@@ -66,6 +70,26 @@ int or_satostop_table(const or_circuit* c, uint32_t unknowns, int threads, uint3
                       uint32_t* ends_out, uint64_t* n_stopped, uint64_t* lowest_u);
 /* table[s] = outputs of state s packed (at most 32 outputs, n <= 26) */
 int or_table_u32(const or_circuit* c, uint32_t* table);
+
+/* naivesatostop hash map (src/satostop.rs:127-1150) */
+typedef struct {
+  uint64_t current, next, steps;
+  uint32_t predecessors, state; /* 0 unused, 1 used, 2 stopped, 3 looped */
+} or_hash_entry;
+typedef struct {
+  uint64_t start, end, steps;
+  uint32_t defined;
+} or_hm_solution;
+uint64_t or_hash64(uint32_t bits, uint64_t value);
+void or_hm_join_to(uint32_t state_len, uint32_t arg_bit_place, uint64_t arg, const uint32_t* outputs,
+                   or_hash_entry* hm, uint64_t len);
+void or_hm_join_itself(uint32_t state_len, const or_hash_entry* in, or_hash_entry* out, uint64_t len,
+                       uint32_t unknown_bits, uint32_t unknown_fill_bits, uint32_t* unknown_fills,
+                       uint64_t* resolved, or_hm_solution* sol);
+void or_hm_add(uint32_t state_len, uint32_t arg_bit_place, uint64_t arg, const uint32_t* outputs,
+               or_hash_entry* hm, uint64_t len, uint32_t unknown_bits, uint32_t unknown_fill_bits,
+               uint32_t* unknown_fills, uint64_t* resolved, or_hm_solution* sol, uint32_t max_predecessors,
+               int test);
 
 #ifdef __cplusplus
 }
