@@ -222,7 +222,8 @@ int ns_satostop_hashmap_solve(ns_ctx* ctx, const ns_circuit* c, uint32_t unknown
  * against the reference's data sets): which = 0 join_to_hashmap (src/satostop.rs:186-266),
  * 1 join_hashmap_itself_and_check_solution (391-594, from hm into hm_out), 2
  * add_to_hashmap_and_check_solution (713-942, `test` skips states whose next state is 0 as
- * the reference's test routine does). hm / hm_out: hm_len entries of 32 bytes (HashEntry);
+ * the reference's test routine does), 3 join_to_hashmap then join_hashmap_itself as the
+ * search runs them, in one pass (from hm into hm_out). hm / hm_out: hm_len entries of 32 bytes (HashEntry);
  * outputs: the block's output words, two per state when state_len + 1 > 32; fills: the
  * unknown-fill counters; sol: {resolved_unknowns, start, end, steps: u64; defined: u32}. */
 int ns_hashmap_step(ns_ctx* ctx, uint32_t which, uint32_t state_len, uint32_t arg_bit_place, uint64_t arg,

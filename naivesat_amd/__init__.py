@@ -414,8 +414,10 @@ class Context:
 
     def hashmap_step(self, which: int, state_len: int, arg_bit_place: int, arg: int, unknown_bits: int,
                      unknown_fill_bits: int, max_predecessors: int, test: bool, outputs, hm, hm_out, fills, sol):
-        """One device step of the hash-map search on numpy arrays (hm: 32-byte entries,
-        sol: 40-byte {resolved_unknowns, start, end, steps, defined}); updates them in place."""
+        """One device step of the hash-map search on numpy arrays: which = 0 join_to, 1
+        join_itself (hm -> hm_out), 2 add, 3 join_to + join_itself in one pass (hm -> hm_out).
+        hm: 32-byte entries, sol: 40-byte {resolved_unknowns, start, end, steps, defined};
+        updated in place."""
         out_ptr = _ptr(outputs) if outputs is not None else None
         _check(load_library().ns_hashmap_step(
             self._h, which, state_len, arg_bit_place, arg, unknown_bits, unknown_fill_bits, max_predecessors,

@@ -1,17 +1,20 @@
 // naivesatostop: SATOSTOP answer (start, end, steps) on one HIP device.
 // Command line and printed lines follow the reference's naivesatostop
-// (src/satostop.rs:62-99,1650-1782,4443-4467). The search runs on the device-resident
-// next-state table (at most 31 state bits); the hash-map options are accepted.
+// (src/satostop.rs:62-99,1650-1782,4443-4467). Up to 33 state bits the search runs on the
+// device-resident next-state table (the lowest stopping unknown); beyond that, or with
+// --hashmap-search, it runs the reference's hash-map search (any stopping unknown) with the
+// -e/-H/-U/-P/-C options.
 #include "cli_common.h"
 
 static const char* kUsage =
     "Usage: naivesatostop [OPTIONS] --exec-type <EXEC_TYPE> <CIRCUIT> <UNKNOWNS>\n"
     "  -S, --simple                          lowest stopping unknown, printed as 'Stopped at'\n"
-    "  -e, --elem-inputs <N>                 accepted [default: 24]\n"
-    "  -H, --hashmap-len-bits <N>            accepted [default: 24]\n"
-    "  -U, --unknown-fill-bits <N>           accepted\n"
-    "  -P, --max-predecessors <N>            accepted [default: 0]\n"
-    "  -C, --iters-per-clear-predecessors <N> accepted [default: 20]\n"
+    "  -e, --elem-inputs <N>                 states per block of the hash-map search [default: 24]\n"
+    "  -H, --hashmap-len-bits <N>            log2 of hash-map entries [default: 24]\n"
+    "  -U, --unknown-fill-bits <N>           [default: unknowns - (H - 1), at least 0]\n"
+    "  -P, --max-predecessors <N>            [default: 0]\n"
+    "  -C, --iters-per-clear-predecessors <N> [default: 20]\n"
+    "  -M, --hashmap-search                  hash-map search also at 33 state bits or fewer\n"
     "  -t, --exec-type <TYPE>                cpu|opencl:N|hip:N (all run on one HIP device)\n"
     "  -n, --no-optimize-negs                accepted for compatibility\n"
     "  -G, --opencl-group-len <G>            threads per block of the table build\n"
@@ -28,7 +31,8 @@ int main(int argc, char** argv) {
                             {'t', "exec-type", true},
                             {'n', "no-optimize-negs", false},
                             {'G', "opencl-group-len", true},
-                            {'v', "verify", false}},
+                            {'v', "verify", false},
+                            {'M', "hashmap-search", false}},
                            kUsage);
   if (a.positional.size() != 2) cli::die(std::string("expected <CIRCUIT> <UNKNOWNS>\n") + kUsage);
   if (!a.has("exec-type")) cli::die(std::string("--exec-type is required\n") + kUsage);
@@ -42,20 +46,44 @@ int main(int argc, char** argv) {
   long long U = std::atoll(a.positional[1].c_str());
   if (U < 0 || U > (long long)n) cli::die("unknowns must not exceed the inputs");
   const bool simple = a.has("simple");
+  const bool hashmap = !simple && (n > 33 || a.has("hashmap-search"));
+  if (simple && n > 33) cli::die("--simple takes at most 33 state bits");
+  if (hashmap && n < 10) cli::die("the hash-map search takes at least 10 state bits");
   int dev = cli::device_from_exec_type(a.get("exec-type"), !simple);
+  const long long H = a.num("hashmap-len-bits", 24);
+  long long e = a.num("elem-inputs", 24);
+  if (e >= (long long)n) e = n - 1;
+  // calculate_default_unknown_fill_bits, src/satostop.rs:101-107
+  const long long ufb = a.num("unknown-fill-bits", H - 1 < U ? U - (H - 1) : 0);
   if (!simple) {
-    long long e = a.num("elem-inputs", 24);
-    if (e >= (long long)n) e = n - 1;
     std::printf("Elem inputs: %lld\n", e);
+    if (hashmap) std::printf("HashMap length: %llu bytes, bits=%lld\n", (1ull << H) * 32ull, H);
     std::printf("Unknowns: %lld\n", U);
+    if (hashmap) std::printf("Unknown fill bits: %lld\n", ufb);
     std::printf("Execute in HIP device=%d\n", dev);
   }
   ns_ctx* ctx = nullptr;
   cli::check(ns_ctx_create(dev, &ctx), "device");
-  ns_satostop_opts o{};
-  o.group_len = (uint32_t)a.num("opencl-group-len", 0);
   ns_satostop_result r{};
-  cli::check(ns_satostop_solve(ctx, c, (uint32_t)U, &o, &r, nullptr, nullptr), "search");
+  if (hashmap) {
+    ns_hashmap_opts ho{};
+    ho.hashmap_len_bits = (uint32_t)H;
+    ho.elem_bits = (uint32_t)e;
+    ho.unknown_fill_bits = (int32_t)ufb;
+    ho.max_predecessors = (uint32_t)a.num("max-predecessors", 0);
+    ho.iters_per_clear_predecessors = (uint32_t)a.num("iters-per-clear-predecessors", 20);
+    ns_hashmap_result hr{};
+    cli::check(ns_satostop_hashmap_solve(ctx, c, (uint32_t)U, &ho, &hr), "search");
+    r.n_stopped = hr.found == 1 ? 1 : 0;
+    r.start = hr.start;
+    r.end = hr.end;
+    r.steps = hr.steps;
+    r.jump_ms = hr.wall_ms;
+  } else {
+    ns_satostop_opts o{};
+    o.group_len = (uint32_t)a.num("opencl-group-len", 0);
+    cli::check(ns_satostop_solve(ctx, c, (uint32_t)U, &o, &r, nullptr, nullptr), "search");
+  }
   if (simple) {
     // src/satostop.rs:1776-1781: decimal, zero-padded to the unknown / state widths
     if (r.n_stopped)

@@ -7,9 +7,9 @@
 namespace ns {
 
 // HashEntry (src/satostop.rs:168-174), 32 bytes
-struct HmEntry {
+struct alignas(16) HmEntry {
   unsigned long long current, next, steps;
-  unsigned int predecessors, state;  // state: 0 unused, 1 used, 2 stopped, 3 looped (+4 reserved)
+  unsigned int predecessors, state;  // state: 0 unused, 1 used, 2 stopped, 3 looped
 };
 static_assert(sizeof(HmEntry) == 32, "hash entry layout");
 
@@ -31,12 +31,20 @@ struct HmOutputs {
   unsigned long long stride;
 };
 
+// join_to_hashmap alone, in place
 hipError_t hm_join_to(hipStream_t s, const HmParams& p, unsigned long long arg, const HmOutputs& o, HmEntry* hm,
                       unsigned long long len);
-hipError_t hm_join_itself(hipStream_t s, const HmParams& p, const HmEntry* in, HmEntry* out, unsigned long long len,
-                          unsigned int* fills, HmSolution* sol);
+// join_to_hashmap (when o is given) and join_hashmap_itself from `in` into `out`; new
+// predecessor counts go to jumps (one u32 per slot, zero before) until hm_add folds them in
+hipError_t hm_join(hipStream_t s, const HmParams& p, unsigned long long arg, const HmOutputs* o, const HmEntry* in,
+                   HmEntry* out, unsigned long long len, unsigned int* jumps, unsigned int* fills, HmSolution* sol);
+// add_to_hashmap: claims (one u64 per slot) and jumps are zero before and after; clear
+// zeroes every predecessor count afterwards
 hipError_t hm_add(hipStream_t s, const HmParams& p, unsigned long long arg, const HmOutputs& o, HmEntry* hm,
-                  unsigned int* fills, HmSolution* sol, unsigned int max_predecessors, bool test);
-hipError_t hm_clear_preds(hipStream_t s, HmEntry* hm, unsigned long long len);
+                  unsigned long long len, unsigned int* fills, HmSolution* sol, unsigned long long* claims,
+                  unsigned int* jumps, unsigned int max_predecessors, bool clear, bool test);
+// adds jumps into the entries' predecessor counts (claims must be zero), leaving jumps zero
+hipError_t hm_fold(hipStream_t s, const HmParams& p, HmEntry* hm, unsigned long long len, const unsigned int* fills,
+                   unsigned long long* claims, unsigned int* jumps);
 
 }  // namespace ns

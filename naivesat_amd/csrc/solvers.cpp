@@ -675,17 +675,20 @@ void hashmap_step(Lane& ctx, uint32_t which, uint32_t state_len, uint32_t arg_bi
                   uint32_t unknown_bits, uint32_t unknown_fill_bits, uint32_t max_predecessors, bool test,
                   const uint32_t* outputs, uint64_t output_words, void* hm, void* hm_out, uint64_t hm_len,
                   uint32_t* fills, uint64_t fills_len, void* sol) {
-  if (which > 2 || state_len == 0 || state_len > 63 || hm_len == 0 || (hm_len & (hm_len - 1)))
+  if (which > 3 || state_len == 0 || state_len > 63 || hm_len == 0 || (hm_len & (hm_len - 1)))
     throw Error(ST_INVALID, "bad hash-map step arguments");
   if (unknown_fill_bits > unknown_bits || unknown_bits > state_len || arg_bit_place >= state_len)
     throw Error(ST_INVALID, "bad unknown / block widths");
   uint32_t hbits = 0;
   while ((1ull << hbits) < hm_len) hbits++;
   if (hbits > state_len) throw Error(ST_INVALID, "hash map larger than the state space");
+  const bool uses_block = which != 1;
+  const bool to_out = which == 1 || which == 3;
   const uint64_t wpe = (state_len + 1 + 31) >> 5;
-  if (which != 1 && output_words < (wpe << arg_bit_place)) throw Error(ST_INVALID, "outputs array too small");
+  if (uses_block && output_words < (wpe << arg_bit_place)) throw Error(ST_INVALID, "outputs array too small");
   if (fills_len < (1ull << (unknown_bits - unknown_fill_bits))) throw Error(ST_INVALID, "fills array too small");
-  if (which != 1 && (arg >> (state_len - arg_bit_place)) != 0) throw Error(ST_INVALID, "block index out of range");
+  if (uses_block && (arg >> (state_len - arg_bit_place)) != 0) throw Error(ST_INVALID, "block index out of range");
+  if (to_out && !hm_out) throw Error(ST_INVALID, "the join steps write a second hash map");
   // the kernels index the map and the fill counters by the entries' states
   for (uint64_t i = 0; i < hm_len; i++) {
     const HmEntry& he = static_cast<const HmEntry*>(hm)[i];
@@ -694,9 +697,14 @@ void hashmap_step(Lane& ctx, uint32_t which, uint32_t state_len, uint32_t arg_bi
       throw Error(ST_INVALID, "hash entry state wider than state_len");
   }
   NS_HIP(hipSetDevice(ctx.device));
-  DevBuf d_out, d_hm, d_hm2, d_fills, d_sol;
+  DevBuf d_out, d_hm, d_hm2, d_fills, d_sol, d_side;
   const uint64_t hm_bytes = hm_len * sizeof(HmEntry);
   d_hm.ensure(hm_bytes);
+  // claims (u64 per slot) and jumps (u32 per slot), zero
+  d_side.ensure(hm_len * 12);
+  unsigned long long* claims = d_side.as<unsigned long long>();
+  unsigned int* jumps = reinterpret_cast<unsigned int*>(claims + hm_len);
+  NS_HIP(launch_fill_u32(ctx.stream, d_side.as<unsigned int>(), 0u, hm_len * 3));
   d_fills.ensure(fills_len * 4);
   d_sol.ensure(sizeof(HmSolution));
   NS_HIP(hipMemcpyAsync(d_hm.p, hm, hm_bytes, hipMemcpyHostToDevice, ctx.stream));
@@ -704,24 +712,26 @@ void hashmap_step(Lane& ctx, uint32_t which, uint32_t state_len, uint32_t arg_bi
   NS_HIP(hipMemcpyAsync(d_sol.p, sol, sizeof(HmSolution), hipMemcpyHostToDevice, ctx.stream));
   HmParams hp{state_len, arg_bit_place, hbits, unknown_bits, unknown_fill_bits};
   HmOutputs ho{nullptr, nullptr, wpe};
-  if (which != 1) {
+  if (uses_block) {
     d_out.ensure(output_words * 4);
     NS_HIP(hipMemcpyAsync(d_out.p, outputs, output_words * 4, hipMemcpyHostToDevice, ctx.stream));
     ho.lo = d_out.as<unsigned int>();
     ho.hi = wpe == 2 ? ho.lo + 1 : nullptr;
   }
+  unsigned int* dfills = d_fills.as<unsigned int>();
+  HmSolution* dsol = d_sol.as<HmSolution>();
   if (which == 0) {
     NS_HIP(hm_join_to(ctx.stream, hp, arg, ho, d_hm.as<HmEntry>(), hm_len));
-  } else if (which == 1) {
-    d_hm2.ensure(hm_bytes);
-    NS_HIP(hm_join_itself(ctx.stream, hp, d_hm.as<HmEntry>(), d_hm2.as<HmEntry>(), hm_len,
-                          d_fills.as<unsigned int>(), d_sol.as<HmSolution>()));
+  } else if (which == 2) {
+    NS_HIP(hm_add(ctx.stream, hp, arg, ho, d_hm.as<HmEntry>(), hm_len, dfills, dsol, claims, jumps, max_predecessors,
+                  false, test));
   } else {
-    NS_HIP(hm_add(ctx.stream, hp, arg, ho, d_hm.as<HmEntry>(), d_fills.as<unsigned int>(), d_sol.as<HmSolution>(),
-                  max_predecessors, test));
+    d_hm2.ensure(hm_bytes);
+    NS_HIP(hm_join(ctx.stream, hp, arg, which == 3 ? &ho : nullptr, d_hm.as<HmEntry>(), d_hm2.as<HmEntry>(), hm_len,
+                   jumps, dfills, dsol));
+    NS_HIP(hm_fold(ctx.stream, hp, d_hm2.as<HmEntry>(), hm_len, dfills, claims, jumps));
   }
-  NS_HIP(hipMemcpyAsync(which == 1 ? hm_out : hm, which == 1 ? d_hm2.p : d_hm.p, hm_bytes, hipMemcpyDeviceToHost,
-                        ctx.stream));
+  NS_HIP(hipMemcpyAsync(to_out ? hm_out : hm, to_out ? d_hm2.p : d_hm.p, hm_bytes, hipMemcpyDeviceToHost, ctx.stream));
   NS_HIP(hipMemcpyAsync(fills, d_fills.p, fills_len * 4, hipMemcpyDeviceToHost, ctx.stream));
   NS_HIP(hipMemcpyAsync(sol, d_sol.p, sizeof(HmSolution), hipMemcpyDeviceToHost, ctx.stream));
   NS_HIP(hipStreamSynchronize(ctx.stream));
@@ -773,10 +783,15 @@ void satostop_hashmap_solve(Lane& ctx, const Circuit& c, uint32_t unknowns, cons
   ctx.tab_b.ensure(hm_len * sizeof(HmEntry));
   ctx.tab_c.ensure(hm_len * sizeof(HmEntry));
   const uint64_t nfills = 1ull << (unknowns - ufb);
-  ctx.tab_d.ensure(nfills * 4 + 256);
+  // tab_d: the fill counters, then per hash slot a claim word (u64) and a jump count (u32)
+  const uint64_t fills_bytes = (nfills * 4 + 255) / 256 * 256;
+  ctx.tab_d.ensure(fills_bytes + hm_len * 12);
   HmEntry* hm1 = ctx.tab_b.as<HmEntry>();
   HmEntry* hm2 = ctx.tab_c.as<HmEntry>();
   unsigned int* fills = ctx.tab_d.as<unsigned int>();
+  unsigned long long* claims = reinterpret_cast<unsigned long long*>(ctx.tab_d.as<char>() + fills_bytes);
+  unsigned int* jumps = reinterpret_cast<unsigned int*>(claims + hm_len);
+  NS_HIP(launch_fill_u32(ctx.stream, reinterpret_cast<unsigned int*>(claims), 0u, hm_len * 3));
   HmSolution* sol = reinterpret_cast<HmSolution*>(ctx.stats.as<char>() + 128);
   NS_HIP(launch_fill_u64(ctx.stream, reinterpret_cast<unsigned long long*>(hm1), 0ull, hm_len * 4));
   NS_HIP(launch_fill_u64(ctx.stream, reinterpret_cast<unsigned long long*>(hm2), 0ull, hm_len * 4));
@@ -801,14 +816,13 @@ void satostop_hashmap_solve(Lane& ctx, const Circuit& c, uint32_t unknowns, cons
         void* extra[6] = {&T, &flag, &wpl_v, &last, &valid, &small};
         launch_range(ctx, kern[h].mod->fn, g, w0, block_words, extra, kern[h].native ? 6 : 2, w0 + block_words);
       }
-      // the reference's process(): join_to, join_itself, add, clear, swap
-      NS_HIP(hm_join_to(ctx.stream, hp, arg, ho, hm1, hm_len));
-      NS_HIP(hm_join_itself(ctx.stream, hp, hm1, hm2, hm_len, fills, sol));
-      NS_HIP(hm_add(ctx.stream, hp, arg, ho, hm2, fills, sol, o.max_predecessors, false));
-      if (++iter >= clear_every) {
-        NS_HIP(hm_clear_preds(ctx.stream, hm2, hm_len));
-        iter = 0;
-      }
+      // the reference's process() (src/satostop.rs:1211-1257): join_to and join_itself in
+      // one pass, add (which also clears the predecessor counts every clear_every blocks),
+      // swap
+      NS_HIP(hm_join(ctx.stream, hp, arg, &ho, hm1, hm2, hm_len, jumps, fills, sol));
+      const bool clear = ++iter >= clear_every;
+      if (clear) iter = 0;
+      NS_HIP(hm_add(ctx.stream, hp, arg, ho, hm2, hm_len, fills, sol, claims, jumps, o.max_predecessors, clear, false));
       std::swap(hm1, hm2);
       r.blocks++;
       NS_HIP(hipMemcpyAsync(&host, sol, sizeof(host), hipMemcpyDeviceToHost, ctx.stream));

@@ -96,3 +96,27 @@ def test_naivesatostop_verify_walk(tmp_path):
         assert f"Verified: end={r['ends'][lo]} steps={r['steps'][lo]}" in out and "INCORRECT" not in out
     else:
         assert "Unsatisfiable!" in out
+
+
+def test_naivesatostop_hashmap_search(tmp_path):
+    # 34 state bits (past the table's 33): the reference's hash-map search, with the next-state
+    # outputs in two 32-bit words; unknown starts are multiples of 2^32
+    n, U, nxt = 34, 2, 0x2_1234_5678
+    p = 1 << (n - U)
+    f = tmp_path / "stop34.circuit"
+    f.write_text(mk(point_stop_circuit(n, p, nxt)).text())
+    out = run("naivesatostop", "-t", "opencl:0", "-e", 30, "-H", 20, "-v", f, U)
+    assert "HashMap length: 33554432 bytes, bits=20" in out and "Unknown fill bits: 0" in out
+    assert f"Solution: Solution {{ start: {p}, end: {nxt}, steps: 1 }}" in out
+    assert "Verified: end=%d steps=1" % nxt in out and "INCORRECT" not in out
+    # STOP at a state that is no unknown start: every unknown start is a fixed point
+    g = tmp_path / "nostop34.circuit"
+    g.write_text(mk(point_stop_circuit(n, p + 1, nxt)).text())
+    assert "Unsatisfiable!" in run("naivesatostop", "-t", "opencl:0", "-e", 30, "-H", 20, g, U)
+    # --hashmap-search below 34 state bits: the reference's mem-image case 2 again
+    n, U, p, nxt = 24, 11, (3 << 20) + 79 * 8192, 58591
+    h = tmp_path / "stop24.circuit"
+    h.write_text(mk(point_stop_circuit(n, p, nxt)).text())
+    out = run("naivesatostop", "-t", "opencl:0", "-M", "-e", 16, "-H", 18, h, U)
+    assert "Unknown fill bits: 0" in out
+    assert f"Solution: Solution {{ start: {p}, end: {nxt}, steps: 1 }}" in out

@@ -53,6 +53,34 @@ def test_join_hashmap_itself_reference_data(ctx):
     assert sol_dict(sol) == case["expected_solution"]
 
 
+def test_fused_join_equals_join_to_then_join_itself(ctx):
+    # the search's one-pass join against the oracle's two steps on the reference's join_to
+    # data sets, with every unknown start (the low half of the state bits zero) resolving
+    for case in load()["join_to_hashmap"]:
+        sl, U = case["state_len"], case["state_len"] // 2
+        outputs = u32_array(case["outputs"])
+        hm = hashmap(case["hashmap"])
+        mid = hm.copy()
+        oracle.hm_join_to(sl, case["arg_bit_place"], case["arg"], outputs, mid)
+        want = np.zeros_like(hm)
+        st = oracle.HashMapState(np.zeros(1 << U, np.uint32))
+        oracle.hm_join_itself(sl, mid, want, U, 0, st)
+        out = np.zeros_like(hm)
+        fills = np.zeros(1 << U, np.uint32)
+        sol = sol_array()
+        ctx.hashmap_step(3, sl, case["arg_bit_place"], case["arg"], U, 0, 0, False, outputs, hm, out, fills, sol)
+        assert np.array_equal(out, want), sl
+        assert np.array_equal(fills, st.unknown_fills) and int(sol[0]) == st.resolved.value
+        assert (sol_dict(sol) is None) == (st.solution() is None)
+        # the device's own two steps agree
+        two = hm.copy()
+        ctx.hashmap_step(0, sl, case["arg_bit_place"], case["arg"], U, 0, 0, False, outputs, two, None,
+                         np.zeros(1 << U, np.uint32), sol_array())
+        out2 = np.zeros_like(hm)
+        ctx.hashmap_step(1, sl, 0, 0, U, 0, 0, False, None, two, out2, np.zeros(1 << U, np.uint32), sol_array())
+        assert np.array_equal(out2, want), sl
+
+
 def test_add_to_hashmap_reference_data(ctx):
     for case in load()["add_to_hashmap"]:
         hm = hashmap(case["hashmap"])
