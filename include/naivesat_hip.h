@@ -154,6 +154,7 @@ int ns_qsat_solve(ns_ctx* ctx, const ns_circuit* c, const uint8_t* quants_all,
  * is u << (n - unknowns). For every u the engine finds whether iterating the circuit from
  * its start reaches STOP and after how many evaluations (the stopping one included). */
 #define NS_STOP_FIRST_PASS 1u /* end after the first jump pass where some start stopped */
+#define NS_STOP_PASSES 2u     /* full search by passes over the whole table, not waypoint walks */
 typedef struct {
   uint32_t flags;
   uint32_t max_passes;     /* 0 = n (enough for every path) */
@@ -177,7 +178,15 @@ typedef struct {
   double compile_ms;
   double wall_ms;
   uint64_t walk_jumps; /* per start, up to this many jumps of 2^P evaluations follow the passes */
-  double walk_ms;      /* kernel time of those walks */
+  double walk_ms;      /* kernel time of those walks (waypoint method: of the waypoint walks) */
+  /* The full search (no NS_STOP_FIRST_PASS, max_passes 0 or n) runs as waypoint walks
+   * when the unknowns allow: one walk from every state whose low waypoint_bits bits are
+   * zero (the starts among them) to the next such state or STOP, then pointer jumping
+   * over the 2^(n - waypoint_bits) walk records (waypoint_passes passes, jump_ms). */
+  uint32_t method;          /* 0 passes over the table + walks, 1 waypoint walks */
+  uint32_t waypoint_bits;
+  uint32_t waypoint_passes;
+  uint32_t reserved;
 } ns_satostop_result;
 /* steps_out / ends_out (optional): 2^unknowns entries each; steps 0 = never stops.
  * These 32-bit arrays need n <= 31; ns_satostop_solve_wide takes up to 33 state bits. */

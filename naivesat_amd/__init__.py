@@ -25,6 +25,7 @@ BIN_DIR = os.path.join(_HERE, "bin")
 
 NS_SAT_STOP_AT_FIRST = 1
 NS_STOP_FIRST_PASS = 1
+NS_STOP_PASSES = 2
 FUNC_NAMES = ("and", "nor", "nimpl", "xor")
 
 
@@ -69,7 +70,9 @@ class _StopResult(C.Structure):
                 ("steps", C.c_uint64), ("cone_gates", C.c_uint32), ("valu_ops", C.c_uint32),
                 ("backend", C.c_uint32), ("vgpr_class", C.c_uint32),
                 ("build_ms", C.c_double), ("jump_ms", C.c_double), ("compile_ms", C.c_double),
-                ("wall_ms", C.c_double), ("walk_jumps", C.c_uint64), ("walk_ms", C.c_double)]
+                ("wall_ms", C.c_double), ("walk_jumps", C.c_uint64), ("walk_ms", C.c_double),
+                ("method", C.c_uint32), ("waypoint_bits", C.c_uint32), ("waypoint_passes", C.c_uint32),
+                ("reserved", C.c_uint32)]
 
 
 class _HmOpts(C.Structure):
@@ -387,10 +390,12 @@ class Context:
 
     def satostop_solve(self, c: Circuit, unknowns: int, first_pass: bool = False, max_passes: int = 0,
                        group_len: int = 0, words_per_lane: int = 0, arrays: bool = True,
-                       wide: bool = True) -> dict:
+                       wide: bool = True, passes: bool = False) -> dict:
         """Per-start steps/ends arrays are uint64 (ns_satostop_solve_wide, up to 33 state
-        bits) or, with wide=False, uint32 (ns_satostop_solve, up to 31)."""
-        o = _StopOpts(NS_STOP_FIRST_PASS if first_pass else 0, max_passes, group_len, words_per_lane)
+        bits) or, with wide=False, uint32 (ns_satostop_solve, up to 31). passes=True runs the
+        full search as passes over the whole table instead of waypoint walks."""
+        flags = (NS_STOP_FIRST_PASS if first_pass else 0) | (NS_STOP_PASSES if passes else 0)
+        o = _StopOpts(flags, max_passes, group_len, words_per_lane)
         r = _StopResult()
         dt = np.uint64 if wide else np.uint32
         steps = np.zeros(1 << unknowns, dt) if arrays else None

@@ -42,7 +42,7 @@ int main(int argc, char** argv) {
   ns_satostop_result r{};
   cli::check(ns_satostop_solve(ctx, c, (uint32_t)U, &o, &r, nullptr, nullptr), "search");
   for (uint32_t i = 0; i < r.passes; i++) std::printf("Joining nexts: Stage: %u / %u\n", i, n);
-  std::printf("Time: %g\n", (r.build_ms + r.jump_ms) / 1000.0);
+  std::printf("Time: %g\n", (r.build_ms + r.jump_ms + r.walk_ms) / 1000.0);
   if (a.has("all")) std::printf("Stopping unknowns: %llu\n", (unsigned long long)r.n_stopped);
   if (r.n_stopped) {
     std::printf("Solution: Solution { start: %llu, end: %llu }\n", (unsigned long long)r.start,

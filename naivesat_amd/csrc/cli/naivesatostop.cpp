@@ -92,7 +92,7 @@ int main(int argc, char** argv) {
     else
       std::printf("Unsatisfiable!\n");
   } else {
-    std::printf("Time: %g\n", (r.build_ms + r.jump_ms) / 1000.0);
+    std::printf("Time: %g\n", (r.build_ms + r.jump_ms + r.walk_ms) / 1000.0);
     if (r.n_stopped) {
       std::printf("Solution: Solution { start: %llu, end: %llu, steps: %llu }\n", (unsigned long long)r.start,
                   (unsigned long long)r.end, (unsigned long long)r.steps);

@@ -367,8 +367,9 @@ std::string gen_sat_kernel(const Circuit& c, const Geometry& g, GenInfo* info) {
         "      c64 += __shfl_xor(c64, off);\n"
         "    }\n"
         "    if (lane == 0u) {\n"
-        "      atomicAdd(&res[0], c64);\n"
-        "      atomicMin(&res[1], best);\n"
+        "      atomicAdd(&res[512u + ((blockIdx.x & 63u) << 4)], c64);\n"
+        "      if (best < __hip_atomic_load(&res[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))\n"
+        "        atomicMin(&res[1], best);\n"
         "    }\n"
         "  }\n}\n";
   return os.str();
@@ -493,8 +494,9 @@ std::string gen_qsat_kernel(const Circuit& c, const std::vector<uint8_t>& q, con
           "      c64 += __shfl_xor(c64, off);\n"
           "    }\n"
           "    if (lane == 0u) {\n"
-          "      atomicAdd(&stats[0], c64);\n"
-          "      atomicMin(&stats[1], best);\n"
+          "      atomicAdd(&stats[512u + ((blockIdx.x & 63u) << 4)], c64);\n"
+          "      if (best < __hip_atomic_load(&stats[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))\n"
+          "        atomicMin(&stats[1], best);\n"
           "    }\n"
           "  }\n}\n";
   }
@@ -563,7 +565,11 @@ std::string gen_table_kernel(const Circuit& c, const Geometry& g, GenInfo* info)
     os << "    }\n";
   }
   os << "  }\n";
-  os << "  if (last_any != 0u) atomicOr(anylast, 1u);\n}\n";
+  // one flag write per wave, none once it is set (atomics on one word serialize)
+  os << "  const unsigned long long any_b = __ballot(last_any != 0u);\n"
+        "  if (any_b != 0ull && (threadIdx.x & 63u) == (unsigned int)(__ffsll((long long)any_b) - 1) &&\n"
+        "      __hip_atomic_load(anylast, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u)\n"
+        "    atomicOr(anylast, 1u);\n}\n";
   return os.str();
 }
 

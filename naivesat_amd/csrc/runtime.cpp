@@ -279,7 +279,7 @@ Lane::Lane(int dev, Jit& j) : device(dev), jit(j) {
   NS_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   NS_HIP(hipEventCreate(&ev0));
   NS_HIP(hipEventCreate(&ev1));
-  stats.ensure(4096);
+  stats.ensure(16384);  // counters and flags; search-kernel count slots from byte 4096
 }
 
 void Lane::put_graph(const std::string& key, hipGraphExec_t exec, std::shared_ptr<JitModule> mod) {

@@ -196,7 +196,7 @@ void serve(Context& ctx, const std::vector<ServeRequest>& reqs, const ns_serve_o
           res.v2 = tr.steps;
           res.v3 = tr.end;
           res.compile_ms = tr.compile_ms;
-          res.kernel_ms = tr.build_ms + tr.jump_ms;
+          res.kernel_ms = tr.build_ms + tr.jump_ms + tr.walk_ms;
         }
         res.status = ST_OK;
       } catch (const Error& e) {

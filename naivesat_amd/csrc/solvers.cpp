@@ -95,6 +95,37 @@ uint32_t word_valid_mask(uint32_t n) { return n >= 5 ? 0xffffffffu : (uint32_t)(
 
 using Chosen = BuiltKernel;
 
+// Search counters in ctx.stats: [0] count, [1] lowest hit, and from u64 index 512 on 64
+// count slots 16 words apart that the search kernels' waves add to (by block), so that
+// many hits do not queue on one word. The count is stats[0] plus the slots.
+constexpr uint64_t kSlot0 = 512, kSlots = 64, kSlotStride = 16;
+
+void reset_counts(Lane& ctx) {
+  unsigned long long* st = ctx.stats.as<unsigned long long>();
+  NS_HIP(launch_fill_u64(ctx.stream, st, 0ull, 1));
+  NS_HIP(launch_fill_u64(ctx.stream, st + 1, ~0ull, 1));
+  NS_HIP(launch_fill_u64(ctx.stream, st + kSlot0, 0ull, kSlots * kSlotStride));
+}
+
+// enqueues the copy; host[0..1] hold (count, lowest) after read_counts_done
+struct CountsCopy {
+  unsigned long long head[2];
+  unsigned long long slots[kSlots * kSlotStride];
+};
+
+void read_counts_async(Lane& ctx, CountsCopy& cc) {
+  const unsigned long long* st = ctx.stats.as<unsigned long long>();
+  NS_HIP(hipMemcpyAsync(cc.head, st, sizeof(cc.head), hipMemcpyDeviceToHost, ctx.stream));
+  NS_HIP(hipMemcpyAsync(cc.slots, st + kSlot0, sizeof(cc.slots), hipMemcpyDeviceToHost, ctx.stream));
+}
+
+void read_counts_done(const CountsCopy& cc, unsigned long long host[2]) {
+  unsigned long long total = cc.head[0];
+  for (uint64_t s = 0; s < kSlots; s++) total += cc.slots[s * kSlotStride];
+  host[0] = total;
+  host[1] = cc.head[1];
+}
+
 // 128-bit fingerprint of a circuit's content (two independent 64-bit hashes)
 std::string circuit_fingerprint(const Circuit& c) {
   uint64_t h1 = 0xcbf29ce484222325ull, h2 = 0x9e3779b97f4a7c15ull;
@@ -205,8 +236,7 @@ void sat_solve(Lane& ctx, const Circuit& c, const ns_sat_opts& o, ns_sat_result&
       NS_HIP(hipStreamBeginCapture(ctx.stream, hipStreamCaptureModeThreadLocal));
       hipGraph_t graph = nullptr;
       try {
-        NS_HIP(launch_fill_u64(ctx.stream, res, 0ull, 1));
-        NS_HIP(launch_fill_u64(ctx.stream, res + 1, ~0ull, 1));
+        reset_counts(ctx);
         for (uint64_t a = 0; a < total_words; a += cw) {
           uint64_t words = total_words - a < cw ? total_words - a : cw;
           void* extra[3] = {&res, &wpl, &valid};
@@ -226,8 +256,10 @@ void sat_solve(Lane& ctx, const Circuit& c, const ns_sat_opts& o, ns_sat_result&
     NS_HIP(hipEventRecord(ctx.ev0, ctx.stream));
     NS_HIP(hipGraphLaunch(exec, ctx.stream));
     NS_HIP(hipEventRecord(ctx.ev1, ctx.stream));
-    NS_HIP(hipMemcpyAsync(host, res, sizeof(host), hipMemcpyDeviceToHost, ctx.stream));
+    CountsCopy cc;
+    read_counts_async(ctx, cc);
     NS_HIP(hipStreamSynchronize(ctx.stream));
+    read_counts_done(cc, host);
     r.launches = (uint32_t)((total_words + cw - 1) / cw);
     r.count = host[0];
     r.found = host[0] != 0;
@@ -236,8 +268,7 @@ void sat_solve(Lane& ctx, const Circuit& c, const ns_sat_opts& o, ns_sat_result&
     r.wall_ms = ms_since(t0);
     return;
   }
-  NS_HIP(launch_fill_u64(ctx.stream, res, 0ull, 1));
-  NS_HIP(launch_fill_u64(ctx.stream, res + 1, ~0ull, 1));
+  reset_counts(ctx);
   NS_HIP(hipEventRecord(ctx.ev0, ctx.stream));
   if (tune && !ch.tuned_wpl) {
     static const uint32_t cand[5] = {1, 2, 4, 8, 16};
@@ -276,17 +307,20 @@ void sat_solve(Lane& ctx, const Circuit& c, const ns_sat_opts& o, ns_sat_result&
     r.launches++;
     w0 += cw;
     if (stop_first) {
+      // some hit so far: the lowest one is set
       NS_HIP(hipMemcpyAsync(host, res, sizeof(host), hipMemcpyDeviceToHost, ctx.stream));
       NS_HIP(hipStreamSynchronize(ctx.stream));
-      if (host[0] != 0) {
+      if (host[1] != ~0ull) {
         r.complete = w0 >= total_words;
         break;
       }
     }
   }
   NS_HIP(hipEventRecord(ctx.ev1, ctx.stream));
-  NS_HIP(hipMemcpyAsync(host, res, sizeof(host), hipMemcpyDeviceToHost, ctx.stream));
+  CountsCopy cc;
+  read_counts_async(ctx, cc);
   NS_HIP(hipStreamSynchronize(ctx.stream));
+  read_counts_done(cc, host);
   kms = ctx.elapsed_ms();
   r.count = host[0];
   r.found = host[0] != 0;
@@ -340,8 +374,7 @@ void qsat_solve(Lane& ctx, const Circuit& c, const std::vector<uint8_t>& q, cons
   for (uint32_t l = 0; l < 32 && l < n; l++)
     if (q[n - 1 - l]) opm |= 1u << l;
   unsigned long long* stats = ctx.stats.as<unsigned long long>();
-  NS_HIP(launch_fill_u64(ctx.stream, stats, 0ull, 1));
-  NS_HIP(launch_fill_u64(ctx.stream, stats + 1, ~0ull, 1));
+  reset_counts(ctx);
   const uint64_t cw = chunk_words(o.elem_bits, g, total_words);
   unsigned int* dev_bitmap = nullptr;  // final 2^fb-bit array on the device
   NS_HIP(hipEventRecord(ctx.ev0, ctx.stream));
@@ -399,12 +432,14 @@ void qsat_solve(Lane& ctx, const Circuit& c, const std::vector<uint8_t>& q, cons
   }
   NS_HIP(hipEventRecord(ctx.ev1, ctx.stream));
   unsigned long long host[2];
-  NS_HIP(hipMemcpyAsync(host, stats, sizeof(host), hipMemcpyDeviceToHost, ctx.stream));
+  CountsCopy cc;
+  read_counts_async(ctx, cc);
   if (bitmap && dev_bitmap) {
     uint64_t need = fb >= 5 ? (1ull << (fb - 5)) : 1ull;
     NS_HIP(hipMemcpyAsync(bitmap, dev_bitmap, need * 4, hipMemcpyDeviceToHost, ctx.stream));
   }
   NS_HIP(hipStreamSynchronize(ctx.stream));
+  read_counts_done(cc, host);
   if (bitmap && fb < 5) bitmap[0] &= (uint32_t)((1ull << (1u << fb)) - 1);
   r.kernel_ms = ctx.elapsed_ms();
   r.witness_count = host[0];
@@ -525,6 +560,76 @@ Circuit with_outputs(const Circuit& c, uint32_t from, uint32_t to) {
 
 }  // namespace
 
+// Waypoint spacing 2^k: at most 2^22 waypoints (their records stay cache-resident), states
+// 2^8 apart where the state space allows, and k <= n - unknowns so that every start is a
+// waypoint. NS_WAYPOINT_BITS overrides it.
+uint32_t waypoint_bits(uint32_t n, uint32_t unknowns) {
+  if (const char* e = std::getenv("NS_WAYPOINT_BITS")) {
+    const long v = std::atol(e);
+    if (v >= 0 && v <= (long)n) return (uint32_t)v;
+  }
+  const uint32_t want = std::max(n > 22 ? n - 22 : 0u, std::min(8u, n));
+  return std::min(n - unknowns, want);
+}
+
+namespace {
+
+// The full search as waypoint walks (see k_wp_walk): fills the per-start arrays and
+// host[0..1] = (stopping starts, lowest one). Returns false, with nothing changed, when a
+// walk exceeds its evaluation limit (a long path or cycle without waypoints); the passes
+// then answer instead. split: the wide table's split (0 = one u32 table).
+bool waypoint_search(Lane& ctx, uint32_t n, uint32_t unknowns, uint32_t k, uint32_t split, ns_satostop_result& r,
+                     unsigned long long* d_steps, unsigned long long* d_ends, unsigned long long* stats,
+                     unsigned long long host[2]) {
+  const uint64_t count = 1ull << (n - k);
+  uint64_t limit = 1ull << 18;
+  if (const char* e = std::getenv("NS_WAYPOINT_LIMIT")) {
+    const long long v = std::atoll(e);
+    if (v > 0) limit = (uint64_t)v;
+  }
+  ctx.tab_b.ensure(count * 16);
+  ctx.tab_c.ensure(count * 16);
+  void* cur = ctx.tab_b.p;
+  void* nxt = ctx.tab_c.p;
+  unsigned int* flag = reinterpret_cast<unsigned int*>(ctx.stats.as<char>() + 192);
+  NS_HIP(launch_fill_u32(ctx.stream, flag, 0u, 1));
+  NS_HIP(hipEventRecord(ctx.ev0, ctx.stream));
+  NS_HIP(launch_wp_walk(ctx.stream, ctx.tab_a.as<unsigned int>(), split ? ctx.tab_e.as<unsigned int>() : nullptr,
+                        split, n, k, count, limit, cur, flag));
+  NS_HIP(hipEventRecord(ctx.ev1, ctx.stream));
+  unsigned int hf = 0;
+  NS_HIP(hipMemcpyAsync(&hf, flag, 4, hipMemcpyDeviceToHost, ctx.stream));
+  NS_HIP(hipStreamSynchronize(ctx.stream));
+  const double walk_ms = ctx.elapsed_ms();
+  if (hf != 0) return false;
+  r.method = 1;
+  r.waypoint_bits = k;
+  r.walk_ms = walk_ms;
+  // a path visits at most 2^(n-k) waypoints before it repeats one
+  for (uint32_t p = 0; p < n - k; p++) {
+    NS_HIP(launch_fill_u32(ctx.stream, flag, 0u, 1));
+    NS_HIP(hipEventRecord(ctx.ev0, ctx.stream));
+    NS_HIP(launch_wp_jump(ctx.stream, cur, nxt, count, flag));
+    NS_HIP(hipEventRecord(ctx.ev1, ctx.stream));
+    NS_HIP(hipMemcpyAsync(&hf, flag, 4, hipMemcpyDeviceToHost, ctx.stream));
+    NS_HIP(hipStreamSynchronize(ctx.stream));
+    const double ms = ctx.elapsed_ms();
+    ctx.pass_ms.push_back(ms);
+    r.jump_ms += ms;
+    r.waypoint_passes++;
+    std::swap(cur, nxt);
+    if (hf == 0) break;
+  }
+  NS_HIP(launch_fill_u64(ctx.stream, stats, 0ull, 1));
+  NS_HIP(launch_fill_u64(ctx.stream, stats + 1, ~0ull, 1));
+  NS_HIP(launch_wp_starts(ctx.stream, cur, n, unknowns, k, d_steps, d_ends, stats));
+  NS_HIP(hipMemcpyAsync(host, stats, 16, hipMemcpyDeviceToHost, ctx.stream));
+  NS_HIP(hipStreamSynchronize(ctx.stream));
+  return true;
+}
+
+}  // namespace
+
 void satostop_solve(Lane& ctx, const Circuit& c, uint32_t unknowns, const ns_satostop_opts& o,
                     ns_satostop_result& r, uint64_t* steps_out, uint64_t* ends_out) {
   auto t0 = clk::now();
@@ -581,7 +686,15 @@ void satostop_solve(Lane& ctx, const Circuit& c, uint32_t unknowns, const ns_sat
   const uint64_t walk_jumps = 1ull << (cover - max_passes);
   unsigned long long host[2] = {0, ~0ull};
   const unsigned long long* final_tab = nullptr;
-  if (any) {
+  bool searched = false;
+  // The full search runs as waypoint walks when every start is a waypoint and the records
+  // stay small.
+  if (any && !first_pass_mode && cover == n && (o.flags & NS_STOP_PASSES) == 0) {
+    const uint32_t k = waypoint_bits(n, unknowns);
+    if (k <= n - unknowns && n - k <= 24) searched = waypoint_search(ctx, n, unknowns, k, wide ? split : 0, r, d_steps,
+                                                                       d_ends, stats, host);
+  }
+  if (any && !searched) {
     // even entry count for the two-entries-per-thread pass
     const uint64_t cnt = states < 2 ? 2 : states;
     ctx.tab_b.ensure(cnt * 8);
@@ -643,7 +756,7 @@ void satostop_solve(Lane& ctx, const Circuit& c, uint32_t unknowns, const ns_sat
     r.walk_ms = wms;
     (void)hipEventDestroy(w0);
     (void)hipEventDestroy(w1);
-  } else {
+  } else if (!searched) {
     NS_HIP(launch_fill_u64(ctx.stream, d_steps, 0ull, 2 * starts));
     host[0] = 0;
   }

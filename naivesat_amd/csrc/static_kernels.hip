@@ -210,6 +210,135 @@ __global__ void __launch_bounds__(256) k_starts(const unsigned long long* __rest
   wave_stats(cnt, best, stats);
 }
 
+// ---------------------------------------------------------------------------
+// Waypoint walks: the full search without passes over the whole table.
+//
+// Waypoints are the states whose low k bits are zero; k <= n - unknowns, so every
+// unknown's start state is one. One walk per waypoint follows the next-state table from
+// it, evaluation by evaluation, until the first STOP, the next waypoint or a cycle of
+// non-waypoint states (Brent's test against a state saved at power-of-two distances). Its
+// outcome is a 16-byte record of the waypoint graph:
+//   x = target | kind << 62, y = evaluations
+//   WP_NEXT: target = index of the waypoint reached (its state >> k)
+//   WP_STOP: target = state produced by the stopping evaluation
+//   WP_LOOP: the walk entered a cycle without STOP or waypoint: it never stops
+//   WP_LONG: more than `limit` evaluations (the caller falls back to the passes)
+// Pointer jumping over these records (at most n - k passes, since a path visits each of
+// the 2^(n-k) waypoints at most once before it repeats) resolves every waypoint to STOP
+// or never; a start's answer is its own waypoint's record. Every state's path is followed
+// exactly as in the passes, so the answers are the same; only the order of work differs.
+// ---------------------------------------------------------------------------
+constexpr unsigned int WP_NEXT = 0u, WP_STOP = 1u, WP_LOOP = 2u, WP_LONG = 3u;
+constexpr unsigned long long WP_TARGET = (1ull << 62) - 1ull;
+
+// The table: lo[s] | hi[s] << split (hi null: lo alone holds next state and STOP).
+// A lane that finishes a walk takes its next waypoint (grid stride) at once, so lanes
+// stay busy while walks of different lengths end. A lane also stops, raising the flag,
+// after `budget` evaluations in all, and every lane stops once it sees the flag (checked
+// every 256 evaluations): maps whose walks are far longer than the waypoint spacing end
+// the kernel early and go to the passes.
+__global__ void __launch_bounds__(256) k_wp_walk(const unsigned int* __restrict__ lo,
+                                                 const unsigned int* __restrict__ hi, unsigned int split,
+                                                 unsigned int n, unsigned int k, unsigned long long count,
+                                                 unsigned long long limit, unsigned long long budget,
+                                                 ulonglong2* __restrict__ out, unsigned int* long_flag) {
+  const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
+  const unsigned long long mask = (1ull << n) - 1ull, wmask = (1ull << k) - 1ull;
+  unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  unsigned long long s = i << k, saved = s, steps = 0, power = 1, lam = 0, used = 0;
+  for (;;) {
+    if (++used > budget) {
+      atomicOr(long_flag, 1u);
+      break;
+    }
+    if ((used & 255ull) == 0ull &&
+        __hip_atomic_load(long_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u)
+      break;
+    unsigned long long e = lo[s];
+    if (hi) e |= (unsigned long long)hi[s] << split;
+    steps++;
+    const unsigned long long nx = e & mask;
+    unsigned int kind = 4u;
+    unsigned long long target = 0;
+    if ((e >> n) & 1ull) {
+      kind = WP_STOP;
+      target = nx;
+    } else if ((nx & wmask) == 0ull) {
+      kind = WP_NEXT;
+      target = nx >> k;
+    } else if (nx == saved) {
+      kind = WP_LOOP;
+    } else if (steps >= limit) {
+      kind = WP_LONG;
+      atomicOr(long_flag, 1u);
+    } else {
+      s = nx;
+      if (++lam == power) {
+        saved = s;
+        power <<= 1;
+        lam = 0;
+      }
+    }
+    if (kind != 4u) {
+      out[i] = make_ulonglong2(target | ((unsigned long long)kind << 62), steps);
+      i += stride;
+      if (i >= count) break;
+      s = i << k;
+      saved = s;
+      steps = 0;
+      power = 1;
+      lam = 0;
+    }
+  }
+}
+
+// One pointer-jumping pass over the waypoint records; *pending |= some record still
+// points at a waypoint.
+__global__ void __launch_bounds__(256) k_wp_jump(const ulonglong2* __restrict__ in, ulonglong2* __restrict__ out,
+                                                 unsigned long long count, unsigned int* pending) {
+  const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
+  unsigned int any = 0u;
+  for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
+    ulonglong2 e = in[i];
+    if ((unsigned int)(e.x >> 62) == WP_NEXT) {
+      const ulonglong2 t = in[e.x & WP_TARGET];
+      const unsigned int tk = (unsigned int)(t.x >> 62);
+      e.x = t.x;
+      e.y = tk == WP_NEXT || tk == WP_STOP ? e.y + t.y : 0ull;
+      any |= tk == WP_NEXT ? 1u : 0u;
+    }
+    out[i] = e;
+  }
+  // one flag write per block at most, and none once some block has set it: atomics from
+  // every wave to this one word would serialize
+  if (__syncthreads_or((int)any) && threadIdx.x == 0 &&
+      __hip_atomic_load(pending, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u)
+    atomicOr(pending, 1u);
+}
+
+// Per unknown u: the record of its start's waypoint (u << (n - unknowns - k)).
+__global__ void __launch_bounds__(256) k_wp_starts(const ulonglong2* __restrict__ rec, unsigned int n,
+                                                   unsigned int unknowns, unsigned int k,
+                                                   unsigned long long* steps_out, unsigned long long* ends_out,
+                                                   unsigned long long* stats) {
+  const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
+  const unsigned long long limit = ((1ull << unknowns) + 63ull) & ~63ull;  // whole waves take part
+  unsigned long long cnt = 0, best = ~0ull;
+  for (unsigned long long u = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; u < limit; u += stride) {
+    if (u >= (1ull << unknowns)) continue;
+    const ulonglong2 e = rec[u << (n - unknowns - k)];
+    const bool stop = (unsigned int)(e.x >> 62) == WP_STOP;
+    if (steps_out) steps_out[u] = stop ? e.y : 0ull;
+    if (ends_out) ends_out[u] = stop ? e.x & WP_TARGET : 0ull;
+    if (stop) {
+      cnt++;
+      best = u < best ? u : best;
+    }
+  }
+  wave_stats(cnt, best, stats);
+}
+
 __global__ void k_fill_u64(unsigned long long* p, unsigned long long v, unsigned long long count) {
   const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
   for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride)
@@ -280,6 +409,34 @@ hipError_t launch_starts(hipStream_t s, const unsigned long long* tab, unsigned 
   else
     hipLaunchKernelGGL(k_starts<false>, dim3(looped_blocks(threads, 256)), dim3(256), 0, s, tab, n, unknowns, jumps,
                        steps_out, ends_out, stats);
+  return hipGetLastError();
+}
+
+hipError_t launch_wp_walk(hipStream_t s, const unsigned int* lo, const unsigned int* hi, unsigned int split,
+                          unsigned int n, unsigned int k, unsigned long long count, unsigned long long limit,
+                          void* out, unsigned int* long_flag) {
+  // enough lanes to keep the memory system's queues full; each takes several walks
+  const unsigned long long lanes = count < (1ull << 19) ? count : (1ull << 19);
+  // a lane's walks average 2^k evaluations each on a random map; allow 16 times that
+  // over its walks plus one walk of the full limit
+  const unsigned long long per_lane = (count + lanes - 1) / lanes;
+  const unsigned long long budget = 16ull * per_lane * (1ull << k) + limit;
+  hipLaunchKernelGGL(k_wp_walk, dim3(blocks_for(lanes, 256)), dim3(256), 0, s, lo, hi, split, n, k, count, limit,
+                     budget, static_cast<ulonglong2*>(out), long_flag);
+  return hipGetLastError();
+}
+
+hipError_t launch_wp_jump(hipStream_t s, const void* in, void* out, unsigned long long count, unsigned int* pending) {
+  hipLaunchKernelGGL(k_wp_jump, dim3(looped_blocks(count, 256)), dim3(256), 0, s,
+                     static_cast<const ulonglong2*>(in), static_cast<ulonglong2*>(out), count, pending);
+  return hipGetLastError();
+}
+
+hipError_t launch_wp_starts(hipStream_t s, const void* rec, unsigned int n, unsigned int unknowns, unsigned int k,
+                            unsigned long long* steps_out, unsigned long long* ends_out, unsigned long long* stats) {
+  const unsigned long long threads = ((1ull << unknowns) + 63ull) & ~63ull;
+  hipLaunchKernelGGL(k_wp_starts, dim3(looped_blocks(threads, 256)), dim3(256), 0, s,
+                     static_cast<const ulonglong2*>(rec), n, unknowns, k, steps_out, ends_out, stats);
   return hipGetLastError();
 }
 

@@ -48,6 +48,9 @@ typedef unsigned long long u64;
   const u32 LM2 = 0u - ((lane >> 2) & 1u), LM3 = 0u - ((lane >> 3) & 1u);         \
   const u32 LM4 = 0u - ((lane >> 4) & 1u), LM5 = 0u - ((lane >> 5) & 1u);
 
+// Wave totals into stats: the count goes to one of 64 slots 128 bytes apart from stats[512]
+// on (by block; atomics on one word serialize, and a search of many hits would queue on
+// it), the lowest hit to stats[1], skipped when that is already lower.
 __device__ inline void ns_wave_stats(u64 cnt, u64 best, u64* stats) {
   if (__ballot(cnt != 0ull) == 0ull) return;
   for (int off = 32; off > 0; off >>= 1) {
@@ -56,8 +59,8 @@ __device__ inline void ns_wave_stats(u64 cnt, u64 best, u64* stats) {
     cnt += __shfl_xor(cnt, off);
   }
   if ((threadIdx.x & 63u) == 0u) {
-    atomicAdd(&stats[0], cnt);
-    atomicMin(&stats[1], best);
+    atomicAdd(&stats[512u + ((blockIdx.x & 63u) << 4)], cnt);
+    if (best < __hip_atomic_load(&stats[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMin(&stats[1], best);
   }
 }
 
@@ -224,7 +227,11 @@ extern "C" __global__ void __launch_bounds__(256) {name}(u64 w0, u64 wend, u32* 
       }}
     }}
   }}
-  if (last_any != 0u) atomicOr(anylast, 1u);
+  // one flag write per wave, none once it is set (atomics on one word serialize)
+  const u64 any_b = __ballot(last_any != 0u);
+  if (any_b != 0ull && lane == (u32)(__ffsll((long long)any_b) - 1) &&
+      __hip_atomic_load(anylast, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u)
+    atomicOr(anylast, 1u);
 }}
 '''
 

@@ -137,3 +137,23 @@ def test_graph_replay_matches_eager(ctx):
         assert r["launches"] == 8  # 2^22 words in launches of 2^19
         ro = ctx.sat_solve(other)
         assert (ro["count"], ro["lowest"]) == (ocount, olowest), i
+
+
+def test_dense_solutions_count_exactly(ctx):
+    # most waves of these searches hit: their counts spread over the device's count slots
+    # must add up exactly, on the eager, tuning and graph-replay paths alike
+    n = 30
+    xor = ns.Circuit.from_arrays(n, [0], [1], [3], [n], [0])  # x0 ^ x1: 2^29 hits, lowest 1
+    conj = ns.Circuit.from_arrays(n, [0], [1], [0], [n], [0])  # x0 & x1: 2^28 hits, lowest 3
+    for c, count, lowest in ((xor, 1 << 29, 1), (conj, 1 << 28, 3)):
+        for i in range(4):
+            r = ctx.sat_solve(c, elem_bits=26)
+            assert (r["count"], r["lowest"]) == (count, lowest), i
+    # a random circuit with about half of its assignments satisfying, against the oracle
+    rng = np.random.default_rng(77)
+    d = random_circuit(rng, 24, 120, recent=30)
+    d["in0"] = np.append(d["in0"], d["out_wire"][0]).astype(np.uint32)
+    d["in1"] = np.append(d["in1"], 0).astype(np.uint32)
+    d["func"] = np.append(d["func"], 3).astype(np.uint8)
+    d["out_wire"] = np.array([24 + len(d["func"]) - 1], np.uint32)
+    check(ctx, d)

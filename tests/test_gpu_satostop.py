@@ -33,20 +33,76 @@ def check(ctx, d, U, **kw):
     return r, o
 
 
+@pytest.mark.parametrize("passes", [False, True])
 @pytest.mark.parametrize("n", [1, 2, 3, 5, 6, 8, 11, 14, 17, 20])
-def test_random_next_state_circuits(ctx, n):
+def test_random_next_state_circuits(ctx, n, passes):
     rng = np.random.default_rng(5000 + n)
     d = random_circuit(rng, n, int(rng.integers(n + 1, 4 * n + 20)), nout=n + 1, recent=3 * n + 4)
-    # make STOP rarer: AND the last output with a few more wires
-    check(ctx, d, int(rng.integers(0, n + 1)))
+    r, o = check(ctx, d, int(rng.integers(0, n + 1)), passes=passes)
+    if passes or not r["any_stop"]:
+        assert r["method"] == 0
 
 
+@pytest.mark.parametrize("passes", [False, True])
 @pytest.mark.parametrize("half", [4, 6, 8, 10])
-def test_feistel_workloads(ctx, half):
+def test_feistel_workloads(ctx, half, passes):
     c, target = ns.Circuit.gen_satostop(70 + half, half, 6, 12 * half, min(2 * half, 12))
     d = c.arrays()
-    r, o = check(ctx, d, min(2 * half, 12))
+    r, o = check(ctx, d, min(2 * half, 12), passes=passes)
     assert r["any_stop"] == 1
+    assert r["method"] == (0 if passes else 1)
+
+
+def test_waypoint_spacings(ctx, monkeypatch):
+    # every spacing from one waypoint per state up to the starts' own spacing gives the
+    # oracle's answers (sparse waypoints leave cycles without any, found by the walks)
+    rng = np.random.default_rng(5100)
+    cases = [(random_circuit(rng, 13, 60, nout=14, recent=40), 5),
+             (ns.Circuit.gen_satostop(5101, 7, 6, 70, 6)[0].arrays(), 6)]
+    for d, U in cases:
+        for k in sorted({0, 1, 3, d["n"] - U}):
+            monkeypatch.setenv("NS_WAYPOINT_BITS", str(k))
+            r, _ = check(ctx, d, U)
+            if r["any_stop"]:
+                assert (r["method"], r["waypoint_bits"]) == (1, k)
+
+
+def test_waypoint_walk_limit_falls_back_to_passes(ctx, monkeypatch):
+    # walks longer than the limit hand the search to the passes, with the same answers
+    c, _ = ns.Circuit.gen_satostop(5201, 8, 6, 80, 8)
+    monkeypatch.setenv("NS_WAYPOINT_LIMIT", "3")
+    r, _ = check(ctx, c.arrays(), 8)
+    assert r["method"] == 0 and r["passes"] > 0
+
+
+def test_waypoint_walk_outcomes(ctx, monkeypatch):
+    # next(s) = s | 1, STOP = [s == 769], n = 10, starts 0, 256, 512, 768 (waypoints 2^8
+    # apart): 768 -> 769 stops at its second evaluation; the others reach a fixed point
+    # that is no waypoint and never stop
+    n, p = 10, 769
+    in0, in1, func = [0], [0], [2]  # wire 10 = x0 & ~x0 = 0
+    lits = [(i, (p >> i) & 1 == 1) for i in range(n)]
+    acc, pos = lits[0]
+    for w, pw in lits[1:]:
+        if pos and pw:
+            f, a, b = 0, acc, w
+        elif pos:
+            f, a, b = 2, acc, w
+        elif pw:
+            f, a, b = 2, w, acc
+        else:
+            f, a, b = 1, acc, w
+        in0.append(a)
+        in1.append(b)
+        func.append(f)
+        acc, pos = n + len(func) - 1, True
+    d = {"n": n, "in0": np.array(in0, np.uint32), "in1": np.array(in1, np.uint32), "func": np.array(func, np.uint8),
+         "out_wire": np.array([n] + list(range(1, n)) + [acc], np.uint32),
+         "out_neg": np.array([1] + [0] * (n - 1) + [0 if pos else 1], np.uint8)}
+    monkeypatch.setenv("NS_WAYPOINT_BITS", "8")
+    r, o = check(ctx, d, 2)
+    assert (r["method"], r["waypoint_bits"]) == (1, 8)
+    assert list(o["steps"]) == [0, 0, 0, 2] and int(o["ends"][3]) == 769
 
 
 def test_first_pass_mode_matches_reference_rule(ctx):
@@ -109,14 +165,18 @@ def test_config3_full_size_golden(ctx):
     g = json.load(open(GOLDEN))["cfg3"]
     c, target = ns.Circuit.gen_satostop(4, 15, 6, 300, 14)
     assert target == g["target"]
-    r = ctx.satostop_solve(c, 14, wide=False)  # the golden hashes are of uint32 arrays
-    # P passes over the table plus walks of 2^(30-P) jumps cover 2^30 evaluations
-    assert r["walk_jumps"] << r["passes"] == 1 << 30
-    assert r["n_stopped"] == g["n_stopped"]
-    assert hashlib.sha256(r["steps_arr"].tobytes()).hexdigest() == g["steps_sha256"]
-    assert hashlib.sha256(r["ends_arr"].tobytes()).hexdigest() == g["ends_sha256"]
-    if g["lowest_u"] is not None:
-        assert (r["lowest_u"], r["steps"], r["end"]) == (g["lowest_u"], g["steps"], g["end"])
+    for passes in (False, True):
+        r = ctx.satostop_solve(c, 14, wide=False, passes=passes)  # the golden hashes are of uint32 arrays
+        if passes:
+            # P passes over the table plus walks of 2^(30-P) jumps cover 2^30 evaluations
+            assert r["method"] == 0 and r["walk_jumps"] << r["passes"] == 1 << 30
+        else:
+            assert (r["method"], r["waypoint_bits"]) == (1, 8) and r["waypoint_passes"] <= 22
+        assert r["n_stopped"] == g["n_stopped"]
+        assert hashlib.sha256(r["steps_arr"].tobytes()).hexdigest() == g["steps_sha256"]
+        assert hashlib.sha256(r["ends_arr"].tobytes()).hexdigest() == g["ends_sha256"]
+        if g["lowest_u"] is not None:
+            assert (r["lowest_u"], r["steps"], r["end"]) == (g["lowest_u"], g["steps"], g["end"])
 
 
 def test_mem_image_find_solution_vectors(ctx):
@@ -147,11 +207,11 @@ def test_walks_after_fewer_passes(ctx, monkeypatch, passes):
     monkeypatch.setenv("NS_JUMP_PASSES", str(passes))
     for half, U in ((6, 10), (8, 9)):
         c, _ = ns.Circuit.gen_satostop(120 + half, half, 6, 10 * half, U)
-        r, o = check(ctx, c.arrays(), U)
+        r, o = check(ctx, c.arrays(), U, passes=True)
         assert r["passes"] == passes and r["walk_jumps"] == 1 << (2 * half - passes)
     rng = np.random.default_rng(3100 + passes)
     d = random_circuit(rng, 15, 70, nout=16, recent=40)
-    r, o = check(ctx, d, 11)
+    r, o = check(ctx, d, 11, passes=True)
     assert r["walk_jumps"] == 1 << (15 - passes)
 
 
@@ -178,9 +238,11 @@ def test_wide_layout_matches_oracle(ctx, monkeypatch):
         check(ctx, d, int(rng.integers(0, n + 1)))
     for half in (6, 8):
         c, _ = ns.Circuit.gen_satostop(7200 + half, half, 6, 10 * half, 12)
-        check(ctx, c.arrays(), 12)
+        r, _ = check(ctx, c.arrays(), 12)
+        assert r["method"] == 1
+        check(ctx, c.arrays(), 12, passes=True)
         monkeypatch.setenv("NS_JUMP_PASSES", "3")
-        check(ctx, c.arrays(), 12)
+        check(ctx, c.arrays(), 12, passes=True)
         monkeypatch.delenv("NS_JUMP_PASSES")
 
 
@@ -224,7 +286,12 @@ def test_32_bit_state_space(ctx, monkeypatch):
         if steps[u] <= 1 << 21:
             assert oc.walk(int(u) << (n - U), int(steps[u])) == (int(steps[u]), target)
     # another split between passes and walks gives the same arrays
+    assert (r["method"], r["waypoint_bits"]) == (1, 10)
+    # the passes give the same arrays, also with another split between passes and walks
+    r2 = ctx.satostop_solve(c, U, passes=True)
+    assert r2["method"] == 0
+    assert np.array_equal(r2["steps_arr"], steps) and np.array_equal(r2["ends_arr"], ends)
     monkeypatch.setenv("NS_JUMP_PASSES", "12")
-    r2 = ctx.satostop_solve(c, U)
+    r2 = ctx.satostop_solve(c, U, passes=True)
     assert r2["passes"] == 12
     assert np.array_equal(r2["steps_arr"], steps) and np.array_equal(r2["ends_arr"], ends)
