@@ -14,14 +14,17 @@ Also reported on the same line:
                 FETCH_SIZE / WRITE_SIZE (profiles/traffic.txt), when recorded;
   cpu_baseline  the CPU oracle (C, AVX2, pthreads over the host cores) on a bounded slice
                 of the same search;
-  extra         the SATOSTOP table-pass HBM rate (30-bit Feistel, 2^30-entry table), the
+  extra         the SATOSTOP search over the 30-bit Feistel's 2^30-entry table (waypoint walks,
+                their scattered-read rate against a probe of the same table size, and the
+                passes over the whole table with their HBM rate), one sweep of the hash-map
+                search at 34 state bits, the
                 QSAT search time (32 inputs, 10,000 gates) and the served request stream
                 (4,096 SAT/QSAT/SATOSTOP requests, Poisson 200/s, seed 5): solves/s with every
                 request queued at once and p50/p99 latency with requests arriving in real
                 time, each for 8 concurrent worker lanes and for one-at-a-time solving.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-extra] [--no-serve]
-       python bench.py --only {sat,table,qsat,serve,copy}   (one part, for profiling runs;
+       python bench.py --only {sat,table,hashmap,qsat,serve,copy}   (one part, for profiling runs;
        `copy` times a plain 1 GiB device copy used to calibrate the HBM byte counters)
 """
 import argparse
@@ -109,23 +112,59 @@ def cpu_part(c, cone, threads):
 
 
 def table_part(ns, ctx):
+    """The 2^30-state SATOSTOP search: waypoint walks (the default full search) and, beside
+    them, the passes over the whole table, with the same answers."""
     cs, target = ns.Circuit.gen_satostop(4, 15, 6, 300, 14)
+    ctx.satostop_solve(cs, 14, arrays=False)  # kernel build
     rs = ctx.satostop_solve(cs, 14, arrays=False)
-    passes = ctx.pass_times()
-    jump = passes[1:]  # u64 passes (the first reads the u32 table)
+    rp = ctx.satostop_solve(cs, 14, arrays=False, passes=True)
+    assert (rs["n_stopped"], rs["lowest_u"], rs["steps"], rs["end"]) == \
+        (rp["n_stopped"], rp["lowest_u"], rp["steps"], rp["end"])
+    jump = ctx.pass_times()[1:]  # u64 passes (the first reads the u32 table)
     entries = 1 << 30
     bytes_pass = entries * 24  # read entry + gathered entry + write, 8 B each
     avg = sum(jump) / len(jump)
     gbs = bytes_pass / (avg / 1000.0) / 1e9
-    return {"workload": TABLE_WORKLOAD, "build_ms": rs["build_ms"], "jump_ms": rs["jump_ms"],
-            "passes": rs["passes"], "walk_jumps": rs["walk_jumps"], "walk_ms": rs["walk_ms"],
-            "device_ms": rs["build_ms"] + rs["jump_ms"] + rs["walk_ms"], "wall_ms": rs["wall_ms"],
-            "n_stopped": rs["n_stopped"], "lowest_u": rs["lowest_u"],
-            "steps": rs["steps"], "end": rs["end"], "avg_pass_ms": avg,
-            "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": gbs / HBM_PEAK_GBS,
-                         "traffic": measured_traffic().get(("k_jump_pass", "satostop_n30")),
-                         "kernel": "k_jump_pass", "bytes_per_entry": 24}}
+    # the walks read one 4-byte table entry per evaluation at a scattered index
+    probe = ctx.probe_gather(30, 256, 1 << 20)
+    walk_s = rs["walk_ms"] / 1000.0
+    reads_per_s = rs["walk_evaluations"] / walk_s
+    walk_gbs = rs["walk_evaluations"] * 4 / walk_s / 1e9
+    traffic = measured_traffic()
+    return {"workload": TABLE_WORKLOAD, "method": "waypoint walks", "waypoint_bits": rs["waypoint_bits"],
+            "build_ms": rs["build_ms"], "walk_ms": rs["walk_ms"], "walk_evaluations": rs["walk_evaluations"],
+            "waypoint_passes": rs["waypoint_passes"], "waypoint_jump_ms": rs["jump_ms"],
+            "device_ms": rs["build_ms"] + rs["walk_ms"] + rs["jump_ms"], "wall_ms": rs["wall_ms"],
+            "n_stopped": rs["n_stopped"], "lowest_u": rs["lowest_u"], "steps": rs["steps"], "end": rs["end"],
+            "roofline": {"bound": "hbm", "achieved": walk_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": walk_gbs / HBM_PEAK_GBS, "traffic": traffic.get(("k_wp_walk", "satostop_n30")),
+                         "kernel": "k_wp_walk", "bytes_per_evaluation": 4,
+                         "scattered_reads_per_s": reads_per_s,
+                         "scattered_read_ceiling_per_s": probe["reads_per_s"],
+                         "frac_of_scattered_read_ceiling": reads_per_s / probe["reads_per_s"]},
+            "passes_method": {"device_ms": rp["build_ms"] + rp["jump_ms"] + rp["walk_ms"],
+                              "passes": rp["passes"], "walk_jumps": rp["walk_jumps"], "walk_ms": rp["walk_ms"],
+                              "avg_pass_ms": avg,
+                              "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                           "frac": gbs / HBM_PEAK_GBS,
+                                           "traffic": traffic.get(("k_jump_pass", "satostop_n30")),
+                                           "kernel": "k_jump_pass", "bytes_per_entry": 24}},
+            "speedup_vs_passes": (rp["build_ms"] + rp["jump_ms"] + rp["walk_ms"]) /
+                                 (rs["build_ms"] + rs["walk_ms"] + rs["jump_ms"])}
+
+
+HASHMAP_WORKLOAD = ("SATOSTOP n=34 Feistel (6 rounds x 136 gates, seed 9001), 26 unknowns, hash-map search "
+                    "(2^24 states per block, 2^24-entry map), one sweep of 1,024 blocks")
+
+
+def hashmap_part(ns, ctx):
+    """One sweep of naivesatostop's hash-map search past the table's 33 state bits."""
+    c, _ = ns.Circuit.gen_satostop(9001, 17, 6, 136, 26)
+    r = ctx.satostop_hashmap_solve(c, 26, hashmap_len_bits=24, elem_bits=24, max_sweeps=1)
+    per_block = r["kernel_ms"] / max(1, r["blocks"])
+    return {"workload": HASHMAP_WORKLOAD, "found": r["found"], "blocks": r["blocks"], "kernel_ms": r["kernel_ms"],
+            "ms_per_block": per_block, "states_per_s": (1 << 24) / (per_block / 1000.0),
+            "resolved_unknowns": r["resolved_unknowns"]}
 
 
 def qsat_part(ns, ctx):
@@ -192,7 +231,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--serve-requests", type=int, default=4096)
     ap.add_argument("--serve-workers", type=int, default=8)
-    ap.add_argument("--only", choices=["sat", "table", "qsat", "serve", "copy"])
+    ap.add_argument("--only", choices=["sat", "table", "hashmap", "qsat", "serve", "copy"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -213,6 +252,8 @@ def main():
                 part = sat_part(ns, ctx, args.steps, args.warmup)[2]
             elif args.only == "table":
                 part = table_part(ns, ctx)
+            elif args.only == "hashmap":
+                part = hashmap_part(ns, ctx)
             else:
                 part = qsat_part(ns, ctx)
         print(json.dumps({"only": args.only, **part}))
@@ -229,7 +270,8 @@ def main():
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_part(c, cone, args.cpu_threads or min(os.cpu_count() or 1, 64))
     if not args.no_extra:
-        extra = {"satostop_table": table_part(ns, ctx), "qsat": qsat_part(ns, ctx)}
+        extra = {"satostop_table": table_part(ns, ctx), "satostop_hashmap": hashmap_part(ns, ctx),
+                 "qsat": qsat_part(ns, ctx)}
         if not args.no_serve:
             extra["serve"] = serve_part(ns, args.serve_requests, args.serve_workers)
         line["extra"] = extra

@@ -187,6 +187,7 @@ typedef struct {
   uint32_t waypoint_bits;
   uint32_t waypoint_passes;
   uint32_t reserved;
+  uint64_t walk_evaluations; /* waypoint method: table entries read by the waypoint walks */
 } ns_satostop_result;
 /* steps_out / ends_out (optional): 2^unknowns entries each; steps 0 = never stops.
  * These 32-bit arrays need n <= 31; ns_satostop_solve_wide takes up to 33 state bits. */
@@ -306,6 +307,14 @@ int ns_native_code(const ns_circuit* c, uint32_t kind, uint32_t lane_shift, void
 /* Per-pass timing of the last ns_satostop_solve on this context: pass_ms[i] for jump pass i
  * (HIP events on the context stream). Returns the number of passes written. */
 int ns_satostop_pass_times(ns_ctx* ctx, double* pass_ms, uint32_t cap, uint32_t* written);
+
+/* Diagnostic (no counterpart in the reference): the rate of scattered 4-byte reads from a
+ * 2^table_bits-entry device table, lanes x reads_per_lane reads at independent
+ * pseudo-random indices (flags bit 0: with the non-temporal load hint); *ms = kernel time
+ * of one run (HIP events), *reads = reads made. The ceiling against which the waypoint
+ * walks' table reads are measured. */
+int ns_probe_gather(ns_ctx* ctx, uint32_t table_bits, uint32_t reads_per_lane, uint32_t lanes, uint32_t flags,
+                    double* ms, uint64_t* reads);
 
 #ifdef __cplusplus
 }

@@ -72,7 +72,7 @@ class _StopResult(C.Structure):
                 ("build_ms", C.c_double), ("jump_ms", C.c_double), ("compile_ms", C.c_double),
                 ("wall_ms", C.c_double), ("walk_jumps", C.c_uint64), ("walk_ms", C.c_double),
                 ("method", C.c_uint32), ("waypoint_bits", C.c_uint32), ("waypoint_passes", C.c_uint32),
-                ("reserved", C.c_uint32)]
+                ("reserved", C.c_uint32), ("walk_evaluations", C.c_uint64)]
 
 
 class _HmOpts(C.Structure):
@@ -148,6 +148,7 @@ def load_library() -> C.CDLL:
         "ns_native_encode": (I, [C.c_char_p, C.c_size_t, P, C.c_size_t, C.POINTER(C.c_size_t)]),
         "ns_native_code": (I, [P, U32, U32, P, C.c_size_t, C.POINTER(C.c_size_t), C.POINTER(U32),
                                C.POINTER(U64)]),
+        "ns_probe_gather": (I, [P, U32, U32, U32, U32, C.POINTER(C.c_double), C.POINTER(U64)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -164,7 +165,8 @@ def exported_symbols() -> list:
             "ns_circuit_eval", "ns_circuit_cone", "ns_circuit_free", "ns_gen_sat", "ns_gen_qsat",
             "ns_gen_satostop", "ns_ctx_create", "ns_ctx_destroy", "ns_sat_solve", "ns_qsat_solve",
             "ns_satostop_solve", "ns_table_build", "ns_satostop_pass_times", "ns_native_body", "ns_serve", "ns_native_encode",
-            "ns_native_code", "ns_satostop_solve_wide", "ns_satostop_hashmap_solve", "ns_hashmap_step"]
+            "ns_native_code", "ns_satostop_solve_wide", "ns_satostop_hashmap_solve", "ns_hashmap_step",
+            "ns_probe_gather"]
 
 
 def _check(status: int) -> None:
@@ -455,6 +457,14 @@ class Context:
         _check(load_library().ns_serve(self._h, arr, n, C.byref(o), res, C.byref(total)))
         out = [{k: getattr(res[i], k) for k, _ in _RequestResult._fields_} for i in range(n)]
         return out, total.value
+
+    def probe_gather(self, table_bits: int = 30, reads_per_lane: int = 256, lanes: int = 1 << 20,
+                     nontemporal: bool = False) -> dict:
+        """Scattered 4-byte read rate from a 2^table_bits-entry device table."""
+        ms, reads = C.c_double(), C.c_uint64()
+        _check(load_library().ns_probe_gather(self._h, table_bits, reads_per_lane, lanes, 1 if nontemporal else 0,
+                                              C.byref(ms), C.byref(reads)))
+        return {"ms": ms.value, "reads": reads.value, "reads_per_s": reads.value / (ms.value / 1000.0)}
 
     def pass_times(self) -> list:
         buf = np.zeros(64, np.float64)

@@ -11,7 +11,9 @@
 #include "circuit.h"
 #include "runtime.h"
 #include "serve.h"
+#include "static_kernels.h"
 #include "solvers.h"
+#include "static_kernels.h"
 
 struct ns_circuit {
   ns::Circuit c;
@@ -399,6 +401,30 @@ int ns_satostop_pass_times(ns_ctx* ctx, double* pass_ms, uint32_t cap, uint32_t*
     for (; k < v.size() && k < cap; k++)
       if (pass_ms) pass_ms[k] = v[k];
     *written = k;
+  });
+}
+
+int ns_probe_gather(ns_ctx* ctx, uint32_t table_bits, uint32_t reads_per_lane, uint32_t lanes, uint32_t flags,
+                    double* ms, uint64_t* reads) {
+  return guard([&] {
+    need(ctx && ms && reads, "null argument");
+    need(table_bits >= 8 && table_bits <= 33, "table_bits must be in 8..33");
+    need(lanes >= 256 && lanes % 256 == 0 && reads_per_lane > 0, "lanes: a positive multiple of 256");
+    ns::Lane& l = *ctx->ctx.main;
+    NS_HIP(hipSetDevice(l.device));
+    ns::DevBuf table, out;
+    table.ensure((1ull << table_bits) * 4);
+    out.ensure((uint64_t)lanes * 4);
+    NS_HIP(ns::launch_fill_u32(l.stream, table.as<unsigned int>(), 0x5a5a5a5au, 1ull << table_bits));
+    NS_HIP(ns::launch_gather_probe(l.stream, table.as<unsigned int>(), table_bits, reads_per_lane, lanes,
+                                   out.as<unsigned int>(), (flags & 1u) != 0));  // warm-up
+    NS_HIP(hipEventRecord(l.ev0, l.stream));
+    NS_HIP(ns::launch_gather_probe(l.stream, table.as<unsigned int>(), table_bits, reads_per_lane, lanes,
+                                   out.as<unsigned int>(), (flags & 1u) != 0));
+    NS_HIP(hipEventRecord(l.ev1, l.stream));
+    NS_HIP(hipStreamSynchronize(l.stream));
+    *ms = l.elapsed_ms();
+    *reads = (uint64_t)lanes * reads_per_lane;
   });
 }
 

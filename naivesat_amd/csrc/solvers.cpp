@@ -592,19 +592,25 @@ bool waypoint_search(Lane& ctx, uint32_t n, uint32_t unknowns, uint32_t k, uint3
   void* cur = ctx.tab_b.p;
   void* nxt = ctx.tab_c.p;
   unsigned int* flag = reinterpret_cast<unsigned int*>(ctx.stats.as<char>() + 192);
+  reset_counts(ctx);  // the walks count their evaluations in the count slots
   NS_HIP(launch_fill_u32(ctx.stream, flag, 0u, 1));
   NS_HIP(hipEventRecord(ctx.ev0, ctx.stream));
   NS_HIP(launch_wp_walk(ctx.stream, ctx.tab_a.as<unsigned int>(), split ? ctx.tab_e.as<unsigned int>() : nullptr,
-                        split, n, k, count, limit, cur, flag));
+                        split, n, k, count, limit, cur, flag, stats + kSlot0));
   NS_HIP(hipEventRecord(ctx.ev1, ctx.stream));
   unsigned int hf = 0;
   NS_HIP(hipMemcpyAsync(&hf, flag, 4, hipMemcpyDeviceToHost, ctx.stream));
+  CountsCopy cc;
+  read_counts_async(ctx, cc);
   NS_HIP(hipStreamSynchronize(ctx.stream));
   const double walk_ms = ctx.elapsed_ms();
   if (hf != 0) return false;
+  unsigned long long ev[2];
+  read_counts_done(cc, ev);
   r.method = 1;
   r.waypoint_bits = k;
   r.walk_ms = walk_ms;
+  r.walk_evaluations = ev[0];
   // a path visits at most 2^(n-k) waypoints before it repeats one
   for (uint32_t p = 0; p < n - k; p++) {
     NS_HIP(launch_fill_u32(ctx.stream, flag, 0u, 1));

@@ -20,10 +20,13 @@ hipError_t launch_starts(hipStream_t s, const unsigned long long* tab, unsigned 
 // Waypoint walks (records of 16 bytes per waypoint; see static_kernels.hip)
 hipError_t launch_wp_walk(hipStream_t s, const unsigned int* lo, const unsigned int* hi, unsigned int split,
                           unsigned int n, unsigned int k, unsigned long long count, unsigned long long limit,
-                          void* out, unsigned int* long_flag);
+                          void* out, unsigned int* long_flag, unsigned long long* evals);
 hipError_t launch_wp_jump(hipStream_t s, const void* in, void* out, unsigned long long count, unsigned int* pending);
 hipError_t launch_wp_starts(hipStream_t s, const void* rec, unsigned int n, unsigned int unknowns, unsigned int k,
                             unsigned long long* steps_out, unsigned long long* ends_out, unsigned long long* stats);
+// lanes must be a multiple of 256; out holds one word per lane
+hipError_t launch_gather_probe(hipStream_t s, const unsigned int* t, unsigned int bits, unsigned int reads,
+                               unsigned long long lanes, unsigned int* out, bool nontemporal);
 hipError_t launch_fill_u64(hipStream_t s, unsigned long long* p, unsigned long long v,
                            unsigned long long count);
 hipError_t launch_fill_u32(hipStream_t s, unsigned int* p, unsigned int v, unsigned long long count);

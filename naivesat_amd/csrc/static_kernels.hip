@@ -241,13 +241,13 @@ __global__ void __launch_bounds__(256) k_wp_walk(const unsigned int* __restrict_
                                                  const unsigned int* __restrict__ hi, unsigned int split,
                                                  unsigned int n, unsigned int k, unsigned long long count,
                                                  unsigned long long limit, unsigned long long budget,
-                                                 ulonglong2* __restrict__ out, unsigned int* long_flag) {
+                                                 ulonglong2* __restrict__ out, unsigned int* long_flag,
+                                                 unsigned long long* evals) {
   const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
   const unsigned long long mask = (1ull << n) - 1ull, wmask = (1ull << k) - 1ull;
   unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= count) return;
   unsigned long long s = i << k, saved = s, steps = 0, power = 1, lam = 0, used = 0;
-  for (;;) {
+  for (bool active = i < count; active;) {
     if (++used > budget) {
       atomicOr(long_flag, 1u);
       break;
@@ -291,6 +291,10 @@ __global__ void __launch_bounds__(256) k_wp_walk(const unsigned int* __restrict_
       lam = 0;
     }
   }
+  // evaluations done by the wave, into one of 64 count slots 16 words apart
+  unsigned long long total = used;
+  for (int off = 32; off > 0; off >>= 1) total += __shfl_xor(total, off);
+  if ((threadIdx.x & 63u) == 0u && total != 0ull) atomicAdd(evals + ((blockIdx.x & 63u) << 4), total);
 }
 
 // One pointer-jumping pass over the waypoint records; *pending |= some record still
@@ -337,6 +341,27 @@ __global__ void __launch_bounds__(256) k_wp_starts(const ulonglong2* __restrict_
     }
   }
   wave_stats(cnt, best, stats);
+}
+
+// Random-read probe: each lane reads `reads` entries of t (2^bits u32) at indices drawn from
+// its own linear congruential sequence (independent of the values read, so the reads can
+// overlap) and folds them into one output word: the rate of scattered 4-byte reads that
+// the memory system sustains, the ceiling of the waypoint walks' table reads.
+// nontemporal: the loads carry the non-temporal hint (__builtin_nontemporal_load).
+template <bool NT>
+__global__ void __launch_bounds__(256) k_gather_probe(const unsigned int* __restrict__ t, unsigned int bits,
+                                                      unsigned int reads, unsigned int* out) {
+  const unsigned long long tid = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const unsigned long long mask = (1ull << bits) - 1ull;
+  unsigned long long x = (tid + 1ull) * 0x9e3779b97f4a7c15ull;
+  unsigned int acc = 0u;
+#pragma unroll 8
+  for (unsigned int r = 0; r < reads; r++) {
+    x = x * 6364136223846793005ull + 1442695040888963407ull;
+    const unsigned int* p = t + ((x >> 24) & mask);
+    acc ^= NT ? __builtin_nontemporal_load(p) : *p;
+  }
+  out[tid] = acc;
 }
 
 __global__ void k_fill_u64(unsigned long long* p, unsigned long long v, unsigned long long count) {
@@ -414,7 +439,7 @@ hipError_t launch_starts(hipStream_t s, const unsigned long long* tab, unsigned 
 
 hipError_t launch_wp_walk(hipStream_t s, const unsigned int* lo, const unsigned int* hi, unsigned int split,
                           unsigned int n, unsigned int k, unsigned long long count, unsigned long long limit,
-                          void* out, unsigned int* long_flag) {
+                          void* out, unsigned int* long_flag, unsigned long long* evals) {
   // enough lanes to keep the memory system's queues full; each takes several walks
   const unsigned long long lanes = count < (1ull << 19) ? count : (1ull << 19);
   // a lane's walks average 2^k evaluations each on a random map; allow 16 times that
@@ -422,7 +447,7 @@ hipError_t launch_wp_walk(hipStream_t s, const unsigned int* lo, const unsigned 
   const unsigned long long per_lane = (count + lanes - 1) / lanes;
   const unsigned long long budget = 16ull * per_lane * (1ull << k) + limit;
   hipLaunchKernelGGL(k_wp_walk, dim3(blocks_for(lanes, 256)), dim3(256), 0, s, lo, hi, split, n, k, count, limit,
-                     budget, static_cast<ulonglong2*>(out), long_flag);
+                     budget, static_cast<ulonglong2*>(out), long_flag, evals);
   return hipGetLastError();
 }
 
@@ -437,6 +462,15 @@ hipError_t launch_wp_starts(hipStream_t s, const void* rec, unsigned int n, unsi
   const unsigned long long threads = ((1ull << unknowns) + 63ull) & ~63ull;
   hipLaunchKernelGGL(k_wp_starts, dim3(looped_blocks(threads, 256)), dim3(256), 0, s,
                      static_cast<const ulonglong2*>(rec), n, unknowns, k, steps_out, ends_out, stats);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather_probe(hipStream_t s, const unsigned int* t, unsigned int bits, unsigned int reads,
+                               unsigned long long lanes, unsigned int* out, bool nontemporal) {
+  if (nontemporal)
+    hipLaunchKernelGGL(k_gather_probe<true>, dim3(blocks_for(lanes, 256)), dim3(256), 0, s, t, bits, reads, out);
+  else
+    hipLaunchKernelGGL(k_gather_probe<false>, dim3(blocks_for(lanes, 256)), dim3(256), 0, s, t, bits, reads, out);
   return hipGetLastError();
 }
 
