@@ -80,6 +80,10 @@ int ns_gen_satostop(uint64_t seed, uint32_t half, uint32_t rounds, uint32_t roun
  *      src/main.rs:292-301) ---- */
 int ns_ctx_create(int device, ns_ctx** out);
 void ns_ctx_destroy(ns_ctx* ctx);
+/* Releases the context's SATOSTOP tables and images (device memory kept between solves so a
+ * repeated search reuses it); the next solve allocates again. No counterpart in the
+ * reference, whose buffers live as long as its executors. */
+int ns_ctx_trim(ns_ctx* ctx);
 
 /* ---- SAT (replaces the naivesat mapper run: BasicMapperBuilder + OpenCLBuilder with
  *      AGGR_OUTPUT_OPENCL_CODE and the per-arg callback, src/main.rs:91-104,157-201) ----
@@ -195,10 +199,13 @@ int ns_satostop_solve(ns_ctx* ctx, const ns_circuit* c, uint32_t unknowns,
                       const ns_satostop_opts* opts, ns_satostop_result* res, uint32_t* steps_out,
                       uint32_t* ends_out);
 
-/* Same contract for state spaces of up to 33 bits (the reference handles states wider
+/* Same contract for state spaces of up to 35 bits (the reference handles states wider
  * than 32 bits with u64 entries: join_nexts_64, src/greedysatostop.rs:169-194, and
- * HashEntry, src/satostop.rs:168-174): the next-state table (2^n entries) and its jump
- * tables stay in device memory (n = 33: 192 GiB), and the per-start results are 64-bit. */
+ * HashEntry, src/satostop.rs:168-174), with 64-bit per-start results. From 32 state bits
+ * on the full search reads a packed (n+1)-bit image (the reference's MemImage layout;
+ * n = 35: 155 GiB); 34 and 35 state bits take only the full search, with at most n - 10
+ * unknowns. The passes (first-pass mode, max_passes, fallback) use split u32 tables and
+ * u64 jump tables, up to 33 state bits (n = 33: 192 GiB). */
 int ns_satostop_solve_wide(ns_ctx* ctx, const ns_circuit* c, uint32_t unknowns,
                            const ns_satostop_opts* opts, ns_satostop_result* res, uint64_t* steps_out,
                            uint64_t* ends_out);
@@ -307,6 +314,13 @@ int ns_native_code(const ns_circuit* c, uint32_t kind, uint32_t lane_shift, void
 /* Per-pass timing of the last ns_satostop_solve on this context: pass_ms[i] for jump pass i
  * (HIP events on the context stream). Returns the number of passes written. */
 int ns_satostop_pass_times(ns_ctx* ctx, double* pass_ms, uint32_t cap, uint32_t* written);
+
+/* Diagnostic: the device's packing of next-state entries into the reference's MemImage
+ * layout (src/greedysatostop.rs:339-400: entry i in bits [i*width, (i+1)*width) of
+ * little-endian u64 words), as the packed SATOSTOP image is built: values[i] masked to width
+ * bits, count a multiple of 64, count * width / 64 words out. */
+int ns_pack_entries(ns_ctx* ctx, uint32_t width, const uint64_t* values, uint64_t count, uint64_t* words,
+                    uint64_t words_cap);
 
 /* Diagnostic (no counterpart in the reference): the rate of scattered 4-byte reads from a
  * 2^table_bits-entry device table, lanes x reads_per_lane reads at independent

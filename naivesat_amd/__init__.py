@@ -149,6 +149,8 @@ def load_library() -> C.CDLL:
         "ns_native_code": (I, [P, U32, U32, P, C.c_size_t, C.POINTER(C.c_size_t), C.POINTER(U32),
                                C.POINTER(U64)]),
         "ns_probe_gather": (I, [P, U32, U32, U32, U32, C.POINTER(C.c_double), C.POINTER(U64)]),
+        "ns_pack_entries": (I, [P, U32, P, U64, P, U64]),
+        "ns_ctx_trim": (I, [P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -166,7 +168,7 @@ def exported_symbols() -> list:
             "ns_gen_satostop", "ns_ctx_create", "ns_ctx_destroy", "ns_sat_solve", "ns_qsat_solve",
             "ns_satostop_solve", "ns_table_build", "ns_satostop_pass_times", "ns_native_body", "ns_serve", "ns_native_encode",
             "ns_native_code", "ns_satostop_solve_wide", "ns_satostop_hashmap_solve", "ns_hashmap_step",
-            "ns_probe_gather"]
+            "ns_probe_gather", "ns_pack_entries", "ns_ctx_trim"]
 
 
 def _check(status: int) -> None:
@@ -457,6 +459,17 @@ class Context:
         _check(load_library().ns_serve(self._h, arr, n, C.byref(o), res, C.byref(total)))
         out = [{k: getattr(res[i], k) for k, _ in _RequestResult._fields_} for i in range(n)]
         return out, total.value
+
+    def trim(self) -> None:
+        """Releases the SATOSTOP tables and images the context keeps between solves."""
+        _check(load_library().ns_ctx_trim(self._h))
+
+    def pack_entries(self, width: int, values) -> np.ndarray:
+        """The device's MemImage packing of `values` (uint64, a multiple of 64 entries)."""
+        v = np.ascontiguousarray(values, np.uint64)
+        words = np.zeros(v.size * width // 64, np.uint64)
+        _check(load_library().ns_pack_entries(self._h, width, _ptr(v), v.size, _ptr(words), words.size))
+        return words
 
     def probe_gather(self, table_bits: int = 30, reads_per_lane: int = 256, lanes: int = 1 << 20,
                      nontemporal: bool = False) -> dict:

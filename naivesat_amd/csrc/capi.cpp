@@ -207,6 +207,15 @@ void ns_ctx_destroy(ns_ctx* ctx) {
   }
 }
 
+int ns_ctx_trim(ns_ctx* ctx) {
+  return guard([&] {
+    need(ctx, "null argument");
+    NS_HIP(hipSetDevice(ctx->ctx.main->device));
+    NS_HIP(hipStreamSynchronize(ctx->ctx.main->stream));
+    ctx->ctx.main->trim_tables();
+  });
+}
+
 int ns_sat_solve(ns_ctx* ctx, const ns_circuit* c, const ns_sat_opts* opts, ns_sat_result* res) {
   return guard([&] {
     need(ctx && c && res, "null argument");
@@ -401,6 +410,34 @@ int ns_satostop_pass_times(ns_ctx* ctx, double* pass_ms, uint32_t cap, uint32_t*
     for (; k < v.size() && k < cap; k++)
       if (pass_ms) pass_ms[k] = v[k];
     *written = k;
+  });
+}
+
+int ns_pack_entries(ns_ctx* ctx, uint32_t width, const uint64_t* values, uint64_t count, uint64_t* words,
+                    uint64_t words_cap) {
+  return guard([&] {
+    need(ctx && values && words, "null argument");
+    need(width >= 1 && width <= 64, "width must be in 1..64");
+    need(count > 0 && count % 64 == 0, "count must be a positive multiple of 64");
+    const uint64_t nwords = count * width / 64;
+    need(words_cap >= nwords, "words array too small");
+    ns::Lane& l = *ctx->ctx.main;
+    NS_HIP(hipSetDevice(l.device));
+    std::vector<uint32_t> lo(count), hi(count);
+    for (uint64_t i = 0; i < count; i++) {
+      lo[i] = (uint32_t)values[i];
+      hi[i] = (uint32_t)(values[i] >> 32);
+    }
+    ns::DevBuf dlo, dhi, dout;
+    dlo.ensure(count * 4);
+    dhi.ensure(count * 4);
+    dout.ensure(nwords * 8);
+    NS_HIP(hipMemcpyAsync(dlo.p, lo.data(), count * 4, hipMemcpyHostToDevice, l.stream));
+    NS_HIP(hipMemcpyAsync(dhi.p, hi.data(), count * 4, hipMemcpyHostToDevice, l.stream));
+    NS_HIP(ns::launch_pack(l.stream, dlo.as<unsigned int>(), width > 32 ? dhi.as<unsigned int>() : nullptr, 32, width,
+                           count, dout.as<unsigned long long>()));
+    NS_HIP(hipMemcpyAsync(words, dout.p, nwords * 8, hipMemcpyDeviceToHost, l.stream));
+    NS_HIP(hipStreamSynchronize(l.stream));
   });
 }
 

@@ -1,9 +1,9 @@
 // naivesatostop: SATOSTOP answer (start, end, steps) on one HIP device.
 // Command line and printed lines follow the reference's naivesatostop
-// (src/satostop.rs:62-99,1650-1782,4443-4467). Up to 33 state bits the search runs on the
-// device-resident next-state table (the lowest stopping unknown); beyond that, or with
-// --hashmap-search, it runs the reference's hash-map search (any stopping unknown) with the
-// -e/-H/-U/-P/-C options.
+// (src/satostop.rs:62-99,1650-1782,4443-4467). Up to 33 state bits (35 with at most n - 10
+// unknowns) the search runs on the device-resident next-state table (the lowest stopping
+// unknown); beyond that, or with --hashmap-search, it runs the reference's hash-map search
+// (any stopping unknown) with the -e/-H/-U/-P/-C options.
 #include "cli_common.h"
 
 static const char* kUsage =
@@ -14,7 +14,7 @@ static const char* kUsage =
     "  -U, --unknown-fill-bits <N>           [default: unknowns - (H - 1), at least 0]\n"
     "  -P, --max-predecessors <N>            [default: 0]\n"
     "  -C, --iters-per-clear-predecessors <N> [default: 20]\n"
-    "  -M, --hashmap-search                  hash-map search also at 33 state bits or fewer\n"
+    "  -M, --hashmap-search                  hash-map search also where the table would do\n"
     "  -t, --exec-type <TYPE>                cpu|opencl:N|hip:N (all run on one HIP device)\n"
     "  -n, --no-optimize-negs                accepted for compatibility\n"
     "  -G, --opencl-group-len <G>            threads per block of the table build\n"
@@ -46,8 +46,10 @@ int main(int argc, char** argv) {
   long long U = std::atoll(a.positional[1].c_str());
   if (U < 0 || U > (long long)n) cli::die("unknowns must not exceed the inputs");
   const bool simple = a.has("simple");
-  const bool hashmap = !simple && (n > 33 || a.has("hashmap-search"));
-  if (simple && n > 33) cli::die("--simple takes at most 33 state bits");
+  // the device table holds up to 35 state bits (packed from 32 on, with at most n - 10 unknowns)
+  const bool table_ok = n <= 33 || (n <= 35 && U <= (long long)n - 10);
+  const bool hashmap = !simple && (!table_ok || a.has("hashmap-search"));
+  if (simple && !table_ok) cli::die("--simple takes at most 33 state bits (35 with at most n - 10 unknowns)");
   if (hashmap && n < 10) cli::die("the hash-map search takes at least 10 state bits");
   int dev = cli::device_from_exec_type(a.get("exec-type"), !simple);
   const long long H = a.num("hashmap-len-bits", 24);

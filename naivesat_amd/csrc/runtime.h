@@ -115,6 +115,7 @@ struct Lane {
   DevBuf buf_a, buf_b, buf_c;         // QSAT scratch (bit arrays)
   DevBuf tab_a, tab_b, tab_c, tab_d;  // SATOSTOP: next-state table, jump tables, starts
   DevBuf tab_e;                       // SATOSTOP over 31 state bits: outputs 32..n of the table
+  DevBuf tab_img;                     // SATOSTOP: packed (n+1)-bit next-state image
   std::vector<double> pass_ms;  // jump-pass kernel times of the last SATOSTOP solve
   // launch sequences of repeated searches, captured once and replayed as HIP graphs; each
   // holds its kernel's module so the module outlives the graph
@@ -137,7 +138,8 @@ struct Lane {
   float elapsed_ms();
   // device bytes held by the scratch buffers
   size_t held_bytes() const {
-    return buf_a.cap + buf_b.cap + buf_c.cap + tab_a.cap + tab_b.cap + tab_c.cap + tab_d.cap + tab_e.cap;
+    return buf_a.cap + buf_b.cap + buf_c.cap + tab_a.cap + tab_b.cap + tab_c.cap + tab_d.cap + tab_e.cap +
+           tab_img.cap;
   }
   void trim_tables() {
     tab_a.release();
@@ -145,6 +147,7 @@ struct Lane {
     tab_c.release();
     tab_d.release();
     tab_e.release();
+    tab_img.release();
   }
 };
 

@@ -574,13 +574,81 @@ uint32_t waypoint_bits(uint32_t n, uint32_t unknowns) {
 
 namespace {
 
-// The full search as waypoint walks (see k_wp_walk): fills the per-start arrays and
-// host[0..1] = (stopping starts, lowest one). Returns false, with nothing changed, when a
-// walk exceeds its evaluation limit (a long path or cycle without waypoints); the passes
-// then answer instead. split: the wide table's split (0 = one u32 table).
-bool waypoint_search(Lane& ctx, uint32_t n, uint32_t unknowns, uint32_t k, uint32_t split, ns_satostop_result& r,
-                     unsigned long long* d_steps, unsigned long long* d_ends, unsigned long long* stats,
-                     unsigned long long host[2]) {
+// Packed next-state image of c in ctx.tab_img: 2^n entries of n + 1 bits (next state, STOP
+// at bit n) in the reference's MemImage layout. Built 2^30 states at a time (all of them
+// below 30 state bits): the table kernels write outputs 0..31 (and 32..n) of the chunk to
+// tab_a (tab_e), the pack kernel appends the chunk's words. Returns kernel ms; *any = some
+// state's STOP output is 1.
+double build_packed_table(Lane& ctx, const Circuit& c, const ns_satostop_opts& o, bool* any, double* compile_ms,
+                          uint32_t* cone, TableKernelInfo* info) {
+  const uint32_t n = c.n, width = n + 1;
+  if (n < 6 || n > 35) throw Error(ST_INTERNAL, "packed images take 6..35 state bits");
+  const bool two = width > 32;
+  const uint32_t split = two ? 32 : width;
+  const uint32_t cb = std::min(n, 30u);
+  const uint64_t chunk = 1ull << cb, chunks = 1ull << (n - cb), words_chunk = chunk >> 5;
+  Geometry g = pick_geometry(words_chunk, o.group_len, o.words_per_lane ? o.words_per_lane : 2, false);
+  std::vector<uint32_t> vb(n);
+  for (uint32_t k = 0; k < n; k++) vb[k] = k;
+  const Circuit parts[2] = {with_outputs(c, 0, split), two ? with_outputs(c, split, width) : Circuit()};
+  Chosen kern[2];
+  *compile_ms = 0;
+  *cone = 0;
+  info->info = GenInfo();
+  info->native = true;
+  info->vgpr_class = 0;
+  for (int h = 0; h < (two ? 2 : 1); h++) {
+    std::vector<uint32_t> outs;
+    for (uint32_t k = 0; k < parts[h].outputs.size(); k++) outs.push_back(k);
+    double cms = 0;
+    kern[h] = choose_kernel(
+        ctx, parts[h], NativeKind::TABLE, vb, 0, outs, g, "",
+        [&](GenInfo* gi) { return gen_table_kernel(parts[h], g, gi); }, "ns_table", &cms);
+    *compile_ms += cms;
+    *cone += kern[h].info.cone_gates;
+    info->info.valu_ops += kern[h].info.valu_ops;
+    if (!kern[h].native) info->native = false;
+    info->vgpr_class = std::max(info->vgpr_class, kern[h].vgpr_class);
+  }
+  const uint64_t wpb = g.words_per_block();
+  const uint64_t tab_bytes = (words_chunk + wpb - 1) / wpb * wpb * 32 * 4;
+  ctx.tab_a.ensure(tab_bytes);
+  if (two) ctx.tab_e.ensure(tab_bytes);
+  const uint64_t total_words = ((1ull << n) * width) >> 6;
+  ctx.tab_img.ensure((total_words + 1) * 8);  // one word of padding after the last entry
+  unsigned long long* img = ctx.tab_img.as<unsigned long long>();
+  unsigned int* flags = reinterpret_cast<unsigned int*>(ctx.stats.as<char>() + 64);  // [0] STOP seen, [1] unused
+  NS_HIP(launch_fill_u32(ctx.stream, flags, 0u, 2));
+  NS_HIP(launch_fill_u64(ctx.stream, img + total_words, 0ull, 1));
+  NS_HIP(hipEventRecord(ctx.ev0, ctx.stream));
+  for (uint64_t ci = 0; ci < chunks; ci++) {
+    const uint64_t w0 = ci * words_chunk;
+    for (int h = 0; h < (two ? 2 : 1); h++) {
+      unsigned int* T = h ? ctx.tab_e.as<unsigned int>() : ctx.tab_a.as<unsigned int>();
+      // the STOP output is the last output of the last part
+      unsigned int* flag = h == (two ? 1 : 0) ? flags : flags + 1;
+      uint32_t wpl_v = g.wpl, last = (uint32_t)parts[h].outputs.size() - 1, valid = 0xffffffffu, small = 0;
+      void* extra[6] = {&T, &flag, &wpl_v, &last, &valid, &small};
+      launch_range(ctx, kern[h].mod->fn, g, w0, words_chunk, extra, kern[h].native ? 6 : 2, w0 + words_chunk);
+    }
+    NS_HIP(launch_pack(ctx.stream, ctx.tab_a.as<unsigned int>(), two ? ctx.tab_e.as<unsigned int>() : nullptr, split,
+                       width, chunk, img + ((ci * chunk * width) >> 6)));
+  }
+  NS_HIP(hipEventRecord(ctx.ev1, ctx.stream));
+  unsigned int hf = 0;
+  NS_HIP(hipMemcpyAsync(&hf, flags, 4, hipMemcpyDeviceToHost, ctx.stream));
+  NS_HIP(hipStreamSynchronize(ctx.stream));
+  *any = hf != 0;
+  return ctx.elapsed_ms();
+}
+
+// The full search as waypoint walks (see k_wp_walk) over the table tv: fills the per-start
+// arrays and host[0..1] = (stopping starts, lowest one). Returns false, with nothing
+// changed, when a walk exceeds its evaluation limit (a long path or cycle without
+// waypoints); the passes then answer instead.
+bool waypoint_search(Lane& ctx, uint32_t n, uint32_t unknowns, uint32_t k, const TableView& tv,
+                     ns_satostop_result& r, unsigned long long* d_steps, unsigned long long* d_ends,
+                     unsigned long long* stats, unsigned long long host[2]) {
   const uint64_t count = 1ull << (n - k);
   uint64_t limit = 1ull << 18;
   if (const char* e = std::getenv("NS_WAYPOINT_LIMIT")) {
@@ -595,8 +663,7 @@ bool waypoint_search(Lane& ctx, uint32_t n, uint32_t unknowns, uint32_t k, uint3
   reset_counts(ctx);  // the walks count their evaluations in the count slots
   NS_HIP(launch_fill_u32(ctx.stream, flag, 0u, 1));
   NS_HIP(hipEventRecord(ctx.ev0, ctx.stream));
-  NS_HIP(launch_wp_walk(ctx.stream, ctx.tab_a.as<unsigned int>(), split ? ctx.tab_e.as<unsigned int>() : nullptr,
-                        split, n, k, count, limit, cur, flag, stats + kSlot0));
+  NS_HIP(launch_wp_walk(ctx.stream, tv, n, k, count, limit, cur, flag, stats + kSlot0));
   NS_HIP(hipEventRecord(ctx.ev1, ctx.stream));
   unsigned int hf = 0;
   NS_HIP(hipMemcpyAsync(&hf, flag, 4, hipMemcpyDeviceToHost, ctx.stream));
@@ -642,63 +709,93 @@ void satostop_solve(Lane& ctx, const Circuit& c, uint32_t unknowns, const ns_sat
   std::memset(&r, 0, sizeof(r));
   const uint32_t n = c.n;
   if (c.outputs.size() != (size_t)n + 1) throw Error(ST_INVALID, "SATOSTOP needs n inputs and n+1 outputs");
-  if (n == 0 || n > 33) throw Error(ST_UNSUPPORTED, "SATOSTOP tables support 1..33 state bits");
+  if (n == 0 || n > 35) throw Error(ST_UNSUPPORTED, "SATOSTOP tables support 1..35 state bits");
   if (unknowns > n) throw Error(ST_INVALID, "more unknowns than state bits");
-  // More than 31 state bits: the next-state table is split into outputs 0..31 and 32..n
-  // and the jump entries use the wide layout. NS_WIDE_TABLE=1 takes that path for any n.
-  const char* we = std::getenv("NS_WIDE_TABLE");
-  const bool wide = n > 31 || (we && *we == '1');
   NS_HIP(hipSetDevice(ctx.device));
   ctx.pass_ms.clear();
-  bool any = false;
-  TableKernelInfo tki;
-  uint32_t split = 0;
-  if (!wide) {
-    r.build_ms = build_table_dev(ctx, c, o.group_len, o.words_per_lane, ctx.tab_a, &any, &r.compile_ms,
-                                 &r.cone_gates, &tki);
-  } else {
-    split = n + 1 > 32 ? 32 : n;  // outputs 0..split-1 in the low table
-    bool any_lo = false;
-    double cms_lo = 0, cms_hi = 0;
-    uint32_t cone_lo = 0, cone_hi = 0;
-    TableKernelInfo lo_info;
-    r.build_ms = build_table_dev(ctx, with_outputs(c, 0, split), o.group_len, o.words_per_lane, ctx.tab_a,
-                                 &any_lo, &cms_lo, &cone_lo, &lo_info);
-    r.build_ms += build_table_dev(ctx, with_outputs(c, split, n + 1), o.group_len, o.words_per_lane, ctx.tab_e,
-                                  &any, &cms_hi, &cone_hi, &tki);
-    r.compile_ms = cms_lo + cms_hi;
-    r.cone_gates = cone_lo + cone_hi;
-    tki.info.valu_ops += lo_info.info.valu_ops;
-    if (!lo_info.native) tki.native = false;
-    tki.vgpr_class = std::max(tki.vgpr_class, lo_info.vgpr_class);
-  }
-  r.valu_ops = tki.info.valu_ops;
-  r.backend = tki.native ? 1 : 0;
-  r.vgpr_class = tki.vgpr_class;
-  r.any_stop = any;
+  // C = log2 of the evaluations every walk covers; the wide layout counts up to 2^30 - 1
+  uint32_t cover = o.max_passes ? o.max_passes : n;
+  if (cover > n) cover = n;
+  const bool first_pass_mode = (o.flags & NS_STOP_FIRST_PASS) != 0;
+  // The full search runs as waypoint walks when every start is a waypoint and the records
+  // stay small; from 32 state bits on (NS_PACKED_TABLE=1: from 6) the walks read the packed
+  // (n+1)-bit image, which holds 34 and 35 state bits in HBM too.
+  const uint32_t k = waypoint_bits(n, unknowns);
+  const bool walks = !first_pass_mode && cover == n && (o.flags & NS_STOP_PASSES) == 0 && k <= n - unknowns &&
+                     n - k <= 24;
+  const char* pe = std::getenv("NS_PACKED_TABLE");
+  const bool packed = walks && n >= 6 && (n >= 32 || (pe && *pe == '1'));
+  if (n > 33 && !packed)
+    throw Error(ST_UNSUPPORTED, "34 and 35 state bits: only the full search (waypoint walks over the packed image, "
+                                "at most n - 10 unknowns)");
   const uint64_t states = 1ull << n;
   const uint64_t starts = 1ull << unknowns;
   ctx.tab_d.ensure(starts * 16 + 64);
   unsigned long long* d_steps = ctx.tab_d.as<unsigned long long>();
   unsigned long long* d_ends = d_steps + starts;
   unsigned long long* stats = ctx.stats.as<unsigned long long>();
-  // C = log2 of the evaluations every walk covers; the wide layout counts up to 2^30 - 1
-  uint32_t cover = o.max_passes ? o.max_passes : n;
-  if (cover > n) cover = n;
-  const bool first_pass_mode = (o.flags & NS_STOP_FIRST_PASS) != 0;
+  unsigned long long host[2] = {0, ~0ull};
+  bool any = false, searched = false, walks_failed = false;
+  TableKernelInfo tki;
+  if (packed) {
+    r.build_ms = build_packed_table(ctx, c, o, &any, &r.compile_ms, &r.cone_gates, &tki);
+    r.any_stop = any;
+    if (!any) {
+      NS_HIP(launch_fill_u64(ctx.stream, d_steps, 0ull, 2 * starts));
+      host[0] = 0;
+      searched = true;
+    } else {
+      const TableView tv{nullptr, nullptr, 0, ctx.tab_img.as<unsigned long long>(), n + 1};
+      searched = waypoint_search(ctx, n, unknowns, k, tv, r, d_steps, d_ends, stats, host);
+      if (!searched) {
+        if (n > 33)
+          throw Error(ST_UNSUPPORTED, "a waypoint walk exceeded its limit (a long path or cycle without waypoints); "
+                                      "over 33 state bits there are no passes to fall back on: use the hash-map search");
+        walks_failed = true;
+        ctx.tab_img.release();
+      }
+    }
+  }
+  // More than 31 state bits: the next-state table is split into outputs 0..31 and 32..n
+  // and the jump entries use the wide layout. NS_WIDE_TABLE=1 takes that path for any n.
+  const char* we = std::getenv("NS_WIDE_TABLE");
+  const bool wide = n > 31 || (we && *we == '1');
+  uint32_t split = 0;
+  if (!searched) {
+    double build_ms = 0, cms = 0;
+    if (!wide) {
+      build_ms = build_table_dev(ctx, c, o.group_len, o.words_per_lane, ctx.tab_a, &any, &cms, &r.cone_gates, &tki);
+    } else {
+      split = n + 1 > 32 ? 32 : n;  // outputs 0..split-1 in the low table
+      bool any_lo = false;
+      double cms_lo = 0, cms_hi = 0;
+      uint32_t cone_lo = 0, cone_hi = 0;
+      TableKernelInfo lo_info;
+      build_ms = build_table_dev(ctx, with_outputs(c, 0, split), o.group_len, o.words_per_lane, ctx.tab_a, &any_lo,
+                                 &cms_lo, &cone_lo, &lo_info);
+      build_ms += build_table_dev(ctx, with_outputs(c, split, n + 1), o.group_len, o.words_per_lane, ctx.tab_e, &any,
+                                  &cms_hi, &cone_hi, &tki);
+      cms = cms_lo + cms_hi;
+      r.cone_gates = cone_lo + cone_hi;
+      tki.info.valu_ops += lo_info.info.valu_ops;
+      if (!lo_info.native) tki.native = false;
+      tki.vgpr_class = std::max(tki.vgpr_class, lo_info.vgpr_class);
+    }
+    r.build_ms += build_ms;
+    r.compile_ms += cms;
+    r.any_stop = any;
+  }
+  r.valu_ops = tki.info.valu_ops;
+  r.backend = tki.native ? 1 : 0;
+  r.vgpr_class = tki.vgpr_class;
   uint32_t max_passes = first_pass_mode ? cover : choose_passes(n, unknowns, cover);
   if (wide && max_passes > 29) max_passes = 29;
   if (first_pass_mode && wide && cover > 29) throw Error(ST_UNSUPPORTED, "first-pass mode covers at most 2^29 evaluations for more than 31 state bits");
   const uint64_t walk_jumps = 1ull << (cover - max_passes);
-  unsigned long long host[2] = {0, ~0ull};
   const unsigned long long* final_tab = nullptr;
-  bool searched = false;
-  // The full search runs as waypoint walks when every start is a waypoint and the records
-  // stay small.
-  if (any && !first_pass_mode && cover == n && (o.flags & NS_STOP_PASSES) == 0) {
-    const uint32_t k = waypoint_bits(n, unknowns);
-    if (k <= n - unknowns && n - k <= 24) searched = waypoint_search(ctx, n, unknowns, k, wide ? split : 0, r, d_steps,
-                                                                       d_ends, stats, host);
+  if (any && !searched && walks && !walks_failed) {
+    const TableView tv{ctx.tab_a.as<unsigned int>(), wide ? ctx.tab_e.as<unsigned int>() : nullptr, split, nullptr, 0};
+    searched = waypoint_search(ctx, n, unknowns, k, tv, r, d_steps, d_ends, stats, host);
   }
   if (any && !searched) {
     // even entry count for the two-entries-per-thread pass

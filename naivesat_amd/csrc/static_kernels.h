@@ -17,10 +17,24 @@ hipError_t launch_jump_pass(hipStream_t s, const unsigned long long* src, unsign
 hipError_t launch_starts(hipStream_t s, const unsigned long long* tab, unsigned int n, unsigned int unknowns,
                          unsigned long long jumps, bool wide, unsigned long long* steps_out,
                          unsigned long long* ends_out, unsigned long long* stats);
+// A next-state table as the walks read it: packed (n+1)-bit entries (packed non-null,
+// width = n + 1) or lo[s] | hi[s] << split (hi null: lo alone).
+struct TableView {
+  const unsigned int* lo;
+  const unsigned int* hi;
+  unsigned int split;
+  const unsigned long long* packed;
+  unsigned int width;
+};
+
+// Packs lo[e] | hi[e] << split, e < entries (a multiple of 64), into width-bit fields.
+hipError_t launch_pack(hipStream_t s, const unsigned int* lo, const unsigned int* hi, unsigned int split,
+                       unsigned int width, unsigned long long entries, unsigned long long* out);
+
 // Waypoint walks (records of 16 bytes per waypoint; see static_kernels.hip)
-hipError_t launch_wp_walk(hipStream_t s, const unsigned int* lo, const unsigned int* hi, unsigned int split,
-                          unsigned int n, unsigned int k, unsigned long long count, unsigned long long limit,
-                          void* out, unsigned int* long_flag, unsigned long long* evals);
+hipError_t launch_wp_walk(hipStream_t s, const TableView& tab, unsigned int n, unsigned int k,
+                          unsigned long long count, unsigned long long limit, void* out, unsigned int* long_flag,
+                          unsigned long long* evals);
 hipError_t launch_wp_jump(hipStream_t s, const void* in, void* out, unsigned long long count, unsigned int* pending);
 hipError_t launch_wp_starts(hipStream_t s, const void* rec, unsigned int n, unsigned int unknowns, unsigned int k,
                             unsigned long long* steps_out, unsigned long long* ends_out, unsigned long long* stats);

@@ -231,15 +231,30 @@ __global__ void __launch_bounds__(256) k_starts(const unsigned long long* __rest
 constexpr unsigned int WP_NEXT = 0u, WP_STOP = 1u, WP_LOOP = 2u, WP_LONG = 3u;
 constexpr unsigned long long WP_TARGET = (1ull << 62) - 1ull;
 
-// The table: lo[s] | hi[s] << split (hi null: lo alone holds next state and STOP).
+// Entry s of the next-state table (next state in bits 0..n-1, STOP at bit n; bits above n
+// undefined): packed (n+1)-bit entries, entry s at bit s*(n+1) of little-endian u64 words
+// (the reference's MemImage layout, src/greedysatostop.rs:339-400), or lo[s] | hi[s] << split.
+__device__ inline unsigned long long table_entry(const TableView& t, unsigned long long s) {
+  if (t.packed) {
+    const unsigned long long bit = s * t.width;
+    const unsigned long long idx = bit >> 6;
+    const unsigned int b = (unsigned int)(bit & 63ull);
+    unsigned long long v = t.packed[idx] >> b;
+    if (b + t.width > 64u) v |= t.packed[idx + 1] << (64u - b);
+    return v;
+  }
+  unsigned long long e = t.lo[s];
+  if (t.hi) e |= (unsigned long long)t.hi[s] << t.split;
+  return e;
+}
+
 // A lane that finishes a walk takes its next waypoint (grid stride) at once, so lanes
 // stay busy while walks of different lengths end. A lane also stops, raising the flag,
 // after `budget` evaluations in all, and every lane stops once it sees the flag (checked
 // every 256 evaluations): maps whose walks are far longer than the waypoint spacing end
 // the kernel early and go to the passes.
-__global__ void __launch_bounds__(256) k_wp_walk(const unsigned int* __restrict__ lo,
-                                                 const unsigned int* __restrict__ hi, unsigned int split,
-                                                 unsigned int n, unsigned int k, unsigned long long count,
+__global__ void __launch_bounds__(256) k_wp_walk(TableView tab, unsigned int n, unsigned int k,
+                                                 unsigned long long count,
                                                  unsigned long long limit, unsigned long long budget,
                                                  ulonglong2* __restrict__ out, unsigned int* long_flag,
                                                  unsigned long long* evals) {
@@ -255,8 +270,7 @@ __global__ void __launch_bounds__(256) k_wp_walk(const unsigned int* __restrict_
     if ((used & 255ull) == 0ull &&
         __hip_atomic_load(long_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u)
       break;
-    unsigned long long e = lo[s];
-    if (hi) e |= (unsigned long long)hi[s] << split;
+    const unsigned long long e = table_entry(tab, s);
     steps++;
     const unsigned long long nx = e & mask;
     unsigned int kind = 4u;
@@ -364,6 +378,32 @@ __global__ void __launch_bounds__(256) k_gather_probe(const unsigned int* __rest
   out[tid] = acc;
 }
 
+// Packs entries lo[e] | hi[e] << split (e < entries, entries a multiple of 64) into
+// width-bit fields: output word w holds bits [64w, 64w + 64) of the packed stream, so every
+// lane writes one whole word and reads the two or three entries it overlaps.
+__global__ void __launch_bounds__(256) k_pack(const unsigned int* __restrict__ lo, const unsigned int* __restrict__ hi,
+                                              unsigned int split, unsigned int width, unsigned long long entries,
+                                              unsigned long long* __restrict__ out) {
+  const unsigned long long words = (entries * width) >> 6;
+  const unsigned long long fmask = width >= 64 ? ~0ull : (1ull << width) - 1ull;
+  const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
+  for (unsigned long long w = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; w < words; w += stride) {
+    const unsigned long long bit = w << 6;
+    unsigned long long e = bit / width;
+    unsigned int off = (unsigned int)(bit - e * width), pos = 0;
+    unsigned long long word = 0;
+    while (pos < 64u && e < entries) {
+      unsigned long long v = lo[e];
+      if (hi) v |= (unsigned long long)hi[e] << split;
+      word |= ((v & fmask) >> off) << pos;
+      pos += width - off;
+      off = 0;
+      e++;
+    }
+    out[w] = word;
+  }
+}
+
 __global__ void k_fill_u64(unsigned long long* p, unsigned long long v, unsigned long long count) {
   const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
   for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride)
@@ -437,17 +477,24 @@ hipError_t launch_starts(hipStream_t s, const unsigned long long* tab, unsigned 
   return hipGetLastError();
 }
 
-hipError_t launch_wp_walk(hipStream_t s, const unsigned int* lo, const unsigned int* hi, unsigned int split,
-                          unsigned int n, unsigned int k, unsigned long long count, unsigned long long limit,
-                          void* out, unsigned int* long_flag, unsigned long long* evals) {
+hipError_t launch_pack(hipStream_t s, const unsigned int* lo, const unsigned int* hi, unsigned int split,
+                       unsigned int width, unsigned long long entries, unsigned long long* out) {
+  hipLaunchKernelGGL(k_pack, dim3(looped_blocks((entries * width) >> 6, 256)), dim3(256), 0, s, lo, hi, split, width,
+                     entries, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_wp_walk(hipStream_t s, const TableView& tab, unsigned int n, unsigned int k,
+                          unsigned long long count, unsigned long long limit, void* out, unsigned int* long_flag,
+                          unsigned long long* evals) {
   // enough lanes to keep the memory system's queues full; each takes several walks
   const unsigned long long lanes = count < (1ull << 19) ? count : (1ull << 19);
   // a lane's walks average 2^k evaluations each on a random map; allow 16 times that
   // over its walks plus one walk of the full limit
   const unsigned long long per_lane = (count + lanes - 1) / lanes;
   const unsigned long long budget = 16ull * per_lane * (1ull << k) + limit;
-  hipLaunchKernelGGL(k_wp_walk, dim3(blocks_for(lanes, 256)), dim3(256), 0, s, lo, hi, split, n, k, count, limit,
-                     budget, static_cast<ulonglong2*>(out), long_flag, evals);
+  hipLaunchKernelGGL(k_wp_walk, dim3(blocks_for(lanes, 256)), dim3(256), 0, s, tab, n, k, count, limit, budget,
+                     static_cast<ulonglong2*>(out), long_flag, evals);
   return hipGetLastError();
 }
 

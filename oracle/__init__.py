@@ -67,6 +67,14 @@ def lib() -> C.CDLL:
         L.or_table_u32.argtypes = [C.POINTER(_Circ), P]
         L.or_hash64.restype = U64
         L.or_hash64.argtypes = [U32, U64]
+        L.or_mem_get.restype = U64
+        L.or_mem_get.argtypes = [P, U32, U64]
+        L.or_mem_set.restype = None
+        L.or_mem_set.argtypes = [P, U32, U64, U64]
+        L.or_mem_join_nexts.restype = None
+        L.or_mem_join_nexts.argtypes = [P, U64, U64, P, U64, U64, U32]
+        L.or_mem_find_solution.restype = C.c_int
+        L.or_mem_find_solution.argtypes = [P, U64, U64, U32, U32, C.POINTER(U64), C.POINTER(U64)]
         L.or_hm_join_to.restype = None
         L.or_hm_join_to.argtypes = [U32, U32, U64, P, P, U64]
         L.or_hm_join_itself.restype = None
@@ -211,3 +219,39 @@ def hm_add(state_len, arg_bit_place, arg, outputs, hm, unknown_bits, unknown_fil
                     hm.ctypes.data_as(C.c_void_p), hm.size, unknown_bits, unknown_fill_bits,
                     st.unknown_fills.ctypes.data_as(C.c_void_p), C.byref(st.resolved), C.byref(st.sol),
                     max_predecessors, 1 if test else 0)
+
+
+# ---- MemImage (src/greedysatostop.rs:339-475) ----
+def mem_words(width: int, length: int) -> np.ndarray:
+    """zeroed storage of `length` entries (a multiple of 64) of `width` bits"""
+    assert length % 64 == 0
+    return np.zeros(length * width // 64, np.uint64)
+
+
+def mem_set(words: np.ndarray, width: int, i: int, value: int) -> None:
+    lib().or_mem_set(words.ctypes.data_as(C.c_void_p), width, i, value & ((1 << 64) - 1))
+
+
+def mem_get(words: np.ndarray, width: int, i: int) -> int:
+    return int(lib().or_mem_get(words.ctypes.data_as(C.c_void_p), width, i))
+
+
+def mem_pack(width: int, values) -> np.ndarray:
+    v = np.ascontiguousarray(values, np.uint64)
+    w = mem_words(width, v.size)
+    for i, x in enumerate(v.tolist()):
+        mem_set(w, width, i, x)
+    return w
+
+
+def mem_join_nexts(dest: np.ndarray, dest_start: int, length: int, second: np.ndarray, second_start: int,
+                   second_len: int, state_len: int) -> None:
+    lib().or_mem_join_nexts(dest.ctypes.data_as(C.c_void_p), dest_start, length,
+                            second.ctypes.data_as(C.c_void_p), second_start, second_len, state_len)
+
+
+def mem_find_solution(words: np.ndarray, start: int, length: int, state_len: int, unknowns: int):
+    s, e = C.c_uint64(), C.c_uint64()
+    ok = lib().or_mem_find_solution(words.ctypes.data_as(C.c_void_p), start, length, state_len, unknowns,
+                                    C.byref(s), C.byref(e))
+    return {"start": s.value, "end": e.value} if ok else None

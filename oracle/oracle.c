@@ -736,3 +736,60 @@ void or_hm_add(uint32_t state_len, uint32_t arg_bit_place, uint64_t arg, const u
                      sol);
   }
 }
+
+/* ---- MemImage: packed (state_len + 1)-bit entries (src/greedysatostop.rs:339-475) ---- */
+static uint64_t mem_mask(uint32_t width) { return width >= 64 ? ~0ull : ((1ull << width) - 1ull); }
+
+/* MemImage::get, src/greedysatostop.rs:373-382 */
+uint64_t or_mem_get(const uint64_t* data, uint32_t width, uint64_t i) {
+  const uint64_t mask = mem_mask(width);
+  const uint64_t idx = (i * width) >> 6;
+  const uint32_t bit = (uint32_t)((i * width) & 63);
+  if (64 - bit >= width) return (data[idx] >> bit) & mask;
+  return ((data[idx] >> bit) | (data[idx + 1] << (64 - bit))) & mask;
+}
+
+/* MemImage::set, src/greedysatostop.rs:384-394 */
+void or_mem_set(uint64_t* data, uint32_t width, uint64_t i, uint64_t value) {
+  const uint64_t mask = mem_mask(width);
+  const uint64_t idx = (i * width) >> 6;
+  const uint32_t bit = (uint32_t)((i * width) & 63);
+  value &= mask;
+  data[idx] = (data[idx] & ~(mask << bit)) | (value << bit);
+  if (64 - bit < width) data[idx + 1] = (data[idx + 1] & ~(mask >> (64 - bit))) | (value >> (64 - bit));
+}
+
+/* MemImage::join_nexts, src/greedysatostop.rs:419-444 (the chunks there only split the
+ * index range among threads; each entry's update reads `second` alone) */
+void or_mem_join_nexts(uint64_t* dest, uint64_t dest_start, uint64_t len, const uint64_t* second,
+                       uint64_t second_start, uint64_t second_len, uint32_t state_len) {
+  (void)dest_start;
+  const uint32_t width = state_len + 1;
+  const uint64_t state_mask = mem_mask(width) >> 1, stop_mask = 1ull << state_len;
+  const uint64_t end = second_start + second_len;
+  for (uint64_t i = 0; i < len; i++) {
+    const uint64_t old = or_mem_get(dest, width, i);
+    const uint64_t old_next = old & state_mask;
+    if ((old & stop_mask) == 0 && second_start <= old_next && old_next < end)
+      or_mem_set(dest, width, i, or_mem_get(second, width, old_next - second_start));
+  }
+}
+
+/* MemImage::find_solution, src/greedysatostop.rs:446-475 */
+int or_mem_find_solution(const uint64_t* data, uint64_t start, uint64_t len, uint32_t state_len,
+                         uint32_t unknowns, uint64_t* sol_start, uint64_t* sol_end) {
+  const uint32_t width = state_len + 1;
+  const uint64_t before = (1ull << (state_len - unknowns)) - 1ull, step = before + 1ull;
+  uint64_t u = (start & before) != 0 ? (start & ~before) + step : (start & ~before);
+  const uint64_t end = start + len;
+  const uint64_t state_mask = mem_mask(width) >> 1, stop_mask = 1ull << state_len;
+  for (; u < end; u += step) {
+    const uint64_t v = or_mem_get(data, width, u - start);
+    if (v & stop_mask) {
+      *sol_start = u;
+      *sol_end = v & state_mask;
+      return 1;
+    }
+  }
+  return 0;
+}

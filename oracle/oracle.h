@@ -21,6 +21,10 @@
  *                      x3, join_hashmap_itself x1, add_to_hashmap x2), transcribed to
  *                      tests/golden/hashmap_cases.json; entries are placed and looked up with
  *                      or_hash64, so the add_to_hashmap collisions pin the hash too.
+ *   or_mem_*           the reference's test_mem_image (src/greedysatostop.rs:1145-1201: the
+ *                      packed layout checked bit by bit against its closed form) and
+ *                      test_mem_image_find_solution (1203-1268); or_mem_join_nexts by
+ *                      tables whose joins have closed forms (tests/test_oracle_memimage.py).
  */
 #ifndef NS_ORACLE_H
 #define NS_ORACLE_H
@@ -81,6 +85,20 @@ typedef struct {
   uint32_t defined;
 } or_hm_solution;
 uint64_t or_hash64(uint32_t bits, uint64_t value);
+
+/* MemImage (src/greedysatostop.rs:339-458): entry i of `width` = state_len + 1 bits at bit
+ * i*width of little-endian u64 words. */
+uint64_t or_mem_get(const uint64_t* data, uint32_t width, uint64_t i);
+void or_mem_set(uint64_t* data, uint32_t width, uint64_t i, uint64_t value);
+/* join_nexts (src/greedysatostop.rs:419-444): entries of dest (states dest_start..+len) that
+ * have not stopped and whose next state lies in second (second_start..+second_len) take
+ * second's entry for it; in place, in index order. */
+void or_mem_join_nexts(uint64_t* dest, uint64_t dest_start, uint64_t len, const uint64_t* second,
+                       uint64_t second_start, uint64_t second_len, uint32_t state_len);
+/* find_solution (src/greedysatostop.rs:446-475): the lowest unknown start in the image's
+ * range whose entry has stopped; returns 1 and (start, end) or 0. */
+int or_mem_find_solution(const uint64_t* data, uint64_t start, uint64_t len, uint32_t state_len,
+                         uint32_t unknowns, uint64_t* sol_start, uint64_t* sol_end);
 void or_hm_join_to(uint32_t state_len, uint32_t arg_bit_place, uint64_t arg, const uint32_t* outputs,
                    or_hash_entry* hm, uint64_t len);
 void or_hm_join_itself(uint32_t state_len, const or_hash_entry* in, or_hash_entry* out, uint64_t len,

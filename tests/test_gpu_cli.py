@@ -99,21 +99,32 @@ def test_naivesatostop_verify_walk(tmp_path):
 
 
 def test_naivesatostop_hashmap_search(tmp_path):
-    # 34 state bits (past the table's 33): the reference's hash-map search, with the next-state
-    # outputs in two 32-bit words; unknown starts are multiples of 2^32
+    # 34 state bits: the reference's hash-map search (-M; without it the packed table
+    # answers), next-state outputs in two 32-bit words; unknown starts are multiples of 2^32
     n, U, nxt = 34, 2, 0x2_1234_5678
     p = 1 << (n - U)
     f = tmp_path / "stop34.circuit"
     f.write_text(mk(point_stop_circuit(n, p, nxt)).text())
-    out = run("naivesatostop", "-t", "opencl:0", "-e", 30, "-H", 20, "-v", f, U)
+    out = run("naivesatostop", "-t", "opencl:0", "-M", "-e", 30, "-H", 20, "-v", f, U)
     assert "HashMap length: 33554432 bytes, bits=20" in out and "Unknown fill bits: 0" in out
     assert f"Solution: Solution {{ start: {p}, end: {nxt}, steps: 1 }}" in out
     assert "Verified: end=%d steps=1" % nxt in out and "INCORRECT" not in out
+    out = run("naivesatostop", "-t", "opencl:0", f, U)
+    assert "HashMap length" not in "".join(out)
+    assert f"Solution: Solution {{ start: {p}, end: {nxt}, steps: 1 }}" in out
     # STOP at a state that is no unknown start: every unknown start is a fixed point
     g = tmp_path / "nostop34.circuit"
     g.write_text(mk(point_stop_circuit(n, p + 1, nxt)).text())
-    assert "Unsatisfiable!" in run("naivesatostop", "-t", "opencl:0", "-e", 30, "-H", 20, g, U)
-    # --hashmap-search below 34 state bits: the reference's mem-image case 2 again
+    assert "Unsatisfiable!" in run("naivesatostop", "-t", "opencl:0", "-M", "-e", 30, "-H", 20, g, U)
+    # 36 state bits: past the table, the hash-map search without -M
+    n, U, nxt = 36, 2, 0x9_1234_5678
+    p = 2 << (n - U)
+    h36 = tmp_path / "stop36.circuit"
+    h36.write_text(mk(point_stop_circuit(n, p, nxt)).text())
+    out = run("naivesatostop", "-t", "opencl:0", "-e", 30, "-H", 20, h36, U)
+    assert "HashMap length: 33554432 bytes, bits=20" in out
+    assert f"Solution: Solution {{ start: {p}, end: {nxt}, steps: 1 }}" in out
+    # --hashmap-search at 24 state bits: the reference's mem-image case 2 again
     n, U, p, nxt = 24, 11, (3 << 20) + 79 * 8192, 58591
     h = tmp_path / "stop24.circuit"
     h.write_text(mk(point_stop_circuit(n, p, nxt)).text())

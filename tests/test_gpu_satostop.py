@@ -257,12 +257,13 @@ def test_u32_and_u64_entry_points_agree(ctx):
 
 
 def test_32_bit_state_space(ctx, monkeypatch):
-    # n = 32: a 2^32-entry table (16 GiB per output half, 32 GiB per jump table). The
-    # Feistel map is a permutation with one STOP transition (into the planted target), so
+    # n = 32: a 2^32-entry table, walked as the packed 33-bit image (17 GiB), then by the
+    # passes (16 GiB per output half, 32 GiB per jump table). The Feistel map is a permutation with one STOP transition (into the planted target), so
     # every stopping start ends at the target after a distinct number of evaluations; the
     # starts that stop within 64 evaluations are found independently by walking on the host
     n, U = 32, 14
     c, target = ns.Circuit.gen_satostop(7401, 16, 6, 100, U)
+    ctx.trim()
     r = ctx.satostop_solve(c, U)
     steps, ends = r["steps_arr"], r["ends_arr"]
     stopped = steps > 0
@@ -292,6 +293,9 @@ def test_32_bit_state_space(ctx, monkeypatch):
     assert r2["method"] == 0
     assert np.array_equal(r2["steps_arr"], steps) and np.array_equal(r2["ends_arr"], ends)
     monkeypatch.setenv("NS_JUMP_PASSES", "12")
-    r2 = ctx.satostop_solve(c, U, passes=True)
+    try:
+        r2 = ctx.satostop_solve(c, U, passes=True)
+    finally:
+        ctx.trim()  # 114 GiB of tables
     assert r2["passes"] == 12
     assert np.array_equal(r2["steps_arr"], steps) and np.array_equal(r2["ends_arr"], ends)
