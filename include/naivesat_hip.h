@@ -248,6 +248,36 @@ int ns_hashmap_step(ns_ctx* ctx, uint32_t which, uint32_t state_len, uint32_t ar
                     const uint32_t* outputs, uint64_t output_words, void* hm, void* hm_out, uint64_t hm_len,
                     uint32_t* fills, uint64_t fills_len, void* sol);
 
+/* greedysatostop's partitioned search (do_solve_with_cpu_builder_with_partitions,
+ * src/greedysatostop.rs:692-837) with the next-state image on disk (FileImage, 478-594, in
+ * `<file_prefix>greedy_temp_<ns>`, removed afterwards): the packed image is written chunk by
+ * chunk, then for stage = 0..n-1 and every partition i: partition i is joined in place with
+ * every partition j in order (j = i: a copy of its current self; otherwise j as last saved),
+ * searched for its lowest stopping unknown start, and saved. The joins run on the device, a
+ * partition (2^n / partitions packed entries) at a time in HBM. */
+typedef struct {
+  uint32_t found;      /* 1: start/end hold the solution; 0: none */
+  uint32_t stages;     /* stages begun (0 when no state's STOP output is 1) */
+  uint64_t start, end;
+  uint64_t partitions;
+  double build_ms;     /* table build and image write */
+  double join_ms;      /* device join kernels */
+  double io_ms;        /* partition reads, writes and host-device copies */
+  double compile_ms;
+  double wall_ms;
+} ns_partitioned_result;
+int ns_greedy_partitioned_solve(ns_ctx* ctx, const ns_circuit* c, uint32_t unknowns, uint64_t partitions,
+                                const char* file_prefix, ns_partitioned_result* res);
+
+/* FileImage on its own (src/greedysatostop.rs:478-594; host only, no device): create,
+ * append chunks of packed words, then read and rewrite partitions. */
+typedef struct ns_file_image ns_file_image;
+int ns_file_image_create(uint32_t state_len, uint64_t partitions, const char* prefix, ns_file_image** out);
+int ns_file_image_save_chunk(ns_file_image* fi, const uint64_t* words, uint64_t nwords);
+int ns_file_image_load_partition(ns_file_image* fi, uint64_t part, uint64_t* words, uint64_t words_cap);
+int ns_file_image_save_partition(ns_file_image* fi, uint64_t part, const uint64_t* words, uint64_t nwords);
+void ns_file_image_free(ns_file_image* fi);
+
 /* Table build alone (replaces greedysatostop's first-nexts launch with the transpose
  * epilogue, src/greedysatostop.rs:1042-1072; identity layout pinned by
  * src/satostop.rs:1788-1915): table_out gets 2^n u32, entry s = sum_i output_i(s) << i.

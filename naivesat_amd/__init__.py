@@ -75,6 +75,12 @@ class _StopResult(C.Structure):
                 ("reserved", C.c_uint32), ("walk_evaluations", C.c_uint64)]
 
 
+class _PartResult(C.Structure):
+    _fields_ = [("found", C.c_uint32), ("stages", C.c_uint32), ("start", C.c_uint64), ("end", C.c_uint64),
+                ("partitions", C.c_uint64), ("build_ms", C.c_double), ("join_ms", C.c_double),
+                ("io_ms", C.c_double), ("compile_ms", C.c_double), ("wall_ms", C.c_double)]
+
+
 class _HmOpts(C.Structure):
     _fields_ = [("hashmap_len_bits", C.c_uint32), ("elem_bits", C.c_uint32), ("unknown_fill_bits", C.c_int32),
                 ("max_predecessors", C.c_uint32), ("iters_per_clear_predecessors", C.c_uint32),
@@ -151,6 +157,12 @@ def load_library() -> C.CDLL:
         "ns_probe_gather": (I, [P, U32, U32, U32, U32, C.POINTER(C.c_double), C.POINTER(U64)]),
         "ns_pack_entries": (I, [P, U32, P, U64, P, U64]),
         "ns_ctx_trim": (I, [P]),
+        "ns_greedy_partitioned_solve": (I, [P, P, U32, U64, C.c_char_p, C.POINTER(_PartResult)]),
+        "ns_file_image_create": (I, [U32, U64, C.c_char_p, C.POINTER(P)]),
+        "ns_file_image_save_chunk": (I, [P, P, U64]),
+        "ns_file_image_load_partition": (I, [P, U64, P, U64]),
+        "ns_file_image_save_partition": (I, [P, U64, P, U64]),
+        "ns_file_image_free": (None, [P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -168,7 +180,9 @@ def exported_symbols() -> list:
             "ns_gen_satostop", "ns_ctx_create", "ns_ctx_destroy", "ns_sat_solve", "ns_qsat_solve",
             "ns_satostop_solve", "ns_table_build", "ns_satostop_pass_times", "ns_native_body", "ns_serve", "ns_native_encode",
             "ns_native_code", "ns_satostop_solve_wide", "ns_satostop_hashmap_solve", "ns_hashmap_step",
-            "ns_probe_gather", "ns_pack_entries", "ns_ctx_trim"]
+            "ns_probe_gather", "ns_pack_entries", "ns_ctx_trim", "ns_greedy_partitioned_solve",
+            "ns_file_image_create", "ns_file_image_save_chunk", "ns_file_image_load_partition",
+            "ns_file_image_save_partition", "ns_file_image_free"]
 
 
 def _check(status: int) -> None:
@@ -460,6 +474,13 @@ class Context:
         out = [{k: getattr(res[i], k) for k, _ in _RequestResult._fields_} for i in range(n)]
         return out, total.value
 
+    def greedy_partitioned_solve(self, c: Circuit, unknowns: int, partitions: int, file_prefix: str = "") -> dict:
+        """greedysatostop's partitioned search with the image on disk (see the C header)."""
+        r = _PartResult()
+        _check(load_library().ns_greedy_partitioned_solve(self._h, c._h, unknowns, partitions,
+                                                          file_prefix.encode(), C.byref(r)))
+        return {k: getattr(r, k) for k, _ in _PartResult._fields_}
+
     def trim(self) -> None:
         """Releases the SATOSTOP tables and images the context keeps between solves."""
         _check(load_library().ns_ctx_trim(self._h))
@@ -502,3 +523,38 @@ def qsat_result_string(r: dict) -> str:
     if r["first_all"]:
         return f"(Unsatisfiable: {sol:0{fb}b})" if sol is not None else "(Satisfiable)"
     return f"(Satisfiable: {sol:0{fb}b})" if sol is not None else "(Unsatisfiable)"
+
+
+class FileImage:
+    """greedysatostop's on-disk image (host only): packed words in, partitions out."""
+
+    def __init__(self, state_len: int, partitions: int, prefix: str = ""):
+        h = C.c_void_p()
+        _check(load_library().ns_file_image_create(state_len, partitions, prefix.encode(), C.byref(h)))
+        self._h = h.value
+        self.state_len, self.partitions = state_len, partitions
+        self.partition_words = ((1 << state_len) // partitions) * (state_len + 1) // 64
+
+    def save_chunk(self, words: np.ndarray) -> None:
+        w = np.ascontiguousarray(words, np.uint64)
+        _check(load_library().ns_file_image_save_chunk(self._h, _ptr(w), w.size))
+
+    def load_partition(self, part: int) -> np.ndarray:
+        w = np.zeros(self.partition_words, np.uint64)
+        _check(load_library().ns_file_image_load_partition(self._h, part, _ptr(w), w.size))
+        return w
+
+    def save_partition(self, part: int, words: np.ndarray) -> None:
+        w = np.ascontiguousarray(words, np.uint64)
+        _check(load_library().ns_file_image_save_partition(self._h, part, _ptr(w), w.size))
+
+    def close(self) -> None:
+        if self._h:
+            load_library().ns_file_image_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

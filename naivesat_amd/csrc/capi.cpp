@@ -10,6 +10,7 @@
 #include "asmgen.h"
 #include "circuit.h"
 #include "runtime.h"
+#include "file_image.h"
 #include "serve.h"
 #include "static_kernels.h"
 #include "solvers.h"
@@ -411,6 +412,59 @@ int ns_satostop_pass_times(ns_ctx* ctx, double* pass_ms, uint32_t cap, uint32_t*
       if (pass_ms) pass_ms[k] = v[k];
     *written = k;
   });
+}
+
+int ns_greedy_partitioned_solve(ns_ctx* ctx, const ns_circuit* c, uint32_t unknowns, uint64_t partitions,
+                                const char* file_prefix, ns_partitioned_result* res) {
+  return guard([&] {
+    need(ctx && c && res, "null argument");
+    ns::greedy_partitioned_solve(*ctx->ctx.main, c->c, unknowns, partitions, file_prefix ? file_prefix : "", *res);
+  });
+}
+
+}  // extern "C"
+
+struct ns_file_image {
+  ns::FileImage fi;
+};
+
+extern "C" {
+
+int ns_file_image_create(uint32_t state_len, uint64_t partitions, const char* prefix, ns_file_image** out) {
+  return guard([&] {
+    need(out, "null argument");
+    *out = new ns_file_image{ns::FileImage(state_len, partitions, prefix ? prefix : "")};
+  });
+}
+
+int ns_file_image_save_chunk(ns_file_image* fi, const uint64_t* words, uint64_t nwords) {
+  return guard([&] {
+    need(fi && words, "null argument");
+    fi->fi.save_chunk(words, nwords);
+  });
+}
+
+int ns_file_image_load_partition(ns_file_image* fi, uint64_t part, uint64_t* words, uint64_t words_cap) {
+  return guard([&] {
+    need(fi && words, "null argument");
+    need(words_cap >= fi->fi.partition_words(), "words array too small");
+    fi->fi.load_partition(part, words);
+  });
+}
+
+int ns_file_image_save_partition(ns_file_image* fi, uint64_t part, const uint64_t* words, uint64_t nwords) {
+  return guard([&] {
+    need(fi && words, "null argument");
+    need(nwords == fi->fi.partition_words(), "a partition has partition_words words");
+    fi->fi.save_partition(part, words);
+  });
+}
+
+void ns_file_image_free(ns_file_image* fi) {
+  try {
+    delete fi;
+  } catch (...) {
+  }
 }
 
 int ns_pack_entries(ns_ctx* ctx, uint32_t width, const uint64_t* values, uint64_t count, uint64_t* words,

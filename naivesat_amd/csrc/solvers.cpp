@@ -13,6 +13,7 @@
 
 #include "asmgen.h"
 #include "codegen.h"
+#include "file_image.h"
 #include "hashmap_kernels.h"
 #include "static_kernels.h"
 
@@ -574,18 +575,19 @@ uint32_t waypoint_bits(uint32_t n, uint32_t unknowns) {
 
 namespace {
 
-// Packed next-state image of c in ctx.tab_img: 2^n entries of n + 1 bits (next state, STOP
-// at bit n) in the reference's MemImage layout. Built 2^30 states at a time (all of them
-// below 30 state bits): the table kernels write outputs 0..31 (and 32..n) of the chunk to
-// tab_a (tab_e), the pack kernel appends the chunk's words. Returns kernel ms; *any = some
-// state's STOP output is 1.
-double build_packed_table(Lane& ctx, const Circuit& c, const ns_satostop_opts& o, bool* any, double* compile_ms,
-                          uint32_t* cone, TableKernelInfo* info) {
+// Packed next-state image of c: 2^n entries of n + 1 bits (next state, STOP at bit n) in the
+// reference's MemImage layout, built 2^cb states at a time: the table kernels write outputs
+// 0..31 (and 32..n) of the chunk to tab_a (tab_e) and the pack kernel writes the chunk's
+// words to chunk_out(ci); after(ci) runs once they are enqueued. Returns kernel ms of the
+// whole build (the stream is synchronised at the end); *any = some state's STOP output is 1.
+double build_packed_chunks(Lane& ctx, const Circuit& c, const ns_satostop_opts& o, uint32_t cb,
+                           const std::function<unsigned long long*(uint64_t)>& chunk_out,
+                           const std::function<void(uint64_t)>& after, bool* any, double* compile_ms,
+                           uint32_t* cone, TableKernelInfo* info) {
   const uint32_t n = c.n, width = n + 1;
-  if (n < 6 || n > 35) throw Error(ST_INTERNAL, "packed images take 6..35 state bits");
+  if (n < 6 || n > 62 || cb < 6 || cb > n) throw Error(ST_INTERNAL, "packed images take 6..62 state bits");
   const bool two = width > 32;
   const uint32_t split = two ? 32 : width;
-  const uint32_t cb = std::min(n, 30u);
   const uint64_t chunk = 1ull << cb, chunks = 1ull << (n - cb), words_chunk = chunk >> 5;
   Geometry g = pick_geometry(words_chunk, o.group_len, o.words_per_lane ? o.words_per_lane : 2, false);
   std::vector<uint32_t> vb(n);
@@ -614,13 +616,12 @@ double build_packed_table(Lane& ctx, const Circuit& c, const ns_satostop_opts& o
   const uint64_t tab_bytes = (words_chunk + wpb - 1) / wpb * wpb * 32 * 4;
   ctx.tab_a.ensure(tab_bytes);
   if (two) ctx.tab_e.ensure(tab_bytes);
-  const uint64_t total_words = ((1ull << n) * width) >> 6;
-  ctx.tab_img.ensure((total_words + 1) * 8);  // one word of padding after the last entry
-  unsigned long long* img = ctx.tab_img.as<unsigned long long>();
   unsigned int* flags = reinterpret_cast<unsigned int*>(ctx.stats.as<char>() + 64);  // [0] STOP seen, [1] unused
   NS_HIP(launch_fill_u32(ctx.stream, flags, 0u, 2));
-  NS_HIP(launch_fill_u64(ctx.stream, img + total_words, 0ull, 1));
-  NS_HIP(hipEventRecord(ctx.ev0, ctx.stream));
+  hipEvent_t e0, e1;
+  NS_HIP(hipEventCreate(&e0));
+  NS_HIP(hipEventCreate(&e1));
+  NS_HIP(hipEventRecord(e0, ctx.stream));
   for (uint64_t ci = 0; ci < chunks; ci++) {
     const uint64_t w0 = ci * words_chunk;
     for (int h = 0; h < (two ? 2 : 1); h++) {
@@ -632,14 +633,34 @@ double build_packed_table(Lane& ctx, const Circuit& c, const ns_satostop_opts& o
       launch_range(ctx, kern[h].mod->fn, g, w0, words_chunk, extra, kern[h].native ? 6 : 2, w0 + words_chunk);
     }
     NS_HIP(launch_pack(ctx.stream, ctx.tab_a.as<unsigned int>(), two ? ctx.tab_e.as<unsigned int>() : nullptr, split,
-                       width, chunk, img + ((ci * chunk * width) >> 6)));
+                       width, chunk, chunk_out(ci)));
+    if (after) after(ci);
   }
-  NS_HIP(hipEventRecord(ctx.ev1, ctx.stream));
+  NS_HIP(hipEventRecord(e1, ctx.stream));
   unsigned int hf = 0;
   NS_HIP(hipMemcpyAsync(&hf, flags, 4, hipMemcpyDeviceToHost, ctx.stream));
   NS_HIP(hipStreamSynchronize(ctx.stream));
+  float ms = 0.f;
+  NS_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
   *any = hf != 0;
-  return ctx.elapsed_ms();
+  return ms;
+}
+
+// The whole packed image in ctx.tab_img (one word of padding after the last entry).
+double build_packed_table(Lane& ctx, const Circuit& c, const ns_satostop_opts& o, bool* any, double* compile_ms,
+                          uint32_t* cone, TableKernelInfo* info) {
+  const uint32_t n = c.n, width = n + 1;
+  if (n > 35) throw Error(ST_INTERNAL, "device images take at most 35 state bits");
+  const uint32_t cb = std::min(n, 30u);
+  const uint64_t total_words = ((1ull << n) * width) >> 6;
+  ctx.tab_img.ensure((total_words + 1) * 8);
+  unsigned long long* img = ctx.tab_img.as<unsigned long long>();
+  NS_HIP(launch_fill_u64(ctx.stream, img + total_words, 0ull, 1));
+  return build_packed_chunks(
+      ctx, c, o, cb, [&](uint64_t ci) { return img + (((ci << cb) * width) >> 6); }, nullptr, any, compile_ms, cone,
+      info);
 }
 
 // The full search as waypoint walks (see k_wp_walk) over the table tv: fills the per-start
@@ -1065,6 +1086,129 @@ void satostop_hashmap_solve(Lane& ctx, const Circuit& c, uint32_t unknowns, cons
   kms_total = ctx.elapsed_ms();
   r.resolved_unknowns = host.resolved_unknowns;
   r.kernel_ms = kms_total;
+  r.wall_ms = ms_since(t0);
+}
+
+// ---------------------------------------------------------------------------
+// greedysatostop's partitioned search with the image on disk
+// ---------------------------------------------------------------------------
+void greedy_partitioned_solve(Lane& ctx, const Circuit& c, uint32_t unknowns, uint64_t partitions,
+                              const std::string& file_prefix, ns_partitioned_result& r) {
+  auto t0 = clk::now();
+  std::memset(&r, 0, sizeof(r));
+  const uint32_t n = c.n, width = n + 1;
+  if (c.outputs.size() != (size_t)n + 1) throw Error(ST_INVALID, "SATOSTOP needs n inputs and n+1 outputs");
+  if (n < 10 || n > 48) throw Error(ST_UNSUPPORTED, "the partitioned search takes 10..48 state bits");
+  if (unknowns > n) throw Error(ST_INVALID, "more unknowns than state bits");
+  // do_solve_with_cpu_builder_with_partitions: at least two partitions, a power of two
+  if (partitions < 2 || (partitions & (partitions - 1))) throw Error(ST_INVALID, "partitions: a power of two >= 2");
+  FileImage fi(n, partitions, file_prefix);
+  r.partitions = partitions;
+  NS_HIP(hipSetDevice(ctx.device));
+  // the image, chunk by chunk, through a device chunk buffer and a pinned host buffer
+  const uint32_t cb = std::min(n, 30u);
+  const uint64_t chunk_words = ((1ull << cb) * width) >> 6;
+  const uint64_t part_words = fi.partition_words(), part_states = fi.partition_states();
+  const uint64_t host_words = std::max(chunk_words, part_words);
+  unsigned long long* host = nullptr;
+  NS_HIP(hipHostMalloc(reinterpret_cast<void**>(&host), host_words * 8, hipHostMallocDefault));
+  struct HostFree {
+    unsigned long long* p;
+    ~HostFree() { (void)hipHostFree(p); }
+  } host_free{host};
+  ctx.tab_img.ensure(chunk_words * 8);
+  unsigned long long* chunk_dev = ctx.tab_img.as<unsigned long long>();
+  bool any = false;
+  TableKernelInfo tki;
+  ns_satostop_opts o{};
+  uint32_t cone = 0;
+  auto tb = clk::now();
+  build_packed_chunks(
+      ctx, c, o, cb, [&](uint64_t) { return chunk_dev; },
+      [&](uint64_t) {
+        NS_HIP(hipMemcpyAsync(host, chunk_dev, chunk_words * 8, hipMemcpyDeviceToHost, ctx.stream));
+        NS_HIP(hipStreamSynchronize(ctx.stream));
+        fi.save_chunk(reinterpret_cast<const uint64_t*>(host), chunk_words);
+      },
+      &any, &r.compile_ms, &cone, &tki);
+  r.build_ms = ms_since(tb);
+  ctx.tab_img.release();
+  if (any) {
+    // partitions in HBM: dest (ping-pong) and second
+    DevBuf d_dest, d_next, d_second;
+    d_dest.ensure((part_words + 1) * 8);
+    d_next.ensure((part_words + 1) * 8);
+    d_second.ensure((part_words + 1) * 8);
+    unsigned long long* dest = d_dest.as<unsigned long long>();
+    unsigned long long* nxt = d_next.as<unsigned long long>();
+    unsigned long long* second = d_second.as<unsigned long long>();
+    unsigned long long* best = ctx.stats.as<unsigned long long>() + 2;
+    hipEvent_t j0, j1;
+    NS_HIP(hipEventCreate(&j0));
+    NS_HIP(hipEventCreate(&j1));
+    struct EvFree {
+      hipEvent_t a, b;
+      ~EvFree() {
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+      }
+    } ev_free{j0, j1};
+    auto load = [&](uint64_t part, unsigned long long* dst) {
+      auto ti = clk::now();
+      fi.load_partition(part, reinterpret_cast<uint64_t*>(host));
+      NS_HIP(hipMemcpyAsync(dst, host, part_words * 8, hipMemcpyHostToDevice, ctx.stream));
+      NS_HIP(hipStreamSynchronize(ctx.stream));
+      r.io_ms += ms_since(ti);
+    };
+    bool done = false;
+    for (uint32_t stage = 0; stage < n && !done; stage++) {
+      r.stages = stage + 1;
+      for (uint64_t i = 0; i < partitions && !done; i++) {
+        load(i, dest);
+        for (uint64_t j = 0; j < partitions; j++) {
+          if (j != i) {
+            load(j, second);
+          } else {
+            NS_HIP(hipMemcpyAsync(second, dest, part_words * 8, hipMemcpyDeviceToDevice, ctx.stream));
+          }
+          NS_HIP(hipEventRecord(j0, ctx.stream));
+          NS_HIP(launch_mem_join(ctx.stream, dest, part_states, second, j * part_states, part_states, n, nxt));
+          NS_HIP(hipEventRecord(j1, ctx.stream));
+          NS_HIP(hipEventSynchronize(j1));
+          float ms = 0.f;
+          NS_HIP(hipEventElapsedTime(&ms, j0, j1));
+          r.join_ms += ms;
+          std::swap(dest, nxt);
+        }
+        // the lowest stopping unknown start of partition i
+        NS_HIP(launch_fill_u64(ctx.stream, best, ~0ull, 1));
+        NS_HIP(launch_mem_find(ctx.stream, dest, i * part_states, part_states, n, unknowns, best));
+        unsigned long long b = ~0ull;
+        NS_HIP(hipMemcpyAsync(&b, best, 8, hipMemcpyDeviceToHost, ctx.stream));
+        NS_HIP(hipStreamSynchronize(ctx.stream));
+        if (b != ~0ull) {
+          // its entry: the two words it may span
+          const uint64_t bit = (b - i * part_states) * width, idx = bit >> 6;
+          unsigned long long w2[2] = {0, 0};
+          NS_HIP(hipMemcpyAsync(w2, dest + idx, 16, hipMemcpyDeviceToHost, ctx.stream));
+          NS_HIP(hipStreamSynchronize(ctx.stream));
+          const uint32_t sh = (uint32_t)(bit & 63);
+          unsigned long long v = w2[0] >> sh;
+          if (sh + width > 64) v |= w2[1] << (64 - sh);
+          r.found = 1;
+          r.start = b;
+          r.end = v & ((1ull << n) - 1ull);
+          done = true;
+          break;
+        }
+        auto ts = clk::now();
+        NS_HIP(hipMemcpyAsync(host, dest, part_words * 8, hipMemcpyDeviceToHost, ctx.stream));
+        NS_HIP(hipStreamSynchronize(ctx.stream));
+        fi.save_partition(i, reinterpret_cast<const uint64_t*>(host));
+        r.io_ms += ms_since(ts);
+      }
+    }
+  }
   r.wall_ms = ms_since(t0);
 }
 

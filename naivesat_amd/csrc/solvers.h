@@ -1,5 +1,6 @@
 // Solver drivers: SAT, QSAT and SATOSTOP searches on one device context.
 #pragma once
+#include <string>
 #include <vector>
 
 #include "../../include/naivesat_hip.h"
@@ -24,6 +25,10 @@ void hashmap_step(Lane& ctx, uint32_t which, uint32_t state_len, uint32_t arg_bi
                   uint32_t unknown_bits, uint32_t unknown_fill_bits, uint32_t max_predecessors, bool test,
                   const uint32_t* outputs, uint64_t output_words, void* hm, void* hm_out, uint64_t hm_len,
                   uint32_t* fills, uint64_t fills_len, void* sol);
+
+// greedysatostop's partitioned search with the image on disk
+void greedy_partitioned_solve(Lane& ctx, const Circuit& c, uint32_t unknowns, uint64_t partitions,
+                              const std::string& file_prefix, ns_partitioned_result& r);
 
 // per-pass jump times of the last satostop_solve on a context
 const std::vector<double>& last_pass_times(Lane& ctx);

@@ -41,6 +41,13 @@ hipError_t launch_wp_starts(hipStream_t s, const void* rec, unsigned int n, unsi
 // lanes must be a multiple of 256; out holds one word per lane
 hipError_t launch_gather_probe(hipStream_t s, const unsigned int* t, unsigned int bits, unsigned int reads,
                                unsigned long long lanes, unsigned int* out, bool nontemporal);
+// greedysatostop's partitioned search over packed (state_len+1)-bit images
+hipError_t launch_mem_join(hipStream_t s, const unsigned long long* dest, unsigned long long len,
+                           const unsigned long long* second, unsigned long long s_start, unsigned long long s_len,
+                           unsigned int state_len, unsigned long long* out);
+hipError_t launch_mem_find(hipStream_t s, const unsigned long long* words, unsigned long long start,
+                           unsigned long long len, unsigned int state_len, unsigned int unknowns,
+                           unsigned long long* best);
 hipError_t launch_fill_u64(hipStream_t s, unsigned long long* p, unsigned long long v,
                            unsigned long long count);
 hipError_t launch_fill_u32(hipStream_t s, unsigned int* p, unsigned int v, unsigned long long count);

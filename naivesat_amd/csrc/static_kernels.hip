@@ -404,6 +404,72 @@ __global__ void __launch_bounds__(256) k_pack(const unsigned int* __restrict__ l
   }
 }
 
+// ---------------------------------------------------------------------------
+// greedysatostop's partitioned search over packed images (src/greedysatostop.rs:419-475).
+// ---------------------------------------------------------------------------
+__device__ inline unsigned long long mem_get(const unsigned long long* __restrict__ w, unsigned int width,
+                                             unsigned long long i) {
+  const unsigned long long bit = i * width, idx = bit >> 6;
+  const unsigned int b = (unsigned int)(bit & 63ull);
+  unsigned long long v = w[idx] >> b;
+  if (b + width > 64u) v |= w[idx + 1] << (64u - b);
+  return width >= 64 ? v : v & ((1ull << width) - 1ull);
+}
+
+// join_nexts: entry e of dest (len entries) that has not stopped and whose next state lies in
+// [s_start, s_start + s_len) takes second's entry for it. Out of place (every output word is
+// written by one lane from the entries it overlaps, read from the unchanged dest), so each
+// entry's update reads the old dest and second only, as in the reference.
+__global__ void __launch_bounds__(256) k_mem_join(const unsigned long long* __restrict__ dest, unsigned long long len,
+                                                  const unsigned long long* __restrict__ second,
+                                                  unsigned long long s_start, unsigned long long s_len,
+                                                  unsigned int state_len, unsigned long long* __restrict__ out) {
+  const unsigned int width = state_len + 1;
+  const unsigned long long state_mask = (1ull << state_len) - 1ull, stop = 1ull << state_len;
+  const unsigned long long words = (len * width) >> 6;
+  const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
+  for (unsigned long long w = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; w < words; w += stride) {
+    const unsigned long long bit = w << 6;
+    unsigned long long e = bit / width;
+    unsigned int off = (unsigned int)(bit - e * width), pos = 0;
+    unsigned long long word = 0;
+    while (pos < 64u && e < len) {
+      unsigned long long v = mem_get(dest, width, e);
+      const unsigned long long nx = v & state_mask;
+      if ((v & stop) == 0ull && nx >= s_start && nx - s_start < s_len) v = mem_get(second, width, nx - s_start);
+      word |= (v >> off) << pos;
+      pos += width - off;
+      off = 0;
+      e++;
+    }
+    out[w] = word;
+  }
+}
+
+// find_solution: *best = min(*best, lowest unknown start in [start, start + len) whose entry
+// has stopped); unknown starts are the multiples of 2^(state_len - unknowns).
+__global__ void __launch_bounds__(256) k_mem_find(const unsigned long long* __restrict__ words, unsigned long long start,
+                                                  unsigned long long len, unsigned int state_len, unsigned int unknowns,
+                                                  unsigned long long* best) {
+  const unsigned int width = state_len + 1, low = state_len - unknowns;
+  const unsigned long long step = 1ull << low;
+  const unsigned long long first = (start + step - 1ull) & ~(step - 1ull), end = start + len;
+  const unsigned long long count = first < end ? (end - first + step - 1ull) >> low : 0ull;
+  const unsigned long long limit = (count + 63ull) & ~63ull;  // whole waves take part
+  const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
+  unsigned long long mine = ~0ull;
+  for (unsigned long long k = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; k < limit; k += stride) {
+    if (k >= count) continue;
+    const unsigned long long u = first + (k << low);
+    if ((mem_get(words, width, u - start) >> state_len) & 1ull) mine = u < mine ? u : mine;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long o = __shfl_xor(mine, off);
+    mine = o < mine ? o : mine;
+  }
+  if ((threadIdx.x & 63u) == 0u && mine != ~0ull) atomicMin(best, mine);
+}
+
 __global__ void k_fill_u64(unsigned long long* p, unsigned long long v, unsigned long long count) {
   const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
   for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride)
@@ -518,6 +584,24 @@ hipError_t launch_gather_probe(hipStream_t s, const unsigned int* t, unsigned in
     hipLaunchKernelGGL(k_gather_probe<true>, dim3(blocks_for(lanes, 256)), dim3(256), 0, s, t, bits, reads, out);
   else
     hipLaunchKernelGGL(k_gather_probe<false>, dim3(blocks_for(lanes, 256)), dim3(256), 0, s, t, bits, reads, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_mem_join(hipStream_t s, const unsigned long long* dest, unsigned long long len,
+                           const unsigned long long* second, unsigned long long s_start, unsigned long long s_len,
+                           unsigned int state_len, unsigned long long* out) {
+  hipLaunchKernelGGL(k_mem_join, dim3(looped_blocks((len * (state_len + 1)) >> 6, 256)), dim3(256), 0, s, dest, len,
+                     second, s_start, s_len, state_len, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_mem_find(hipStream_t s, const unsigned long long* words, unsigned long long start,
+                           unsigned long long len, unsigned int state_len, unsigned int unknowns,
+                           unsigned long long* best) {
+  const unsigned int low = state_len - unknowns;
+  const unsigned long long starts = (len >> low) + 1ull;
+  hipLaunchKernelGGL(k_mem_find, dim3(looped_blocks((starts + 63ull) & ~63ull, 256)), dim3(256), 0, s, words, start,
+                     len, state_len, unknowns, best);
   return hipGetLastError();
 }
 

@@ -71,6 +71,10 @@ def lib() -> C.CDLL:
         L.or_mem_get.argtypes = [P, U32, U64]
         L.or_mem_set.restype = None
         L.or_mem_set.argtypes = [P, U32, U64, U64]
+        L.or_mem_pack.restype = None
+        L.or_mem_pack.argtypes = [P, U64, U32, P]
+        L.or_mem_unpack.restype = None
+        L.or_mem_unpack.argtypes = [P, U64, U32, P]
         L.or_mem_join_nexts.restype = None
         L.or_mem_join_nexts.argtypes = [P, U64, U64, P, U64, U64, U32]
         L.or_mem_find_solution.restype = C.c_int
@@ -239,9 +243,40 @@ def mem_get(words: np.ndarray, width: int, i: int) -> int:
 def mem_pack(width: int, values) -> np.ndarray:
     v = np.ascontiguousarray(values, np.uint64)
     w = mem_words(width, v.size)
-    for i, x in enumerate(v.tolist()):
-        mem_set(w, width, i, x)
+    lib().or_mem_pack(v.ctypes.data_as(C.c_void_p), v.size, width, w.ctypes.data_as(C.c_void_p))
     return w
+
+
+def mem_unpack(width: int, words: np.ndarray, count: int) -> np.ndarray:
+    v = np.zeros(count, np.uint64)
+    lib().or_mem_unpack(words.ctypes.data_as(C.c_void_p), count, width, v.ctypes.data_as(C.c_void_p))
+    return v
+
+
+def greedy_partitioned(table: np.ndarray, state_len: int, unknowns: int, partitions: int):
+    """do_solve_with_cpu_builder_with_partitions (src/greedysatostop.rs:692-837) on a full
+    next-state table (entries next | STOP << n): the image split into partitions, every
+    stage joining each partition in place with every partition in order (itself as a copy
+    of its current state, the others as last saved), then searching it. Returns
+    {start, end} or None."""
+    width = state_len + 1
+    stop = 1 << state_len
+    if not np.any(np.asarray(table, np.uint64) & np.uint64(stop)):
+        return None
+    part_len = (1 << state_len) // partitions
+    parts = [mem_pack(width, np.asarray(table[i * part_len:(i + 1) * part_len], np.uint64))
+             for i in range(partitions)]
+    for _stage in range(state_len):
+        for i in range(partitions):
+            dest = parts[i].copy()
+            for j in range(partitions):
+                second = parts[j].copy() if j != i else dest.copy()
+                mem_join_nexts(dest, i * part_len, part_len, second, j * part_len, part_len, state_len)
+            sol = mem_find_solution(dest, i * part_len, part_len, state_len, unknowns)
+            if sol is not None:
+                return sol
+            parts[i] = dest
+    return None
 
 
 def mem_join_nexts(dest: np.ndarray, dest_start: int, length: int, second: np.ndarray, second_start: int,

@@ -759,6 +759,14 @@ void or_mem_set(uint64_t* data, uint32_t width, uint64_t i, uint64_t value) {
   if (64 - bit < width) data[idx + 1] = (data[idx + 1] & ~(mask >> (64 - bit))) | (value >> (64 - bit));
 }
 
+void or_mem_pack(const uint64_t* values, uint64_t count, uint32_t width, uint64_t* words) {
+  for (uint64_t i = 0; i < count; i++) or_mem_set(words, width, i, values[i]);
+}
+
+void or_mem_unpack(const uint64_t* words, uint64_t count, uint32_t width, uint64_t* values) {
+  for (uint64_t i = 0; i < count; i++) values[i] = or_mem_get(words, width, i);
+}
+
 /* MemImage::join_nexts, src/greedysatostop.rs:419-444 (the chunks there only split the
  * index range among threads; each entry's update reads `second` alone) */
 void or_mem_join_nexts(uint64_t* dest, uint64_t dest_start, uint64_t len, const uint64_t* second,

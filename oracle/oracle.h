@@ -90,6 +90,9 @@ uint64_t or_hash64(uint32_t bits, uint64_t value);
  * i*width of little-endian u64 words. */
 uint64_t or_mem_get(const uint64_t* data, uint32_t width, uint64_t i);
 void or_mem_set(uint64_t* data, uint32_t width, uint64_t i, uint64_t value);
+/* every entry of values[0..count) through or_mem_set / or_mem_get */
+void or_mem_pack(const uint64_t* values, uint64_t count, uint32_t width, uint64_t* words);
+void or_mem_unpack(const uint64_t* words, uint64_t count, uint32_t width, uint64_t* values);
 /* join_nexts (src/greedysatostop.rs:419-444): entries of dest (states dest_start..+len) that
  * have not stopped and whose next state lies in second (second_start..+second_len) take
  * second's entry for it; in place, in index order. */

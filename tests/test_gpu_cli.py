@@ -131,3 +131,20 @@ def test_naivesatostop_hashmap_search(tmp_path):
     out = run("naivesatostop", "-t", "opencl:0", "-M", "-e", 16, "-H", 18, h, U)
     assert "Unknown fill bits: 0" in out
     assert f"Solution: Solution {{ start: {p}, end: {nxt}, steps: 1 }}" in out
+
+
+def test_greedysatostop_partitions(tmp_path):
+    # -p: the reference's partitioned search with the image on disk under -x
+    n, U, p, nxt = 24, 11, (3 << 20) + 79 * 8192, 58591
+    f = tmp_path / "stop.circuit"
+    f.write_text(mk(point_stop_circuit(n, p, nxt)).text())
+    img = tmp_path / "img"
+    img.mkdir()
+    out = run("greedysatostop", "-p", 8, "-x", str(img) + "/", "-v", f, U)
+    assert "Load partition bits: 5" in out and "Stage 0 / 24" in out
+    assert f"Solution: Solution {{ start: {p}, end: {nxt} }}" in out
+    assert f"Verified: end={nxt}" in out and "INCORRECT" not in out
+    assert not any(img.iterdir())  # the image file is removed afterwards
+    g = tmp_path / "nostop.circuit"
+    g.write_text(mk(point_stop_circuit(n, p + 1, nxt)).text())
+    assert "Unsatisfiable!" in run("greedysatostop", "-p", 4, "-x", str(img) + "/", g, U)
