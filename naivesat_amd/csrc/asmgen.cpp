@@ -9,6 +9,7 @@
 #include "asmgen.h"
 
 #include <algorithm>
+#include <cstdlib>
 #include <sstream>
 
 namespace ns {
@@ -398,7 +399,10 @@ class Allocator {
 // list scheduling over the lane-varying ops (an op is ready when its lane-varying sources
 // exist), preferring the op that ends the most lifetimes, then program order. Wave-uniform
 // ops are emitted just before their first consumer, so scalar values live briefly.
-LutProgram pressure_order(const LutProgram& p, const std::vector<uint32_t>& out_wires, uint32_t wires) {
+// With wgroup (a group number per wire), the ready ops of the lowest group go first, so a
+// circuit built as a sequence of groups is scheduled one group after another.
+LutProgram pressure_order(const LutProgram& p, const std::vector<uint32_t>& out_wires, uint32_t wires,
+                          const std::vector<uint32_t>* wgroup = nullptr) {
   const uint32_t m = (uint32_t)p.ops.size();
   std::vector<int32_t> prod(wires, -1);
   for (uint32_t i = 0; i < m; i++) prod[p.ops[i].dst] = (int32_t)i;
@@ -460,8 +464,15 @@ LutProgram pressure_order(const LutProgram& p, const std::vector<uint32_t>& out_
   while (!ready.empty()) {
     size_t bi = 0;
     int bcost = 0;
+    uint32_t gmin = 0;
+    if (wgroup) {
+      gmin = ~0u;
+      for (uint32_t r : ready) gmin = std::min(gmin, (*wgroup)[p.ops[r].dst]);
+    }
+    bool have = false;
     for (size_t r = 0; r < ready.size(); r++) {
       const LutOp& op = p.ops[ready[r]];
+      if (wgroup && (*wgroup)[op.dst] != gmin) continue;
       int freed = 0;
       for (int k = 0; k < nsrc(op); k++) {
         uint32_t s = op.src[k];
@@ -470,9 +481,10 @@ LutProgram pressure_order(const LutProgram& p, const std::vector<uint32_t>& out_
         if (!dup && prod[s] >= 0 && !p.uniform[s] && rem[s] == 1) freed++;
       }
       int cost = 1 - freed;
-      if (r == 0 || cost < bcost || (cost == bcost && ready[r] < ready[bi])) {
+      if (!have || cost < bcost || (cost == bcost && ready[r] < ready[bi])) {
         bi = r;
         bcost = cost;
+        have = true;
       }
     }
     uint32_t i = ready[bi];
@@ -485,6 +497,133 @@ LutProgram pressure_order(const LutProgram& p, const std::vector<uint32_t>& out_
   for (uint32_t i = 0; i < m; i++)
     if (!done[i]) emit(i);
   return q;
+}
+
+// The output as a conjunction of literals (wire, negated): descends through And, Nor and
+// Nimpl gates from a non-negated output (a & b, ~a & ~b, a & ~b); Xor gates, inputs and
+// negated literals are leaves.
+std::vector<std::pair<uint32_t, bool>> and_literals(const Circuit& c, uint32_t wire) {
+  std::vector<std::pair<uint32_t, bool>> lits, stack{{wire, false}};
+  std::vector<uint8_t> seen(2 * (size_t)c.wires(), 0);
+  while (!stack.empty()) {
+    auto [w, neg] = stack.back();
+    stack.pop_back();
+    if (seen[2 * (size_t)w + (neg ? 1 : 0)]++) continue;
+    if (!neg && w >= c.n && c.gates[w - c.n].func != FUNC_XOR) {
+      const Gate& g = c.gates[w - c.n];
+      const bool na = g.func == FUNC_NOR, nb = g.func != FUNC_AND;
+      stack.push_back({g.i1, nb});
+      stack.push_back({g.i0, na});
+      continue;
+    }
+    lits.push_back({w, neg});
+  }
+  return lits;
+}
+
+// Output k of c rebuilt as K groups of its conjunction's literals (grouped by shared cone
+// gates): each group's cone is copied (gates shared by groups are recomputed), its literals
+// are ANDed, and each group's value is ANDed into an accumulator right after it. wgroup gets
+// the group (1..K) of every new wire, so the schedule can keep groups apart and only the
+// accumulator lives from one group into the next. Returns false when the output has fewer
+// than K literals.
+bool grouped_circuit(const Circuit& c, uint32_t out_k, uint32_t K, Circuit& out, std::vector<uint32_t>& wgroup) {
+  const Output o = c.outputs.at(out_k);
+  const auto lits = and_literals(c, o.wire);
+  if (K < 2 || lits.size() < K) return false;
+  const uint32_t G = (uint32_t)c.gates.size();
+  // cone of each literal (gate marks)
+  std::vector<std::vector<uint8_t>> cone(lits.size());
+  for (size_t i = 0; i < lits.size(); i++) {
+    cone[i].assign(G, 0);
+    std::vector<uint32_t> st{lits[i].first};
+    while (!st.empty()) {
+      uint32_t w = st.back();
+      st.pop_back();
+      if (w < c.n || cone[i][w - c.n]) continue;
+      cone[i][w - c.n] = 1;
+      st.push_back(c.gates[w - c.n].i0);
+      st.push_back(c.gates[w - c.n].i1);
+    }
+  }
+  auto csize = [&](size_t i) {
+    uint32_t k = 0;
+    for (uint8_t b : cone[i]) k += b;
+    return k;
+  };
+  // greedy groups of ceil(m / K): the largest remaining cone, then the literals sharing most
+  std::vector<uint8_t> used(lits.size(), 0);
+  const size_t per = (lits.size() + K - 1) / K;
+  std::vector<std::vector<size_t>> groups;
+  for (size_t left = lits.size(); left;) {
+    size_t first = SIZE_MAX;
+    uint32_t best = 0;
+    for (size_t i = 0; i < lits.size(); i++)
+      if (!used[i] && (first == SIZE_MAX || csize(i) > best)) {
+        first = i;
+        best = csize(i);
+      }
+    std::vector<size_t> grp{first};
+    used[first] = 1;
+    left--;
+    std::vector<uint8_t> u = cone[first];
+    while (grp.size() < per && left) {
+      size_t pick = SIZE_MAX;
+      double pbest = 0;
+      for (size_t i = 0; i < lits.size(); i++) {
+        if (used[i]) continue;
+        uint32_t shared = 0, extra = 0;
+        for (uint32_t g = 0; g < G; g++) {
+          shared += cone[i][g] & u[g];
+          extra += cone[i][g] & (u[g] ^ 1);
+        }
+        const double sc = shared - 0.5 * extra;
+        if (pick == SIZE_MAX || sc > pbest) {
+          pick = i;
+          pbest = sc;
+        }
+      }
+      grp.push_back(pick);
+      used[pick] = 1;
+      left--;
+      for (uint32_t g = 0; g < G; g++) u[g] |= cone[pick][g];
+    }
+    groups.push_back(std::move(grp));
+  }
+  out = Circuit();
+  out.n = c.n;
+  wgroup.assign(c.n, 0);
+  auto add = [&](uint32_t a, uint32_t b, uint8_t f, uint32_t grp) {
+    out.gates.push_back({a, b, f});
+    wgroup.push_back(grp);
+    return c.n + (uint32_t)out.gates.size() - 1;
+  };
+  // a & b for literals (wire, negated), as one positive wire
+  auto conj = [&](std::pair<uint32_t, bool> x, std::pair<uint32_t, bool> y, uint32_t grp) {
+    if (!x.second && !y.second) return add(x.first, y.first, FUNC_AND, grp);
+    if (!x.second) return add(x.first, y.first, FUNC_NIMPL, grp);
+    if (!y.second) return add(y.first, x.first, FUNC_NIMPL, grp);
+    return add(x.first, y.first, FUNC_NOR, grp);
+  };
+  std::pair<uint32_t, bool> acc{0, false};
+  bool have_acc = false;
+  std::vector<uint32_t> remap(c.wires());
+  for (uint32_t k = 0; k < c.n; k++) remap[k] = k;
+  for (size_t gi = 0; gi < groups.size(); gi++) {
+    const uint32_t grp = (uint32_t)gi + 1;
+    std::vector<uint8_t> u(G, 0);
+    for (size_t i : groups[gi])
+      for (uint32_t g = 0; g < G; g++) u[g] |= cone[i][g];
+    for (uint32_t g = 0; g < G; g++)
+      if (u[g]) remap[c.n + g] = add(remap[c.gates[g].i0], remap[c.gates[g].i1], c.gates[g].func, grp);
+    std::pair<uint32_t, bool> v{remap[lits[groups[gi][0]].first], lits[groups[gi][0]].second};
+    for (size_t q = 1; q < groups[gi].size(); q++)
+      v = {conj(v, {remap[lits[groups[gi][q]].first], lits[groups[gi][q]].second}, grp), false};
+    acc = have_acc ? std::pair<uint32_t, bool>{conj(acc, v, grp), false} : v;
+    have_acc = true;
+  }
+  out.outputs.push_back({acc.first, acc.second != o.neg});
+  return true;
 }
 
 const char* kind_name(NativeKind k) {
@@ -501,22 +640,13 @@ const char* kind_name(NativeKind k) {
 NativeKernel build_native(const Circuit& c, NativeKind kind, const std::vector<uint32_t>& vbit,
                           uint32_t lane_shift, const std::vector<uint32_t>& outs) {
   NativeKernel nk;
-  std::vector<uint32_t> out_wires;
-  std::vector<uint8_t> out_neg;
-  for (uint32_t k : outs) {
-    out_wires.push_back(c.outputs.at(k).wire);
-    out_neg.push_back(c.outputs.at(k).neg ? 1 : 0);
-  }
   auto cone = output_cone(c, outs);
   auto order = schedule_gates(c, cone, outs);
+  nk.info.cone_gates = (uint32_t)order.size();
+  nk.info.peak_live = peak_live(c, order, outs);
   std::vector<uint8_t> uni(c.n, 1);
   for (uint32_t k = 0; k < c.n; k++)
     if (vbit[k] >= 5 && vbit[k] - 5 >= lane_shift && vbit[k] - 5 < lane_shift + 6) uni[k] = 0;
-  LutProgram p = pressure_order(map_luts(c, order, out_wires, uni), out_wires, c.wires());
-  nk.info.cone_gates = (uint32_t)order.size();
-  nk.info.peak_live = peak_live(c, order, outs);
-  nk.info.valu_ops = p.valu_ops;
-  nk.info.scalar_ops = p.scalar_ops;
   const bool table = kind == NativeKind::TABLE;
   const uint32_t lm = table ? 32 : 1;
   const uint32_t pool_lo = table ? 40 : 8;
@@ -526,46 +656,103 @@ NativeKernel build_native(const Circuit& c, NativeKind kind, const std::vector<u
   // each register class first without, then with two VGPRs of lane slots for uniform values
   const Cls classes[8] = {{64, 55, 0, 0},   {64, 55, 0, 2},   {128, 119, 0, 0}, {128, 119, 0, 2},
                           {256, 239, 0, 0}, {256, 239, 0, 2}, {512, 239, 256, 0}, {512, 239, 256, 2}};
-  // lane values live at once in this order: classes whose pool cannot hold them and that
-  // have no AGPRs to spill to are skipped
-  uint32_t peak = 0;
-  {
-    std::vector<uint32_t> rem(c.wires(), 0);
-    for (const LutOp& op : p.ops)
-      for (int k = 0; k < (op.lut ? 3 : 2); k++) {
-        bool dup = false;
-        for (int q = 0; q < k; q++) dup |= op.src[q] == op.src[k];
-        if (!dup) rem[op.src[k]]++;
+  struct Attempt {
+    std::string body, why = "no register class tried";
+    uint32_t vgpr_class = 0, spills = 0, reloads = 0, valu = 0, scalar = 0, vector_lines = 0;
+  };
+  // LUT mapping, schedule and the smallest register class that holds the program of cc
+  auto attempt = [&](const Circuit& cc, const std::vector<uint32_t>& outs_cc, const std::vector<uint32_t>* wgroup) {
+    Attempt at;
+    std::vector<uint32_t> out_wires;
+    std::vector<uint8_t> out_neg;
+    for (uint32_t k : outs_cc) {
+      out_wires.push_back(cc.outputs.at(k).wire);
+      out_neg.push_back(cc.outputs.at(k).neg ? 1 : 0);
+    }
+    auto cone_cc = output_cone(cc, outs_cc);
+    auto order_cc = schedule_gates(cc, cone_cc, outs_cc);
+    LutProgram p = pressure_order(map_luts(cc, order_cc, out_wires, uni), out_wires, cc.wires(), wgroup);
+    at.valu = p.valu_ops;
+    at.scalar = p.scalar_ops;
+    // lane values live at once in this order: classes whose pool cannot hold them and that
+    // have no AGPRs to spill to are skipped
+    uint32_t peak = 0;
+    {
+      std::vector<uint32_t> rem(cc.wires(), 0);
+      for (const LutOp& op : p.ops)
+        for (int k = 0; k < (op.lut ? 3 : 2); k++) {
+          bool dup = false;
+          for (int q = 0; q < k; q++) dup |= op.src[q] == op.src[k];
+          if (!dup) rem[op.src[k]]++;
+        }
+      for (uint32_t w : out_wires) rem[w]++;
+      uint32_t live = 0;
+      for (const LutOp& op : p.ops) {
+        for (int k = 0; k < (op.lut ? 3 : 2); k++) {
+          uint32_t w = op.src[k];
+          bool dup = false;
+          for (int q = 0; q < k; q++) dup |= op.src[q] == w;
+          if (!dup && --rem[w] == 0 && w >= cc.n && !p.uniform[w]) live--;
+        }
+        if (rem[op.dst] > 0 && !p.uniform[op.dst]) live++;
+        peak = std::max(peak, live);
       }
-    for (uint32_t w : out_wires) rem[w]++;
-    uint32_t live = 0;
-    for (const LutOp& op : p.ops) {
-      for (int k = 0; k < (op.lut ? 3 : 2); k++) {
-        uint32_t w = op.src[k];
-        bool dup = false;
-        for (int q = 0; q < k; q++) dup |= op.src[q] == w;
-        if (!dup && --rem[w] == 0 && w >= c.n && !p.uniform[w]) live--;
+    }
+    for (const Cls& cl : classes) {
+      if (table && cl.budget < 128) continue;
+      if (!cl.agprs && peak + cl.slots > cl.hi - pool_lo + 1) continue;
+      try {
+        Allocator a(cc, p, vbit, lane_shift, lm, pool_lo, cl.hi, cl.agprs, cl.slots, out_wires, out_neg);
+        at.body = a.run();
+        at.vgpr_class = cl.budget;
+        at.spills = a.spills();
+        at.reloads = a.reloads();
+        break;
+      } catch (const NativeUnfit& e) {
+        at.why = e.what();
+        continue;
       }
-      if (rem[op.dst] > 0 && !p.uniform[op.dst]) live++;
-      peak = std::max(peak, live);
+    }
+    // vector instructions of the body (LUTs, copies, AGPR and lane moves)
+    for (size_t pos = 0; (pos = at.body.find("\tv_", pos)) != std::string::npos; pos += 3) at.vector_lines++;
+    return at;
+  };
+  Attempt best = attempt(c, outs, nullptr);
+  uint32_t groups = 1;
+  // One wave per SIMD (the AGPR class) issues fewer VALU ops per cycle than two or more. A
+  // single output that is a conjunction can be evaluated as K groups of its literals, one
+  // after the other, recomputing gates the groups share. Measured on the 36-input,
+  // 20,000-gate search, two waves per SIMD ran 1.45 times the vector instructions per cycle
+  // of one (not twice: lane moves and scalar work grow with the groups), so the groups are
+  // taken when they reach a two-wave class at under 1.45 times the vector instructions.
+  // NS_OUTPUT_GROUPS=K forces K (1: never).
+  long forced = -1;
+  if (const char* e = std::getenv("NS_OUTPUT_GROUPS")) forced = std::atol(e);
+  if (!table && outs.size() == 1 && forced != 1 && (forced > 1 || best.vgpr_class == 512)) {
+    const uint32_t ks[6] = {2, 3, 4, 5, 6, 8};
+    for (uint32_t K : ks) {
+      if (forced > 1 && (uint32_t)forced != K) continue;
+      Circuit g;
+      std::vector<uint32_t> wg;
+      if (!grouped_circuit(c, outs[0], K, g, wg)) break;
+      Attempt at = attempt(g, {0}, &wg);
+      if (!at.vgpr_class) continue;
+      if (forced > 1 ||
+          (at.vgpr_class <= 256 && best.vgpr_class == 512 && at.vector_lines * 100 < best.vector_lines * 145)) {
+        best = std::move(at);
+        groups = K;
+        break;
+      }
+      if (at.vgpr_class <= 256) break;  // more groups only add work
     }
   }
-  std::string body, why = "no register class tried";
-  for (const Cls& cl : classes) {
-    if (table && cl.budget < 128) continue;
-    if (!cl.agprs && peak + cl.slots > cl.hi - pool_lo + 1) continue;
-    try {
-      Allocator a(c, p, vbit, lane_shift, lm, pool_lo, cl.hi, cl.agprs, cl.slots, out_wires, out_neg);
-      body = a.run();
-      nk.vgpr_class = cl.budget;
-      nk.spills = a.spills();
-      nk.reloads = a.reloads();
-      break;
-    } catch (const NativeUnfit& e) {
-      why = e.what();
-      continue;
-    }
-  }
+  nk.info.valu_ops = best.valu;
+  nk.info.scalar_ops = best.scalar;
+  nk.vgpr_class = best.vgpr_class;
+  nk.spills = best.spills;
+  nk.reloads = best.reloads;
+  nk.output_groups = groups;
+  std::string body = std::move(best.body), why = best.why;
   if (!nk.vgpr_class) throw NativeUnfit("circuit does not fit the native templates: " + why);
   // the table kernel packs all 32 output registers: those without an output are zero
   if (table)

@@ -26,6 +26,7 @@ struct NativeKernel {
   uint32_t vgpr_class = 0;  // 64 / 128 / 256 / 512 (VGPRs + AGPR spill slots)
   uint32_t spills = 0;      // values copied to AGPRs per word
   uint32_t reloads = 0;     // AGPR reads per word
+  uint32_t output_groups = 1;  // the output's conjunction evaluated as this many groups
   GenInfo info;
 };
 

@@ -154,7 +154,9 @@ Chosen choose_kernel(Lane& ctx, const Circuit& c, NativeKind kind, const std::ve
                      const std::string& extra_key, const std::function<std::string(GenInfo*)>& hiprtc_src,
                      const char* hiprtc_kernel, double* compile_ms, std::string* fp_out = nullptr) {
   const bool native_ok = native_wanted(g);
-  std::string fp = circuit_fingerprint(c) + "/" + std::to_string((int)kind) + "/" + std::to_string(lane_shift) +
+  const char* og = std::getenv("NS_OUTPUT_GROUPS");  // changes the native body
+  std::string fp = circuit_fingerprint(c) + "/" + (og ? og : "") + "/" + std::to_string((int)kind) + "/" +
+                   std::to_string(lane_shift) +
                    "/" + std::to_string(g.block) + "/" + std::to_string(g.wpl) + "/" + std::to_string(g.j_shift) +
                    (native_ok ? "/n/" : "/r/") + extra_key;
   for (uint32_t k : outs) fp += "," + std::to_string(k);

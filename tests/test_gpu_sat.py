@@ -157,3 +157,30 @@ def test_dense_solutions_count_exactly(ctx):
     d["func"] = np.append(d["func"], 3).astype(np.uint8)
     d["out_wire"] = np.array([24 + len(d["func"]) - 1], np.uint32)
     check(ctx, d)
+
+
+@pytest.mark.parametrize("K", [2, 3, 5])
+def test_output_groups_keep_answers(ctx, monkeypatch, K):
+    # a conjunction output evaluated as K groups of its literals (the shared gates recomputed
+    # per group) gives the oracle's count and lowest hit, for SAT and QSAT kernels
+    monkeypatch.setenv("NS_OUTPUT_GROUPS", str(K))
+    for seed in range(3):
+        c, key = ns.Circuit.gen_sat(4100 + 10 * K + seed, 22, 500, 16)
+        body, cls, _ = c.native_body(0)
+        r = ctx.sat_solve(c)
+        assert (r["count"], r["lowest"]) == oracle.OracleCircuit.from_dict(c.arrays()).sat(threads=8), seed
+    rng = np.random.default_rng(4200 + K)
+    d = random_circuit(rng, 18, 120, recent=30)
+    # AND the output with three more wires (one negated) so it is a conjunction of 4+ literals
+    g = len(d["func"])
+    w = [int(d["out_wire"][0]), 18 + g - 5, 18 + g - 9, 18 + g - 13]
+    d["in0"] = np.append(d["in0"], [w[0], 18 + g, 18 + g + 1]).astype(np.uint32)
+    d["in1"] = np.append(d["in1"], [w[1], w[2], w[3]]).astype(np.uint32)
+    d["func"] = np.append(d["func"], [0, 2, 0]).astype(np.uint8)
+    d["out_wire"] = np.array([18 + g + 2], np.uint32)
+    check(ctx, d)
+    c, key = ns.Circuit.gen_sat(4300 + K, 16, 300, 12)
+    q = [0] * 4 + [1] * 4 + [0] * 4 + [1] * 4
+    o = oracle.OracleCircuit.from_dict(c.arrays()).qsat(q)
+    r = ctx.qsat_solve(c, q)
+    assert (r["truth"], r["witness_count"]) == (o["truth"], o["witness_count"])

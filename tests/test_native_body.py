@@ -177,12 +177,42 @@ def test_small_spaces():
         check(c, 0, 0, [0])
 
 
-def test_config1_body_with_agpr_spills():
-    # 2,102 cone gates with ~500 live values: the 512-register class spills to AGPRs
+def test_config1_body_with_agpr_spills(monkeypatch):
+    # 2,102 cone gates with ~370 live LUT values: as one group the 512-register class spills
+    # to AGPRs
+    monkeypatch.setenv("NS_OUTPUT_GROUPS", "1")
     c, _ = ns.Circuit.gen_sat(2, 36, 20000, 40)
     body, cls, size = c.native_body(0)
     assert cls == 512 and "v_accvgpr_write_b32" in body
     check(c, 0, 0, [64 * 12345])
+
+
+def test_config1_body_in_output_groups():
+    # its output is the conjunction of 40 comparisons: evaluated as four groups of ten
+    # (shared gates recomputed), every group fits 256 VGPRs, two waves per SIMD
+    c, _ = ns.Circuit.gen_sat(2, 36, 20000, 40)
+    body, cls, size = c.native_body(0)
+    assert cls == 256 and "v_accvgpr" not in body
+    check(c, 0, 0, [64 * 12345, (1 << 31) - 64])
+
+
+@pytest.mark.parametrize("K", [2, 3, 4, 7])
+def test_forced_output_groups(monkeypatch, K):
+    monkeypatch.setenv("NS_OUTPUT_GROUPS", str(K))
+    c, _ = ns.Circuit.gen_sat(600 + K, 20, 400, 12)
+    check(c, 0, 0, [0, 64 * 77])
+    # negated and Nor/Nimpl literals, and a negated output over the conjunction
+    rng = np.random.default_rng(650 + K)
+    d = random_circuit(rng, 16, 90, recent=25)
+    g = len(d["func"])
+    top = [int(d["out_wire"][0]), 16 + g - 3, 16 + g - 7, 16 + g - 11, 16 + g - 15]
+    d["in0"] = np.append(d["in0"], [top[0], top[2], 16 + g, 16 + g + 2]).astype(np.uint32)
+    d["in1"] = np.append(d["in1"], [top[1], top[3], top[4], 16 + g + 1]).astype(np.uint32)
+    d["func"] = np.append(d["func"], [2, 1, 0, 0]).astype(np.uint8)
+    for neg in (0, 1):
+        c2 = ns.Circuit.from_arrays(16, d["in0"], d["in1"], d["func"], [16 + g + 3], [neg])
+        check(c2, 0, 0, [0, 64 * 9])
+        check(c2, 1, 2, [0, 1 << 8])
 
 
 def test_register_budget_is_reported():
