@@ -159,6 +159,8 @@ int ns_qsat_solve(ns_ctx* ctx, const ns_circuit* c, const uint8_t* quants_all,
  * its start reaches STOP and after how many evaluations (the stopping one included). */
 #define NS_STOP_FIRST_PASS 1u /* end after the first jump pass where some start stopped */
 #define NS_STOP_PASSES 2u     /* full search by passes over the whole table, not waypoint walks */
+#define NS_STOP_TABLE_WALKS 4u /* full search by waypoint walks over the table (up to 35 state bits) */
+#define NS_STOP_BIT_WALKS 8u   /* full search by waypoint walks without a table (the default from 34 bits) */
 typedef struct {
   uint32_t flags;
   uint32_t max_passes;     /* 0 = n (enough for every path) */
@@ -187,7 +189,9 @@ typedef struct {
    * when the unknowns allow: one walk from every state whose low waypoint_bits bits are
    * zero (the starts among them) to the next such state or STOP, then pointer jumping
    * over the 2^(n - waypoint_bits) walk records (waypoint_passes passes, jump_ms). */
-  uint32_t method;          /* 0 passes over the table + walks, 1 waypoint walks */
+  uint32_t method;          /* 0 passes over the table + walks, 1 waypoint walks over the table,
+                             2 waypoint walks without a table (bit-sliced, 32 walks per lane;
+                             build_ms 0, any_stop = some start stops) */
   uint32_t waypoint_bits;
   uint32_t waypoint_passes;
   uint32_t reserved;
@@ -199,13 +203,16 @@ int ns_satostop_solve(ns_ctx* ctx, const ns_circuit* c, uint32_t unknowns,
                       const ns_satostop_opts* opts, ns_satostop_result* res, uint32_t* steps_out,
                       uint32_t* ends_out);
 
-/* Same contract for state spaces of up to 35 bits (the reference handles states wider
+/* Same contract for state spaces of up to 62 bits (the reference handles states wider
  * than 32 bits with u64 entries: join_nexts_64, src/greedysatostop.rs:169-194, and
  * HashEntry, src/satostop.rs:168-174), with 64-bit per-start results. From 32 state bits
  * on the full search reads a packed (n+1)-bit image (the reference's MemImage layout;
  * n = 35: 155 GiB); 34 and 35 state bits take only the full search, with at most n - 10
  * unknowns. The passes (first-pass mode, max_passes, fallback) use split u32 tables and
- * u64 jump tables, up to 33 state bits (n = 33: 192 GiB). */
+ * u64 jump tables, up to 33 state bits (n = 33: 192 GiB). From 34 state bits on (or with
+ * NS_STOP_BIT_WALKS) the full search walks without a table: the circuit is evaluated for 32
+ * walks per lane at each step, and only the waypoint records (at most 2^28 of 16 bytes)
+ * stay in device memory; unknowns <= n - 8 there. */
 int ns_satostop_solve_wide(ns_ctx* ctx, const ns_circuit* c, uint32_t unknowns,
                            const ns_satostop_opts* opts, ns_satostop_result* res, uint64_t* steps_out,
                            uint64_t* ends_out);

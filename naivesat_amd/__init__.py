@@ -26,6 +26,8 @@ BIN_DIR = os.path.join(_HERE, "bin")
 NS_SAT_STOP_AT_FIRST = 1
 NS_STOP_FIRST_PASS = 1
 NS_STOP_PASSES = 2
+NS_STOP_TABLE_WALKS = 4
+NS_STOP_BIT_WALKS = 8
 FUNC_NAMES = ("and", "nor", "nimpl", "xor")
 
 
@@ -408,11 +410,13 @@ class Context:
 
     def satostop_solve(self, c: Circuit, unknowns: int, first_pass: bool = False, max_passes: int = 0,
                        group_len: int = 0, words_per_lane: int = 0, arrays: bool = True,
-                       wide: bool = True, passes: bool = False) -> dict:
-        """Per-start steps/ends arrays are uint64 (ns_satostop_solve_wide, up to 33 state
+                       wide: bool = True, passes: bool = False, method: str = "") -> dict:
+        """Per-start steps/ends arrays are uint64 (ns_satostop_solve_wide, up to 62 state
         bits) or, with wide=False, uint32 (ns_satostop_solve, up to 31). passes=True runs the
-        full search as passes over the whole table instead of waypoint walks."""
+        full search as passes over the whole table instead of waypoint walks; method
+        "table" / "bits" picks the waypoint walks over the table / without one."""
         flags = (NS_STOP_FIRST_PASS if first_pass else 0) | (NS_STOP_PASSES if passes else 0)
+        flags |= {"": 0, "table": NS_STOP_TABLE_WALKS, "bits": NS_STOP_BIT_WALKS}[method]
         o = _StopOpts(flags, max_passes, group_len, words_per_lane)
         r = _StopResult()
         dt = np.uint64 if wide else np.uint32

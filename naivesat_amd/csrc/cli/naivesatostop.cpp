@@ -1,9 +1,10 @@
 // naivesatostop: SATOSTOP answer (start, end, steps) on one HIP device.
 // Command line and printed lines follow the reference's naivesatostop
-// (src/satostop.rs:62-99,1650-1782,4443-4467). Up to 33 state bits (35 with at most n - 10
-// unknowns) the search runs on the device-resident next-state table (the lowest stopping
-// unknown); beyond that, or with --hashmap-search, it runs the reference's hash-map search
-// (any stopping unknown) with the -e/-H/-U/-P/-C options.
+// (src/satostop.rs:62-99,1650-1782,4443-4467). The search finds every unknown's answer (and
+// prints the lowest stopping one): on the device-resident next-state table up to 35 state
+// bits, by walks that evaluate the circuit without a table up to 62 (at most 28 unknowns);
+// otherwise, or with --hashmap-search, it runs the reference's hash-map search (any stopping
+// unknown) with the -e/-H/-U/-P/-C options.
 #include "cli_common.h"
 
 static const char* kUsage =
@@ -46,10 +47,11 @@ int main(int argc, char** argv) {
   long long U = std::atoll(a.positional[1].c_str());
   if (U < 0 || U > (long long)n) cli::die("unknowns must not exceed the inputs");
   const bool simple = a.has("simple");
-  // the device table holds up to 35 state bits (packed from 32 on, with at most n - 10 unknowns)
-  const bool table_ok = n <= 33 || (n <= 35 && U <= (long long)n - 10);
+  // the per-start search: the device table up to 35 state bits (packed from 32 on, with at
+  // most n - 10 unknowns), then walks without a table (at most 28 unknowns)
+  const bool table_ok = n <= 33 || (n <= 35 && U <= (long long)n - 10) || (n <= 62 && U <= 28);
   const bool hashmap = !simple && (!table_ok || a.has("hashmap-search"));
-  if (simple && !table_ok) cli::die("--simple takes at most 33 state bits (35 with at most n - 10 unknowns)");
+  if (simple && !table_ok) cli::die("--simple needs the per-start search (at most 28 unknowns over 35 state bits)");
   if (hashmap && n < 10) cli::die("the hash-map search takes at least 10 state bits");
   int dev = cli::device_from_exec_type(a.get("exec-type"), !simple);
   const long long H = a.num("hashmap-len-bits", 24);

@@ -573,4 +573,133 @@ std::string gen_table_kernel(const Circuit& c, const Geometry& g, GenInfo* info)
   return os.str();
 }
 
+std::string gen_walk_kernel(const Circuit& c, uint32_t k, GenInfo* info) {
+  const uint32_t n = c.n;
+  if (c.outputs.size() != (size_t)n + 1) throw std::runtime_error("SATOSTOP needs n inputs and n+1 outputs");
+  if (n < 1 || n > 62 || k > n) throw std::runtime_error("walks take 1..62 state bits");
+  std::vector<uint32_t> outs;
+  for (uint32_t i = 0; i <= n; i++) outs.push_back(i);
+  auto cone = output_cone(c, outs);
+  auto order = schedule_gates(c, cone, outs);
+  if (info) {
+    info->cone_gates = (uint32_t)order.size();
+    info->peak_live = peak_live(c, order, outs);
+  }
+  const std::vector<uint8_t> lane_inputs(n, 0);  // every input varies per walk
+  std::ostringstream os;
+  prelude(os);
+  auto each = [&](const std::string& line) {
+    // the line with @ replaced by 0..n-1, once per state bit
+    for (uint32_t i = 0; i < n; i++) {
+      std::string l = line;
+      for (size_t p; (p = l.find('@')) != std::string::npos;) l.replace(p, 1, std::to_string(i));
+      os << l;
+    }
+  };
+  os << "extern \"C\" __global__ void __launch_bounds__(256) ns_walk(unsigned long long count, "
+        "unsigned long long limit, unsigned long long budget, ulonglong2* __restrict__ out, unsigned int* long_flag, "
+        "unsigned long long* evals) {\n";
+  // per slot: start time (bits 0..31), queue position (32..55), save exponent (56..63)
+  os << "  __shared__ unsigned long long meta[32 * 256];\n"
+        "  const unsigned int tid = threadIdx.x;\n"
+        "  const unsigned long long lanes = (unsigned long long)gridDim.x * blockDim.x;\n"
+        "  const unsigned long long lid = (unsigned long long)blockIdx.x * blockDim.x + tid;\n"
+        "  const unsigned long long lim = limit < 0x7fffffffull ? limit : 0x7fffffffull;\n";
+  each("  unsigned int s@ = 0u, v@ = 0u;\n");
+  os << "  unsigned int active = 0u, t = 0u, next_event = 0xffffffffu;\n"
+        "  unsigned long long q = 0ull, used = 0ull, ev = 0ull;\n";
+  // slot b takes the lane's next walk at time t: its bits of the state and the saved state
+  // become the waypoint's state
+  std::ostringstream asg;
+  asg << "    {\n"
+         "      const unsigned long long idx = lid + q * lanes;\n"
+         "      if (idx < count && q < 0xffffffull) {\n"
+         "        const unsigned long long st = idx << "
+      << k << ";\n";
+  for (uint32_t i = 0; i < n; i++)
+    asg << "        s" << i << " = (s" << i << " & ~mb) | ((unsigned int)((st >> " << i << ") & 1ull) * mb); v" << i
+        << " = (v" << i << " & ~mb) | (s" << i << " & mb);\n";
+  asg << "        meta[b * 256u + tid] = (unsigned long long)t | (q << 32);\n"
+         "        active |= mb;\n"
+         "        q++;\n"
+         "        next_event = next_event < t + 1u ? next_event : t + 1u;\n"
+         "      }\n"
+         "    }\n";
+  os << "  for (unsigned int b = 0; b < 32u; b++) {\n    const unsigned int mb = 1u << b;\n" << asg.str() << "  }\n";
+  os << "  while (active) {\n"
+        "    if (++used > budget) {\n"
+        "      atomicOr(long_flag, 1u);\n"
+        "      break;\n"
+        "    }\n"
+        "    if ((used & 255ull) == 0ull && __hip_atomic_load(long_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u)\n"
+        "      break;\n";
+  for (uint32_t i = 0; i < n; i++) os << "    const unsigned int x" << i << " = s" << i << ";\n";
+  uint32_t sops = 0;
+  const uint32_t ops = emit_gates(os, c, order, outs, lane_inputs, &sops);
+  if (info) {
+    info->valu_ops = ops;
+    info->scalar_ops = sops;
+  }
+  for (uint32_t i = 0; i <= n; i++) os << "    const unsigned int o" << i << " = " << out_expr(c, i) << ";\n";
+  os << "    ev += (unsigned long long)__popc(active);\n"
+        "    const unsigned int stop = o"
+     << n << " & active;\n";
+  os << "    const unsigned int nz = 0u";
+  for (uint32_t i = 0; i < k; i++) os << " | o" << i;
+  os << ";\n    const unsigned int wp = ~nz & active & ~stop;\n    const unsigned int eq = 0xffffffffu";
+  for (uint32_t i = 0; i < n; i++) os << " & ~(o" << i << " ^ v" << i << ")";
+  os << ";\n    const unsigned int loop = eq & active & ~stop & ~wp;\n";
+  each("    s@ = o@;\n");
+  os << "    t++;\n"
+        "    unsigned int done = stop | wp | loop, lng = 0u;\n"
+        // per-slot events: the saves of Brent's test (after 1, 2, 4, ... evaluations) and the limit
+        "    if (t >= next_event) {\n"
+        "      next_event = 0xffffffffu;\n"
+        "      unsigned int save = 0u;\n"
+        "      for (unsigned int rem = active & ~done; rem; rem &= rem - 1u) {\n"
+        "        const unsigned int b = __builtin_ctz(rem);\n"
+        "        unsigned long long m = meta[b * 256u + tid];\n"
+        "        const unsigned int start = (unsigned int)m, e = (unsigned int)(m >> 56), steps = t - start;\n"
+        "        if ((unsigned long long)steps >= lim) {\n"
+        "          lng |= 1u << b;\n"
+        "          continue;\n"
+        "        }\n"
+        "        unsigned int e2 = e;\n"
+        "        if (steps >= (1u << e)) {\n"
+        "          save |= 1u << b;\n"
+        "          e2 = e + 1u;\n"
+        "          meta[b * 256u + tid] = (m & 0x00ffffffffffffffull) | ((unsigned long long)e2 << 56);\n"
+        "        }\n"
+        "        const unsigned int due = start + (e2 < 31u ? (1u << e2) : 0x80000000u);\n"
+        "        const unsigned int dl = start + (unsigned int)lim;\n"
+        "        const unsigned int nx = due < dl ? due : dl;\n"
+        "        next_event = next_event < nx ? next_event : nx;\n"
+        "      }\n"
+        "      if (save) {\n";
+  each("        v@ = (v@ & ~save) | (s@ & save);\n");
+  os << "      }\n"
+        "    }\n"
+        "    if (lng) atomicOr(long_flag, 1u);\n"
+        "    done |= lng;\n"
+        "    for (; done; done &= done - 1u) {\n"
+        "      const unsigned int b = __builtin_ctz(done), mb = 1u << b;\n"
+        "      const unsigned long long m = meta[b * 256u + tid];\n"
+        "      const unsigned long long idx = lid + ((m >> 32) & 0xffffffull) * lanes;\n"
+        "      const unsigned int steps = t - (unsigned int)m;\n"
+        "      const unsigned int kind = (stop & mb) ? 1u : (wp & mb) ? 0u : (loop & mb) ? 2u : 3u;\n"
+        "      unsigned long long st = 0ull;\n";
+  each("      st |= (unsigned long long)((s@ >> b) & 1u) << @;\n");
+  os << "      const unsigned long long target = kind == 1u ? st : kind == 0u ? (st >> " << k
+     << ") : 0ull;\n"
+        "      out[idx] = make_ulonglong2(target | ((unsigned long long)kind << 62), steps);\n"
+        "      active &= ~mb;\n"
+     << asg.str() << "    }\n"
+     << "  }\n";
+  // evaluations done by the wave, into one of 64 count slots 16 words apart
+  os << "  for (int off = 32; off > 0; off >>= 1) ev += __shfl_xor(ev, off);\n"
+        "  if ((tid & 63u) == 0u && ev != 0ull) atomicAdd(evals + ((blockIdx.x & 63u) << 4), ev);\n"
+        "}\n";
+  return os.str();
+}
+
 }  // namespace ns

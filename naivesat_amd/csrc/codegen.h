@@ -73,4 +73,18 @@ std::string gen_qsat_kernel(const Circuit& c, const std::vector<uint8_t>& quants
 // kernel ns_table(u64 w0, u64 wend, u32* table, u32* anylast)
 std::string gen_table_kernel(const Circuit& c, const Geometry& g, GenInfo* info);
 
+// SATOSTOP waypoint walks without a table: kernel ns_walk(u64 count, u64 limit, u64 budget,
+// ulonglong2* out, u32* long_flag, u64* evals). Each lane carries 32 walks in the bits of its
+// n state registers (bit b of register i = bit i of walk b's state) and advances all of them
+// with one evaluation of the circuit (n inputs, n + 1 outputs, STOP last). A walk starts at
+// waypoint idx (state idx << k, idx < count; the lane's walks are idx = lane + q * lanes,
+// q = 0, 1, ...) and ends at the first STOP, at a next state whose low k bits are zero, at
+// a repeated state (Brent's test against its state saved after 1, 2, 4, ... evaluations) or
+// after `limit` evaluations; then out[idx] = {target | kind << 62, evaluations} with the
+// kinds and targets of the waypoint records (0 next waypoint, 1 STOP, 2 never, 3 too long,
+// which also sets *long_flag), and the slot takes the lane's next walk. A lane stops after
+// `budget` evaluations (raising the flag) and every lane once it sees the flag. evals[64
+// slots, 16 words apart] += evaluations done.
+std::string gen_walk_kernel(const Circuit& c, uint32_t k, GenInfo* info);
+
 }  // namespace ns

@@ -669,11 +669,15 @@ double build_packed_table(Lane& ctx, const Circuit& c, const ns_satostop_opts& o
 // arrays and host[0..1] = (stopping starts, lowest one). Returns false, with nothing
 // changed, when a walk exceeds its evaluation limit (a long path or cycle without
 // waypoints); the passes then answer instead.
-bool waypoint_search(Lane& ctx, uint32_t n, uint32_t unknowns, uint32_t k, const TableView& tv,
+// A walk kernel enqueued on ctx.stream: (records, flag, evaluation slots, waypoints, limit).
+using WalkLaunch = std::function<void(void*, unsigned int*, unsigned long long*, uint64_t, uint64_t)>;
+
+bool waypoint_search(Lane& ctx, uint32_t n, uint32_t unknowns, uint32_t k, const WalkLaunch& walk,
                      ns_satostop_result& r, unsigned long long* d_steps, unsigned long long* d_ends,
                      unsigned long long* stats, unsigned long long host[2]) {
   const uint64_t count = 1ull << (n - k);
-  uint64_t limit = 1ull << 18;
+  // a walk ends after this many evaluations (far above the 2^k a walk takes on average)
+  uint64_t limit = std::max<uint64_t>(1ull << 18, 1ull << std::min(k + 6, 31u));
   if (const char* e = std::getenv("NS_WAYPOINT_LIMIT")) {
     const long long v = std::atoll(e);
     if (v > 0) limit = (uint64_t)v;
@@ -686,7 +690,7 @@ bool waypoint_search(Lane& ctx, uint32_t n, uint32_t unknowns, uint32_t k, const
   reset_counts(ctx);  // the walks count their evaluations in the count slots
   NS_HIP(launch_fill_u32(ctx.stream, flag, 0u, 1));
   NS_HIP(hipEventRecord(ctx.ev0, ctx.stream));
-  NS_HIP(launch_wp_walk(ctx.stream, tv, n, k, count, limit, cur, flag, stats + kSlot0));
+  walk(cur, flag, stats + kSlot0, count, limit);
   NS_HIP(hipEventRecord(ctx.ev1, ctx.stream));
   unsigned int hf = 0;
   NS_HIP(hipMemcpyAsync(&hf, flag, 4, hipMemcpyDeviceToHost, ctx.stream));
@@ -697,7 +701,6 @@ bool waypoint_search(Lane& ctx, uint32_t n, uint32_t unknowns, uint32_t k, const
   if (hf != 0) return false;
   unsigned long long ev[2];
   read_counts_done(cc, ev);
-  r.method = 1;
   r.waypoint_bits = k;
   r.walk_ms = walk_ms;
   r.walk_evaluations = ev[0];
@@ -724,6 +727,48 @@ bool waypoint_search(Lane& ctx, uint32_t n, uint32_t unknowns, uint32_t k, const
   return true;
 }
 
+// the waypoint walks over a table in device memory
+WalkLaunch table_walk(Lane& ctx, uint32_t n, uint32_t k, const TableView& tv) {
+  return [&ctx, n, k, tv](void* out, unsigned int* flag, unsigned long long* evals, uint64_t count, uint64_t limit) {
+    NS_HIP(launch_wp_walk(ctx.stream, tv, n, k, count, limit, out, flag, evals));
+  };
+}
+
+// Waypoint spacing of the walks without a table: 2^22 waypoints up to 30 state bits, then
+// more (up to 2^26) so that each lane's 32 slots take many walks each; k <= n - unknowns.
+uint32_t bitwalk_bits(uint32_t n, uint32_t unknowns) {
+  if (const char* e = std::getenv("NS_WAYPOINT_BITS")) {
+    const long long v = std::atoll(e);
+    if (v >= 0 && v <= (long long)n) return (uint32_t)v;
+  }
+  const uint32_t wbits = std::min(n, std::max(22u, std::min(26u, n > 8 ? n - 8 : 0u)));
+  return std::min(n - unknowns, n - wbits);
+}
+
+// The waypoint walks without a table: the generated ns_walk kernel (bit-sliced walks, 32 per
+// lane) for this circuit.
+WalkLaunch bit_walk(Lane& ctx, const Circuit& c, uint32_t k, double* compile_ms, uint32_t* cone, uint32_t* valu) {
+  GenInfo info;
+  const std::string src = gen_walk_kernel(c, k, &info);
+  double cms = 0;
+  auto mod = ctx.jit.get(src, "ns_walk", &cms);
+  *compile_ms += cms;
+  *cone = info.cone_gates;
+  *valu = info.valu_ops;
+  return [&ctx, mod, k](void* out, unsigned int* flag, unsigned long long* evals, uint64_t count, uint64_t limit) {
+    // enough lanes for two waves per SIMD, and at least 32 walks per lane where there are
+    uint64_t lanes = std::min<uint64_t>(1ull << 17, std::max<uint64_t>(256, count / 32));
+    lanes = (lanes + 255) / 256 * 256;
+    const uint64_t per_lane = (count + lanes - 1) / lanes;
+    // a lane's 32 slots advance together; its walks average 2^k evaluations each on a
+    // random map: allow 16 times that, plus one walk of the full limit
+    unsigned long long budget = 16ull * ((per_lane + 31) / 32) * (1ull << k) + limit;
+    unsigned long long cnt = count, lim = limit;
+    void* args[6] = {&cnt, &lim, &budget, &out, &flag, &evals};
+    NS_HIP(hipModuleLaunchKernel(mod->fn, (unsigned)(lanes / 256), 1, 1, 256, 1, 1, 0, ctx.stream, args, nullptr));
+  };
+}
+
 }  // namespace
 
 void satostop_solve(Lane& ctx, const Circuit& c, uint32_t unknowns, const ns_satostop_opts& o,
@@ -732,7 +777,7 @@ void satostop_solve(Lane& ctx, const Circuit& c, uint32_t unknowns, const ns_sat
   std::memset(&r, 0, sizeof(r));
   const uint32_t n = c.n;
   if (c.outputs.size() != (size_t)n + 1) throw Error(ST_INVALID, "SATOSTOP needs n inputs and n+1 outputs");
-  if (n == 0 || n > 35) throw Error(ST_UNSUPPORTED, "SATOSTOP tables support 1..35 state bits");
+  if (n == 0 || n > 62) throw Error(ST_UNSUPPORTED, "SATOSTOP supports 1..62 state bits");
   if (unknowns > n) throw Error(ST_INVALID, "more unknowns than state bits");
   NS_HIP(hipSetDevice(ctx.device));
   ctx.pass_ms.clear();
@@ -740,6 +785,57 @@ void satostop_solve(Lane& ctx, const Circuit& c, uint32_t unknowns, const ns_sat
   uint32_t cover = o.max_passes ? o.max_passes : n;
   if (cover > n) cover = n;
   const bool first_pass_mode = (o.flags & NS_STOP_FIRST_PASS) != 0;
+  const bool full = !first_pass_mode && cover == n && (o.flags & NS_STOP_PASSES) == 0;
+  // The full search without a table: walks of 32 states per lane that evaluate the circuit
+  // as they go. The default from 34 state bits on: the 30-bit Feistel search takes 10 ms of
+  // device time this way against 28 ms walking the table, but the generated kernel is
+  // compiled (~0.2 s) where the table kernels come from the native templates (~4 ms), and the
+  // packed image of 34 and 35 state bits would take 75 and 155 GiB.
+  const uint32_t kb = bitwalk_bits(n, unknowns);
+  const bool bit_ok = full && kb <= n - unknowns && n - kb <= 28;
+  const bool use_bit = bit_ok && (o.flags & NS_STOP_TABLE_WALKS) == 0 && ((o.flags & NS_STOP_BIT_WALKS) != 0 || n >= 34);
+  if (n > 35 && !use_bit)
+    throw Error(ST_UNSUPPORTED, "over 35 state bits only the full search without a table, with at most 28 unknowns");
+  const uint64_t starts_bw = 1ull << unknowns;
+  if (use_bit) {
+    ctx.tab_d.ensure(starts_bw * 16 + 64);
+    unsigned long long* d_steps = ctx.tab_d.as<unsigned long long>();
+    unsigned long long* d_ends = d_steps + starts_bw;
+    unsigned long long* stats = ctx.stats.as<unsigned long long>();
+    unsigned long long host[2] = {0, ~0ull};
+    const WalkLaunch wl = bit_walk(ctx, c, kb, &r.compile_ms, &r.cone_gates, &r.valu_ops);
+    r.backend = 0;
+    if (waypoint_search(ctx, n, unknowns, kb, wl, r, d_steps, d_ends, stats, host)) {
+      r.method = 2;
+      r.n_stopped = host[0];
+      r.any_stop = r.n_stopped != 0;
+      if (r.n_stopped) {
+        r.lowest_u = host[1];
+        r.start = r.lowest_u << (n - unknowns);
+      }
+      unsigned long long se[2] = {0, 0};
+      if (r.n_stopped) {
+        NS_HIP(hipMemcpyAsync(&se[0], d_steps + r.lowest_u, 8, hipMemcpyDeviceToHost, ctx.stream));
+        NS_HIP(hipMemcpyAsync(&se[1], d_ends + r.lowest_u, 8, hipMemcpyDeviceToHost, ctx.stream));
+      }
+      if (steps_out) NS_HIP(hipMemcpyAsync(steps_out, d_steps, starts_bw * 8, hipMemcpyDeviceToHost, ctx.stream));
+      if (ends_out) NS_HIP(hipMemcpyAsync(ends_out, d_ends, starts_bw * 8, hipMemcpyDeviceToHost, ctx.stream));
+      NS_HIP(hipStreamSynchronize(ctx.stream));
+      if (r.n_stopped) {
+        r.steps = se[0];
+        r.end = se[1];
+      }
+      r.wall_ms = ms_since(t0);
+      return;
+    }
+    if (n > 35)
+      throw Error(ST_UNSUPPORTED, "a waypoint walk exceeded its limit (a long path or cycle without waypoints); "
+                                  "over 35 state bits there is no table to fall back on: use the hash-map search");
+    const double cms = r.compile_ms;
+    std::memset(&r, 0, sizeof(r));
+    r.compile_ms = cms;
+    ctx.pass_ms.clear();
+  }
   // The full search runs as waypoint walks when every start is a waypoint and the records
   // stay small; from 32 state bits on (NS_PACKED_TABLE=1: from 6) the walks read the packed
   // (n+1)-bit image, which holds 34 and 35 state bits in HBM too.
@@ -769,7 +865,8 @@ void satostop_solve(Lane& ctx, const Circuit& c, uint32_t unknowns, const ns_sat
       searched = true;
     } else {
       const TableView tv{nullptr, nullptr, 0, ctx.tab_img.as<unsigned long long>(), n + 1};
-      searched = waypoint_search(ctx, n, unknowns, k, tv, r, d_steps, d_ends, stats, host);
+      searched = waypoint_search(ctx, n, unknowns, k, table_walk(ctx, n, k, tv), r, d_steps, d_ends, stats, host);
+      if (searched) r.method = 1;
       if (!searched) {
         if (n > 33)
           throw Error(ST_UNSUPPORTED, "a waypoint walk exceeded its limit (a long path or cycle without waypoints); "
@@ -818,7 +915,8 @@ void satostop_solve(Lane& ctx, const Circuit& c, uint32_t unknowns, const ns_sat
   const unsigned long long* final_tab = nullptr;
   if (any && !searched && walks && !walks_failed) {
     const TableView tv{ctx.tab_a.as<unsigned int>(), wide ? ctx.tab_e.as<unsigned int>() : nullptr, split, nullptr, 0};
-    searched = waypoint_search(ctx, n, unknowns, k, tv, r, d_steps, d_ends, stats, host);
+    searched = waypoint_search(ctx, n, unknowns, k, table_walk(ctx, n, k, tv), r, d_steps, d_ends, stats, host);
+    if (searched) r.method = 1;
   }
   if (any && !searched) {
     // even entry count for the two-entries-per-thread pass

@@ -116,12 +116,15 @@ def test_naivesatostop_hashmap_search(tmp_path):
     g = tmp_path / "nostop34.circuit"
     g.write_text(mk(point_stop_circuit(n, p + 1, nxt)).text())
     assert "Unsatisfiable!" in run("naivesatostop", "-t", "opencl:0", "-M", "-e", 30, "-H", 20, g, U)
-    # 36 state bits: past the table, the hash-map search without -M
+    # 36 state bits: past the table, the walks without one answer; -M runs the hash map
     n, U, nxt = 36, 2, 0x9_1234_5678
     p = 2 << (n - U)
     h36 = tmp_path / "stop36.circuit"
     h36.write_text(mk(point_stop_circuit(n, p, nxt)).text())
-    out = run("naivesatostop", "-t", "opencl:0", "-e", 30, "-H", 20, h36, U)
+    out = run("naivesatostop", "-t", "opencl:0", h36, U)
+    assert "HashMap length" not in "".join(out)
+    assert f"Solution: Solution {{ start: {p}, end: {nxt}, steps: 1 }}" in out
+    out = run("naivesatostop", "-t", "opencl:0", "-M", "-e", 30, "-H", 20, h36, U)
     assert "HashMap length: 33554432 bytes, bits=20" in out
     assert f"Solution: Solution {{ start: {p}, end: {nxt}, steps: 1 }}" in out
     # --hashmap-search at 24 state bits: the reference's mem-image case 2 again

@@ -70,10 +70,12 @@ def test_34_bit_state_space(ctx):
     c, target = ns.Circuit.gen_satostop(7501, 17, 6, 100, U)
     ctx.trim()
     try:
-        r = ctx.satostop_solve(c, U)
+        r = ctx.satostop_solve(c, U, method="table")
+        rb = ctx.satostop_solve(c, U)  # the default there: walks without a table
     finally:
         ctx.trim()  # 83 GiB of image and chunk tables
-    assert (r["method"], r["waypoint_bits"]) == (1, 12)
+    assert (r["method"], r["waypoint_bits"]) == (1, 12) and rb["method"] == 2
+    assert np.array_equal(rb["steps_arr"], r["steps_arr"]) and np.array_equal(rb["ends_arr"], r["ends_arr"])
     steps, ends = r["steps_arr"], r["ends_arr"]
     stopped = steps > 0
     assert r["n_stopped"] == int(stopped.sum()) > 0
