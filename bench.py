@@ -16,15 +16,16 @@ Also reported on the same line:
                 of the same search;
   extra         the SATOSTOP search over the 30-bit Feistel's 2^30-entry table (waypoint walks,
                 their scattered-read rate against a probe of the same table size, and the
-                passes over the whole table with their HBM rate), one sweep of the hash-map
-                search at 34 state bits, the
+                passes over the whole table with their HBM rate, and the walks that evaluate
+                the circuit instead of reading a table), the same without a table at 36 state
+                bits, one sweep of the hash-map search at 34 state bits, the
                 QSAT search time (32 inputs, 10,000 gates) and the served request stream
                 (4,096 SAT/QSAT/SATOSTOP requests, Poisson 200/s, seed 5): solves/s with every
                 request queued at once and p50/p99 latency with requests arriving in real
                 time, each for 8 concurrent worker lanes and for one-at-a-time solving.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-extra] [--no-serve]
-       python bench.py --only {sat,table,hashmap,qsat,serve,copy}   (one part, for profiling runs;
+       python bench.py --only {sat,table,stop36,hashmap,qsat,serve,copy}   (one part, for profiling runs;
        `copy` times a plain 1 GiB device copy used to calibrate the HBM byte counters)
 """
 import argparse
@@ -118,8 +119,11 @@ def table_part(ns, ctx):
     ctx.satostop_solve(cs, 14, arrays=False)  # kernel build
     rs = ctx.satostop_solve(cs, 14, arrays=False)
     rp = ctx.satostop_solve(cs, 14, arrays=False, passes=True)
-    assert (rs["n_stopped"], rs["lowest_u"], rs["steps"], rs["end"]) == \
-        (rp["n_stopped"], rp["lowest_u"], rp["steps"], rp["end"])
+    ctx.satostop_solve(cs, 14, arrays=False, method="bits")  # kernel build
+    rb = ctx.satostop_solve(cs, 14, arrays=False, method="bits")
+    for r in (rp, rb):
+        assert (rs["n_stopped"], rs["lowest_u"], rs["steps"], rs["end"]) == \
+            (r["n_stopped"], r["lowest_u"], r["steps"], r["end"])
     jump = ctx.pass_times()[1:]  # u64 passes (the first reads the u32 table)
     entries = 1 << 30
     bytes_pass = entries * 24  # read entry + gathered entry + write, 8 B each
@@ -150,7 +154,35 @@ def table_part(ns, ctx):
                                            "traffic": traffic.get(("k_jump_pass", "satostop_n30")),
                                            "kernel": "k_jump_pass", "bytes_per_entry": 24}},
             "speedup_vs_passes": (rp["build_ms"] + rp["jump_ms"] + rp["walk_ms"]) /
-                                 (rs["build_ms"] + rs["walk_ms"] + rs["jump_ms"])}
+                                 (rs["build_ms"] + rs["walk_ms"] + rs["jump_ms"]),
+            "walks_without_table": walks_without_table(rb, "ns_walk")}
+
+
+def walks_without_table(r, kernel):
+    """the search by walks that evaluate the circuit (32 per lane per step): VALU-bound, one
+    op per cone gate per 32 evaluations"""
+    walk_s = r["walk_ms"] / 1000.0
+    tops = r["walk_evaluations"] * r["cone_gates"] / 32 / walk_s / 1e12
+    return {"method": r["method"], "waypoint_bits": r["waypoint_bits"], "walk_ms": r["walk_ms"],
+            "waypoint_jump_ms": r["jump_ms"], "device_ms": r["walk_ms"] + r["jump_ms"],
+            "walk_evaluations": r["walk_evaluations"], "evaluations_per_s": r["walk_evaluations"] / walk_s,
+            "cone_gates": r["cone_gates"], "n_stopped": r["n_stopped"], "lowest_u": r["lowest_u"],
+            "roofline": {"bound": "valu", "achieved": tops, "peak": VALU_PEAK_OPS / 1e12, "unit": "TOP/s",
+                         "frac": tops * 1e12 / VALU_PEAK_OPS, "kernel": kernel}}
+
+
+STOP36_WORKLOAD = ("SATOSTOP n=36 Feistel (6 rounds x 200 gates, seed 10601), 14 unknowns: every unknown's "
+                   "answer by walks without a table")
+
+
+def stop36_part(ns, ctx):
+    c, target = ns.Circuit.gen_satostop(10601, 18, 6, 200, 14)
+    ctx.satostop_solve(c, 14, arrays=False)  # kernel build
+    r = ctx.satostop_solve(c, 14, arrays=False)
+    ctx.trim()
+    d = walks_without_table(r, "ns_walk")
+    d.update({"workload": STOP36_WORKLOAD, "wall_ms": r["wall_ms"], "steps": r["steps"], "end": r["end"]})
+    return d
 
 
 HASHMAP_WORKLOAD = ("SATOSTOP n=34 Feistel (6 rounds x 136 gates, seed 9001), 26 unknowns, hash-map search "
@@ -231,7 +263,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--serve-requests", type=int, default=4096)
     ap.add_argument("--serve-workers", type=int, default=8)
-    ap.add_argument("--only", choices=["sat", "table", "hashmap", "qsat", "serve", "copy"])
+    ap.add_argument("--only", choices=["sat", "table", "stop36", "hashmap", "qsat", "serve", "copy"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -254,6 +286,8 @@ def main():
                 part = table_part(ns, ctx)
             elif args.only == "hashmap":
                 part = hashmap_part(ns, ctx)
+            elif args.only == "stop36":
+                part = stop36_part(ns, ctx)
             else:
                 part = qsat_part(ns, ctx)
         print(json.dumps({"only": args.only, **part}))
@@ -270,8 +304,8 @@ def main():
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_part(c, cone, args.cpu_threads or min(os.cpu_count() or 1, 64))
     if not args.no_extra:
-        extra = {"satostop_table": table_part(ns, ctx), "satostop_hashmap": hashmap_part(ns, ctx),
-                 "qsat": qsat_part(ns, ctx)}
+        extra = {"satostop_table": table_part(ns, ctx), "satostop_36": stop36_part(ns, ctx),
+                 "satostop_hashmap": hashmap_part(ns, ctx), "qsat": qsat_part(ns, ctx)}
         if not args.no_serve:
             extra["serve"] = serve_part(ns, args.serve_requests, args.serve_workers)
         line["extra"] = extra

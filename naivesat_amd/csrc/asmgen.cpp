@@ -725,10 +725,12 @@ NativeKernel build_native(const Circuit& c, NativeKind kind, const std::vector<u
   // 20,000-gate search, two waves per SIMD ran 1.45 times the vector instructions per cycle
   // of one (not twice: lane moves and scalar work grow with the groups), so the groups are
   // taken when they reach a two-wave class at under 1.45 times the vector instructions.
+  // Each try costs about one more code generation (tens of ms for large bodies), so they are
+  // tried for spaces of 2^34 assignments and more only, where a search takes seconds.
   // NS_OUTPUT_GROUPS=K forces K (1: never).
   long forced = -1;
   if (const char* e = std::getenv("NS_OUTPUT_GROUPS")) forced = std::atol(e);
-  if (!table && outs.size() == 1 && forced != 1 && (forced > 1 || best.vgpr_class == 512)) {
+  if (!table && outs.size() == 1 && forced != 1 && (forced > 1 || (best.vgpr_class == 512 && c.n >= 34))) {
     const uint32_t ks[6] = {2, 3, 4, 5, 6, 8};
     for (uint32_t K : ks) {
       if (forced > 1 && (uint32_t)forced != K) continue;
