@@ -119,12 +119,12 @@ def table_part(ns, ctx):
     ctx.satostop_solve(cs, 14, arrays=False)  # kernel build
     rs = ctx.satostop_solve(cs, 14, arrays=False)
     rp = ctx.satostop_solve(cs, 14, arrays=False, passes=True)
+    jump = ctx.pass_times()[1:]  # u64 passes (the first reads the u32 table)
     ctx.satostop_solve(cs, 14, arrays=False, method="bits")  # kernel build
     rb = ctx.satostop_solve(cs, 14, arrays=False, method="bits")
     for r in (rp, rb):
         assert (rs["n_stopped"], rs["lowest_u"], rs["steps"], rs["end"]) == \
             (r["n_stopped"], r["lowest_u"], r["steps"], r["end"])
-    jump = ctx.pass_times()[1:]  # u64 passes (the first reads the u32 table)
     entries = 1 << 30
     bytes_pass = entries * 24  # read entry + gathered entry + write, 8 B each
     avg = sum(jump) / len(jump)

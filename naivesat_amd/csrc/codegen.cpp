@@ -598,35 +598,55 @@ std::string gen_walk_kernel(const Circuit& c, uint32_t k, GenInfo* info) {
   };
   os << "extern \"C\" __global__ void __launch_bounds__(256) ns_walk(unsigned long long count, "
         "unsigned long long limit, unsigned long long budget, ulonglong2* __restrict__ out, unsigned int* long_flag, "
-        "unsigned long long* evals) {\n";
-  // per slot: start time (bits 0..31), queue position (32..55), save exponent (56..63)
+        "unsigned long long* evals, unsigned long long* queue) {\n";
+  // per slot: start time (bits 0..31) and waypoint index (32..63) of its walk
   os << "  __shared__ unsigned long long meta[32 * 256];\n"
-        "  const unsigned int tid = threadIdx.x;\n"
-        "  const unsigned long long lanes = (unsigned long long)gridDim.x * blockDim.x;\n"
-        "  const unsigned long long lid = (unsigned long long)blockIdx.x * blockDim.x + tid;\n"
+        "  const unsigned int tid = threadIdx.x, lane = tid & 63u;\n"
         "  const unsigned long long lim = limit < 0x7fffffffull ? limit : 0x7fffffffull;\n";
   each("  unsigned int s@ = 0u, v@ = 0u;\n");
-  os << "  unsigned int active = 0u, t = 0u, next_event = 0xffffffffu;\n"
-        "  unsigned long long q = 0ull, used = 0ull, ev = 0ull;\n";
-  // slot b takes the lane's next walk at time t: its bits of the state and the saved state
-  // become the waypoint's state
-  std::ostringstream asg;
-  asg << "    {\n"
-         "      const unsigned long long idx = lid + q * lanes;\n"
-         "      if (idx < count && q < 0xffffffull) {\n"
-         "        const unsigned long long st = idx << "
-      << k << ";\n";
+  // fin: slots whose walk ended since the last check (their state frozen), with the kind of
+  // ending in fstop / fwp / floop (none of them: the limit) and, bit-sliced in a0..a3, the
+  // evaluations since it ended
+  os << "  unsigned int active = 0u, fin = 0u, fstop = 0u, fwp = 0u, floop = 0u, t = 0u;\n"
+        "  unsigned int a0 = 0u, a1 = 0u, a2 = 0u, a3 = 0u;\n"
+        "  unsigned long long used = 0ull, ev = 0ull;\n"
+        "  bool drained = false;\n";
+  // The slots in `need` take the next waypoints of the global queue at time t (one atomic
+  // per wave: the lanes' shares are an exclusive prefix sum of their counts); a slot's bits
+  // of the state and of the saved state become its waypoint's state.
+  std::ostringstream refill;
+  refill << "    if (!drained) {\n"
+            "      const unsigned int r = __popc(need);\n"
+            "      unsigned int incl = r;\n"
+            "      for (unsigned int off = 1; off < 64u; off <<= 1) {\n"
+            "        const unsigned int y = __shfl_up(incl, off);\n"
+            "        if (lane >= off) incl += y;\n"
+            "      }\n"
+            "      const unsigned int total = __shfl(incl, 63);\n"
+            "      unsigned long long base = 0ull;\n"
+            "      if (total != 0u) {\n"
+            "        if (lane == 0u) base = atomicAdd(queue, (unsigned long long)total);\n"
+            "        base = __shfl(base, 0);\n"
+            "      }\n"
+            "      if (base + total >= count) drained = true;\n"
+            "      unsigned long long nx = base + (incl - r);\n"
+            "      for (unsigned int rem = need; rem; rem &= rem - 1u) {\n"
+            "        const unsigned int b = __builtin_ctz(rem), mb = 1u << b;\n"
+            "        const unsigned long long idx = nx++;\n"
+            "        if (idx >= count) break;\n"
+            "        const unsigned long long st = idx << "
+         << k << ";\n";
   for (uint32_t i = 0; i < n; i++)
-    asg << "        s" << i << " = (s" << i << " & ~mb) | ((unsigned int)((st >> " << i << ") & 1ull) * mb); v" << i
-        << " = (v" << i << " & ~mb) | (s" << i << " & mb);\n";
-  asg << "        meta[b * 256u + tid] = (unsigned long long)t | (q << 32);\n"
-         "        active |= mb;\n"
-         "        q++;\n"
-         "        next_event = next_event < t + 1u ? next_event : t + 1u;\n"
-         "      }\n"
-         "    }\n";
-  os << "  for (unsigned int b = 0; b < 32u; b++) {\n    const unsigned int mb = 1u << b;\n" << asg.str() << "  }\n";
-  os << "  while (active) {\n"
+    refill << "        s" << i << " = (s" << i << " & ~mb) | ((unsigned int)((st >> " << i << ") & 1ull) * mb); v" << i
+           << " = (v" << i << " & ~mb) | (s" << i << " & mb);\n";
+  refill << "        meta[b * 256u + tid] = (unsigned long long)t | (idx << 32);\n"
+            "        active |= mb;\n"
+            "      }\n"
+            "    }\n";
+  os << "  {\n    const unsigned int need = 0xffffffffu;\n" << refill.str() << "  }\n";
+  // the loop runs while any lane of the wave has a walk, so every lane takes part in the
+  // queue's prefix sums
+  os << "  while (__any((active | fin) != 0u)) {\n"
         "    if (++used > budget) {\n"
         "      atomicOr(long_flag, 1u);\n"
         "      break;\n"
@@ -648,53 +668,70 @@ std::string gen_walk_kernel(const Circuit& c, uint32_t k, GenInfo* info) {
   for (uint32_t i = 0; i < k; i++) os << " | o" << i;
   os << ";\n    const unsigned int wp = ~nz & active & ~stop;\n    const unsigned int eq = 0xffffffffu";
   for (uint32_t i = 0; i < n; i++) os << " & ~(o" << i << " ^ v" << i << ")";
-  os << ";\n    const unsigned int loop = eq & active & ~stop & ~wp;\n";
-  each("    s@ = o@;\n");
-  os << "    t++;\n"
-        "    unsigned int done = stop | wp | loop, lng = 0u;\n"
-        // per-slot events: the saves of Brent's test (after 1, 2, 4, ... evaluations) and the limit
-        "    if (t >= next_event) {\n"
-        "      next_event = 0xffffffffu;\n"
-        "      unsigned int save = 0u;\n"
-        "      for (unsigned int rem = active & ~done; rem; rem &= rem - 1u) {\n"
+  os << ";\n    const unsigned int loop = eq & active & ~stop & ~wp;\n"
+        "    const unsigned int done = stop | wp | loop;\n";
+  // running slots advance; a slot that ends keeps the state its last evaluation gave
+  each("    s@ = (o@ & active) | (s@ & ~active);\n");
+  os << "    {\n"
+        "      unsigned int cy = fin;\n"
+        "      a0 ^= cy;\n"
+        "      cy &= ~a0;\n"
+        "      a1 ^= cy;\n"
+        "      cy &= ~a1;\n"
+        "      a2 ^= cy;\n"
+        "      cy &= ~a2;\n"
+        "      a3 ^= cy;\n"
+        "    }\n"
+        "    fin |= done;\n"
+        "    fstop |= stop;\n"
+        "    fwp |= wp;\n"
+        "    floop |= loop;\n"
+        "    active &= ~done;\n"
+        "    t++;\n"
+        // every 16 evaluations: Brent's saves (a slot saves its state at the first check
+        // after each power of two of its evaluations), the limit, the records of the ended
+        // walks and new walks in their slots
+        "    if ((t & 15u) == 0u) {\n"
+        "      unsigned int save = 0u, lng = 0u;\n"
+        "      for (unsigned int rem = active; rem; rem &= rem - 1u) {\n"
         "        const unsigned int b = __builtin_ctz(rem);\n"
-        "        unsigned long long m = meta[b * 256u + tid];\n"
-        "        const unsigned int start = (unsigned int)m, e = (unsigned int)(m >> 56), steps = t - start;\n"
-        "        if ((unsigned long long)steps >= lim) {\n"
-        "          lng |= 1u << b;\n"
-        "          continue;\n"
-        "        }\n"
-        "        unsigned int e2 = e;\n"
-        "        if (steps >= (1u << e)) {\n"
-        "          save |= 1u << b;\n"
-        "          e2 = e + 1u;\n"
-        "          meta[b * 256u + tid] = (m & 0x00ffffffffffffffull) | ((unsigned long long)e2 << 56);\n"
-        "        }\n"
-        "        const unsigned int due = start + (e2 < 31u ? (1u << e2) : 0x80000000u);\n"
-        "        const unsigned int dl = start + (unsigned int)lim;\n"
-        "        const unsigned int nx = due < dl ? due : dl;\n"
-        "        next_event = next_event < nx ? next_event : nx;\n"
+        "        const unsigned int start = (unsigned int)meta[b * 256u + tid];\n"
+        "        const unsigned int steps = t - start, before = steps > 16u ? steps - 16u : 0u;\n"
+        "        if ((unsigned long long)steps >= lim) lng |= 1u << b;\n"
+        "        else if (before == 0u || __builtin_clz(steps) < __builtin_clz(before)) save |= 1u << b;\n"
         "      }\n"
         "      if (save) {\n";
   each("        v@ = (v@ & ~save) | (s@ & save);\n");
   os << "      }\n"
-        "    }\n"
-        "    if (lng) atomicOr(long_flag, 1u);\n"
-        "    done |= lng;\n"
-        "    for (; done; done &= done - 1u) {\n"
-        "      const unsigned int b = __builtin_ctz(done), mb = 1u << b;\n"
-        "      const unsigned long long m = meta[b * 256u + tid];\n"
-        "      const unsigned long long idx = lid + ((m >> 32) & 0xffffffull) * lanes;\n"
-        "      const unsigned int steps = t - (unsigned int)m;\n"
-        "      const unsigned int kind = (stop & mb) ? 1u : (wp & mb) ? 0u : (loop & mb) ? 2u : 3u;\n"
-        "      unsigned long long st = 0ull;\n";
-  each("      st |= (unsigned long long)((s@ >> b) & 1u) << @;\n");
-  os << "      const unsigned long long target = kind == 1u ? st : kind == 0u ? (st >> " << k
+        "      if (lng) {\n"
+        "        atomicOr(long_flag, 1u);\n"
+        "        fin |= lng;\n"
+        "        active &= ~lng;\n"
+        "      }\n"
+        "      for (unsigned int rem = fin; rem; rem &= rem - 1u) {\n"
+        "        const unsigned int b = __builtin_ctz(rem), mb = 1u << b;\n"
+        "        const unsigned long long m = meta[b * 256u + tid];\n"
+        "        const unsigned int age = ((a0 >> b) & 1u) | (((a1 >> b) & 1u) << 1) | (((a2 >> b) & 1u) << 2) |\n"
+        "                                 (((a3 >> b) & 1u) << 3);\n"
+        "        const unsigned int steps = t - age - (unsigned int)m;\n"
+        "        const unsigned int kind = (fstop & mb) ? 1u : (fwp & mb) ? 0u : (floop & mb) ? 2u : 3u;\n"
+        "        unsigned long long st = 0ull;\n";
+  each("        st |= (unsigned long long)((s@ >> b) & 1u) << @;\n");
+  os << "        const unsigned long long target = kind == 1u ? st : kind == 0u ? (st >> " << k
      << ") : 0ull;\n"
-        "      out[idx] = make_ulonglong2(target | ((unsigned long long)kind << 62), steps);\n"
-        "      active &= ~mb;\n"
-     << asg.str() << "    }\n"
-     << "  }\n";
+        "        out[m >> 32] = make_ulonglong2(target | ((unsigned long long)kind << 62), steps);\n"
+        "      }\n"
+        "      {\n"
+        "        const unsigned int need = fin;\n"
+     << refill.str()
+     << "      }\n"
+        "      fin = 0u;\n"
+        "      fstop = 0u;\n"
+        "      fwp = 0u;\n"
+        "      floop = 0u;\n"
+        "      a0 = a1 = a2 = a3 = 0u;\n"
+        "    }\n"
+        "  }\n";
   // evaluations done by the wave, into one of 64 count slots 16 words apart
   os << "  for (int off = 32; off > 0; off >>= 1) ev += __shfl_xor(ev, off);\n"
         "  if ((tid & 63u) == 0u && ev != 0ull) atomicAdd(evals + ((blockIdx.x & 63u) << 4), ev);\n"

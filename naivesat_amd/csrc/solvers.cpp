@@ -734,15 +734,18 @@ WalkLaunch table_walk(Lane& ctx, uint32_t n, uint32_t k, const TableView& tv) {
   };
 }
 
-// Waypoint spacing of the walks without a table: 2^22 waypoints up to 30 state bits, then
-// more (up to 2^26) so that each lane's 32 slots take many walks each; k <= n - unknowns.
+// Waypoint spacing of the walks without a table (k <= n - unknowns): 2^22 waypoints up to
+// 30 state bits; beyond, k balances the walks' tail (the walks still running when the
+// waypoints run out last ~2^k evaluations more) against the record jumps ((n - k) passes
+// over 2^(n-k) records): measured best at k = 10 for 34 state bits (37 ms against 58 ms at
+// k = 8) and k = 11 for 36, about (n - 14) / 2; at most 2^28 records.
 uint32_t bitwalk_bits(uint32_t n, uint32_t unknowns) {
   if (const char* e = std::getenv("NS_WAYPOINT_BITS")) {
     const long long v = std::atoll(e);
     if (v >= 0 && v <= (long long)n) return (uint32_t)v;
   }
-  const uint32_t wbits = std::min(n, std::max(22u, std::min(26u, n > 8 ? n - 8 : 0u)));
-  return std::min(n - unknowns, n - wbits);
+  const uint32_t k = n <= 30 ? (n > 22 ? n - 22 : 0u) : std::max(n - 28, (n - 14) / 2);
+  return std::min(n - unknowns, k);
 }
 
 // The waypoint walks without a table: the generated ns_walk kernel (bit-sliced walks, 32 per
@@ -764,7 +767,11 @@ WalkLaunch bit_walk(Lane& ctx, const Circuit& c, uint32_t k, double* compile_ms,
     // random map: allow 16 times that, plus one walk of the full limit
     unsigned long long budget = 16ull * ((per_lane + 31) / 32) * (1ull << k) + limit;
     unsigned long long cnt = count, lim = limit;
-    void* args[6] = {&cnt, &lim, &budget, &out, &flag, &evals};
+    // the waypoints are handed out from one counter, so that every slot keeps taking walks
+    // while any are left
+    unsigned long long* queue = reinterpret_cast<unsigned long long*>(ctx.stats.as<char>() + 256);
+    NS_HIP(launch_fill_u64(ctx.stream, queue, 0ull, 1));
+    void* args[7] = {&cnt, &lim, &budget, &out, &flag, &evals, &queue};
     NS_HIP(hipModuleLaunchKernel(mod->fn, (unsigned)(lanes / 256), 1, 1, 256, 1, 1, 0, ctx.stream, args, nullptr));
   };
 }

@@ -126,7 +126,7 @@ def test_config3_full_size_golden(ctx):
 
 
 def test_36_bit_state_space(ctx):
-    # n = 36: past the packed image; 2^26 waypoints 2^10 states apart. The Feistel map is a
+    # n = 36: past the packed image; 2^25 waypoints 2^11 states apart. The Feistel map is a
     # permutation with one STOP transition, so every stopping start ends at the target after a
     # distinct number of evaluations; the short ones are checked by walking on the host
     n, U = 36, 14
@@ -135,7 +135,7 @@ def test_36_bit_state_space(ctx):
         r = ctx.satostop_solve(c, U)
     finally:
         ctx.trim()
-    assert (r["method"], r["waypoint_bits"]) == (2, 10)
+    assert (r["method"], r["waypoint_bits"]) == (2, 11)
     steps, ends = r["steps_arr"], r["ends_arr"]
     stopped = steps > 0
     assert r["n_stopped"] == int(stopped.sum()) > 0
