@@ -499,6 +499,130 @@ LutProgram pressure_order(const LutProgram& p, const std::vector<uint32_t>& out_
   return q;
 }
 
+// A LUT over one lane-varying value x and two wave-uniform values u1, u2 has to copy one of
+// the uniform operands to a VGPR first (a vector instruction reads at most one SGPR here).
+// When its cofactors in (u1, u2) take at most two functions of x, the LUT equals g(x, U) with
+// U = h(u1, u2): one scalar gate (appended to cx) and a LUT reading x and the SGPR of U.
+void split_uniform_pairs(Circuit& cx, LutProgram& p) {
+  std::vector<LutOp> ops;
+  ops.reserve(p.ops.size() + 64);
+  for (const LutOp& op : p.ops) {
+    if (!op.lut || op.src[0] == op.src[1] || op.src[0] == op.src[2] || op.src[1] == op.src[2]) {
+      ops.push_back(op);
+      continue;
+    }
+    int xi = -1, ui = -1, uj = -1;
+    for (int k = 0; k < 3; k++) {
+      if (!p.uniform[op.src[k]]) {
+        if (xi >= 0) xi = 3;  // two lane-varying sources
+        else xi = k;
+      } else if (ui < 0) {
+        ui = k;
+      } else {
+        uj = k;
+      }
+    }
+    if (xi < 0 || xi > 2 || uj < 0) {
+      ops.push_back(op);
+      continue;
+    }
+    // cofactor code of (a, b) = (u1, u2): bit 0 = value at x = 0, bit 1 = value at x = 1
+    auto bit_at = [&](uint32_t x, uint32_t a, uint32_t b) {
+      uint32_t v[3];
+      v[xi] = x;
+      v[ui] = a;
+      v[uj] = b;
+      return (op.fn >> ((v[0] << 2) | (v[1] << 1) | v[2])) & 1u;
+    };
+    uint32_t code[2][2];
+    for (uint32_t a = 0; a < 2; a++)
+      for (uint32_t b = 0; b < 2; b++) code[a][b] = bit_at(0, a, b) | (bit_at(1, a, b) << 1);
+    const uint32_t f0 = code[0][0];
+    uint32_t f1 = f0, distinct = 1;
+    for (uint32_t a = 0; a < 2; a++)
+      for (uint32_t b = 0; b < 2; b++)
+        if (code[a][b] != f0) {
+          if (distinct == 1) {
+            f1 = code[a][b];
+            distinct = 2;
+          } else if (code[a][b] != f1) {
+            distinct = 3;
+          }
+        }
+    if (distinct != 2) {
+      ops.push_back(op);
+      continue;
+    }
+    uint32_t t = 0;  // bit (a << 1 | b): cofactor f1
+    for (uint32_t a = 0; a < 2; a++)
+      for (uint32_t b = 0; b < 2; b++)
+        if (code[a][b] == f1) t |= 1u << ((a << 1) | b);
+    const uint32_t u1 = op.src[ui], u2 = op.src[uj];
+    // U as one gate of the circuit's four functions over (u1, u2) or (u2, u1), or its complement
+    auto gate2 = [](int f, uint32_t a, uint32_t b) -> uint32_t {
+      switch (f) {
+        case FUNC_AND: return a & b;
+        case FUNC_NOR: return (a | b) ^ 1u;
+        case FUNC_NIMPL: return a & (b ^ 1u);
+        default: return a ^ b;
+      }
+    };
+    int func = -1;
+    bool swap = false, pol = true;
+    for (int f = 0; f < 4 && func < 0; f++)
+      for (int sw = 0; sw < 2 && func < 0; sw++) {
+        uint32_t h = 0;
+        for (uint32_t a = 0; a < 2; a++)
+          for (uint32_t b = 0; b < 2; b++)
+            if (gate2(f, sw ? b : a, sw ? a : b)) h |= 1u << ((a << 1) | b);
+        if (h == t || h == (~t & 15u)) {
+          func = f;
+          swap = sw != 0;
+          pol = h == t;
+        }
+      }
+    uint32_t uw;
+    if (t == 0x3 || t == 0xc) {  // U = u1 or its complement
+      uw = u1;
+      pol = t == 0xc;
+    } else if (t == 0x5 || t == 0xa) {  // U = u2 or its complement
+      uw = u2;
+      pol = t == 0xa;
+    } else if (func >= 0) {
+      const uint32_t a = swap ? u2 : u1, b = swap ? u1 : u2;
+      cx.gates.push_back({a, b, (uint8_t)func});
+      uw = cx.n + (uint32_t)cx.gates.size() - 1;
+      p.uniform.push_back(1);
+      LutOp sop;
+      sop.lut = 0;
+      sop.fn = (uint8_t)func;
+      sop.dst = uw;
+      sop.src[0] = a;
+      sop.src[1] = b;
+      sop.src[2] = a;
+      ops.push_back(sop);
+      p.scalar_ops++;
+    } else {
+      ops.push_back(op);
+      continue;
+    }
+    // g(x, U): sources (x, U, x); the third position repeats x
+    LutOp nop = op;
+    nop.src[0] = op.src[xi];
+    nop.src[1] = uw;
+    nop.src[2] = op.src[xi];
+    uint8_t tt = 0;
+    for (uint32_t idx = 0; idx < 8; idx++) {
+      const uint32_t xv = (idx >> 2) & 1u, uv = (idx >> 1) & 1u;
+      const uint32_t sel = (uv != 0) == pol ? f1 : f0;
+      if ((sel >> xv) & 1u) tt |= (uint8_t)(1u << idx);
+    }
+    nop.fn = tt;
+    ops.push_back(nop);
+  }
+  p.ops = std::move(ops);
+}
+
 // The output as a conjunction of literals (wire, negated): descends through And, Nor and
 // Nimpl gates from a non-negated output (a & b, ~a & ~b, a & ~b); Xor gates, inputs and
 // negated literals are leaves.
@@ -671,14 +795,17 @@ NativeKernel build_native(const Circuit& c, NativeKind kind, const std::vector<u
     }
     auto cone_cc = output_cone(cc, outs_cc);
     auto order_cc = schedule_gates(cc, cone_cc, outs_cc);
-    LutProgram p = pressure_order(map_luts(cc, order_cc, out_wires, uni), out_wires, cc.wires(), wgroup);
+    Circuit cx = cc;  // with the scalar gates of split_uniform_pairs appended
+    LutProgram p0 = map_luts(cc, order_cc, out_wires, uni);
+    split_uniform_pairs(cx, p0);
+    LutProgram p = pressure_order(p0, out_wires, cx.wires(), wgroup);
     at.valu = p.valu_ops;
     at.scalar = p.scalar_ops;
     // lane values live at once in this order: classes whose pool cannot hold them and that
     // have no AGPRs to spill to are skipped
     uint32_t peak = 0;
     {
-      std::vector<uint32_t> rem(cc.wires(), 0);
+      std::vector<uint32_t> rem(cx.wires(), 0);
       for (const LutOp& op : p.ops)
         for (int k = 0; k < (op.lut ? 3 : 2); k++) {
           bool dup = false;
@@ -692,7 +819,7 @@ NativeKernel build_native(const Circuit& c, NativeKind kind, const std::vector<u
           uint32_t w = op.src[k];
           bool dup = false;
           for (int q = 0; q < k; q++) dup |= op.src[q] == w;
-          if (!dup && --rem[w] == 0 && w >= cc.n && !p.uniform[w]) live--;
+          if (!dup && --rem[w] == 0 && w >= cx.n && !p.uniform[w]) live--;
         }
         if (rem[op.dst] > 0 && !p.uniform[op.dst]) live++;
         peak = std::max(peak, live);
@@ -702,7 +829,7 @@ NativeKernel build_native(const Circuit& c, NativeKind kind, const std::vector<u
       if (table && cl.budget < 128) continue;
       if (!cl.agprs && peak + cl.slots > cl.hi - pool_lo + 1) continue;
       try {
-        Allocator a(cc, p, vbit, lane_shift, lm, pool_lo, cl.hi, cl.agprs, cl.slots, out_wires, out_neg);
+        Allocator a(cx, p, vbit, lane_shift, lm, pool_lo, cl.hi, cl.agprs, cl.slots, out_wires, out_neg);
         at.body = a.run();
         at.vgpr_class = cl.budget;
         at.spills = a.spills();
