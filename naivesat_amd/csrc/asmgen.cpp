@@ -17,7 +17,9 @@ namespace ns {
 namespace {
 
 const uint32_t kWordMasks[5] = {0xAAAAAAAAu, 0xCCCCCCCCu, 0xF0F0F0F0u, 0xFF00FF00u, 0xFFFF0000u};
-const uint32_t kSgprLo = 40, kSgprHi = 95;
+// SGPRs of the body: s[40..95]; the SAT templates, whose own code keeps to s0..s13, also
+// give it s[14..19] and s[22..39] (s20 and s21 carry the word index)
+const uint32_t kSgprLoSat = 14, kSgprLo = 40, kSgprHi = 95;
 
 struct Loc {
   int v = -1;   // VGPR holding the value
@@ -42,14 +44,16 @@ class Allocator {
  public:
   Allocator(const Circuit& c, const LutProgram& p, const std::vector<uint32_t>& vbit, uint32_t lane_shift,
             uint32_t lm_base, uint32_t pool_lo, uint32_t pool_hi, uint32_t agprs, uint32_t slot_regs,
-            const std::vector<uint32_t>& out_wires, const std::vector<uint8_t>& out_neg)
+            const std::vector<uint32_t>& out_wires, const std::vector<uint8_t>& out_neg, uint32_t sgpr_lo)
       : c_(c), p_(p), vbit_(vbit), ls_(lane_shift), lm_(lm_base), pool_lo_(pool_lo), pool_hi_(pool_hi),
+        sgpr_lo_(sgpr_lo),
         loc_(c.wires()), uses_(c.wires()), ptr_(c.wires(), 0), outw_(out_wires), outneg_(out_neg) {
     slot_base_ = (int)pool_hi - (int)slot_regs + 1;
     pool_hi_ = pool_hi - slot_regs;
     for (int r = (int)pool_hi_; r >= (int)pool_lo; r--) vfree_[r & 3].push_back(r);
     for (int r = (int)(64 * slot_regs) - 1; r >= 0; r--) slotfree_.push_back(r);
-    for (int r = (int)kSgprHi; r >= (int)kSgprLo; r--) sfree_.push_back(r);
+    for (int r = (int)kSgprHi; r >= (int)sgpr_lo_; r--)
+      if (r != 20 && r != 21) sfree_.push_back(r);
     for (int r = (int)agprs - 1; r >= 0; r--) afree_.push_back(r);
     vowner_.assign(512, UINT32_MAX);
     end_ = (uint32_t)p.ops.size();
@@ -275,7 +279,8 @@ class Allocator {
   void evict_sgpr() {
     uint32_t best = UINT32_MAX;
     uint64_t best_use = 0;
-    for (uint32_t r = kSgprLo; r <= kSgprHi; r++) {
+    for (uint32_t r = sgpr_lo_; r <= kSgprHi; r++) {
+      if (r == 20 || r == 21) continue;
       const uint32_t w = sowner_[r];
       if (w == UINT32_MAX || loc_[w].dead) continue;
       bool pin = false;
@@ -378,7 +383,7 @@ class Allocator {
   const Circuit& c_;
   const LutProgram& p_;
   const std::vector<uint32_t>& vbit_;
-  uint32_t ls_, lm_, pool_lo_, pool_hi_;
+  uint32_t ls_, lm_, pool_lo_, pool_hi_, sgpr_lo_;
   std::vector<Loc> loc_;
   std::vector<std::vector<uint32_t>> uses_;
   std::vector<uint32_t> ptr_;
@@ -829,7 +834,8 @@ NativeKernel build_native(const Circuit& c, NativeKind kind, const std::vector<u
       if (table && cl.budget < 128) continue;
       if (!cl.agprs && peak + cl.slots > cl.hi - pool_lo + 1) continue;
       try {
-        Allocator a(cx, p, vbit, lane_shift, lm, pool_lo, cl.hi, cl.agprs, cl.slots, out_wires, out_neg);
+        Allocator a(cx, p, vbit, lane_shift, lm, pool_lo, cl.hi, cl.agprs, cl.slots, out_wires, out_neg,
+                    kind == NativeKind::SAT ? kSgprLoSat : kSgprLo);
         at.body = a.run();
         at.vgpr_class = cl.budget;
         at.spills = a.spills();

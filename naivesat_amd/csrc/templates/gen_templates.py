@@ -9,7 +9,8 @@ Register contract of every placeholder ("; NS_BODY <name>"):
            s20, s21      wave-uniform part of the word index (lo, hi): every index bit except
                          the six lane bits
   outputs  v0 .. v[OUT-1] circuit outputs for the current word
-  scratch  v[POOL_LO..POOL_HI], s[40..95], scc (clobbered)
+  scratch  v[POOL_LO..POOL_HI], s[40..95], scc (clobbered); the SAT kernels, whose own code
+           keeps to s0..s13, also give the body s[14..19] and s[22..39]
 Register classes bound the VGPR budget (and with it the occupancy).
 
 Usage: python gen_templates.py OUT.hip
@@ -22,14 +23,15 @@ CLASSES = {  # name: (total VGPR budget, pool high register); r512 adds a0..a255
     "r256": (256, 239),
     "r512": (512, 239),
 }
-SGPR_POOL = (40, 95)
+SGPR_POOL = list(range(40, 96))
+SAT_SGPR_POOL = list(range(14, 20)) + list(range(22, 96))  # s20, s21 carry the word index
 
 
-def clobbers(lo, hi, agprs=False):
+def clobbers(lo, hi, agprs=False, sgprs=SGPR_POOL):
     regs = [f'"v{i}"' for i in range(lo, hi + 1)]
     if agprs:
         regs += [f'"a{i}"' for i in range(256)]
-    regs += [f'"s{i}"' for i in range(SGPR_POOL[0], SGPR_POOL[1] + 1)]
+    regs += [f'"s{i}"' for i in sgprs]
     regs.append('"scc"')
     return ", ".join(regs)
 
@@ -68,12 +70,12 @@ __device__ inline u32 ns_red(u32 x, u32 sh, u32 all) { return all ? (x & (x >> s
 '''
 
 
-def body1(name, lm, lo, hi):
+def body1(name, lm, lo, hi, sgprs=SGPR_POOL):
     """placeholder with one output in v0"""
     ins = ", ".join([f'"{{v{lm + i}}}"(LM{i})' for i in range(6)] + ['"{s20}"(wlo)', '"{s21}"(whi)'])
     agprs = name.endswith("r512")
     return (f'    u32 o;\n'
-            f'    asm volatile("; NS_BODY {name}" : "={{v0}}"(o) : {ins} : {clobbers(lo, hi, agprs)});\n')
+            f'    asm volatile("; NS_BODY {name}" : "={{v0}}"(o) : {ins} : {clobbers(lo, hi, agprs, sgprs)});\n')
 
 
 def sat_kernel(cls, vb, pool_hi):
@@ -89,7 +91,7 @@ extern "C" __global__ void __launch_bounds__(256) {name}(u64 w0, u64 wend, u64* 
     const u64 wu = U + ((u64)j << 6);
     const u64 w = wu + lane;
     const u32 wlo = (u32)wu, whi = (u32)(wu >> 32);
-{body1(name, 1, 8, pool_hi)}    o &= valid;
+{body1(name, 1, 8, pool_hi, SAT_SGPR_POOL)}    o &= valid;
     if (w >= wend) o = 0u;
     cnt += __popc(o);
     if (o != 0u && best == ~0ull) best = (w << 5) | (u64)__builtin_ctz(o);
