@@ -656,7 +656,8 @@ std::vector<std::pair<uint32_t, bool>> and_literals(const Circuit& c, uint32_t w
 // the group (1..K) of every new wire, so the schedule can keep groups apart and only the
 // accumulator lives from one group into the next. Returns false when the output has fewer
 // than K literals.
-bool grouped_circuit(const Circuit& c, uint32_t out_k, uint32_t K, Circuit& out, std::vector<uint32_t>& wgroup) {
+bool grouped_circuit(const Circuit& c, uint32_t out_k, uint32_t K, Circuit& out, std::vector<uint32_t>& wgroup,
+                     bool refine) {
   const Output o = c.outputs.at(out_k);
   const auto lits = and_literals(c, o.wire);
   if (K < 2 || lits.size() < K) return false;
@@ -718,6 +719,57 @@ bool grouped_circuit(const Circuit& c, uint32_t out_k, uint32_t K, Circuit& out,
       for (uint32_t g = 0; g < G; g++) u[g] |= cone[pick][g];
     }
     groups.push_back(std::move(grp));
+  }
+  if (refine) {
+    // Local search from the greedy groups: move one literal to another group while that
+    // lowers the total of the groups' cone gates (the gates evaluated) and no group grows past
+    // the largest greedy group (a proxy for the registers it needs). Each move strictly
+    // lowers the total, so the search ends. On the 36-input, 20,000-gate search it takes the
+    // 5 groups from 3,966 cone gates to 3,248.
+    const size_t ng = groups.size();
+    std::vector<std::vector<uint32_t>> clist(lits.size());
+    for (size_t i = 0; i < lits.size(); i++)
+      for (uint32_t g = 0; g < G; g++)
+        if (cone[i][g]) clist[i].push_back(g);
+    std::vector<std::vector<uint16_t>> cnt(ng, std::vector<uint16_t>(G, 0));
+    std::vector<uint32_t> gsize(ng, 0);
+    for (size_t a = 0; a < ng; a++)
+      for (size_t i : groups[a])
+        for (uint32_t g : clist[i])
+          if (cnt[a][g]++ == 0) gsize[a]++;
+    const uint32_t cap = *std::max_element(gsize.begin(), gsize.end());
+    for (int pass = 0; pass < 64; pass++) {
+      bool moved = false;
+      for (size_t a = 0; a < ng; a++)
+        for (size_t pos = 0; pos < groups[a].size();) {
+          const size_t i = groups[a][pos];
+          if (groups[a].size() == 1) break;
+          int64_t freed = 0;
+          for (uint32_t g : clist[i]) freed += cnt[a][g] == 1;
+          size_t to = SIZE_MAX;
+          int64_t gain = 0;
+          for (size_t b = 0; b < ng; b++) {
+            if (b == a) continue;
+            int64_t added = 0;
+            for (uint32_t g : clist[i]) added += cnt[b][g] == 0;
+            if (gsize[b] + added > cap || freed - added <= gain) continue;
+            gain = freed - added;
+            to = b;
+          }
+          if (to == SIZE_MAX) {
+            pos++;
+            continue;
+          }
+          for (uint32_t g : clist[i]) {
+            if (--cnt[a][g] == 0) gsize[a]--;
+            if (cnt[to][g]++ == 0) gsize[to]++;
+          }
+          groups[a].erase(groups[a].begin() + (long)pos);
+          groups[to].push_back(i);
+          moved = true;
+        }
+      if (!moved) break;
+    }
   }
   out = Circuit();
   out.n = c.n;
@@ -865,20 +917,30 @@ NativeKernel build_native(const Circuit& c, NativeKind kind, const std::vector<u
   if (const char* e = std::getenv("NS_OUTPUT_GROUPS")) forced = std::atol(e);
   if (!table && outs.size() == 1 && forced != 1 && (forced > 1 || (best.vgpr_class == 512 && c.n >= 34))) {
     const uint32_t ks[6] = {2, 3, 4, 5, 6, 8};
+    bool done = false;
     for (uint32_t K : ks) {
       if (forced > 1 && (uint32_t)forced != K) continue;
-      Circuit g;
-      std::vector<uint32_t> wg;
-      if (!grouped_circuit(c, outs[0], K, g, wg)) break;
-      Attempt at = attempt(g, {0}, &wg);
-      if (!at.vgpr_class) continue;
-      if (forced > 1 ||
-          (at.vgpr_class <= 256 && best.vgpr_class == 512 && at.vector_lines * 100 < best.vector_lines * 145)) {
-        best = std::move(at);
-        groups = K;
-        break;
+      // the refined groups first (fewer gates), then the greedy ones (smaller groups)
+      bool two_wave = false;
+      for (int refine = 1; refine >= 0 && !done; refine--) {
+        Circuit g;
+        std::vector<uint32_t> wg;
+        if (!grouped_circuit(c, outs[0], K, g, wg, refine != 0)) {
+          done = true;
+          break;
+        }
+        Attempt at = attempt(g, {0}, &wg);
+        if (!at.vgpr_class) continue;
+        if ((forced > 1 && (refine || at.vgpr_class <= 256)) ||
+            (at.vgpr_class <= 256 && best.vgpr_class == 512 && at.vector_lines * 100 < best.vector_lines * 145)) {
+          best = std::move(at);
+          groups = K;
+          done = true;
+          break;
+        }
+        two_wave |= at.vgpr_class <= 256;
       }
-      if (at.vgpr_class <= 256) break;  // more groups only add work
+      if (done || two_wave) break;  // more groups only add work
     }
   }
   nk.info.valu_ops = best.valu;
