@@ -657,7 +657,7 @@ std::vector<std::pair<uint32_t, bool>> and_literals(const Circuit& c, uint32_t w
 // accumulator lives from one group into the next. Returns false when the output has fewer
 // than K literals.
 bool grouped_circuit(const Circuit& c, uint32_t out_k, uint32_t K, Circuit& out, std::vector<uint32_t>& wgroup,
-                     bool refine) {
+                     uint32_t refine_cap_pct) {
   const Output o = c.outputs.at(out_k);
   const auto lits = and_literals(c, o.wire);
   if (K < 2 || lits.size() < K) return false;
@@ -720,12 +720,12 @@ bool grouped_circuit(const Circuit& c, uint32_t out_k, uint32_t K, Circuit& out,
     }
     groups.push_back(std::move(grp));
   }
-  if (refine) {
+  if (refine_cap_pct) {
     // Local search from the greedy groups: move one literal to another group while that
     // lowers the total of the groups' cone gates (the gates evaluated) and no group grows past
-    // the largest greedy group (a proxy for the registers it needs). Each move strictly
-    // lowers the total, so the search ends. On the 36-input, 20,000-gate search it takes the
-    // 5 groups from 3,966 cone gates to 3,248.
+    // refine_cap_pct percent of the largest greedy group (a proxy for the registers it needs).
+    // Each move strictly lowers the total, so the search ends. On the 36-input, 20,000-gate
+    // search it takes the 5 groups from 3,966 cone gates to 3,248 at 100 percent.
     const size_t ng = groups.size();
     std::vector<std::vector<uint32_t>> clist(lits.size());
     for (size_t i = 0; i < lits.size(); i++)
@@ -737,7 +737,7 @@ bool grouped_circuit(const Circuit& c, uint32_t out_k, uint32_t K, Circuit& out,
       for (size_t i : groups[a])
         for (uint32_t g : clist[i])
           if (cnt[a][g]++ == 0) gsize[a]++;
-    const uint32_t cap = *std::max_element(gsize.begin(), gsize.end());
+    const uint32_t cap = (uint32_t)((uint64_t)*std::max_element(gsize.begin(), gsize.end()) * refine_cap_pct / 100);
     for (int pass = 0; pass < 64; pass++) {
       bool moved = false;
       for (size_t a = 0; a < ng; a++)
@@ -917,30 +917,46 @@ NativeKernel build_native(const Circuit& c, NativeKind kind, const std::vector<u
   if (const char* e = std::getenv("NS_OUTPUT_GROUPS")) forced = std::atol(e);
   if (!table && outs.size() == 1 && forced != 1 && (forced > 1 || (best.vgpr_class == 512 && c.n >= 34))) {
     const uint32_t ks[6] = {2, 3, 4, 5, 6, 8};
-    bool done = false;
+    // For each K: the refined groups (fewer gates), then the greedy ones (smaller groups).
+    // Once refined groups reach two waves, looser caps are tried too (from 2^36
+    // assignments, where a search takes a tenth of a second or more): the first K that
+    // reaches two waves keeps its cheapest fitting program.
+    auto better = [&](const Attempt& at) {
+      return at.vgpr_class && at.vgpr_class <= 256 &&
+             (best.vgpr_class == 512 ? at.vector_lines * 100 < best.vector_lines * 145
+                                     : at.vector_lines < best.vector_lines);
+    };
     for (uint32_t K : ks) {
       if (forced > 1 && (uint32_t)forced != K) continue;
-      // the refined groups first (fewer gates), then the greedy ones (smaller groups)
-      bool two_wave = false;
-      for (int refine = 1; refine >= 0 && !done; refine--) {
+      bool fits = false;
+      for (uint32_t cap_pct : {100u, 0u}) {
         Circuit g;
         std::vector<uint32_t> wg;
-        if (!grouped_circuit(c, outs[0], K, g, wg, refine != 0)) {
-          done = true;
-          break;
-        }
+        if (!grouped_circuit(c, outs[0], K, g, wg, cap_pct)) break;
         Attempt at = attempt(g, {0}, &wg);
-        if (!at.vgpr_class) continue;
-        if ((forced > 1 && (refine || at.vgpr_class <= 256)) ||
-            (at.vgpr_class <= 256 && best.vgpr_class == 512 && at.vector_lines * 100 < best.vector_lines * 145)) {
+        if (forced > 1 && at.vgpr_class && (cap_pct || at.vgpr_class <= 256) && !fits) {
           best = std::move(at);
           groups = K;
-          done = true;
-          break;
+          fits = true;
+          continue;
         }
-        two_wave |= at.vgpr_class <= 256;
+        if (better(at)) {
+          best = std::move(at);
+          groups = K;
+          fits = true;
+        }
+        if (fits) break;
       }
-      if (done || two_wave) break;  // more groups only add work
+      if (fits && c.n >= 36 && forced <= 1) {
+        for (uint32_t cap_pct = 102; cap_pct <= 110; cap_pct += 2) {
+          Circuit g;
+          std::vector<uint32_t> wg;
+          grouped_circuit(c, outs[0], K, g, wg, cap_pct);
+          Attempt at = attempt(g, {0}, &wg);
+          if (better(at)) best = std::move(at);
+        }
+      }
+      if (fits) break;  // more groups only add work
     }
   }
   nk.info.valu_ops = best.valu;
