@@ -1004,20 +1004,25 @@ NativeKernel build_native(const Circuit& c, NativeKind kind, const std::vector<u
   // NS_OUTPUT_GROUPS=K forces K (1: never).
   long forced = -1;
   if (const char* e = std::getenv("NS_OUTPUT_GROUPS")) forced = std::atol(e);
-  if (!table && outs.size() == 1 && forced != 1 && (forced > 1 || (best.vgpr_class == 512 && c.n >= 34))) {
-    const uint32_t ks[6] = {2, 3, 4, 5, 6, 8};
+  // Groups are tried when the single-group program needs one wave per SIMD (from 2^34
+  // assignments) or fits no class at all (any size: the alternative is the compiler path).
+  const bool unfit = best.vgpr_class == 0;
+  if (!table && outs.size() == 1 && forced != 1 && (forced > 1 || unfit || (best.vgpr_class == 512 && c.n >= 34))) {
+    const uint32_t ks[9] = {2, 3, 4, 5, 6, 8, 10, 12, 16};
     // For each K: the refined groups (fewer gates), then the greedy ones (smaller groups).
     // Once refined groups reach two waves, looser caps are tried too (from 2^36
     // assignments, where a search takes a tenth of a second or more): the first K that
-    // reaches two waves keeps its cheapest fitting program.
+    // reaches two waves keeps its cheapest fitting program. A program that fits no class
+    // takes the first grouped one that does, and the search goes on for two waves.
     auto better = [&](const Attempt& at) {
-      return at.vgpr_class && at.vgpr_class <= 256 &&
-             (best.vgpr_class == 512 ? at.vector_lines * 100 < best.vector_lines * 145
-                                     : at.vector_lines < best.vector_lines);
+      if (!at.vgpr_class) return false;
+      if (!best.vgpr_class) return true;
+      return at.vgpr_class <= 256 && (best.vgpr_class == 512 ? at.vector_lines * 100 < best.vector_lines * 145
+                                                            : at.vector_lines < best.vector_lines);
     };
     for (uint32_t K : ks) {
       if (forced > 1 && (uint32_t)forced != K) continue;
-      bool fits = false;
+      bool fits = false;  // a two-wave program (or, forced, any program) at this K
       for (uint32_t cap_pct : {100u, 0u}) {
         Circuit g;
         std::vector<uint32_t> wg;
@@ -1030,9 +1035,9 @@ NativeKernel build_native(const Circuit& c, NativeKind kind, const std::vector<u
           continue;
         }
         if (better(at)) {
+          fits = at.vgpr_class <= 256;
           best = std::move(at);
           groups = K;
-          fits = true;
         }
         if (fits) break;
       }

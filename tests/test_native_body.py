@@ -188,12 +188,27 @@ def test_config1_body_with_agpr_spills(monkeypatch):
 
 
 def test_config1_body_in_output_groups():
-    # its output is the conjunction of 40 comparisons: evaluated as four groups of ten
-    # (shared gates recomputed), every group fits 256 VGPRs, two waves per SIMD
+    # its output is the conjunction of 40 comparisons: evaluated as five groups of its
+    # literals (shared gates recomputed), every group fits 256 VGPRs, two waves per SIMD; the
+    # refined groups take fewer LUTs than the 2,165 of the greedy ones
     c, _ = ns.Circuit.gen_sat(2, 36, 20000, 40)
     body, cls, size = c.native_body(0)
     assert cls == 256 and "v_accvgpr" not in body
+    assert body.count("v_bitop3_b32") < 1700
     check(c, 0, 0, [64 * 12345, (1 << 31) - 64])
+
+
+def test_groups_when_one_group_fits_no_class():
+    # 3,204 cone gates with ~800 live values fit no register class as one group (the
+    # compiler path would take them); as output groups they fit the AGPR class
+    c, _ = ns.Circuit.gen_sat(2, 36, 40000, 48)
+    with pytest.MonkeyPatch.context() as mp:
+        mp.setenv("NS_OUTPUT_GROUPS", "1")
+        with pytest.raises(ns.NaivesatError, match="does not fit"):
+            c.native_body(0)
+    body, cls, size = c.native_body(0)
+    assert cls in (256, 512)
+    check(c, 0, 0, [64 * 777])
 
 
 @pytest.mark.parametrize("K", [2, 3, 4, 7])
