@@ -603,9 +603,9 @@ std::string gen_walk_kernel(const Circuit& c, uint32_t k, GenInfo* info) {
   os << "  __shared__ unsigned long long meta[32 * 256];\n"
         "  const unsigned int tid = threadIdx.x, lane = tid & 63u;\n"
         "  const unsigned long long lim = limit < 0x7fffffffull ? limit : 0x7fffffffull;\n";
-  each("  unsigned int s@ = 0u, v@ = 0u;\n");
-  // fin: slots whose walk ended since the last check (their state frozen), with the kind of
-  // ending in fstop / fwp / floop (none of them: the limit) and, bit-sliced in a0..a3, the
+  each("  unsigned int s@ = 0u, v@ = 0u, f@ = 0u;\n");
+  // fin: slots whose walk ended since the last check (their last state in f@), with the kind
+  // of ending in fstop / fwp / floop (none of them: the limit) and, bit-sliced in a0..a3, the
   // evaluations since it ended
   os << "  unsigned int active = 0u, fin = 0u, fstop = 0u, fwp = 0u, floop = 0u, t = 0u;\n"
         "  unsigned int a0 = 0u, a1 = 0u, a2 = 0u, a3 = 0u;\n"
@@ -647,51 +647,70 @@ std::string gen_walk_kernel(const Circuit& c, uint32_t k, GenInfo* info) {
   // the loop runs while any lane of the wave has a walk, so every lane takes part in the
   // queue's prefix sums
   os << "  while (__any((active | fin) != 0u)) {\n"
-        "    if (++used > budget) {\n"
+        "    used += 2ull;\n"
+        "    if (used > budget) {\n"
         "      atomicOr(long_flag, 1u);\n"
         "      break;\n"
         "    }\n"
         "    if ((used & 255ull) == 0ull && __hip_atomic_load(long_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u)\n"
         "      break;\n";
-  for (uint32_t i = 0; i < n; i++) os << "    const unsigned int x" << i << " = s" << i << ";\n";
-  uint32_t sops = 0;
-  const uint32_t ops = emit_gates(os, c, order, outs, lane_inputs, &sops);
+  // Two steps per iteration: A from s@ to p@, B from p@ back to s@. The saved-state test runs
+  // in step B only: a walk that returns to its saved state does so again after its cycle
+  // length, and saves happen at even times, so it is caught at most one cycle later.
+  uint32_t sops = 0, ops = 0;
+  each("    unsigned int p@;\n");
+  for (int half = 0; half < 2; half++) {
+    const char* src = half == 0 ? "s" : "p";
+    const char* dst = half == 0 ? "p" : "s";
+    os << "    {\n";
+    for (uint32_t i = 0; i < n; i++) os << "      const unsigned int x" << i << " = " << src << i << ";\n";
+    ops = emit_gates(os, c, order, outs, lane_inputs, &sops);
+    os << "      const unsigned int on = " << out_expr(c, n) << ";\n";
+    for (uint32_t i = 0; i < n; i++) os << "      " << dst << i << " = " << out_expr(c, i) << ";\n";
+    os << "      const unsigned int stop = on & active;\n";
+    os << "      const unsigned int nz = 0u";
+    for (uint32_t i = 0; i < k; i++) os << " | " << dst << i;
+    os << ";\n      const unsigned int wp = ~nz & active & ~stop;\n";
+    if (half == 1) {
+      os << "      const unsigned int eq = 0xffffffffu";
+      for (uint32_t i = 0; i < n; i++) os << " & ~(" << dst << i << " ^ v" << i << ")";
+      os << ";\n      const unsigned int loop = eq & active & ~stop & ~wp;\n"
+            "      const unsigned int done = stop | wp | loop;\n"
+            "      floop |= loop;\n";
+    } else {
+      os << "      const unsigned int done = stop | wp;\n";
+    }
+    // an ending slot keeps the state its last evaluation gave in f@
+    os << "      if (__any(done != 0u)) {\n";
+    for (uint32_t i = 0; i < n; i++)
+      os << "        f" << i << " = (f" << i << " & ~done) | (" << dst << i << " & done);\n";
+    os << "      }\n"
+          "      unsigned int cy = fin;\n"
+          "      a0 ^= cy;\n"
+          "      cy &= ~a0;\n"
+          "      a1 ^= cy;\n"
+          "      cy &= ~a1;\n"
+          "      a2 ^= cy;\n"
+          "      cy &= ~a2;\n"
+          "      a3 ^= cy;\n"
+          "      fin |= done;\n"
+          "      fstop |= stop;\n"
+          "      fwp |= wp;\n"
+          "      active &= ~done;\n"
+          "      t++;\n"
+          "    }\n";
+  }
   if (info) {
     info->valu_ops = ops;
     info->scalar_ops = sops;
   }
-  for (uint32_t i = 0; i <= n; i++) os << "    const unsigned int o" << i << " = " << out_expr(c, i) << ";\n";
-  os << "    ev += (unsigned long long)__popc(active);\n"
-        "    const unsigned int stop = o"
-     << n << " & active;\n";
-  os << "    const unsigned int nz = 0u";
-  for (uint32_t i = 0; i < k; i++) os << " | o" << i;
-  os << ";\n    const unsigned int wp = ~nz & active & ~stop;\n    const unsigned int eq = 0xffffffffu";
-  for (uint32_t i = 0; i < n; i++) os << " & ~(o" << i << " ^ v" << i << ")";
-  os << ";\n    const unsigned int loop = eq & active & ~stop & ~wp;\n"
-        "    const unsigned int done = stop | wp | loop;\n";
-  // running slots advance; a slot that ends keeps the state its last evaluation gave
-  each("    s@ = (o@ & active) | (s@ & ~active);\n");
-  os << "    {\n"
-        "      unsigned int cy = fin;\n"
-        "      a0 ^= cy;\n"
-        "      cy &= ~a0;\n"
-        "      a1 ^= cy;\n"
-        "      cy &= ~a1;\n"
-        "      a2 ^= cy;\n"
-        "      cy &= ~a2;\n"
-        "      a3 ^= cy;\n"
-        "    }\n"
-        "    fin |= done;\n"
-        "    fstop |= stop;\n"
-        "    fwp |= wp;\n"
-        "    floop |= loop;\n"
-        "    active &= ~done;\n"
-        "    t++;\n"
-        // every 16 evaluations: Brent's saves (a slot saves its state at the first check
-        // after each power of two of its evaluations), the limit, the records of the ended
-        // walks and new walks in their slots
-        "    if ((t & 15u) == 0u) {\n"
+  // every 16 evaluations: the evaluations of the window (16 per slot running at its start,
+  // less the steps since each ended slot finished), Brent's saves (a slot saves its state at
+  // the first check after each power of two of its evaluations), the limit, the records of
+  // the ended walks and new walks in their slots
+  os << "    if ((t & 15u) == 0u) {\n"
+        "      ev += 16ull * (unsigned long long)__popc(active | fin) -\n"
+        "            (unsigned long long)(__popc(a0) + 2u * __popc(a1) + 4u * __popc(a2) + 8u * __popc(a3));\n"
         "      unsigned int save = 0u, lng = 0u;\n"
         "      for (unsigned int rem = active; rem; rem &= rem - 1u) {\n"
         "        const unsigned int b = __builtin_ctz(rem);\n"
@@ -716,7 +735,7 @@ std::string gen_walk_kernel(const Circuit& c, uint32_t k, GenInfo* info) {
         "        const unsigned int steps = t - age - (unsigned int)m;\n"
         "        const unsigned int kind = (fstop & mb) ? 1u : (fwp & mb) ? 0u : (floop & mb) ? 2u : 3u;\n"
         "        unsigned long long st = 0ull;\n";
-  each("        st |= (unsigned long long)((s@ >> b) & 1u) << @;\n");
+  each("        st |= (unsigned long long)((f@ >> b) & 1u) << @;\n");
   os << "        const unsigned long long target = kind == 1u ? st : kind == 0u ? (st >> " << k
      << ") : 0ull;\n"
         "        out[m >> 32] = make_ulonglong2(target | ((unsigned long long)kind << 62), steps);\n"
