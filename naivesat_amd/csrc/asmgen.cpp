@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <sstream>
+#include <thread>
 
 namespace ns {
 
@@ -651,42 +652,67 @@ std::vector<std::pair<uint32_t, bool>> and_literals(const Circuit& c, uint32_t w
 }
 
 // The local search of grouped_circuit: groups of literal indices, clist[i] = the cone gates
-// of literal i (of G gates).
+// of literal i (of G gates). One descent from the given groups, then two rounds of eight
+// perturbed descents from the best so far (seeded, run on threads of their own); the lowest
+// total is kept (the first on ties), so the result does not depend on timing.
 std::vector<std::vector<size_t>> refine_groups(std::vector<std::vector<size_t>> groups,
                                                const std::vector<std::vector<uint32_t>>& clist, uint32_t G,
                                                uint32_t cap_pct) {
   const size_t ng = groups.size();
-  std::vector<std::vector<uint16_t>> cnt(ng, std::vector<uint16_t>(G, 0));
-  std::vector<uint32_t> gsize(ng, 0);
-  auto load = [&]() {
+  uint32_t cap = 0;
+  {
+    std::vector<uint8_t> seen(G, 0);
     for (size_t a = 0; a < ng; a++) {
-      std::fill(cnt[a].begin(), cnt[a].end(), 0);
-      gsize[a] = 0;
+      std::fill(seen.begin(), seen.end(), 0);
+      uint32_t sz = 0;
       for (size_t i : groups[a])
         for (uint32_t g : clist[i])
+          if (!seen[g]++) sz++;
+      cap = std::max(cap, sz);
+    }
+    cap = (uint32_t)((uint64_t)cap * cap_pct / 100);
+  }
+  // descent from g after `perturb` seeded random moves; returns the total cone gates
+  auto descend = [&](std::vector<std::vector<size_t>>& gr, uint64_t seed, int perturb) -> uint64_t {
+    std::vector<std::vector<uint16_t>> cnt(ng, std::vector<uint16_t>(G, 0));
+    std::vector<uint32_t> gsize(ng, 0);
+    for (size_t a = 0; a < ng; a++)
+      for (size_t i : gr[a])
+        for (uint32_t g : clist[i])
           if (cnt[a][g]++ == 0) gsize[a]++;
+    auto move = [&](size_t i, size_t a, size_t b) {
+      for (uint32_t g : clist[i]) {
+        if (--cnt[a][g] == 0) gsize[a]--;
+        if (cnt[b][g]++ == 0) gsize[b]++;
+      }
+      auto& ga = gr[a];
+      ga.erase(std::find(ga.begin(), ga.end(), i));
+      gr[b].push_back(i);
+    };
+    uint64_t rng = seed * 0x9e3779b97f4a7c15ull + 0x632be59bd9b4e019ull;
+    auto next = [&]() {
+      rng ^= rng << 13;
+      rng ^= rng >> 7;
+      rng ^= rng << 17;
+      return rng;
+    };
+    for (int k = 0; k < perturb; k++) {
+      const size_t a = next() % ng, b = next() % ng;
+      if (a == b || gr[a].size() < 2) continue;
+      const size_t i = gr[a][next() % gr[a].size()];
+      int64_t added = 0;
+      for (uint32_t g : clist[i]) added += cnt[b][g] == 0;
+      if (gsize[b] + added > cap) continue;
+      move(i, a, b);
     }
-  };
-  load();
-  const uint32_t cap = (uint32_t)((uint64_t)*std::max_element(gsize.begin(), gsize.end()) * cap_pct / 100);
-  auto move = [&](size_t i, size_t a, size_t b) {
-    for (uint32_t g : clist[i]) {
-      if (--cnt[a][g] == 0) gsize[a]--;
-      if (cnt[b][g]++ == 0) gsize[b]++;
-    }
-    auto& ga = groups[a];
-    ga.erase(std::find(ga.begin(), ga.end(), i));
-    groups[b].push_back(i);
-  };
-  std::vector<uint8_t> mark(G, 0);
-  auto search = [&]() {
+    std::vector<uint8_t> mark(G, 0);
     for (int pass = 0; pass < 256; pass++) {
       bool moved = false;
       // single moves, the best target for each literal
       for (size_t a = 0; a < ng; a++)
-        for (size_t pos = 0; pos < groups[a].size();) {
-          const size_t i = groups[a][pos];
-          if (groups[a].size() == 1) break;
+        for (size_t pos = 0; pos < gr[a].size();) {
+          const size_t i = gr[a][pos];
+          if (gr[a].size() == 1) break;
           int64_t freed = 0;
           for (uint32_t g : clist[i]) freed += cnt[a][g] == 1;
           size_t to = SIZE_MAX;
@@ -710,21 +736,20 @@ std::vector<std::vector<size_t>> refine_groups(std::vector<std::vector<size_t>> 
       // swaps: i in a and j in b trade groups (the first that lowers the total)
       for (size_t a = 0; a < ng && !moved; a++)
         for (size_t b = a + 1; b < ng && !moved; b++)
-          for (size_t pi = 0; pi < groups[a].size() && !moved; pi++) {
-            const size_t i = groups[a][pi];
+          for (size_t pi = 0; pi < gr[a].size() && !moved; pi++) {
+            const size_t i = gr[a][pi];
             for (uint32_t g : clist[i]) mark[g] = 1;
-            int64_t freed_a = 0, added_b = 0;
+            int64_t freed_a = 0;
             for (uint32_t g : clist[i]) freed_a += cnt[a][g] == 1;
-            for (size_t pj = 0; pj < groups[b].size(); pj++) {
-              const size_t j = groups[b][pj];
+            for (size_t pj = 0; pj < gr[b].size(); pj++) {
+              const size_t j = gr[b][pj];
               // a loses i and gains j; b loses j and gains i (mark bit 0: cone i, bit 1: cone j)
-              int64_t added_a = 0, freed_b = 0;
+              int64_t added_a = 0, freed_b = 0, added_b = 0;
               for (uint32_t g : clist[j]) {
                 added_a += cnt[a][g] - (mark[g] & 1) == 0;
                 freed_b += cnt[b][g] == 1;
                 mark[g] |= 2;
               }
-              added_b = 0;
               for (uint32_t g : clist[i]) added_b += cnt[b][g] - ((mark[g] >> 1) & 1) == 0;
               for (uint32_t g : clist[j]) mark[g] &= 1;
               const int64_t na = (int64_t)gsize[a] - freed_a + added_a, nb = (int64_t)gsize[b] - freed_b + added_b;
@@ -740,41 +765,26 @@ std::vector<std::vector<size_t>> refine_groups(std::vector<std::vector<size_t>> 
           }
       if (!moved) break;
     }
-  };
-  search();
-  auto total = [&]() {
     uint64_t t = 0;
     for (uint32_t v : gsize) t += v;
     return t;
   };
-  std::vector<std::vector<size_t>> best = groups;
-  uint64_t best_total = total();
-  uint64_t rng = 0x9e3779b97f4a7c15ull;
-  auto next = [&]() {
-    rng ^= rng << 13;
-    rng ^= rng >> 7;
-    rng ^= rng << 17;
-    return rng;
-  };
-  for (int restart = 0; restart < 8; restart++) {
-    groups = best;
-    load();
-    for (int k = 0; k < 6; k++) {
-      const size_t a = next() % ng, b = next() % ng;
-      if (a == b || groups[a].size() < 2) continue;
-      const size_t i = groups[a][next() % groups[a].size()];
-      int64_t added = 0;
-      for (uint32_t g : clist[i]) added += cnt[b][g] == 0;
-      if (gsize[b] + added > cap) continue;
-      move(i, a, b);
-    }
-    search();
-    if (total() < best_total) {
-      best_total = total();
-      best = groups;
-    }
+  uint64_t best_total = descend(groups, 0, 0);
+  constexpr int kRounds = 2, kRestarts = 8;
+  for (int round = 0; round < kRounds; round++) {
+    std::vector<std::vector<std::vector<size_t>>> tries(kRestarts, groups);
+    std::vector<uint64_t> totals(kRestarts, 0);
+    std::vector<std::thread> th;
+    for (int r = 0; r < kRestarts; r++)
+      th.emplace_back([&, r, round]() { totals[r] = descend(tries[r], (uint64_t)(round * kRestarts + r + 1), 6); });
+    for (auto& t : th) t.join();
+    for (int r = 0; r < kRestarts; r++)
+      if (totals[r] < best_total) {
+        best_total = totals[r];
+        groups = tries[r];
+      }
   }
-  return best;
+  return groups;
 }
 
 // Output k of c rebuilt as K groups of its conjunction's literals (grouped by shared cone
@@ -1042,13 +1052,19 @@ NativeKernel build_native(const Circuit& c, NativeKind kind, const std::vector<u
         if (fits) break;
       }
       if (fits && c.n >= 36 && forced <= 1) {
-        for (uint32_t cap_pct = 102; cap_pct <= 110; cap_pct += 2) {
-          Circuit g;
-          std::vector<uint32_t> wg;
-          grouped_circuit(c, outs[0], K, g, wg, cap_pct);
-          Attempt at = attempt(g, {0}, &wg);
+        // the looser caps on threads of their own, compared in cap order
+        std::vector<Attempt> ats(5);
+        std::vector<std::thread> th;
+        for (uint32_t q = 0; q < 5; q++)
+          th.emplace_back([&, q]() {
+            Circuit g;
+            std::vector<uint32_t> wg;
+            grouped_circuit(c, outs[0], K, g, wg, 102 + 2 * q);
+            ats[q] = attempt(g, {0}, &wg);
+          });
+        for (auto& t : th) t.join();
+        for (Attempt& at : ats)
           if (better(at)) best = std::move(at);
-        }
       }
       if (fits) break;  // more groups only add work
     }
