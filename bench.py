@@ -204,9 +204,18 @@ def qsat_part(ns, ctx):
     q = [0] * 8 + [1] * 8 + [0] * 8 + [1] * 8
     ctx.qsat_solve(cq, q)
     rq = ctx.qsat_solve(cq, q)
+    # one op per cone gate per 32-assignment word, over the whole search (the circuit kernel
+    # and the reduction passes after it)
+    words = (1 << 32) // 32
+    tops = words * rq["cone_gates"] / (rq["kernel_ms"] / 1000.0) / 1e12
     return {"workload": "QSAT n=32, 10000 gates (seed 3), E8 A8 E8 A8", "kernel_ms": rq["kernel_ms"],
             "truth": rq["truth"], "result": ns.qsat_result_string(rq), "cone_gates": rq["cone_gates"],
-            "gate_evals_per_s": (1 << 32) * rq["cone_gates"] / (rq["kernel_ms"] / 1000.0)}
+            "gate_evals_per_s": (1 << 32) * rq["cone_gates"] / (rq["kernel_ms"] / 1000.0),
+            "roofline": {"bound": "valu", "achieved": tops, "peak": VALU_PEAK_OPS / 1e12, "unit": "TOP/s",
+                         "frac": tops * 1e12 / VALU_PEAK_OPS, "kernel": "ns_nat_qsat* + k_qsat_reduce",
+                         "luts_per_word": rq["valu_ops"], "scalar_ops_per_word": rq["scalar_ops"],
+                         "vgpr_class": rq["vgpr_class"],
+                         "lut_ops_per_s": words * rq["valu_ops"] / (rq["kernel_ms"] / 1000.0)}}
 
 
 def serve_part(ns, n_requests, workers):
