@@ -184,3 +184,15 @@ def test_output_groups_keep_answers(ctx, monkeypatch, K):
     o = oracle.OracleCircuit.from_dict(c.arrays()).qsat(q)
     r = ctx.qsat_solve(c, q)
     assert (r["truth"], r["witness_count"]) == (o["truth"], o["witness_count"])
+
+
+def test_groups_for_a_circuit_no_class_holds(ctx):
+    # 40,000 gates with a 48-gate word: ~800 values live at once fit no register class as one
+    # group, so the native body takes output groups (refined) instead of the compiler path
+    c, key = ns.Circuit.gen_sat(2, 26, 40000, 48)
+    body, cls, _ = c.native_body(0)
+    assert cls in (256, 512)
+    r = ctx.sat_solve(c)
+    assert r["backend"] == 1 and r["vgpr_class"] == cls
+    assert (r["count"], r["lowest"]) == oracle.OracleCircuit.from_dict(c.arrays()).sat(threads=8)
+    assert r["lowest"] <= key
