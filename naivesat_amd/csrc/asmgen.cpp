@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <cstdlib>
+#include <exception>
 #include <sstream>
 #include <thread>
 
@@ -775,9 +776,18 @@ std::vector<std::vector<size_t>> refine_groups(std::vector<std::vector<size_t>> 
     std::vector<std::vector<std::vector<size_t>>> tries(kRestarts, groups);
     std::vector<uint64_t> totals(kRestarts, 0);
     std::vector<std::thread> th;
+    std::vector<std::exception_ptr> err(kRestarts);
     for (int r = 0; r < kRestarts; r++)
-      th.emplace_back([&, r, round]() { totals[r] = descend(tries[r], (uint64_t)(round * kRestarts + r + 1), 6); });
+      th.emplace_back([&, r, round]() {
+        try {
+          totals[r] = descend(tries[r], (uint64_t)(round * kRestarts + r + 1), 6);
+        } catch (...) {
+          err[r] = std::current_exception();
+        }
+      });
     for (auto& t : th) t.join();
+    for (auto& e : err)
+      if (e) std::rethrow_exception(e);
     for (int r = 0; r < kRestarts; r++)
       if (totals[r] < best_total) {
         best_total = totals[r];
@@ -1055,14 +1065,21 @@ NativeKernel build_native(const Circuit& c, NativeKind kind, const std::vector<u
         // the looser caps on threads of their own, compared in cap order
         std::vector<Attempt> ats(5);
         std::vector<std::thread> th;
+        std::vector<std::exception_ptr> err(5);
         for (uint32_t q = 0; q < 5; q++)
           th.emplace_back([&, q]() {
-            Circuit g;
-            std::vector<uint32_t> wg;
-            grouped_circuit(c, outs[0], K, g, wg, 102 + 2 * q);
-            ats[q] = attempt(g, {0}, &wg);
+            try {
+              Circuit g;
+              std::vector<uint32_t> wg;
+              grouped_circuit(c, outs[0], K, g, wg, 102 + 2 * q);
+              ats[q] = attempt(g, {0}, &wg);
+            } catch (...) {
+              err[q] = std::current_exception();
+            }
           });
         for (auto& t : th) t.join();
+        for (auto& e : err)
+          if (e) std::rethrow_exception(e);
         for (Attempt& at : ats)
           if (better(at)) best = std::move(at);
       }
