@@ -798,13 +798,14 @@ std::vector<std::vector<size_t>> refine_groups(std::vector<std::vector<size_t>> 
 }
 
 // Output k of c rebuilt as K groups of its conjunction's literals (grouped by shared cone
-// gates): each group's cone is copied (gates shared by groups are recomputed), its literals
-// are ANDed, and each group's value is ANDed into an accumulator right after it. wgroup gets
-// the group (1..K) of every new wire, so the schedule can keep groups apart and only the
-// accumulator lives from one group into the next. Returns false when the output has fewer
-// than K literals.
+// gates; refined by refine_groups when refine_cap_pct is non-zero): each group's cone is
+// copied (gates shared by groups are recomputed, except `pin` kept values that later groups
+// reuse), its literals are ANDed, and each group's value is ANDed into an accumulator right
+// after it. wgroup gets the group (1..K) of every new wire, so the schedule can keep groups
+// apart and only the accumulator (and kept values) live from one group into the next.
+// Returns false when the output has fewer than K literals.
 bool grouped_circuit(const Circuit& c, uint32_t out_k, uint32_t K, Circuit& out, std::vector<uint32_t>& wgroup,
-                     uint32_t refine_cap_pct) {
+                     uint32_t refine_cap_pct, uint32_t pin = 0) {
   const Output o = c.outputs.at(out_k);
   const auto lits = and_literals(c, o.wire);
   if (K < 2 || lits.size() < K) return false;
@@ -895,17 +896,63 @@ bool grouped_circuit(const Circuit& c, uint32_t out_k, uint32_t K, Circuit& out,
     if (!y.second) return add(y.first, x.first, FUNC_NIMPL, grp);
     return add(x.first, y.first, FUNC_NOR, grp);
   };
+  // Kept values: the `pin` gates that the most groups share, weighted by their own cone, are
+  // computed once (by the first group that needs them) and stay live into the later groups,
+  // whose cones stop there instead of recomputing them.
+  std::vector<uint8_t> pinned(G, 0), have(G, 0);
+  if (pin) {
+    std::vector<uint32_t> use(G, 0);
+    for (size_t gi = 0; gi < groups.size(); gi++) {
+      std::vector<uint8_t> u(G, 0);
+      for (size_t i : groups[gi])
+        for (uint32_t g = 0; g < G; g++) u[g] |= cone[i][g];
+      for (uint32_t g = 0; g < G; g++) use[g] += u[g];
+    }
+    std::vector<std::pair<uint64_t, uint32_t>> cand;
+    std::vector<uint32_t> mark(G, UINT32_MAX);
+    for (uint32_t g = 0; g < G; g++) {
+      if (use[g] < 2) continue;
+      uint64_t sz = 0;
+      std::vector<uint32_t> st{c.n + g};
+      while (!st.empty()) {
+        const uint32_t w = st.back();
+        st.pop_back();
+        if (w < c.n || mark[w - c.n] == g) continue;
+        mark[w - c.n] = g;
+        sz++;
+        st.push_back(c.gates[w - c.n].i0);
+        st.push_back(c.gates[w - c.n].i1);
+      }
+      cand.push_back({(uint64_t)(use[g] - 1) * sz, g});
+    }
+    std::sort(cand.begin(), cand.end(), [](const auto& a, const auto& b) { return a.first > b.first || (a.first == b.first && a.second < b.second); });
+    for (size_t q = 0; q < cand.size() && q < pin; q++) pinned[cand[q].second] = 1;
+  }
   std::pair<uint32_t, bool> acc{0, false};
   bool have_acc = false;
   std::vector<uint32_t> remap(c.wires());
   for (uint32_t k = 0; k < c.n; k++) remap[k] = k;
   for (size_t gi = 0; gi < groups.size(); gi++) {
     const uint32_t grp = (uint32_t)gi + 1;
+    // the group's gates: its literals' cones, stopping at kept values already computed
     std::vector<uint8_t> u(G, 0);
-    for (size_t i : groups[gi])
-      for (uint32_t g = 0; g < G; g++) u[g] |= cone[i][g];
+    std::vector<uint32_t> st;
+    for (size_t i : groups[gi]) st.push_back(lits[i].first);
+    while (!st.empty()) {
+      const uint32_t w = st.back();
+      st.pop_back();
+      if (w < c.n) continue;
+      const uint32_t g = w - c.n;
+      if (u[g] || (pinned[g] && have[g])) continue;
+      u[g] = 1;
+      st.push_back(c.gates[g].i0);
+      st.push_back(c.gates[g].i1);
+    }
     for (uint32_t g = 0; g < G; g++)
-      if (u[g]) remap[c.n + g] = add(remap[c.gates[g].i0], remap[c.gates[g].i1], c.gates[g].func, grp);
+      if (u[g]) {
+        remap[c.n + g] = add(remap[c.gates[g].i0], remap[c.gates[g].i1], c.gates[g].func, grp);
+        if (pinned[g]) have[g] = 1;
+      }
     std::pair<uint32_t, bool> v{remap[lits[groups[gi][0]].first], lits[groups[gi][0]].second};
     for (size_t q = 1; q < groups[gi].size(); q++)
       v = {conj(v, {remap[lits[groups[gi][q]].first], lits[groups[gi][q]].second}, grp), false};
@@ -1062,16 +1109,24 @@ NativeKernel build_native(const Circuit& c, NativeKind kind, const std::vector<u
         if (fits) break;
       }
       if (fits && c.n >= 36 && forced <= 1) {
-        // the looser caps on threads of their own, compared in cap order
-        std::vector<Attempt> ats(5);
+        // looser caps, each with and without 16 kept values (pin), on threads of their own,
+        // compared in list order; on the 36-input search the best takes 1,601 LUTs per word
+        // against 1,629 without kept values
+        const uint32_t caps[6] = {102, 103, 104, 106, 108, 110};
+        std::vector<std::pair<uint32_t, uint32_t>> tries{{100, 16}};
+        for (uint32_t cp : caps) {
+          tries.push_back({cp, 0});
+          tries.push_back({cp, 16});
+        }
+        std::vector<Attempt> ats(tries.size());
+        std::vector<std::exception_ptr> err(tries.size());
         std::vector<std::thread> th;
-        std::vector<std::exception_ptr> err(5);
-        for (uint32_t q = 0; q < 5; q++)
+        for (size_t q = 0; q < tries.size(); q++)
           th.emplace_back([&, q]() {
             try {
               Circuit g;
               std::vector<uint32_t> wg;
-              grouped_circuit(c, outs[0], K, g, wg, 102 + 2 * q);
+              grouped_circuit(c, outs[0], K, g, wg, tries[q].first, tries[q].second);
               ats[q] = attempt(g, {0}, &wg);
             } catch (...) {
               err[q] = std::current_exception();
