@@ -135,13 +135,22 @@ def table_part(ns, ctx):
     reads_per_s = rs["walk_evaluations"] / walk_s
     walk_gbs = rs["walk_evaluations"] * 4 / walk_s / 1e9
     traffic = measured_traffic()
+    walk_traffic = traffic.get(("k_wp_walk", "satostop_n30"))
+    pass_traffic = traffic.get(("k_jump_pass", "satostop_n30"))
+
+    def traffic_rate(t, seconds):
+        # measured HBM bytes of one launch (rocprofv3 counters, profiles/traffic.txt) over its time
+        return None if t is None else t / seconds / 1e9
+    walk_tr = traffic_rate(walk_traffic, walk_s)
+    pass_tr = traffic_rate(pass_traffic, avg / 1000.0)
     return {"workload": TABLE_WORKLOAD, "method": "waypoint walks", "waypoint_bits": rs["waypoint_bits"],
             "build_ms": rs["build_ms"], "walk_ms": rs["walk_ms"], "walk_evaluations": rs["walk_evaluations"],
             "waypoint_passes": rs["waypoint_passes"], "waypoint_jump_ms": rs["jump_ms"],
             "device_ms": rs["build_ms"] + rs["walk_ms"] + rs["jump_ms"], "wall_ms": rs["wall_ms"],
             "n_stopped": rs["n_stopped"], "lowest_u": rs["lowest_u"], "steps": rs["steps"], "end": rs["end"],
             "roofline": {"bound": "hbm", "achieved": walk_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": walk_gbs / HBM_PEAK_GBS, "traffic": traffic.get(("k_wp_walk", "satostop_n30")),
+                         "frac": walk_gbs / HBM_PEAK_GBS, "traffic": walk_traffic,
+                         "traffic_gbs": walk_tr, "traffic_frac": None if walk_tr is None else walk_tr / HBM_PEAK_GBS,
                          "kernel": "k_wp_walk", "bytes_per_evaluation": 4,
                          "scattered_reads_per_s": reads_per_s,
                          "scattered_read_ceiling_per_s": probe["reads_per_s"],
@@ -150,8 +159,9 @@ def table_part(ns, ctx):
                               "passes": rp["passes"], "walk_jumps": rp["walk_jumps"], "walk_ms": rp["walk_ms"],
                               "avg_pass_ms": avg,
                               "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                                           "frac": gbs / HBM_PEAK_GBS,
-                                           "traffic": traffic.get(("k_jump_pass", "satostop_n30")),
+                                           "frac": gbs / HBM_PEAK_GBS, "traffic": pass_traffic,
+                                           "traffic_gbs": pass_tr,
+                                           "traffic_frac": None if pass_tr is None else pass_tr / HBM_PEAK_GBS,
                                            "kernel": "k_jump_pass", "bytes_per_entry": 24}},
             "speedup_vs_passes": (rp["build_ms"] + rp["jump_ms"] + rp["walk_ms"]) /
                                  (rs["build_ms"] + rs["walk_ms"] + rs["jump_ms"]),
