@@ -810,6 +810,10 @@ bool grouped_circuit(const Circuit& c, uint32_t out_k, uint32_t K, Circuit& out,
   const auto lits = and_literals(c, o.wire);
   if (K < 2 || lits.size() < K) return false;
   const uint32_t G = (uint32_t)c.gates.size();
+  // the greedy grouping takes ~L^2 G steps and the cone marks L G bytes (L literals, G gates):
+  // outputs past these bounds are not grouped
+  const uint64_t L = lits.size();
+  if (L * G > (1ull << 27) || L * L * G > (1ull << 32)) return false;
   // cone of each literal (gate marks)
   std::vector<std::vector<uint8_t>> cone(lits.size());
   for (size_t i = 0; i < lits.size(); i++) {
