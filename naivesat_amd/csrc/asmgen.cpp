@@ -19,9 +19,9 @@ namespace ns {
 namespace {
 
 const uint32_t kWordMasks[5] = {0xAAAAAAAAu, 0xCCCCCCCCu, 0xF0F0F0F0u, 0xFF00FF00u, 0xFFFF0000u};
-// SGPRs of the body: s[40..95]; the SAT templates, whose own code keeps to s0..s13, also
+// SGPRs of the body: s[40..101]; the SAT templates, whose own code keeps to s0..s13, also
 // give it s[14..19] and s[22..39] (s20 and s21 carry the word index)
-const uint32_t kSgprLoSat = 14, kSgprLo = 40, kSgprHi = 95;
+const uint32_t kSgprLoSat = 14, kSgprLo = 40, kSgprHi = 101;
 
 struct Loc {
   int v = -1;   // VGPR holding the value

@@ -9,7 +9,7 @@ Register contract of every placeholder ("; NS_BODY <name>"):
            s20, s21      wave-uniform part of the word index (lo, hi): every index bit except
                          the six lane bits
   outputs  v0 .. v[OUT-1] circuit outputs for the current word
-  scratch  v[POOL_LO..POOL_HI], s[40..95], scc (clobbered); the SAT kernels, whose own code
+  scratch  v[POOL_LO..POOL_HI], s[40..101], scc (clobbered); the SAT kernels, whose own code
            keeps to s0..s13, also give the body s[14..19] and s[22..39]
 Register classes bound the VGPR budget (and with it the occupancy).
 
@@ -23,8 +23,8 @@ CLASSES = {  # name: (total VGPR budget, pool high register); r512 adds a0..a255
     "r256": (256, 239),
     "r512": (512, 239),
 }
-SGPR_POOL = list(range(40, 96))
-SAT_SGPR_POOL = list(range(14, 20)) + list(range(22, 96))  # s20, s21 carry the word index
+SGPR_POOL = list(range(40, 102))
+SAT_SGPR_POOL = list(range(14, 20)) + list(range(22, 102))  # s20, s21 carry the word index
 
 
 def clobbers(lo, hi, agprs=False, sgprs=SGPR_POOL):
