@@ -293,20 +293,23 @@ int ns_table_build(ns_ctx* ctx, const ns_circuit* c, uint32_t* table_out, uint64
                    double* kernel_ms);
 
 /* ---- Serving (no counterpart in the reference, which solves one circuit per process):
- *      a stream of requests solved by `workers` lanes (a HIP stream and buffers each) that
- *      run concurrently on the device. A free worker takes the next request that has
- *      arrived, so requests that arrive while others run are admitted as soon as a lane and
- *      device memory are free. With realtime = 1 arrivals are honoured (a request waits
- *      until arrival_s after the start); otherwise all requests are available at once. ---- */
+ *      requests taken as they arrive and solved by `workers` lanes (a host thread, a HIP
+ *      stream and buffers each) that run concurrently on the device. A free lane takes the
+ *      oldest admissible request: SATOSTOP requests are admitted when their tables fit in
+ *      the device arena (a request waiting for memory holds back only the memory-needing
+ *      requests behind it); with batching on, a lane takes every waiting SAT / QSAT request
+ *      (up to batch_max) and solves them together in one persistent kernel whose code
+ *      object holds all their circuit bodies. Every answer is the one the request gets
+ *      alone. ---- */
 #define NS_REQ_SAT 0u
 #define NS_REQ_QSAT 1u
 #define NS_REQ_SATOSTOP 2u
 typedef struct {
   uint32_t kind;
-  uint32_t unknowns;          /* SATOSTOP: unknown state bits */
+  uint32_t unknowns;          /* SATOSTOP: unknown state bits (at most 31 state bits served) */
   const ns_circuit* circuit;
   const uint8_t* quants_all;  /* QSAT: one quantifier per input */
-  double arrival_s;           /* arrival time after the start of the run */
+  double arrival_s;           /* ns_serve: arrival time after the start of the run */
 } ns_request;
 typedef struct {
   int32_t status;             /* NS_OK or the error of this request */
@@ -314,23 +317,55 @@ typedef struct {
   /* SAT: count, lowest, found; QSAT: truth, witness, witness_count, has_witness;
    * SATOSTOP: n_stopped, lowest_u, steps, end */
   uint64_t v0, v1, v2, v3;
-  double start_s, finish_s;   /* service start and completion after the start of the run */
+  double start_s, finish_s;   /* service start and completion after the start of the server */
   double compile_ms, kernel_ms;
+  double submit_s;            /* arrival (submission) after the start of the server */
 } ns_request_result;
 typedef struct {
-  uint32_t workers;           /* concurrent lanes (0 = 8) */
-  uint32_t realtime;
-  uint64_t hbm_budget_bytes;  /* device arena for SATOSTOP tables (0 = workers x the largest
-                                 request's tables, at most 16 GiB, at least the largest) */
+  uint32_t workers;           /* concurrent lanes (0 = 8, at most 64) */
+  uint32_t realtime;          /* ns_serve: submit each request at its arrival_s */
+  uint64_t hbm_budget_bytes;  /* device arena for SATOSTOP tables; ns_serve: 0 = workers x the
+                                 largest request's tables, at most 16 GiB; ns_serve_open: 0 =
+                                 4 GiB */
+  uint32_t batch_max;         /* > 1: batch SAT / QSAT requests, at most this many per kernel */
+  uint32_t reserved;
 } ns_serve_opts;
+/* Replays a trace: requests submitted at their arrival times (realtime) or all at once,
+ * every result waited for; *total_s = time to the last completion. */
 int ns_serve(ns_ctx* ctx, const ns_request* reqs, uint32_t n, const ns_serve_opts* opts,
              ns_request_result* results, double* total_s);
+
+/* Online serving: open a server on a context, submit requests as they arrive (each gets a
+ * ticket; its arrival is the submit time), poll or wait for results, close. The server
+ * copies each request's circuit and quantifiers, so the caller may free them after
+ * ns_serve_submit returns. ns_serve_poll / ns_serve_wait set *done = 1 and fill *res once
+ * the request has finished; a finished ticket's result is handed out once, after which the
+ * ticket is unknown (NS_ERR_INVALID). ns_serve_wait with timeout_s < 0 waits without limit.
+ * ns_serve_close waits for every submitted request, stops the lanes and frees the server.
+ * A server may be used from several threads; its context's own solves may run meanwhile
+ * from another thread. */
+typedef struct ns_server ns_server;
+int ns_serve_open(ns_ctx* ctx, const ns_serve_opts* opts, ns_server** out);
+int ns_serve_submit(ns_server* srv, const ns_request* req, uint64_t* ticket);
+int ns_serve_poll(ns_server* srv, uint64_t ticket, ns_request_result* res, int* done);
+int ns_serve_wait(ns_server* srv, uint64_t ticket, double timeout_s, ns_request_result* res, int* done);
+int ns_serve_close(ns_server* srv);
+
+/* Many SAT / QSAT circuits solved together by the persistent batch kernels on the
+ * context's lane (no counterpart in the reference: its mapper runs one circuit per
+ * process). SAT results as ns_sat_solve's count / lowest / found, QSAT as ns_qsat_solve's
+ * truth / witness / witness_count / has_witness, in v0..v3 of results[i]; requests of
+ * kind SATOSTOP, and circuits the batch kernels cannot take, are solved one at a time.
+ * info (optional, 8 doubles): requests, fallbacks, register class, code bytes, code
+ * generation ms, module load ms, kernel ms, wall ms. */
+int ns_batch_solve(ns_ctx* ctx, const ns_request* reqs, uint32_t n, ns_request_result* results, double* info);
 
 /* Diagnostic (no counterpart in the reference; no device needed): builds the native-path
  * kernel for a circuit, assembles and links it, and returns the circuit body's assembly.
  * kind: 0 SAT (output 0), 1 QSAT one-bit-per-thread, 2 QSAT outer-block-in-thread,
- * 3 next-state table (all outputs). Input k sits at index bit k (SAT, table) or n-1-k
- * (QSAT); lane_shift places the six lane bits at index bits 5+lane_shift..10+lane_shift.
+ * 3 next-state table (all outputs), 4 batch body with the SAT input order, 5 batch body with
+ * the QSAT input order. Input k sits at index bit k (SAT, table, 4) or n-1-k (QSAT, 5);
+ * lane_shift places the six lane bits at index bits 5+lane_shift..10+lane_shift.
  * Returns NS_ERR_UNSUPPORTED when the circuit exceeds the native register budget. */
 int ns_native_body(const ns_circuit* c, uint32_t kind, uint32_t lane_shift, char* body, size_t cap,
                    size_t* needed, uint32_t* vgpr_class, uint64_t* code_bytes);
@@ -344,7 +379,8 @@ int ns_native_encode(const char* body, size_t len, uint32_t* words, size_t cap, 
  * engine loads for a circuit's native kernel (kind and lane_shift as for ns_native_body):
  * the template's cached code object with the encoded body copied into its body slot at
  * byte *slot_offset (*patched = 1), or the fully assembled module (*patched = 0) when the
- * body does not fit the largest slot. */
+ * body does not fit the largest slot. Kinds 4 and 5: the batch code object of the body's
+ * register class with this one body in its region, *slot_offset = the body's first byte. */
 int ns_native_code(const ns_circuit* c, uint32_t kind, uint32_t lane_shift, void* code, size_t cap,
                    size_t* needed, uint32_t* patched, uint64_t* slot_offset);
 

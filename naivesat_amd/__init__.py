@@ -103,11 +103,12 @@ class _Request(C.Structure):
 class _RequestResult(C.Structure):
     _fields_ = [("status", C.c_int32), ("worker", C.c_uint32), ("v0", C.c_uint64), ("v1", C.c_uint64),
                 ("v2", C.c_uint64), ("v3", C.c_uint64), ("start_s", C.c_double), ("finish_s", C.c_double),
-                ("compile_ms", C.c_double), ("kernel_ms", C.c_double)]
+                ("compile_ms", C.c_double), ("kernel_ms", C.c_double), ("submit_s", C.c_double)]
 
 
 class _ServeOpts(C.Structure):
-    _fields_ = [("workers", C.c_uint32), ("realtime", C.c_uint32), ("hbm_budget_bytes", C.c_uint64)]
+    _fields_ = [("workers", C.c_uint32), ("realtime", C.c_uint32), ("hbm_budget_bytes", C.c_uint64),
+                ("batch_max", C.c_uint32), ("reserved", C.c_uint32)]
 
 
 REQ_SAT, REQ_QSAT, REQ_SATOSTOP = 0, 1, 2
@@ -153,6 +154,12 @@ def load_library() -> C.CDLL:
                                C.POINTER(U64)]),
         "ns_serve": (I, [P, C.POINTER(_Request), U32, C.POINTER(_ServeOpts), C.POINTER(_RequestResult),
                          C.POINTER(C.c_double)]),
+        "ns_serve_open": (I, [P, C.POINTER(_ServeOpts), C.POINTER(P)]),
+        "ns_serve_submit": (I, [P, C.POINTER(_Request), C.POINTER(U64)]),
+        "ns_serve_poll": (I, [P, U64, C.POINTER(_RequestResult), C.POINTER(I)]),
+        "ns_serve_wait": (I, [P, U64, C.c_double, C.POINTER(_RequestResult), C.POINTER(I)]),
+        "ns_serve_close": (I, [P]),
+        "ns_batch_solve": (I, [P, C.POINTER(_Request), U32, C.POINTER(_RequestResult), P]),
         "ns_native_encode": (I, [C.c_char_p, C.c_size_t, P, C.c_size_t, C.POINTER(C.c_size_t)]),
         "ns_native_code": (I, [P, U32, U32, P, C.c_size_t, C.POINTER(C.c_size_t), C.POINTER(U32),
                                C.POINTER(U64)]),
@@ -184,7 +191,8 @@ def exported_symbols() -> list:
             "ns_native_code", "ns_satostop_solve_wide", "ns_satostop_hashmap_solve", "ns_hashmap_step",
             "ns_probe_gather", "ns_pack_entries", "ns_ctx_trim", "ns_greedy_partitioned_solve",
             "ns_file_image_create", "ns_file_image_save_chunk", "ns_file_image_load_partition",
-            "ns_file_image_save_partition", "ns_file_image_free"]
+            "ns_file_image_save_partition", "ns_file_image_free", "ns_serve_open", "ns_serve_submit",
+            "ns_serve_poll", "ns_serve_wait", "ns_serve_close", "ns_batch_solve"]
 
 
 def _check(status: int) -> None:
@@ -458,25 +466,35 @@ class Context:
         _check(load_library().ns_table_build(self._h, c._h, _ptr(t), t.size, C.byref(ms)))
         return t
 
-    def serve(self, requests, workers: int = 8, realtime: bool = True, hbm_budget_bytes: int = 0):
+    def serve(self, requests, workers: int = 8, realtime: bool = True, hbm_budget_bytes: int = 0,
+              batch_max: int = 0):
         """Solves a stream of requests on concurrent lanes. Each request is a dict with
         kind (REQ_SAT / REQ_QSAT / REQ_SATOSTOP), circuit, quants (QSAT), unknowns
-        (SATOSTOP) and arrival_s. Returns (list of result dicts, total seconds)."""
+        (SATOSTOP) and arrival_s. batch_max > 1 batches waiting SAT / QSAT requests into
+        persistent kernels. Returns (list of result dicts, total seconds)."""
         n = len(requests)
-        arr = (_Request * max(1, n))()
-        keep = []
-        for i, rq in enumerate(requests):
-            q = None
-            if rq["kind"] == REQ_QSAT:
-                q = np.ascontiguousarray(rq["quants"], dtype=np.uint8)
-                keep.append(q)
-            arr[i] = _Request(rq["kind"], rq.get("unknowns", 0), rq["circuit"]._h, _ptr(q), rq.get("arrival_s", 0.0))
+        arr, keep = _requests(requests)
         res = (_RequestResult * max(1, n))()
         total = C.c_double()
-        o = _ServeOpts(workers, 1 if realtime else 0, hbm_budget_bytes)
+        o = _ServeOpts(workers, 1 if realtime else 0, hbm_budget_bytes, batch_max, 0)
         _check(load_library().ns_serve(self._h, arr, n, C.byref(o), res, C.byref(total)))
-        out = [{k: getattr(res[i], k) for k, _ in _RequestResult._fields_} for i in range(n)]
+        out = [_result_dict(res[i]) for i in range(n)]
         return out, total.value
+
+    def open_server(self, workers: int = 8, batch_max: int = 0, hbm_budget_bytes: int = 0) -> "Server":
+        """An online server on this context (ns_serve_open)."""
+        return Server(self, workers, batch_max, hbm_budget_bytes)
+
+    def batch_solve(self, requests):
+        """Solves SAT / QSAT requests together in the persistent batch kernels (SATOSTOP and
+        circuits they cannot take one at a time). Returns (result dicts, info dict)."""
+        n = len(requests)
+        arr, keep = _requests(requests)
+        res = (_RequestResult * max(1, n))()
+        info = np.zeros(8, np.float64)
+        _check(load_library().ns_batch_solve(self._h, arr, n, res, _ptr(info)))
+        names = ["requests", "fallbacks", "vgpr_class", "code_bytes", "gen_ms", "load_ms", "kernel_ms", "wall_ms"]
+        return [_result_dict(res[i]) for i in range(n)], dict(zip(names, info.tolist()))
 
     def greedy_partitioned_solve(self, c: Circuit, unknowns: int, partitions: int, file_prefix: str = "") -> dict:
         """greedysatostop's partitioned search with the image on disk (see the C header)."""
@@ -509,6 +527,73 @@ class Context:
         w = C.c_uint32()
         _check(load_library().ns_satostop_pass_times(self._h, _ptr(buf), 64, C.byref(w)))
         return [float(x) for x in buf[: w.value]]
+
+
+def _requests(requests):
+    arr = (_Request * max(1, len(requests)))()
+    keep = []
+    for i, rq in enumerate(requests):
+        arr[i] = _request(rq, keep)
+    return arr, keep
+
+
+def _request(rq, keep):
+    q = None
+    if rq["kind"] == REQ_QSAT:
+        q = np.ascontiguousarray(rq["quants"], dtype=np.uint8)
+        keep.append(q)
+    return _Request(rq["kind"], rq.get("unknowns", 0), rq["circuit"]._h, _ptr(q), rq.get("arrival_s", 0.0))
+
+
+def _result_dict(r) -> dict:
+    return {k: getattr(r, k) for k, _ in _RequestResult._fields_}
+
+
+class Server:
+    """Online serving (ns_serve_open / submit / poll / wait / close): requests are taken as
+    they are submitted; results are handed out once per ticket."""
+
+    def __init__(self, ctx: "Context", workers: int = 8, batch_max: int = 0, hbm_budget_bytes: int = 0):
+        h = C.c_void_p()
+        o = _ServeOpts(workers, 0, hbm_budget_bytes, batch_max, 0)
+        _check(load_library().ns_serve_open(ctx._h, C.byref(o), C.byref(h)))
+        self._h = h.value
+        self._ctx = ctx  # the context outlives the server
+
+    def submit(self, rq: dict) -> int:
+        keep = []
+        t = C.c_uint64()
+        _check(load_library().ns_serve_submit(self._h, C.byref(_request(rq, keep)), C.byref(t)))
+        return t.value
+
+    def poll(self, ticket: int):
+        """The result dict once the ticket's request has finished, else None."""
+        r, done = _RequestResult(), C.c_int()
+        _check(load_library().ns_serve_poll(self._h, ticket, C.byref(r), C.byref(done)))
+        return _result_dict(r) if done.value else None
+
+    def wait(self, ticket: int, timeout_s: float = -1.0):
+        """The result dict; None when timeout_s (>= 0) passes first."""
+        r, done = _RequestResult(), C.c_int()
+        _check(load_library().ns_serve_wait(self._h, ticket, timeout_s, C.byref(r), C.byref(done)))
+        return _result_dict(r) if done.value else None
+
+    def close(self) -> None:
+        if self._h:
+            h, self._h = self._h, None
+            _check(load_library().ns_serve_close(h))
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def quants_from_string(s: str) -> np.ndarray:

@@ -252,6 +252,102 @@ std::shared_ptr<SledTemplate> sled_template(const std::string& kernel_name, size
 
 }  // namespace
 
+namespace {
+
+// batch region sizes (the bodies are called, so no branch range limits them)
+const size_t kRegionBytes[5] = {65536, 262144, 1048576, 4194304, 16777216};
+// literals of the marker instructions "s_mov_b32 s36, <literal>" that follow each batch
+// kernel's s_getpc_b64 (template's pc base)
+const uint32_t kPcMarkSat = 0x53425342u, kPcMarkQsat = 0x51425342u;
+const uint32_t kSMovS36Lit = 0xBEA400FFu;  // s_mov_b32 s36, <32-bit literal>
+const uint32_t kSGetpcS = 0xBE801C00u;     // s_getpc_b64 s[d:d+1] with the sdst field cleared
+
+std::mutex g_batch_mu;
+std::map<std::string, std::shared_ptr<BatchTemplate>> g_batch;
+
+// offset of the unique "s_getpc_b64; s_mov_b32 s36, lit" pair's s_mov (the pc base)
+size_t find_pc_base(const std::vector<char>& elf, uint32_t lit) {
+  size_t found = 0, at = 0;
+  for (size_t i = 8; i + 8 <= elf.size(); i += 4) {
+    uint32_t w0, w1, w2;
+    std::memcpy(&w0, &elf[i - 4], 4);
+    std::memcpy(&w1, &elf[i], 4);
+    std::memcpy(&w2, &elf[i + 4], 4);
+    if (w1 == kSMovS36Lit && w2 == lit && (w0 & 0xFF80FF00u) == kSGetpcS) {
+      found++;
+      at = i;
+    }
+  }
+  if (found != 1) throw std::runtime_error("batch kernel pc base marker not unique in the template code object");
+  return at;
+}
+
+}  // namespace
+
+std::shared_ptr<const BatchTemplate> batch_template(uint32_t vgpr_class, size_t min_bytes) {
+  size_t bytes = 0;
+  for (size_t b : kRegionBytes)
+    if (min_bytes <= b) {
+      bytes = b;
+      break;
+    }
+  if (!bytes) return nullptr;
+  const std::string name = "ns_nat_batch_r" + std::to_string(vgpr_class);
+  const std::string key = name + "/" + std::to_string(bytes);
+  std::lock_guard<std::mutex> lk(g_batch_mu);
+  auto it = g_batch.find(key);
+  if (it != g_batch.end()) return it->second;
+  const char* tpl = native_template(name);
+  if (!tpl) throw std::runtime_error("missing native template " + name);
+  // the region goes at the end of .text, after both kernels: never reached by falling
+  // through, entered only by the kernels' computed calls
+  std::ostringstream text;
+  text << tpl << "\n\t.text\n\t.p2align 8\n; batch region\n\t.long 0x4e53424d\n\t.long 0x59444f42\n\t.fill "
+       << (bytes / 4 - 2) << ", 4, 0xbf800000\n";
+  auto bt = std::make_shared<BatchTemplate>();
+  bt->elf = assemble_native(text.str());
+  bt->region_bytes = bytes;
+  size_t found = 0;
+  for (size_t i = 0; i + 8 <= bt->elf.size(); i++)
+    if (std::memcmp(&bt->elf[i], kMarker, 8) == 0) {
+      found++;
+      bt->region = i;
+    }
+  if (found != 1 || bt->region % 4) throw std::runtime_error("batch region marker not unique in the template code object");
+  bt->pc_base[0] = find_pc_base(bt->elf, kPcMarkSat);
+  bt->pc_base[1] = find_pc_base(bt->elf, kPcMarkQsat);
+  // both pc bases precede the region (entries are unsigned offsets)
+  if (bt->pc_base[0] >= bt->region || bt->pc_base[1] >= bt->region)
+    throw std::runtime_error("batch region placed before a kernel");
+  bt->sat_kernel = "ns_nat_bsat_r" + std::to_string(vgpr_class);
+  bt->qsat_kernel = "ns_nat_bqsat_r" + std::to_string(vgpr_class);
+  g_batch[key] = bt;
+  return bt;
+}
+
+std::vector<char> compose_batch(const BatchTemplate& t, const std::vector<std::vector<uint32_t>>& bodies,
+                                std::vector<uint64_t>& body_offset) {
+  std::vector<char> elf = t.elf;
+  body_offset.assign(bodies.size(), 0);
+  size_t at = 0;  // byte offset in the region
+  for (size_t i = 0; i < bodies.size(); i++) {
+    const size_t need = bodies[i].size() * 4 + 4;  // body + return
+    if (at + need > t.region_bytes) throw std::runtime_error("batch bodies exceed the region");
+    std::memcpy(&elf[t.region + at], bodies[i].data(), bodies[i].size() * 4);
+    const uint32_t ret = kSSetpcRet;
+    std::memcpy(&elf[t.region + at + bodies[i].size() * 4], &ret, 4);
+    body_offset[i] = at;
+    at = (at + need + 255) & ~(size_t)255;  // bodies start on 256-byte boundaries
+  }
+  return elf;
+}
+
+size_t batch_bytes(const std::vector<std::vector<uint32_t>>& bodies) {
+  size_t at = 0;
+  for (const auto& b : bodies) at = (at + b.size() * 4 + 4 + 255) & ~(size_t)255;
+  return at;
+}
+
 std::vector<char> patch_native(const std::string& kernel_name, const std::string& body, size_t* slot_offset) {
   std::vector<uint32_t> words;
   std::string err;

@@ -4,6 +4,7 @@
 // template with a sled is assembled once per (template, sled size) and cached.
 #pragma once
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -18,5 +19,28 @@ bool encode_body(const std::string& body, std::vector<uint32_t>& words, std::str
 // full text instead).
 std::vector<char> patch_native(const std::string& kernel_name, const std::string& body,
                                size_t* slot_offset = nullptr);
+
+// Batch code objects: the two persistent batch kernels of a register class (SAT and QSAT,
+// ns_nat_bsat_rC / ns_nat_bqsat_rC) with a region after them in .text that holds circuit
+// bodies as subroutines. A kernel calls body i at its pc base + entry, where
+// entry = region + body_offset[i] - pc_base[kernel] (byte offsets in the code object; the
+// loader keeps .text contiguous, so the same distances hold on the device). Each body
+// returns with s_setpc_b64 s[38:39].
+struct BatchTemplate {
+  std::vector<char> elf;
+  size_t region = 0;        // byte offset of the region in the code object
+  size_t region_bytes = 0;
+  size_t pc_base[2] = {0, 0};  // SAT kernel, QSAT kernel
+  std::string sat_kernel, qsat_kernel;
+};
+constexpr uint32_t kSSetpcRet = 0xBE801D26u;  // s_setpc_b64 s[38:39]
+// The cached template of a class whose region holds at least min_bytes (null: too large).
+std::shared_ptr<const BatchTemplate> batch_template(uint32_t vgpr_class, size_t min_bytes);
+// Region bytes the bodies take (each body + its return, on 256-byte boundaries).
+size_t batch_bytes(const std::vector<std::vector<uint32_t>>& bodies);
+// The template's code object with the bodies (encoded words) in its region, each followed
+// by the return; body_offset[i] = byte offset of body i in the region.
+std::vector<char> compose_batch(const BatchTemplate& t, const std::vector<std::vector<uint32_t>>& bodies,
+                                std::vector<uint64_t>& body_offset);
 
 }  // namespace ns

@@ -21,7 +21,7 @@ namespace {
 const uint32_t kWordMasks[5] = {0xAAAAAAAAu, 0xCCCCCCCCu, 0xF0F0F0F0u, 0xFF00FF00u, 0xFFFF0000u};
 // SGPRs of the body: s[40..101]; the SAT templates, whose own code keeps to s0..s13, also
 // give it s[14..19] and s[22..39] (s20 and s21 carry the word index)
-const uint32_t kSgprLoSat = 14, kSgprLo = 40, kSgprHi = 101;
+const uint32_t kSgprLoSat = 14, kSgprLo = 40, kSgprLoBatch = 48, kSgprHi = 101;
 
 struct Loc {
   int v = -1;   // VGPR holding the value
@@ -972,6 +972,7 @@ const char* kind_name(NativeKind k) {
     case NativeKind::SAT: return "sat";
     case NativeKind::QSAT_A: return "qsata";
     case NativeKind::QSAT_B: return "qsatb";
+    case NativeKind::BATCH: return "batch";
     default: return "table";
   }
 }
@@ -1047,7 +1048,7 @@ NativeKernel build_native(const Circuit& c, NativeKind kind, const std::vector<u
       if (!cl.agprs && peak + cl.slots > cl.hi - pool_lo + 1) continue;
       try {
         Allocator a(cx, p, vbit, lane_shift, lm, pool_lo, cl.hi, cl.agprs, cl.slots, out_wires, out_neg,
-                    kind == NativeKind::SAT ? kSgprLoSat : kSgprLo);
+                    kind == NativeKind::SAT ? kSgprLoSat : kind == NativeKind::BATCH ? kSgprLoBatch : kSgprLo);
         at.body = a.run();
         at.vgpr_class = cl.budget;
         at.spills = a.spills();
@@ -1157,6 +1158,10 @@ NativeKernel build_native(const Circuit& c, NativeKind kind, const std::vector<u
   if (table)
     for (size_t k = outs.size(); k < 32; k++) body += "\tv_mov_b32 v" + std::to_string(k) + ", 0\n";
   nk.kernel_name = std::string("ns_nat_") + kind_name(kind) + "_r" + std::to_string(nk.vgpr_class);
+  if (kind == NativeKind::BATCH) {  // called from a batch code object: no module text of its own
+    nk.body = std::move(body);
+    return nk;
+  }
   const char* tpl = native_template(nk.kernel_name);
   if (!tpl) throw std::runtime_error("missing native template " + nk.kernel_name);
   std::string text(tpl);

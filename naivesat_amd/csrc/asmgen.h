@@ -2,6 +2,7 @@
 // kernel templates (templates/gen_templates.py) and assembled in-process.
 #pragma once
 #include <cstdint>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -11,7 +12,9 @@
 
 namespace ns {
 
-enum class NativeKind { SAT, QSAT_A, QSAT_B, TABLE };
+// BATCH: a body for the persistent batch kernels (templates' ns_nat_bsat / ns_nat_bqsat):
+// one output in v0, lane masks in v1..v6, SGPRs s48..s101; called as a subroutine.
+enum class NativeKind { SAT, QSAT_A, QSAT_B, TABLE, BATCH };
 
 // Thrown when a circuit does not fit the register budget of any template (the caller then
 // uses the compiler path).

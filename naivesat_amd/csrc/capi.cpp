@@ -1,6 +1,7 @@
 // extern "C" boundary: converts exceptions to status codes and keeps the last error
 // message per thread.
 #include <cstring>
+#include <memory>
 #include <new>
 #include <string>
 #include <vector>
@@ -11,6 +12,7 @@
 #include "circuit.h"
 #include "runtime.h"
 #include "file_image.h"
+#include "batch.h"
 #include "serve.h"
 #include "static_kernels.h"
 #include "solvers.h"
@@ -56,6 +58,24 @@ int guard(F&& f) {
 
 void need(bool ok, const char* what) {
   if (!ok) throw ns::Error(NS_ERR_INVALID, what);
+}
+
+// native kernel of a diagnostic kind: 0 SAT, 1 QSAT-A, 2 QSAT-B, 3 table, 4 batch body in
+// the SAT input order, 5 batch body in the QSAT input order
+ns::NativeKernel native_kernel(const ns::Circuit& cc, uint32_t kind, uint32_t lane_shift) {
+  std::vector<uint32_t> vb(cc.n), outs;
+  for (uint32_t k = 0; k < cc.n; k++) vb[k] = (kind == 1 || kind == 2 || kind == 5) ? cc.n - 1 - k : k;
+  if (kind == 3)
+    for (uint32_t k = 0; k < cc.outputs.size(); k++) outs.push_back(k);
+  else
+    outs.push_back(0);
+  static const ns::NativeKind kinds[6] = {ns::NativeKind::SAT,   ns::NativeKind::QSAT_A, ns::NativeKind::QSAT_B,
+                                          ns::NativeKind::TABLE, ns::NativeKind::BATCH,  ns::NativeKind::BATCH};
+  try {
+    return ns::build_native(cc, kinds[kind], vb, lane_shift, outs);
+  } catch (const ns::NativeUnfit& e) {
+    throw ns::Error(NS_ERR_UNSUPPORTED, e.what());
+  }
 }
 }  // namespace
 
@@ -296,24 +316,35 @@ int ns_table_build(ns_ctx* ctx, const ns_circuit* c, uint32_t* table_out, uint64
   });
 }
 
+namespace {
+// request of the C ABI -> server request; `own` copies the circuit (online submit)
+ns::ServeRequest to_request(const ns_request& q, bool own) {
+  need(q.circuit != nullptr, "request without a circuit");
+  need(q.kind <= NS_REQ_SATOSTOP, "bad request kind");
+  ns::ServeRequest r;
+  r.kind = q.kind;
+  r.unknowns = q.unknowns;
+  if (own)
+    r.circuit = std::make_shared<const ns::Circuit>(q.circuit->c);
+  else
+    r.circuit = std::shared_ptr<const ns::Circuit>(&q.circuit->c, [](const ns::Circuit*) {});
+  r.arrival_s = q.arrival_s;
+  if (q.kind == NS_REQ_QSAT) {
+    need(q.quants_all != nullptr, "QSAT request without quantifiers");
+    r.quants.assign(q.quants_all, q.quants_all + r.circuit->n);
+    for (auto& v : r.quants) v = v ? 1 : 0;
+  }
+  return r;
+}
+}  // namespace
+
 int ns_serve(ns_ctx* ctx, const ns_request* reqs, uint32_t n, const ns_serve_opts* opts,
              ns_request_result* results, double* total_s) {
   return guard([&] {
     need(ctx && (n == 0 || (reqs && results)), "null argument");
-    std::vector<ns::ServeRequest> rq(n);
-    for (uint32_t i = 0; i < n; i++) {
-      need(reqs[i].circuit != nullptr, "request without a circuit");
-      need(reqs[i].kind <= NS_REQ_SATOSTOP, "bad request kind");
-      rq[i].kind = reqs[i].kind;
-      rq[i].unknowns = reqs[i].unknowns;
-      rq[i].circuit = &reqs[i].circuit->c;
-      rq[i].arrival_s = reqs[i].arrival_s;
-      if (reqs[i].kind == NS_REQ_QSAT) {
-        need(reqs[i].quants_all != nullptr, "QSAT request without quantifiers");
-        rq[i].quants.assign(reqs[i].quants_all, reqs[i].quants_all + rq[i].circuit->n);
-        for (auto& v : rq[i].quants) v = v ? 1 : 0;
-      }
-    }
+    std::vector<ns::ServeRequest> rq;
+    rq.reserve(n);
+    for (uint32_t i = 0; i < n; i++) rq.push_back(to_request(reqs[i], false));
     ns_serve_opts o{};
     if (opts) o = *opts;
     std::vector<ns_request_result> out;
@@ -322,33 +353,142 @@ int ns_serve(ns_ctx* ctx, const ns_request* reqs, uint32_t n, const ns_serve_opt
   });
 }
 
+struct ns_server {
+  std::unique_ptr<ns::Server> srv;
+};
+
+int ns_serve_open(ns_ctx* ctx, const ns_serve_opts* opts, ns_server** out) {
+  return guard([&] {
+    need(ctx && out, "null argument");
+    ns_serve_opts o{};
+    if (opts) o = *opts;
+    const uint64_t budget = o.hbm_budget_bytes ? o.hbm_budget_bytes : (4ull << 30);
+    auto h = std::make_unique<ns_server>();
+    h->srv = std::make_unique<ns::Server>(ctx->ctx, o, budget);
+    *out = h.release();
+  });
+}
+
+int ns_serve_submit(ns_server* srv, const ns_request* req, uint64_t* ticket) {
+  return guard([&] {
+    need(srv && req && ticket, "null argument");
+    *ticket = srv->srv->submit(to_request(*req, true));
+  });
+}
+
+int ns_serve_poll(ns_server* srv, uint64_t ticket, ns_request_result* res, int* done) {
+  return guard([&] {
+    need(srv && done, "null argument");
+    *done = srv->srv->poll(ticket, res) ? 1 : 0;
+  });
+}
+
+int ns_serve_wait(ns_server* srv, uint64_t ticket, double timeout_s, ns_request_result* res, int* done) {
+  return guard([&] {
+    need(srv && done, "null argument");
+    *done = srv->srv->wait(ticket, timeout_s, res) ? 1 : 0;
+  });
+}
+
+int ns_serve_close(ns_server* srv) {
+  return guard([&] {
+    if (!srv) return;
+    std::unique_ptr<ns_server> h(srv);
+    h->srv->close();
+  });
+}
+
+int ns_batch_solve(ns_ctx* ctx, const ns_request* reqs, uint32_t n, ns_request_result* results, double* info) {
+  return guard([&] {
+    need(ctx && (n == 0 || (reqs && results)), "null argument");
+    std::vector<ns::ServeRequest> rq;
+    rq.reserve(n);
+    for (uint32_t i = 0; i < n; i++) rq.push_back(to_request(reqs[i], false));
+    std::vector<const ns::ServeRequest*> batchable, alone;
+    std::vector<size_t> bidx;
+    for (uint32_t i = 0; i < n; i++) {
+      if (ns::Batcher::batchable(rq[i])) {
+        batchable.push_back(&rq[i]);
+        bidx.push_back(i);
+      }
+    }
+    std::vector<ns_request_result> res(n);
+    ns::BatchStats bs;
+    if (!batchable.empty()) {
+      std::vector<ns_request_result> br;
+      ns::batch_solve(ctx->ctx, batchable, br, &bs);
+      for (size_t k = 0; k < bidx.size(); k++) res[bidx[k]] = br[k];
+    }
+    // the others (SATOSTOP and circuits outside the batch kernels) one at a time
+    for (uint32_t i = 0; i < n; i++) {
+      if (ns::Batcher::batchable(rq[i])) continue;
+      bs.fallbacks++;
+      ns_request_result& r = res[i];
+      const ns::ServeRequest& q = rq[i];
+      if (q.kind == NS_REQ_SATOSTOP) {
+        ns_satostop_opts to{};
+        ns_satostop_result tr;
+        ns::satostop_solve(*ctx->ctx.main, *q.circuit, q.unknowns, to, tr, nullptr, nullptr);
+        r.v0 = tr.n_stopped;
+        r.v1 = tr.lowest_u;
+        r.v2 = tr.steps;
+        r.v3 = tr.end;
+        r.kernel_ms = tr.build_ms + tr.jump_ms + tr.walk_ms;
+        r.compile_ms = tr.compile_ms;
+      } else if (q.kind == NS_REQ_SAT) {
+        ns_sat_opts so{};
+        ns_sat_result sr;
+        ns::sat_solve(*ctx->ctx.main, *q.circuit, so, sr);
+        r.v0 = sr.count;
+        r.v1 = sr.lowest;
+        r.v2 = sr.found;
+        r.kernel_ms = sr.kernel_ms;
+        r.compile_ms = sr.compile_ms;
+      } else {
+        ns_qsat_opts qo{};
+        ns_qsat_result qr;
+        ns::qsat_solve(*ctx->ctx.main, *q.circuit, q.quants, qo, qr, nullptr, 0);
+        r.v0 = qr.truth;
+        r.v1 = qr.witness;
+        r.v2 = qr.witness_count;
+        r.v3 = qr.has_witness;
+        r.kernel_ms = qr.kernel_ms;
+        r.compile_ms = qr.compile_ms;
+      }
+      r.status = NS_OK;
+    }
+    if (n) std::memcpy(results, res.data(), sizeof(ns_request_result) * n);
+    if (info) {
+      const double v[8] = {(double)n, (double)bs.fallbacks, (double)bs.vgpr_class, (double)bs.code_bytes,
+                           bs.gen_ms, bs.load_ms, bs.kernel_ms, bs.wall_ms};
+      std::memcpy(info, v, sizeof(v));
+    }
+  });
+}
+
 int ns_native_body(const ns_circuit* c, uint32_t kind, uint32_t lane_shift, char* body, size_t cap,
                    size_t* needed, uint32_t* vgpr_class, uint64_t* code_bytes) {
   return guard([&] {
-    need(c && kind <= 3, "bad argument");
+    need(c && kind <= 5, "bad argument");
     const ns::Circuit& cc = c->c;
-    std::vector<uint32_t> vb(cc.n), outs;
-    for (uint32_t k = 0; k < cc.n; k++) vb[k] = (kind == 1 || kind == 2) ? cc.n - 1 - k : k;
-    if (kind == 3)
-      for (uint32_t k = 0; k < cc.outputs.size(); k++) outs.push_back(k);
-    else
-      outs.push_back(0);
-    static const ns::NativeKind kinds[4] = {ns::NativeKind::SAT, ns::NativeKind::QSAT_A, ns::NativeKind::QSAT_B,
-                                            ns::NativeKind::TABLE};
-    ns::NativeKernel nk;
-    try {
-      nk = ns::build_native(cc, kinds[kind], vb, lane_shift, outs);
-    } catch (const ns::NativeUnfit& e) {
-      throw ns::Error(NS_ERR_UNSUPPORTED, e.what());
+    ns::NativeKernel nk = native_kernel(cc, kind, lane_shift);
+    std::string text;
+    if (kind >= 4) {
+      std::vector<uint32_t> w;
+      std::string err;
+      if (!ns::encode_body(nk.body, w, &err)) throw ns::Error(NS_ERR_UNSUPPORTED, err);
+      text = "; circuit body\n" + nk.body;
+      if (code_bytes) *code_bytes = w.size() * 4;
+    } else {
+      std::vector<char> code = ns::assemble_native(nk.asm_text);
+      // the body sits between the marker comment and the template's ";;#ASMEND"
+      size_t a = nk.asm_text.find("; circuit body\n");
+      size_t b = nk.asm_text.find(";;#ASMEND", a);
+      text = nk.asm_text.substr(a, b - a);
+      if (code_bytes) *code_bytes = code.size();
     }
-    std::vector<char> code = ns::assemble_native(nk.asm_text);
-    // the body sits between the marker comment and the template's ";;#ASMEND"
-    size_t a = nk.asm_text.find("; circuit body\n");
-    size_t b = nk.asm_text.find(";;#ASMEND", a);
-    std::string text = nk.asm_text.substr(a, b - a);
     if (needed) *needed = text.size();
     if (vgpr_class) *vgpr_class = nk.vgpr_class;
-    if (code_bytes) *code_bytes = code.size();
     if (body && cap) {
       need(cap > text.size(), "body buffer too small");
       std::memcpy(body, text.c_str(), text.size() + 1);
@@ -373,26 +513,25 @@ int ns_native_encode(const char* body, size_t len, uint32_t* words, size_t cap, 
 int ns_native_code(const ns_circuit* c, uint32_t kind, uint32_t lane_shift, void* code, size_t cap,
                    size_t* needed, uint32_t* patched, uint64_t* slot_offset) {
   return guard([&] {
-    need(c && kind <= 3, "bad argument");
-    const ns::Circuit& cc = c->c;
-    std::vector<uint32_t> vb(cc.n), outs;
-    for (uint32_t k = 0; k < cc.n; k++) vb[k] = (kind == 1 || kind == 2) ? cc.n - 1 - k : k;
-    if (kind == 3)
-      for (uint32_t k = 0; k < cc.outputs.size(); k++) outs.push_back(k);
-    else
-      outs.push_back(0);
-    static const ns::NativeKind kinds[4] = {ns::NativeKind::SAT, ns::NativeKind::QSAT_A, ns::NativeKind::QSAT_B,
-                                            ns::NativeKind::TABLE};
-    ns::NativeKernel nk;
-    try {
-      nk = ns::build_native(cc, kinds[kind], vb, lane_shift, outs);
-    } catch (const ns::NativeUnfit& e) {
-      throw ns::Error(NS_ERR_UNSUPPORTED, e.what());
-    }
+    need(c && kind <= 5, "bad argument");
+    ns::NativeKernel nk = native_kernel(c->c, kind, lane_shift);
     size_t off = 0;
-    std::vector<char> obj = ns::patch_native(nk.kernel_name, nk.body, &off);
-    const bool p = !obj.empty();
-    if (!p) obj = ns::assemble_native(nk.asm_text);
+    std::vector<char> obj;
+    bool p = true;
+    if (kind >= 4) {
+      std::vector<std::vector<uint32_t>> bodies(1);
+      std::string err;
+      if (!ns::encode_body(nk.body, bodies[0], &err)) throw ns::Error(NS_ERR_UNSUPPORTED, err);
+      auto tpl = ns::batch_template(nk.vgpr_class, ns::batch_bytes(bodies));
+      if (!tpl) throw ns::Error(NS_ERR_UNSUPPORTED, "body exceeds the largest batch region");
+      std::vector<uint64_t> bo;
+      obj = ns::compose_batch(*tpl, bodies, bo);
+      off = tpl->region + bo[0];
+    } else {
+      obj = ns::patch_native(nk.kernel_name, nk.body, &off);
+      p = !obj.empty();
+      if (!p) obj = ns::assemble_native(nk.asm_text);
+    }
     if (needed) *needed = obj.size();
     if (patched) *patched = p ? 1u : 0u;
     if (slot_offset) *slot_offset = p ? off : 0;

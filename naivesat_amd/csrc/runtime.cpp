@@ -301,6 +301,9 @@ Lane::~Lane() {
   buf_a.release();
   buf_b.release();
   buf_c.release();
+  bstats.release();
+  bdesc.release();
+  bchunk.release();
   trim_tables();
   stats.release();
   if (ev0) (void)hipEventDestroy(ev0);

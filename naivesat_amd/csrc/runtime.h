@@ -116,6 +116,7 @@ struct Lane {
   DevBuf tab_a, tab_b, tab_c, tab_d;  // SATOSTOP: next-state table, jump tables, starts
   DevBuf tab_e;                       // SATOSTOP over 31 state bits: outputs 32..n of the table
   DevBuf tab_img;                     // SATOSTOP: packed (n+1)-bit next-state image
+  DevBuf bstats, bdesc, bchunk;       // batches: per-request results, work descriptors, chunk results
   std::vector<double> pass_ms;  // jump-pass kernel times of the last SATOSTOP solve
   // launch sequences of repeated searches, captured once and replayed as HIP graphs; each
   // holds its kernel's module so the module outlives the graph
@@ -139,7 +140,7 @@ struct Lane {
   // device bytes held by the scratch buffers
   size_t held_bytes() const {
     return buf_a.cap + buf_b.cap + buf_c.cap + tab_a.cap + tab_b.cap + tab_c.cap + tab_d.cap + tab_e.cap +
-           tab_img.cap;
+           tab_img.cap + bstats.cap + bdesc.cap + bchunk.cap;
   }
   void trim_tables() {
     tab_a.release();

@@ -778,6 +778,21 @@ WalkLaunch bit_walk(Lane& ctx, const Circuit& c, uint32_t k, double* compile_ms,
 
 }  // namespace
 
+void satostop_buffer_plan(uint32_t n, uint32_t unknowns, uint64_t bytes[5]) {
+  if (n == 0 || n > 31 || unknowns > n)
+    throw Error(ST_UNSUPPORTED, "the buffer plan covers the default search of 1..31 state bits");
+  const uint64_t states = 1ull << n;
+  // the waypoint walks over the table (two record buffers of 2^(n-k) x 16 bytes) where
+  // satostop_solve takes them, and the passes (two u64 jump tables) they fall back on
+  const uint32_t k = waypoint_bits(n, unknowns);
+  const uint64_t records = (k <= n - unknowns && n - k <= 24) ? (1ull << (n - k)) * 16 : 0;
+  bytes[0] = (states < 32 ? 32 : states) * 4;
+  bytes[1] = std::max<uint64_t>((states < 2 ? 2 : states) * 8, records);
+  bytes[2] = bytes[1];
+  bytes[3] = (1ull << unknowns) * 16 + 64;
+  bytes[4] = 0;  // outputs 32..n: only past 31 state bits
+}
+
 void satostop_solve(Lane& ctx, const Circuit& c, uint32_t unknowns, const ns_satostop_opts& o,
                     ns_satostop_result& r, uint64_t* steps_out, uint64_t* ends_out) {
   auto t0 = clk::now();

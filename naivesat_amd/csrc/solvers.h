@@ -16,6 +16,9 @@ void qsat_solve(Lane& ctx, const Circuit& c, const std::vector<uint8_t>& quants_
 void satostop_solve(Lane& ctx, const Circuit& c, uint32_t unknowns, const ns_satostop_opts& o,
                     ns_satostop_result& r, uint64_t* steps_out, uint64_t* ends_out);
 double table_build(Lane& ctx, const Circuit& c, uint32_t* table_out, uint64_t table_len);
+// Bytes satostop_solve's default full search (no flags) may need in ctx.tab_a .. tab_e for
+// n <= 31 state bits: table, two record or jump-table buffers, per-start results.
+void satostop_buffer_plan(uint32_t n, uint32_t unknowns, uint64_t bytes[5]);
 
 // naivesatostop's hash-map search (state spaces beyond the device table)
 void satostop_hashmap_solve(Lane& ctx, const Circuit& c, uint32_t unknowns, const ns_hashmap_opts& o,

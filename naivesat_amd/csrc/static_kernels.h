@@ -51,5 +51,19 @@ hipError_t launch_mem_find(hipStream_t s, const unsigned long long* words, unsig
 hipError_t launch_fill_u64(hipStream_t s, unsigned long long* p, unsigned long long v,
                            unsigned long long count);
 hipError_t launch_fill_u32(hipStream_t s, unsigned int* p, unsigned int v, unsigned long long count);
+// A SAT batch's work descriptor as the fold reads it (the batch kernels' NsBatchDesc:
+// 64 bytes; out = the request's {count, lowest})
+struct BatchFoldDesc {
+  unsigned int wend, chunk0;
+  unsigned long long* out;
+  unsigned int entry, words, jb, opm, valid;
+  unsigned int pad[7];
+};
+// per descriptor (one block each): its chunks' counts summed and lowest hits' minimum
+hipError_t launch_batch_fold(hipStream_t s, const BatchFoldDesc* descs, unsigned int n,
+                             const unsigned long long* chunk_out);
+// p[i * stride] = v for i < count
+hipError_t launch_fill_strided_u64(hipStream_t s, unsigned long long* p, unsigned long long v,
+                                   unsigned long long count, unsigned long long stride);
 
 }  // namespace ns

@@ -476,6 +476,43 @@ __global__ void k_fill_u64(unsigned long long* p, unsigned long long v, unsigned
     p[i] = v;
 }
 
+// Per request of a SAT batch (one block each): the sum of its chunks' counts and the lowest
+// of their lowest hits (chunk_out[2c], [2c + 1]) into its result {count, lowest}.
+__global__ void __launch_bounds__(256) k_batch_fold(const BatchFoldDesc* descs, const unsigned long long* chunk_out) {
+  const BatchFoldDesc d = descs[blockIdx.x];
+  const unsigned int n_ch = (d.wend + d.words - 1u) / d.words;
+  unsigned long long cnt = 0, best = ~0ull;
+  for (unsigned int j = threadIdx.x; j < n_ch; j += blockDim.x) {
+    const unsigned long long c = d.chunk0 + j;
+    cnt += chunk_out[2 * c];
+    const unsigned long long b = chunk_out[2 * c + 1];
+    best = b < best ? b : best;
+  }
+  __shared__ unsigned long long s_cnt[256], s_best[256];
+  s_cnt[threadIdx.x] = cnt;
+  s_best[threadIdx.x] = best;
+  __syncthreads();
+  for (unsigned int h = blockDim.x / 2; h > 0; h >>= 1) {
+    if (threadIdx.x < h) {
+      s_cnt[threadIdx.x] += s_cnt[threadIdx.x + h];
+      const unsigned long long o = s_best[threadIdx.x + h];
+      if (o < s_best[threadIdx.x]) s_best[threadIdx.x] = o;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    d.out[0] = s_cnt[0];
+    d.out[1] = s_best[0];
+  }
+}
+
+__global__ void k_fill_strided_u64(unsigned long long* p, unsigned long long v, unsigned long long count,
+                                   unsigned long long stride) {
+  const unsigned long long step = (unsigned long long)gridDim.x * blockDim.x;
+  for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += step)
+    p[i * stride] = v;
+}
+
 __global__ void k_fill_u32(unsigned int* p, unsigned int v, unsigned long long count) {
   const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
   for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride)
@@ -609,6 +646,20 @@ hipError_t launch_fill_u64(hipStream_t s, unsigned long long* p, unsigned long l
                            unsigned long long count) {
   if (!count) return hipSuccess;
   hipLaunchKernelGGL(k_fill_u64, dim3(looped_blocks(count, 256)), dim3(256), 0, s, p, v, count);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_fold(hipStream_t s, const BatchFoldDesc* descs, unsigned int n,
+                             const unsigned long long* chunk_out) {
+  if (!n) return hipSuccess;
+  hipLaunchKernelGGL(k_batch_fold, dim3(n), dim3(256), 0, s, descs, chunk_out);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_strided_u64(hipStream_t s, unsigned long long* p, unsigned long long v,
+                                   unsigned long long count, unsigned long long stride) {
+  if (!count) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_strided_u64, dim3(looped_blocks(count, 256)), dim3(256), 0, s, p, v, count, stride);
   return hipGetLastError();
 }
 

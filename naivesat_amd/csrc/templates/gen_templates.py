@@ -85,18 +85,19 @@ def sat_kernel(cls, vb, pool_hi):
 extern "C" __global__ void __launch_bounds__(256) {name}(u64 w0, u64 wend, u64* res, u32 wpl, u32 valid) {{
   NS_PROLOGUE
   u32 cnt = 0u;
-  u64 best = ~0ull;
+  u32 brel = ~0u;  // lowest hit as ((word - U) << 5) | bit: one VGPR across the body
 #pragma unroll 1
   for (u32 j = 0; j < wpl; j++) {{
     const u64 wu = U + ((u64)j << 6);
-    const u64 w = wu + lane;
     const u32 wlo = (u32)wu, whi = (u32)(wu >> 32);
+    // words wu + lane at or past wend are not counted: lane >= rem (uniform)
+    const u64 rem = wend > wu ? wend - wu : 0ull;
 {body1(name, 1, 8, pool_hi, SAT_SGPR_POOL)}    o &= valid;
-    if (w >= wend) o = 0u;
+    if ((u64)lane >= rem) o = 0u;
     cnt += __popc(o);
-    if (o != 0u && best == ~0ull) best = (w << 5) | (u64)__builtin_ctz(o);
+    if (o != 0u && brel == ~0u) brel = (((j << 6) + lane) << 5) | (u32)__builtin_ctz(o);
   }}
-  ns_wave_stats(cnt, best, res);
+  ns_wave_stats(cnt, brel == ~0u ? ~0ull : (U << 5) + brel, res);
 }}
 '''
 
@@ -238,11 +239,157 @@ extern "C" __global__ void __launch_bounds__(256) {name}(u64 w0, u64 wend, u32* 
 '''
 
 
+BATCH_HEAD = r'''
+// Work descriptor of one request in a batch (64 bytes): its circuit body is a subroutine in
+// the batch's code object, entered at pc base + entry with the return address in s[38:39];
+// the body may use s48..s101 (and the class's VGPR pool), the caller keeps s0..s35, s40..s47.
+struct NsBatchDesc {
+  u32 wend;      // words of the request's space; words at or past it are not counted
+  u32 chunk0;    // first chunk of the request in the batch's chunk numbering
+  u64* out;      // SAT: its result {count, lowest}; QSAT: its bit array
+  u32 entry;     // byte offset of the body from its kernel's pc base
+  u32 words;     // words per chunk (a multiple of 64 << jb)
+  u32 jb;        // QSAT: in-thread levels above the word
+  u32 opm;       // QSAT: bit l set = AND at level l (level 0 = index bit 0)
+  u32 valid;     // valid assignments of a word (spaces under 32 assignments)
+  u32 pad[7];
+};
+
+// Wave totals of a batch chunk: sum of cnt and min of rel over the wave, wave-uniform. The
+// reduction swizzles within each half-wave (XOR pattern as an immediate: no address
+// registers) and joins the halves by readlane; no branch, so the batch loops stay uniform.
+#define NS_SWZ(v, m) ((u32)__builtin_amdgcn_ds_swizzle((int)(v), ((m) << 10) | 0x1f))
+__device__ inline void ns_chunk_totals(u32 cnt, u32 rel, u32* c_out, u32* r_out) {
+  cnt += NS_SWZ(cnt, 16);
+  { const u32 o = NS_SWZ(rel, 16); rel = o < rel ? o : rel; }
+  cnt += NS_SWZ(cnt, 8);
+  { const u32 o = NS_SWZ(rel, 8); rel = o < rel ? o : rel; }
+  cnt += NS_SWZ(cnt, 4);
+  { const u32 o = NS_SWZ(rel, 4); rel = o < rel ? o : rel; }
+  cnt += NS_SWZ(cnt, 2);
+  { const u32 o = NS_SWZ(rel, 2); rel = o < rel ? o : rel; }
+  cnt += NS_SWZ(cnt, 1);
+  { const u32 o = NS_SWZ(rel, 1); rel = o < rel ? o : rel; }
+  *c_out = __builtin_amdgcn_readlane(cnt, 0) + __builtin_amdgcn_readlane(cnt, 32);
+  const u32 r0 = __builtin_amdgcn_readlane(rel, 0), r1 = __builtin_amdgcn_readlane(rel, 32);
+  *r_out = r0 < r1 ? r0 : r1;
+}
+
+// a wave-uniform 64-bit value in SGPRs (readfirstlane returns int: each half goes through
+// u32 so that the low half is not sign-extended into the high one)
+__device__ inline u64 ns_uni64(u64 x) {
+  const u32 hi = (u32)__builtin_amdgcn_readfirstlane((u32)(x >> 32));
+  const u32 lo = (u32)__builtin_amdgcn_readfirstlane((u32)x);
+  return ((u64)hi << 32) | (u64)lo;
+}
+'''
+
+
+BATCH_SGPRS = [36, 37, 38, 39] + list(range(48, 102))  # call target, return address, body pool
+
+
+def batch_call(cls, lo, hi):
+    """computed call of the descriptor's body: s[36:37] = pc base + entry, return address in s[38:39]"""
+    ins = ", ".join([f'"{{v{1 + i}}}"(LM{i})' for i in range(6)] +
+                    ['"{s20}"(wlo)', '"{s21}"(whi)', '[pl] "s"(pclo)', '[ph] "s"(pchi)', '[en] "s"(entry)'])
+    clob = clobbers(lo, hi, cls == "r512", BATCH_SGPRS)
+    return (f'          u32 o;\n'
+            f'          asm volatile("s_add_u32 s36, %[pl], %[en]\\n\\ts_addc_u32 s37, %[ph], 0\\n\\t'
+            f's_swappc_b64 s[38:39], s[36:37]" : "={{v0}}"(o) : {ins} : {clob});\n')
+
+
+def batch_kernels(cls, pool_hi):
+    common = """  const u32 lane = threadIdx.x & 63u;
+  const u32 LM0 = 0u - (lane & 1u), LM1 = 0u - ((lane >> 1) & 1u);
+  const u32 LM2 = 0u - ((lane >> 2) & 1u), LM3 = 0u - ((lane >> 3) & 1u);
+  const u32 LM4 = 0u - ((lane >> 4) & 1u), LM5 = 0u - ((lane >> 5) & 1u);
+  const u32 whi = 0u;
+  const u32 waves = gridDim.x * (blockDim.x >> 6);
+  const u32 wave0 = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+#pragma unroll 1
+  for (u32 c = wave0; c < chunks; c += waves) {
+    u32 lo = 0u, hi = nd;
+#pragma unroll 1
+    while (hi - lo > 1u) {
+      const u32 mid = (lo + hi) >> 1;
+      if ((u32)__builtin_amdgcn_readfirstlane(descs[mid].chunk0) <= c) lo = mid; else hi = mid;
+    }
+    // every descriptor field is wave-uniform: read into SGPRs
+    const NsBatchDesc* d = descs + lo;
+    const u32 wend = __builtin_amdgcn_readfirstlane(d->wend);
+    const u32 first = (c - __builtin_amdgcn_readfirstlane(d->chunk0)) * __builtin_amdgcn_readfirstlane(d->words);
+    const u32 last = min(first + __builtin_amdgcn_readfirstlane(d->words), wend);
+    u64* out = (u64*)ns_uni64((u64)d->out);
+    const u32 entry = __builtin_amdgcn_readfirstlane(d->entry);
+"""
+    sat = f"""
+// Persistent SAT batch kernel. Wave w of the grid takes chunks w, w + W, w + 2W, ... (W =
+// waves in the grid); a chunk is `words` consecutive words of one request (found by binary
+// search over chunk0), each 64-word step running that request's body with the lane bits at
+// index bits 5..10. The chunk's count and lowest hit go to chunk_out[2c], [2c + 1] (stored
+// by every lane: the same value at the same address), folded per request afterwards. The
+// loops hold no divergent branch: every condition is wave-uniform. The bodies are
+// subroutines in the code object's batch region; the pc base is the address of the marker
+// instruction that follows s_getpc_b64. Word indices are 32-bit (at most 2^36 assignments).
+extern "C" __global__ void __launch_bounds__(256) ns_nat_bsat_{cls}(const NsBatchDesc* descs, u32 nd, u32 chunks,
+                                                      u64* chunk_out) {{
+  u64 pcb;
+  asm volatile("s_getpc_b64 %0\\n\\ts_mov_b32 s36, 0x53425342" : "=s"(pcb) : : "s36");
+  const u32 pclo = __builtin_amdgcn_readfirstlane((u32)pcb), pchi = __builtin_amdgcn_readfirstlane((u32)(pcb >> 32));
+{common}    const u32 valid = __builtin_amdgcn_readfirstlane(d->valid);
+    u32 cnt = 0u;
+    u32 brel = ~0u;  // lowest hit of the chunk as ((word - first) << 5) | bit
+#pragma unroll 1
+    for (u32 w = first; w < last; w += 64u) {{
+      const u32 wlo = __builtin_amdgcn_readfirstlane(w);
+      const u32 rem = last - wlo;  // words wlo + lane with lane >= rem are not counted
+{batch_call(cls, 8, pool_hi)}      o &= valid;
+      o = lane < rem ? o : 0u;
+      cnt += __popc(o);
+      const u32 hit = ((wlo - first + lane) << 5) | (u32)__builtin_ctz(o | 0x80000000u);
+      brel = (o != 0u && brel == ~0u) ? hit : brel;
+    }}
+    u32 tc, tr;
+    ns_chunk_totals(cnt, brel, &tc, &tr);
+    chunk_out[2ull * c] = (u64)tc;
+    chunk_out[2ull * c + 1ull] = tr == ~0u ? ~0ull : ((u64)first << 5) + tr;
+  }}
+}}
+"""
+    qsat = f"""
+// Persistent QSAT batch kernel (one reduced bit per thread): chunks as in the SAT one, with
+// the lane bits at index bits 5+jb..10+jb; a 64 x 2^jb-word tile gives one ballot word of
+// the request's bit array (bit array index = tile index), stored by every lane.
+extern "C" __global__ void __launch_bounds__(256) ns_nat_bqsat_{cls}(const NsBatchDesc* descs, u32 nd, u32 chunks) {{
+  u64 pcb;
+  asm volatile("s_getpc_b64 %0\\n\\ts_mov_b32 s36, 0x51425342" : "=s"(pcb) : : "s36");
+  const u32 pclo = __builtin_amdgcn_readfirstlane((u32)pcb), pchi = __builtin_amdgcn_readfirstlane((u32)(pcb >> 32));
+{common}    const u32 jb = __builtin_amdgcn_readfirstlane(d->jb), opm = __builtin_amdgcn_readfirstlane(d->opm);
+    const u32 tw = 64u << jb;
+#pragma unroll 1
+    for (u32 U = first; U < last; U += tw) {{
+      u32 R = 0u;
+#pragma unroll 1
+      for (u32 j = 0; j < (1u << jb); j++) {{
+        const u32 wlo = __builtin_amdgcn_readfirstlane(U + j);
+{batch_call(cls, 8, pool_hi)}#pragma unroll
+        for (u32 l = 0; l < 5; l++) o = ns_red(o, 1u << l, (opm >> l) & 1u);
+        R |= (o & 1u) << j;
+      }}
+      for (u32 l = 0; l < jb; l++) R = ns_red(R, 1u << l, (opm >> (5 + l)) & 1u);
+      out[U >> (jb + 6)] = __ballot((R & 1u) != 0u && wend - U > (lane << jb));
+    }}
+  }}
+}}
+"""
+    return sat + qsat
+
+
 def main():
-    out = [HEAD]
+    out = [HEAD, BATCH_HEAD]
     for cls, (budget, hi) in CLASSES.items():
         kernels = [("sat", sat_kernel(cls, budget, hi)), ("qsata", qsat_a_kernel(cls, hi)),
-                   ("qsatb", qsat_b_kernel(cls, hi))]
+                   ("qsatb", qsat_b_kernel(cls, hi)), ("batch", batch_kernels(cls, hi))]
         if budget >= 128:
             kernels.append(("table", table_kernel(cls, hi)))
         for kind, src in kernels:

@@ -12,10 +12,16 @@ import numpy as np
 from . import REQ_QSAT, REQ_SAT, REQ_SATOSTOP, Circuit
 
 
-def make_trace(seed: int = 5, n_requests: int = 4096, rate: float = 200.0):
+STREAM_LEN = 4096
+
+
+def make_trace(seed: int = 5, n_requests: int = STREAM_LEN, rate: float = 200.0):
+    """The first n_requests requests of the stream: arrivals and kinds are drawn for at
+    least STREAM_LEN requests, so a shorter trace is a prefix of the full one."""
     rng = np.random.default_rng(seed)
-    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_requests))
-    kinds = rng.choice([REQ_SAT, REQ_QSAT, REQ_SATOSTOP], size=n_requests, p=[0.60, 0.25, 0.15])
+    drawn = max(n_requests, STREAM_LEN)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, drawn))
+    kinds = rng.choice([REQ_SAT, REQ_QSAT, REQ_SATOSTOP], size=drawn, p=[0.60, 0.25, 0.15])
     reqs = []
     for i in range(n_requests):
         kind = int(kinds[i])

@@ -103,7 +103,7 @@ def check(c, kind, lane_shift, wus):
     n = d["n"]
     oc = oracle.OracleCircuit.from_dict(d)
     outs = list(range(len(d["out_wire"]))) if kind == 3 else [0]
-    vbit = [n - 1 - k for k in range(n)] if kind in (1, 2) else list(range(n))
+    vbit = [n - 1 - k for k in range(n)] if kind in (1, 2, 5) else list(range(n))
     lm = 32 if kind == 3 else 1
     for wu in wus:
         v = run_body(body, lm, wu)
@@ -228,6 +228,27 @@ def test_forced_output_groups(monkeypatch, K):
         c2 = ns.Circuit.from_arrays(16, d["in0"], d["in1"], d["func"], [16 + g + 3], [neg])
         check(c2, 0, 0, [0, 64 * 9])
         check(c2, 1, 2, [0, 1 << 8])
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_batch_bodies(seed):
+    # bodies of the persistent batch kernels (kinds 4 and 5: SAT and QSAT input orders, SGPRs
+    # from s48 on) compute the same words as the circuit
+    rng = np.random.default_rng(1700 + seed)
+    n = int(rng.integers(13, 24))  # the batch kernels take QSAT with n >= 11 + jb
+    d = random_circuit(rng, n, int(rng.integers(40, 300)), recent=int(rng.integers(6, 40)))
+    c = ns.Circuit.from_arrays(d["n"], d["in0"], d["in1"], d["func"], d["out_wire"], d["out_neg"])
+    words = 1 << (n - 5)
+    check(c, 4, 0, sorted({0, min(64 * 3, words - 64), words - 64}))
+    for ls in (0, 2):
+        tile = 1 << (ls + 6)
+        base = sorted({0, min(tile, words - tile), words - tile})
+        check(c, 5, ls, [b | j for b in base for j in range(min(2, 1 << ls))])
+
+
+def test_config0_batch_body():
+    c, _ = ns.Circuit.gen_sat(1, 24, 2000, 32)
+    check(c, 4, 0, [0, 64 * 1000 + 128])
 
 
 def test_register_budget_is_reported():
