@@ -20,9 +20,11 @@ Also reported on the same line:
                 the circuit instead of reading a table), the same without a table at 36 state
                 bits, one sweep of the hash-map search at 34 state bits, the
                 QSAT search time (32 inputs, 10,000 gates) and the served request stream
-                (4,096 SAT/QSAT/SATOSTOP requests, Poisson 200/s, seed 5): solves/s with every
-                request queued at once and p50/p99 latency with requests arriving in real
-                time, each for 8 concurrent worker lanes and for one-at-a-time solving.
+                (4,096 SAT/QSAT/SATOSTOP requests, Poisson 200/s, seed 5): solves/s and
+                p50/p99 latency with every request queued at once and with requests
+                submitted online as they arrive (200/s and 10x that), one at a time against
+                8 concurrent lanes with and without batching into persistent kernels; every
+                answer compared with the oracle's (tests/golden/cfg4_answers.json).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-extra] [--no-serve]
        python bench.py --only {sat,table,stop36,hashmap,qsat,serve,copy}   (one part, for profiling runs;
@@ -228,28 +230,111 @@ def qsat_part(ns, ctx):
                          "lut_ops_per_s": words * rq["valu_ops"] / (rq["kernel_ms"] / 1000.0)}}
 
 
-def serve_part(ns, n_requests, workers):
-    """Served stream vs one-at-a-time; every run starts with an empty kernel cache."""
+GOLDEN_CFG4 = os.path.join(ROOT, "tests", "golden", "cfg4_answers.json")
+
+
+def golden_answers(n_requests):
+    """(v0, v1, v2, v3) per request of the served stream from the committed oracle answers
+    (tests/golden/cfg4_answers.json), or None when absent or for another trace length"""
+    if not os.path.exists(GOLDEN_CFG4):
+        return None
+    with open(GOLDEN_CFG4) as f:
+        doc = json.load(f)
+    rows = doc["rows"][:n_requests]
+    if len(rows) < n_requests:
+        return None
+    out = []
+    for kind, n, a, b, c, d in rows:
+        if kind == 0:
+            out.append((a, b if a else 0, int(a > 0), 0))
+        elif kind == 1:
+            out.append((a, c if b else 0, b, int(b > 0)))
+        else:
+            out.append((a, b or 0, c or 0, d or 0))
+    return out
+
+
+def online_run(ns, reqs, workers, batch_max, speedup):
+    """The online API driven the way a service would: an arrival thread submits each request
+    at its Poisson arrival time (divided by `speedup`), the main thread waits for every
+    ticket. Returns (results, seconds from the first submit to the last completion)."""
+    import threading
+
+    ctx = ns.Context(0)
+    srv = ctx.open_server(workers=workers, batch_max=batch_max)
+    tickets = [None] * len(reqs)
+    t0 = time.perf_counter()
+
+    def feed():
+        for i, rq in enumerate(reqs):
+            delay = rq["arrival_s"] / speedup - (time.perf_counter() - t0)
+            if delay > 0:
+                time.sleep(delay)
+            tickets[i] = srv.submit(rq)
+
+    th = threading.Thread(target=feed)
+    th.start()
+    res = []
+    for i in range(len(reqs)):
+        while tickets[i] is None:
+            time.sleep(0.0002)
+        res.append(srv.wait(tickets[i]))
+    th.join()
+    total = max(r["finish_s"] for r in res) - min(r["submit_s"] for r in res)
+    srv.close()
+    ctx.close()
+    return res, total
+
+
+def serve_part(ns, n_requests, workers, batch_max=32):
+    """The served stream (cfg4): saturated (every request queued at once) and online (arrivals
+    submitted as they happen), each one at a time and served (concurrent lanes, with and
+    without batching); every answer checked against the oracle's. Every run starts with an
+    empty kernel cache."""
+    import numpy as np
     from naivesat_amd import trace
 
     reqs = trace.make_trace(seed=5, n_requests=n_requests, rate=200.0)
+    gold = golden_answers(n_requests)
     out = {"workload": f"{n_requests} requests (60% SAT / 25% QSAT / 15% SATOSTOP, n 16..32, gates "
-                       "200..20000 log-uniform), Poisson 200/s, seed 5"}
-    for name, w, realtime in (("batched", workers, False), ("batched_one_at_a_time", 1, False),
-                              ("served", workers, True), ("served_one_at_a_time", 1, True)):
-        ctx = ns.Context(0)
-        res, total = ctx.serve(reqs, workers=w, realtime=realtime)
-        ctx.close()
+                       "200..20000 log-uniform), Poisson 200/s, seed 5",
+           "answers_checked_against": "tests/golden/cfg4_answers.json (oracle)" if gold else None}
+    mismatches = 0
+
+    def summarize(name, res, total, w, b, mode):
+        nonlocal mismatches
+        lat = np.array([r["finish_s"] - r["submit_s"] for r in res])
         bad = sum(1 for r in res if r["status"] != 0)
-        d = {"workers": w, "seconds": total, "solves_per_s": len(reqs) / total, "failed": bad,
-             "compile_ms_total": sum(r["compile_ms"] for r in res),
-             "kernel_ms_total": sum(r["kernel_ms"] for r in res)}
-        if realtime:
-            d.update(trace.latency_stats(reqs, res))
+        wrong = None
+        if gold is not None:
+            wrong = sum(1 for r, g in zip(res, gold) if (r["v0"], r["v1"], r["v2"], r["v3"]) != g)
+            mismatches += wrong
+        d = {"mode": mode, "workers": w, "batch_max": b, "seconds": total, "solves_per_s": len(res) / total,
+             "failed": bad, "wrong_answers": wrong, "p50_ms": float(np.percentile(lat, 50)) * 1e3,
+             "p99_ms": float(np.percentile(lat, 99)) * 1e3,
+             "compile_ms_total": sum(r["compile_ms"] for r in res)}
         out[name] = d
         print(f"serve {name}: {json.dumps(d)}", file=sys.stderr, flush=True)
-    out["batched_speedup"] = out["batched"]["solves_per_s"] / out["batched_one_at_a_time"]["solves_per_s"]
-    out["p99_speedup"] = out["served_one_at_a_time"]["p99_s"] / out["served"]["p99_s"]
+
+    # saturated: all 4,096 requests at t = 0 (latency here is queueing)
+    for name, w, b in (("saturated_one_at_a_time", 1, 0), ("saturated_lanes", workers, 0),
+                       ("saturated_batched", workers, batch_max)):
+        ctx = ns.Context(0)
+        res, total = ctx.serve(reqs, workers=w, realtime=False, batch_max=b)
+        ctx.close()
+        summarize(name, res, total, w, b, "all queued at t=0")
+    # online at the trace's rate (200/s) and 10x compressed (~2,000/s, beyond one lane)
+    for name, w, b, sp in (("online_200_one_at_a_time", 1, 0, 1.0), ("online_200_served", workers, batch_max, 1.0),
+                           ("online_2000_one_at_a_time", 1, 0, 10.0), ("online_2000_lanes", workers, 0, 10.0),
+                           ("online_2000_served", workers, batch_max, 10.0)):
+        res, total = online_run(ns, reqs, w, b, sp)
+        summarize(name, res, total, w, b, f"online submit at arrival/{sp:g} (Poisson {200 * sp:g}/s)")
+    out["saturated_speedup"] = out["saturated_batched"]["solves_per_s"] / out["saturated_one_at_a_time"]["solves_per_s"]
+    out["online_200_p99_speedup"] = out["online_200_one_at_a_time"]["p99_ms"] / out["online_200_served"]["p99_ms"]
+    out["online_2000_p99_speedup"] = out["online_2000_one_at_a_time"]["p99_ms"] / out["online_2000_served"]["p99_ms"]
+    out["online_2000_solves_speedup"] = (out["online_2000_served"]["solves_per_s"] /
+                                         out["online_2000_one_at_a_time"]["solves_per_s"])
+    out["wrong_answers_total"] = mismatches if gold is not None else None
     return out
 
 
@@ -282,6 +367,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--serve-requests", type=int, default=4096)
     ap.add_argument("--serve-workers", type=int, default=8)
+    ap.add_argument("--serve-batch", type=int, default=32)
     ap.add_argument("--only", choices=["sat", "table", "stop36", "hashmap", "qsat", "serve", "copy"])
     args = ap.parse_args()
     if args.gpus != 1:
@@ -296,7 +382,7 @@ def main():
         if args.only == "copy":
             part = copy_part()
         elif args.only == "serve":
-            part = serve_part(ns, args.serve_requests, args.serve_workers)
+            part = serve_part(ns, args.serve_requests, args.serve_workers, args.serve_batch)
         else:
             ctx = ns.Context(0)
             if args.only == "sat":
@@ -326,9 +412,13 @@ def main():
         extra = {"satostop_table": table_part(ns, ctx), "satostop_36": stop36_part(ns, ctx),
                  "satostop_hashmap": hashmap_part(ns, ctx), "qsat": qsat_part(ns, ctx)}
         if not args.no_serve:
-            extra["serve"] = serve_part(ns, args.serve_requests, args.serve_workers)
+            extra["serve"] = serve_part(ns, args.serve_requests, args.serve_workers, args.serve_batch)
         line["extra"] = extra
     print(json.dumps(line))
+    wrong = line.get("extra", {}).get("serve", {}).get("wrong_answers_total")
+    if wrong:
+        print(f"bench.py: {wrong} served answers differ from the oracle's", file=sys.stderr)
+        sys.exit(3)
 
 
 if __name__ == "__main__":

@@ -395,7 +395,8 @@ int ns_satostop_pass_times(ns_ctx* ctx, double* pass_ms, uint32_t cap, uint32_t*
 int ns_pack_entries(ns_ctx* ctx, uint32_t width, const uint64_t* values, uint64_t count, uint64_t* words,
                     uint64_t words_cap);
 
-/* Diagnostic (no counterpart in the reference): the rate of scattered 4-byte reads from a
+/* Measurement probe, not part of the search path (bench.py uses it as the ceiling for the
+ * waypoint walks; no counterpart in the reference): the rate of scattered 4-byte reads from a
  * 2^table_bits-entry device table, lanes x reads_per_lane reads at independent
  * pseudo-random indices (flags bit 0: with the non-temporal load hint); *ms = kernel time
  * of one run (HIP events), *reads = reads made. The ceiling against which the waypoint

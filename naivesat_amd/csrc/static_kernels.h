@@ -4,6 +4,7 @@
 
 namespace ns {
 
+
 hipError_t launch_qsat_reduce(hipStream_t s, const unsigned int* in, unsigned int m, unsigned int k,
                               unsigned int opmask, unsigned int* out, bool final_pass, bool q0all,
                               unsigned long long* stats);

@@ -1,5 +1,6 @@
 """QSAT parity: HIP engine vs the CPU oracle (truth, lowest witness, witness count and
 the outermost block's value bitmap, bit-exact)."""
+import hashlib
 import json
 import os
 
@@ -95,3 +96,50 @@ def test_config2_golden(ctx):
     assert r["witness_count"] == g["witness_count"]
     bm = np.packbits(r["bitmap"].astype(np.uint8), bitorder="little").tobytes().hex()
     assert bm == g["bitmap_hex"]
+
+
+QSAT_FULL = os.path.join(os.path.dirname(__file__), "golden", "qsat_full_goldens.json")
+
+
+def full_cases():
+    with open(QSAT_FULL) as f:
+        return json.load(f)["cases"]
+
+
+@pytest.mark.parametrize("idx", range(4))
+def test_full_size_mixed_bitmap_goldens(ctx, idx):
+    """2^32-assignment searches whose outermost-block bitmap is mixed (oracle goldens,
+    tests/golden/qsat_full_goldens.json): truth, witness count, lowest witness and the bitmap
+    bit for bit, alone and through the batch kernels."""
+    g = full_cases()[idx]
+    assert 0 < g["bitmap_ones"] < (1 << g["first_bits"])
+    c = ns.Circuit.gen_qsat(g["seed"], g["n"], g["gates"], g["last"])
+    q = ns.quants_from_string(g["quants"])
+    r = ctx.qsat_solve(c, q, bitmap=True)
+    assert r["truth"] == g["truth"]
+    assert r["witness_count"] == g["witness_count"]
+    assert (r["witness"] if r["has_witness"] else None) == g["witness"]
+    packed = np.packbits(r["bitmap"].astype(np.uint8), bitorder="little").tobytes()
+    assert int(r["bitmap"].sum()) == g["bitmap_ones"]
+    assert hashlib.sha256(packed).hexdigest() == g["bitmap_sha256"]
+    if "bitmap_hex" in g:
+        assert packed.hex() == g["bitmap_hex"]
+    res, info = ctx.batch_solve([{"kind": ns.REQ_QSAT, "circuit": c, "quants": q}])
+    assert info["fallbacks"] == 0
+    b = res[0]
+    assert (b["v0"], b["v2"], b["v1"] if b["v3"] else None) == (g["truth"], g["witness_count"], g["witness"])
+
+
+def test_multi_launch_and_multi_pass(ctx):
+    """elem_bits far below the space forces 256 search launches into one bit array and two
+    reduction passes over it; the same answer as one launch and as the oracle."""
+    rng = np.random.default_rng(77)
+    n = 28
+    d = random_circuit(rng, n, 200, recent=40)
+    q = [0] * 6 + [1, 0] * 11  # outermost block 6 inputs, 22 alternating levels below
+    r = check(ctx, d, q, elem_bits=20)
+    assert r["passes"] >= 2
+    c = ns.Circuit.from_arrays(d["n"], d["in0"], d["in1"], d["func"], d["out_wire"], d["out_neg"])
+    one = ctx.qsat_solve(c, q, bitmap=True)
+    assert (one["truth"], one["witness_count"], one["witness"]) == (r["truth"], r["witness_count"], r["witness"])
+    assert np.array_equal(one["bitmap"], r["bitmap"])
