@@ -112,6 +112,15 @@ typedef struct {
   double wall_ms;
 } ns_sat_result;
 int ns_sat_solve(ns_ctx* ctx, const ns_circuit* c, const ns_sat_opts* opts, ns_sat_result* res);
+/* No counterpart in the reference: ns_sat_solve for callers that cannot allocate (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). It makes no
+ * device or host allocation, loads no module and builds no graph: the circuit's kernel must
+ * already be built on this context (an earlier ns_sat_solve of the same circuit with the same
+ * opts; otherwise NS_ERR_UNSUPPORTED), and the search counters live in a 12 KiB device array
+ * the library holds per GPU for the life of the process. Every call on one GPU shares that
+ * array and clears it first: calls that overlap on one GPU race. Same results as
+ * ns_sat_solve (compile_ms 0; no graph replay, no words-per-lane tuning run). */
+int ns_sat_solve_noalloc(ns_ctx* ctx, const ns_circuit* c, const ns_sat_opts* opts, ns_sat_result* res);
 
 /* ---- QSAT (replaces the naiveqsat run: AGGR_OUTPUT_OPENCL_CODE tree + OpenCLQuantReducer +
  *      QuantReducer, src/qsat.rs:172-291,405-720,942-1321,1658-1720) ----

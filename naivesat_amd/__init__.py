@@ -143,6 +143,7 @@ def load_library() -> C.CDLL:
         "ns_ctx_create": (I, [I, C.POINTER(P)]),
         "ns_ctx_destroy": (None, [P]),
         "ns_sat_solve": (I, [P, P, C.POINTER(_SatOpts), C.POINTER(_SatResult)]),
+        "ns_sat_solve_noalloc": (I, [P, P, C.POINTER(_SatOpts), C.POINTER(_SatResult)]),
         "ns_qsat_solve": (I, [P, P, P, C.POINTER(_QsatOpts), C.POINTER(_QsatResult), P, U64]),
         "ns_satostop_solve": (I, [P, P, U32, C.POINTER(_StopOpts), C.POINTER(_StopResult), P, P]),
         "ns_satostop_solve_wide": (I, [P, P, U32, C.POINTER(_StopOpts), C.POINTER(_StopResult), P, P]),
@@ -192,7 +193,7 @@ def exported_symbols() -> list:
             "ns_probe_gather", "ns_pack_entries", "ns_ctx_trim", "ns_greedy_partitioned_solve",
             "ns_file_image_create", "ns_file_image_save_chunk", "ns_file_image_load_partition",
             "ns_file_image_save_partition", "ns_file_image_free", "ns_serve_open", "ns_serve_submit",
-            "ns_serve_poll", "ns_serve_wait", "ns_serve_close", "ns_batch_solve"]
+            "ns_serve_poll", "ns_serve_wait", "ns_serve_close", "ns_batch_solve", "ns_sat_solve_noalloc"]
 
 
 def _check(status: int) -> None:
@@ -395,6 +396,15 @@ class Context:
         o = _SatOpts(elem_bits, group_len, words_per_lane, NS_SAT_STOP_AT_FIRST if stop_at_first else 0)
         r = _SatResult()
         _check(load_library().ns_sat_solve(self._h, c._h, C.byref(o), C.byref(r)))
+        return {k: getattr(r, k) for k, _ in _SatResult._fields_}
+
+    def sat_solve_noalloc(self, c: Circuit, elem_bits: int = 0, group_len: int = 0, words_per_lane: int = 0,
+                          stop_at_first: bool = False) -> dict:
+        """ns_sat_solve_noalloc: the search without any device allocation; the circuit's kernel
+        must have been built by an earlier sat_solve with the same options."""
+        o = _SatOpts(elem_bits, group_len, words_per_lane, NS_SAT_STOP_AT_FIRST if stop_at_first else 0)
+        r = _SatResult()
+        _check(load_library().ns_sat_solve_noalloc(self._h, c._h, C.byref(o), C.byref(r)))
         return {k: getattr(r, k) for k, _ in _SatResult._fields_}
 
     def qsat_solve(self, c: Circuit, quants_all: Sequence[int], elem_bits: int = 0, group_len: int = 0,

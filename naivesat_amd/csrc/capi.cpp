@@ -246,6 +246,15 @@ int ns_sat_solve(ns_ctx* ctx, const ns_circuit* c, const ns_sat_opts* opts, ns_s
   });
 }
 
+int ns_sat_solve_noalloc(ns_ctx* ctx, const ns_circuit* c, const ns_sat_opts* opts, ns_sat_result* res) {
+  return guard([&] {
+    need(ctx && c && res, "null argument");
+    ns_sat_opts o{};
+    if (opts) o = *opts;
+    ns::sat_solve_noalloc(*ctx->ctx.main, c->c, o, *res, ns::noalloc_scratch(ctx->ctx.device));
+  });
+}
+
 int ns_qsat_solve(ns_ctx* ctx, const ns_circuit* c, const uint8_t* quants_all, const ns_qsat_opts* opts,
                   ns_qsat_result* res, uint32_t* bitmap, uint64_t bitmap_words) {
   return guard([&] {

@@ -1,6 +1,7 @@
 #include "runtime.h"
 
 #include "asmenc.h"
+#include "static_kernels.h"
 
 #include <cstdio>
 
@@ -196,6 +197,8 @@ Context::Context(int dev) : device(dev) {
   if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
     throw Error(ST_UNSUPPORTED, std::string("device arch ") + prop.gcnArchName +
                                     " is not gfx950 (kernels are built for gfx950 only)");
+  // the allocation-free SAT entry point's counters: their address on this device
+  NS_HIP(noalloc_scratch_setup(dev));
   main = std::make_unique<Lane>(dev, jit);
 }
 
@@ -205,6 +208,7 @@ Lane::Lane(int dev, Jit& j) : device(dev), jit(j) {
   NS_HIP(hipEventCreate(&ev0));
   NS_HIP(hipEventCreate(&ev1));
   stats.ensure(16384);  // counters and flags; search-kernel count slots from byte 4096
+  NS_HIP(hipHostMalloc(reinterpret_cast<void**>(&pinned), kPinnedBytes, hipHostMallocDefault));
 }
 
 void Lane::put_graph(const std::string& key, hipGraphExec_t exec, std::shared_ptr<JitModule> mod) {
@@ -231,6 +235,7 @@ Lane::~Lane() {
   bchunk.release();
   trim_tables();
   stats.release();
+  if (pinned) (void)hipHostFree(pinned);
   if (ev0) (void)hipEventDestroy(ev0);
   if (ev1) (void)hipEventDestroy(ev1);
   if (stream) (void)hipStreamDestroy(stream);

@@ -112,6 +112,9 @@ struct Lane {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   DevBuf stats;             // small result block
+  // pinned host block the allocation-free SAT path reads its counters back into
+  static constexpr size_t kPinnedBytes = 16384;
+  unsigned long long* pinned = nullptr;
   DevBuf buf_a, buf_b, buf_c;         // QSAT scratch (bit arrays)
   DevBuf tab_a, tab_b, tab_c, tab_d;  // SATOSTOP: next-state table, jump tables, starts
   DevBuf tab_e;                       // SATOSTOP over 31 state bits: outputs 32..n of the table

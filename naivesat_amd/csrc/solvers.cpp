@@ -114,8 +114,8 @@ struct CountsCopy {
   unsigned long long slots[kSlots * kSlotStride];
 };
 
-void read_counts_async(Lane& ctx, CountsCopy& cc) {
-  const unsigned long long* st = ctx.stats.as<unsigned long long>();
+void read_counts_async(Lane& ctx, CountsCopy& cc, const unsigned long long* counters = nullptr) {
+  const unsigned long long* st = counters ? counters : ctx.stats.as<unsigned long long>();
   NS_HIP(hipMemcpyAsync(cc.head, st, sizeof(cc.head), hipMemcpyDeviceToHost, ctx.stream));
   NS_HIP(hipMemcpyAsync(cc.slots, st + kSlot0, sizeof(cc.slots), hipMemcpyDeviceToHost, ctx.stream));
 }
@@ -152,7 +152,8 @@ std::string circuit_fingerprint(const Circuit& c) {
 Chosen choose_kernel(Lane& ctx, const Circuit& c, NativeKind kind, const std::vector<uint32_t>& vb,
                      uint32_t lane_shift, const std::vector<uint32_t>& outs, const Geometry& g,
                      const std::string& extra_key, const std::function<std::string(GenInfo*)>& hiprtc_src,
-                     const char* hiprtc_kernel, double* compile_ms, std::string* fp_out = nullptr) {
+                     const char* hiprtc_kernel, double* compile_ms, std::string* fp_out = nullptr,
+                     bool lookup_only = false) {
   const bool native_ok = native_wanted(g);
   const char* og = std::getenv("NS_OUTPUT_GROUPS");  // changes the native body
   std::string fp = circuit_fingerprint(c) + "/" + (og ? og : "") + "/" + std::to_string((int)kind) + "/" +
@@ -166,6 +167,9 @@ Chosen choose_kernel(Lane& ctx, const Circuit& c, NativeKind kind, const std::ve
     if (compile_ms) *compile_ms = 0.0;
     return ch;
   }
+  if (lookup_only)
+    throw Error(ST_UNSUPPORTED, "the circuit's kernel is not built on this context: solve it once with the "
+                                "same options first (building and loading a kernel allocates)");
   auto t0 = clk::now();
   double cms = 0.0;
   if (native_ok) {
@@ -329,6 +333,70 @@ void sat_solve(Lane& ctx, const Circuit& c, const ns_sat_opts& o, ns_sat_result&
   r.found = host[0] != 0;
   r.lowest = r.found ? host[1] : 0;
   r.kernel_ms = kms;
+  r.wall_ms = ms_since(t0);
+}
+
+// The same search with the counters in `scratch` (the file-scope device array of the
+// allocation-free entry point) and its kernel taken from the context's cache: no device
+// or host allocation, no module load, no graph. Counters are cleared by fill kernels and
+// read back into the lane's pinned block.
+void sat_solve_noalloc(Lane& ctx, const Circuit& c, const ns_sat_opts& o, ns_sat_result& r,
+                       unsigned long long* scratch) {
+  auto t0 = clk::now();
+  std::memset(&r, 0, sizeof(r));
+  if (!scratch) throw Error(ST_INTERNAL, "the allocation-free counters are not set up on this device");
+  if (c.outputs.size() != 1) throw Error(ST_INVALID, "SAT needs a circuit with exactly one output");
+  if (c.n == 0 || c.n > 64) throw Error(ST_UNSUPPORTED, "SAT supports 1..64 inputs");
+  static_assert(kSlot0 + kSlots * kSlotStride <= kNoallocScratchWords, "scratch holds the SAT counters");
+  static_assert(sizeof(CountsCopy) <= Lane::kPinnedBytes, "pinned block holds the counters");
+  NS_HIP(hipSetDevice(ctx.device));
+  const uint64_t total_words = c.n >= 5 ? (c.n == 64 ? (1ull << 59) : (1ull << (c.n - 5))) : 1ull;
+  Geometry g = pick_geometry(total_words, o.group_len, o.words_per_lane, false);
+  std::vector<uint32_t> vb(c.n);
+  for (uint32_t k = 0; k < c.n; k++) vb[k] = k;
+  Chosen ch = choose_kernel(
+      ctx, c, NativeKind::SAT, vb, 0, {0}, g, "", [&](GenInfo* info) { return gen_sat_kernel(c, g, info); },
+      "ns_sat", &r.compile_ms, nullptr, true);
+  const uint64_t cw = chunk_words(o.elem_bits, g, total_words);
+  if (ch.native && ch.tuned_wpl && o.words_per_lane == 0 && cw % ((uint64_t)g.block * ch.tuned_wpl) == 0)
+    g.wpl = ch.tuned_wpl;
+  r.cone_gates = ch.info.cone_gates;
+  r.valu_ops = ch.info.valu_ops;
+  r.scalar_ops = ch.info.scalar_ops;
+  r.backend = ch.native ? 1 : 0;
+  r.vgpr_class = ch.vgpr_class;
+  uint32_t wpl = g.wpl, valid = word_valid_mask(c.n);
+  unsigned long long* res = scratch;
+  const bool stop_first = (o.flags & NS_SAT_STOP_AT_FIRST) != 0;
+  CountsCopy* cc = reinterpret_cast<CountsCopy*>(ctx.pinned);
+  NS_HIP(launch_fill_u64(ctx.stream, res, 0ull, 1));
+  NS_HIP(launch_fill_u64(ctx.stream, res + 1, ~0ull, 1));
+  NS_HIP(launch_fill_u64(ctx.stream, res + kSlot0, 0ull, kSlots * kSlotStride));
+  NS_HIP(hipEventRecord(ctx.ev0, ctx.stream));
+  r.complete = 1;
+  for (uint64_t w0 = 0; w0 < total_words; w0 += cw) {
+    const uint64_t words = total_words - w0 < cw ? total_words - w0 : cw;
+    void* extra[3] = {&res, &wpl, &valid};
+    launch_range(ctx, ch.mod->fn, g, w0, words, extra, ch.native ? 3 : 1, total_words);
+    r.launches++;
+    if (stop_first) {
+      NS_HIP(hipMemcpyAsync(cc->head, res, sizeof(cc->head), hipMemcpyDeviceToHost, ctx.stream));
+      NS_HIP(hipStreamSynchronize(ctx.stream));
+      if (cc->head[1] != ~0ull) {
+        r.complete = w0 + words >= total_words;
+        break;
+      }
+    }
+  }
+  NS_HIP(hipEventRecord(ctx.ev1, ctx.stream));
+  read_counts_async(ctx, *cc, res);
+  NS_HIP(hipStreamSynchronize(ctx.stream));
+  unsigned long long host[2];
+  read_counts_done(*cc, host);
+  r.count = host[0];
+  r.found = host[0] != 0;
+  r.lowest = r.found ? host[1] : 0;
+  r.kernel_ms = ctx.elapsed_ms();
   r.wall_ms = ms_since(t0);
 }
 

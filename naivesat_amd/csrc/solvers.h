@@ -10,6 +10,10 @@
 namespace ns {
 
 void sat_solve(Lane& ctx, const Circuit& c, const ns_sat_opts& o, ns_sat_result& r);
+// sat_solve without any allocation: the kernel must be in the context's cache (built by an
+// earlier sat_solve with the same options), counters in `scratch` (device address)
+void sat_solve_noalloc(Lane& ctx, const Circuit& c, const ns_sat_opts& o, ns_sat_result& r,
+                       unsigned long long* scratch);
 void qsat_solve(Lane& ctx, const Circuit& c, const std::vector<uint8_t>& quants_all,
                 const ns_qsat_opts& o, ns_qsat_result& r, uint32_t* bitmap, uint64_t bitmap_words);
 // steps_out / ends_out (optional): 2^unknowns host entries each

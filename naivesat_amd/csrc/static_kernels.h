@@ -4,6 +4,14 @@
 
 namespace ns {
 
+// The allocation-free SAT entry point's counters: a device array of this many u64 words per
+// device (12 KiB), held for the life of the process.
+constexpr unsigned long long kNoallocScratchWords = 512 + 64 * 16;
+constexpr unsigned int kNoallocMaxDevices = 64;
+// Resolves the array's address on `device`, which must be the current device (once per
+// device, at context set-up); noalloc_scratch returns it afterwards (null before).
+hipError_t noalloc_scratch_setup(int device);
+unsigned long long* noalloc_scratch(int device);
 
 hipError_t launch_qsat_reduce(hipStream_t s, const unsigned int* in, unsigned int m, unsigned int k,
                               unsigned int opmask, unsigned int* out, bool final_pass, bool q0all,

@@ -2,6 +2,8 @@
 // next-state table's pointer-jumping passes and the unknown-start query.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "static_kernels.h"
 
 namespace ns {
@@ -517,6 +519,41 @@ __global__ void k_fill_u32(unsigned int* p, unsigned int v, unsigned long long c
   const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
   for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride)
     p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Counters of the allocation-free SAT entry point (ns_sat_solve_noalloc): [0] count, [1]
+// lowest hit, 64 count slots 16 words apart from [512], as the SAT kernels write them. One
+// array per device for the life of the process; calls that overlap on one device race.
+// In host code `scratch` names a host-side stand-in: only the address that
+// hipGetSymbolAddress returns for the current device may reach a kernel or a copy.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned long long scratch[kNoallocScratchWords];
+
+namespace {
+std::mutex g_scratch_mu;
+unsigned long long* g_scratch_ptr[kNoallocMaxDevices] = {};
+}  // namespace
+
+hipError_t noalloc_scratch_setup(int device) {
+  if (device < 0 || device >= (int)kNoallocMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return hipSuccess;
+  int cur = 0;
+  hipError_t e = hipGetDevice(&cur);
+  if (e != hipSuccess) return e;
+  if (cur != device) return hipErrorInvalidDevice;  // the caller makes the device current
+  void* p = nullptr;
+  e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  g_scratch_ptr[device] = static_cast<unsigned long long*>(p);
+  return hipSuccess;
+}
+
+unsigned long long* noalloc_scratch(int device) {
+  if (device < 0 || device >= (int)kNoallocMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return g_scratch_ptr[device];
 }
 
 // ---------------------------------------------------------------------------
