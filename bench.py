@@ -27,7 +27,7 @@ Also reported on the same line:
                 answer compared with the oracle's (tests/golden/cfg4_answers.json).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-extra] [--no-serve]
-       python bench.py --only {sat,table,stop36,hashmap,qsat,serve,copy}   (one part, for profiling runs;
+       python bench.py --only {sat,table,stop36,hashmap,qsat,serve,copy,batch}   (one part, for profiling runs;
        `copy` times a plain 1 GiB device copy used to calibrate the HBM byte counters)
 """
 import argparse
@@ -230,6 +230,50 @@ def qsat_part(ns, ctx):
                          "lut_ops_per_s": words * rq["valu_ops"] / (rq["kernel_ms"] / 1000.0)}}
 
 
+def batch_repeat_part(ns, n_requests=64, repeats=5):
+    """64 SAT / QSAT requests of the served stream: one at a time on a fresh context, as one
+    batch the first time (code generation, one module load, eager launches) and as the same
+    batch again (its kept plan replayed as a HIP graph); answers compared across all three."""
+    from naivesat_amd import trace
+
+    reqs = [rq for rq in trace.make_trace(seed=5, n_requests=200) if rq["kind"] != ns.REQ_SATOSTOP][:n_requests]
+    ctx = ns.Context(0)
+    t0 = time.perf_counter()
+    alone = []
+    for rq in reqs:
+        if rq["kind"] == ns.REQ_SAT:
+            r = ctx.sat_solve(rq["circuit"])
+            alone.append((r["count"], r["lowest"], r["found"], 0))
+        else:
+            r = ctx.qsat_solve(rq["circuit"], rq["quants"])
+            alone.append((r["truth"], r["witness"], r["witness_count"], r["has_witness"]))
+    alone_ms = (time.perf_counter() - t0) * 1e3
+    ctx.close()
+    ctx = ns.Context(0)
+    t0 = time.perf_counter()
+    first, info0 = ctx.batch_solve(reqs)
+    first_ms = (time.perf_counter() - t0) * 1e3
+    rep_ms, rep_kernel = [], []
+    same = [(r["v0"], r["v1"], r["v2"], r["v3"]) for r in first] == alone
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        res, info = ctx.batch_solve(reqs)
+        rep_ms.append((time.perf_counter() - t0) * 1e3)
+        rep_kernel.append(info["kernel_ms"])
+        same &= [(r["v0"], r["v1"], r["v2"], r["v3"]) for r in res] == alone
+    ctx.close()
+    out = {"workload": f"{len(reqs)} SAT/QSAT requests of the served stream (seed 5)", "answers_equal": same,
+           "one_at_a_time_ms": alone_ms, "batch_first_ms": first_ms, "batch_first_kernel_ms": info0["kernel_ms"],
+           "batch_first_gen_ms": info0["gen_ms"], "batch_first_load_ms": info0["load_ms"],
+           "batch_code_bytes": info0["code_bytes"], "graph_replay_ms": min(rep_ms),
+           "graph_replay_kernel_ms": min(rep_kernel), "graph_runs": info["graph_runs"],
+           "solves_per_s_one_at_a_time": len(reqs) / (alone_ms / 1e3),
+           "solves_per_s_batch_first": len(reqs) / (first_ms / 1e3),
+           "solves_per_s_graph_replay": len(reqs) / (min(rep_ms) / 1e3)}
+    print(f"batch_repeat: {json.dumps(out)}", file=sys.stderr, flush=True)
+    return out
+
+
 GOLDEN_CFG4 = os.path.join(ROOT, "tests", "golden", "cfg4_answers.json")
 
 
@@ -368,7 +412,7 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=4096)
     ap.add_argument("--serve-workers", type=int, default=8)
     ap.add_argument("--serve-batch", type=int, default=32)
-    ap.add_argument("--only", choices=["sat", "table", "stop36", "hashmap", "qsat", "serve", "copy"])
+    ap.add_argument("--only", choices=["sat", "table", "stop36", "hashmap", "qsat", "serve", "copy", "batch"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -381,6 +425,8 @@ def main():
     if args.only:
         if args.only == "copy":
             part = copy_part()
+        elif args.only == "batch":
+            part = batch_repeat_part(ns)
         elif args.only == "serve":
             part = serve_part(ns, args.serve_requests, args.serve_workers, args.serve_batch)
         else:
@@ -411,6 +457,7 @@ def main():
     if not args.no_extra:
         extra = {"satostop_table": table_part(ns, ctx), "satostop_36": stop36_part(ns, ctx),
                  "satostop_hashmap": hashmap_part(ns, ctx), "qsat": qsat_part(ns, ctx)}
+        extra["batch_repeat"] = batch_repeat_part(ns)
         if not args.no_serve:
             extra["serve"] = serve_part(ns, args.serve_requests, args.serve_workers, args.serve_batch)
         line["extra"] = extra

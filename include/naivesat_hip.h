@@ -365,8 +365,12 @@ int ns_serve_close(ns_server* srv);
  * process). SAT results as ns_sat_solve's count / lowest / found, QSAT as ns_qsat_solve's
  * truth / witness / witness_count / has_witness, in v0..v3 of results[i]; requests of
  * kind SATOSTOP, and circuits the batch kernels cannot take, are solved one at a time.
- * info (optional, 8 doubles): requests, fallbacks, register class, code bytes, code
- * generation ms, module load ms, kernel ms, wall ms. */
+ * The context keeps its last four batches: the same batch again (same circuits, kinds and
+ * prefixes, in order) skips code generation and the module load and runs as a HIP graph,
+ * captured on the first repeat and replayed afterwards.
+ * info (optional, 10 doubles): requests, fallbacks, largest register class, code bytes, code
+ * generation ms, module load ms, kernel ms, wall ms, graph launches of this batch so far,
+ * batch kernel launches (one per kind and register class present). */
 int ns_batch_solve(ns_ctx* ctx, const ns_request* reqs, uint32_t n, ns_request_result* results, double* info);
 
 /* Diagnostic (no counterpart in the reference; no device needed): builds the native-path
@@ -388,8 +392,9 @@ int ns_native_encode(const char* body, size_t len, uint32_t* words, size_t cap, 
  * engine loads for a circuit's native kernel (kind and lane_shift as for ns_native_body):
  * the template's cached code object with the encoded body copied into its body slot at
  * byte *slot_offset (*patched = 1), or the fully assembled module (*patched = 0) when the
- * body does not fit the largest slot. Kinds 4 and 5: the batch code object of the body's
- * register class with this one body in its region, *slot_offset = the body's first byte. */
+ * body does not fit the largest slot. Kinds 4 and 5: the batch code object (the batch
+ * kernels of every register class) with this one body in its region, *slot_offset = the
+ * body's first byte. */
 int ns_native_code(const ns_circuit* c, uint32_t kind, uint32_t lane_shift, void* code, size_t cap,
                    size_t* needed, uint32_t* patched, uint64_t* slot_offset);
 

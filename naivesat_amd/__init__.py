@@ -501,9 +501,10 @@ class Context:
         n = len(requests)
         arr, keep = _requests(requests)
         res = (_RequestResult * max(1, n))()
-        info = np.zeros(8, np.float64)
+        info = np.zeros(10, np.float64)
         _check(load_library().ns_batch_solve(self._h, arr, n, res, _ptr(info)))
-        names = ["requests", "fallbacks", "vgpr_class", "code_bytes", "gen_ms", "load_ms", "kernel_ms", "wall_ms"]
+        names = ["requests", "fallbacks", "vgpr_class", "code_bytes", "gen_ms", "load_ms", "kernel_ms", "wall_ms",
+                 "graph_runs", "launches"]
         return [_result_dict(res[i]) for i in range(n)], dict(zip(names, info.tolist()))
 
     def greedy_partitioned_solve(self, c: Circuit, unknowns: int, partitions: int, file_prefix: str = "") -> dict:

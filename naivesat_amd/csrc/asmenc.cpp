@@ -257,8 +257,9 @@ namespace {
 // batch region sizes (the bodies are called, so no branch range limits them)
 const size_t kRegionBytes[5] = {65536, 262144, 1048576, 4194304, 16777216};
 // literals of the marker instructions "s_mov_b32 s36, <literal>" that follow each batch
-// kernel's s_getpc_b64 (template's pc base)
-const uint32_t kPcMarkSat = 0x53425342u, kPcMarkQsat = 0x51425342u;
+// kernel's s_getpc_b64 (its pc base): SAT 0x534253c0 + class index, QSAT 0x514253c0 + index
+const uint32_t kPcMarkSat = 0x534253c0u, kPcMarkQsat = 0x514253c0u;
+const char* const kClassNames[4] = {"64", "128", "256", "512"};
 const uint32_t kSMovS36Lit = 0xBEA400FFu;  // s_mov_b32 s36, <32-bit literal>
 const uint32_t kSGetpcS = 0xBE801C00u;     // s_getpc_b64 s[d:d+1] with the sdst field cleared
 
@@ -284,7 +285,17 @@ size_t find_pc_base(const std::vector<char>& elf, uint32_t lit) {
 
 }  // namespace
 
-std::shared_ptr<const BatchTemplate> batch_template(uint32_t vgpr_class, size_t min_bytes) {
+int batch_class_index(uint32_t vgpr_class) {
+  switch (vgpr_class) {
+    case 64: return 0;
+    case 128: return 1;
+    case 256: return 2;
+    case 512: return 3;
+    default: throw std::runtime_error("no batch kernels for register class " + std::to_string(vgpr_class));
+  }
+}
+
+std::shared_ptr<const BatchTemplate> batch_template(size_t min_bytes) {
   size_t bytes = 0;
   for (size_t b : kRegionBytes)
     if (min_bytes <= b) {
@@ -292,14 +303,14 @@ std::shared_ptr<const BatchTemplate> batch_template(uint32_t vgpr_class, size_t 
       break;
     }
   if (!bytes) return nullptr;
-  const std::string name = "ns_nat_batch_r" + std::to_string(vgpr_class);
+  const std::string name = "ns_nat_batch";
   const std::string key = name + "/" + std::to_string(bytes);
   std::lock_guard<std::mutex> lk(g_batch_mu);
   auto it = g_batch.find(key);
   if (it != g_batch.end()) return it->second;
   const char* tpl = native_template(name);
   if (!tpl) throw std::runtime_error("missing native template " + name);
-  // the region goes at the end of .text, after both kernels: never reached by falling
+  // the region goes at the end of .text, after every kernel: never reached by falling
   // through, entered only by the kernels' computed calls
   std::ostringstream text;
   text << tpl << "\n\t.text\n\t.p2align 8\n; batch region\n\t.long 0x4e53424d\n\t.long 0x59444f42\n\t.fill "
@@ -314,13 +325,15 @@ std::shared_ptr<const BatchTemplate> batch_template(uint32_t vgpr_class, size_t 
       bt->region = i;
     }
   if (found != 1 || bt->region % 4) throw std::runtime_error("batch region marker not unique in the template code object");
-  bt->pc_base[0] = find_pc_base(bt->elf, kPcMarkSat);
-  bt->pc_base[1] = find_pc_base(bt->elf, kPcMarkQsat);
-  // both pc bases precede the region (entries are unsigned offsets)
-  if (bt->pc_base[0] >= bt->region || bt->pc_base[1] >= bt->region)
-    throw std::runtime_error("batch region placed before a kernel");
-  bt->sat_kernel = "ns_nat_bsat_r" + std::to_string(vgpr_class);
-  bt->qsat_kernel = "ns_nat_bqsat_r" + std::to_string(vgpr_class);
+  for (int c = 0; c < 4; c++) {
+    bt->pc_base[0][c] = find_pc_base(bt->elf, kPcMarkSat + c);
+    bt->pc_base[1][c] = find_pc_base(bt->elf, kPcMarkQsat + c);
+    // every pc base precedes the region (entries are unsigned offsets)
+    if (bt->pc_base[0][c] >= bt->region || bt->pc_base[1][c] >= bt->region)
+      throw std::runtime_error("batch region placed before a kernel");
+    bt->kernel[0][c] = std::string("ns_nat_bsat_r") + kClassNames[c];
+    bt->kernel[1][c] = std::string("ns_nat_bqsat_r") + kClassNames[c];
+  }
   g_batch[key] = bt;
   return bt;
 }

@@ -20,22 +20,24 @@ bool encode_body(const std::string& body, std::vector<uint32_t>& words, std::str
 std::vector<char> patch_native(const std::string& kernel_name, const std::string& body,
                                size_t* slot_offset = nullptr);
 
-// Batch code objects: the two persistent batch kernels of a register class (SAT and QSAT,
-// ns_nat_bsat_rC / ns_nat_bqsat_rC) with a region after them in .text that holds circuit
-// bodies as subroutines. A kernel calls body i at its pc base + entry, where
-// entry = region + body_offset[i] - pc_base[kernel] (byte offsets in the code object; the
-// loader keeps .text contiguous, so the same distances hold on the device). Each body
-// returns with s_setpc_b64 s[38:39].
+// Batch code objects: the persistent batch kernels of every register class (SAT and QSAT:
+// ns_nat_bsat_rC / ns_nat_bqsat_rC, C = 64, 128, 256, 512) with a region after them in .text
+// that holds circuit bodies as subroutines. Kernel (kind k, class c) calls body i at its pc
+// base + entry, where entry = region + body_offset[i] - pc_base[k][c] (byte offsets in the
+// code object; the loader keeps .text contiguous, so the same distances hold on the device).
+// Each body returns with s_setpc_b64 s[38:39].
 struct BatchTemplate {
   std::vector<char> elf;
-  size_t region = 0;        // byte offset of the region in the code object
+  size_t region = 0;  // byte offset of the region in the code object
   size_t region_bytes = 0;
-  size_t pc_base[2] = {0, 0};  // SAT kernel, QSAT kernel
-  std::string sat_kernel, qsat_kernel;
+  size_t pc_base[2][4] = {};  // [0 SAT / 1 QSAT][class index: 64, 128, 256, 512]
+  std::string kernel[2][4];
 };
 constexpr uint32_t kSSetpcRet = 0xBE801D26u;  // s_setpc_b64 s[38:39]
-// The cached template of a class whose region holds at least min_bytes (null: too large).
-std::shared_ptr<const BatchTemplate> batch_template(uint32_t vgpr_class, size_t min_bytes);
+// The cached template whose region holds at least min_bytes (null: too large).
+std::shared_ptr<const BatchTemplate> batch_template(size_t min_bytes);
+// index of a register class in BatchTemplate's tables (64 -> 0 ... 512 -> 3)
+int batch_class_index(uint32_t vgpr_class);
 // Region bytes the bodies take (each body + its return, on 256-byte boundaries).
 size_t batch_bytes(const std::vector<std::vector<uint32_t>>& bodies);
 // The template's code object with the bodies (encoded words) in its region, each followed

@@ -1,13 +1,19 @@
 // Batched SAT / QSAT solving with the persistent batch kernels (templates/gen_templates.py:
 // ns_nat_bsat_rC, ns_nat_bqsat_rC). One batch = one code object holding every request's
-// body, one descriptor array per kernel, at most two batch kernel launches (plus the fold of
-// the SAT chunk results and the QSAT reduction passes of each QSAT request), one read-back
-// of all results.
+// body and the batch kernels of every register class, one persistent launch per kind and
+// register class present (each at its class's occupancy; plus the fold of the SAT chunk
+// results and the QSAT reduction passes of each QSAT request), one read-back of all results. A batcher that keeps plans (the context's own, behind ns_batch_solve)
+// remembers its recent batches: the same batch again skips code generation and the module
+// load, is captured as a HIP graph the second time and replayed from then on.
 #include "batch.h"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <cstdlib>
 #include <cstring>
+#include <exception>
+#include <thread>
 
 #include "asmenc.h"
 #include "asmgen.h"
@@ -37,18 +43,11 @@ static_assert(sizeof(Desc) == 64 && sizeof(Desc) == sizeof(BatchFoldDesc), "batc
 constexpr uint64_t kStatsWords = 512 + 64 * 16;
 constexpr uint64_t kStatsBytes = kStatsWords * 8;
 constexpr uint32_t kMaxChunkWords = 1u << 24;
-
-int class_index(uint32_t vgpr_class) {
-  switch (vgpr_class) {
-    case 64: return 0;
-    case 128: return 1;
-    case 256: return 2;
-    default: return 3;
-  }
-}
+constexpr size_t kKeptPlans = 4;
 
 // waves per SIMD the batch kernels reach in each class (SGPRs limit r64 to 7)
 const uint32_t kOccupancy[4] = {7, 4, 2, 1};
+const uint32_t kClassOf[4] = {64, 128, 256, 512};
 
 uint32_t c_valid_mask(uint32_t n) { return n >= 5 ? 0xffffffffu : (uint32_t)((1ull << (1u << n)) - 1); }
 
@@ -64,18 +63,6 @@ uint32_t qsat_first_bits(const std::vector<uint8_t>& q) {
   while (fb < q.size() && q[fb] == q[0]) fb++;
   return fb;
 }
-
-struct Item {
-  size_t idx = 0;
-  bool qsat = false, fallback = false;
-  uint32_t n = 0, jb = 0, vgpr = 0, cost = 0, m0 = 0;
-  uint64_t words = 0;     // words of the space
-  uint64_t chunk_w = 0;   // words per chunk
-  uint64_t chunks = 0;
-  uint64_t bits_off = 0;  // QSAT: byte offset of its bit array in lane.buf_a
-  std::vector<uint32_t> code;
-  double gen_ms = 0;
-};
 
 void solve_alone(Lane& lane, const ServeRequest& r, ns_request_result& res) {
   if (r.kind == NS_REQ_SAT) {
@@ -103,12 +90,188 @@ void solve_alone(Lane& lane, const ServeRequest& r, ns_request_result& res) {
 
 }  // namespace
 
-Batcher::Batcher(Context& ctx, const ns_serve_opts& o) : ctx_(ctx) {
+// One request of a batch: its body and its place in the batch's layout.
+struct BatchItem {
+  size_t idx = 0;  // position in the caller's request list
+  bool qsat = false, fallback = false, q0all = false;
+  uint32_t n = 0, jb = 0, vgpr = 0, cost = 0, m0 = 0, L = 0, opm = 0;
+  uint64_t words = 0, chunk_w = 0, chunks = 0;
+  uint64_t bits_off = 0;        // QSAT: byte offset of its bit array
+  std::vector<uint8_t> quants;  // QSAT: its prefix (reduction-pass operations)
+  std::vector<uint32_t> code;
+  double gen_ms = 0;
+};
+
+// The requests of one kind and register class in a batch: one persistent launch of that
+// class's kernel, at that class's occupancy.
+struct BatchGroup {
+  bool qsat = false;
+  int ci = 0;                   // register class index (64, 128, 256, 512)
+  size_t first = 0, count = 0;  // its descriptors (and items)
+  uint64_t chunks = 0;
+  uint64_t chunk_base = 0;      // SAT: its first chunk result (in chunk_out, 16 bytes each)
+  uint32_t grid = 0, tune_idx = 0;
+  double cost = 0;
+  hipFunction_t fn = nullptr;
+  hipEvent_t ev[2] = {nullptr, nullptr};  // its launch, timed on eager runs
+};
+
+// Everything a batch launch needs, and for a kept plan the device memory it owns.
+struct BatchPlan {
+  std::string key;
+  const Lane* lane = nullptr;
+  JitModule mod;
+  std::vector<BatchItem> items;  // the batched requests, in descriptor order (by group)
+  std::vector<Desc> descs;       // out bound per buffer set
+  std::vector<BatchGroup> groups;
+  uint64_t csat = 0, qbytes = 0, pp_bytes = 0, K = 0;
+  uint32_t cls = 64;             // the largest register class of its bodies
+  double load_ms = 0, gen_ms = 0, gen_wall_ms = 0;
+  uint64_t code_bytes = 0;
+  // kept plans: their own buffers and, from the second run, their graph
+  DevBuf d_desc, d_stats, d_chunk, d_bits, d_pp_a, d_pp_b;
+  hipGraphExec_t graph = nullptr;
+  uint32_t runs = 0;
+  ~BatchPlan() {
+    if (graph) (void)hipGraphExecDestroy(graph);
+    for (BatchGroup& g : groups)
+      for (hipEvent_t e : g.ev)
+        if (e) (void)hipEventDestroy(e);
+  }
+};
+
+namespace {
+
+// device buffers a batch launch writes
+struct BatchBufs {
+  Desc* desc;
+  unsigned long long* stats;
+  unsigned long long* chunk;
+  char* bits;
+  unsigned int* pp[2];
+};
+
+// descriptors with their out pointers bound to the buffers
+std::vector<Desc> bind(const BatchPlan& p, const BatchBufs& b) {
+  std::vector<Desc> d = p.descs;
+  for (size_t i = 0; i < d.size(); i++) {
+    const BatchItem& it = p.items[i];
+    d[i].out = it.qsat ? (uint64_t)(b.bits + it.bits_off) : (uint64_t)(b.stats + it.idx * kStatsWords);
+  }
+  return d;
+}
+
+// the batch's device work on `s` (also what a capture records): counter fills, each group's
+// batch kernel, the fold of the SAT chunks, the QSAT reduction passes; `timed` records each
+// group's launch between its events (eager runs)
+void enqueue(const BatchPlan& p, const BatchBufs& b, hipStream_t s, bool timed) {
+  NS_HIP(launch_fill_u64(s, b.stats, 0ull, p.K * kStatsWords));
+  NS_HIP(launch_fill_strided_u64(s, b.stats + 1, ~0ull, p.K, kStatsWords));
+  for (const BatchGroup& g : p.groups) {
+    const Desc* dp = b.desc + g.first;
+    uint32_t nd = (uint32_t)g.count, ch = (uint32_t)g.chunks;
+    if (timed) NS_HIP(hipEventRecord(g.ev[0], s));
+    if (!g.qsat) {
+      unsigned long long* chunk_out = b.chunk + 2 * g.chunk_base;
+      void* args[4] = {&dp, &nd, &ch, &chunk_out};
+      NS_HIP(hipModuleLaunchKernel(g.fn, g.grid, 1, 1, 256, 1, 1, 0, s, args, nullptr));
+    } else {
+      void* args[3] = {&dp, &nd, &ch};
+      NS_HIP(hipModuleLaunchKernel(g.fn, g.grid, 1, 1, 256, 1, 1, 0, s, args, nullptr));
+    }
+    if (timed) NS_HIP(hipEventRecord(g.ev[1], s));
+    if (!g.qsat)
+      NS_HIP(launch_batch_fold(s, reinterpret_cast<const BatchFoldDesc*>(dp), nd, b.chunk + 2 * g.chunk_base));
+  }
+  // the reduction passes of each QSAT request over its bit array, in prefix order
+  for (const BatchItem& it : p.items) {
+    if (!it.qsat) continue;
+    const std::vector<uint8_t>& q = it.quants;
+    uint32_t m = it.m0, remaining = it.L - 5 - it.jb, level = 5 + it.jb;
+    const unsigned int* in = reinterpret_cast<const unsigned int*>(b.bits + it.bits_off);
+    int which = 0;
+    while (true) {
+      const uint32_t k = remaining < 11 ? remaining : 11;
+      const bool final_pass = k == remaining;
+      const uint32_t mo = m - k;
+      const uint64_t owords = mo >= 5 ? (1ull << (mo - 5)) : 1ull;
+      unsigned int* o = b.pp[which];
+      NS_HIP(launch_fill_u32(s, o, 0u, owords + 1));
+      uint32_t opmask = 0;
+      for (uint32_t l = 0; l < k; l++)
+        if (q[it.n - 1 - (level + l)]) opmask |= 1u << l;
+      NS_HIP(launch_qsat_reduce(s, in, m, k, opmask, o, final_pass, it.q0all, b.stats + it.idx * kStatsWords));
+      if (final_pass) break;
+      in = o;
+      m = mo;
+      remaining -= k;
+      level += k;
+      which ^= 1;
+    }
+  }
+}
+
+// request results from the read-back counters
+void decode(const BatchPlan& p, const std::vector<unsigned long long>& host, double kernel_ms,
+            std::vector<ns_request_result>& res) {
+  for (const BatchItem& it : p.items) {
+    const unsigned long long* h = host.data() + it.idx * kStatsWords;
+    unsigned long long count = h[0];
+    for (uint64_t s = 0; s < 64; s++) count += h[512 + s * 16];
+    const bool hit = count != 0;
+    ns_request_result& rr = res[it.idx];
+    if (it.qsat) {
+      rr.v0 = (hit ? 1u : 0u) ^ (it.q0all ? 1u : 0u);
+      rr.v1 = hit ? h[1] : 0;
+      rr.v2 = count;
+      rr.v3 = hit ? 1 : 0;
+    } else {
+      rr.v0 = count;
+      rr.v1 = hit ? h[1] : 0;
+      rr.v2 = hit ? 1 : 0;
+      rr.v3 = 0;
+    }
+    rr.status = ST_OK;
+    rr.kernel_ms = kernel_ms;
+  }
+}
+
+// the batch's identity: every request's circuit, kind and prefix, in order
+std::string batch_key(const std::vector<const ServeRequest*>& reqs) {
+  std::string k;
+  for (const ServeRequest* r : reqs) {
+    k += circuit_fingerprint(*r->circuit);
+    k += '/' + std::to_string(r->kind) + '/';
+    for (uint8_t q : r->quants) k += q ? 'a' : 'e';
+    k += ';';
+  }
+  return k;
+}
+
+}  // namespace
+
+Batcher::Batcher(Context& ctx, const ns_serve_opts& o, bool keep_plans) : ctx_(ctx), keep_plans_(keep_plans) {
   limit_ = o.batch_max > 1 ? std::min<uint32_t>(o.batch_max, 256) : 32;
+  // body generation threads per batch: the host's cores (at most 16), shared among the
+  // server's workers when the batcher serves several
+  const uint32_t hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  gen_threads_ = keep_plans ? hw : std::max(1u, hw / std::max(1u, o.workers ? o.workers : 8u));
   hipDeviceProp_t prop;
   NS_HIP(hipGetDeviceProperties(&prop, ctx.device));
   cus_ = (uint32_t)prop.multiProcessorCount;
+  // the autotuner's candidates per class: persistent blocks per CU (the class's occupancy, or
+  // twice it: the second half waits for free slots) x chunks per resident wave
+  for (int c = 0; c < 4; c++) {
+    for (uint32_t f : {1u, 2u})
+      for (uint32_t cpw : {2u, 4u, 8u, 16u}) tune_[c].cand.push_back({kOccupancy[c] * f, cpw});
+    tune_[c].sum.assign(tune_[c].cand.size(), 0.0);
+    tune_[c].n.assign(tune_[c].cand.size(), 0);
+  }
+  // NS_BATCH_TUNE=0 keeps the default launch shape (no search)
+  if (const char* e = std::getenv("NS_BATCH_TUNE")) fixed_tuning_ = e[0] == '0';
 }
+
+Batcher::~Batcher() = default;
 
 bool Batcher::batchable(const ServeRequest& r) {
   const uint32_t n = r.circuit->n;
@@ -121,14 +284,63 @@ bool Batcher::batchable(const ServeRequest& r) {
   return L >= 5 && n >= 11 + std::min<uint32_t>(2, L - 5);
 }
 
-BatchTuning Batcher::tuning(uint32_t vgpr_class) const {
+BatchTuning Batcher::tuning(uint32_t vgpr_class, uint32_t* idx) {
   std::lock_guard<std::mutex> lk(mu_);
-  return tune_[class_index(vgpr_class)];
+  TuneState& t = tune_[batch_class_index(vgpr_class)];
+  uint32_t k = kDefaultCandidate;
+  if (!fixed_tuning_) {
+    if (t.best >= 0) {
+      k = (uint32_t)t.best;
+    } else {
+      // every candidate measured twice, in turn; then the best mean is kept
+      k = 0;
+      for (uint32_t i = 1; i < t.cand.size(); i++)
+        if (t.n[i] < t.n[k]) k = i;
+    }
+  }
+  if (idx) *idx = k;
+  return t.cand[k];
 }
 
-void Batcher::set_tuning(uint32_t vgpr_class, const BatchTuning& t) {
+void Batcher::report(uint32_t vgpr_class, uint32_t idx, double kernel_ms, double total_cost) {
+  // batches shorter than this say little about the launch shape
+  if (fixed_tuning_ || kernel_ms < 0.2 || total_cost <= 0) return;
   std::lock_guard<std::mutex> lk(mu_);
-  tune_[class_index(vgpr_class)] = t;
+  TuneState& t = tune_[batch_class_index(vgpr_class)];
+  if (t.best >= 0 || idx >= t.cand.size()) return;
+  t.sum[idx] += kernel_ms / total_cost;
+  t.n[idx]++;
+  for (int v : t.n)
+    if (v < 2) return;
+  int best = 0;
+  for (size_t i = 1; i < t.cand.size(); i++)
+    if (t.sum[i] / t.n[i] < t.sum[best] / t.n[best]) best = (int)i;
+  t.best = best;
+}
+
+BatchTuning Batcher::chosen_tuning(uint32_t vgpr_class, bool* settled) const {
+  std::lock_guard<std::mutex> lk(mu_);
+  const TuneState& t = tune_[batch_class_index(vgpr_class)];
+  if (settled) *settled = t.best >= 0;
+  return t.cand[t.best >= 0 ? (size_t)t.best : kDefaultCandidate];
+}
+
+std::shared_ptr<BatchPlan> Batcher::find_plan(const Lane* lane, const std::string& key) {
+  std::lock_guard<std::mutex> lk(plan_mu_);
+  for (auto it = plans_.begin(); it != plans_.end(); ++it)
+    if ((*it)->lane == lane && (*it)->key == key) {
+      auto p = *it;
+      plans_.erase(it);
+      plans_.push_front(p);  // most recent first
+      return p;
+    }
+  return nullptr;
+}
+
+void Batcher::keep_plan(std::shared_ptr<BatchPlan> p) {
+  std::lock_guard<std::mutex> lk(plan_mu_);
+  plans_.push_front(std::move(p));
+  while (plans_.size() > kKeptPlans) plans_.pop_back();
 }
 
 void Batcher::run(Lane& lane, const std::vector<const ServeRequest*>& reqs, std::vector<ns_request_result>& res,
@@ -136,10 +348,24 @@ void Batcher::run(Lane& lane, const std::vector<const ServeRequest*>& reqs, std:
   const auto t0 = clk::now();
   NS_HIP(hipSetDevice(lane.device));
   const size_t K = reqs.size();
-  std::vector<Item> items(K);
-  // code generation: one body per request (the caller's thread)
-  for (size_t i = 0; i < K; i++) {
-    Item& it = items[i];
+  std::string key;
+  if (keep_plans_) {
+    key = batch_key(reqs);
+    if (auto p = find_plan(&lane, key)) {
+      replay(lane, *p, res, bs);
+      if (bs) bs->wall_ms = ms_since(t0);
+      return;
+    }
+  }
+  auto plan = std::make_shared<BatchPlan>();
+  BatchPlan& p = *plan;
+  p.key = key;
+  p.lane = &lane;
+  p.K = K;
+  // code generation: one body per request, on up to gen_threads_ threads
+  std::vector<BatchItem> all(K);
+  auto gen_one = [&](size_t i) {
+    BatchItem& it = all[i];
     const ServeRequest& r = *reqs[i];
     const Circuit& c = *r.circuit;
     it.idx = i;
@@ -148,15 +374,19 @@ void Batcher::run(Lane& lane, const std::vector<const ServeRequest*>& reqs, std:
     it.words = c.n >= 5 ? (1ull << (c.n - 5)) : 1ull;
     if (!batchable(r)) {
       it.fallback = true;
-      continue;
+      return;
     }
     const auto tg = clk::now();
     std::vector<uint32_t> vb(c.n);
     uint32_t lane_shift = 0;
     if (it.qsat) {
-      const uint32_t L = c.n - qsat_first_bits(r.quants);
-      it.jb = std::min<uint32_t>(2, L - 5);
+      it.L = c.n - qsat_first_bits(r.quants);
+      it.jb = std::min<uint32_t>(2, it.L - 5);
       it.m0 = c.n - 5 - it.jb;
+      it.q0all = r.quants[0] != 0;
+      it.quants = r.quants;
+      for (uint32_t l = 0; l < 32 && l < c.n; l++)
+        if (r.quants[c.n - 1 - l]) it.opm |= 1u << l;
       lane_shift = it.jb;
       for (uint32_t k = 0; k < c.n; k++) vb[k] = c.n - 1 - k;
     } else {
@@ -175,186 +405,138 @@ void Batcher::run(Lane& lane, const std::vector<const ServeRequest*>& reqs, std:
       it.fallback = true;
     }
     it.gen_ms = ms_since(tg);
+  };
+  const size_t nth = std::min<size_t>(gen_threads_, K);
+  if (nth <= 1) {
+    for (size_t i = 0; i < K; i++) gen_one(i);
+  } else {
+    std::atomic<size_t> next{0};
+    std::vector<std::exception_ptr> err(nth);
+    std::vector<std::thread> th;
+    for (size_t t = 0; t < nth; t++)
+      th.emplace_back([&, t] {
+        try {
+          for (size_t i; (i = next.fetch_add(1)) < K;) gen_one(i);
+        } catch (...) {
+          err[t] = std::current_exception();
+          next = K;
+        }
+      });
+    for (auto& x : th) x.join();
+    for (auto& e : err)
+      if (e) std::rethrow_exception(e);
   }
-  std::vector<size_t> batched;
-  uint32_t cls = 64;
-  for (size_t i = 0; i < K; i++)
-    if (!items[i].fallback) {
-      batched.push_back(i);
-      cls = std::max(cls, items[i].vgpr);
+  for (const BatchItem& it : all) p.gen_ms += it.gen_ms;
+  p.gen_wall_ms = ms_since(t0);
+  // descriptor order: by kind (SAT, QSAT), then by register class: one group each
+  for (int kind = 0; kind < 2; kind++)
+    for (int ci = 0; ci < 4; ci++) {
+      BatchGroup g;
+      g.qsat = kind == 1;
+      g.ci = ci;
+      g.first = p.items.size();
+      for (BatchItem& it : all)
+        if (!it.fallback && it.qsat == g.qsat && batch_class_index(it.vgpr) == ci) {
+          p.cls = std::max(p.cls, it.vgpr);
+          p.items.push_back(it);
+        }
+      g.count = p.items.size() - g.first;
+      if (g.count) p.groups.push_back(g);
     }
-  double load_ms = 0, kernel_ms = 0;
-  uint64_t code_bytes = 0;
-  if (!batched.empty()) {
+  double kernel_ms = 0;
+  if (!p.items.empty()) {
     std::vector<std::vector<uint32_t>> bodies;
-    for (size_t i : batched) bodies.push_back(items[i].code);
-    auto tpl = batch_template(cls, batch_bytes(bodies));
+    for (const BatchItem& it : p.items) bodies.push_back(it.code);
+    auto tpl = batch_template(batch_bytes(bodies));
     if (!tpl) throw Error(ST_UNSUPPORTED, "the batch's circuit bodies exceed the largest batch region");
     const auto tl = clk::now();
     std::vector<uint64_t> off;
     std::vector<char> elf = compose_batch(*tpl, bodies, off);
-    code_bytes = elf.size();
-    JitModule mod;  // unloaded when the batch is done, after the drain below
-    NS_HIP(hipModuleLoadData(&mod.mod, elf.data()));
+    p.code_bytes = elf.size();
+    NS_HIP(hipModuleLoadData(&p.mod.mod, elf.data()));
     StreamDrain drain{lane.stream};
-    hipFunction_t fsat = nullptr, fqsat = nullptr;
-    NS_HIP(hipModuleGetFunction(&fsat, mod.mod, tpl->sat_kernel.c_str()));
-    NS_HIP(hipModuleGetFunction(&fqsat, mod.mod, tpl->qsat_kernel.c_str()));
-    load_ms = ms_since(tl);
-    // launch shape and chunks: about chunks_per_wave chunks per resident wave over the
-    // batch, each request's chunks of similar cost (words x body cost)
-    const BatchTuning tu = tuning(cls);
-    const uint32_t bpc = tu.blocks_per_cu ? tu.blocks_per_cu : kOccupancy[class_index(cls)];
-    const uint64_t waves = (uint64_t)cus_ * bpc * 4;
-    double total_cost = 0;
-    for (size_t i : batched) total_cost += (double)items[i].words * items[i].cost;
-    const double target = std::max(1.0, total_cost / (double)(waves * std::max<uint32_t>(1, tu.chunks_per_wave)));
-    std::vector<Desc> dsat, dqsat;
-    uint64_t qbytes = 0, pp_bytes = 0;
-    for (size_t b = 0; b < batched.size(); b++) {
-      Item& it = items[batched[b]];
-      const uint64_t tile = 64ull << it.jb;
-      uint64_t cw = (uint64_t)(target / it.cost);
-      cw = (cw + tile - 1) / tile * tile;
-      cw = std::max(cw, tile);
-      cw = std::min<uint64_t>(cw, (it.words + tile - 1) / tile * tile);
-      cw = std::min<uint64_t>(cw, kMaxChunkWords);
-      it.chunk_w = cw;
-      it.chunks = (it.words + cw - 1) / cw;
-      if (it.qsat) {
-        it.bits_off = qbytes;
-        qbytes += ((1ull << it.m0) / 8 + 255) & ~255ull;
-        // ping-pong outputs of the reduction passes: the first is the largest
-        const uint32_t L = it.n - qsat_first_bits(reqs[it.idx]->quants);
-        const uint32_t k = std::min<uint32_t>(L - 5 - it.jb, 11);
-        const uint32_t mo = it.m0 - k;
-        const uint64_t ow = mo >= 5 ? (1ull << (mo - 5)) : 1ull;
-        pp_bytes = std::max(pp_bytes, (ow + 1) * 4);
-      }
+    for (BatchGroup& g : p.groups) {
+      NS_HIP(hipModuleGetFunction(&g.fn, p.mod.mod, tpl->kernel[g.qsat ? 1 : 0][g.ci].c_str()));
+      for (hipEvent_t& e : g.ev) NS_HIP(hipEventCreate(&e));
     }
-    lane.bstats.ensure(K * kStatsBytes);
-    lane.bdesc.ensure(K * sizeof(Desc));
-    if (qbytes) {
-      lane.buf_a.ensure(qbytes);
-      lane.buf_b.ensure(pp_bytes);
-      lane.buf_c.ensure(pp_bytes);
-    }
-    unsigned long long* st = lane.bstats.as<unsigned long long>();
-    uint64_t csat = 0, cqsat = 0;
-    for (size_t b = 0; b < batched.size(); b++) {
-      Item& it = items[batched[b]];
-      const ServeRequest& r = *reqs[it.idx];
-      Desc d{};
-      d.wend = (uint32_t)it.words;
-      d.words = (uint32_t)it.chunk_w;
-      const size_t pc = tpl->pc_base[it.qsat ? 1 : 0];
-      const uint64_t entry = tpl->region + off[b] - pc;
-      if (entry > 0xffffffffull) throw Error(ST_INTERNAL, "batch body entry out of range");
-      d.entry = (uint32_t)entry;
-      d.jb = it.jb;
-      d.valid = c_valid_mask(it.n);
-      if (it.qsat) {
-        d.out = (uint64_t)(lane.buf_a.as<char>() + it.bits_off);
-        for (uint32_t l = 0; l < 32 && l < it.n; l++)
-          if (r.quants[it.n - 1 - l]) d.opm |= 1u << l;
-        d.chunk0 = (uint32_t)cqsat;
-        cqsat += it.chunks;
-        dqsat.push_back(d);
-      } else {
-        d.out = (uint64_t)(st + it.idx * kStatsWords);
-        d.chunk0 = (uint32_t)csat;
-        csat += it.chunks;
-        dsat.push_back(d);
-      }
-    }
-    if (csat > 0xffffffffull || cqsat > 0xffffffffull) throw Error(ST_INTERNAL, "too many batch chunks");
-    Desc* ddev = lane.bdesc.as<Desc>();
-    std::vector<Desc> all = dsat;
-    all.insert(all.end(), dqsat.begin(), dqsat.end());
-    NS_HIP(hipMemcpyAsync(ddev, all.data(), all.size() * sizeof(Desc), hipMemcpyHostToDevice, lane.stream));
-    NS_HIP(launch_fill_u64(lane.stream, st, 0ull, K * kStatsWords));
-    NS_HIP(launch_fill_strided_u64(lane.stream, st + 1, ~0ull, K, kStatsWords));
-    if (csat) lane.bchunk.ensure(csat * 16);
-    NS_HIP(hipEventRecord(lane.ev0, lane.stream));
-    // a persistent grid: every wave resident at once, each taking its chunks round-robin
-    const unsigned grid = (unsigned)(cus_ * bpc);
-    if (!dsat.empty()) {
-      const Desc* dp = ddev;
-      uint32_t nd = (uint32_t)dsat.size(), ch = (uint32_t)csat;
-      unsigned long long* chunk_out = lane.bchunk.as<unsigned long long>();
-      void* args[4] = {&dp, &nd, &ch, &chunk_out};
-      NS_HIP(hipModuleLaunchKernel(fsat, grid, 1, 1, 256, 1, 1, 0, lane.stream, args, nullptr));
-      NS_HIP(launch_batch_fold(lane.stream, reinterpret_cast<const BatchFoldDesc*>(dp), nd, chunk_out));
-    }
-    if (!dqsat.empty()) {
-      const Desc* dp = ddev + dsat.size();
-      uint32_t nd = (uint32_t)dqsat.size(), ch = (uint32_t)cqsat;
-      void* args[3] = {&dp, &nd, &ch};
-      NS_HIP(hipModuleLaunchKernel(fqsat, grid, 1, 1, 256, 1, 1, 0, lane.stream, args, nullptr));
-      // the reduction passes of each QSAT request over its bit array, in prefix order
-      for (size_t i : batched) {
-        const Item& it = items[i];
-        if (!it.qsat) continue;
-        const std::vector<uint8_t>& q = reqs[it.idx]->quants;
-        const uint32_t n = it.n, L = n - qsat_first_bits(q);
-        const bool q0all = q[0] != 0;
-        uint32_t m = it.m0, remaining = L - 5 - it.jb, level = 5 + it.jb;
-        const unsigned int* in = reinterpret_cast<const unsigned int*>(lane.buf_a.as<char>() + it.bits_off);
-        DevBuf* outs[2] = {&lane.buf_b, &lane.buf_c};
-        int which = 0;
-        while (true) {
-          const uint32_t k = remaining < 11 ? remaining : 11;
-          const bool final_pass = k == remaining;
-          const uint32_t mo = m - k;
-          const uint64_t owords = mo >= 5 ? (1ull << (mo - 5)) : 1ull;
-          unsigned int* o = outs[which]->as<unsigned int>();
-          NS_HIP(launch_fill_u32(lane.stream, o, 0u, owords + 1));
-          uint32_t opmask = 0;
-          for (uint32_t l = 0; l < k; l++)
-            if (q[n - 1 - (level + l)]) opmask |= 1u << l;
-          NS_HIP(launch_qsat_reduce(lane.stream, in, m, k, opmask, o, final_pass, q0all, st + it.idx * kStatsWords));
-          if (final_pass) break;
-          in = o;
-          m = mo;
-          remaining -= k;
-          level += k;
-          which ^= 1;
+    p.load_ms = ms_since(tl);
+    for (BatchGroup& g : p.groups) {
+      // launch shape and chunks: about chunks_per_wave chunks per resident wave over the
+      // group, each request's chunks of similar cost (words x body cost)
+      const BatchTuning tu = tuning(kClassOf[g.ci], &g.tune_idx);
+      g.grid = cus_ * tu.blocks_per_cu;
+      const uint64_t waves = (uint64_t)g.grid * 4;
+      for (size_t b = g.first; b < g.first + g.count; b++) g.cost += (double)p.items[b].words * p.items[b].cost;
+      const double target = std::max(1.0, g.cost / (double)(waves * std::max<uint32_t>(1, tu.chunks_per_wave)));
+      if (!g.qsat) g.chunk_base = p.csat;
+      for (size_t b = g.first; b < g.first + g.count; b++) {
+        BatchItem& it = p.items[b];
+        const uint64_t tile = 64ull << it.jb;
+        uint64_t cw = (uint64_t)(target / it.cost);
+        cw = (cw + tile - 1) / tile * tile;
+        cw = std::max(cw, tile);
+        cw = std::min<uint64_t>(cw, (it.words + tile - 1) / tile * tile);
+        cw = std::min<uint64_t>(cw, kMaxChunkWords);
+        it.chunk_w = cw;
+        it.chunks = (it.words + cw - 1) / cw;
+        Desc d{};
+        d.wend = (uint32_t)it.words;
+        d.words = (uint32_t)it.chunk_w;
+        const uint64_t entry = tpl->region + off[b] - tpl->pc_base[it.qsat ? 1 : 0][g.ci];
+        if (entry > 0xffffffffull) throw Error(ST_INTERNAL, "batch body entry out of range");
+        d.entry = (uint32_t)entry;
+        d.jb = it.jb;
+        d.opm = it.opm;
+        d.valid = c_valid_mask(it.n);
+        d.chunk0 = (uint32_t)g.chunks;  // chunks numbered within the group
+        g.chunks += it.chunks;
+        if (it.qsat) {
+          it.bits_off = p.qbytes;
+          p.qbytes += ((1ull << it.m0) / 8 + 255) & ~255ull;
+          // ping-pong outputs of the reduction passes: the first is the largest
+          const uint32_t k = std::min<uint32_t>(it.L - 5 - it.jb, 11);
+          const uint32_t mo = it.m0 - k;
+          const uint64_t ow = mo >= 5 ? (1ull << (mo - 5)) : 1ull;
+          p.pp_bytes = std::max(p.pp_bytes, (ow + 1) * 4);
         }
+        p.descs.push_back(d);
       }
+      if (g.chunks > 0xffffffffull) throw Error(ST_INTERNAL, "too many batch chunks");
+      if (!g.qsat) p.csat += g.chunks;
     }
+    // a first run: the lane's own buffers
+    lane.bstats.ensure(K * kStatsBytes);
+    lane.bdesc.ensure(p.descs.size() * sizeof(Desc));
+    if (p.csat) lane.bchunk.ensure(p.csat * 16);
+    if (p.qbytes) {
+      lane.buf_a.ensure(p.qbytes);
+      lane.buf_b.ensure(p.pp_bytes);
+      lane.buf_c.ensure(p.pp_bytes);
+    }
+    const BatchBufs bufs{lane.bdesc.as<Desc>(), lane.bstats.as<unsigned long long>(),
+                         lane.bchunk.as<unsigned long long>(), lane.buf_a.as<char>(),
+                         {lane.buf_b.as<unsigned int>(), lane.buf_c.as<unsigned int>()}};
+    const std::vector<Desc> bound = bind(p, bufs);
+    NS_HIP(hipMemcpyAsync(bufs.desc, bound.data(), bound.size() * sizeof(Desc), hipMemcpyHostToDevice, lane.stream));
+    NS_HIP(hipEventRecord(lane.ev0, lane.stream));
+    enqueue(p, bufs, lane.stream, true);
     NS_HIP(hipEventRecord(lane.ev1, lane.stream));
     std::vector<unsigned long long> host(K * kStatsWords);
-    NS_HIP(hipMemcpyAsync(host.data(), st, K * kStatsBytes, hipMemcpyDeviceToHost, lane.stream));
+    NS_HIP(hipMemcpyAsync(host.data(), bufs.stats, K * kStatsBytes, hipMemcpyDeviceToHost, lane.stream));
     NS_HIP(hipStreamSynchronize(lane.stream));
     kernel_ms = lane.elapsed_ms();
-    for (size_t b = 0; b < batched.size(); b++) {
-      const Item& it = items[batched[b]];
-      const unsigned long long* h = host.data() + it.idx * kStatsWords;
-      unsigned long long count = h[0];
-      for (uint64_t s = 0; s < 64; s++) count += h[512 + s * 16];
-      const bool hit = count != 0;
-      ns_request_result& rr = res[it.idx];
-      if (it.qsat) {
-        const bool q0all = reqs[it.idx]->quants[0] != 0;
-        rr.v0 = (hit ? 1u : 0u) ^ (q0all ? 1u : 0u);
-        rr.v1 = hit ? h[1] : 0;
-        rr.v2 = count;
-        rr.v3 = hit ? 1 : 0;
-      } else {
-        rr.v0 = count;
-        rr.v1 = hit ? h[1] : 0;
-        rr.v2 = hit ? 1 : 0;
-        rr.v3 = 0;
-      }
-      rr.status = ST_OK;
-      rr.kernel_ms = kernel_ms;
-      rr.compile_ms = it.gen_ms + load_ms / (double)batched.size();
+    for (const BatchGroup& g : p.groups) {
+      float ms = 0.f;
+      NS_HIP(hipEventElapsedTime(&ms, g.ev[0], g.ev[1]));
+      report(kClassOf[g.ci], g.tune_idx, ms, g.cost);
     }
+    decode(p, host, kernel_ms, res);
+    for (const BatchItem& it : p.items) res[it.idx].compile_ms = it.gen_ms + p.load_ms / (double)p.items.size();
   }
   // the requests the batch kernels could not take, one at a time on the lane
   uint32_t fallbacks = 0;
   for (size_t i = 0; i < K; i++) {
-    if (!items[i].fallback) continue;
+    if (!all[i].fallback) continue;
     fallbacks++;
     try {
       solve_alone(lane, *reqs[i], res[i]);
@@ -366,25 +548,82 @@ void Batcher::run(Lane& lane, const std::vector<const ServeRequest*>& reqs, std:
     }
   }
   if (bs) {
+    *bs = BatchStats{};
     bs->requests = (uint32_t)K;
     bs->fallbacks = fallbacks;
-    bs->vgpr_class = batched.empty() ? 0 : cls;
-    bs->code_bytes = code_bytes;
-    bs->load_ms = load_ms;
+    bs->vgpr_class = p.items.empty() ? 0 : p.cls;
+    bs->launches = (uint32_t)p.groups.size();
+    bs->code_bytes = p.code_bytes;
+    bs->gen_ms = p.gen_wall_ms;
+    bs->load_ms = p.load_ms;
     bs->kernel_ms = kernel_ms;
-    bs->gen_ms = 0;
-    for (const Item& it : items) bs->gen_ms += it.gen_ms;
     bs->wall_ms = ms_since(t0);
+  }
+  p.runs = 1;
+  // kept for a repeat when every request ran in the batch kernels
+  if (keep_plans_ && fallbacks == 0 && !p.items.empty()) {
+    for (BatchItem& it : p.items) std::vector<uint32_t>().swap(it.code);  // the module holds the code
+    keep_plan(std::move(plan));
+  }
+}
+
+// A kept batch again: its own buffers and graph (captured on the first repeat, replayed
+// afterwards); no code generation, no module load.
+void Batcher::replay(Lane& lane, BatchPlan& p, std::vector<ns_request_result>& res, BatchStats* bs) {
+  StreamDrain drain{lane.stream};
+  if (!p.graph) {
+    p.d_stats.ensure(p.K * kStatsBytes);
+    p.d_desc.ensure(p.descs.size() * sizeof(Desc));
+    if (p.csat) p.d_chunk.ensure(p.csat * 16);
+    if (p.qbytes) {
+      p.d_bits.ensure(p.qbytes);
+      p.d_pp_a.ensure(p.pp_bytes);
+      p.d_pp_b.ensure(p.pp_bytes);
+    }
+    const BatchBufs bufs{p.d_desc.as<Desc>(), p.d_stats.as<unsigned long long>(), p.d_chunk.as<unsigned long long>(),
+                         p.d_bits.as<char>(), {p.d_pp_a.as<unsigned int>(), p.d_pp_b.as<unsigned int>()}};
+    const std::vector<Desc> bound = bind(p, bufs);
+    NS_HIP(hipMemcpyAsync(bufs.desc, bound.data(), bound.size() * sizeof(Desc), hipMemcpyHostToDevice, lane.stream));
+    NS_HIP(hipStreamSynchronize(lane.stream));
+    hipGraph_t graph = nullptr;
+    NS_HIP(hipStreamBeginCapture(lane.stream, hipStreamCaptureModeThreadLocal));
+    try {
+      enqueue(p, bufs, lane.stream, false);
+    } catch (...) {
+      (void)hipStreamEndCapture(lane.stream, &graph);
+      if (graph) (void)hipGraphDestroy(graph);
+      throw;
+    }
+    NS_HIP(hipStreamEndCapture(lane.stream, &graph));
+    const hipError_t ie = hipGraphInstantiate(&p.graph, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    NS_HIP(ie);
+  }
+  NS_HIP(hipEventRecord(lane.ev0, lane.stream));
+  NS_HIP(hipGraphLaunch(p.graph, lane.stream));
+  NS_HIP(hipEventRecord(lane.ev1, lane.stream));
+  std::vector<unsigned long long> host(p.K * kStatsWords);
+  NS_HIP(hipMemcpyAsync(host.data(), p.d_stats.p, p.K * kStatsBytes, hipMemcpyDeviceToHost, lane.stream));
+  NS_HIP(hipStreamSynchronize(lane.stream));
+  const double kernel_ms = lane.elapsed_ms();
+  decode(p, host, kernel_ms, res);
+  for (const BatchItem& it : p.items) res[it.idx].compile_ms = 0;
+  p.runs++;
+  if (bs) {
+    *bs = BatchStats{};
+    bs->requests = (uint32_t)p.K;
+    bs->vgpr_class = p.cls;
+    bs->launches = (uint32_t)p.groups.size();
+    bs->code_bytes = p.code_bytes;
+    bs->kernel_ms = kernel_ms;
+    bs->graph_runs = p.runs - 1;
   }
 }
 
 void batch_solve(Context& ctx, const std::vector<const ServeRequest*>& reqs, std::vector<ns_request_result>& res,
                  BatchStats* stats) {
-  ns_serve_opts o{};
-  o.batch_max = (uint32_t)std::max<size_t>(2, reqs.size());
-  Batcher b(ctx, o);
   res.assign(reqs.size(), ns_request_result{});
-  b.run(*ctx.main, reqs, res, stats);
+  ctx.batcher().run(*ctx.main, reqs, res, stats);
 }
 
 }  // namespace ns

@@ -468,8 +468,9 @@ int ns_batch_solve(ns_ctx* ctx, const ns_request* reqs, uint32_t n, ns_request_r
     }
     if (n) std::memcpy(results, res.data(), sizeof(ns_request_result) * n);
     if (info) {
-      const double v[8] = {(double)n, (double)bs.fallbacks, (double)bs.vgpr_class, (double)bs.code_bytes,
-                           bs.gen_ms, bs.load_ms, bs.kernel_ms, bs.wall_ms};
+      const double v[10] = {(double)n, (double)bs.fallbacks, (double)bs.vgpr_class, (double)bs.code_bytes,
+                            bs.gen_ms, bs.load_ms, bs.kernel_ms, bs.wall_ms, (double)bs.graph_runs,
+                            (double)bs.launches};
       std::memcpy(info, v, sizeof(v));
     }
   });
@@ -531,7 +532,7 @@ int ns_native_code(const ns_circuit* c, uint32_t kind, uint32_t lane_shift, void
       std::vector<std::vector<uint32_t>> bodies(1);
       std::string err;
       if (!ns::encode_body(nk.body, bodies[0], &err)) throw ns::Error(NS_ERR_UNSUPPORTED, err);
-      auto tpl = ns::batch_template(nk.vgpr_class, ns::batch_bytes(bodies));
+      auto tpl = ns::batch_template(ns::batch_bytes(bodies));
       if (!tpl) throw ns::Error(NS_ERR_UNSUPPORTED, "body exceeds the largest batch region");
       std::vector<uint64_t> bo;
       obj = ns::compose_batch(*tpl, bodies, bo);

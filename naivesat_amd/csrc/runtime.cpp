@@ -1,6 +1,7 @@
 #include "runtime.h"
 
 #include "asmenc.h"
+#include "batch.h"
 #include "static_kernels.h"
 
 #include <cstdio>
@@ -200,6 +201,20 @@ Context::Context(int dev) : device(dev) {
   // the allocation-free SAT entry point's counters: their address on this device
   NS_HIP(noalloc_scratch_setup(dev));
   main = std::make_unique<Lane>(dev, jit);
+}
+
+Context::~Context() {
+  batcher_.reset();  // its kept batches (modules, graphs, buffers) before the lanes and the cache
+}
+
+Batcher& Context::batcher() {
+  std::lock_guard<std::mutex> lk(batcher_mu_);
+  if (!batcher_) {
+    ns_serve_opts o{};
+    o.batch_max = 256;
+    batcher_ = std::make_unique<Batcher>(*this, o, true);
+  }
+  return *batcher_;
 }
 
 Lane::Lane(int dev, Jit& j) : device(dev), jit(j) {

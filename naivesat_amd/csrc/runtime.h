@@ -49,6 +49,7 @@ struct BuiltKernel {
   bool native = false;      // register-allocated assembly in a precompiled template
   uint32_t vgpr_class = 0;  // template register class (native only)
   uint32_t tuned_wpl = 0;   // words per lane chosen by timing (0: not tuned yet)
+  uint32_t tuned_block = 256;  // threads per block chosen with it
 };
 
 // Compiles generated sources once per context; keyed by the full source text. Holds at
@@ -155,11 +156,20 @@ struct Lane {
   }
 };
 
+class Batcher;
+
 struct Context {
   int device = 0;
   Jit jit;
   std::unique_ptr<Lane> main;
   explicit Context(int dev);
+  ~Context();
+  // the context's own batcher (ns_batch_solve): keeps recent batches, made on first use
+  Batcher& batcher();
+
+ private:
+  std::mutex batcher_mu_;
+  std::unique_ptr<Batcher> batcher_;
 };
 
 }  // namespace ns

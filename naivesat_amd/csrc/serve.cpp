@@ -98,7 +98,7 @@ Server::Server(Context& ctx, const ns_serve_opts& o, uint64_t budget_bytes) : ct
   NS_HIP(hipSetDevice(ctx.device));
   arena_ = std::make_unique<Arena>(budget_bytes);
   for (uint32_t w = 0; w < workers; w++) lanes_.push_back(std::make_unique<Lane>(ctx.device, ctx.jit));
-  if (o.batch_max > 1) batcher_ = std::make_unique<Batcher>(ctx, o);
+  if (o.batch_max > 1) batcher_ = std::make_unique<Batcher>(ctx, o, false);
   t0_ = clk::now();
   for (uint32_t w = 0; w < workers; w++) threads_.emplace_back([this, w] { work(w); });
 }

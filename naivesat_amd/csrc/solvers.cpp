@@ -96,6 +96,8 @@ uint32_t word_valid_mask(uint32_t n) { return n >= 5 ? 0xffffffffu : (uint32_t)(
 
 using Chosen = BuiltKernel;
 
+constexpr uint32_t kTuneCandidates = 10;  // launch geometries the SAT autotuner times
+
 // Search counters in ctx.stats: [0] count, [1] lowest hit, and from u64 index 512 on 64
 // count slots 16 words apart that the search kernels' waves add to (by block), so that
 // many hits do not queue on one word. The count is stats[0] plus the slots.
@@ -127,23 +129,6 @@ void read_counts_done(const CountsCopy& cc, unsigned long long host[2]) {
   host[1] = cc.head[1];
 }
 
-// 128-bit fingerprint of a circuit's content (two independent 64-bit hashes)
-std::string circuit_fingerprint(const Circuit& c) {
-  uint64_t h1 = 0xcbf29ce484222325ull, h2 = 0x9e3779b97f4a7c15ull;
-  auto mix = [&](uint64_t v) {
-    h1 = (h1 ^ v) * 0x100000001b3ull;
-    h2 ^= v + 0x9e3779b97f4a7c15ull + (h2 << 6) + (h2 >> 2);
-    h2 *= 0xff51afd7ed558ccdull;
-  };
-  mix(c.n);
-  mix(c.gates.size());
-  for (const Gate& g : c.gates) mix(((uint64_t)g.i0 << 34) | ((uint64_t)g.i1 << 2) | g.func);
-  mix(c.outputs.size());
-  for (const Output& o : c.outputs) mix(((uint64_t)o.wire << 1) | (o.neg ? 1u : 0u));
-  char buf[40];
-  std::snprintf(buf, sizeof(buf), "%016llx%016llx", (unsigned long long)h1, (unsigned long long)h2);
-  return buf;
-}
 
 // Kernel for a circuit: register-allocated assembly in a precompiled template when the body
 // fits one, else generated HIP compiled by hiprtc. Cached per context under the circuit's
@@ -193,6 +178,24 @@ Chosen choose_kernel(Lane& ctx, const Circuit& c, NativeKind kind, const std::ve
 
 }  // namespace
 
+// 128-bit fingerprint of a circuit's content (two independent 64-bit hashes)
+std::string circuit_fingerprint(const Circuit& c) {
+  uint64_t h1 = 0xcbf29ce484222325ull, h2 = 0x9e3779b97f4a7c15ull;
+  auto mix = [&](uint64_t v) {
+    h1 = (h1 ^ v) * 0x100000001b3ull;
+    h2 ^= v + 0x9e3779b97f4a7c15ull + (h2 << 6) + (h2 >> 2);
+    h2 *= 0xff51afd7ed558ccdull;
+  };
+  mix(c.n);
+  mix(c.gates.size());
+  for (const Gate& g : c.gates) mix(((uint64_t)g.i0 << 34) | ((uint64_t)g.i1 << 2) | g.func);
+  mix(c.outputs.size());
+  for (const Output& o : c.outputs) mix(((uint64_t)o.wire << 1) | (o.neg ? 1u : 0u));
+  char buf[40];
+  std::snprintf(buf, sizeof(buf), "%016llx%016llx", (unsigned long long)h1, (unsigned long long)h2);
+  return buf;
+}
+
 // ---------------------------------------------------------------------------
 // SAT
 // ---------------------------------------------------------------------------
@@ -210,14 +213,19 @@ void sat_solve(Lane& ctx, const Circuit& c, const ns_sat_opts& o, ns_sat_result&
   Chosen ch = choose_kernel(
       ctx, c, NativeKind::SAT, vb, 0, {0}, g, "",
       [&](GenInfo* info) { return gen_sat_kernel(c, g, info); }, "ns_sat", &r.compile_ms, &fp);
-  // Words per lane is a run-time argument of the native kernel: a long search times its
-  // first launches with 1..16 words per lane and keeps the fastest (cached with the kernel).
-  // Every word is still evaluated exactly once, so the answers do not depend on it.
+  // Words per lane and threads per block are run-time parameters of the native kernel: a long
+  // search times its first launches with each candidate pair and keeps the fastest (cached
+  // with the kernel). Every word is still evaluated exactly once, so the answers do not
+  // depend on them.
   const uint64_t cw_tune = chunk_words(o.elem_bits, g, total_words);
-  const bool tune = ch.native && o.words_per_lane == 0 && (o.flags & NS_SAT_STOP_AT_FIRST) == 0 &&
-                    total_words >= 16 * cw_tune && cw_tune % ((uint64_t)g.block * 16) == 0;
-  if (ch.native && ch.tuned_wpl && o.words_per_lane == 0 && cw_tune % ((uint64_t)g.block * ch.tuned_wpl) == 0)
+  const bool tune = ch.native && o.words_per_lane == 0 && o.group_len == 0 &&
+                    (o.flags & NS_SAT_STOP_AT_FIRST) == 0 && total_words >= 16 * cw_tune &&
+                    cw_tune % ((uint64_t)g.block * 16) == 0;
+  if (ch.native && ch.tuned_wpl && o.words_per_lane == 0 && o.group_len == 0 &&
+      cw_tune % ((uint64_t)ch.tuned_block * ch.tuned_wpl) == 0) {
     g.wpl = ch.tuned_wpl;
+    g.block = ch.tuned_block;
+  }
   auto mod = ch.mod;
   r.cone_gates = ch.info.cone_gates;
   r.valu_ops = ch.info.valu_ops;
@@ -278,33 +286,39 @@ void sat_solve(Lane& ctx, const Circuit& c, const ns_sat_opts& o, ns_sat_result&
   reset_counts(ctx);
   NS_HIP(hipEventRecord(ctx.ev0, ctx.stream));
   if (tune && !ch.tuned_wpl) {
-    static const uint32_t cand[5] = {1, 2, 4, 8, 16};
-    hipEvent_t ev[6];
+    // the autotuner: each (threads per block, words per lane) candidate runs one launch of
+    // the search's own word ranges, timed with events; the fastest is kept with the kernel
+    static const uint32_t cand[kTuneCandidates][2] = {{256, 1}, {256, 2}, {256, 4}, {256, 8}, {256, 16},
+                                                      {128, 2}, {128, 4}, {128, 8}, {64, 4},  {64, 8}};
+    hipEvent_t ev[kTuneCandidates + 1];
     for (auto& e : ev) NS_HIP(hipEventCreate(&e));
     NS_HIP(hipEventRecord(ev[0], ctx.stream));
-    for (int k = 0; k < 5; k++) {
+    for (uint32_t k = 0; k < kTuneCandidates; k++) {
       Geometry gk = g;
-      gk.wpl = cand[k];
-      uint32_t wk = cand[k];
+      gk.block = cand[k][0];
+      gk.wpl = cand[k][1];
+      uint32_t wk = gk.wpl;
       void* extra[3] = {&res, &wk, &valid};
       launch_range(ctx, mod->fn, gk, w0, cw, extra, 3, total_words);
       NS_HIP(hipEventRecord(ev[k + 1], ctx.stream));
       r.launches++;
       w0 += cw;
     }
-    NS_HIP(hipEventSynchronize(ev[5]));
+    NS_HIP(hipEventSynchronize(ev[kTuneCandidates]));
     float best = 0.f;
-    for (int k = 0; k < 5; k++) {
+    for (uint32_t k = 0; k < kTuneCandidates; k++) {
       float ms = 0.f;
       NS_HIP(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
       if (k == 0 || ms < best) {
         best = ms;
-        g.wpl = cand[k];
+        g.block = cand[k][0];
+        g.wpl = cand[k][1];
       }
     }
     for (auto& e : ev) (void)hipEventDestroy(e);
     wpl = g.wpl;
     ch.tuned_wpl = g.wpl;
+    ch.tuned_block = g.block;
     ctx.jit.put_built(fp, ch);
   }
   while (w0 < total_words) {
@@ -358,8 +372,11 @@ void sat_solve_noalloc(Lane& ctx, const Circuit& c, const ns_sat_opts& o, ns_sat
       ctx, c, NativeKind::SAT, vb, 0, {0}, g, "", [&](GenInfo* info) { return gen_sat_kernel(c, g, info); },
       "ns_sat", &r.compile_ms, nullptr, true);
   const uint64_t cw = chunk_words(o.elem_bits, g, total_words);
-  if (ch.native && ch.tuned_wpl && o.words_per_lane == 0 && cw % ((uint64_t)g.block * ch.tuned_wpl) == 0)
+  if (ch.native && ch.tuned_wpl && o.words_per_lane == 0 && o.group_len == 0 &&
+      cw % ((uint64_t)ch.tuned_block * ch.tuned_wpl) == 0) {
     g.wpl = ch.tuned_wpl;
+    g.block = ch.tuned_block;
+  }
   r.cone_gates = ch.info.cone_gates;
   r.valu_ops = ch.info.valu_ops;
   r.scalar_ops = ch.info.scalar_ops;

@@ -9,6 +9,9 @@
 
 namespace ns {
 
+// 128-bit fingerprint of a circuit's content (hex): the kernel caches' key
+std::string circuit_fingerprint(const Circuit& c);
+
 void sat_solve(Lane& ctx, const Circuit& c, const ns_sat_opts& o, ns_sat_result& r);
 // sat_solve without any allocation: the kernel must be in the context's cache (built by an
 // earlier sat_solve with the same options), counters in `scratch` (device address)

@@ -13,8 +13,8 @@ for path in sys.argv[2:]:
     text = open(path).read()
     # the batch templates call their bodies (s_swappc_b64 into the region appended at run
     # time); the others hold one body placeholder
-    batch = name.startswith("ns_nat_batch_")
-    ok = text.count("s_swappc_b64 s[38:39], s[36:37]") == 2 if batch else text.count("; NS_BODY " + name) == 1
+    batch = name == "ns_nat_batch"  # 2 kernels x 4 register classes
+    ok = text.count("s_swappc_b64 s[38:39], s[36:37]") == 8 if batch else text.count("; NS_BODY " + name) == 1
     if ")NSTPL" in text or not ok:
         sys.exit(f"{path}: placeholder missing or text unusable")
     out.append(f'  {{"{name}", R"NSTPL({text})NSTPL"}},')

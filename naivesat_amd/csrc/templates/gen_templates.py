@@ -298,7 +298,7 @@ def batch_call(cls, lo, hi):
             f's_swappc_b64 s[38:39], s[36:37]" : "={{v0}}"(o) : {ins} : {clob});\n')
 
 
-def batch_kernels(cls, pool_hi):
+def batch_kernels(cls, pool_hi, ci):
     common = """  const u32 lane = threadIdx.x & 63u;
   const u32 LM0 = 0u - (lane & 1u), LM1 = 0u - ((lane >> 1) & 1u);
   const u32 LM2 = 0u - ((lane >> 2) & 1u), LM3 = 0u - ((lane >> 3) & 1u);
@@ -334,7 +334,7 @@ def batch_kernels(cls, pool_hi):
 extern "C" __global__ void __launch_bounds__(256) ns_nat_bsat_{cls}(const NsBatchDesc* descs, u32 nd, u32 chunks,
                                                       u64* chunk_out) {{
   u64 pcb;
-  asm volatile("s_getpc_b64 %0\\n\\ts_mov_b32 s36, 0x53425342" : "=s"(pcb) : : "s36");
+  asm volatile("s_getpc_b64 %0\\n\\ts_mov_b32 s36, 0x{0x534253c0 + ci:08x}" : "=s"(pcb) : : "s36");
   const u32 pclo = __builtin_amdgcn_readfirstlane((u32)pcb), pchi = __builtin_amdgcn_readfirstlane((u32)(pcb >> 32));
 {common}    const u32 valid = __builtin_amdgcn_readfirstlane(d->valid);
     u32 cnt = 0u;
@@ -362,7 +362,7 @@ extern "C" __global__ void __launch_bounds__(256) ns_nat_bsat_{cls}(const NsBatc
 // the request's bit array (bit array index = tile index), stored by every lane.
 extern "C" __global__ void __launch_bounds__(256) ns_nat_bqsat_{cls}(const NsBatchDesc* descs, u32 nd, u32 chunks) {{
   u64 pcb;
-  asm volatile("s_getpc_b64 %0\\n\\ts_mov_b32 s36, 0x51425342" : "=s"(pcb) : : "s36");
+  asm volatile("s_getpc_b64 %0\\n\\ts_mov_b32 s36, 0x{0x514253c0 + ci:08x}" : "=s"(pcb) : : "s36");
   const u32 pclo = __builtin_amdgcn_readfirstlane((u32)pcb), pchi = __builtin_amdgcn_readfirstlane((u32)(pcb >> 32));
 {common}    const u32 jb = __builtin_amdgcn_readfirstlane(d->jb), opm = __builtin_amdgcn_readfirstlane(d->opm);
     const u32 tw = 64u << jb;
@@ -387,14 +387,19 @@ extern "C" __global__ void __launch_bounds__(256) ns_nat_bqsat_{cls}(const NsBat
 
 def main():
     out = [HEAD, BATCH_HEAD]
-    for cls, (budget, hi) in CLASSES.items():
+    batch = []
+    for ci, (cls, (budget, hi)) in enumerate(CLASSES.items()):
         kernels = [("sat", sat_kernel(cls, budget, hi)), ("qsata", qsat_a_kernel(cls, hi)),
-                   ("qsatb", qsat_b_kernel(cls, hi)), ("batch", batch_kernels(cls, hi))]
+                   ("qsatb", qsat_b_kernel(cls, hi))]
         if budget >= 128:
             kernels.append(("table", table_kernel(cls, hi)))
         for kind, src in kernels:
             # one kernel per compilation: NS_T_<name> selects it
             out.append(f"#if defined(NS_T_ns_nat_{kind}_{cls})\n{src}#endif\n")
+        batch.append(batch_kernels(cls, hi, ci))
+    # the batch kernels of every class in one compilation (NS_T_ns_nat_batch): a batch's code
+    # object holds them all over one body region, so each class runs at its own occupancy
+    out.append("#if defined(NS_T_ns_nat_batch)\n" + "".join(batch) + "#endif\n")
     open(sys.argv[1], "w").write("".join(out))
 
 

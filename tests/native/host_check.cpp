@@ -4,8 +4,10 @@
 // encoding, template patching and batch code objects, for every kernel kind, on random and
 // workload circuits. No device is touched. Exit status 0 and "host_check ok" when every
 // step ran; sanitizer reports make the process fail.
+#include <atomic>
 #include <cstdio>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../naivesat_amd/csrc/asmenc.h"
@@ -16,7 +18,7 @@ using namespace ns;
 
 namespace {
 
-int g_fail = 0;
+std::atomic<int> g_fail{0};
 
 void check(bool ok, const char* what) {
   if (!ok) {
@@ -46,7 +48,7 @@ void run(const Circuit& c, NativeKind kind, uint32_t lane_shift, bool full) {
   if (!full) return;
   if (kind == NativeKind::BATCH) {
     std::vector<std::vector<uint32_t>> bodies{words, words};
-    auto t = batch_template(nk.vgpr_class, batch_bytes(bodies));
+    auto t = batch_template(batch_bytes(bodies));
     check(t != nullptr, "no batch template");
     if (t) {
       std::vector<uint64_t> off;
@@ -92,6 +94,15 @@ int main() {
   run(gen_satostop_circuit(4, 15, 6, 300, 14, &target), NativeKind::TABLE, 0, true);
   // the 36-input, 20,000-gate workload: output groups refined on threads
   run(gen_sat_circuit(2, 36, 20000, 40, &key), NativeKind::SAT, 0, false);
+  // batch bodies generated concurrently, as a batch's thread pool does
+  std::vector<Circuit> cs;
+  for (int i = 0; i < 16; i++) cs.push_back(gen_sat_circuit(100 + i, 16 + i, 300 + 200 * i, 16, &key));
+  std::vector<std::thread> th;
+  for (int t = 0; t < 4; t++)
+    th.emplace_back([&, t] {
+      for (size_t i = t; i < cs.size(); i += 4) run(cs[i], NativeKind::BATCH, 0, i < 4);
+    });
+  for (auto& x : th) x.join();
   if (!g_fail) std::printf("host_check ok\n");
   return g_fail;
 }

@@ -18,7 +18,8 @@ from helpers import random_circuit
 OBJDUMP = "/opt/rocm/llvm/bin/llvm-objdump"
 S_SETPC_RET = 0xBE801D26  # s_setpc_b64 s[38:39]
 S_MOV_S36_LIT = 0xBEA400FF  # s_mov_b32 s36, <literal>
-MARK_SAT, MARK_QSAT = 0x53425342, 0x51425342
+MARK_SAT, MARK_QSAT = 0x534253C0, 0x514253C0  # + register class index (64, 128, 256, 512)
+CLASSES = (64, 128, 256, 512)
 
 
 def words_of(code: bytes):
@@ -64,7 +65,10 @@ def check_batch_code(c, kind, lane_shift=0):
     W = words_of(code)
     text, loads = elf_layout(code)
     t0, t1 = text[0], text[0] + text[1]
-    ps, pq = pc_base(W, MARK_SAT), pc_base(W, MARK_QSAT)
+    ci = CLASSES.index(cls)
+    ps, pq = pc_base(W, MARK_SAT + ci), pc_base(W, MARK_QSAT + ci)
+    for other in range(4):  # every class's kernels sit before the region
+        assert pc_base(W, MARK_SAT + other) < start and pc_base(W, MARK_QSAT + other) < start
     # both pc bases, the body and its return lie in .text, in one executable load segment
     for off in (ps, pq, start, start + size):
         assert t0 <= off < t1
@@ -82,14 +86,14 @@ def check_batch_code(c, kind, lane_shift=0):
 
 
 def test_kernels_call_through_the_pc_base(tmp_path):
-    """The disassembled kernels add the descriptor's entry to the very registers s_getpc_b64
-    wrote, and call s[36:37] with the return address in s[38:39]."""
+    """The disassembled kernels (every register class) add the descriptor's entry to the very
+    registers s_getpc_b64 wrote, and call s[36:37] with the return address in s[38:39]."""
     c, _ = ns.Circuit.gen_sat(1, 24, 2000, 32)
     code, cls, ps, pq, start = check_batch_code(c, 4)
     f = tmp_path / "batch.co"
     f.write_bytes(code)
     dis = subprocess.run([OBJDUMP, "-d", "--mcpu=gfx950", str(f)], capture_output=True, text=True, check=True).stdout
-    for kern in (f"ns_nat_bsat_r{cls}", f"ns_nat_bqsat_r{cls}"):
+    for kern in [f"ns_nat_b{k}_r{c}" for c in CLASSES for k in ("sat", "qsat")]:
         # the kernel's instructions up to its s_endpgm (the region follows the last kernel)
         m = re.search(rf"<{kern}>:\n(.*?s_endpgm)", dis, re.S)
         assert m, kern

@@ -72,14 +72,14 @@ def test_batch_matches_alone_trace_prefix(ctx):
 
 
 def test_batch_mixed_register_classes(ctx):
-    # one circuit whose body needs the 256-register class beside small ones: the whole batch
-    # runs in that class, the small bodies use a subset of its registers
+    # one circuit whose body needs a larger register class beside small ones: each class's
+    # requests run in their own class's kernel, launched at that class's occupancy
     reqs = random_requests(77, 6, 14, 20)
     big, _ = ns.Circuit.gen_sat(2, 26, 8000, 40)
     reqs.append({"kind": ns.REQ_SAT, "circuit": big, "n": 26})
     res, info = ctx.batch_solve(reqs)
     print("batch info", info, flush=True)
-    assert info["vgpr_class"] >= 128
+    assert info["vgpr_class"] >= 128 and info["launches"] >= 2
     for i, (rq, r) in enumerate(zip(reqs, res)):
         assert r["status"] == 0
         assert (r["v0"], r["v1"], r["v2"], r["v3"]) == alone(ctx, rq), i
@@ -118,3 +118,34 @@ def test_served_batches_equal_alone(ctx, workers):
     for i, (rq, r) in enumerate(zip(reqs, res)):
         assert r["status"] == 0, (i, r)
         assert (r["v0"], r["v1"], r["v2"], r["v3"]) == alone(ctx, rq), (i, rq["kind"], rq["n"])
+
+
+def test_repeated_batch_replays_its_graph(ctx):
+    """The same batch again reuses its code object and runs as a HIP graph (captured on the
+    first repeat, replayed after): every replay, with the program's own eager solves in
+    between, gives the eager run's answers."""
+    reqs = random_requests(123, 12, 13, 20)
+    first, info0 = ctx.batch_solve(reqs)
+    assert info0["graph_runs"] == 0 and info0["fallbacks"] == 0
+    ref = [(r["v0"], r["v1"], r["v2"], r["v3"]) for r in first]
+    for rep in range(1, 5):
+        c, _ = ns.Circuit.gen_sat(500 + rep, 18, 200, 16)
+        ctx.sat_solve(c)  # eager work on the same stream between replays
+        res, info = ctx.batch_solve(reqs)
+        assert info["graph_runs"] == rep and info["gen_ms"] == 0 and info["load_ms"] == 0
+        assert [(r["v0"], r["v1"], r["v2"], r["v3"]) for r in res] == ref, rep
+    for rq, r in zip(reqs, ref):
+        assert r == oracle_answer(rq)
+
+
+def test_tuned_launch_shapes_keep_answers(ctx):
+    """Batches of one register class run under every candidate launch shape of the autotuner
+    (blocks per CU x chunks per wave) while it explores: the answers never change."""
+    reqs = random_requests(321, 10, 28, 30)  # about a millisecond of device time per batch
+    ref = None
+    for rep in range(18):
+        # ten rotations of the list: new batches each time (more than the four kept)
+        res, _ = ctx.batch_solve(reqs[rep % 10:] + reqs[: rep % 10])
+        got = sorted((r["v0"], r["v1"], r["v2"], r["v3"]) for r in res)
+        ref = ref or got
+        assert got == ref, rep
