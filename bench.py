@@ -59,6 +59,19 @@ def measured_traffic():
     return out
 
 
+def measured_valu_insts():
+    """kernel -> wave-level VALU instructions per launch (rocprofv3 SQ_INSTS_VALU), from
+    profiles/sq_insts.txt (lines: kernel workload count)"""
+    path = os.path.join(ROOT, "profiles", "sq_insts.txt")
+    out = {}
+    if os.path.exists(path):
+        for line in open(path):
+            t = line.split()
+            if len(t) == 3 and not line.startswith("#"):
+                out[(t[0], t[1])] = float(t[2])
+    return out
+
+
 def sat_part(ns, ctx, steps, warmup):
     c, key = ns.Circuit.gen_sat(2, 36, 20000, 40)
     cone, peak_live = c.cone()
@@ -87,6 +100,11 @@ def sat_part(ns, ctx, steps, warmup):
     peak = VALU_PEAK_OPS / 1e12
     kernel = f"ns_nat_sat_r{r['vgpr_class']}" if r["backend"] else "ns_sat"
     traffic = measured_traffic().get((kernel, "sat_n36"))
+    # issued VALU work: the recorded wave-level VALU instruction count of one launch (64 lanes
+    # each) over the live launch time, against the same lane-op peak
+    insts = measured_valu_insts().get((kernel, "sat_n36"))
+    issued = None if insts is None else insts * 64 / (avg_launch_ms / 1000.0) / VALU_PEAK_OPS
+    lut_ops_per_s = words_per_launch * r["valu_ops"] / (avg_launch_ms / 1000.0)
     return c, cone, {
         "ms_per_step": ms_per_step, "value": value,
         "config": {"workload": SAT_WORKLOAD, "cone_gates": cone, "gates_nominal": 20000,
@@ -97,8 +115,14 @@ def sat_part(ns, ctx, steps, warmup):
                    "vgpr_class": r["vgpr_class"], "compile_ms_first": first_compile_ms},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TOP/s",
                      "frac": achieved / peak, "traffic": traffic, "kernel": kernel,
+                     "traffic_source": "profiles/traffic.txt: rocprofv3 FETCH_SIZE x 1.99993 + WRITE_SIZE x 1.0 "
+                                       "per dispatch (copy-kernel calibration, profiles/hbm_counters.md)"
+                     if traffic is not None else None,
                      "avg_launch_ms": avg_launch_ms, "ops_per_launch": ops_per_launch,
-                     "lut_ops_per_s": words_per_launch * r["valu_ops"] / (avg_launch_ms / 1000.0)},
+                     "lut_ops_per_s": lut_ops_per_s, "lut_frac": lut_ops_per_s / VALU_PEAK_OPS,
+                     "valu_issue_frac": issued,
+                     "valu_issue_source": "profiles/sq_insts.txt: rocprofv3 SQ_INSTS_VALU per dispatch x 64 lanes"
+                     if issued is not None else None},
     }
 
 
@@ -111,7 +135,9 @@ def cpu_part(c, cone, threads):
     oc.sat_range(0, words, threads=threads)
     tc = time.perf_counter() - tc0
     return {"value": (words * 64) * cone / tc, "unit": "gate-evals/s", "cores": threads, "kind": "port",
-            "sample": f"first 2^31 of the 2^36 assignments, {tc:.1f} s"}
+            "sample": f"first 2^31 of the 2^36 assignments, {tc:.1f} s", "cpu_model": oracle.cpu_model(),
+            "isa": oracle.simd_isa(), "implementation": "oracle/oracle.c: C, bit-sliced 8 x 64-bit words per gate, "
+                                                          "pthreads over the assignment range"}
 
 
 def table_part(ns, ctx):

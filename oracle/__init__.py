@@ -45,6 +45,8 @@ def lib() -> C.CDLL:
         P, U32, U64, I = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int
         L.or_eval_u64.restype = U64
         L.or_eval_u64.argtypes = [C.POINTER(_Circ), U64]
+        L.or_simd_isa.restype = C.c_char_p
+        L.or_simd_isa.argtypes = []
         L.or_sat_count.restype = I
         L.or_sat_count.argtypes = [C.POINTER(_Circ), I, C.POINTER(U64), C.POINTER(U64)]
         L.or_sat_count_range.restype = I
@@ -88,6 +90,21 @@ def lib() -> C.CDLL:
                                 U32, I]
         _lib = L
     return _lib
+
+
+def simd_isa() -> str:
+    """the vector ISA of the oracle's bit-sliced SAT evaluation on this CPU"""
+    return lib().or_simd_isa().decode()
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 class OracleCircuit:

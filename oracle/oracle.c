@@ -105,9 +105,11 @@ typedef struct {
   uint64_t count, lowest;
 } sat_job;
 
-/* 8 consecutive words at once (the inner loops vectorise): val[wire * 8 + i] */
+/* 8 consecutive words at once (the inner loops vectorise): val[wire * 8 + i]; cloned for
+   AVX-512 (one 512-bit op per gate) and the AVX2 build default, picked at load time by the
+   CPU's features */
 #define OR_BW 8
-static void eval_block8(const or_circuit* c, const uint32_t* cone, uint32_t ncone, uint64_t w0, uint64_t* val) {
+__attribute__((target_clones("avx512f", "default"))) static void eval_block8(const or_circuit* c, const uint32_t* cone, uint32_t ncone, uint64_t w0, uint64_t* val) {
   for (uint32_t k = 0; k < c->n; k++)
     for (int i = 0; i < OR_BW; i++)
       val[k * OR_BW + i] = k < 6 ? kMask64[k] : ((((w0 + (uint64_t)i) >> (k - 6)) & 1) ? ~0ull : 0ull);
@@ -158,6 +160,11 @@ static void* sat_worker(void* arg) {
   free(val);
   free(vbit);
   return NULL;
+}
+
+const char* or_simd_isa(void) {
+  __builtin_cpu_init();
+  return __builtin_cpu_supports("avx512f") ? "avx512f" : "avx2";
 }
 
 int or_sat_count(const or_circuit* c, int threads, uint64_t* count, uint64_t* lowest) {

@@ -49,6 +49,8 @@ uint64_t or_eval_u64(const or_circuit* c, uint64_t x);
 
 /* satisfying assignments of output 0 over all 2^n x, and the lowest one (~0 if none) */
 int or_sat_count(const or_circuit* c, int threads, uint64_t* count, uint64_t* lowest);
+/* the vector ISA the bit-sliced SAT evaluation runs with on this CPU ("avx512f" or "avx2") */
+const char* or_simd_isa(void);
 /* same over 64-assignment words [w_lo, w_hi) (assignment x = 64*w + b) */
 int or_sat_count_range(const or_circuit* c, uint64_t w_lo, uint64_t w_hi, int threads, uint64_t* count,
                        uint64_t* lowest);
