@@ -288,6 +288,9 @@ def batch_repeat_part(ns, n_requests=64, repeats=5):
         rep_kernel.append(info["kernel_ms"])
         same &= [(r["v0"], r["v1"], r["v2"], r["v3"]) for r in res] == alone
     ctx.close()
+    # algorithmic work of the batch: one op per cone gate per 32-assignment word, every request
+    gate_ops = sum(float(1 << max(0, rq["circuit"].n_inputs - 5)) * rq["circuit"].cone()[0] for rq in reqs)
+    rep_rate = gate_ops / (min(rep_kernel) / 1e3)
     out = {"workload": f"{len(reqs)} SAT/QSAT requests of the served stream (seed 5)", "answers_equal": same,
            "one_at_a_time_ms": alone_ms, "batch_first_ms": first_ms, "batch_first_kernel_ms": info0["kernel_ms"],
            "batch_first_gen_ms": info0["gen_ms"], "batch_first_load_ms": info0["load_ms"],
@@ -295,7 +298,12 @@ def batch_repeat_part(ns, n_requests=64, repeats=5):
            "graph_replay_kernel_ms": min(rep_kernel), "graph_runs": info["graph_runs"],
            "solves_per_s_one_at_a_time": len(reqs) / (alone_ms / 1e3),
            "solves_per_s_batch_first": len(reqs) / (first_ms / 1e3),
-           "solves_per_s_graph_replay": len(reqs) / (min(rep_ms) / 1e3)}
+           "solves_per_s_graph_replay": len(reqs) / (min(rep_ms) / 1e3),
+           "roofline": {"bound": "valu", "kernel": "ns_nat_bsat_r* / ns_nat_bqsat_r* (one launch per kind and "
+                                                   "register class)",
+                        "gate_ops_per_batch": gate_ops, "achieved": rep_rate / 1e12, "peak": VALU_PEAK_OPS / 1e12,
+                        "unit": "TOP/s", "frac": rep_rate / VALU_PEAK_OPS,
+                        "note": "graph replay's kernel time (HIP events around the batch's launches)"}}
     print(f"batch_repeat: {json.dumps(out)}", file=sys.stderr, flush=True)
     return out
 

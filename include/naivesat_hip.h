@@ -27,6 +27,7 @@ extern "C" {
 #define NS_ERR_UNSUPPORTED 4 /* size or device outside what the engine handles */
 #define NS_ERR_NOMEM 5       /* device or host allocation failed */
 #define NS_ERR_INTERNAL 6
+#define NS_ERR_CANCELLED 7 /* a progress callback stopped a search that has no partial answer */
 
 /* gate functions (gatesim's gate set) */
 #define NS_FUNC_AND 0   /* a & b    */
@@ -78,6 +79,10 @@ int ns_gen_satostop(uint64_t seed, uint32_t half, uint32_t rounds, uint32_t roun
 
 /* ---- device context (replaces OpenCLBuilder::new(&device, cfg) + its command queue,
  *      src/main.rs:292-301) ---- */
+/* The HIP devices (replaces get_all_devices(CL_DEVICE_TYPE_GPU), src/main.rs:385-388, that
+ * the tools list first): *count devices, their gfx architecture names comma-separated in
+ * names (NUL-terminated, cut to cap bytes). No device: count 0. */
+int ns_device_list(uint32_t* count, char* names, size_t cap);
 int ns_ctx_create(int device, ns_ctx** out);
 void ns_ctx_destroy(ns_ctx* ctx);
 /* Releases the context's SATOSTOP tables and images (device memory kept between solves so a
@@ -112,6 +117,19 @@ typedef struct {
   double wall_ms;
 } ns_sat_result;
 int ns_sat_solve(ns_ctx* ctx, const ns_circuit* c, const ns_sat_opts* opts, ns_sat_result* res);
+/* ns_sat_solve with progress and cancellation (replaces execute_direct's per-arg callback and
+ * stop predicate, src/main.rs:178-196: the reference prints "Step: arg / arg_steps" from it).
+ * The search runs as `steps` launch ranges of 2^range_bits assignments each, range k covering
+ * assignments [k << range_bits, (k + 1) << range_bits); elem_bits in opts sets range_bits
+ * exactly (from 11 to 40; 0 = the engine default, 2^32). fn is called on the calling thread
+ * once with step = NS_SAT_STEP_START when the kernel is built and nothing is launched yet
+ * (the reference's "Start execution"), then once per range, in order, when it has completed.
+ * A nonzero return stops the search after the ranges already launched: the result covers
+ * the first res->launches ranges and res->complete is 0. Answers equal ns_sat_solve's. */
+#define NS_SAT_STEP_START 0xFFFFFFFFFFFFFFFFull
+typedef int (*ns_sat_progress_fn)(void* user, uint64_t step, uint64_t steps, uint32_t range_bits);
+int ns_sat_solve_progress(ns_ctx* ctx, const ns_circuit* c, const ns_sat_opts* opts, ns_sat_progress_fn fn,
+                          void* user, ns_sat_result* res);
 /* No counterpart in the reference: ns_sat_solve for callers that cannot allocate (code under
  * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). It makes no
  * device or host allocation, loads no module and builds no graph: the circuit's kernel must
@@ -157,6 +175,13 @@ typedef struct {
 int ns_qsat_solve(ns_ctx* ctx, const ns_circuit* c, const uint8_t* quants_all,
                   const ns_qsat_opts* opts, ns_qsat_result* res, uint32_t* bitmap,
                   uint64_t bitmap_words);
+/* ns_qsat_solve with per-range progress (the reference's "Step: arg / arg_steps" callback of
+ * the QSAT mapper, src/qsat.rs:1634-1654,1701-1718): fn as for ns_sat_solve_progress, over
+ * the circuit kernel's launch ranges (the reduction passes follow them). A nonzero return
+ * stops the search and the call fails with NS_ERR_CANCELLED: a QSAT answer needs every range. */
+int ns_qsat_solve_progress(ns_ctx* ctx, const ns_circuit* c, const uint8_t* quants_all,
+                           const ns_qsat_opts* opts, ns_sat_progress_fn fn, void* user, ns_qsat_result* res,
+                           uint32_t* bitmap, uint64_t bitmap_words);
 
 /* ---- SATOSTOP on a device-resident next-state table (replaces greedysatostop's
  *      do_solve_with_opencl_builder: transpose epilogue + check_stop / join_nexts /

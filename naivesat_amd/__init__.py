@@ -24,6 +24,8 @@ LIB_PATH = os.path.join(_HERE, "lib", "libnaivesat_hip.so")
 BIN_DIR = os.path.join(_HERE, "bin")
 
 NS_SAT_STOP_AT_FIRST = 1
+NS_ERR_CANCELLED = 7
+SAT_STEP_START = 0xFFFFFFFFFFFFFFFF  # NS_SAT_STEP_START: the progress call before the first launch
 NS_STOP_FIRST_PASS = 1
 NS_STOP_PASSES = 2
 NS_STOP_TABLE_WALKS = 4
@@ -40,6 +42,9 @@ class NaivesatError(RuntimeError):
 class _SatOpts(C.Structure):
     _fields_ = [("elem_bits", C.c_uint32), ("group_len", C.c_uint32),
                 ("words_per_lane", C.c_uint32), ("flags", C.c_uint32)]
+
+
+_SatProgressFn = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint32)
 
 
 class _SatResult(C.Structure):
@@ -144,7 +149,11 @@ def load_library() -> C.CDLL:
         "ns_ctx_destroy": (None, [P]),
         "ns_sat_solve": (I, [P, P, C.POINTER(_SatOpts), C.POINTER(_SatResult)]),
         "ns_sat_solve_noalloc": (I, [P, P, C.POINTER(_SatOpts), C.POINTER(_SatResult)]),
+        "ns_sat_solve_progress": (I, [P, P, C.POINTER(_SatOpts), _SatProgressFn, P, C.POINTER(_SatResult)]),
         "ns_qsat_solve": (I, [P, P, P, C.POINTER(_QsatOpts), C.POINTER(_QsatResult), P, U64]),
+        "ns_qsat_solve_progress": (I, [P, P, P, C.POINTER(_QsatOpts), _SatProgressFn, P, C.POINTER(_QsatResult),
+                                       P, U64]),
+        "ns_device_list": (I, [C.POINTER(U32), C.c_char_p, C.c_size_t]),
         "ns_satostop_solve": (I, [P, P, U32, C.POINTER(_StopOpts), C.POINTER(_StopResult), P, P]),
         "ns_satostop_solve_wide": (I, [P, P, U32, C.POINTER(_StopOpts), C.POINTER(_StopResult), P, P]),
         "ns_satostop_hashmap_solve": (I, [P, P, U32, C.POINTER(_HmOpts), C.POINTER(_HmResult)]),
@@ -193,7 +202,17 @@ def exported_symbols() -> list:
             "ns_probe_gather", "ns_pack_entries", "ns_ctx_trim", "ns_greedy_partitioned_solve",
             "ns_file_image_create", "ns_file_image_save_chunk", "ns_file_image_load_partition",
             "ns_file_image_save_partition", "ns_file_image_free", "ns_serve_open", "ns_serve_submit",
-            "ns_serve_poll", "ns_serve_wait", "ns_serve_close", "ns_batch_solve", "ns_sat_solve_noalloc"]
+            "ns_serve_poll", "ns_serve_wait", "ns_serve_close", "ns_batch_solve", "ns_sat_solve_noalloc",
+            "ns_sat_solve_progress", "ns_device_list", "ns_qsat_solve_progress"]
+
+
+def device_list() -> list:
+    """ns_device_list: the gfx architecture name of each HIP device ([] without one)."""
+    n = C.c_uint32(0)
+    buf = C.create_string_buffer(4096)
+    _check(load_library().ns_device_list(C.byref(n), buf, 4096))
+    names = buf.value.decode()
+    return names.split(",") if n.value else []
 
 
 def _check(status: int) -> None:
@@ -398,6 +417,30 @@ class Context:
         _check(load_library().ns_sat_solve(self._h, c._h, C.byref(o), C.byref(r)))
         return {k: getattr(r, k) for k, _ in _SatResult._fields_}
 
+    def sat_solve_progress(self, c: Circuit, callback, elem_bits: int = 0, group_len: int = 0,
+                           words_per_lane: int = 0, stop_at_first: bool = False) -> dict:
+        """ns_sat_solve_progress: callback(step, steps, range_bits) -> bool (True stops the
+        search), called once with step = SAT_STEP_START before the first launch, then after
+        each launch range in order. An exception raised by the callback stops the search and is
+        raised again afterwards."""
+        o = _SatOpts(elem_bits, group_len, words_per_lane, NS_SAT_STOP_AT_FIRST if stop_at_first else 0)
+        r = _SatResult()
+        err = []
+
+        def tramp(user, step, steps, bits):
+            try:
+                return 1 if callback(step, steps, bits) else 0
+            except BaseException as e:  # noqa: BLE001 - re-raised after the search
+                err.append(e)
+                return 1
+
+        fn = _SatProgressFn(tramp)
+        st = load_library().ns_sat_solve_progress(self._h, c._h, C.byref(o), fn, None, C.byref(r))
+        if err:
+            raise err[0]
+        _check(st)
+        return {k: getattr(r, k) for k, _ in _SatResult._fields_}
+
     def sat_solve_noalloc(self, c: Circuit, elem_bits: int = 0, group_len: int = 0, words_per_lane: int = 0,
                           stop_at_first: bool = False) -> dict:
         """ns_sat_solve_noalloc: the search without any device allocation; the circuit's kernel
@@ -408,7 +451,10 @@ class Context:
         return {k: getattr(r, k) for k, _ in _SatResult._fields_}
 
     def qsat_solve(self, c: Circuit, quants_all: Sequence[int], elem_bits: int = 0, group_len: int = 0,
-                   words_per_lane: int = 0, bitmap: bool = False) -> dict:
+                   words_per_lane: int = 0, bitmap: bool = False, progress=None) -> dict:
+        """progress: callback(step, steps, range_bits) -> bool as for sat_solve_progress
+        (ns_qsat_solve_progress); True stops the search, which then raises NaivesatError
+        (status NS_ERR_CANCELLED)."""
         q = np.ascontiguousarray(quants_all, dtype=np.uint8)
         if q.size != c.n_inputs:
             raise ValueError("one quantifier per input is required")
@@ -418,8 +464,25 @@ class Context:
         while fb < q.size and q[fb] == q[0]:
             fb += 1
         bm = np.zeros(max(1, (1 << fb) // 32), np.uint32) if bitmap else None
-        _check(load_library().ns_qsat_solve(self._h, c._h, _ptr(q), C.byref(o), C.byref(r), _ptr(bm),
-                                            0 if bm is None else bm.size))
+        if progress is None:
+            _check(load_library().ns_qsat_solve(self._h, c._h, _ptr(q), C.byref(o), C.byref(r), _ptr(bm),
+                                                0 if bm is None else bm.size))
+        else:
+            err = []
+
+            def tramp(user, step, steps, bits):
+                try:
+                    return 1 if progress(step, steps, bits) else 0
+                except BaseException as e:  # noqa: BLE001 - re-raised after the search
+                    err.append(e)
+                    return 1
+
+            fn = _SatProgressFn(tramp)
+            st = load_library().ns_qsat_solve_progress(self._h, c._h, _ptr(q), C.byref(o), fn, None, C.byref(r),
+                                                       _ptr(bm), 0 if bm is None else bm.size)
+            if err:
+                raise err[0]
+            _check(st)
         d = {k: getattr(r, k) for k, _ in _QsatResult._fields_}
         if bm is not None:
             bits = np.unpackbits(bm.view(np.uint8), bitorder="little")[: 1 << fb]

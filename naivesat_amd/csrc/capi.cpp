@@ -1,5 +1,6 @@
 // extern "C" boundary: converts exceptions to status codes and keeps the last error
 // message per thread.
+#include <algorithm>
 #include <cstring>
 #include <memory>
 #include <new>
@@ -214,6 +215,33 @@ int ns_gen_satostop(uint64_t seed, uint32_t half, uint32_t rounds, uint32_t roun
   });
 }
 
+int ns_device_list(uint32_t* count, char* names, size_t cap) {
+  return guard([&] {
+    need(count, "null argument");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    (void)hipGetLastError();
+    std::string all;
+    for (int d = 0; d < n; d++) {
+      hipDeviceProp_t prop;
+      std::string name = "unknown";
+      if (hipGetDeviceProperties(&prop, d) == hipSuccess) {
+        name = prop.gcnArchName;
+        size_t colon = name.find(':');
+        if (colon != std::string::npos) name = name.substr(0, colon);
+      }
+      if (d) all += ',';
+      all += name;
+    }
+    *count = (uint32_t)n;
+    if (names && cap) {
+      size_t k = std::min(cap - 1, all.size());
+      std::memcpy(names, all.data(), k);
+      names[k] = 0;
+    }
+  });
+}
+
 int ns_ctx_create(int device, ns_ctx** out) {
   return guard([&] {
     need(out, "null argument");
@@ -246,6 +274,16 @@ int ns_sat_solve(ns_ctx* ctx, const ns_circuit* c, const ns_sat_opts* opts, ns_s
   });
 }
 
+int ns_sat_solve_progress(ns_ctx* ctx, const ns_circuit* c, const ns_sat_opts* opts, ns_sat_progress_fn fn,
+                          void* user, ns_sat_result* res) {
+  return guard([&] {
+    need(ctx && c && res && fn, "null argument");
+    ns_sat_opts o{};
+    if (opts) o = *opts;
+    ns::sat_solve(*ctx->ctx.main, c->c, o, *res, fn, user);
+  });
+}
+
 int ns_sat_solve_noalloc(ns_ctx* ctx, const ns_circuit* c, const ns_sat_opts* opts, ns_sat_result* res) {
   return guard([&] {
     need(ctx && c && res, "null argument");
@@ -264,6 +302,19 @@ int ns_qsat_solve(ns_ctx* ctx, const ns_circuit* c, const uint8_t* quants_all, c
     std::vector<uint8_t> q(quants_all, quants_all + c->c.n);
     for (auto& v : q) v = v ? 1 : 0;
     ns::qsat_solve(*ctx->ctx.main, c->c, q, o, *res, bitmap, bitmap_words);
+  });
+}
+
+int ns_qsat_solve_progress(ns_ctx* ctx, const ns_circuit* c, const uint8_t* quants_all,
+                           const ns_qsat_opts* opts, ns_sat_progress_fn fn, void* user, ns_qsat_result* res,
+                           uint32_t* bitmap, uint64_t bitmap_words) {
+  return guard([&] {
+    need(ctx && c && res && quants_all && fn, "null argument");
+    ns_qsat_opts o{};
+    if (opts) o = *opts;
+    std::vector<uint8_t> q(quants_all, quants_all + c->c.n);
+    for (auto& v : q) v = v ? 1 : 0;
+    ns::qsat_solve(*ctx->ctx.main, c->c, q, o, *res, bitmap, bitmap_words, fn, user);
   });
 }
 

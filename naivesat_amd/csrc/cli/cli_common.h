@@ -1,6 +1,7 @@
 // Helpers shared by the command-line tools (flag parsing, file loading, printing).
 #pragma once
 #include <chrono>
+#include <csignal>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -137,9 +138,73 @@ inline std::string bits_msb_first(unsigned long long v, unsigned width) {
   return s;
 }
 
+// The first line of every tool: the reference lists its OpenCL GPU devices
+// (src/main.rs:385-388, "OpenCL devices: [...]"); here the HIP devices in the same form,
+// each as hip:<index> <architecture>.
 inline void print_devices() {
-  // the reference lists its OpenCL devices first (src/main.rs:385-388)
-  std::printf("HIP devices: [%s]\n", "gfx950");
+  uint32_t n = 0;
+  char names[4096] = {0};
+  if (ns_device_list(&n, names, sizeof(names)) != NS_OK) n = 0;
+  std::string line;
+  const char* p = names;
+  for (uint32_t d = 0; d < n; d++) {
+    const char* q = std::strchr(p, ',');
+    std::string name = q ? std::string(p, q) : std::string(p);
+    if (d) line += ", ";
+    line += "hip:" + std::to_string(d) + " " + name;
+    p = q ? q + 1 : p + std::strlen(p);
+  }
+  std::printf("OpenCL devices: [%s]\n", line.c_str());
+}
+
+// Progress of a search made of launch ranges (ns_sat_solve_progress / ns_qsat_solve_progress):
+// "Start execution" and the clock start once the kernel is built, "Step: k / steps" after
+// each range completes (the reference's per-arg callback, src/main.rs:125-153), and SIGINT
+// stops the search after the ranges in flight (a second SIGINT ends the process).
+inline volatile std::sig_atomic_t& interrupt_flag() {
+  static volatile std::sig_atomic_t flag = 0;
+  return flag;
+}
+extern "C" inline void on_interrupt(int) {
+  interrupt_flag() = 1;
+  std::signal(SIGINT, SIG_DFL);
+}
+
+struct Progress {
+  bool print = true;       // the reference prints these lines for n >= 14 only
+  long long elem = -1;     // elem inputs as printed before the build
+  bool started = false;
+  uint64_t done = 0, steps = 0;
+  uint32_t bits = 0;
+  std::chrono::steady_clock::time_point t0;
+  Progress() { std::signal(SIGINT, on_interrupt); }
+  ~Progress() { std::signal(SIGINT, SIG_DFL); }
+  bool cancelled() const { return interrupt_flag() != 0; }
+  // wall time from "Start execution" to now
+  double seconds() const {
+    return started ? std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() : 0.0;
+  }
+};
+
+extern "C" inline int progress_step(void* user, uint64_t step, uint64_t steps, uint32_t bits) {
+  auto* p = static_cast<Progress*>(user);
+  p->steps = steps;
+  p->bits = bits;
+  if (step == NS_SAT_STEP_START) {
+    if (p->print) {
+      if (p->elem >= 0 && (long long)bits != p->elem)
+        std::printf("Note: launch ranges of 2^%u assignments (elem inputs %u)\n", bits, bits);
+      std::printf("Start execution\n");
+    }
+    std::fflush(stdout);
+    p->started = true;
+    p->t0 = std::chrono::steady_clock::now();
+  } else {
+    p->done = step + 1;
+    if (p->print) std::printf("Step: %llu / %llu\n", (unsigned long long)step, (unsigned long long)steps);
+    std::fflush(stdout);
+  }
+  return interrupt_flag() ? 1 : 0;
 }
 
 }  // namespace cli

@@ -5,11 +5,13 @@
 
 static const char* kUsage =
     "Usage: naiveqsat [OPTIONS] --exec-type <EXEC_TYPE> <CIRCUIT>\n"
-    "  -e, --elem-inputs <N>      log2 of assignments per launch [default: 24]\n"
+    "  -e, --elem-inputs <N>      log2 of assignments per launch range, 13..64, kept exactly up to 40\n"
+    "                             [default: 32; the reference's 24 leaves the GPU idle between launches]\n"
     "  -t, --exec-type <TYPE>     cpu|opencl:N|hip:N|cpu_and_opencl* (all run on one HIP device)\n"
     "  -n, --no-optimize-negs     accepted for compatibility\n"
     "  -G, --opencl-group-len <G> threads per block\n"
-    "  -W, --words-per-lane <W>   32-assignment words per lane\n";
+    "  -W, --words-per-lane <W>   32-assignment words per lane\n"
+    "Ctrl-C stops the search after the launch ranges in flight (no answer).\n";
 
 int main(int argc, char** argv) {
   auto a = cli::parse_args(argc, argv,
@@ -31,7 +33,7 @@ int main(int argc, char** argv) {
   uint32_t n = 0, g = 0, no = 0;
   cli::check(ns_circuit_info(c, &n, &g, &no), "circuit");
   if (no != 1) cli::die("circuit must have exactly one output");
-  long long e = a.num("elem-inputs", 24);
+  long long e = a.num("elem-inputs", 32);
   int dev = cli::device_from_exec_type(a.get("exec-type"), n >= 14);
   long long G = a.num("opencl-group-len", 256);
   if (n >= 14) {
@@ -44,13 +46,23 @@ int main(int argc, char** argv) {
   ns_ctx* ctx = nullptr;
   cli::check(ns_ctx_create(dev, &ctx), "device");
   ns_qsat_opts o{};
-  o.elem_bits = (uint32_t)(e < 32 ? 32 : e);
+  o.elem_bits = n >= 14 ? (uint32_t)e : 0u;
   o.group_len = (uint32_t)G;
   o.words_per_lane = (uint32_t)a.num("words-per-lane", 0);
-  if (n >= 14) std::printf("Start execution\n");
+  cli::Progress prog;
+  prog.print = n >= 14;
+  prog.elem = n >= 14 ? e : -1;
   ns_qsat_result r{};
-  cli::check(ns_qsat_solve(ctx, c, q.data(), &o, &r, nullptr, 0), "search");
-  if (n >= 14) std::printf("Time: %g\n", r.kernel_ms / 1000.0);
+  int st = ns_qsat_solve_progress(ctx, c, q.data(), &o, cli::progress_step, &prog, &r, nullptr, 0);
+  if (st == NS_ERR_CANCELLED) {
+    std::printf("Cancelled after %llu / %llu launch ranges: no result\n", (unsigned long long)prog.done,
+                (unsigned long long)prog.steps);
+    ns_ctx_destroy(ctx);
+    ns_circuit_free(c);
+    return 130;
+  }
+  cli::check(st, "search");
+  if (n >= 14) std::printf("Time: %g\n", prog.seconds());
   // FinalResult display: witness printed with bit b = input b, most significant first
   std::string res;
   if (r.has_witness) {

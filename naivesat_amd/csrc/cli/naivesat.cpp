@@ -4,12 +4,14 @@
 
 static const char* kUsage =
     "Usage: naivesat [OPTIONS] --exec-type <EXEC_TYPE> <CIRCUIT>\n"
-    "  -e, --elem-inputs <N>      log2 of assignments per launch [default: 24]\n"
+    "  -e, --elem-inputs <N>      log2 of assignments per launch range, 13..64, kept exactly up to 40\n"
+    "                             [default: 32; the reference's 24 leaves the GPU idle between launches]\n"
     "  -t, --exec-type <TYPE>     cpu|opencl:N|hip:N|cpu_and_opencl* (all run on one HIP device)\n"
     "  -n, --no-optimize-negs     accepted for compatibility\n"
     "  -G, --opencl-group-len <G> threads per block\n"
     "  -W, --words-per-lane <W>   32-assignment words per lane\n"
-    "  -c, --count                search the whole space and print the satisfying count\n";
+    "  -c, --count                search the whole space and print the satisfying count\n"
+    "Ctrl-C stops the search after the launch ranges in flight.\n";
 
 int main(int argc, char** argv) {
   auto a = cli::parse_args(argc, argv,
@@ -29,7 +31,7 @@ int main(int argc, char** argv) {
   uint32_t n = 0, g = 0, no = 0;
   cli::check(ns_circuit_info(c, &n, &g, &no), "circuit");
   if (no != 1) cli::die("circuit must have exactly one output");
-  long long e = a.num("elem-inputs", 24);
+  long long e = a.num("elem-inputs", 32);
   if (n >= 14) {
     if (e >= (long long)n) e = n - 1;
     if (!(e > 12 && e <= 64)) cli::die("elem inputs must be in 13..64");
@@ -40,17 +42,29 @@ int main(int argc, char** argv) {
   ns_ctx* ctx = nullptr;
   cli::check(ns_ctx_create(dev, &ctx), "device");
   ns_sat_opts o{};
-  // one launch covers at least 2^32 assignments: many launches of 2^24 cannot fill the GPU
-  o.elem_bits = (uint32_t)(e < 32 ? 32 : e);
+  o.elem_bits = n >= 14 ? (uint32_t)e : 0u;
   o.group_len = (uint32_t)a.num("opencl-group-len", 0);
   o.words_per_lane = (uint32_t)a.num("words-per-lane", 0);
   o.flags = a.has("count") ? 0u : NS_SAT_STOP_AT_FIRST;
-  if (n >= 14) std::printf("Start execution\n");
+  cli::Progress prog;
+  prog.print = n >= 14;
+  prog.elem = n >= 14 ? e : -1;
   ns_sat_result r{};
-  cli::check(ns_sat_solve(ctx, c, &o, &r), "search");
-  if (n >= 14) std::printf("Time: %g\n", r.kernel_ms / 1000.0);
-  if (a.has("count")) std::printf("Count: %llu\n", (unsigned long long)r.count);
-  if (r.found) {
+  cli::check(ns_sat_solve_progress(ctx, c, &o, cli::progress_step, &prog, &r), "search");
+  if (n >= 14) std::printf("Time: %g\n", prog.seconds());
+  // interrupted: the answer over the ranges searched (a prefix of the space), exit status 130
+  const bool cancelled = !r.complete && prog.cancelled();
+  if (cancelled) {
+    std::printf("Cancelled after %llu / %llu launch ranges (%llu assignments searched)\n",
+                (unsigned long long)r.launches, (unsigned long long)prog.steps,
+                (unsigned long long)((unsigned long long)r.launches << prog.bits));
+    if (a.has("count")) std::printf("Count: %llu (searched ranges only)\n", (unsigned long long)r.count);
+  } else if (a.has("count")) {
+    std::printf("Count: %llu\n", (unsigned long long)r.count);
+  }
+  if (cancelled && !r.found) {
+    std::printf("No solution in the searched ranges\n");
+  } else if (r.found) {
     std::vector<uint8_t> in(n), out(1);
     for (uint32_t k = 0; k < n; k++) in[k] = (uint8_t)((r.lowest >> k) & 1ull);
     cli::check(ns_circuit_eval(c, in.data(), out.data()), "verify");
@@ -63,5 +77,5 @@ int main(int argc, char** argv) {
   }
   ns_ctx_destroy(ctx);
   ns_circuit_free(c);
-  return 0;
+  return cancelled ? 130 : 0;
 }

@@ -29,6 +29,7 @@ enum : int {
   ST_UNSUPPORTED = 4,
   ST_NOMEM = 5,
   ST_INTERNAL = 6,
+  ST_CANCELLED = 7,
 };
 
 void check_hip(hipError_t e, const char* what);

@@ -59,6 +59,75 @@ Geometry pick_geometry(uint64_t total_words, uint32_t group_len, uint32_t wpl, b
   return g;
 }
 
+// SAT launch shape: words per lane, then threads per block, shrink until one block fits in a
+// launch range of 2^(elem_bits-5) words, so a requested range size is kept exactly (down to
+// 64 words = 2^11 assignments per range).
+Geometry sat_geometry(uint64_t total_words, const ns_sat_opts& o) {
+  uint64_t cap = total_words;
+  if (o.elem_bits >= 5 && o.elem_bits <= 40 && (1ull << (o.elem_bits - 5)) < cap) cap = 1ull << (o.elem_bits - 5);
+  return pick_geometry(cap, o.group_len, o.words_per_lane, false);
+}
+
+// Progress reports of a search made of launch ranges (ns_sat_solve_progress): one call before
+// the first launch, then one per range in order once it has completed. With a callback, an
+// event follows each launch and the host waits for range k only after range k+1 is queued,
+// so the device stays busy while the host reports. A nonzero return from the callback
+// cancels: the caller launches no further range.
+class RangeProgress {
+ public:
+  RangeProgress(Lane& lane, ns_sat_progress_fn fn, void* user, uint64_t steps, uint32_t bits)
+      : lane_(lane), fn_(fn), user_(user), steps_(steps), bits_(bits) {}
+  ~RangeProgress() {
+    for (uint32_t k = 0; k < npend_; k++) (void)hipEventDestroy(pend_[k]);
+  }
+  RangeProgress(const RangeProgress&) = delete;
+  RangeProgress& operator=(const RangeProgress&) = delete;
+  bool active() const { return fn_ != nullptr; }
+  bool cancelled() const { return cancel_; }
+  void start() { call(NS_SAT_STEP_START); }
+  // range `step` is complete (the stream was synchronized after it)
+  void completed(uint64_t step) { call(step); }
+  // range `step` was just launched on the lane's stream
+  void launched(uint64_t step) {
+    if (!fn_) return;
+    hipEvent_t e = nullptr;
+    NS_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    pend_[npend_] = e;
+    step_[npend_] = step;
+    npend_++;
+    NS_HIP(hipEventRecord(e, lane_.stream));
+    if (npend_ == 2) wait_oldest();
+  }
+  void drain() {
+    while (npend_) wait_oldest();
+  }
+
+ private:
+  void call(uint64_t step) {
+    if (fn_ && !cancel_ && fn_(user_, step, steps_, bits_) != 0) cancel_ = true;
+  }
+  void wait_oldest() {
+    hipEvent_t e = pend_[0];
+    uint64_t st = step_[0];
+    pend_[0] = pend_[1];
+    step_[0] = step_[1];
+    npend_--;
+    hipError_t err = hipEventSynchronize(e);
+    (void)hipEventDestroy(e);
+    NS_HIP(err);
+    call(st);
+  }
+  Lane& lane_;
+  ns_sat_progress_fn fn_;
+  void* user_;
+  uint64_t steps_;
+  uint32_t bits_;
+  bool cancel_ = false;
+  hipEvent_t pend_[2] = {nullptr, nullptr};
+  uint64_t step_[2] = {0, 0};
+  uint32_t npend_ = 0;
+};
+
 // Words per launch: 2^(elem_bits-5) rounded up to whole blocks.
 uint64_t chunk_words(uint32_t elem_bits, const Geometry& g, uint64_t total_words) {
   uint32_t eb = elem_bits ? elem_bits : 32;
@@ -199,14 +268,15 @@ std::string circuit_fingerprint(const Circuit& c) {
 // ---------------------------------------------------------------------------
 // SAT
 // ---------------------------------------------------------------------------
-void sat_solve(Lane& ctx, const Circuit& c, const ns_sat_opts& o, ns_sat_result& r) {
+void sat_solve(Lane& ctx, const Circuit& c, const ns_sat_opts& o, ns_sat_result& r, ns_sat_progress_fn progress,
+               void* user) {
   auto t0 = clk::now();
   std::memset(&r, 0, sizeof(r));
   if (c.outputs.size() != 1) throw Error(ST_INVALID, "SAT needs a circuit with exactly one output");
   if (c.n == 0 || c.n > 64) throw Error(ST_UNSUPPORTED, "SAT supports 1..64 inputs");
   NS_HIP(hipSetDevice(ctx.device));
   const uint64_t total_words = c.n >= 5 ? (c.n == 64 ? (1ull << 59) : (1ull << (c.n - 5))) : 1ull;
-  Geometry g = pick_geometry(total_words, o.group_len, o.words_per_lane, false);
+  Geometry g = sat_geometry(total_words, o);
   std::vector<uint32_t> vb(c.n);
   for (uint32_t k = 0; k < c.n; k++) vb[k] = k;
   std::string fp;
@@ -243,7 +313,14 @@ void sat_solve(Lane& ctx, const Circuit& c, const ns_sat_opts& o, ns_sat_result&
   // A repeated complete search (kernel from the cache, geometry settled) is captured once as
   // a HIP graph (result reset + every word-range launch) and replayed afterwards.
   const bool repeated = r.compile_ms == 0.0;
-  if (repeated && !stop_first && !(tune && !ch.tuned_wpl)) {
+  RangeProgress prog(ctx, progress, user, (total_words + cw - 1) / cw, ilog2(cw) + 5);
+  prog.start();  // the kernel is built, nothing is launched yet
+  if (prog.cancelled()) {
+    r.complete = 0;
+    r.wall_ms = ms_since(t0);
+    return;
+  }
+  if (repeated && !stop_first && !prog.active() && !(tune && !ch.tuned_wpl)) {
     const std::string key = fp + "/graph/" + std::to_string(g.block) + "/" + std::to_string(g.wpl) + "/" +
                             std::to_string(cw) + "/" + std::to_string(total_words);
     hipGraphExec_t exec = ctx.find_graph(key);
@@ -320,8 +397,9 @@ void sat_solve(Lane& ctx, const Circuit& c, const ns_sat_opts& o, ns_sat_result&
     ch.tuned_wpl = g.wpl;
     ch.tuned_block = g.block;
     ctx.jit.put_built(fp, ch);
+    for (uint32_t k = 0; k < kTuneCandidates; k++) prog.completed(k);
   }
-  while (w0 < total_words) {
+  while (w0 < total_words && !prog.cancelled()) {
     uint64_t words = total_words - w0 < cw ? total_words - w0 : cw;
     void* extra[3] = {&res, &wpl, &valid};
     launch_range(ctx, mod->fn, g, w0, words, extra, ch.native ? 3 : 1, total_words);
@@ -331,12 +409,17 @@ void sat_solve(Lane& ctx, const Circuit& c, const ns_sat_opts& o, ns_sat_result&
       // some hit so far: the lowest one is set
       NS_HIP(hipMemcpyAsync(host, res, sizeof(host), hipMemcpyDeviceToHost, ctx.stream));
       NS_HIP(hipStreamSynchronize(ctx.stream));
+      prog.completed(r.launches - 1);
       if (host[1] != ~0ull) {
         r.complete = w0 >= total_words;
         break;
       }
+    } else {
+      prog.launched(r.launches - 1);
     }
   }
+  prog.drain();
+  if (prog.cancelled() && w0 < total_words) r.complete = 0;
   NS_HIP(hipEventRecord(ctx.ev1, ctx.stream));
   CountsCopy cc;
   read_counts_async(ctx, cc);
@@ -365,7 +448,7 @@ void sat_solve_noalloc(Lane& ctx, const Circuit& c, const ns_sat_opts& o, ns_sat
   static_assert(sizeof(CountsCopy) <= Lane::kPinnedBytes, "pinned block holds the counters");
   NS_HIP(hipSetDevice(ctx.device));
   const uint64_t total_words = c.n >= 5 ? (c.n == 64 ? (1ull << 59) : (1ull << (c.n - 5))) : 1ull;
-  Geometry g = pick_geometry(total_words, o.group_len, o.words_per_lane, false);
+  Geometry g = sat_geometry(total_words, o);
   std::vector<uint32_t> vb(c.n);
   for (uint32_t k = 0; k < c.n; k++) vb[k] = k;
   Chosen ch = choose_kernel(
@@ -421,7 +504,8 @@ void sat_solve_noalloc(Lane& ctx, const Circuit& c, const ns_sat_opts& o, ns_sat
 // QSAT
 // ---------------------------------------------------------------------------
 void qsat_solve(Lane& ctx, const Circuit& c, const std::vector<uint8_t>& q, const ns_qsat_opts& o,
-                ns_qsat_result& r, uint32_t* bitmap, uint64_t bitmap_words) {
+                ns_qsat_result& r, uint32_t* bitmap, uint64_t bitmap_words, ns_sat_progress_fn progress,
+                void* user) {
   auto t0 = clk::now();
   std::memset(&r, 0, sizeof(r));
   if (c.outputs.size() != 1) throw Error(ST_INVALID, "QSAT needs a circuit with exactly one output");
@@ -441,7 +525,10 @@ void qsat_solve(Lane& ctx, const Circuit& c, const std::vector<uint8_t>& q, cons
     if (bitmap_words < need) throw Error(ST_INVALID, "bitmap buffer too small");
   }
   const uint64_t total_words = n >= 5 ? (1ull << (n - 5)) : 1ull;
-  Geometry g = pick_geometry(total_words, o.group_len, o.words_per_lane, true);
+  // as for SAT, a requested range size is kept exactly (one block fits in a range)
+  uint64_t gcap = total_words;
+  if (o.elem_bits >= 5 && o.elem_bits <= 40 && (1ull << (o.elem_bits - 5)) < gcap) gcap = 1ull << (o.elem_bits - 5);
+  Geometry g = pick_geometry(gcap, o.group_len, o.words_per_lane, true);
   const uint32_t jb = ilog2(g.wpl);
   const bool case_a = L >= 5 + jb;
   std::vector<uint32_t> vb(n);
@@ -465,6 +552,16 @@ void qsat_solve(Lane& ctx, const Circuit& c, const std::vector<uint8_t>& q, cons
   reset_counts(ctx);
   const uint64_t cw = chunk_words(o.elem_bits, g, total_words);
   unsigned int* dev_bitmap = nullptr;  // final 2^fb-bit array on the device
+  RangeProgress prog(ctx, progress, user, (total_words + cw - 1) / cw, ilog2(cw) + 5);
+  prog.start();
+  auto check_cancel = [&]() {
+    if (prog.cancelled()) {
+      prog.drain();
+      throw Error(ST_CANCELLED, "QSAT search cancelled by its progress callback");
+    }
+  };
+  check_cancel();
+  uint64_t step = 0;
   NS_HIP(hipEventRecord(ctx.ev0, ctx.stream));
   if (case_a) {
     // phase 1: one bit per thread
@@ -476,7 +573,11 @@ void qsat_solve(Lane& ctx, const Circuit& c, const std::vector<uint8_t>& q, cons
       uint64_t words = total_words - w0 < cw ? total_words - w0 : cw;
       void* extra[4] = {&X, &wpl, &jbv, &opm};
       launch_range(ctx, mod->fn, g, w0, words, extra, ch.native ? 4 : 1, total_words);
+      prog.launched(step++);
+      check_cancel();
     }
+    prog.drain();
+    check_cancel();
     // phase 2: reduction passes over the bit array
     uint32_t m = m0, remaining = L - 5 - jb, level = 5 + jb;
     const unsigned int* in = reinterpret_cast<unsigned int*>(X);
@@ -516,7 +617,11 @@ void qsat_solve(Lane& ctx, const Circuit& c, const std::vector<uint8_t>& q, cons
       uint64_t words = total_words - w0 < cw ? total_words - w0 : cw;
       void* extra[8] = {&stats, &dev_bitmap, &wpl, &jbv, &opm, &Lv, &q0v, &valid};
       launch_range(ctx, mod->fn, g, w0, words, extra, ch.native ? 8 : 2, total_words);
+      prog.launched(step++);
+      check_cancel();
     }
+    prog.drain();
+    check_cancel();
   }
   NS_HIP(hipEventRecord(ctx.ev1, ctx.stream));
   unsigned long long host[2];

@@ -12,13 +12,18 @@ namespace ns {
 // 128-bit fingerprint of a circuit's content (hex): the kernel caches' key
 std::string circuit_fingerprint(const Circuit& c);
 
-void sat_solve(Lane& ctx, const Circuit& c, const ns_sat_opts& o, ns_sat_result& r);
+// progress: optional per-range callback (ns_sat_solve_progress); a nonzero return stops the
+// search after the ranges already launched
+void sat_solve(Lane& ctx, const Circuit& c, const ns_sat_opts& o, ns_sat_result& r,
+               ns_sat_progress_fn progress = nullptr, void* user = nullptr);
 // sat_solve without any allocation: the kernel must be in the context's cache (built by an
 // earlier sat_solve with the same options), counters in `scratch` (device address)
 void sat_solve_noalloc(Lane& ctx, const Circuit& c, const ns_sat_opts& o, ns_sat_result& r,
                        unsigned long long* scratch);
+// progress: as for sat_solve; a cancelled QSAT search fails with ST_CANCELLED (no answer)
 void qsat_solve(Lane& ctx, const Circuit& c, const std::vector<uint8_t>& quants_all,
-                const ns_qsat_opts& o, ns_qsat_result& r, uint32_t* bitmap, uint64_t bitmap_words);
+                const ns_qsat_opts& o, ns_qsat_result& r, uint32_t* bitmap, uint64_t bitmap_words,
+                ns_sat_progress_fn progress = nullptr, void* user = nullptr);
 // steps_out / ends_out (optional): 2^unknowns host entries each
 void satostop_solve(Lane& ctx, const Circuit& c, uint32_t unknowns, const ns_satostop_opts& o,
                     ns_satostop_result& r, uint64_t* steps_out, uint64_t* ends_out);
