@@ -141,14 +141,15 @@ def cpu_part(c, cone, threads):
 
 
 def table_part(ns, ctx):
-    """The 2^30-state SATOSTOP search: waypoint walks (the default full search) and, beside
-    them, the passes over the whole table, with the same answers."""
+    """The 2^30-state SATOSTOP search over its next-state table: waypoint walks over the
+    table and, beside them, the passes over the whole table; and the default full search,
+    walks without a table (walks_without_table); all with the same answers."""
     cs, target = ns.Circuit.gen_satostop(4, 15, 6, 300, 14)
-    ctx.satostop_solve(cs, 14, arrays=False)  # kernel build
-    rs = ctx.satostop_solve(cs, 14, arrays=False)
+    ctx.satostop_solve(cs, 14, arrays=False, method="table")  # kernel build
+    rs = ctx.satostop_solve(cs, 14, arrays=False, method="table")
     rp = ctx.satostop_solve(cs, 14, arrays=False, passes=True)
     jump = ctx.pass_times()[1:]  # u64 passes (the first reads the u32 table)
-    ctx.satostop_solve(cs, 14, arrays=False, method="bits")  # kernel build
+    rb0 = ctx.satostop_solve(cs, 14, arrays=False, method="bits")  # kernel build
     rb = ctx.satostop_solve(cs, 14, arrays=False, method="bits")
     for r in (rp, rb):
         assert (rs["n_stopped"], rs["lowest_u"], rs["steps"], rs["end"]) == \
@@ -193,18 +194,21 @@ def table_part(ns, ctx):
                                            "kernel": "k_jump_pass", "bytes_per_entry": 24}},
             "speedup_vs_passes": (rp["build_ms"] + rp["jump_ms"] + rp["walk_ms"]) /
                                  (rs["build_ms"] + rs["walk_ms"] + rs["jump_ms"]),
-            "walks_without_table": walks_without_table(rb, "ns_walk")}
+            "walks_without_table": walks_without_table(rb, rb0)}
 
 
-def walks_without_table(r, kernel):
+def walks_without_table(r, first=None):
     """the search by walks that evaluate the circuit (32 per lane per step): VALU-bound, one
-    op per cone gate per 32 evaluations"""
+    op per cone gate per 32 evaluations. first: the run that built the kernel"""
+    kernel = f"ns_nat_walk*_r{r['vgpr_class']}" if r["backend"] else "ns_walk"
     walk_s = r["walk_ms"] / 1000.0
     tops = r["walk_evaluations"] * r["cone_gates"] / 32 / walk_s / 1e12
     return {"method": r["method"], "waypoint_bits": r["waypoint_bits"], "walk_ms": r["walk_ms"],
             "waypoint_jump_ms": r["jump_ms"], "device_ms": r["walk_ms"] + r["jump_ms"],
             "walk_evaluations": r["walk_evaluations"], "evaluations_per_s": r["walk_evaluations"] / walk_s,
-            "cone_gates": r["cone_gates"], "n_stopped": r["n_stopped"], "lowest_u": r["lowest_u"],
+            "cone_gates": r["cone_gates"], "luts_per_step": r["valu_ops"], "n_stopped": r["n_stopped"],
+            "lowest_u": r["lowest_u"], "backend": "native" if r["backend"] else "hiprtc",
+            "vgpr_class": r["vgpr_class"], "compile_ms_first": None if first is None else first["compile_ms"],
             "roofline": {"bound": "valu", "achieved": tops, "peak": VALU_PEAK_OPS / 1e12, "unit": "TOP/s",
                          "frac": tops * 1e12 / VALU_PEAK_OPS, "kernel": kernel}}
 
@@ -215,10 +219,10 @@ STOP36_WORKLOAD = ("SATOSTOP n=36 Feistel (6 rounds x 200 gates, seed 10601), 14
 
 def stop36_part(ns, ctx):
     c, target = ns.Circuit.gen_satostop(10601, 18, 6, 200, 14)
-    ctx.satostop_solve(c, 14, arrays=False)  # kernel build
+    r0 = ctx.satostop_solve(c, 14, arrays=False)  # kernel build
     r = ctx.satostop_solve(c, 14, arrays=False)
     ctx.trim()
-    d = walks_without_table(r, "ns_walk")
+    d = walks_without_table(r, r0)
     d.update({"workload": STOP36_WORKLOAD, "wall_ms": r["wall_ms"], "steps": r["steps"], "end": r["end"]})
     return d
 

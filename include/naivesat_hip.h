@@ -402,7 +402,8 @@ int ns_batch_solve(ns_ctx* ctx, const ns_request* reqs, uint32_t n, ns_request_r
  * kernel for a circuit, assembles and links it, and returns the circuit body's assembly.
  * kind: 0 SAT (output 0), 1 QSAT one-bit-per-thread, 2 QSAT outer-block-in-thread,
  * 3 next-state table (all outputs), 4 batch body with the SAT input order, 5 batch body with
- * the QSAT input order. Input k sits at index bit k (SAT, table, 4) or n-1-k (QSAT, 5);
+ * the QSAT input order, 6 SATOSTOP walk body (two walk steps and their bookkeeping; lane_shift
+ * = the waypoint bits k). Input k sits at index bit k (SAT, table, 4) or n-1-k (QSAT, 5);
  * lane_shift places the six lane bits at index bits 5+lane_shift..10+lane_shift.
  * Returns NS_ERR_UNSUPPORTED when the circuit exceeds the native register budget. */
 int ns_native_body(const ns_circuit* c, uint32_t kind, uint32_t lane_shift, char* body, size_t cap,

@@ -22,6 +22,8 @@ const uint32_t kWordMasks[5] = {0xAAAAAAAAu, 0xCCCCCCCCu, 0xF0F0F0F0u, 0xFF00FF0
 // SGPRs of the body: s[40..101]; the SAT templates, whose own code keeps to s0..s13, also
 // give it s[14..19] and s[22..39] (s20 and s21 carry the word index)
 const uint32_t kSgprLoSat = 14, kSgprLo = 40, kSgprLoBatch = 48, kSgprHi = 101;
+// state tiers of the walk templates (templates/gen_templates.py WALK_TIERS)
+const uint32_t kWalkTiers[5] = {16, 24, 32, 40, 48};
 
 struct Loc {
   int v = -1;   // VGPR holding the value
@@ -44,12 +46,18 @@ std::string hex(uint32_t v) {
 // copied to an AGPR (v_accvgpr_write) and read back (v_accvgpr_read) before its next use.
 class Allocator {
  public:
+  // in_regs (optional): input k sits in VGPR in_regs[k] (the walk bodies: every input is a
+  // template register); out_regs (optional): output k goes to VGPR out_regs[k] instead of vk,
+  // which may be an input's register: an output is computed into it only once that input is
+  // dead, otherwise copied there at the end
   Allocator(const Circuit& c, const LutProgram& p, const std::vector<uint32_t>& vbit, uint32_t lane_shift,
             uint32_t lm_base, uint32_t pool_lo, uint32_t pool_hi, uint32_t agprs, uint32_t slot_regs,
-            const std::vector<uint32_t>& out_wires, const std::vector<uint8_t>& out_neg, uint32_t sgpr_lo)
+            const std::vector<uint32_t>& out_wires, const std::vector<uint8_t>& out_neg, uint32_t sgpr_lo,
+            const std::vector<int>* in_regs = nullptr, const std::vector<uint32_t>* out_regs = nullptr)
       : c_(c), p_(p), vbit_(vbit), ls_(lane_shift), lm_(lm_base), pool_lo_(pool_lo), pool_hi_(pool_hi),
         sgpr_lo_(sgpr_lo),
         loc_(c.wires()), uses_(c.wires()), ptr_(c.wires(), 0), outw_(out_wires), outneg_(out_neg) {
+    for (size_t k = 0; k < out_wires.size(); k++) outreg_.push_back(out_regs ? (*out_regs)[k] : (uint32_t)k);
     slot_base_ = (int)pool_hi - (int)slot_regs + 1;
     pool_hi_ = pool_hi - slot_regs;
     for (int r = (int)pool_hi_; r >= (int)pool_lo; r--) vfree_[r & 3].push_back(r);
@@ -72,18 +80,25 @@ class Allocator {
     // lane inputs sit in their template registers
     for (uint32_t k = 0; k < c.n; k++) {
       uint32_t vb = vbit[k];
-      if (vb >= 5 && vb - 5 >= ls_ && vb - 5 < ls_ + 6) loc_[k].v = (int)(lm_ + (vb - 5 - ls_));
+      if (in_regs && (*in_regs)[k] >= 0) {
+        loc_[k].v = (*in_regs)[k];
+        vowner_[(size_t)(*in_regs)[k]] = k;
+      } else if (vb >= 5 && vb - 5 >= ls_ && vb - 5 < ls_ + 6) {
+        loc_[k].v = (int)(lm_ + (vb - 5 - ls_));
+      }
     }
     // first output position of each wire that is produced non-negated by a LUT
     for (size_t k = 0; k < out_wires.size(); k++)
       if (!out_neg[k] && out_wires[k] >= c.n && !p.uniform[out_wires[k]]) {
         bool seen = false;
         for (auto& pc : precolor_) seen |= pc.first == out_wires[k];
-        if (!seen) precolor_.push_back({out_wires[k], (uint32_t)k});
+        if (!seen) precolor_.push_back({out_wires[k], outreg_[k]});
       }
   }
 
   std::string run() {
+    for (uint32_t k = 0; k < c_.n; k++)
+      if (uses_[k].empty()) kill(k);  // unread inputs: their registers may take outputs
     for (uint32_t i = 0; i < p_.ops.size(); i++) {
       const LutOp& op = p_.ops[i];
       cur_ = i;
@@ -135,8 +150,9 @@ class Allocator {
       for (int t : temps) free_v(t);
       int d = -1;
       for (auto& pc : precolor_)
-        if (pc.first == op.dst) d = (int)pc.second;
+        if (pc.first == op.dst && output_reg_free((int)pc.second)) d = (int)pc.second;
       if (d < 0) d = alloc_v(op.dst);
+      else vowner_[(size_t)d] = op.dst;
       loc_[op.dst].v = d;
       out_ << "\tv_bitop3_b32 v" << d << ", " << src[0] << ", " << src[1] << ", " << src[2] << " bitop3:"
            << hex(op.fn) << "\n";
@@ -145,23 +161,42 @@ class Allocator {
   }
 
   std::string finish() {
-    // outputs into v0..
+    // outputs into their registers (v0.. or out_regs)
     cur_ = end_;
+    // a value still to be copied from a register that another output is about to receive is
+    // first moved to a free pool register (outputs may go to the inputs' registers)
+    auto pending = [&](size_t k) {
+      const uint32_t w = outw_[k];
+      return p_.uniform[w] || outneg_[k] || loc_[w].v != (int)outreg_[k];
+    };
     for (size_t k = 0; k < outw_.size(); k++) {
+      const uint32_t w = outw_[k];
+      if (!pending(k) || p_.uniform[w] || loc_[w].v < 0) continue;
+      bool clash = false;
+      for (size_t j = 0; j < outw_.size(); j++) clash |= j != k && pending(j) && (int)outreg_[j] == loc_[w].v;
+      if (!clash) continue;
+      const int t = alloc_v(UINT32_MAX);
+      out_ << "\tv_mov_b32 v" << t << ", v" << loc_[w].v << "\n";
+      loc_[w].v = t;
+    }
+    std::vector<uint8_t> done(outw_.size(), 0);
+    for (size_t k = 0; k < outw_.size(); k++) done[k] = !pending(k);
+    for (size_t k = 0; k < outw_.size(); k++) {
+      if (done[k]) continue;
       uint32_t w = outw_[k];
+      const uint32_t dk = outreg_[k];
       const char* mn = outneg_[k] ? "v_not_b32" : "v_mov_b32";
       if (p_.uniform[w]) {
         std::string r = sreg(w);  // may emit its own definition first
-        out_ << "\t" << mn << " v" << k << ", " << r << "\n";
+        out_ << "\t" << mn << " v" << dk << ", " << r << "\n";
         continue;
       }
       const Loc& l = loc_[w];
-      if (l.v == (int)k && !outneg_[k]) continue;
       if (l.v >= 0) {
-        out_ << "\t" << mn << " v" << k << ", v" << l.v << "\n";
+        out_ << "\t" << mn << " v" << dk << ", v" << l.v << "\n";
       } else {
-        out_ << "\tv_accvgpr_read_b32 v" << k << ", a" << l.a << "\n";
-        if (outneg_[k]) out_ << "\tv_not_b32 v" << k << ", v" << k << "\n";
+        out_ << "\tv_accvgpr_read_b32 v" << dk << ", a" << l.a << "\n";
+        if (outneg_[k]) out_ << "\tv_not_b32 v" << dk << ", v" << dk << "\n";
       }
     }
     return out_.str();
@@ -172,6 +207,12 @@ class Allocator {
 
  private:
   bool is_pool_v(int r) const { return r >= (int)pool_lo_ && r <= (int)pool_hi_; }
+
+  // an output may be computed straight into register r when r holds no value still needed
+  bool output_reg_free(int r) const {
+    const uint32_t w = vowner_[(size_t)r];
+    return w == UINT32_MAX || loc_[w].dead || loc_[w].v != r;
+  }
 
   // SGPR of a wave-uniform value: defines a uniform input on first use and reloads a value
   // parked in a VGPR lane
@@ -395,6 +436,7 @@ class Allocator {
   std::vector<uint32_t> sowner_ = std::vector<uint32_t>(128, UINT32_MAX);
   std::vector<uint32_t> pinned_;
   std::vector<std::pair<uint32_t, uint32_t>> precolor_;
+  std::vector<uint32_t> outreg_;
   const std::vector<uint32_t>& outw_;
   const std::vector<uint8_t>& outneg_;
   std::ostringstream out_;
@@ -979,6 +1021,85 @@ const char* kind_name(NativeKind k) {
 
 }  // namespace
 
+// One iteration of the walk templates (ns_nat_walk<T>) as a circuit: two walk steps with
+// their per-slot bookkeeping. Inputs: s[0..n) the slots' states, f[0..n) their final states,
+// v[0..n) their saved states, then active, fin, fstop, fwp, floop, a0..a3; outputs: the new
+// s, f and masks in the same order. Per step, from state x to y = next(x):
+//   stop = STOP(x) & active, wp = (y's low k bits all zero) & active & ~stop,
+//   loop (second step only) = (y == v) & active & ~stop & ~wp, done = stop | wp | loop,
+//   f = done ? y : f, the ages a3..a0 (4-bit counters) of the slots in fin grow by one,
+//   fin |= done, fstop |= stop, fwp |= wp, floop |= loop, active &= ~done.
+Circuit walk_iteration(const Circuit& c, uint32_t k) {
+  const uint32_t n = c.n;
+  Circuit w;
+  w.n = 3 * n + 9;
+  auto add = [&](uint8_t fn, uint32_t a, uint32_t b) {
+    w.gates.push_back({a, b, fn});
+    return w.n + (uint32_t)w.gates.size() - 1;
+  };
+  auto AND = [&](uint32_t a, uint32_t b) { return add(FUNC_AND, a, b); };
+  auto NOR = [&](uint32_t a, uint32_t b) { return add(FUNC_NOR, a, b); };
+  auto NIMPL = [&](uint32_t a, uint32_t b) { return add(FUNC_NIMPL, a, b); };
+  auto XOR = [&](uint32_t a, uint32_t b) { return add(FUNC_XOR, a, b); };
+  auto OR = [&](uint32_t a, uint32_t b) { return XOR(a, NIMPL(b, a)); };
+  // AND over ~x for the wires xs (all zero), as nor pairs
+  auto all_zero = [&](const std::vector<uint32_t>& xs) {
+    uint32_t acc = UINT32_MAX;
+    for (size_t i = 0; i < xs.size(); i += 2) {
+      const uint32_t t = NOR(xs[i], i + 1 < xs.size() ? xs[i + 1] : xs[i]);
+      acc = acc == UINT32_MAX ? t : AND(acc, t);
+    }
+    return acc;
+  };
+  std::vector<uint32_t> S(n), F(n), V(n);
+  for (uint32_t i = 0; i < n; i++) {
+    S[i] = i;
+    F[i] = n + i;
+    V[i] = 2 * n + i;
+  }
+  uint32_t active = 3 * n, fin = active + 1, fstop = active + 2, fwp = active + 3, floop = active + 4;
+  uint32_t a[4] = {active + 5, active + 6, active + 7, active + 8};
+  std::vector<uint32_t> map(c.wires());
+  for (int half = 0; half < 2; half++) {
+    for (uint32_t i = 0; i < n; i++) map[i] = S[i];
+    for (size_t g = 0; g < c.gates.size(); g++)
+      map[n + g] = add(c.gates[g].func, map[c.gates[g].i0], map[c.gates[g].i1]);
+    std::vector<uint32_t> Y(n + 1);
+    for (uint32_t i = 0; i <= n; i++) {
+      const uint32_t x = map[c.outputs[i].wire];
+      Y[i] = c.outputs[i].neg ? NOR(x, x) : x;
+    }
+    const uint32_t stop = AND(Y[n], active);
+    const uint32_t ok = NIMPL(active, stop);  // active & ~stop
+    const uint32_t wp = k ? AND(all_zero(std::vector<uint32_t>(Y.begin(), Y.begin() + k)), ok) : ok;
+    uint32_t done = OR(stop, wp);
+    if (half == 1) {
+      std::vector<uint32_t> e(n);
+      for (uint32_t i = 0; i < n; i++) e[i] = XOR(Y[i], V[i]);
+      const uint32_t loop = AND(all_zero(e), NIMPL(ok, wp));
+      done = OR(done, loop);
+      floop = OR(floop, loop);
+    }
+    for (uint32_t i = 0; i < n; i++) F[i] = XOR(F[i], AND(done, XOR(Y[i], F[i])));
+    a[0] = XOR(a[0], fin);
+    uint32_t cy = NIMPL(fin, a[0]);
+    a[1] = XOR(a[1], cy);
+    cy = NIMPL(cy, a[1]);
+    a[2] = XOR(a[2], cy);
+    cy = NIMPL(cy, a[2]);
+    a[3] = XOR(a[3], cy);
+    fin = OR(fin, done);
+    fstop = OR(fstop, stop);
+    fwp = OR(fwp, wp);
+    active = NIMPL(active, done);
+    for (uint32_t i = 0; i < n; i++) S[i] = Y[i];
+  }
+  for (uint32_t i = 0; i < n; i++) w.outputs.push_back({S[i], false});
+  for (uint32_t i = 0; i < n; i++) w.outputs.push_back({F[i], false});
+  for (uint32_t x : {active, fin, fstop, fwp, floop, a[0], a[1], a[2], a[3]}) w.outputs.push_back({x, false});
+  return w;
+}
+
 NativeKernel build_native(const Circuit& c, NativeKind kind, const std::vector<uint32_t>& vbit,
                           uint32_t lane_shift, const std::vector<uint32_t>& outs) {
   NativeKernel nk;
@@ -1172,6 +1293,82 @@ NativeKernel build_native(const Circuit& c, NativeKind kind, const std::vector<u
   text.replace(pos, eol - pos, "; circuit body\n" + body);
   nk.asm_text = std::move(text);
   nk.body = std::move(body);
+  return nk;
+}
+
+NativeKernel build_native_walk(const Circuit& c, uint32_t k) {
+  const uint32_t n = c.n;
+  if (c.outputs.size() != (size_t)n + 1) throw NativeUnfit("a walk circuit has n inputs and n+1 outputs");
+  if (n == 0 || k > n) throw NativeUnfit("bad walk circuit or waypoint bits");
+  uint32_t T = 0;
+  for (uint32_t t : kWalkTiers)
+    if (n <= t) {
+      T = t;
+      break;
+    }
+  if (!T) throw NativeUnfit("walk templates hold at most 48 state bits");
+  NativeKernel nk;
+  {
+    auto cone = output_cone(c);
+    nk.info.cone_gates = (uint32_t)schedule_gates(c, cone).size();
+  }
+  const Circuit w = walk_iteration(c, k);
+  std::vector<uint32_t> outs(w.outputs.size()), out_wires, out_regs;
+  std::vector<uint8_t> out_neg;
+  for (uint32_t i = 0; i < outs.size(); i++) {
+    outs[i] = i;
+    out_wires.push_back(w.outputs[i].wire);
+    out_neg.push_back(0);
+  }
+  // template registers (gen_templates.py walk_regs): s at v8, f at v8+T, v at v8+2T, masks at v8+3T
+  std::vector<int> in_regs(w.n);
+  for (uint32_t i = 0; i < n; i++) {
+    in_regs[i] = (int)(8 + i);
+    in_regs[n + i] = (int)(8 + T + i);
+    in_regs[2 * n + i] = (int)(8 + 2 * T + i);
+  }
+  for (uint32_t j = 0; j < 9; j++) in_regs[3 * n + j] = (int)(8 + 3 * T + j);
+  for (uint32_t i = 0; i < n; i++) out_regs.push_back(8 + i);
+  for (uint32_t i = 0; i < n; i++) out_regs.push_back(8 + T + i);
+  for (uint32_t j = 0; j < 9; j++) out_regs.push_back(8 + 3 * T + j);
+  const uint32_t pool_lo = 8 + 3 * T + 9;
+  auto cone = output_cone(w, outs);
+  auto order = schedule_gates(w, cone, outs);
+  const std::vector<uint8_t> uni(w.n, 0);
+  const std::vector<uint32_t> vb(w.n, 0);
+  Circuit cx = w;
+  LutProgram p0 = map_luts(w, order, out_wires, uni);
+  split_uniform_pairs(cx, p0);
+  LutProgram p = pressure_order(p0, out_wires, cx.wires(), nullptr);
+  std::string why = "no register class tried";
+  struct Cls {
+    uint32_t budget, hi, agprs;
+  };
+  for (const Cls& cl : {Cls{256, 239, 0}, Cls{512, 239, 256}}) {
+    try {
+      Allocator a(cx, p, vb, 0, 0, pool_lo, cl.hi, cl.agprs, 0, out_wires, out_neg, kSgprLo, &in_regs, &out_regs);
+      nk.body = a.run();
+      nk.vgpr_class = cl.budget;
+      nk.spills = a.spills();
+      nk.reloads = a.reloads();
+      break;
+    } catch (const NativeUnfit& e) {
+      why = e.what();
+    }
+  }
+  if (!nk.vgpr_class) throw NativeUnfit("walk body does not fit the native templates: " + why);
+  nk.info.valu_ops = (p.valu_ops + 1) / 2;  // per step
+  nk.info.scalar_ops = p.scalar_ops;
+  nk.kernel_name = "ns_nat_walk" + std::to_string(T) + "_r" + std::to_string(nk.vgpr_class);
+  const char* tpl = native_template(nk.kernel_name);
+  if (!tpl) throw std::runtime_error("missing native template " + nk.kernel_name);
+  std::string text(tpl);
+  const std::string marker = "; NS_BODY " + nk.kernel_name;
+  size_t pos = text.find(marker);
+  if (pos == std::string::npos) throw std::runtime_error("template placeholder not found");
+  size_t eol = text.find('\n', pos);
+  text.replace(pos, eol - pos, "; circuit body\n" + nk.body);
+  nk.asm_text = std::move(text);
   return nk;
 }
 

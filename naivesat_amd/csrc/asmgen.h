@@ -39,6 +39,12 @@ struct NativeKernel {
 NativeKernel build_native(const Circuit& c, NativeKind kind, const std::vector<uint32_t>& vbit_of_input,
                           uint32_t lane_shift, const std::vector<uint32_t>& outs);
 
+// Walk body of the table-free SATOSTOP walks (templates' ns_nat_walk<T>_r<class>, T = the
+// smallest state tier of 16, 24, 32, 40, 48 that holds c.n): two walk steps and their
+// bookkeeping, with waypoints every 2^k states. Throws NativeUnfit past 48 state bits or
+// when the body fits no register class.
+NativeKernel build_native_walk(const Circuit& c, uint32_t k);
+
 // Number of embedded templates and lookup (templates_data.cpp, generated at build time).
 const char* native_template(const std::string& kernel_name);
 

@@ -62,8 +62,16 @@ void need(bool ok, const char* what) {
 }
 
 // native kernel of a diagnostic kind: 0 SAT, 1 QSAT-A, 2 QSAT-B, 3 table, 4 batch body in
-// the SAT input order, 5 batch body in the QSAT input order
+// the SAT input order, 5 batch body in the QSAT input order, 6 table-free walk body
+// (lane_shift = waypoint bits)
 ns::NativeKernel native_kernel(const ns::Circuit& cc, uint32_t kind, uint32_t lane_shift) {
+  if (kind == 6) {
+    try {
+      return ns::build_native_walk(cc, lane_shift);
+    } catch (const ns::NativeUnfit& e) {
+      throw ns::Error(NS_ERR_UNSUPPORTED, e.what());
+    }
+  }
   std::vector<uint32_t> vb(cc.n), outs;
   for (uint32_t k = 0; k < cc.n; k++) vb[k] = (kind == 1 || kind == 2 || kind == 5) ? cc.n - 1 - k : k;
   if (kind == 3)
@@ -530,7 +538,7 @@ int ns_batch_solve(ns_ctx* ctx, const ns_request* reqs, uint32_t n, ns_request_r
 int ns_native_body(const ns_circuit* c, uint32_t kind, uint32_t lane_shift, char* body, size_t cap,
                    size_t* needed, uint32_t* vgpr_class, uint64_t* code_bytes) {
   return guard([&] {
-    need(c && kind <= 5, "bad argument");
+    need(c && kind <= 6, "bad argument");
     const ns::Circuit& cc = c->c;
     ns::NativeKernel nk = native_kernel(cc, kind, lane_shift);
     std::string text;

@@ -57,7 +57,9 @@ int main(int argc, char** argv) {
     cli::check(ns_ctx_create(dev, &ctx), "device");
     std::printf("Calculate first nexts\n");
     ns_satostop_opts o{};
-    o.flags = a.has("all") ? 0u : NS_STOP_FIRST_PASS;
+    // greedysatostop's own search runs over the next-state table (its rule: the first jump
+    // pass in which a start stops; --all: every start, still over the table)
+    o.flags = a.has("all") ? NS_STOP_TABLE_WALKS : NS_STOP_FIRST_PASS;
     cli::check(ns_satostop_solve(ctx, c, (uint32_t)U, &o, &r, nullptr, nullptr), "search");
     for (uint32_t i = 0; i < r.passes; i++) std::printf("Joining nexts: Stage: %u / %u\n", i, n);
     // wall time after the kernel build (src/greedysatostop.rs:655-688)

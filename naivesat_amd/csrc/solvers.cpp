@@ -894,11 +894,36 @@ bool waypoint_search(Lane& ctx, uint32_t n, uint32_t unknowns, uint32_t k, const
   r.waypoint_bits = k;
   r.walk_ms = walk_ms;
   r.walk_evaluations = ev[0];
-  // a path visits at most 2^(n-k) waypoints before it repeats one
-  for (uint32_t p = 0; p < n - k; p++) {
+  // Coarser levels while the records are many: each chase pass keeps the waypoints whose
+  // index has its low 4 bits zero (no coarser than the starts' spacing n - unknowns) and
+  // follows the finer records between them (one dependent read per record); pointer jumping
+  // then runs over 16 times fewer records per level. NS_WAYPOINT_CHASE=0: no chase.
+  uint32_t kc = k;
+  uint64_t cnt = count;
+  const char* ce = std::getenv("NS_WAYPOINT_CHASE");
+  const bool chase = !(ce && *ce == '0');
+  constexpr uint32_t kChaseBits = 4, kChaseMaxHops = 1u << 16;
+  while (chase && cnt > (1ull << 20) && kc + kChaseBits <= n - unknowns) {
     NS_HIP(launch_fill_u32(ctx.stream, flag, 0u, 1));
     NS_HIP(hipEventRecord(ctx.ev0, ctx.stream));
-    NS_HIP(launch_wp_jump(ctx.stream, cur, nxt, count, flag));
+    NS_HIP(launch_wp_chase(ctx.stream, cur, kChaseBits, nxt, cnt >> kChaseBits, kChaseMaxHops, flag));
+    NS_HIP(hipEventRecord(ctx.ev1, ctx.stream));
+    NS_HIP(hipMemcpyAsync(&hf, flag, 4, hipMemcpyDeviceToHost, ctx.stream));
+    NS_HIP(hipStreamSynchronize(ctx.stream));
+    const double ms = ctx.elapsed_ms();
+    ctx.pass_ms.push_back(ms);
+    r.jump_ms += ms;
+    r.waypoint_passes++;
+    if (hf != 0) break;  // a chain of more than kChaseMaxHops records: jump over this level
+    std::swap(cur, nxt);
+    cnt >>= kChaseBits;
+    kc += kChaseBits;
+  }
+  // a path visits at most 2^(n-kc) waypoints before it repeats one
+  for (uint32_t p = 0; p < n - kc; p++) {
+    NS_HIP(launch_fill_u32(ctx.stream, flag, 0u, 1));
+    NS_HIP(hipEventRecord(ctx.ev0, ctx.stream));
+    NS_HIP(launch_wp_jump(ctx.stream, cur, nxt, cnt, flag));
     NS_HIP(hipEventRecord(ctx.ev1, ctx.stream));
     NS_HIP(hipMemcpyAsync(&hf, flag, 4, hipMemcpyDeviceToHost, ctx.stream));
     NS_HIP(hipStreamSynchronize(ctx.stream));
@@ -911,7 +936,7 @@ bool waypoint_search(Lane& ctx, uint32_t n, uint32_t unknowns, uint32_t k, const
   }
   NS_HIP(launch_fill_u64(ctx.stream, stats, 0ull, 1));
   NS_HIP(launch_fill_u64(ctx.stream, stats + 1, ~0ull, 1));
-  NS_HIP(launch_wp_starts(ctx.stream, cur, n, unknowns, k, d_steps, d_ends, stats));
+  NS_HIP(launch_wp_starts(ctx.stream, cur, n, unknowns, kc, d_steps, d_ends, stats));
   NS_HIP(hipMemcpyAsync(host, stats, 16, hipMemcpyDeviceToHost, ctx.stream));
   NS_HIP(hipStreamSynchronize(ctx.stream));
   return true;
@@ -924,30 +949,59 @@ WalkLaunch table_walk(Lane& ctx, uint32_t n, uint32_t k, const TableView& tv) {
   };
 }
 
-// Waypoint spacing of the walks without a table (k <= n - unknowns): 2^22 waypoints up to
-// 30 state bits; beyond, k balances the walks' tail (the walks still running when the
-// waypoints run out last ~2^k evaluations more) against the record jumps ((n - k) passes
-// over 2^(n-k) records): measured best at k = 10 for 34 state bits (37 ms against 58 ms at
-// k = 8) and k = 11 for 36, about (n - 14) / 2; at most 2^28 records.
+// Waypoint spacing of the walks without a table (k <= n - unknowns). The walks still running
+// when the waypoints run out last ~2^k evaluations more (the tail: each wave waits for the
+// longest of its 2,048 slots, ~8 x 2^k), so the waypoints should be many: 2^(n-k) >= 2^25,
+// at least 8 walks for each of the 2^22 slots, and 2^5 states apart or more (the records and
+// refills of shorter walks cost more than the tail). The chase passes keep the record jumps
+// cheap (without them, k = 8 against 5 was best at 30 state bits). Measured on the
+// Feistel maps: 1.90 ms of walks at k = 5 against 6.7 ms at k = 8 for 30 state bits, 41 ms
+// at k = 8 against 72 ms at k = 11 for 36. At most 2^28 records.
 uint32_t bitwalk_bits(uint32_t n, uint32_t unknowns) {
   if (const char* e = std::getenv("NS_WAYPOINT_BITS")) {
     const long long v = std::atoll(e);
     if (v >= 0 && v <= (long long)n) return (uint32_t)v;
   }
-  const uint32_t k = n <= 30 ? (n > 22 ? n - 22 : 0u) : std::max(n - 28, (n - 14) / 2);
+  const uint32_t k = n >= 27 ? std::max(5u, n - 28) : (n > 22 ? n - 22 : 0u);
   return std::min(n - unknowns, k);
 }
 
-// The waypoint walks without a table: the generated ns_walk kernel (bit-sliced walks, 32 per
-// lane) for this circuit.
-WalkLaunch bit_walk(Lane& ctx, const Circuit& c, uint32_t k, double* compile_ms, uint32_t* cone, uint32_t* valu) {
-  GenInfo info;
-  const std::string src = gen_walk_kernel(c, k, &info);
-  double cms = 0;
-  auto mod = ctx.jit.get(src, "ns_walk", &cms);
-  *compile_ms += cms;
-  *cone = info.cone_gates;
-  *valu = info.valu_ops;
+// The waypoint walks without a table (bit-sliced walks, 32 per lane) for this circuit: the
+// native walk template with the circuit's two-step body (register-allocated gfx950 code,
+// ns_nat_walk<T>_r<class>) up to 48 state bits, the compiled ns_walk kernel otherwise
+// (NS_BACKEND=hiprtc: always). Both take the same arguments (the native one also k) and
+// write the same records.
+WalkLaunch bit_walk(Lane& ctx, const Circuit& c, uint32_t k, double* compile_ms, uint32_t* cone, uint32_t* valu,
+                    uint32_t* backend, uint32_t* vgpr_class) {
+  const char* be = std::getenv("NS_BACKEND");
+  const bool native_ok = !(be && std::strcmp(be, "hiprtc") == 0);
+  const std::string fp = circuit_fingerprint(c) + "/walk/" + std::to_string(k) + (native_ok ? "/n" : "/r");
+  Chosen ch;
+  if (!ctx.jit.find_built(fp, &ch)) {
+    auto t0 = clk::now();
+    double cms = 0;
+    if (native_ok) {
+      try {
+        NativeKernel nk = build_native_walk(c, k);
+        ch.mod = ctx.jit.get_native(nk.kernel_name, nk.body, nk.asm_text, &cms);
+        ch.info = nk.info;
+        ch.native = true;
+        ch.vgpr_class = nk.vgpr_class;
+      } catch (const NativeUnfit&) {
+      }
+    }
+    if (!ch.native) {
+      const std::string src = gen_walk_kernel(c, k, &ch.info);
+      ch.mod = ctx.jit.get(src, "ns_walk", &cms);
+    }
+    *compile_ms += ms_since(t0);
+    ctx.jit.put_built(fp, ch);
+  }
+  *cone = ch.info.cone_gates;
+  *valu = ch.info.valu_ops;
+  *backend = ch.native ? 1 : 0;
+  *vgpr_class = ch.vgpr_class;
+  auto mod = ch.mod;
   return [&ctx, mod, k](void* out, unsigned int* flag, unsigned long long* evals, uint64_t count, uint64_t limit) {
     // enough lanes for two waves per SIMD, and at least 32 walks per lane where there are
     uint64_t lanes = std::min<uint64_t>(1ull << 17, std::max<uint64_t>(256, count / 32));
@@ -957,11 +1011,14 @@ WalkLaunch bit_walk(Lane& ctx, const Circuit& c, uint32_t k, double* compile_ms,
     // random map: allow 16 times that, plus one walk of the full limit
     unsigned long long budget = 16ull * ((per_lane + 31) / 32) * (1ull << k) + limit;
     unsigned long long cnt = count, lim = limit;
+    unsigned int kk = k;
     // the waypoints are handed out from one counter, so that every slot keeps taking walks
     // while any are left
     unsigned long long* queue = reinterpret_cast<unsigned long long*>(ctx.stats.as<char>() + 256);
     NS_HIP(launch_fill_u64(ctx.stream, queue, 0ull, 1));
-    void* args[7] = {&cnt, &lim, &budget, &out, &flag, &evals, &queue};
+    // the launch copies as many arguments as the kernel declares: the compiled ns_walk takes
+    // the first seven, the native template all eight
+    void* args[8] = {&cnt, &lim, &budget, &out, &flag, &evals, &queue, &kk};
     NS_HIP(hipModuleLaunchKernel(mod->fn, (unsigned)(lanes / 256), 1, 1, 256, 1, 1, 0, ctx.stream, args, nullptr));
   };
 }
@@ -976,8 +1033,12 @@ void satostop_buffer_plan(uint32_t n, uint32_t unknowns, uint64_t bytes[5]) {
   // satostop_solve takes them, and the passes (two u64 jump tables) they fall back on
   const uint32_t k = waypoint_bits(n, unknowns);
   const uint64_t records = (k <= n - unknowns && n - k <= 24) ? (1ull << (n - k)) * 16 : 0;
+  // the walks without a table (the default full search) keep their records in the same two
+  // buffers, and fall back on the table when a walk runs too long
+  const uint32_t kb = bitwalk_bits(n, unknowns);
+  const uint64_t bit_records = (kb <= n - unknowns && n - kb <= 28) ? (1ull << (n - kb)) * 16 : 0;
   bytes[0] = (states < 32 ? 32 : states) * 4;
-  bytes[1] = std::max<uint64_t>((states < 2 ? 2 : states) * 8, records);
+  bytes[1] = std::max<uint64_t>(std::max<uint64_t>((states < 2 ? 2 : states) * 8, records), bit_records);
   bytes[2] = bytes[1];
   bytes[3] = (1ull << unknowns) * 16 + 64;
   bytes[4] = 0;  // outputs 32..n: only past 31 state bits
@@ -999,13 +1060,13 @@ void satostop_solve(Lane& ctx, const Circuit& c, uint32_t unknowns, const ns_sat
   const bool first_pass_mode = (o.flags & NS_STOP_FIRST_PASS) != 0;
   const bool full = !first_pass_mode && cover == n && (o.flags & NS_STOP_PASSES) == 0;
   // The full search without a table: walks of 32 states per lane that evaluate the circuit
-  // as they go. The default from 34 state bits on: the 30-bit Feistel search takes 10 ms of
-  // device time this way against 28 ms walking the table, but the generated kernel is
-  // compiled (~0.2 s) where the table kernels come from the native templates (~4 ms), and the
-  // packed image of 34 and 35 state bits would take 75 and 155 GiB.
+  // as they go (the native walk template, ~5 ms of code generation and load), the default
+  // wherever its waypoints fit: the 30-bit Feistel search takes 3.1 ms of device time this way
+  // against 27 ms walking the table, and the packed image of 34 and 35 state bits would take
+  // 75 and 155 GiB. NS_STOP_TABLE_WALKS keeps the table.
   const uint32_t kb = bitwalk_bits(n, unknowns);
   const bool bit_ok = full && kb <= n - unknowns && n - kb <= 28;
-  const bool use_bit = bit_ok && (o.flags & NS_STOP_TABLE_WALKS) == 0 && ((o.flags & NS_STOP_BIT_WALKS) != 0 || n >= 34);
+  const bool use_bit = bit_ok && (o.flags & NS_STOP_TABLE_WALKS) == 0;
   if (n > 35 && !use_bit)
     throw Error(ST_UNSUPPORTED, "over 35 state bits only the full search without a table, with at most 28 unknowns");
   const uint64_t starts_bw = 1ull << unknowns;
@@ -1015,8 +1076,7 @@ void satostop_solve(Lane& ctx, const Circuit& c, uint32_t unknowns, const ns_sat
     unsigned long long* d_ends = d_steps + starts_bw;
     unsigned long long* stats = ctx.stats.as<unsigned long long>();
     unsigned long long host[2] = {0, ~0ull};
-    const WalkLaunch wl = bit_walk(ctx, c, kb, &r.compile_ms, &r.cone_gates, &r.valu_ops);
-    r.backend = 0;
+    const WalkLaunch wl = bit_walk(ctx, c, kb, &r.compile_ms, &r.cone_gates, &r.valu_ops, &r.backend, &r.vgpr_class);
     if (waypoint_search(ctx, n, unknowns, kb, wl, r, d_steps, d_ends, stats, host)) {
       r.method = 2;
       r.n_stopped = host[0];

@@ -45,6 +45,9 @@ hipError_t launch_wp_walk(hipStream_t s, const TableView& tab, unsigned int n, u
                           unsigned long long count, unsigned long long limit, void* out, unsigned int* long_flag,
                           unsigned long long* evals);
 hipError_t launch_wp_jump(hipStream_t s, const void* in, void* out, unsigned long long count, unsigned int* pending);
+// coarse waypoint records (spacing k + d, count_out of them) from the spacing-k ones
+hipError_t launch_wp_chase(hipStream_t s, const void* in, unsigned int d, void* out, unsigned long long count_out,
+                           unsigned int max_hops, unsigned int* overflow);
 hipError_t launch_wp_starts(hipStream_t s, const void* rec, unsigned int n, unsigned int unknowns, unsigned int k,
                             unsigned long long* steps_out, unsigned long long* ends_out, unsigned long long* stats);
 // lanes must be a multiple of 256; out holds one word per lane

@@ -337,6 +337,54 @@ __global__ void __launch_bounds__(256) k_wp_jump(const ulonglong2* __restrict__ 
     atomicOr(pending, 1u);
 }
 
+// Coarser waypoints from finer ones: the waypoints of spacing k + d are the spacing-k ones
+// whose index has its low d bits zero. Coarse record j follows the fine records from fine
+// index j << d until a fine WP_NEXT points at a coarse waypoint (WP_NEXT to its coarse
+// index), a WP_STOP (its stopping state) or a WP_LOOP / WP_LONG record; evaluations add up
+// along the way. A chain that comes back to a fine record it passed (Brent's test against
+// the index saved at power-of-two hop counts) is a cycle without a coarse waypoint: WP_LOOP.
+// A chain of more than max_hops records sets *overflow (the caller then keeps the fine
+// level). Pointer jumping over the coarse records gives the answers pointer jumping over the
+// fine ones would: a path between two coarse waypoints is the same sequence of walks.
+__global__ void __launch_bounds__(256) k_wp_chase(const ulonglong2* __restrict__ in, unsigned int d,
+                                                  ulonglong2* __restrict__ out, unsigned long long count_out,
+                                                  unsigned int max_hops, unsigned int* overflow) {
+  const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
+  const unsigned long long dmask = (1ull << d) - 1ull;
+  unsigned int over = 0u;
+  for (unsigned long long j = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; j < count_out; j += stride) {
+    unsigned long long idx = j << d, saved = idx, steps = 0ull;
+    ulonglong2 res = make_ulonglong2((unsigned long long)WP_LOOP << 62, 0ull);
+    for (unsigned int h = 1;; h++) {
+      const ulonglong2 e = in[idx];
+      const unsigned int kind = (unsigned int)(e.x >> 62);
+      if (kind == WP_STOP) {
+        res = make_ulonglong2(e.x, steps + e.y);
+        break;
+      }
+      if (kind != WP_NEXT) {  // WP_LOOP or WP_LONG: never stops
+        res = make_ulonglong2(e.x & ~WP_TARGET, 0ull);
+        break;
+      }
+      steps += e.y;
+      const unsigned long long t = e.x & WP_TARGET;
+      if ((t & dmask) == 0ull) {
+        res = make_ulonglong2((t >> d) | ((unsigned long long)WP_NEXT << 62), steps);
+        break;
+      }
+      if (t == saved) break;  // res stays WP_LOOP
+      if (h >= max_hops) {
+        over = 1u;
+        break;
+      }
+      if ((h & (h - 1u)) == 0u) saved = t;
+      idx = t;
+    }
+    out[j] = res;
+  }
+  if (__syncthreads_or((int)over) && threadIdx.x == 0) atomicOr(overflow, 1u);
+}
+
 // Per unknown u: the record of its start's waypoint (u << (n - unknowns - k)).
 __global__ void __launch_bounds__(256) k_wp_starts(const ulonglong2* __restrict__ rec, unsigned int n,
                                                    unsigned int unknowns, unsigned int k,
@@ -641,6 +689,14 @@ hipError_t launch_wp_walk(hipStream_t s, const TableView& tab, unsigned int n, u
 hipError_t launch_wp_jump(hipStream_t s, const void* in, void* out, unsigned long long count, unsigned int* pending) {
   hipLaunchKernelGGL(k_wp_jump, dim3(looped_blocks(count, 256)), dim3(256), 0, s,
                      static_cast<const ulonglong2*>(in), static_cast<ulonglong2*>(out), count, pending);
+  return hipGetLastError();
+}
+
+hipError_t launch_wp_chase(hipStream_t s, const void* in, unsigned int d, void* out, unsigned long long count_out,
+                           unsigned int max_hops, unsigned int* overflow) {
+  hipLaunchKernelGGL(k_wp_chase, dim3(looped_blocks(count_out, 256)), dim3(256), 0, s,
+                     static_cast<const ulonglong2*>(in), d, static_cast<ulonglong2*>(out), count_out, max_hops,
+                     overflow);
   return hipGetLastError();
 }
 

@@ -385,6 +385,139 @@ extern "C" __global__ void __launch_bounds__(256) ns_nat_bqsat_{cls}(const NsBat
     return sat + qsat
 
 
+# Walks without a table (SATOSTOP): state tiers (the template holds T state bits per walk
+# slot; circuits of n <= T state bits leave the bits above n at zero) and the register classes
+# each tier is built for. The 64 KiB of slot records in LDS per block already hold a CU to two
+# blocks (two waves per SIMD), so no class below r256 is built.
+WALK_TIERS = {16: ("r256", "r512"), 24: ("r256", "r512"), 32: ("r256", "r512"), 40: ("r256", "r512"),
+              48: ("r256", "r512")}
+WALK_MASKS = ("active", "fin", "fstop", "fwp", "floop", "a0", "a1", "a2", "a3")
+
+
+def walk_regs(T):
+    """fixed registers of the walk body: s (state), f (final state), v (saved state), masks"""
+    base = 8
+    return {"s": base, "f": base + T, "v": base + 2 * T, "m": base + 3 * T, "pool_lo": base + 3 * T + len(WALK_MASKS)}
+
+
+def walk_kernel(T, cls, pool_hi):
+    name = f"ns_nat_walk{T}_{cls}"
+    R = walk_regs(T)
+
+    def each(fmt):
+        return "".join(fmt.replace("@", str(i)) for i in range(T))
+
+    ops = ([f'"+{{v{R["s"] + i}}}"(s{i})' for i in range(T)] + [f'"+{{v{R["f"] + i}}}"(f{i})' for i in range(T)] +
+           [f'"+{{v{R["v"] + i}}}"(v{i})' for i in range(T)] +
+           [f'"+{{v{R["m"] + j}}}"({m})' for j, m in enumerate(WALK_MASKS)])
+    refill = (
+        "    if (!drained) {\n"
+        "      const u32 r = __popc(need);\n"
+        "      u32 incl = r;\n"
+        "      for (u32 off = 1; off < 64u; off <<= 1) {\n"
+        "        const u32 y = __shfl_up(incl, off);\n"
+        "        if (lane >= off) incl += y;\n"
+        "      }\n"
+        "      const u32 total = __shfl(incl, 63);\n"
+        "      u64 base = 0ull;\n"
+        "      if (total != 0u) {\n"
+        "        if (lane == 0u) base = atomicAdd(queue, (u64)total);\n"
+        "        base = __shfl(base, 0);\n"
+        "      }\n"
+        "      if (base + total >= count) drained = true;\n"
+        "      u64 nx = base + (incl - r);\n"
+        "      for (u32 rem = need; rem; rem &= rem - 1u) {\n"
+        "        const u32 b = __builtin_ctz(rem), mb = 1u << b;\n"
+        "        const u64 idx = nx++;\n"
+        "        if (idx >= count) break;\n"
+        "        const u64 st = idx << k;\n"
+        + each("        s@ = (s@ & ~mb) | ((u32)((st >> @) & 1ull) * mb);\n        v@ = (v@ & ~mb) | (s@ & mb);\n") +
+        "        meta[b * 256u + tid] = (u64)t | (idx << 32);\n"
+        "        active |= mb;\n"
+        "      }\n"
+        "    }\n").replace("\\n", "\n")
+    return f'''
+// SATOSTOP walks without a table: 32 walk slots per lane, bit-sliced (bit b of s@ = state
+// bit @ of slot b; up to {T} state bits, the bits above the circuit's n stay zero). A slot
+// walks from its waypoint (a state whose low k bits are zero) until the next waypoint, STOP,
+// a return to its saved state (Brent: saved after each power of two of its evaluations) or
+// the limit, and writes out[waypoint] = (target | kind << 62, evaluations), kind 0 waypoint
+// (target = its index), 1 STOP (target = the stopping state), 2 loop, 3 limit; waypoints
+// come from one counter (queue), so every slot keeps taking walks while any are left. The
+// body runs two walk steps with their bookkeeping (per slot: STOP, waypoint and loop tests,
+// the final state, the evaluation ages of ended slots); every 16 steps this code records
+// the ended walks, saves states and refills the slots.
+extern "C" __global__ void __launch_bounds__(256) {name}(u64 count, u64 limit, u64 budget, ulonglong2* __restrict__ out,
+                                                      u32* long_flag, u64* evals, u64* queue, u32 k) {{
+  __shared__ u64 meta[32 * 256];  // per slot: start time (bits 0..31), waypoint index (32..63)
+  const u32 tid = threadIdx.x, lane = tid & 63u;
+  const u64 lim = limit < 0x7fffffffull ? limit : 0x7fffffffull;
+{each("  u32 s@ = 0u, f@ = 0u, v@ = 0u;" + chr(10))}  u32 active = 0u, fin = 0u, fstop = 0u, fwp = 0u, floop = 0u, t = 0u;
+  u32 a0 = 0u, a1 = 0u, a2 = 0u, a3 = 0u;
+  u64 used = 0ull, ev = 0ull;
+  bool drained = false;
+  {{
+    const u32 need = 0xffffffffu;
+{refill}  }}
+  // every lane of the wave runs every iteration (the loop test is wave-uniform), so all take
+  // part in the refills' prefix sums
+  while (__any((active | fin) != 0u)) {{
+    used += 2ull;
+    if (used > budget) {{
+      atomicOr(long_flag, 1u);
+      break;
+    }}
+    if ((used & 255ull) == 0ull && __hip_atomic_load(long_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u)
+      break;
+    asm volatile("; NS_BODY {name}" : {", ".join(ops)} : : {clobbers(R["pool_lo"], pool_hi, cls == "r512")});
+    t += 2u;
+    if ((t & 15u) == 0u) {{
+      // evaluations of the window: 16 per slot running at its start, less the steps since
+      // each ended slot finished (its age)
+      ev += 16ull * (u64)__popc(active | fin) -
+            (u64)(__popc(a0) + 2u * __popc(a1) + 4u * __popc(a2) + 8u * __popc(a3));
+      u32 save = 0u, lng = 0u;
+      for (u32 rem = active; rem; rem &= rem - 1u) {{
+        const u32 b = __builtin_ctz(rem);
+        const u32 start = (u32)meta[b * 256u + tid];
+        const u32 steps = t - start, before = steps > 16u ? steps - 16u : 0u;
+        if ((u64)steps >= lim) lng |= 1u << b;
+        else if (before == 0u || __builtin_clz(steps) < __builtin_clz(before)) save |= 1u << b;
+      }}
+      if (save) {{
+{each("        v@ = (v@ & ~save) | (s@ & save);" + chr(10))}      }}
+      if (lng) {{
+        atomicOr(long_flag, 1u);
+        fin |= lng;
+        active &= ~lng;
+      }}
+      for (u32 rem = fin; rem; rem &= rem - 1u) {{
+        const u32 b = __builtin_ctz(rem), mb = 1u << b;
+        const u64 m = meta[b * 256u + tid];
+        const u32 age = ((a0 >> b) & 1u) | (((a1 >> b) & 1u) << 1) | (((a2 >> b) & 1u) << 2) | (((a3 >> b) & 1u) << 3);
+        const u32 steps = t - age - (u32)m;
+        const u32 kind = (fstop & mb) ? 1u : (fwp & mb) ? 0u : (floop & mb) ? 2u : 3u;
+        u64 st = 0ull;
+{each("        st |= (u64)((f@ >> b) & 1u) << @;" + chr(10))}        const u64 target = kind == 1u ? st : kind == 0u ? (st >> k) : 0ull;
+        out[m >> 32] = make_ulonglong2(target | ((u64)kind << 62), steps);
+      }}
+      {{
+        const u32 need = fin;
+{refill}      }}
+      fin = 0u;
+      fstop = 0u;
+      fwp = 0u;
+      floop = 0u;
+      a0 = a1 = a2 = a3 = 0u;
+    }}
+  }}
+  // evaluations of the wave, into one of 64 count slots 16 words apart
+  for (int off = 32; off > 0; off >>= 1) ev += __shfl_xor(ev, off);
+  if (lane == 0u && ev != 0ull) atomicAdd(evals + ((blockIdx.x & 63u) << 4), ev);
+}}
+'''
+
+
 def main():
     out = [HEAD, BATCH_HEAD]
     batch = []
@@ -397,6 +530,9 @@ def main():
             # one kernel per compilation: NS_T_<name> selects it
             out.append(f"#if defined(NS_T_ns_nat_{kind}_{cls})\n{src}#endif\n")
         batch.append(batch_kernels(cls, hi, ci))
+    for T, classes in WALK_TIERS.items():
+        for cls in classes:
+            out.append(f"#if defined(NS_T_ns_nat_walk{T}_{cls})\n{walk_kernel(T, cls, CLASSES[cls][1])}#endif\n")
     # the batch kernels of every class in one compilation (NS_T_ns_nat_batch): a batch's code
     # object holds them all over one body region, so each class runs at its own occupancy
     out.append("#if defined(NS_T_ns_nat_batch)\n" + "".join(batch) + "#endif\n")

@@ -117,7 +117,7 @@ def test_config3_full_size_golden(ctx):
     g = json.load(open(GOLDEN))["cfg3"]
     c, target = ns.Circuit.gen_satostop(4, 15, 6, 300, 14)
     r = ctx.satostop_solve(c, 14, wide=False, method="bits")
-    assert (r["method"], r["waypoint_bits"]) == (2, 8)
+    assert (r["method"], r["waypoint_bits"]) == (2, 5)
     # every state once, but those on the few short cycles without a waypoint
     assert (1 << 30) - (1 << 12) < r["walk_evaluations"] <= 1 << 30
     assert r["n_stopped"] == g["n_stopped"]
@@ -126,7 +126,7 @@ def test_config3_full_size_golden(ctx):
 
 
 def test_36_bit_state_space(ctx):
-    # n = 36: past the packed image; 2^25 waypoints 2^11 states apart. The Feistel map is a
+    # n = 36: past the packed image; 2^28 waypoints 2^8 states apart. The Feistel map is a
     # permutation with one STOP transition, so every stopping start ends at the target after a
     # distinct number of evaluations; the short ones are checked by walking on the host
     n, U = 36, 14
@@ -135,7 +135,7 @@ def test_36_bit_state_space(ctx):
         r = ctx.satostop_solve(c, U)
     finally:
         ctx.trim()
-    assert (r["method"], r["waypoint_bits"]) == (2, 11)
+    assert (r["method"], r["waypoint_bits"]) == (2, 8)
     steps, ends = r["steps_arr"], r["ends_arr"]
     stopped = steps > 0
     assert r["n_stopped"] == int(stopped.sum()) > 0
@@ -151,3 +151,33 @@ def test_36_bit_state_space(ctx):
         else:
             assert steps[u] == 0 or steps[u] > 64, u
     assert short >= 1
+
+
+def test_native_and_compiled_walks_agree(ctx, monkeypatch):
+    # the native walk template and the compiled ns_walk kernel: same records, so the same
+    # per-start answers and the same evaluation count
+    c, _ = ns.Circuit.gen_satostop(10601, 12, 6, 150, 12)
+    r1 = check(ctx, c.arrays(), 12)
+    assert r1["backend"] == 1 and r1["vgpr_class"] in (256, 512)
+    monkeypatch.setenv("NS_BACKEND", "hiprtc")
+    r2 = ctx.satostop_solve(c, 12, method="bits")
+    assert r2["backend"] == 0
+    assert np.array_equal(r1["steps_arr"], r2["steps_arr"]) and np.array_equal(r1["ends_arr"], r2["ends_arr"])
+    assert r1["walk_evaluations"] == r2["walk_evaluations"]
+
+
+@pytest.mark.parametrize("n,bits", [(22, 1), (24, 1), (24, 3)])
+def test_chase_levels_on_random_maps(ctx, monkeypatch, n, bits):
+    # dense waypoints: 2^(n-bits) records, coarsened by chase passes (one waypoint in 16 per
+    # level) before the pointer jumping; random maps run into cycles of fine waypoints
+    # without a coarse one, which the chase's Brent test ends. Oracle answers, and the same
+    # records as pointer jumping alone
+    rng = np.random.default_rng(10700 + n + bits)
+    d = random_circuit(rng, n, 4 * n + 40, nout=n + 1, recent=3 * n + 4)
+    monkeypatch.setenv("NS_WAYPOINT_BITS", str(bits))
+    r = check(ctx, d, 8)
+    assert (r["method"], r["waypoint_bits"]) == (2, bits)
+    monkeypatch.setenv("NS_WAYPOINT_CHASE", "0")
+    r0 = ctx.satostop_solve(mk(d), 8, method="bits")
+    assert np.array_equal(r["steps_arr"], r0["steps_arr"]) and np.array_equal(r["ends_arr"], r0["ends_arr"])
+    assert r0["waypoint_passes"] != r["waypoint_passes"]

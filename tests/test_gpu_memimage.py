@@ -27,7 +27,7 @@ def test_pack_matches_reference_layout(ctx):
 
 
 def check_packed(ctx, d, U):
-    r = ctx.satostop_solve(mk(d), U)
+    r = ctx.satostop_solve(mk(d), U, method="table")
     o = oracle.OracleCircuit.from_dict(d).satostop(U, threads=8)
     assert np.array_equal(r["steps_arr"], o["steps"]), (d["n"], U)
     assert np.array_equal(r["ends_arr"], o["ends"]), (d["n"], U)

@@ -20,6 +20,9 @@ def mk(d):
 
 
 def check(ctx, d, U, **kw):
+    # these tests are of the search over the table (its walks and passes); the default full
+    # search (walks without a table) has its own in test_gpu_bitwalk.py
+    kw.setdefault("method", "table")
     r = ctx.satostop_solve(mk(d), U, **kw)
     o = oracle.OracleCircuit.from_dict(d).satostop(U, threads=8)
     assert np.array_equal(r["steps_arr"], o["steps"]), (d["n"], U)
@@ -166,7 +169,7 @@ def test_config3_full_size_golden(ctx):
     c, target = ns.Circuit.gen_satostop(4, 15, 6, 300, 14)
     assert target == g["target"]
     for passes in (False, True):
-        r = ctx.satostop_solve(c, 14, wide=False, passes=passes)  # the golden hashes are of uint32 arrays
+        r = ctx.satostop_solve(c, 14, wide=False, passes=passes, method="table")  # golden hashes: uint32 arrays
         if passes:
             # P passes over the table plus walks of 2^(30-P) jumps cover 2^30 evaluations
             assert r["method"] == 0 and r["walk_jumps"] << r["passes"] == 1 << 30
@@ -264,7 +267,7 @@ def test_32_bit_state_space(ctx, monkeypatch):
     n, U = 32, 14
     c, target = ns.Circuit.gen_satostop(7401, 16, 6, 100, U)
     ctx.trim()
-    r = ctx.satostop_solve(c, U)
+    r = ctx.satostop_solve(c, U, method="table")
     steps, ends = r["steps_arr"], r["ends_arr"]
     stopped = steps > 0
     assert r["n_stopped"] == int(stopped.sum()) > 0

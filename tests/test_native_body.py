@@ -12,10 +12,14 @@ M32 = 0xFFFFFFFF
 LANES = np.arange(64, dtype=np.uint32)
 
 
-def run_body(body, lm_base, wu):
+def run_body(body, lm_base, wu, vregs=None):
+    """vregs: initial VGPRs (register -> 64 lane values) instead of the lane masks"""
     v, s, acc = {}, {20: wu & M32, 21: (wu >> 32) & M32}, {}
-    for i in range(6):
-        v[lm_base + i] = np.where((LANES >> i) & 1, np.uint32(M32), np.uint32(0)).astype(np.uint32)
+    if vregs is not None:
+        v = {r: x.copy() for r, x in vregs.items()}
+    else:
+        for i in range(6):
+            v[lm_base + i] = np.where((LANES >> i) & 1, np.uint32(M32), np.uint32(0)).astype(np.uint32)
 
     def val(tok):
         if tok.startswith("v"):
