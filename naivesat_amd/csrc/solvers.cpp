@@ -862,9 +862,11 @@ double build_packed_table(Lane& ctx, const Circuit& c, const ns_satostop_opts& o
 // A walk kernel enqueued on ctx.stream: (records, flag, evaluation slots, waypoints, limit).
 using WalkLaunch = std::function<void(void*, unsigned int*, unsigned long long*, uint64_t, uint64_t)>;
 
+// check_records: the walk kernel also counts the records it writes (word 1 of each count
+// slot); every waypoint must have exactly one.
 bool waypoint_search(Lane& ctx, uint32_t n, uint32_t unknowns, uint32_t k, const WalkLaunch& walk,
                      ns_satostop_result& r, unsigned long long* d_steps, unsigned long long* d_ends,
-                     unsigned long long* stats, unsigned long long host[2]) {
+                     unsigned long long* stats, unsigned long long host[2], bool check_records = false) {
   const uint64_t count = 1ull << (n - k);
   // a walk ends after this many evaluations (far above the 2^k a walk takes on average)
   uint64_t limit = std::max<uint64_t>(1ull << 18, 1ull << std::min(k + 6, 31u));
@@ -891,6 +893,13 @@ bool waypoint_search(Lane& ctx, uint32_t n, uint32_t unknowns, uint32_t k, const
   if (hf != 0) return false;
   unsigned long long ev[2];
   read_counts_done(cc, ev);
+  if (check_records) {
+    uint64_t recs = 0;
+    for (uint64_t s = 0; s < kSlots; s++) recs += cc.slots[s * kSlotStride + 1];
+    if (recs != count)
+      throw Error(ST_INTERNAL, "waypoint walks wrote " + std::to_string(recs) + " records for " +
+                                   std::to_string(count) + " waypoints");
+  }
   r.waypoint_bits = k;
   r.walk_ms = walk_ms;
   r.walk_evaluations = ev[0];
@@ -1077,7 +1086,7 @@ void satostop_solve(Lane& ctx, const Circuit& c, uint32_t unknowns, const ns_sat
     unsigned long long* stats = ctx.stats.as<unsigned long long>();
     unsigned long long host[2] = {0, ~0ull};
     const WalkLaunch wl = bit_walk(ctx, c, kb, &r.compile_ms, &r.cone_gates, &r.valu_ops, &r.backend, &r.vgpr_class);
-    if (waypoint_search(ctx, n, unknowns, kb, wl, r, d_steps, d_ends, stats, host)) {
+    if (waypoint_search(ctx, n, unknowns, kb, wl, r, d_steps, d_ends, stats, host, r.backend == 1)) {
       r.method = 2;
       r.n_stopped = host[0];
       r.any_stop = r.n_stopped != 0;

@@ -67,6 +67,21 @@ __device__ inline void ns_wave_stats(u64 cnt, u64 best, u64* stats) {
 }
 
 __device__ inline u32 ns_red(u32 x, u32 sh, u32 all) { return all ? (x & (x >> sh)) : (x | (x >> sh)); }
+
+// Exclusive prefix sum over the wave's lanes of r (0..63) and, in *total, the wave's sum: one
+// ballot per bit of r, each counted below the lane (mbcnt) and in all (no lane shuffles, no
+// per-lane address registers)
+__device__ inline u32 ns_wave_excl(u32 r, u32* total) {
+  u32 ex = 0u, tot = 0u;
+#pragma unroll
+  for (u32 j = 0; j < 6u; j++) {
+    const u64 m = __ballot((r >> j) & 1u);
+    ex += __builtin_amdgcn_mbcnt_hi((u32)(m >> 32), __builtin_amdgcn_mbcnt_lo((u32)m, 0u)) << j;
+    tot += (u32)__popcll(m) << j;
+  }
+  *total = tot;
+  return ex;
+}
 '''
 
 
@@ -400,54 +415,124 @@ def walk_regs(T):
     return {"s": base, "f": base + T, "v": base + 2 * T, "m": base + 3 * T, "pool_lo": base + 3 * T + len(WALK_MASKS)}
 
 
+WALK_CHUNK = 2048  # waypoints a wave takes from the queue at once (one full refill of its slots)
+WALK_BULK = 384  # refills or ended walks per wave from which the transposed path is taken
+
+
+def t32(arr):
+    """32 x 32 bit transpose of arr[0..31] (word k bit b <-> word b bit k) in registers"""
+    return (f"    {{\n      u32 msk = 0x0000FFFFu;\n#pragma unroll\n      for (int sh = 16; sh != 0; sh >>= 1, msk ^= (msk << sh)) {{\n"
+            f"#pragma unroll\n        for (int q = 0; q < 32; q = ((q | sh) + 1) & ~sh) {{\n"
+            f"          const u32 tt = (({arr}[q] >> sh) ^ {arr}[q | sh]) & msk;\n"
+            f"          {arr}[q] ^= tt << sh;\n          {arr}[q | sh] ^= tt;\n        }}\n      }}\n    }}\n")
+
+
 def walk_kernel(T, cls, pool_hi):
     name = f"ns_nat_walk{T}_{cls}"
     R = walk_regs(T)
 
-    def each(fmt):
-        return "".join(fmt.replace("@", str(i)) for i in range(T))
+    def each(fmt, lo=0, hi=None):
+        return "".join(fmt.replace("@", str(i)) for i in range(lo, T if hi is None else hi))
 
     ops = ([f'"+{{v{R["s"] + i}}}"(s{i})' for i in range(T)] + [f'"+{{v{R["f"] + i}}}"(f{i})' for i in range(T)] +
            [f'"+{{v{R["v"] + i}}}"(v{i})' for i in range(T)] +
            [f'"+{{v{R["m"] + j}}}"({m})' for j, m in enumerate(WALK_MASKS)])
+    wide = T > 32
+    # Refill of the slots in `need` from the wave's waypoint range [lb, le), then its next
+    # chunk nb (taken from the prefetch or the queue): rank q of the wave's refills (lanes in
+    # order, a lane's slots in bit order) gets waypoint q < avail ? lb + q : nb + q - avail.
     refill = (
-        "    if (!drained) {\n"
-        "      const u32 r = __popc(need);\n"
-        "      u32 incl = r;\n"
-        "      for (u32 off = 1; off < 64u; off <<= 1) {\n"
-        "        const u32 y = __shfl_up(incl, off);\n"
-        "        if (lane >= off) incl += y;\n"
-        "      }\n"
-        "      const u32 total = __shfl(incl, 63);\n"
-        "      u64 base = 0ull;\n"
+        "    if (!exhausted) {\n"
+        "      u32 total;\n"
+        "      const u32 q0 = ns_wave_excl(__popc(need), &total);\n"
         "      if (total != 0u) {\n"
-        "        if (lane == 0u) base = atomicAdd(queue, (u64)total);\n"
-        "        base = __shfl(base, 0);\n"
+        "        const u32 avail = le - lb;\n"
+        "        u32 nb = 0u;\n"
+        "        if (total > avail) {\n"
+        "          if (have_pf) {\n"
+        "            nb = __builtin_amdgcn_readfirstlane(pf);\n"
+        "            have_pf = false;\n"
+        "          } else {\n"
+        "            u32 b0 = 0u;\n"
+        f"            if (lane == 0u) b0 = (u32)atomicAdd(queue, {WALK_CHUNK}ull);\n"
+        "            nb = __builtin_amdgcn_readfirstlane(b0);\n"
+        "          }\n"
+        "        }\n"
+        f"        if (total < {WALK_BULK}u) {{\n"
+        "          u32 q = q0;\n"
+        "          for (u32 rem = need; rem; rem &= rem - 1u) {\n"
+        "            const u32 b = __builtin_ctz(rem), mb = 1u << b;\n"
+        "            const u64 idx = q < avail ? (u64)(lb + q) : (u64)nb + (q - avail);\n"
+        "            q++;\n"
+        "            if (idx >= count) break;\n"
+        "            const u64 st = idx << k;\n"
+        + each("            s@ = (s@ & ~mb) | ((u32)((st >> @) & 1ull) * mb);\n            v@ = (v@ & ~mb) | (s@ & mb);\n") +
+        "            meta[b * 256u + tid] = (u64)t | (idx << 32);\n"
+        "            active |= mb;\n"
+        "          }\n"
+        "        } else {\n"
+        "          // many refills: the new states of all 32 slots at once, by bit transposes\n"
+        "          u32 X[32];\n"
+        + ("          u32 XH[32];\n" if wide else "") +
+        "          u32 vm = 0u, j = 0u;\n"
+        "#pragma unroll\n"
+        "          for (u32 b = 0; b < 32u; b++) {\n"
+        "            const u32 hb = (need >> b) & 1u;\n"
+        "            const u32 q = q0 + j;\n"
+        "            const u64 idx = q < avail ? (u64)(lb + q) : (u64)nb + (q - avail);\n"
+        "            const bool ok = hb != 0u && idx < count;\n"
+        "            const u64 st = ok ? idx << k : 0ull;\n"
+        "            X[b] = (u32)st;\n"
+        + ("            XH[b] = (u32)(st >> 32);\n" if wide else "") +
+        "            if (ok) meta[b * 256u + tid] = (u64)t | (idx << 32);\n"
+        "            vm |= (ok ? 1u : 0u) << b;\n"
+        "            j += hb;\n"
+        "          }\n"
+        + t32("X") + (t32("XH") if wide else "")
+        + each("          s@ = (s@ & ~vm) | (X[@] & vm);\n          v@ = (v@ & ~vm) | (s@ & vm);\n", 0, min(T, 32))
+        + (each("          s@ = (s@ & ~vm) | (XH[@ - 32] & vm);\n          v@ = (v@ & ~vm) | (s@ & vm);\n", 32, T)
+           if wide else "") +
+        "          active |= vm;\n"
+        "        }\n"
+        "        if (total > avail) {\n"
+        "          lb = nb + (total - avail);\n"
+        f"          le = nb + {WALK_CHUNK}u;\n"
+        "        } else {\n"
+        "          lb += total;\n"
+        "        }\n"
+        "        if ((u64)le > count) le = (u32)count;\n"
+        "        if ((u64)lb >= count) exhausted = true;\n"
+        "        // the next chunk is asked for one refill ahead: its atomic's latency overlaps\n"
+        "        // the steps until it is used\n"
+        f"        if (!exhausted && !have_pf && le - lb < {WALK_CHUNK}u) {{\n"
+        f"          if (lane == 0u) pf = (u32)atomicAdd(queue, {WALK_CHUNK}ull);\n"
+        "          have_pf = true;\n"
+        "        }\n"
         "      }\n"
-        "      if (base + total >= count) drained = true;\n"
-        "      u64 nx = base + (incl - r);\n"
-        "      for (u32 rem = need; rem; rem &= rem - 1u) {\n"
-        "        const u32 b = __builtin_ctz(rem), mb = 1u << b;\n"
-        "        const u64 idx = nx++;\n"
-        "        if (idx >= count) break;\n"
-        "        const u64 st = idx << k;\n"
-        + each("        s@ = (s@ & ~mb) | ((u32)((st >> @) & 1ull) * mb);\n        v@ = (v@ & ~mb) | (s@ & mb);\n") +
-        "        meta[b * 256u + tid] = (u64)t | (idx << 32);\n"
-        "        active |= mb;\n"
-        "      }\n"
-        "    }\n").replace("\\n", "\n")
-    return f'''
+        "    }\n")
+    # the records of the slots that ended in the window
+    rec_one = (
+        "        const u64 m = meta[b * 256u + tid];\n"
+        "        const u32 age = ((a0 >> b) & 1u) | (((a1 >> b) & 1u) << 1) | (((a2 >> b) & 1u) << 2) | (((a3 >> b) & 1u) << 3);\n"
+        "        const u32 steps = t - age - (u32)m;\n"
+        "        const u32 kind = (fstop & mb) ? 1u : (fwp & mb) ? 0u : (floop & mb) ? 2u : 3u;\n"
+        "        const u64 target = kind == 1u ? st : kind == 0u ? (st >> k) : 0ull;\n"
+        "        out[m >> 32] = make_ulonglong2(target | ((u64)kind << 62), steps);\n"
+        "        nrec++;\n")
+    return f"""
 // SATOSTOP walks without a table: 32 walk slots per lane, bit-sliced (bit b of s@ = state
 // bit @ of slot b; up to {T} state bits, the bits above the circuit's n stay zero). A slot
 // walks from its waypoint (a state whose low k bits are zero) until the next waypoint, STOP,
 // a return to its saved state (Brent: saved after each power of two of its evaluations) or
 // the limit, and writes out[waypoint] = (target | kind << 62, evaluations), kind 0 waypoint
-// (target = its index), 1 STOP (target = the stopping state), 2 loop, 3 limit; waypoints
-// come from one counter (queue), so every slot keeps taking walks while any are left. The
-// body runs two walk steps with their bookkeeping (per slot: STOP, waypoint and loop tests,
-// the final state, the evaluation ages of ended slots); every 16 steps this code records
-// the ended walks, saves states and refills the slots.
-extern "C" __global__ void __launch_bounds__(256) {name}(u64 count, u64 limit, u64 budget, ulonglong2* __restrict__ out,
+// (target = its index), 1 STOP (target = the stopping state), 2 loop, 3 limit. Waypoints come
+// from one counter (queue) in chunks of {WALK_CHUNK} per wave, the next chunk asked for one
+// refill ahead, so every slot keeps taking walks while any are left. The body runs two walk
+// steps with their bookkeeping (per slot: STOP, waypoint and loop tests, the final state,
+// the evaluation ages of ended slots); every 16 steps this code records the ended walks,
+// saves states and refills the slots (by bit transposes when a wave has many of them). The
+// walks' evaluations go to evals[16 s] and their records to evals[16 s + 1] (64 slots s).
+extern "C" __global__ void __launch_bounds__(256){" __attribute__((amdgpu_waves_per_eu(2, 2)))" if cls == "r256" else ""} {name}(u64 count, u64 limit, u64 budget, ulonglong2* __restrict__ out,
                                                       u32* long_flag, u64* evals, u64* queue, u32 k) {{
   __shared__ u64 meta[32 * 256];  // per slot: start time (bits 0..31), waypoint index (32..63)
   const u32 tid = threadIdx.x, lane = tid & 63u;
@@ -455,13 +540,18 @@ extern "C" __global__ void __launch_bounds__(256) {name}(u64 count, u64 limit, u
 {each("  u32 s@ = 0u, f@ = 0u, v@ = 0u;" + chr(10))}  u32 active = 0u, fin = 0u, fstop = 0u, fwp = 0u, floop = 0u, t = 0u;
   u32 a0 = 0u, a1 = 0u, a2 = 0u, a3 = 0u;
   u64 used = 0ull, ev = 0ull;
-  bool drained = false;
+  u32 nrec = 0u;
+  // the wave's waypoint range [lb, le) (indices stay below 2^32: at most 2^28 waypoints, and the
+  // queue passes them by at most one chunk per wave); pf: the prefetched chunk's base (lane 0)
+  u32 lb = 0u, le = 0u, pf = 0u;
+  bool have_pf = false, exhausted = false;
   {{
     const u32 need = 0xffffffffu;
 {refill}  }}
   // every lane of the wave runs every iteration (the loop test is wave-uniform), so all take
-  // part in the refills' prefix sums
-  while (__any((active | fin) != 0u)) {{
+  // part in the refills' prefix sums; a wave with waypoints left keeps stepping to its next
+  // refill even with no walk running
+  while (__any((active | fin) != 0u) || !exhausted) {{
     used += 2ull;
     if (used > budget) {{
       atomicOr(long_flag, 1u);
@@ -491,18 +581,26 @@ extern "C" __global__ void __launch_bounds__(256) {name}(u64 count, u64 limit, u
         fin |= lng;
         active &= ~lng;
       }}
-      for (u32 rem = fin; rem; rem &= rem - 1u) {{
-        const u32 b = __builtin_ctz(rem), mb = 1u << b;
-        const u64 m = meta[b * 256u + tid];
-        const u32 age = ((a0 >> b) & 1u) | (((a1 >> b) & 1u) << 1) | (((a2 >> b) & 1u) << 2) | (((a3 >> b) & 1u) << 3);
-        const u32 steps = t - age - (u32)m;
-        const u32 kind = (fstop & mb) ? 1u : (fwp & mb) ? 0u : (floop & mb) ? 2u : 3u;
-        u64 st = 0ull;
-{each("        st |= (u64)((f@ >> b) & 1u) << @;" + chr(10))}        const u64 target = kind == 1u ? st : kind == 0u ? (st >> k) : 0ull;
-        out[m >> 32] = make_ulonglong2(target | ((u64)kind << 62), steps);
+      u32 nfin;
+      (void)ns_wave_excl(__popc(fin), &nfin);
+      if (nfin < {WALK_BULK}u) {{
+        for (u32 rem = fin; rem; rem &= rem - 1u) {{
+          const u32 b = __builtin_ctz(rem), mb = 1u << b;
+          u64 st = 0ull;
+{each("          st |= (u64)((f@ >> b) & 1u) << @;" + chr(10))}{rec_one.replace(chr(10) + "        ", chr(10) + "          ").replace("        const u64 m", "          const u64 m", 1)}        }}
+      }} else {{
+        // many ended walks: their final states by bit transposes of f
+        u32 Z[32];
+{each("        Z[@] = f@;" + chr(10), 0, min(T, 32))}{each("        Z[@] = 0u;" + chr(10), T, 32) if T < 32 else ""}{t32("Z")}{("        u32 ZH[32];" + chr(10) + each("        ZH[@ - 32] = f@;" + chr(10), 32, T) + "".join(f"        ZH[{i}] = 0u;" + chr(10) for i in range(T - 32, 32)) + t32("ZH")) if wide else ""}#pragma unroll
+        for (u32 b = 0; b < 32u; b++) {{
+          const u32 mb = 1u << b;
+          if (fin & mb) {{
+            const u64 st = (u64)Z[b]{" | ((u64)ZH[b] << 32)" if wide else ""};
+{rec_one.replace(chr(10) + "        ", chr(10) + "            ").replace("        const u64 m", "            const u64 m", 1)}          }}
+        }}
       }}
       {{
-        const u32 need = fin;
+        const u32 need = ~active;
 {refill}      }}
       fin = 0u;
       fstop = 0u;
@@ -511,11 +609,15 @@ extern "C" __global__ void __launch_bounds__(256) {name}(u64 count, u64 limit, u
       a0 = a1 = a2 = a3 = 0u;
     }}
   }}
-  // evaluations of the wave, into one of 64 count slots 16 words apart
-  for (int off = 32; off > 0; off >>= 1) ev += __shfl_xor(ev, off);
+  // evaluations and records of the wave, into one of 64 count slots 16 words apart
+  for (int off = 32; off > 0; off >>= 1) {{
+    ev += __shfl_xor(ev, off);
+    nrec += __shfl_xor(nrec, off);
+  }}
   if (lane == 0u && ev != 0ull) atomicAdd(evals + ((blockIdx.x & 63u) << 4), ev);
+  if (lane == 0u && nrec != 0u) atomicAdd(evals + ((blockIdx.x & 63u) << 4) + 1, (u64)nrec);
 }}
-'''
+"""
 
 
 def main():
