@@ -1357,6 +1357,11 @@ NativeKernel build_native_walk(const Circuit& c, uint32_t k) {
     }
   }
   if (!nk.vgpr_class) throw NativeUnfit("walk body does not fit the native templates: " + why);
+  // every input varies per slot, so the body has no scalar work: the walk templates leave
+  // the SGPRs to their own code
+  for (size_t pos = 0; (pos = nk.body.find("\ts_", pos)) != std::string::npos; pos += 3)
+    if (nk.body.compare(pos, 7, "\ts_nop ") != 0) throw std::runtime_error("walk body with scalar instructions");
+  if (nk.body.find("lane_b32 s") != std::string::npos) throw std::runtime_error("walk body writes SGPRs");
   nk.info.valu_ops = (p.valu_ops + 1) / 2;  // per step
   nk.info.scalar_ops = p.scalar_ops;
   nk.kernel_name = "ns_nat_walk" + std::to_string(T) + "_r" + std::to_string(nk.vgpr_class);

@@ -400,7 +400,8 @@ extern "C" __global__ void __launch_bounds__(256) ns_nat_bqsat_{cls}(const NsBat
     return sat + qsat
 
 
-# Walks without a table (SATOSTOP): state tiers (the template holds T state bits per walk
+# Walks without a table (SATOSTOP): the walk bodies hold no wave-uniform value, so they write
+# no SGPR and the template keeps its own scalars anywhere across the body. State tiers (the template holds T state bits per walk
 # slot; circuits of n <= T state bits leave the bits above n at zero) and the register classes
 # each tier is built for. The 64 KiB of slot records in LDS per block already hold a CU to two
 # blocks (two waves per SIMD), so no class below r256 is built.
@@ -559,7 +560,7 @@ extern "C" __global__ void __launch_bounds__(256){" __attribute__((amdgpu_waves_
     }}
     if ((used & 255ull) == 0ull && __hip_atomic_load(long_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u)
       break;
-    asm volatile("; NS_BODY {name}" : {", ".join(ops)} : : {clobbers(R["pool_lo"], pool_hi, cls == "r512")});
+    asm volatile("; NS_BODY {name}" : {", ".join(ops)} : : {clobbers(R["pool_lo"], pool_hi, cls == "r512", [])});
     t += 2u;
     if ((t & 15u) == 0u) {{
       // evaluations of the window: 16 per slot running at its start, less the steps since
