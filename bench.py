@@ -47,6 +47,16 @@ SAT_WORKLOAD = ("SAT n=36 inputs, 20000 random AND/OR/XOR gates (seed 2), output
 TABLE_WORKLOAD = "SATOSTOP n=30 Feistel (6 rounds x 300 gates, seed 4), 14 unknowns, 2^30-entry table"
 
 
+def traffic_commit():
+    """the commit profiles/traffic.txt names for its SAT kernel line, or None"""
+    path = os.path.join(ROOT, "profiles", "traffic.txt")
+    if os.path.exists(path):
+        for line in open(path):
+            if "re-measured at commit" in line:
+                return line.split("commit", 1)[1].split()[0]
+    return None
+
+
 def measured_traffic():
     """kernel -> HBM bytes per launch, from profiles/traffic.txt (lines: kernel workload bytes)"""
     path = os.path.join(ROOT, "profiles", "traffic.txt")
@@ -115,8 +125,9 @@ def sat_part(ns, ctx, steps, warmup):
                    "vgpr_class": r["vgpr_class"], "compile_ms_first": first_compile_ms},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TOP/s",
                      "frac": achieved / peak, "traffic": traffic, "kernel": kernel,
-                     "traffic_source": "profiles/traffic.txt: rocprofv3 FETCH_SIZE x 1.99993 + WRITE_SIZE x 1.0 "
-                                       "per dispatch (copy-kernel calibration, profiles/hbm_counters.md)"
+                     "traffic_source": "copied from profiles/traffic.txt (not measured in this run): rocprofv3 "
+                                       "FETCH_SIZE x 1.99994 + WRITE_SIZE x 1.0 per dispatch (copy-kernel "
+                                       f"calibration), measured at commit {traffic_commit()}"
                      if traffic is not None else None,
                      "avg_launch_ms": avg_launch_ms, "ops_per_launch": ops_per_launch,
                      "lut_ops_per_s": lut_ops_per_s, "lut_frac": lut_ops_per_s / VALU_PEAK_OPS,
