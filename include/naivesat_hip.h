@@ -275,6 +275,15 @@ typedef struct {
 } ns_hashmap_result;
 int ns_satostop_hashmap_solve(ns_ctx* ctx, const ns_circuit* c, uint32_t unknowns, const ns_hashmap_opts* opts,
                               ns_hashmap_result* res);
+/* ns_satostop_hashmap_solve with per-block progress (the reference's per-arg callback,
+ * src/satostop.rs:1548-1558, which prints "Step: arg / arg_steps" and "Resolved unknowns
+ * entries: ru"): fn(user, step, steps, resolved_unknowns) after each block of 2^elem_bits
+ * states, in order (steps = blocks per sweep; a new sweep starts again at 0). A nonzero
+ * return stops the search: NS_ERR_CANCELLED. */
+typedef int (*ns_hashmap_progress_fn)(void* user, uint64_t step, uint64_t steps, uint64_t resolved_unknowns);
+int ns_satostop_hashmap_solve_progress(ns_ctx* ctx, const ns_circuit* c, uint32_t unknowns,
+                                       const ns_hashmap_opts* opts, ns_hashmap_progress_fn fn, void* user,
+                                       ns_hashmap_result* res);
 
 /* Diagnostic (one device step of the hash-map search on caller arrays, for checking it
  * against the reference's data sets): which = 0 join_to_hashmap (src/satostop.rs:186-266),

@@ -45,6 +45,7 @@ class _SatOpts(C.Structure):
 
 
 _SatProgressFn = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint32)
+_HmProgressFn = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64)
 
 
 class _SatResult(C.Structure):
@@ -157,6 +158,8 @@ def load_library() -> C.CDLL:
         "ns_satostop_solve": (I, [P, P, U32, C.POINTER(_StopOpts), C.POINTER(_StopResult), P, P]),
         "ns_satostop_solve_wide": (I, [P, P, U32, C.POINTER(_StopOpts), C.POINTER(_StopResult), P, P]),
         "ns_satostop_hashmap_solve": (I, [P, P, U32, C.POINTER(_HmOpts), C.POINTER(_HmResult)]),
+        "ns_satostop_hashmap_solve_progress": (I, [P, P, U32, C.POINTER(_HmOpts), _HmProgressFn, P,
+                                                   C.POINTER(_HmResult)]),
         "ns_hashmap_step": (I, [P, U32, U32, U32, U64, U32, U32, U32, U32, P, U64, P, P, U64, P, U64, P]),
         "ns_table_build": (I, [P, P, P, U64, C.POINTER(C.c_double)]),
         "ns_satostop_pass_times": (I, [P, P, U32, C.POINTER(U32)]),
@@ -203,7 +206,8 @@ def exported_symbols() -> list:
             "ns_file_image_create", "ns_file_image_save_chunk", "ns_file_image_load_partition",
             "ns_file_image_save_partition", "ns_file_image_free", "ns_serve_open", "ns_serve_submit",
             "ns_serve_poll", "ns_serve_wait", "ns_serve_close", "ns_batch_solve", "ns_sat_solve_noalloc",
-            "ns_sat_solve_progress", "ns_device_list", "ns_qsat_solve_progress"]
+            "ns_sat_solve_progress", "ns_device_list", "ns_qsat_solve_progress",
+            "ns_satostop_hashmap_solve_progress"]
 
 
 def device_list() -> list:
@@ -513,11 +517,31 @@ class Context:
 
     def satostop_hashmap_solve(self, c: Circuit, unknowns: int, hashmap_len_bits: int = 0, elem_bits: int = 0,
                                unknown_fill_bits: int = -1, max_predecessors: int = 0,
-                               iters_per_clear_predecessors: int = 0, max_sweeps: int = 0) -> dict:
+                               iters_per_clear_predecessors: int = 0, max_sweeps: int = 0, progress=None) -> dict:
+        """progress: callback(step, steps, resolved_unknowns) -> bool after each block
+        (ns_satostop_hashmap_solve_progress); True stops the search (NaivesatError,
+        NS_ERR_CANCELLED)."""
         o = _HmOpts(hashmap_len_bits, elem_bits, unknown_fill_bits, max_predecessors,
                     iters_per_clear_predecessors, max_sweeps)
         r = _HmResult()
-        _check(load_library().ns_satostop_hashmap_solve(self._h, c._h, unknowns, C.byref(o), C.byref(r)))
+        if progress is None:
+            _check(load_library().ns_satostop_hashmap_solve(self._h, c._h, unknowns, C.byref(o), C.byref(r)))
+        else:
+            err = []
+
+            def tramp(user, step, steps, resolved):
+                try:
+                    return 1 if progress(step, steps, resolved) else 0
+                except BaseException as e:  # noqa: BLE001 - re-raised after the search
+                    err.append(e)
+                    return 1
+
+            fn = _HmProgressFn(tramp)
+            st = load_library().ns_satostop_hashmap_solve_progress(self._h, c._h, unknowns, C.byref(o), fn, None,
+                                                                   C.byref(r))
+            if err:
+                raise err[0]
+            _check(st)
         return {k: getattr(r, k) for k, _ in _HmResult._fields_}
 
     def hashmap_step(self, which: int, state_len: int, arg_bit_place: int, arg: int, unknown_bits: int,

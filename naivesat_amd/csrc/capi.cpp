@@ -365,6 +365,18 @@ int ns_satostop_hashmap_solve(ns_ctx* ctx, const ns_circuit* c, uint32_t unknown
   });
 }
 
+int ns_satostop_hashmap_solve_progress(ns_ctx* ctx, const ns_circuit* c, uint32_t unknowns,
+                                       const ns_hashmap_opts* opts, ns_hashmap_progress_fn fn, void* user,
+                                       ns_hashmap_result* res) {
+  return guard([&] {
+    need(ctx && c && res && fn, "null argument");
+    ns_hashmap_opts o{};
+    o.unknown_fill_bits = -1;
+    if (opts) o = *opts;
+    ns::satostop_hashmap_solve(*ctx->ctx.main, c->c, unknowns, o, *res, fn, user);
+  });
+}
+
 int ns_hashmap_step(ns_ctx* ctx, uint32_t which, uint32_t state_len, uint32_t arg_bit_place, uint64_t arg,
                     uint32_t unknown_bits, uint32_t unknown_fill_bits, uint32_t max_predecessors, uint32_t test,
                     const uint32_t* outputs, uint64_t output_words, void* hm, void* hm_out, uint64_t hm_len,

@@ -77,7 +77,29 @@ int main(int argc, char** argv) {
     ho.max_predecessors = (uint32_t)a.num("max-predecessors", 0);
     ho.iters_per_clear_predecessors = (uint32_t)a.num("iters-per-clear-predecessors", 20);
     ns_hashmap_result hr{};
-    cli::check(ns_satostop_hashmap_solve(ctx, c, (uint32_t)U, &ho, &hr), "search");
+    // per block: the reference's "Step: arg / arg_steps" and resolved unknowns lines
+    // (src/satostop.rs:1548-1558); Ctrl-C stops the search
+    cli::Progress prog;
+    prog.print = !simple;
+    const int st = ns_satostop_hashmap_solve_progress(
+        ctx, c, (uint32_t)U, &ho,
+        [](void* u, uint64_t step, uint64_t steps, uint64_t resolved) -> int {
+          auto* p = static_cast<cli::Progress*>(u);
+          if (p->print) {
+            std::printf("Step: %llu / %llu\n", (unsigned long long)step, (unsigned long long)steps);
+            std::printf("Resolved unknowns entries: %llu\n", (unsigned long long)resolved);
+            std::fflush(stdout);
+          }
+          return p->cancelled() ? 1 : 0;
+        },
+        &prog, &hr);
+    if (st == NS_ERR_CANCELLED) {
+      std::printf("Cancelled: no result\n");
+      ns_ctx_destroy(ctx);
+      ns_circuit_free(c);
+      return 130;
+    }
+    cli::check(st, "search");
     r.n_stopped = hr.found == 1 ? 1 : 0;
     r.start = hr.start;
     r.end = hr.end;

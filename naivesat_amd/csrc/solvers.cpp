@@ -1356,7 +1356,7 @@ void hashmap_step(Lane& ctx, uint32_t which, uint32_t state_len, uint32_t arg_bi
 }
 
 void satostop_hashmap_solve(Lane& ctx, const Circuit& c, uint32_t unknowns, const ns_hashmap_opts& o,
-                            ns_hashmap_result& r) {
+                            ns_hashmap_result& r, ns_hashmap_progress_fn progress, void* user) {
   auto t0 = clk::now();
   std::memset(&r, 0, sizeof(r));
   const uint32_t n = c.n;
@@ -1445,6 +1445,13 @@ void satostop_hashmap_solve(Lane& ctx, const Circuit& c, uint32_t unknowns, cons
       r.blocks++;
       NS_HIP(hipMemcpyAsync(&host, sol, sizeof(host), hipMemcpyDeviceToHost, ctx.stream));
       NS_HIP(hipStreamSynchronize(ctx.stream));
+      // the reference's per-arg callback prints "Step: arg / arg_steps" and the resolved
+      // unknowns here (src/satostop.rs:1548-1558)
+      if (progress && progress(user, arg, args, host.resolved_unknowns) != 0) {
+        NS_HIP(hipEventRecord(ctx.ev1, ctx.stream));
+        NS_HIP(hipEventSynchronize(ctx.ev1));
+        throw Error(ST_CANCELLED, "hash-map search cancelled by its progress callback");
+      }
       if (host.defined) {
         r.found = 1;
         r.start = host.start;

@@ -33,8 +33,9 @@ double table_build(Lane& ctx, const Circuit& c, uint32_t* table_out, uint64_t ta
 void satostop_buffer_plan(uint32_t n, uint32_t unknowns, uint64_t bytes[5]);
 
 // naivesatostop's hash-map search (state spaces beyond the device table)
+// progress (optional): called after each block of the sweep, in order; nonzero cancels
 void satostop_hashmap_solve(Lane& ctx, const Circuit& c, uint32_t unknowns, const ns_hashmap_opts& o,
-                            ns_hashmap_result& r);
+                            ns_hashmap_result& r, ns_hashmap_progress_fn progress = nullptr, void* user = nullptr);
 // one device step of it on host arrays (diagnostic; see ns_hashmap_step)
 void hashmap_step(Lane& ctx, uint32_t which, uint32_t state_len, uint32_t arg_bit_place, uint64_t arg,
                   uint32_t unknown_bits, uint32_t unknown_fill_bits, uint32_t max_predecessors, bool test,

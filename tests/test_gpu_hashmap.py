@@ -127,3 +127,36 @@ def test_search_mem_image_cases(ctx):
         else:
             assert r["found"] == 1, (case, r)
             assert (r["start"], r["end"], r["steps"]) == (exp["start"], exp["end"], 1), case
+
+
+def test_search_progress_and_cancel(ctx):
+    # per-block progress (the reference's "Step: arg / arg_steps" callback): blocks in order
+    # within each sweep, the same answer as without a callback; a stop ends the search
+    c, _ = ns.Circuit.gen_satostop(8102, 10, 6, 80, 10)
+    plain = ctx.satostop_hashmap_solve(c, 10, hashmap_len_bits=18, elem_bits=14, max_sweeps=64)
+    calls = []
+    r = ctx.satostop_hashmap_solve(c, 10, hashmap_len_bits=18, elem_bits=14, max_sweeps=64,
+                                   progress=lambda s, n, ru: calls.append((s, n, ru)) and False)
+    # (which stopping start is found may depend on the order of the device's work: each
+    # answer is checked by walking the circuit)
+    assert r["found"] == plain["found"] == 1
+    assert oracle.OracleCircuit.from_dict(c.arrays()).walk(r["start"], r["steps"]) == (r["steps"], r["end"])
+    assert len(calls) == r["blocks"] and all(n == 1 << 6 for _, n, _ in calls)
+    assert [s for s, _, _ in calls] == [b % 64 for b in range(len(calls))]
+    with pytest.raises(ns.NaivesatError) as e:
+        ctx.satostop_hashmap_solve(c, 10, hashmap_len_bits=18, elem_bits=14, max_sweeps=64,
+                                   progress=lambda s, n, ru: s == 3)
+    assert e.value.status == ns.NS_ERR_CANCELLED
+    tool = os.path.join(ns.BIN_DIR, "naivesatostop")
+    import subprocess
+    import tempfile
+    with tempfile.TemporaryDirectory() as td:
+        f = os.path.join(td, "s.circuit")
+        open(f, "w").write(c.text())
+        out = subprocess.run([tool, "-t", "opencl:0", "-M", "-H", "18", "-e", "14", f, "10"], capture_output=True,
+                             text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    lines = out.stdout.splitlines()
+    steps = [x for x in lines if x.startswith("Step: ")]
+    assert steps[:3] == ["Step: 0 / 64", "Step: 1 / 64", "Step: 2 / 64"]
+    assert sum(1 for x in lines if x.startswith("Resolved unknowns entries: ")) == len(steps)
