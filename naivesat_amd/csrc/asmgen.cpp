@@ -1117,8 +1117,11 @@ NativeKernel build_native(const Circuit& c, NativeKind kind, const std::vector<u
     uint32_t budget, hi, agprs, slots;
   };
   // each register class first without, then with two VGPRs of lane slots for uniform values
-  const Cls classes[8] = {{64, 55, 0, 0},   {64, 55, 0, 2},   {128, 119, 0, 0}, {128, 119, 0, 2},
-                          {256, 239, 0, 0}, {256, 239, 0, 2}, {512, 239, 256, 0}, {512, 239, 256, 2}};
+  // (the QSAT-B templates keep two more VGPRs for their own code: gen_templates.py QSATB_POOL_CUT)
+  const uint32_t cut = kind == NativeKind::QSAT_B ? 2 : 0;
+  const Cls classes[8] = {{64, 55 - cut, 0, 0},    {64, 55 - cut, 0, 2},    {128, 119 - cut, 0, 0},
+                          {128, 119 - cut, 0, 2},  {256, 239 - cut, 0, 0},  {256, 239 - cut, 0, 2},
+                          {512, 239 - cut, 256, 0}, {512, 239 - cut, 256, 2}};
   struct Attempt {
     std::string body, why = "no register class tried";
     uint32_t vgpr_class = 0, spills = 0, reloads = 0, valu = 0, scalar = 0, vector_lines = 0;

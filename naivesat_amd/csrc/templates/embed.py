@@ -17,6 +17,18 @@ for path in sys.argv[2:]:
     ok = text.count("s_swappc_b64 s[38:39], s[36:37]") == 8 if batch else text.count("; NS_BODY " + name) == 1
     if ")NSTPL" in text or not ok:
         sys.exit(f"{path}: placeholder missing or text unusable")
+    # every kernel stays within its register class (the suffix _r64 .. _r512): the VGPRs
+    # (with AGPRs) its metadata declares decide its occupancy
+    kname = None
+    for line in text.splitlines():
+        t = line.strip()
+        if t.startswith(".name:"):
+            kname = t.split()[1]
+        elif t.startswith(".vgpr_count:") and kname:
+            cls = kname.rsplit("_r", 1)[-1]
+            if cls.isdigit() and int(t.split()[1]) > int(cls):
+                sys.exit(f"{path}: {kname} declares {t.split()[1]} VGPRs, over its class of {cls}")
+            kname = None
     out.append(f'  {{"{name}", R"NSTPL({text})NSTPL"}},')
 out += ["};", "}  // namespace", "const char* native_template(const std::string& name) {",
         "  for (const Tpl& t : kTemplates)", "    if (name == t.name) return t.text;", "  return nullptr;", "}",

@@ -24,6 +24,9 @@ CLASSES = {  # name: (total VGPR budget, pool high register); r512 adds a0..a255
     "r512": (512, 239),
 }
 SGPR_POOL = list(range(40, 102))
+# the QSAT-B templates' own code needs two VGPRs more than the others: their bodies' pool ends
+# two registers lower, so that the kernels stay within their class (64 / 128 VGPRs)
+QSATB_POOL_CUT = 2
 SAT_SGPR_POOL = list(range(14, 20)) + list(range(22, 102))  # s20, s21 carry the word index
 
 
@@ -451,6 +454,9 @@ def walk_kernel(T, cls, pool_hi):
         "        u32 nb = 0u;\n"
         "        if (total > avail) {\n"
         "          if (have_pf) {\n"
+        "            // the prefetched chunk is read here, not at its atomic: the wait for its\n"
+        "            // result stays here too\n"
+        "            asm volatile(\"\" : \"+v\"(pf));\n"
         "            nb = __builtin_amdgcn_readfirstlane(pf);\n"
         "            have_pf = false;\n"
         "          } else {\n"
@@ -626,7 +632,7 @@ def main():
     batch = []
     for ci, (cls, (budget, hi)) in enumerate(CLASSES.items()):
         kernels = [("sat", sat_kernel(cls, budget, hi)), ("qsata", qsat_a_kernel(cls, hi)),
-                   ("qsatb", qsat_b_kernel(cls, hi))]
+                   ("qsatb", qsat_b_kernel(cls, hi - QSATB_POOL_CUT))]
         if budget >= 128:
             kernels.append(("table", table_kernel(cls, hi)))
         for kind, src in kernels:
