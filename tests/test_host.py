@@ -129,6 +129,21 @@ def test_context_without_device_fails_cleanly():
     assert e.value.status in (1, 2, 4)
 
 
+def test_device_list_and_cli_device_line(tmp_path):
+    import torch  # noqa: F401  (device discovery only)
+
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    # no device: an empty list, and the tools' first line in the reference's form
+    assert ns.device_list() == []
+    f = tmp_path / "c.circuit"
+    f.write_text(ns.Circuit.from_arrays(3, [0, 3], [1, 2], [2, 0], [4], [0]).text())
+    r = subprocess.run([os.path.join(ns.BIN_DIR, "naivesat"), "-t", "opencl:0", str(f)], capture_output=True,
+                       text=True)
+    assert r.stdout.splitlines()[0] == "OpenCL devices: []"
+    assert r.returncode != 0  # no device to run on
+
+
 def test_cli_usage_and_errors(tmp_path):
     exe = os.path.join(ns.BIN_DIR, "naivesat")
     r = subprocess.run([exe, "--help"], capture_output=True, text=True)
