@@ -573,13 +573,31 @@ extern "C" __global__ void __launch_bounds__(256){" __attribute__((amdgpu_waves_
       // each ended slot finished (its age)
       ev += 16ull * (u64)__popc(active | fin) -
             (u64)(__popc(a0) + 2u * __popc(a1) + 4u * __popc(a2) + 8u * __popc(a3));
+      // saves and limits of the running slots: with most slots running, all 32 start times
+      // are read at once (independent LDS reads, one wait) and the masks built without
+      // branches; with few (the end of the search), slot by slot
       u32 save = 0u, lng = 0u;
-      for (u32 rem = active; rem; rem &= rem - 1u) {{
-        const u32 b = __builtin_ctz(rem);
-        const u32 start = (u32)meta[b * 256u + tid];
-        const u32 steps = t - start, before = steps > 16u ? steps - 16u : 0u;
-        if ((u64)steps >= lim) lng |= 1u << b;
-        else if (before == 0u || __builtin_clz(steps) < __builtin_clz(before)) save |= 1u << b;
+      u32 nact;
+      (void)ns_wave_excl(__popc(active), &nact);
+      if (nact >= 1024u) {{
+        const u32* mlo = reinterpret_cast<const u32*>(meta);
+#pragma unroll
+        for (u32 b = 0; b < 32u; b++) {{
+          const u32 steps = t - mlo[2u * (b * 256u + tid)], before = steps > 16u ? steps - 16u : 0u;
+          const u32 act = (active >> b) & 1u;
+          const u32 over = (u64)steps >= lim ? 1u : 0u;
+          const u32 pow2 = (before == 0u || __builtin_clz(steps) < __builtin_clz(before)) ? 1u : 0u;
+          lng |= (act & over) << b;
+          save |= (act & (over ^ 1u) & pow2) << b;
+        }}
+      }} else {{
+        for (u32 rem = active; rem; rem &= rem - 1u) {{
+          const u32 b = __builtin_ctz(rem);
+          const u32 start = (u32)meta[b * 256u + tid];
+          const u32 steps = t - start, before = steps > 16u ? steps - 16u : 0u;
+          if ((u64)steps >= lim) lng |= 1u << b;
+          else if (before == 0u || __builtin_clz(steps) < __builtin_clz(before)) save |= 1u << b;
+        }}
       }}
       if (save) {{
 {each("        v@ = (v@ & ~save) | (s@ & save);" + chr(10))}      }}
