@@ -429,7 +429,7 @@ int ns_native_encode(const char* body, size_t len, uint32_t* words, size_t cap, 
  * byte *slot_offset (*patched = 1), or the fully assembled module (*patched = 0) when the
  * body does not fit the largest slot. Kinds 4 and 5: the batch code object (the batch
  * kernels of every register class) with this one body in its region, *slot_offset = the
- * body's first byte. */
+ * body's first byte. Kind 6: the walk template's code object with the walk body. */
 int ns_native_code(const ns_circuit* c, uint32_t kind, uint32_t lane_shift, void* code, size_t cap,
                    size_t* needed, uint32_t* patched, uint64_t* slot_offset);
 

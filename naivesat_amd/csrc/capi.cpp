@@ -594,12 +594,12 @@ int ns_native_encode(const char* body, size_t len, uint32_t* words, size_t cap, 
 int ns_native_code(const ns_circuit* c, uint32_t kind, uint32_t lane_shift, void* code, size_t cap,
                    size_t* needed, uint32_t* patched, uint64_t* slot_offset) {
   return guard([&] {
-    need(c && kind <= 5, "bad argument");
+    need(c && kind <= 6, "bad argument");
     ns::NativeKernel nk = native_kernel(c->c, kind, lane_shift);
     size_t off = 0;
     std::vector<char> obj;
     bool p = true;
-    if (kind >= 4) {
+    if (kind == 4 || kind == 5) {
       std::vector<std::vector<uint32_t>> bodies(1);
       std::string err;
       if (!ns::encode_body(nk.body, bodies[0], &err)) throw ns::Error(NS_ERR_UNSUPPORTED, err);

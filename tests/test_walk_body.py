@@ -159,3 +159,16 @@ def test_walk_body_fits_each_tier(n):
         dst = ln.split(None, 1)[1].split(",")[0].strip()
         if dst.startswith("v"):
             assert int(dst[1:]) >= 8, ln
+
+
+def test_walk_code_object_holds_the_body():
+    # the code object the engine loads: the walk template's with the encoded body in its slot
+    import struct
+    c, _ = ns.Circuit.gen_satostop(21, 12, 6, 120, 8)
+    body, cls, size = c.native_body(6, 5)
+    code, patched, off = c.native_code(6, 5)
+    code = bytes(code)
+    assert patched and code[:4] == b"\x7fELF"
+    enc = ns.native_encode(body)
+    got = struct.unpack_from(f"<{len(enc)}I", code, off)
+    assert list(got) == list(enc)
