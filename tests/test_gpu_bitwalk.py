@@ -153,6 +153,33 @@ def test_36_bit_state_space(ctx):
     assert short >= 1
 
 
+def test_42_bit_state_space_widest_tier(ctx):
+    # n = 42: the widest walk template (48 state bits per slot), 2^28 waypoints 2^14 states
+    # apart, 2^42 evaluations; checked like the 36-bit search
+    n, U = 42, 14
+    c, target = ns.Circuit.gen_satostop(10511, 21, 6, 60, U)
+    try:
+        r = ctx.satostop_solve(c, U)
+    finally:
+        ctx.trim()
+    assert (r["method"], r["waypoint_bits"], r["backend"]) == (2, 14, 1)
+    steps, ends = r["steps_arr"], r["ends_arr"]
+    stopped = steps > 0
+    assert r["n_stopped"] == int(stopped.sum()) > 0
+    assert np.all(ends[stopped] == target)
+    assert len(set(steps[stopped].tolist())) == int(stopped.sum())
+    oc = oracle.OracleCircuit.from_dict(c.arrays())
+    short = 0
+    for u in range(1 << U):
+        s, e = oc.walk(u << (n - U), 64)
+        if s:
+            short += 1
+            assert (int(steps[u]), int(ends[u])) == (s, e), u
+        else:
+            assert steps[u] == 0 or steps[u] > 64, u
+    assert short >= 1
+
+
 def test_native_and_compiled_walks_agree(ctx, monkeypatch):
     # the native walk template and the compiled ns_walk kernel: same records, so the same
     # per-start answers and the same evaluation count
