@@ -9,6 +9,8 @@
 #include "asmgen.h"
 
 #include <algorithm>
+#include <cctype>
+#include <cstdio>
 #include <cstdlib>
 #include <exception>
 #include <sstream>
@@ -847,7 +849,8 @@ std::vector<std::vector<size_t>> refine_groups(std::vector<std::vector<size_t>> 
 // apart and only the accumulator (and kept values) live from one group into the next.
 // Returns false when the output has fewer than K literals.
 bool grouped_circuit(const Circuit& c, uint32_t out_k, uint32_t K, Circuit& out, std::vector<uint32_t>& wgroup,
-                     uint32_t refine_cap_pct, uint32_t pin = 0) {
+                     uint32_t refine_cap_pct, uint32_t pin = 0, const std::vector<uint8_t>* uni_in = nullptr,
+                     uint32_t pin_u = 0) {
   const Output o = c.outputs.at(out_k);
   const auto lits = and_literals(c, o.wire);
   if (K < 2 || lits.size() < K) return false;
@@ -974,6 +977,40 @@ bool grouped_circuit(const Circuit& c, uint32_t out_k, uint32_t K, Circuit& out,
     std::sort(cand.begin(), cand.end(), [](const auto& a, const auto& b) { return a.first > b.first || (a.first == b.first && a.second < b.second); });
     for (size_t q = 0; q < cand.size() && q < pin; q++) pinned[cand[q].second] = 1;
   }
+  if (pin_u && uni_in) {
+    // Kept uniform values: wave-uniform gates (over wave-uniform inputs only) that several
+    // groups share are computed once as well. They live in SGPRs (or lane slots), not in
+    // the VGPRs that bound the groups, and each saves its scalar cone in every later group.
+    std::vector<uint8_t> ug(G, 0);
+    auto is_u = [&](uint32_t w) { return w < c.n ? (*uni_in)[w] != 0 : ug[w - c.n] != 0; };
+    for (uint32_t g = 0; g < G; g++) ug[g] = is_u(c.gates[g].i0) && is_u(c.gates[g].i1);
+    std::vector<uint32_t> use(G, 0);
+    for (size_t gi = 0; gi < groups.size(); gi++) {
+      std::vector<uint8_t> u(G, 0);
+      for (size_t i : groups[gi])
+        for (uint32_t g = 0; g < G; g++) u[g] |= cone[i][g];
+      for (uint32_t g = 0; g < G; g++) use[g] += u[g];
+    }
+    std::vector<std::pair<uint64_t, uint32_t>> cand;
+    std::vector<uint32_t> mark(G, UINT32_MAX);
+    for (uint32_t g = 0; g < G; g++) {
+      if (!ug[g] || use[g] < 2 || pinned[g]) continue;
+      uint64_t sz = 0;
+      std::vector<uint32_t> st{c.n + g};
+      while (!st.empty()) {
+        const uint32_t w = st.back();
+        st.pop_back();
+        if (w < c.n || mark[w - c.n] == g) continue;
+        mark[w - c.n] = g;
+        sz++;
+        st.push_back(c.gates[w - c.n].i0);
+        st.push_back(c.gates[w - c.n].i1);
+      }
+      if (sz >= 2) cand.push_back({(uint64_t)(use[g] - 1) * sz, g});
+    }
+    std::sort(cand.begin(), cand.end(), [](const auto& a, const auto& b) { return a.first > b.first || (a.first == b.first && a.second < b.second); });
+    for (size_t q = 0; q < cand.size() && q < pin_u; q++) pinned[cand[q].second] = 1;
+  }
   std::pair<uint32_t, bool> acc{0, false};
   bool have_acc = false;
   std::vector<uint32_t> remap(c.wires());
@@ -1008,6 +1045,12 @@ bool grouped_circuit(const Circuit& c, uint32_t out_k, uint32_t K, Circuit& out,
   out.outputs.push_back({acc.first, acc.second != o.neg});
   return true;
 }
+
+// wave-uniform gates that several output groups share and that are computed once (per word)
+// in the groups tried from 2^36 assignments: on the 36-input, 20,000-gate search 16 of them
+// take 2,296 -> 2,253 issue slots per word, 4.96 -> 4.83 ms per launch (more of them no
+// longer fit the SGPRs and go to lane slots, whose v_readlane costs more than recomputing)
+constexpr uint32_t kKeptUniform = 16;
 
 const char* kind_name(NativeKind k) {
   switch (k) {
@@ -1124,7 +1167,11 @@ NativeKernel build_native(const Circuit& c, NativeKind kind, const std::vector<u
                           {512, 239 - cut, 256, 0}, {512, 239 - cut, 256, 2}};
   struct Attempt {
     std::string body, why = "no register class tried";
-    uint32_t vgpr_class = 0, spills = 0, reloads = 0, valu = 0, scalar = 0, vector_lines = 0;
+    uint32_t vgpr_class = 0, spills = 0, reloads = 0, valu = 0, scalar = 0, vector_lines = 0, scalar_lines = 0,
+             lane_reads = 0;
+    // issue slots per word: a wave issues one instruction at a time, scalar or vector, and a
+    // v_readlane holds its wave for about two
+    uint32_t slots() const { return vector_lines + scalar_lines + lane_reads; }
   };
   // LUT mapping, schedule and the smallest register class that holds the program of cc
   auto attempt = [&](const Circuit& cc, const std::vector<uint32_t>& outs_cc, const std::vector<uint32_t>* wgroup) {
@@ -1185,16 +1232,22 @@ NativeKernel build_native(const Circuit& c, NativeKind kind, const std::vector<u
     }
     // vector instructions of the body (LUTs, copies, AGPR and lane moves)
     for (size_t pos = 0; (pos = at.body.find("\tv_", pos)) != std::string::npos; pos += 3) at.vector_lines++;
+    for (size_t pos = 0; (pos = at.body.find("\ts_", pos)) != std::string::npos; pos += 3)
+      if (at.body.compare(pos, 7, "\ts_nop ") != 0) at.scalar_lines++;
+    for (size_t pos = 0; (pos = at.body.find("\tv_readlane", pos)) != std::string::npos; pos += 3) at.lane_reads++;
     return at;
   };
   Attempt best = attempt(c, outs, nullptr);
   uint32_t groups = 1;
   // One wave per SIMD (the AGPR class) issues fewer VALU ops per cycle than two or more. A
   // single output that is a conjunction can be evaluated as K groups of its literals, one
-  // after the other, recomputing gates the groups share. Measured on the 36-input,
-  // 20,000-gate search, two waves per SIMD ran 1.45 times the vector instructions per cycle
-  // of one (not twice: lane moves and scalar work grow with the groups), so the groups are
-  // taken when they reach a two-wave class at under 1.45 times the vector instructions.
+  // after the other, recomputing gates the groups share. Programs are compared by issue
+  // slots (vector plus scalar instructions, a v_readlane counted twice): a wave issues one
+  // instruction at a time, and at two waves per SIMD a scalar op costs about as much as a
+  // LUT (the issue micro-benchmark). Measured on the 36-input, 20,000-gate search, two waves
+  // per SIMD ran 1.45 times the instructions per cycle of one (not twice: lane moves and
+  // scalar work grow with the groups), so the groups are taken when they reach a two-wave
+  // class at under 1.45 times the slots.
   // Each try costs about one more code generation (tens of ms for large bodies), so they are
   // tried for spaces of 2^34 assignments and more only, where a search takes seconds.
   // NS_OUTPUT_GROUPS=K forces K (1: never).
@@ -1213,8 +1266,8 @@ NativeKernel build_native(const Circuit& c, NativeKind kind, const std::vector<u
     auto better = [&](const Attempt& at) {
       if (!at.vgpr_class) return false;
       if (!best.vgpr_class) return true;
-      return at.vgpr_class <= 256 && (best.vgpr_class == 512 ? at.vector_lines * 100 < best.vector_lines * 145
-                                                            : at.vector_lines < best.vector_lines);
+      return at.vgpr_class <= 256 && (best.vgpr_class == 512 ? at.slots() * 100 < best.slots() * 145
+                                                            : at.slots() < best.slots());
     };
     for (uint32_t K : ks) {
       if (forced > 1 && (uint32_t)forced != K) continue;
@@ -1238,9 +1291,8 @@ NativeKernel build_native(const Circuit& c, NativeKind kind, const std::vector<u
         if (fits) break;
       }
       if (fits && c.n >= 36 && forced <= 1) {
-        // looser caps, each with and without 16 kept values (pin), on threads of their own,
-        // compared in list order; on the 36-input search the best takes 1,601 LUTs per word
-        // against 1,629 without kept values
+        // looser caps, each with and without 16 kept lane values (pin), all with 16 kept
+        // uniform values, on threads of their own, compared in list order
         const uint32_t caps[6] = {102, 103, 104, 106, 108, 110};
         std::vector<std::pair<uint32_t, uint32_t>> tries{{100, 16}};
         for (uint32_t cp : caps) {
@@ -1255,7 +1307,7 @@ NativeKernel build_native(const Circuit& c, NativeKind kind, const std::vector<u
             try {
               Circuit g;
               std::vector<uint32_t> wg;
-              grouped_circuit(c, outs[0], K, g, wg, tries[q].first, tries[q].second);
+              grouped_circuit(c, outs[0], K, g, wg, tries[q].first, tries[q].second, &uni, kKeptUniform);
               ats[q] = attempt(g, {0}, &wg);
             } catch (...) {
               err[q] = std::current_exception();

@@ -183,6 +183,72 @@ __global__ void __launch_bounds__(256) k_lds_long(u32* out, int iters, u64* cyc)
   if (threadIdx.x == 0) cyc[blockIdx.x] = t1 - t0;
 }
 
+// 256 VALU-heavy blocks with one LDS spill and one reload per 9 VALU ops, each reload read
+// 9 ops after it was issued (s_waitcnt lgkmcnt(2): the block's own two LDS ops may still be
+// in flight), the pattern of a body that keeps spilled values in LDS
+__global__ void __launch_bounds__(256) k_lds_pipe(u32* out, int iters, u64* cyc) {
+  __shared__ u32 slots[4 * 256];
+  u32 a0 = threadIdx.x, a1 = a0 + 1, a2 = a0 + 2, a3 = a0 + 3, b = a0 * 3, c = a0 * 7;
+  u32 t0r = 0, t1r = 0;
+  const u32 addr = (u32)(size_t)&slots[threadIdx.x];
+  for (int k = 0; k < 4; k++) slots[threadIdx.x + 256 * k] = 0u;  // keeps the allocation
+  __syncthreads();
+#define ROT4 "v_bitop3_b32 %1, %1, %4, %5 bitop3:0x96\nv_bitop3_b32 %2, %2, %4, %5 bitop3:0x96\n" \
+             "v_bitop3_b32 %3, %3, %4, %5 bitop3:0x96\nv_bitop3_b32 %0, %0, %4, %5 bitop3:0x96\n"
+  u64 t0 = clock64();
+  for (int i = 0; i < iters; i++) {
+    asm volatile(R8(R8("ds_write_b32 %8, %0\nds_read_b32 %6, %8 offset:1024\n" ROT4 ROT4
+                       "s_waitcnt lgkmcnt(2)\nv_bitop3_b32 %1, %1, %7, %5 bitop3:0x96\n"
+                       "ds_write_b32 %8, %1 offset:2048\nds_read_b32 %7, %8 offset:3072\n" ROT4 ROT4
+                       "s_waitcnt lgkmcnt(2)\nv_bitop3_b32 %0, %0, %6, %5 bitop3:0x96\n"))
+                 : "+v"(a0), "+v"(a1), "+v"(a2), "+v"(a3), "+v"(b), "+v"(c), "+v"(t0r), "+v"(t1r)
+                 : "v"(addr) : "memory");
+  }
+#undef ROT4
+  u64 t1 = clock64();
+  __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+  __syncthreads();
+  out[blockIdx.x * 256 + threadIdx.x] = a0 ^ a1 ^ a2 ^ a3 ^ t0r ^ t1r ^ slots[(threadIdx.x + 1) & 255];
+  if (threadIdx.x == 0) cyc[blockIdx.x] = t1 - t0;
+}
+
+// Instruction-stream size: 8 bank-disjoint chains of v_bitop3_b32 (v40, v43, v44, v47, v48,
+// v51, v52, v55 reading v41 and v42) unrolled to 2,048 ops (16 KiB of code per iteration)
+// or 256 ops (2 KiB), alone or with one scalar op after every 4 (s_xor_b32: 4 bytes;
+// s_bfe_i32 with a 32-bit literal: 8 bytes)
+#define B8 "v_bitop3_b32 v40, v40, v41, v42 bitop3:0x96\nv_bitop3_b32 v43, v43, v41, v42 bitop3:0x96\n" \
+           "v_bitop3_b32 v44, v44, v41, v42 bitop3:0x96\nv_bitop3_b32 v47, v47, v41, v42 bitop3:0x96\n" \
+           "v_bitop3_b32 v48, v48, v41, v42 bitop3:0x96\nv_bitop3_b32 v51, v51, v41, v42 bitop3:0x96\n" \
+           "v_bitop3_b32 v52, v52, v41, v42 bitop3:0x96\nv_bitop3_b32 v55, v55, v41, v42 bitop3:0x96\n"
+#define SX "s_xor_b32 s60, s60, s61\n"
+#define SB "s_bfe_i32 s60, s61, 0x10016\n"
+#define BIGK(name, unit, reps)                                                                          \
+  __global__ void __launch_bounds__(256) name(u32* out, int iters, u64* cyc) {                          \
+    u32 r;                                                                                              \
+    asm volatile("v_mov_b32 v40, %1\nv_mov_b32 v41, %1\nv_mov_b32 v42, %1\nv_mov_b32 v43, %1\n"        \
+                 "v_mov_b32 v44, %1\nv_mov_b32 v47, %1\nv_mov_b32 v48, %1\nv_mov_b32 v51, %1\n"         \
+                 "v_mov_b32 v52, %1\nv_mov_b32 v55, %1\ns_mov_b32 s60, 0\ns_mov_b32 s61, 0\n"           \
+                 "v_mov_b32 %0, v40" : "=v"(r) : "v"(threadIdx.x)                                      \
+                 : "v40", "v41", "v42", "v43", "v44", "v47", "v48", "v51", "v52", "v55", "s60", "s61");    \
+    u64 t0 = clock64();                                                                                 \
+    for (int i = 0; i < iters; i++) {                                                                   \
+      asm volatile(reps(unit) : : : "v40", "v41", "v42", "v43", "v44", "v47", "v48", "v51", "v52", "v55", \
+                   "s60", "s61", "scc");                                                                \
+    }                                                                                                   \
+    u64 t1 = clock64();                                                                                 \
+    asm volatile("v_xor_b32 %0, %0, v40\nv_xor_b32 %0, %0, v55" : "+v"(r) : : "v40", "v55");             \
+    out[blockIdx.x * 256 + threadIdx.x] = r;                                                            \
+    if (threadIdx.x == 0) cyc[blockIdx.x] = t1 - t0;                                                   \
+  }
+#define R4(x) x x x x
+#define R256(x) R4(R8(R8(x)))
+#define R32(x) R4(R8(x))
+BIGK(k_big_v, B8, R256)
+BIGK(k_big_xor, B8 SX SX, R256)
+BIGK(k_big_bfe, B8 SB SB, R256)
+BIGK(k_small_v, B8, R32)
+BIGK(k_small_bfe, B8 SB SB, R32)
+
 // 64 dependent 4-byte VOP2 ops (v_xor_b32) per iteration
 __global__ void __launch_bounds__(256) k_vop2(u32* out, int iters, u64* cyc) {
   u32 a = threadIdx.x, b = a * 3;
@@ -215,6 +281,10 @@ int main() {
       {"acc256+128", k_acc, 256, 128}, {"rl256+64+nop", k_rl, 256, 64}, {"rlearly256+64", k_rl_early, 256, 64},
       {"wl256+64", k_wl, 256, 64},     {"saludep256+64", k_salu_dep, 256, 64},
       {"ldsshort256+128", k_lds_short, 256, 128}, {"ldslong256+512", k_lds_long, 256 + 256, 512},
+      {"ldspipe1152+256", k_lds_pipe, 64 * 18, 256},
+      {"big2048", k_big_v, 2048, 0}, {"big2048+xor512", k_big_xor, 2048, 512},
+      {"big2048+bfe512", k_big_bfe, 2048, 512}, {"small256", k_small_v, 256, 0},
+      {"small256+bfe64", k_small_bfe, 256, 64},
       {"vop2dep64", k_vop2, 64, 0},    {"vop2rot256", k_vop2_rot, 256, 0}};
   const int maxb = 256 * 8;
   u32* out;
