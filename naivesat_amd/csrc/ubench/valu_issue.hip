@@ -1,6 +1,7 @@
 // VALU issue micro-benchmark for gfx950: time per instruction of the patterns the native
 // circuit bodies use (v_bitop3_b32 chains, SGPR operands, AGPR copies, v_readlane /
-// v_writelane), at 1, 2 and 4 waves per SIMD (256, 512, 1024 blocks of 4 waves on 256 CUs).
+// v_writelane, scalar ops, LDS spills, operand banks), at 1, 2 and 4 waves per SIMD (256,
+// 512, 1024 blocks of 4 waves on 256 CUs).
 // Prints per-wave clock64 cycles per iteration and the wall-clock lane-op rate.
 #include <hip/hip_runtime.h>
 #include <cstdio>
@@ -249,6 +250,50 @@ BIGK(k_big_bfe, B8 SB SB, R256)
 BIGK(k_small_v, B8, R32)
 BIGK(k_small_bfe, B8 SB SB, R32)
 
+// Operand patterns of real bodies, 2,048 ops per iteration with no dependences between them:
+// destinations rotate over v40..v47, sources are read-only registers v48..v63 (banks 0, 1, 2
+// for the three operands, register index mod 4 being the bank); variants: every op with an
+// SGPR operand, one op in four with one, one op in four a v_bfe_i32 (inline-constant field),
+// every op with two sources of one bank
+#define P3(d, a, b, c) "v_bitop3_b32 v" #d ", v" #a ", v" #b ", v" #c " bitop3:0x96\n"
+#define PS(d, a, c) "v_bitop3_b32 v" #d ", v" #a ", s60, v" #c " bitop3:0x96\n"
+#define PB(d, a) "v_bfe_i32 v" #d ", v" #a ", 5, 1\n"
+#define D8 P3(40, 48, 49, 50) P3(41, 52, 53, 54) P3(42, 56, 57, 58) P3(43, 60, 61, 62) \
+           P3(44, 48, 53, 58) P3(45, 52, 57, 62) P3(46, 56, 61, 50) P3(47, 60, 49, 54)
+#define S8 PS(40, 48, 50) PS(41, 52, 54) PS(42, 56, 58) PS(43, 60, 62) \
+           PS(44, 48, 58) PS(45, 52, 62) PS(46, 56, 50) PS(47, 60, 54)
+#define S48 P3(40, 48, 49, 50) P3(41, 52, 53, 54) P3(42, 56, 57, 58) PS(43, 60, 62) \
+            P3(44, 48, 53, 58) P3(45, 52, 57, 62) P3(46, 56, 61, 50) PS(47, 60, 54)
+#define B48 P3(40, 48, 49, 50) P3(41, 52, 53, 54) P3(42, 56, 57, 58) PB(43, 60) \
+            P3(44, 48, 53, 58) P3(45, 52, 57, 62) P3(46, 56, 61, 50) PB(47, 60)
+#define C8 P3(40, 48, 52, 50) P3(41, 52, 56, 54) P3(42, 56, 60, 58) P3(43, 60, 48, 62) \
+           P3(44, 48, 56, 58) P3(45, 52, 60, 62) P3(46, 56, 48, 50) P3(47, 60, 52, 54)
+#define OPK(name, unit)                                                                              \
+  __global__ void __launch_bounds__(256) name(u32* out, int iters, u64* cyc) {                       \
+    u32 r;                                                                                           \
+    asm volatile(R8(R8("v_mov_b32 v40, %1\n")) "s_mov_b32 s60, 0x5a5a5a5a\nv_mov_b32 %0, v40"       \
+                 : "=v"(r) : "v"(threadIdx.x) : "v40", "s60");                                       \
+    asm volatile("v_mov_b32 v48, %0\nv_mov_b32 v49, %0\nv_mov_b32 v50, %0\nv_mov_b32 v52, %0\n"      \
+                 "v_mov_b32 v53, %0\nv_mov_b32 v54, %0\nv_mov_b32 v56, %0\nv_mov_b32 v57, %0\n"       \
+                 "v_mov_b32 v58, %0\nv_mov_b32 v60, %0\nv_mov_b32 v61, %0\nv_mov_b32 v62, %0\n"       \
+                 : : "v"(r) : "v48", "v49", "v50", "v52", "v53", "v54", "v56", "v57", "v58", "v60",    \
+                   "v61", "v62");                                                                    \
+    u64 t0 = clock64();                                                                              \
+    for (int i = 0; i < iters; i++) {                                                                \
+      asm volatile(R256(unit) : : : "v40", "v41", "v42", "v43", "v44", "v45", "v46", "v47", "v48",    \
+                   "v49", "v50", "v52", "v53", "v54", "v56", "v57", "v58", "v60", "v61", "v62", "s60"); \
+    }                                                                                                \
+    u64 t1 = clock64();                                                                              \
+    asm volatile("v_xor_b32 %0, %0, v47" : "+v"(r) : : "v47");                                        \
+    out[blockIdx.x * 256 + threadIdx.x] = r;                                                         \
+    if (threadIdx.x == 0) cyc[blockIdx.x] = t1 - t0;                                                \
+  }
+OPK(k_op_vgpr, D8)
+OPK(k_op_sgpr, S8)
+OPK(k_op_sgpr4, S48)
+OPK(k_op_bfe4, B48)
+OPK(k_op_bank, C8)
+
 // 64 dependent 4-byte VOP2 ops (v_xor_b32) per iteration
 __global__ void __launch_bounds__(256) k_vop2(u32* out, int iters, u64* cyc) {
   u32 a = threadIdx.x, b = a * 3;
@@ -285,6 +330,9 @@ int main() {
       {"big2048", k_big_v, 2048, 0}, {"big2048+xor512", k_big_xor, 2048, 512},
       {"big2048+bfe512", k_big_bfe, 2048, 512}, {"small256", k_small_v, 256, 0},
       {"small256+bfe64", k_small_bfe, 256, 64},
+      {"ops-vgpr2048", k_op_vgpr, 2048, 0}, {"ops-sgpr2048", k_op_sgpr, 2048, 0},
+      {"ops-sgpr1in4", k_op_sgpr4, 2048, 0}, {"ops-bfe1in4", k_op_bfe4, 2048, 0},
+      {"ops-bank2048", k_op_bank, 2048, 0},
       {"vop2dep64", k_vop2, 64, 0},    {"vop2rot256", k_vop2_rot, 256, 0}};
   const int maxb = 256 * 8;
   u32* out;
