@@ -195,11 +195,13 @@ def test_config1_body_in_output_groups():
     # its output is the conjunction of 40 comparisons: evaluated as five groups of its
     # literals (shared gates recomputed), every group fits 256 VGPRs, two waves per SIMD; the
     # refined groups take fewer LUTs than the 2,165 of the greedy ones
-    c, _ = ns.Circuit.gen_sat(2, 36, 20000, 40)
+    c, key = ns.Circuit.gen_sat(2, 36, 20000, 40)
     body, cls, size = c.native_body(0)
     assert cls == 256 and "v_accvgpr" not in body
     assert body.count("v_bitop3_b32") < 1700
-    check(c, 0, 0, [64 * 12345, (1 << 31) - 64])
+    # also the waves holding the planted key and the lowest solution (798,579,149, the
+    # full-size golden): words with satisfying bits, not only all-zero outputs
+    check(c, 0, 0, [64 * 12345, (1 << 31) - 64, (key >> 5) & ~63, (798579149 >> 5) & ~63])
 
 
 def test_groups_when_one_group_fits_no_class():
