@@ -355,6 +355,8 @@ std::vector<char> compose_batch(const BatchTemplate& t, const std::vector<std::v
   return elf;
 }
 
+size_t max_batch_region_bytes() { return kRegionBytes[sizeof(kRegionBytes) / sizeof(kRegionBytes[0]) - 1]; }
+
 size_t batch_bytes(const std::vector<std::vector<uint32_t>>& bodies) {
   size_t at = 0;
   for (const auto& b : bodies) at = (at + b.size() * 4 + 4 + 255) & ~(size_t)255;

@@ -40,6 +40,8 @@ std::shared_ptr<const BatchTemplate> batch_template(size_t min_bytes);
 int batch_class_index(uint32_t vgpr_class);
 // Region bytes the bodies take (each body + its return, on 256-byte boundaries).
 size_t batch_bytes(const std::vector<std::vector<uint32_t>>& bodies);
+// The largest batch region any template holds.
+size_t max_batch_region_bytes();
 // The template's code object with the bodies (encoded words) in its region, each followed
 // by the return; body_offset[i] = byte offset of body i in the region.
 std::vector<char> compose_batch(const BatchTemplate& t, const std::vector<std::vector<uint32_t>>& bodies,

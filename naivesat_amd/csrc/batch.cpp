@@ -399,7 +399,7 @@ void Batcher::run(Lane& lane, const std::vector<const ServeRequest*>& reqs, std:
         it.fallback = true;
       } else {
         it.vgpr = nk.vgpr_class;
-        it.cost = nk.info.valu_ops + nk.info.scalar_ops / 2 + 16;
+        it.cost = nk.info.valu_ops + nk.info.scalar_ops + 16;  // issue slots per word
       }
     } catch (const NativeUnfit&) {
       it.fallback = true;
@@ -428,6 +428,23 @@ void Batcher::run(Lane& lane, const std::vector<const ServeRequest*>& reqs, std:
   }
   for (const BatchItem& it : all) p.gen_ms += it.gen_ms;
   p.gen_wall_ms = ms_since(t0);
+  // bodies past the largest batch region are solved alone, like the requests the batch
+  // kernels do not take (NS_BATCH_REGION_BYTES lowers that size, for tests)
+  {
+    size_t at = 0;
+    size_t cap = max_batch_region_bytes();
+    if (const char* e = std::getenv("NS_BATCH_REGION_BYTES")) cap = std::min<size_t>(cap, std::strtoull(e, nullptr, 0));
+    for (BatchItem& it : all) {
+      if (it.fallback) continue;
+      const size_t next = (at + it.code.size() * 4 + 4 + 255) & ~(size_t)255;
+      if (next > cap) {
+        it.fallback = true;
+        std::vector<uint32_t>().swap(it.code);
+      } else {
+        at = next;
+      }
+    }
+  }
   // descriptor order: by kind (SAT, QSAT), then by register class: one group each
   for (int kind = 0; kind < 2; kind++)
     for (int ci = 0; ci < 4; ci++) {

@@ -315,7 +315,16 @@ void Server::work(uint32_t wid) {
       }
       const double start = now_s();
       for (auto& x : res) x.start_s = start;
-      batcher_->run(lane, reqs, res);
+      try {
+        batcher_->run(lane, reqs, res);
+      } catch (...) {
+        // a batch that fails as a whole (a device or code-object error): every request
+        // alone, each with its own status
+        for (size_t i = 0; i < got.size(); i++) {
+          res[i] = ns_request_result{};
+          run_one(lane, wid, got[i], res[i]);
+        }
+      }
       const double fin = now_s();
       for (size_t i = 0; i < got.size(); i++) {
         res[i].finish_s = fin;

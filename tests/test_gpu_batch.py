@@ -111,6 +111,28 @@ def test_batch_satostop_and_unbatchable_run_alone(ctx):
         assert (r["v0"], r["v1"], r["v2"], r["v3"]) == alone(ctx, rq), i
 
 
+def test_bodies_past_the_region_run_alone(ctx, monkeypatch):
+    """Bodies that do not fit the batch region (its size lowered here to 4 KiB) are solved
+    alone instead of failing the batch; the others still run in the batch kernels."""
+    monkeypatch.setenv("NS_BATCH_REGION_BYTES", "4096")
+    reqs = random_requests(97, 10, 14, 17)
+    res, info = ctx.batch_solve(reqs)
+    assert 0 < info["fallbacks"] < len(reqs)
+    for i, (rq, r) in enumerate(zip(reqs, res)):
+        assert r["status"] == 0, (i, r)
+        assert (r["v0"], r["v1"], r["v2"], r["v3"]) == alone(ctx, rq), i
+
+
+def test_served_batches_past_the_region(ctx, monkeypatch):
+    """The same through the server's batches (one lane)."""
+    monkeypatch.setenv("NS_BATCH_REGION_BYTES", "4096")
+    reqs = random_requests(98, 12, 14, 17)
+    res, total = ctx.serve(reqs, workers=1, realtime=False, batch_max=16)
+    for i, (rq, r) in enumerate(zip(reqs, res)):
+        assert r["status"] == 0, (i, r)
+        assert (r["v0"], r["v1"], r["v2"], r["v3"]) == alone(ctx, rq), i
+
+
 @pytest.mark.parametrize("workers", [1, 4])
 def test_served_batches_equal_alone(ctx, workers):
     reqs = trace.make_trace(seed=5, n_requests=128)
