@@ -14,7 +14,9 @@ Also reported on the same line:
                 FETCH_SIZE / WRITE_SIZE (profiles/traffic.txt), when recorded;
   cpu_baseline  the CPU oracle (C, AVX2, pthreads over the host cores) on a bounded slice
                 of the same search;
-  extra         the SATOSTOP search over the 30-bit Feistel's 2^30-entry table (waypoint walks,
+  extra         the same search with group exits (NS_SAT_GROUP_EXITS: waves leave a word once
+                the output's groups so far are zero; time to the same answer), the SATOSTOP
+                search over the 30-bit Feistel's 2^30-entry table (waypoint walks,
                 their scattered-read rate against a probe of the same table size, and the
                 passes over the whole table with their HBM rate, and the walks that evaluate
                 the circuit instead of reading a table), the same without a table at 36 state
@@ -135,6 +137,25 @@ def sat_part(ns, ctx, steps, warmup):
                      "valu_issue_source": "profiles/sq_insts.txt: rocprofv3 SQ_INSTS_VALU per dispatch x 64 lanes"
                      if issued is not None else None},
     }
+
+
+def group_exits_part(ns, ctx, c, full_ms_per_step, full_answer, searches=3):
+    """The same 2^36 search with NS_SAT_GROUP_EXITS: a wave leaves a 32-assignment word once
+    the output's groups so far are zero in all 64 lanes. Time to the same answer; the gates
+    skipped are not evaluated, so this is not a gate-evaluation rate."""
+    ctx.sat_solve(c, group_exits=True)  # kernel build (a body of its own)
+    t0 = time.perf_counter()
+    kernel_ms = 0.0
+    for _ in range(searches):
+        r = ctx.sat_solve(c, group_exits=True)
+        kernel_ms += r["kernel_ms"]
+    ms = (time.perf_counter() - t0) * 1000.0 / searches
+    return {"workload": SAT_WORKLOAD + ", NS_SAT_GROUP_EXITS", "ms_per_search": ms,
+            "kernel_ms_per_search": kernel_ms / searches,
+            "speedup_vs_full_evaluation": full_ms_per_step / ms,
+            "assignments_per_s": (1 << 36) / (ms / 1000.0),
+            "count": r["count"], "lowest": r["lowest"],
+            "answers_equal": (r["count"], r["lowest"]) == full_answer}
 
 
 def cpu_part(c, cone, threads):
@@ -504,7 +525,9 @@ def main():
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_part(c, cone, args.cpu_threads or min(os.cpu_count() or 1, 64))
     if not args.no_extra:
-        extra = {"satostop_table": table_part(ns, ctx), "satostop_36": stop36_part(ns, ctx),
+        extra = {"sat_group_exits": group_exits_part(ns, ctx, c, sat["ms_per_step"],
+                                                     (sat["config"]["count"], sat["config"]["lowest"])),
+                 "satostop_table": table_part(ns, ctx), "satostop_36": stop36_part(ns, ctx),
                  "satostop_hashmap": hashmap_part(ns, ctx), "qsat": qsat_part(ns, ctx)}
         extra["batch_repeat"] = batch_repeat_part(ns)
         if not args.no_serve:

@@ -95,6 +95,12 @@ int ns_ctx_trim(ns_ctx* ctx);
  * Input k of the circuit is bit k of the assignment x; the circuit must have one output.
  * Finds the number of satisfying assignments and the lowest one. */
 #define NS_SAT_STOP_AT_FIRST 1u /* stop after the first launch range holding a solution */
+/* Group exits (no counterpart in the reference, which evaluates every gate of every
+ * assignment): when the kernel evaluates the output's conjunction in groups of its literals
+ * (large circuits, from 2^34 assignments), a wave leaves a 32-assignment word as soon as the
+ * groups so far are zero in all of its lanes. The answers are the same; the gates skipped are
+ * not evaluated, so the search's time no longer measures a gate-evaluation rate. */
+#define NS_SAT_GROUP_EXITS 2u
 typedef struct {
   uint32_t elem_bits;      /* log2 assignments per launch (0 = engine default) */
   uint32_t group_len;      /* threads per block, multiple of 64 (0 = default) */
@@ -412,7 +418,8 @@ int ns_batch_solve(ns_ctx* ctx, const ns_request* reqs, uint32_t n, ns_request_r
  * kind: 0 SAT (output 0), 1 QSAT one-bit-per-thread, 2 QSAT outer-block-in-thread,
  * 3 next-state table (all outputs), 4 batch body with the SAT input order, 5 batch body with
  * the QSAT input order, 6 SATOSTOP walk body (two walk steps and their bookkeeping; lane_shift
- * = the waypoint bits k). Input k sits at index bit k (SAT, table, 4) or n-1-k (QSAT, 5);
+ * = the waypoint bits k), 7 SAT with NS_SAT_GROUP_EXITS. Input k sits at index bit k (SAT,
+ * table, 4, 7) or n-1-k (QSAT, 5);
  * lane_shift places the six lane bits at index bits 5+lane_shift..10+lane_shift.
  * Returns NS_ERR_UNSUPPORTED when the circuit exceeds the native register budget. */
 int ns_native_body(const ns_circuit* c, uint32_t kind, uint32_t lane_shift, char* body, size_t cap,

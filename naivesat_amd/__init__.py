@@ -24,6 +24,7 @@ LIB_PATH = os.path.join(_HERE, "lib", "libnaivesat_hip.so")
 BIN_DIR = os.path.join(_HERE, "bin")
 
 NS_SAT_STOP_AT_FIRST = 1
+NS_SAT_GROUP_EXITS = 2
 NS_ERR_CANCELLED = 7
 SAT_STEP_START = 0xFFFFFFFFFFFFFFFF  # NS_SAT_STEP_START: the progress call before the first launch
 NS_STOP_FIRST_PASS = 1
@@ -415,19 +416,21 @@ class Context:
         self.close()
 
     def sat_solve(self, c: Circuit, elem_bits: int = 0, group_len: int = 0, words_per_lane: int = 0,
-                  stop_at_first: bool = False) -> dict:
-        o = _SatOpts(elem_bits, group_len, words_per_lane, NS_SAT_STOP_AT_FIRST if stop_at_first else 0)
+                  stop_at_first: bool = False, group_exits: bool = False) -> dict:
+        """group_exits: NS_SAT_GROUP_EXITS (a wave leaves a word once the output's groups so far
+        are zero in all lanes; same answers)."""
+        o = _SatOpts(elem_bits, group_len, words_per_lane, _sat_flags(stop_at_first, group_exits))
         r = _SatResult()
         _check(load_library().ns_sat_solve(self._h, c._h, C.byref(o), C.byref(r)))
         return {k: getattr(r, k) for k, _ in _SatResult._fields_}
 
     def sat_solve_progress(self, c: Circuit, callback, elem_bits: int = 0, group_len: int = 0,
-                           words_per_lane: int = 0, stop_at_first: bool = False) -> dict:
+                           words_per_lane: int = 0, stop_at_first: bool = False, group_exits: bool = False) -> dict:
         """ns_sat_solve_progress: callback(step, steps, range_bits) -> bool (True stops the
         search), called once with step = SAT_STEP_START before the first launch, then after
         each launch range in order. An exception raised by the callback stops the search and is
         raised again afterwards."""
-        o = _SatOpts(elem_bits, group_len, words_per_lane, NS_SAT_STOP_AT_FIRST if stop_at_first else 0)
+        o = _SatOpts(elem_bits, group_len, words_per_lane, _sat_flags(stop_at_first, group_exits))
         r = _SatResult()
         err = []
 
@@ -446,10 +449,10 @@ class Context:
         return {k: getattr(r, k) for k, _ in _SatResult._fields_}
 
     def sat_solve_noalloc(self, c: Circuit, elem_bits: int = 0, group_len: int = 0, words_per_lane: int = 0,
-                          stop_at_first: bool = False) -> dict:
+                          stop_at_first: bool = False, group_exits: bool = False) -> dict:
         """ns_sat_solve_noalloc: the search without any device allocation; the circuit's kernel
         must have been built by an earlier sat_solve with the same options."""
-        o = _SatOpts(elem_bits, group_len, words_per_lane, NS_SAT_STOP_AT_FIRST if stop_at_first else 0)
+        o = _SatOpts(elem_bits, group_len, words_per_lane, _sat_flags(stop_at_first, group_exits))
         r = _SatResult()
         _check(load_library().ns_sat_solve_noalloc(self._h, c._h, C.byref(o), C.byref(r)))
         return {k: getattr(r, k) for k, _ in _SatResult._fields_}
@@ -625,6 +628,10 @@ class Context:
         w = C.c_uint32()
         _check(load_library().ns_satostop_pass_times(self._h, _ptr(buf), 64, C.byref(w)))
         return [float(x) for x in buf[: w.value]]
+
+
+def _sat_flags(stop_at_first, group_exits):
+    return (NS_SAT_STOP_AT_FIRST if stop_at_first else 0) | (NS_SAT_GROUP_EXITS if group_exits else 0)
 
 
 def _requests(requests):

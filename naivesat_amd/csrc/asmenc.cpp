@@ -1,7 +1,7 @@
 // gfx950 machine-code encoding of native circuit bodies and their placement in templates
-// (see asmenc.h). Encodings follow the GFX9 formats: VOP1, VOP3 (with v_bitop3_b32's
+// (see asmenc.h). Encodings follow the GFX9 formats: VOP1, VOPC, VOP3 (with v_bitop3_b32's
 // truth table spread over the ABS, OMOD and NEG fields), VOP3P (AGPR copies), SOP1, SOP2,
-// SOPP.
+// SOPP (s_nop and branches to the body's own labels).
 #include "asmenc.h"
 
 #include <cstdlib>
@@ -102,12 +102,54 @@ bool encode_body(const std::string& body, std::vector<uint32_t>& w, std::string*
     if (err) *err = m + ": " + line;
     return false;
   };
+  // local labels ("name:") and the branches to them, resolved at the end: a SOPP branch
+  // holds the signed distance in words from the instruction after it
+  std::map<std::string, size_t> labels;
+  std::vector<std::pair<size_t, std::string>> fixups;
   while (std::getline(in, line)) {
     size_t a = line.find_first_not_of(" \t");
     if (a == std::string::npos || line[a] == ';') continue;
     std::string t = line.substr(a);
+    while (!t.empty() && (t.back() == ' ' || t.back() == '\t' || t.back() == '\r')) t.pop_back();
+    if (!t.empty() && t.back() == ':' && t.find_first_of(" \t,") == std::string::npos) {
+      if (!labels.emplace(t.substr(0, t.size() - 1), w.size()).second) return fail("label defined twice");
+      continue;
+    }
     size_t sp = t.find_first_of(" \t");
     std::string mn = t.substr(0, sp);
+    if (mn == "s_branch" || mn == "s_cbranch_vccz") {
+      if (sp == std::string::npos) return fail("branch without a label");
+      std::string target = t.substr(sp + 1);
+      target.erase(0, target.find_first_not_of(" \t"));
+      if (target.empty() || target.find_first_of(" \t,") != std::string::npos) return fail("bad branch label");
+      fixups.push_back({w.size(), target});
+      w.push_back(mn == "s_branch" ? 0xBF820000u : 0xBF860000u);
+      continue;
+    }
+    if (mn == "v_cmp_ne_u32") {
+      // VOPC, result in vcc: "v_cmp_ne_u32 vcc, src0, vN"
+      std::vector<std::string> args;
+      std::string cur;
+      for (char ch : t.substr(sp == std::string::npos ? t.size() : sp + 1)) {
+        if (ch == ',') {
+          args.push_back(cur);
+          cur.clear();
+        } else if (ch != ' ' && ch != '\t') {
+          cur += ch;
+        }
+      }
+      if (!cur.empty()) args.push_back(cur);
+      Operand s0, s1;
+      if (args.size() != 3 || args[0] != "vcc" || !parse_operand(args[1], &s0) || !parse_operand(args[2], &s1) ||
+          s1.kind != Operand::VGPR)
+        return fail("bad v_cmp_ne_u32");
+      uint32_t c0, lv = 0;
+      bool lit = false;
+      if (!vsrc(s0, &c0, &lit, &lv)) return fail("bad source");
+      w.push_back(0x7C000000u | (0xCDu << 17) | ((uint32_t)s1.value << 9) | c0);
+      if (lit) w.push_back(lv);
+      continue;
+    }
     std::vector<std::string> args;
     std::string modifier;
     if (sp != std::string::npos) {
@@ -200,6 +242,19 @@ bool encode_body(const std::string& body, std::vector<uint32_t>& w, std::string*
     } else {
       return fail("instruction outside the encoded subset");
     }
+  }
+  for (const auto& f : fixups) {
+    auto it = labels.find(f.second);
+    if (it == labels.end()) {
+      line = f.second;
+      return fail("branch to an undefined label");
+    }
+    const long long d = (long long)it->second - (long long)(f.first + 1);
+    if (d < -32768 || d > 32767) {
+      line = f.second;
+      return fail("branch out of range");
+    }
+    w[f.first] |= (uint32_t)(d & 0xFFFF);
   }
   return true;
 }

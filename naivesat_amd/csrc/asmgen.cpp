@@ -21,6 +21,9 @@ namespace ns {
 namespace {
 
 const uint32_t kWordMasks[5] = {0xAAAAAAAAu, 0xCCCCCCCCu, 0xF0F0F0F0u, 0xFF00FF00u, 0xFFFF0000u};
+// labels of the group exits (one body per module or code object region: unique there)
+const char* const kExitLabel = ".Lns_exit";
+const char* const kDoneLabel = ".Lns_done";
 // SGPRs of the body: s[40..101]; the SAT templates, whose own code keeps to s0..s13, also
 // give it s[14..19] and s[22..39] (s20 and s21 carry the word index)
 const uint32_t kSgprLoSat = 14, kSgprLo = 40, kSgprLoBatch = 48, kSgprHi = 101;
@@ -52,14 +55,19 @@ class Allocator {
   // template register); out_regs (optional): output k goes to VGPR out_regs[k] instead of vk,
   // which may be an input's register: an output is computed into it only once that input is
   // dead, otherwise copied there at the end
+  // exits (optional): running conjunctions of the output groups, (wire, negated); after each
+  // is computed the body leaves for its end with output 0 when it is zero in every lane
+  // (s_cbranch_vccz), since the later groups cannot make the output nonzero
   Allocator(const Circuit& c, const LutProgram& p, const std::vector<uint32_t>& vbit, uint32_t lane_shift,
             uint32_t lm_base, uint32_t pool_lo, uint32_t pool_hi, uint32_t agprs, uint32_t slot_regs,
             const std::vector<uint32_t>& out_wires, const std::vector<uint8_t>& out_neg, uint32_t sgpr_lo,
-            const std::vector<int>* in_regs = nullptr, const std::vector<uint32_t>* out_regs = nullptr)
+            const std::vector<int>* in_regs = nullptr, const std::vector<uint32_t>* out_regs = nullptr,
+            const std::vector<std::pair<uint32_t, bool>>* exits = nullptr)
       : c_(c), p_(p), vbit_(vbit), ls_(lane_shift), lm_(lm_base), pool_lo_(pool_lo), pool_hi_(pool_hi),
         sgpr_lo_(sgpr_lo),
         loc_(c.wires()), uses_(c.wires()), ptr_(c.wires(), 0), outw_(out_wires), outneg_(out_neg) {
     for (size_t k = 0; k < out_wires.size(); k++) outreg_.push_back(out_regs ? (*out_regs)[k] : (uint32_t)k);
+    if (exits) exits_ = *exits;
     slot_base_ = (int)pool_hi - (int)slot_regs + 1;
     pool_hi_ = pool_hi - slot_regs;
     for (int r = (int)pool_hi_; r >= (int)pool_lo; r--) vfree_[r & 3].push_back(r);
@@ -158,6 +166,13 @@ class Allocator {
       loc_[op.dst].v = d;
       out_ << "\tv_bitop3_b32 v" << d << ", " << src[0] << ", " << src[1] << ", " << src[2] << " bitop3:"
            << hex(op.fn) << "\n";
+      for (const auto& ex : exits_)
+        if (ex.first == op.dst) {
+          // a negated conjunction literal is zero where its wire is all ones
+          out_ << "\tv_cmp_ne_u32 vcc, " << (ex.second ? "-1" : "0") << ", v" << d << "\n\ts_cbranch_vccz "
+               << kExitLabel << "\n";
+          exit_tests_++;
+        }
       if (uses_[op.dst].empty()) kill(op.dst);
     }
   }
@@ -201,8 +216,14 @@ class Allocator {
         if (outneg_[k]) out_ << "\tv_not_b32 v" << dk << ", v" << dk << "\n";
       }
     }
+    if (exit_tests_) {
+      if (outw_.size() != 1 || outreg_[0] != 0) throw std::runtime_error("internal: group exits need one output in v0");
+      out_ << "\ts_branch " << kDoneLabel << "\n" << kExitLabel << ":\n\tv_mov_b32 v0, 0\n" << kDoneLabel << ":\n";
+    }
     return out_.str();
   }
+
+  uint32_t exit_tests() const { return exit_tests_; }
 
   uint32_t spills() const { return spills_; }
   uint32_t reloads() const { return reloads_; }
@@ -444,6 +465,8 @@ class Allocator {
   std::ostringstream out_;
   uint32_t cur_ = 0, end_ = 0;
   uint32_t spills_ = 0, reloads_ = 0;
+  std::vector<std::pair<uint32_t, bool>> exits_;
+  uint32_t exit_tests_ = 0;
 };
 
 // Reorders a LUT program to lower the number of simultaneously live lane values: greedy
@@ -850,7 +873,7 @@ std::vector<std::vector<size_t>> refine_groups(std::vector<std::vector<size_t>> 
 // Returns false when the output has fewer than K literals.
 bool grouped_circuit(const Circuit& c, uint32_t out_k, uint32_t K, Circuit& out, std::vector<uint32_t>& wgroup,
                      uint32_t refine_cap_pct, uint32_t pin = 0, const std::vector<uint8_t>* uni_in = nullptr,
-                     uint32_t pin_u = 0) {
+                     uint32_t pin_u = 0, std::vector<std::pair<uint32_t, bool>>* accs = nullptr) {
   const Output o = c.outputs.at(out_k);
   const auto lits = and_literals(c, o.wire);
   if (K < 2 || lits.size() < K) return false;
@@ -1041,6 +1064,7 @@ bool grouped_circuit(const Circuit& c, uint32_t out_k, uint32_t K, Circuit& out,
       v = {conj(v, {remap[lits[groups[gi][q]].first], lits[groups[gi][q]].second}, grp), false};
     acc = have_acc ? std::pair<uint32_t, bool>{conj(acc, v, grp), false} : v;
     have_acc = true;
+    if (accs && gi + 1 < groups.size()) accs->push_back(acc);
   }
   out.outputs.push_back({acc.first, acc.second != o.neg});
   return true;
@@ -1144,7 +1168,7 @@ Circuit walk_iteration(const Circuit& c, uint32_t k) {
 }
 
 NativeKernel build_native(const Circuit& c, NativeKind kind, const std::vector<uint32_t>& vbit,
-                          uint32_t lane_shift, const std::vector<uint32_t>& outs) {
+                          uint32_t lane_shift, const std::vector<uint32_t>& outs, bool group_exits) {
   NativeKernel nk;
   auto cone = output_cone(c, outs);
   auto order = schedule_gates(c, cone, outs);
@@ -1174,7 +1198,8 @@ NativeKernel build_native(const Circuit& c, NativeKind kind, const std::vector<u
     uint32_t slots() const { return vector_lines + scalar_lines + lane_reads; }
   };
   // LUT mapping, schedule and the smallest register class that holds the program of cc
-  auto attempt = [&](const Circuit& cc, const std::vector<uint32_t>& outs_cc, const std::vector<uint32_t>* wgroup) {
+  auto attempt = [&](const Circuit& cc, const std::vector<uint32_t>& outs_cc, const std::vector<uint32_t>* wgroup,
+                     const std::vector<std::pair<uint32_t, bool>>* accs = nullptr) {
     Attempt at;
     std::vector<uint32_t> out_wires;
     std::vector<uint8_t> out_neg;
@@ -1185,7 +1210,12 @@ NativeKernel build_native(const Circuit& c, NativeKind kind, const std::vector<u
     auto cone_cc = output_cone(cc, outs_cc);
     auto order_cc = schedule_gates(cc, cone_cc, outs_cc);
     Circuit cx = cc;  // with the scalar gates of split_uniform_pairs appended
-    LutProgram p0 = map_luts(cc, order_cc, out_wires, uni);
+    // group exits test the groups' running conjunctions: those are materialised values
+    const bool exits = group_exits && kind == NativeKind::SAT && accs && !accs->empty();
+    std::vector<uint32_t> mapped = out_wires;
+    if (exits)
+      for (const auto& a : *accs) mapped.push_back(a.first);
+    LutProgram p0 = map_luts(cc, order_cc, mapped, uni);
     split_uniform_pairs(cx, p0);
     LutProgram p = pressure_order(p0, out_wires, cx.wires(), wgroup);
     at.valu = p.valu_ops;
@@ -1219,7 +1249,8 @@ NativeKernel build_native(const Circuit& c, NativeKind kind, const std::vector<u
       if (!cl.agprs && peak + cl.slots > cl.hi - pool_lo + 1) continue;
       try {
         Allocator a(cx, p, vbit, lane_shift, lm, pool_lo, cl.hi, cl.agprs, cl.slots, out_wires, out_neg,
-                    kind == NativeKind::SAT ? kSgprLoSat : kind == NativeKind::BATCH ? kSgprLoBatch : kSgprLo);
+                    kind == NativeKind::SAT ? kSgprLoSat : kind == NativeKind::BATCH ? kSgprLoBatch : kSgprLo,
+                    nullptr, nullptr, exits && cl.budget >= 256 ? accs : nullptr);  // vcc: r256, r512
         at.body = a.run();
         at.vgpr_class = cl.budget;
         at.spills = a.spills();
@@ -1275,8 +1306,9 @@ NativeKernel build_native(const Circuit& c, NativeKind kind, const std::vector<u
       for (uint32_t cap_pct : {100u, 0u}) {
         Circuit g;
         std::vector<uint32_t> wg;
-        if (!grouped_circuit(c, outs[0], K, g, wg, cap_pct)) break;
-        Attempt at = attempt(g, {0}, &wg);
+        std::vector<std::pair<uint32_t, bool>> accs;
+        if (!grouped_circuit(c, outs[0], K, g, wg, cap_pct, 0, nullptr, 0, &accs)) break;
+        Attempt at = attempt(g, {0}, &wg, &accs);
         if (forced > 1 && at.vgpr_class && (cap_pct || at.vgpr_class <= 256) && !fits) {
           best = std::move(at);
           groups = K;
@@ -1307,8 +1339,9 @@ NativeKernel build_native(const Circuit& c, NativeKind kind, const std::vector<u
             try {
               Circuit g;
               std::vector<uint32_t> wg;
-              grouped_circuit(c, outs[0], K, g, wg, tries[q].first, tries[q].second, &uni, kKeptUniform);
-              ats[q] = attempt(g, {0}, &wg);
+              std::vector<std::pair<uint32_t, bool>> accs;
+              grouped_circuit(c, outs[0], K, g, wg, tries[q].first, tries[q].second, &uni, kKeptUniform, &accs);
+              ats[q] = attempt(g, {0}, &wg, &accs);
             } catch (...) {
               err[q] = std::current_exception();
             }

@@ -35,9 +35,11 @@ struct NativeKernel {
 
 // Builds the assembly module for `kind`. vbit_of_input[k] = position of input k in the
 // assignment index; lane bits are index bits 5+lane_shift .. 10+lane_shift; outs = the
-// outputs evaluated (their values land in v0.. in order).
+// outputs evaluated (their values land in v0.. in order). group_exits (SAT): when the output
+// is evaluated in groups, the body leaves early with output 0 once the groups so far give 0
+// in every lane of the wave (the answers do not change; the skipped gates are not evaluated).
 NativeKernel build_native(const Circuit& c, NativeKind kind, const std::vector<uint32_t>& vbit_of_input,
-                          uint32_t lane_shift, const std::vector<uint32_t>& outs);
+                          uint32_t lane_shift, const std::vector<uint32_t>& outs, bool group_exits = false);
 
 // Walk body of the table-free SATOSTOP walks (templates' ns_nat_walk<T>_r<class>, T = the
 // smallest state tier of 16, 24, 32, 40, 48 that holds c.n): two walk steps and their

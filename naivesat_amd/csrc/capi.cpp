@@ -63,7 +63,7 @@ void need(bool ok, const char* what) {
 
 // native kernel of a diagnostic kind: 0 SAT, 1 QSAT-A, 2 QSAT-B, 3 table, 4 batch body in
 // the SAT input order, 5 batch body in the QSAT input order, 6 table-free walk body
-// (lane_shift = waypoint bits)
+// (lane_shift = waypoint bits), 7 SAT with group exits
 ns::NativeKernel native_kernel(const ns::Circuit& cc, uint32_t kind, uint32_t lane_shift) {
   if (kind == 6) {
     try {
@@ -78,10 +78,11 @@ ns::NativeKernel native_kernel(const ns::Circuit& cc, uint32_t kind, uint32_t la
     for (uint32_t k = 0; k < cc.outputs.size(); k++) outs.push_back(k);
   else
     outs.push_back(0);
-  static const ns::NativeKind kinds[6] = {ns::NativeKind::SAT,   ns::NativeKind::QSAT_A, ns::NativeKind::QSAT_B,
-                                          ns::NativeKind::TABLE, ns::NativeKind::BATCH,  ns::NativeKind::BATCH};
+  static const ns::NativeKind kinds[8] = {ns::NativeKind::SAT,   ns::NativeKind::QSAT_A, ns::NativeKind::QSAT_B,
+                                          ns::NativeKind::TABLE, ns::NativeKind::BATCH,  ns::NativeKind::BATCH,
+                                          ns::NativeKind::SAT,   ns::NativeKind::SAT};
   try {
-    return ns::build_native(cc, kinds[kind], vb, lane_shift, outs);
+    return ns::build_native(cc, kinds[kind], vb, lane_shift, outs, kind == 7);
   } catch (const ns::NativeUnfit& e) {
     throw ns::Error(NS_ERR_UNSUPPORTED, e.what());
   }
@@ -550,7 +551,7 @@ int ns_batch_solve(ns_ctx* ctx, const ns_request* reqs, uint32_t n, ns_request_r
 int ns_native_body(const ns_circuit* c, uint32_t kind, uint32_t lane_shift, char* body, size_t cap,
                    size_t* needed, uint32_t* vgpr_class, uint64_t* code_bytes) {
   return guard([&] {
-    need(c && kind <= 6, "bad argument");
+    need(c && kind <= 7, "bad argument");
     const ns::Circuit& cc = c->c;
     ns::NativeKernel nk = native_kernel(cc, kind, lane_shift);
     std::string text;
@@ -594,7 +595,7 @@ int ns_native_encode(const char* body, size_t len, uint32_t* words, size_t cap, 
 int ns_native_code(const ns_circuit* c, uint32_t kind, uint32_t lane_shift, void* code, size_t cap,
                    size_t* needed, uint32_t* patched, uint64_t* slot_offset) {
   return guard([&] {
-    need(c && kind <= 6, "bad argument");
+    need(c && kind <= 7, "bad argument");
     ns::NativeKernel nk = native_kernel(c->c, kind, lane_shift);
     size_t off = 0;
     std::vector<char> obj;

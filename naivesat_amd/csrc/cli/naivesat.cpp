@@ -11,6 +11,8 @@ static const char* kUsage =
     "  -G, --opencl-group-len <G> threads per block\n"
     "  -W, --words-per-lane <W>   32-assignment words per lane\n"
     "  -c, --count                search the whole space and print the satisfying count\n"
+    "  -X, --no-group-exits       evaluate every gate of every assignment (by default a wave\n"
+    "                             leaves a word once the output's groups so far are zero)\n"
     "Ctrl-C stops the search after the launch ranges in flight.\n";
 
 int main(int argc, char** argv) {
@@ -20,7 +22,8 @@ int main(int argc, char** argv) {
                             {'n', "no-optimize-negs", false},
                             {'G', "opencl-group-len", true},
                             {'W', "words-per-lane", true},
-                            {'c', "count", false}},
+                            {'c', "count", false},
+                            {'X', "no-group-exits", false}},
                            kUsage);
   if (a.positional.size() != 1) cli::die(std::string("expected one circuit file\n") + kUsage);
   if (!a.has("exec-type")) cli::die(std::string("--exec-type is required\n") + kUsage);
@@ -45,7 +48,7 @@ int main(int argc, char** argv) {
   o.elem_bits = n >= 14 ? (uint32_t)e : 0u;
   o.group_len = (uint32_t)a.num("opencl-group-len", 0);
   o.words_per_lane = (uint32_t)a.num("words-per-lane", 0);
-  o.flags = a.has("count") ? 0u : NS_SAT_STOP_AT_FIRST;
+  o.flags = (a.has("count") ? 0u : NS_SAT_STOP_AT_FIRST) | (a.has("no-group-exits") ? 0u : NS_SAT_GROUP_EXITS);
   cli::Progress prog;
   prog.print = n >= 14;
   prog.elem = n >= 14 ? e : -1;

@@ -207,13 +207,13 @@ Chosen choose_kernel(Lane& ctx, const Circuit& c, NativeKind kind, const std::ve
                      uint32_t lane_shift, const std::vector<uint32_t>& outs, const Geometry& g,
                      const std::string& extra_key, const std::function<std::string(GenInfo*)>& hiprtc_src,
                      const char* hiprtc_kernel, double* compile_ms, std::string* fp_out = nullptr,
-                     bool lookup_only = false) {
+                     bool lookup_only = false, bool group_exits = false) {
   const bool native_ok = native_wanted(g);
   const char* og = std::getenv("NS_OUTPUT_GROUPS");  // changes the native body
   std::string fp = circuit_fingerprint(c) + "/" + (og ? og : "") + "/" + std::to_string((int)kind) + "/" +
                    std::to_string(lane_shift) +
                    "/" + std::to_string(g.block) + "/" + std::to_string(g.wpl) + "/" + std::to_string(g.j_shift) +
-                   (native_ok ? "/n/" : "/r/") + extra_key;
+                   (native_ok ? "/n/" : "/r/") + extra_key + (group_exits ? "/x" : "");
   for (uint32_t k : outs) fp += "," + std::to_string(k);
   if (fp_out) *fp_out = fp;
   Chosen ch;
@@ -228,7 +228,7 @@ Chosen choose_kernel(Lane& ctx, const Circuit& c, NativeKind kind, const std::ve
   double cms = 0.0;
   if (native_ok) {
     try {
-      NativeKernel nk = build_native(c, kind, vb, lane_shift, outs);
+      NativeKernel nk = build_native(c, kind, vb, lane_shift, outs, group_exits);
       ch.mod = ctx.jit.get_native(nk.kernel_name, nk.body, nk.asm_text, &cms);
       ch.info = nk.info;
       ch.native = true;
@@ -282,7 +282,8 @@ void sat_solve(Lane& ctx, const Circuit& c, const ns_sat_opts& o, ns_sat_result&
   std::string fp;
   Chosen ch = choose_kernel(
       ctx, c, NativeKind::SAT, vb, 0, {0}, g, "",
-      [&](GenInfo* info) { return gen_sat_kernel(c, g, info); }, "ns_sat", &r.compile_ms, &fp);
+      [&](GenInfo* info) { return gen_sat_kernel(c, g, info); }, "ns_sat", &r.compile_ms, &fp, false,
+      (o.flags & NS_SAT_GROUP_EXITS) != 0);
   // Words per lane and threads per block are run-time parameters of the native kernel: a long
   // search times its first launches with each candidate pair and keeps the fastest (cached
   // with the kernel). Every word is still evaluated exactly once, so the answers do not
@@ -453,7 +454,7 @@ void sat_solve_noalloc(Lane& ctx, const Circuit& c, const ns_sat_opts& o, ns_sat
   for (uint32_t k = 0; k < c.n; k++) vb[k] = k;
   Chosen ch = choose_kernel(
       ctx, c, NativeKind::SAT, vb, 0, {0}, g, "", [&](GenInfo* info) { return gen_sat_kernel(c, g, info); },
-      "ns_sat", &r.compile_ms, nullptr, true);
+      "ns_sat", &r.compile_ms, nullptr, true, (o.flags & NS_SAT_GROUP_EXITS) != 0);
   const uint64_t cw = chunk_words(o.elem_bits, g, total_words);
   if (ch.native && ch.tuned_wpl && o.words_per_lane == 0 && o.group_len == 0 &&
       cw % ((uint64_t)ch.tuned_block * ch.tuned_wpl) == 0) {

@@ -10,7 +10,8 @@ Register contract of every placeholder ("; NS_BODY <name>"):
                          the six lane bits
   outputs  v0 .. v[OUT-1] circuit outputs for the current word
   scratch  v[POOL_LO..POOL_HI], s[40..101], scc (clobbered); the SAT kernels, whose own code
-           keeps to s0..s13, also give the body s[14..19] and s[22..39]
+           keeps to s0..s13, also give the body s[14..19] and s[22..39], and the r256 / r512
+           ones vcc (with it clobbered the r64 / r128 ones' own code needs one VGPR more)
 Register classes bound the VGPR budget (and with it the occupancy).
 
 Usage: python gen_templates.py OUT.hip
@@ -30,12 +31,14 @@ QSATB_POOL_CUT = 2
 SAT_SGPR_POOL = list(range(14, 20)) + list(range(22, 102))  # s20, s21 carry the word index
 
 
-def clobbers(lo, hi, agprs=False, sgprs=SGPR_POOL):
+def clobbers(lo, hi, agprs=False, sgprs=SGPR_POOL, vcc=False):
     regs = [f'"v{i}"' for i in range(lo, hi + 1)]
     if agprs:
         regs += [f'"a{i}"' for i in range(256)]
     regs += [f'"s{i}"' for i in sgprs]
     regs.append('"scc"')
+    if vcc:  # the group exits of the r256 / r512 SAT bodies compare into vcc
+        regs.append('"vcc"')
     return ", ".join(regs)
 
 
@@ -88,12 +91,12 @@ __device__ inline u32 ns_wave_excl(u32 r, u32* total) {
 '''
 
 
-def body1(name, lm, lo, hi, sgprs=SGPR_POOL):
+def body1(name, lm, lo, hi, sgprs=SGPR_POOL, vcc=False):
     """placeholder with one output in v0"""
     ins = ", ".join([f'"{{v{lm + i}}}"(LM{i})' for i in range(6)] + ['"{s20}"(wlo)', '"{s21}"(whi)'])
     agprs = name.endswith("r512")
     return (f'    u32 o;\n'
-            f'    asm volatile("; NS_BODY {name}" : "={{v0}}"(o) : {ins} : {clobbers(lo, hi, agprs, sgprs)});\n')
+            f'    asm volatile("; NS_BODY {name}" : "={{v0}}"(o) : {ins} : {clobbers(lo, hi, agprs, sgprs, vcc)});\n')
 
 
 def sat_kernel(cls, vb, pool_hi):
@@ -110,7 +113,7 @@ extern "C" __global__ void __launch_bounds__(256) {name}(u64 w0, u64 wend, u64* 
     const u32 wlo = (u32)wu, whi = (u32)(wu >> 32);
     // words wu + lane at or past wend are not counted: lane >= rem (uniform)
     const u64 rem = wend > wu ? wend - wu : 0ull;
-{body1(name, 1, 8, pool_hi, SAT_SGPR_POOL)}    o &= valid;
+{body1(name, 1, 8, pool_hi, SAT_SGPR_POOL, vcc=vb >= 256)}    o &= valid;
     if ((u64)lane >= rem) o = 0u;
     cnt += __popc(o);
     if (o != 0u && brel == ~0u) brel = (((j << 6) + lane) << 5) | (u32)__builtin_ctz(o);

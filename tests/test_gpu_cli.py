@@ -37,6 +37,19 @@ def test_naivesat_config0(tmp_path):
     assert "Start execution" in out and any(x.startswith("Time: ") for x in out)
 
 
+def test_naivesat_group_exits_and_full_evaluation(tmp_path):
+    # a circuit whose output takes groups: the default (group exits) and -X (every gate
+    # evaluated) print the oracle's count and lowest
+    c, key = ns.Circuit.gen_sat(2, 26, 40000, 48)
+    f = tmp_path / "big.circuit"
+    f.write_text(c.text())
+    count, lowest = oracle.OracleCircuit.from_dict(c.arrays()).sat(threads=8)
+    for extra in ([], ["-X"]):
+        out = run("naivesat", "-t", "opencl:0", "-c", *extra, f)
+        assert f"Count: {count}" in out, extra
+        assert f"Found Input: {lowest:026b}" in out, extra
+
+
 def test_naivesat_small_and_unsatisfiable(tmp_path):
     f = tmp_path / "small.circuit"
     # x0 & ~x1 & x2: lowest satisfying assignment is 0b101

@@ -196,3 +196,33 @@ def test_groups_for_a_circuit_no_class_holds(ctx):
     assert r["backend"] == 1 and r["vgpr_class"] == cls
     assert (r["count"], r["lowest"]) == oracle.OracleCircuit.from_dict(c.arrays()).sat(threads=8)
     assert r["lowest"] <= key
+
+
+def test_group_exits_config1_golden(ctx):
+    """NS_SAT_GROUP_EXITS on the full 2^36 search: waves leave a word once the output groups
+    so far are zero in all lanes; count and lowest stay the golden ones."""
+    g = golden("cfg1")
+    c, key = ns.Circuit.gen_sat(2, 36, 20000, 40)
+    r = ctx.sat_solve(c, group_exits=True)
+    assert r["complete"] == 1 and r["backend"] == 1
+    assert (r["count"], r["lowest"]) == (g["count"], g["lowest"])
+
+
+@pytest.mark.parametrize("groups", ["", "3"])
+def test_group_exits_keep_answers(ctx, monkeypatch, groups):
+    """Group exits against the oracle: a 40,000-gate circuit whose output takes groups
+    (48-bit word, few solutions: most waves leave early) and, with three groups forced, one
+    with a 12-bit word (many solutions: waves that hit run every group)."""
+    if groups:
+        monkeypatch.setenv("NS_OUTPUT_GROUPS", groups)
+    for args in ((2, 26, 40000, 48), (3, 26, 40000, 12)):
+        c, key = ns.Circuit.gen_sat(*args)
+        body, cls, _ = c.native_body(7)
+        if "s_cbranch_vccz" not in body:
+            continue  # no groups for this circuit here: nothing to exit from
+        want = oracle.OracleCircuit.from_dict(c.arrays()).sat(threads=8)
+        r = ctx.sat_solve(c, group_exits=True)
+        assert (r["count"], r["lowest"]) == want, args
+        assert r["lowest"] <= key
+        r = ctx.sat_solve(c, group_exits=True, words_per_lane=1)  # also one word per lane
+        assert (r["count"], r["lowest"]) == want, args

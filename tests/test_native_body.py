@@ -12,8 +12,9 @@ M32 = 0xFFFFFFFF
 LANES = np.arange(64, dtype=np.uint32)
 
 
-def run_body(body, lm_base, wu, vregs=None):
-    """vregs: initial VGPRs (register -> 64 lane values) instead of the lane masks"""
+def run_body(body, lm_base, wu, vregs=None, trace=None):
+    """vregs: initial VGPRs (register -> 64 lane values) instead of the lane masks; trace
+    (a dict): trace["branches"] lists the labels the taken branches went to"""
     v, s, acc = {}, {20: wu & M32, 21: (wu >> 32) & M32}, {}
     if vregs is not None:
         v = {r: x.copy() for r, x in vregs.items()}
@@ -26,18 +27,37 @@ def run_body(body, lm_base, wu, vregs=None):
             return v[int(tok[1:])]
         if tok.startswith("s"):
             return np.full(64, s[int(tok[1:])], np.uint32)
-        return np.full(64, int(tok, 0) & M32, np.uint32)
+        return np.full(64, int(tok, 0) & M32, np.uint32)  # constants (-1: all ones)
 
     def sval(tok):
         return s[int(tok[1:])] if tok.startswith("s") else int(tok, 0) & M32
 
-    for line in body.splitlines():
-        line = line.strip()
-        if not line or line.startswith(";"):
+    lines = [ln.strip() for ln in body.splitlines()]
+    lines = [ln for ln in lines if ln and not ln.startswith(";")]
+    labels = {ln[:-1]: i for i, ln in enumerate(lines) if ln.endswith(":")}
+    vcc = 0
+    pc = 0
+    while pc < len(lines):
+        line = lines[pc]
+        pc += 1
+        if line.endswith(":"):
             continue
         op, rest = line.split(None, 1)
         args = [a.strip() for a in rest.split(",")]
-        if op == "s_mov_b32":
+        if op == "v_cmp_ne_u32":
+            assert args[0] == "vcc"
+            a, b = val(args[1]), val(args[2])
+            vcc = sum(1 << i for i in range(64) if a[i] != b[i])
+        elif op == "s_cbranch_vccz":
+            if vcc == 0:
+                pc = labels[args[0]]
+                if trace is not None:
+                    trace.setdefault("branches", []).append(args[0])
+        elif op == "s_branch":
+            pc = labels[args[0]]
+            if trace is not None:
+                trace.setdefault("branches", []).append(args[0])
+        elif op == "s_mov_b32":
             s[int(args[0][1:])] = sval(args[1])
         elif op == "s_bfe_i32":
             src, ctl = sval(args[1]), sval(args[2])
@@ -100,7 +120,7 @@ def expected_words(oc, n, outs, vbit, lane_shift, wu):
     return res
 
 
-def check(c, kind, lane_shift, wus):
+def check(c, kind, lane_shift, wus, traces=None):
     body, cls, size = c.native_body(kind, lane_shift)
     assert size > 0 and cls in (64, 128, 256, 512)
     d = c.arrays()
@@ -110,7 +130,10 @@ def check(c, kind, lane_shift, wus):
     vbit = [n - 1 - k for k in range(n)] if kind in (1, 2, 5) else list(range(n))
     lm = 32 if kind == 3 else 1
     for wu in wus:
-        v = run_body(body, lm, wu)
+        tr = {}
+        v = run_body(body, lm, wu, trace=tr)
+        if traces is not None:
+            traces.append(tr.get("branches", []))
         exp = expected_words(oc, n, outs, vbit, lane_shift, wu)
         for i in range(len(outs)):
             got = v[i]
@@ -202,6 +225,21 @@ def test_config1_body_in_output_groups():
     # also the waves holding the planted key and the lowest solution (798,579,149, the
     # full-size golden): words with satisfying bits, not only all-zero outputs
     check(c, 0, 0, [64 * 12345, (1 << 31) - 64, (key >> 5) & ~63, (798579149 >> 5) & ~63])
+
+
+def test_config1_body_with_group_exits():
+    """NS_SAT_GROUP_EXITS (kind 7): after each output group but the last the body leaves with
+    output 0 when the running conjunction is zero in all 64 lanes. Same outputs as the oracle
+    on waves that leave early and on the waves holding the key and the lowest solution, which
+    run every group."""
+    c, key = ns.Circuit.gen_sat(2, 36, 20000, 40)
+    body, cls, size = c.native_body(7)
+    assert cls == 256 and body.count("s_cbranch_vccz .Lns_exit") == 4
+    traces = []
+    wus = [64 * 12345, 64 * 999_999, (1 << 31) - 64, (key >> 5) & ~63, (798579149 >> 5) & ~63]
+    check(c, 7, 0, wus, traces)
+    assert any(t == [".Lns_exit"] for t in traces[:3])  # a zero wave leaves at a group exit
+    assert traces[3] == [".Lns_done"] and traces[4] == [".Lns_done"]  # solutions: every group
 
 
 def test_groups_when_one_group_fits_no_class():
