@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <cctype>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <exception>
@@ -873,10 +874,12 @@ std::vector<std::vector<size_t>> refine_groups(std::vector<std::vector<size_t>> 
 // Returns false when the output has fewer than K literals.
 bool grouped_circuit(const Circuit& c, uint32_t out_k, uint32_t K, Circuit& out, std::vector<uint32_t>& wgroup,
                      uint32_t refine_cap_pct, uint32_t pin = 0, const std::vector<uint8_t>* uni_in = nullptr,
-                     uint32_t pin_u = 0, std::vector<std::pair<uint32_t, bool>>* accs = nullptr) {
+                     uint32_t pin_u = 0, std::vector<std::pair<uint32_t, bool>>* accs = nullptr,
+                     uint32_t first_lits = 0) {
   const Output o = c.outputs.at(out_k);
   const auto lits = and_literals(c, o.wire);
   if (K < 2 || lits.size() < K) return false;
+  if (first_lits && lits.size() < first_lits + K - 1) return false;
   const uint32_t G = (uint32_t)c.gates.size();
   // the greedy grouping takes ~L^2 G steps and the cone marks L G bytes (L literals, G gates):
   // outputs past these bounds are not grouped
@@ -901,11 +904,38 @@ bool grouped_circuit(const Circuit& c, uint32_t out_k, uint32_t K, Circuit& out,
     for (uint8_t b : cone[i]) k += b;
     return k;
   };
-  // greedy groups of ceil(m / K): the largest remaining cone, then the literals sharing most
   std::vector<uint8_t> used(lits.size(), 0);
-  const size_t per = (lits.size() + K - 1) / K;
   std::vector<std::vector<size_t>> groups;
-  for (size_t left = lits.size(); left;) {
+  size_t left = lits.size(), rest_groups = K;
+  if (first_lits) {
+    // Exit-first (for group exits): the first group takes the first_lits literals whose cones
+    // add the fewest gates to it, one at a time; later groups run only in the waves where
+    // the first one left some assignment standing, so their size matters little.
+    std::vector<uint8_t> u(G, 0);
+    std::vector<size_t> grp;
+    while (grp.size() < first_lits) {
+      size_t pick = SIZE_MAX;
+      uint32_t pbest = 0;
+      for (size_t i = 0; i < lits.size(); i++) {
+        if (used[i]) continue;
+        uint32_t extra = 0;
+        for (uint32_t g = 0; g < G; g++) extra += cone[i][g] & (u[g] ^ 1);
+        if (pick == SIZE_MAX || extra < pbest) {
+          pick = i;
+          pbest = extra;
+        }
+      }
+      grp.push_back(pick);
+      used[pick] = 1;
+      left--;
+      for (uint32_t g = 0; g < G; g++) u[g] |= cone[pick][g];
+    }
+    groups.push_back(std::move(grp));
+    rest_groups = K - 1;
+  }
+  // greedy groups of ceil(m / K): the largest remaining cone, then the literals sharing most
+  const size_t per = (left + rest_groups - 1) / rest_groups;
+  for (; left;) {
     size_t first = SIZE_MAX;
     uint32_t best = 0;
     for (size_t i = 0; i < lits.size(); i++)
@@ -951,7 +981,14 @@ bool grouped_circuit(const Circuit& c, uint32_t out_k, uint32_t K, Circuit& out,
     for (size_t i = 0; i < lits.size(); i++)
       for (uint32_t g = 0; g < G; g++)
         if (cone[i][g]) clist[i].push_back(g);
-    groups = refine_groups(groups, clist, G, refine_cap_pct);
+    if (first_lits) {  // the first group stays as chosen
+      std::vector<std::vector<size_t>> tail(groups.begin() + 1, groups.end());
+      if (tail.size() >= 2) tail = refine_groups(tail, clist, G, refine_cap_pct);
+      tail.insert(tail.begin(), groups[0]);
+      groups = std::move(tail);
+    } else {
+      groups = refine_groups(groups, clist, G, refine_cap_pct);
+    }
   }
   out = Circuit();
   out.n = c.n;
@@ -1287,7 +1324,62 @@ NativeKernel build_native(const Circuit& c, NativeKind kind, const std::vector<u
   // Groups are tried when the single-group program needs one wave per SIMD (from 2^34
   // assignments) or fits no class at all (any size: the alternative is the compiler path).
   const bool unfit = best.vgpr_class == 0;
-  if (!table && outs.size() == 1 && forced != 1 && (forced > 1 || unfit || (best.vgpr_class == 512 && c.n >= 34))) {
+  bool exit_first = false;
+  if (group_exits && kind == NativeKind::SAT && outs.size() == 1 && forced < 0 &&
+      (unfit || (best.vgpr_class == 512 && c.n >= 34))) {
+    // Group exits: the first group takes the literals whose cones are cheapest together, so
+    // that most waves leave after it. Expected issue slots per word = the slots before the
+    // first exit + the chance that a 2,048-assignment word survives the first group's L
+    // literals (1 - (1 - 2^-L)^2048, each literal taken as an even coin) x the slots after.
+    struct Try {
+      uint32_t L, K;
+    };
+    std::vector<Try> tries;
+    for (uint32_t L : {12u, 14u, 16u, 18u, 20u, 24u})
+      for (uint32_t K : {2u, 3u, 4u, 5u}) tries.push_back({L, K});
+    std::vector<Attempt> ats(tries.size());
+    std::vector<std::exception_ptr> err(tries.size());
+    std::vector<std::thread> th;
+    for (size_t q = 0; q < tries.size(); q++)
+      th.emplace_back([&, q]() {
+        try {
+          Circuit g;
+          std::vector<uint32_t> wg;
+          std::vector<std::pair<uint32_t, bool>> accs;
+          if (grouped_circuit(c, outs[0], tries[q].K, g, wg, 100, 16, &uni, kKeptUniform, &accs, tries[q].L))
+            ats[q] = attempt(g, {0}, &wg, &accs);
+        } catch (...) {
+          err[q] = std::current_exception();
+        }
+      });
+    for (auto& t : th) t.join();
+    for (auto& e : err)
+      if (e) std::rethrow_exception(e);
+    double best_cost = 0;
+    int pick = -1;
+    for (size_t q = 0; q < tries.size(); q++) {
+      const Attempt& at = ats[q];
+      if (!at.vgpr_class || at.vgpr_class > 256) continue;
+      const size_t cut = at.body.find("s_cbranch_vccz");
+      if (cut == std::string::npos) continue;
+      uint32_t first = 0;
+      for (size_t pos = 0; (pos = at.body.find('\t', pos)) < cut; pos++)
+        if (at.body.compare(pos, 7, "\ts_nop ") != 0) first++;
+      const double survive = 1.0 - std::pow(1.0 - std::ldexp(1.0, -(int)tries[q].L), 2048.0);
+      const double cost = first + survive * ((double)at.slots() - first);
+      if (pick < 0 || cost < best_cost) {
+        pick = (int)q;
+        best_cost = cost;
+      }
+    }
+    if (pick >= 0) {
+      best = std::move(ats[(size_t)pick]);
+      groups = tries[(size_t)pick].K;
+      exit_first = true;
+    }
+  }
+  if (!exit_first && !table && outs.size() == 1 && forced != 1 &&
+      (forced > 1 || unfit || (best.vgpr_class == 512 && c.n >= 34))) {
     const uint32_t ks[9] = {2, 3, 4, 5, 6, 8, 10, 12, 16};
     // For each K: the refined groups (fewer gates), then the greedy ones (smaller groups).
     // Once refined groups reach two waves, looser caps are tried too (from 2^36

@@ -234,7 +234,11 @@ def test_config1_body_with_group_exits():
     run every group."""
     c, key = ns.Circuit.gen_sat(2, 36, 20000, 40)
     body, cls, size = c.native_body(7)
-    assert cls == 256 and body.count("s_cbranch_vccz .Lns_exit") == 4
+    assert cls == 256 and body.count("s_cbranch_vccz .Lns_exit") >= 1
+    # exit-first groups: the first exit comes within the first third of the body
+    ins = [ln for ln in body.splitlines() if ln.startswith("\t")]
+    first = next(i for i, ln in enumerate(ins) if "s_cbranch_vccz" in ln)
+    assert first < len(ins) / 3, (first, len(ins))
     traces = []
     wus = [64 * 12345, 64 * 999_999, (1 << 31) - 64, (key >> 5) & ~63, (798579149 >> 5) & ~63]
     check(c, 7, 0, wus, traces)
