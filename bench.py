@@ -29,7 +29,7 @@ Also reported on the same line:
                 answer compared with the oracle's (tests/golden/cfg4_answers.json).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-extra] [--no-serve]
-       python bench.py --only {sat,table,stop36,hashmap,qsat,serve,copy,batch}   (one part, for profiling runs;
+       python bench.py --only {sat,exits,table,stop36,hashmap,qsat,serve,copy,batch}   (one part, for profiling runs;
        `copy` times a plain 1 GiB device copy used to calibrate the HBM byte counters)
 """
 import argparse
@@ -482,7 +482,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=4096)
     ap.add_argument("--serve-workers", type=int, default=8)
     ap.add_argument("--serve-batch", type=int, default=32)
-    ap.add_argument("--only", choices=["sat", "table", "stop36", "hashmap", "qsat", "serve", "copy", "batch"])
+    ap.add_argument("--only", choices=["sat", "exits", "table", "stop36", "hashmap", "qsat", "serve", "copy",
+                                       "batch"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -503,6 +504,10 @@ def main():
             ctx = ns.Context(0)
             if args.only == "sat":
                 part = sat_part(ns, ctx, args.steps, args.warmup)[2]
+            elif args.only == "exits":  # the full evaluation, then the same with group exits
+                c, _, sat = sat_part(ns, ctx, args.steps, args.warmup)
+                part = group_exits_part(ns, ctx, c, sat["ms_per_step"],
+                                        (sat["config"]["count"], sat["config"]["lowest"]), args.steps)
             elif args.only == "table":
                 part = table_part(ns, ctx)
             elif args.only == "hashmap":
