@@ -29,7 +29,7 @@ Also reported on the same line:
                 answer compared with the oracle's (tests/golden/cfg4_answers.json).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-extra] [--no-serve]
-       python bench.py --only {sat,exits,table,stop36,hashmap,qsat,serve,copy,batch}   (one part, for profiling runs;
+       python bench.py --only {sat,exits,factor,table,stop36,hashmap,qsat,serve,copy,batch}   (one part, for profiling runs;
        `copy` times a plain 1 GiB device copy used to calibrate the HBM byte counters)
 """
 import argparse
@@ -156,6 +156,43 @@ def group_exits_part(ns, ctx, c, full_ms_per_step, full_answer, searches=3):
             "assignments_per_s": (1 << 36) / (ms / 1000.0),
             "count": r["count"], "lowest": r["lowest"],
             "answers_equal": (r["count"], r["lowest"]) == full_answer}
+
+
+REFERENCE_FACTOR_ANSWER = "000000000000000110111010101000000000000100110011100001"
+
+
+def factor_part(ns):
+    """The search of the reference's published naivesat runs (benchmarks.txt: 54-input
+    circuit for 69,690,421, answer printed above, 14.959 s wall with -e 32 on a Radeon Vega
+    56): the naivesat tool on the rebuilt circuit, a = inputs 0..26, b = 27..53, output
+    [a * b == 69,690,421] and a != 1 and b != 1, stopping at the first launch range of 2^32
+    assignments that holds a solution, with group exits (the tool's default) and with every
+    gate evaluated (-X). Wall time of the whole process, as `time naivesat` measures it."""
+    import subprocess
+    import tempfile
+
+    c = ns.Circuit.gen_factor(69690421, 27)
+    out = {"workload": "naivesat -e 32 on [a * b == 69690421], a != 1, b != 1 (54 inputs, "
+                       f"{c.arrays()['func'].size} gates, rebuilt: the reference's circuit file is not "
+                       "in the reference)",
+           "reference_published_s": 14.959, "reference_hardware": "Radeon Vega 56 (benchmarks.txt:15-19)"}
+    with tempfile.TemporaryDirectory() as d:
+        f = os.path.join(d, "prime_69690421.circuit")
+        with open(f, "w") as fh:
+            fh.write(c.text())
+        for name, extra in (("group_exits", []), ("every_gate", ["-X"])):
+            t0 = time.perf_counter()
+            r = subprocess.run([os.path.join(ns.BIN_DIR, "naivesat"), "-e", "32", "-t", "opencl:0", *extra, f],
+                               capture_output=True, text=True, timeout=300)
+            wall = time.perf_counter() - t0
+            lines = r.stdout.splitlines()
+            tline = [x for x in lines if x.startswith("Time: ")]
+            steps = [x for x in lines if x.startswith("Step: ")]
+            out[name] = {"wall_s": wall, "search_s": float(tline[-1].split()[1]) if tline else None,
+                         "launch_ranges": len(steps),
+                         "answer_equals_reference": f"Found Input: {REFERENCE_FACTOR_ANSWER}" in lines,
+                         "speedup_vs_reference_wall": 14.959 / wall}
+    return out
 
 
 def cpu_part(c, cone, threads):
@@ -482,8 +519,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=4096)
     ap.add_argument("--serve-workers", type=int, default=8)
     ap.add_argument("--serve-batch", type=int, default=32)
-    ap.add_argument("--only", choices=["sat", "exits", "table", "stop36", "hashmap", "qsat", "serve", "copy",
-                                       "batch"])
+    ap.add_argument("--only", choices=["sat", "exits", "factor", "table", "stop36", "hashmap", "qsat", "serve",
+                                       "copy", "batch"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -498,6 +535,8 @@ def main():
             part = copy_part()
         elif args.only == "batch":
             part = batch_repeat_part(ns)
+        elif args.only == "factor":
+            part = factor_part(ns)
         elif args.only == "serve":
             part = serve_part(ns, args.serve_requests, args.serve_workers, args.serve_batch)
         else:
@@ -532,6 +571,7 @@ def main():
     if not args.no_extra:
         extra = {"sat_group_exits": group_exits_part(ns, ctx, c, sat["ms_per_step"],
                                                      (sat["config"]["count"], sat["config"]["lowest"])),
+                 "reference_factoring_run": factor_part(ns),
                  "satostop_table": table_part(ns, ctx), "satostop_36": stop36_part(ns, ctx),
                  "satostop_hashmap": hashmap_part(ns, ctx), "qsat": qsat_part(ns, ctx)}
         extra["batch_repeat"] = batch_repeat_part(ns)

@@ -74,6 +74,11 @@ void ns_circuit_free(ns_circuit* c);
 int ns_gen_sat(uint64_t seed, uint32_t n, uint32_t gates, uint32_t word_bits, ns_circuit** out,
                uint64_t* key);
 int ns_gen_qsat(uint64_t seed, uint32_t n, uint32_t gates, uint32_t last, ns_circuit** out);
+/* Factoring: inputs a = x0..x(bits-1), b = x(bits)..x(2 bits - 1); output = [a * b ==
+ * product] and a != 1 and b != 1 (the kind of circuit the reference's benchmarks.txt runs
+ * naivesat on, prime_69690421.circuit: 54 inputs; that circuit itself is not in the
+ * reference). 2 <= bits <= 32, product < 2^(2 bits). */
+int ns_gen_factor(uint64_t product, uint32_t bits, ns_circuit** out);
 int ns_gen_satostop(uint64_t seed, uint32_t half, uint32_t rounds, uint32_t round_gates,
                     uint32_t unknowns, ns_circuit** out, uint64_t* target);
 

@@ -146,6 +146,7 @@ def load_library() -> C.CDLL:
         "ns_circuit_free": (None, [P]),
         "ns_gen_sat": (I, [U64, U32, U32, U32, C.POINTER(P), C.POINTER(U64)]),
         "ns_gen_qsat": (I, [U64, U32, U32, U32, C.POINTER(P)]),
+        "ns_gen_factor": (I, [U64, U32, C.POINTER(P)]),
         "ns_gen_satostop": (I, [U64, U32, U32, U32, U32, C.POINTER(P), C.POINTER(U64)]),
         "ns_ctx_create": (I, [I, C.POINTER(P)]),
         "ns_ctx_destroy": (None, [P]),
@@ -293,6 +294,14 @@ class Circuit:
         lib = load_library()
         h = C.c_void_p()
         _check(lib.ns_gen_qsat(seed, n, gates, last, C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def gen_factor(cls, product: int, bits: int):
+        """Factoring circuit: output = [a * b == product], a != 1, b != 1 for a = inputs
+        0..bits-1 and b = inputs bits..2 bits - 1."""
+        h = C.c_void_p()
+        _check(load_library().ns_gen_factor(product, bits, C.byref(h)))
         return cls(h.value)
 
     @classmethod

@@ -1222,9 +1222,12 @@ NativeKernel build_native(const Circuit& c, NativeKind kind, const std::vector<u
   };
   // each register class first without, then with two VGPRs of lane slots for uniform values
   // (the QSAT-B templates keep two more VGPRs for their own code: gen_templates.py QSATB_POOL_CUT)
+  // (the SAT templates, whose bodies may write vcc, end the r64 / r128 pools one register lower:
+  // gen_templates.py SAT_VCC_POOL_CUT)
   const uint32_t cut = kind == NativeKind::QSAT_B ? 2 : 0;
-  const Cls classes[8] = {{64, 55 - cut, 0, 0},    {64, 55 - cut, 0, 2},    {128, 119 - cut, 0, 0},
-                          {128, 119 - cut, 0, 2},  {256, 239 - cut, 0, 0},  {256, 239 - cut, 0, 2},
+  const uint32_t cs = cut + (kind == NativeKind::SAT ? 1 : 0);  // r64 / r128 only
+  const Cls classes[8] = {{64, 55 - cs, 0, 0},      {64, 55 - cs, 0, 2},     {128, 119 - cs, 0, 0},
+                          {128, 119 - cs, 0, 2},    {256, 239 - cut, 0, 0},  {256, 239 - cut, 0, 2},
                           {512, 239 - cut, 256, 0}, {512, 239 - cut, 256, 2}};
   struct Attempt {
     std::string body, why = "no register class tried";
@@ -1247,8 +1250,9 @@ NativeKernel build_native(const Circuit& c, NativeKind kind, const std::vector<u
     auto cone_cc = output_cone(cc, outs_cc);
     auto order_cc = schedule_gates(cc, cone_cc, outs_cc);
     Circuit cx = cc;  // with the scalar gates of split_uniform_pairs appended
-    // group exits test the groups' running conjunctions: those are materialised values
-    const bool exits = group_exits && kind == NativeKind::SAT && accs && !accs->empty();
+    // group exits test the groups' running conjunctions: those are materialised values. Only
+    // for an output that is the conjunction itself: a negated one is 1 where it is 0.
+    const bool exits = group_exits && kind == NativeKind::SAT && accs && !accs->empty() && !c.outputs.at(outs[0]).neg;
     std::vector<uint32_t> mapped = out_wires;
     if (exits)
       for (const auto& a : *accs) mapped.push_back(a.first);
@@ -1287,7 +1291,7 @@ NativeKernel build_native(const Circuit& c, NativeKind kind, const std::vector<u
       try {
         Allocator a(cx, p, vbit, lane_shift, lm, pool_lo, cl.hi, cl.agprs, cl.slots, out_wires, out_neg,
                     kind == NativeKind::SAT ? kSgprLoSat : kind == NativeKind::BATCH ? kSgprLoBatch : kSgprLo,
-                    nullptr, nullptr, exits && cl.budget >= 256 ? accs : nullptr);  // vcc: r256, r512
+                    nullptr, nullptr, exits ? accs : nullptr);
         at.body = a.run();
         at.vgpr_class = cl.budget;
         at.spills = a.spills();
@@ -1325,8 +1329,8 @@ NativeKernel build_native(const Circuit& c, NativeKind kind, const std::vector<u
   // assignments) or fits no class at all (any size: the alternative is the compiler path).
   const bool unfit = best.vgpr_class == 0;
   bool exit_first = false;
-  if (group_exits && kind == NativeKind::SAT && outs.size() == 1 && forced < 0 &&
-      (unfit || (best.vgpr_class == 512 && c.n >= 34))) {
+  if (group_exits && kind == NativeKind::SAT && outs.size() == 1 && !c.outputs.at(outs[0]).neg && forced < 0 &&
+      (unfit || c.n >= 34)) {
     // Group exits: the first group takes the literals whose cones are cheapest together, so
     // that most waves leave after it. Expected issue slots per word = the slots before the
     // first exit + the chance that a 2,048-assignment word survives the first group's L
@@ -1355,7 +1359,9 @@ NativeKernel build_native(const Circuit& c, NativeKind kind, const std::vector<u
     for (auto& t : th) t.join();
     for (auto& e : err)
       if (e) std::rethrow_exception(e);
-    double best_cost = 0;
+    // against the program without groups (one wave per SIMD in the AGPR class: 1.45 times
+    // the slots, as below)
+    double best_cost = best.vgpr_class ? (double)best.slots() * (best.vgpr_class == 512 ? 1.45 : 1.0) : 0.0;
     int pick = -1;
     for (size_t q = 0; q < tries.size(); q++) {
       const Attempt& at = ats[q];
@@ -1367,7 +1373,7 @@ NativeKernel build_native(const Circuit& c, NativeKind kind, const std::vector<u
         if (at.body.compare(pos, 7, "\ts_nop ") != 0) first++;
       const double survive = 1.0 - std::pow(1.0 - std::ldexp(1.0, -(int)tries[q].L), 2048.0);
       const double cost = first + survive * ((double)at.slots() - first);
-      if (pick < 0 || cost < best_cost) {
+      if ((pick < 0 && !best.vgpr_class) || cost < best_cost) {
         pick = (int)q;
         best_cost = cost;
       }

@@ -209,6 +209,13 @@ int ns_gen_sat(uint64_t seed, uint32_t n, uint32_t gates, uint32_t word_bits, ns
   });
 }
 
+int ns_gen_factor(uint64_t product, uint32_t bits, ns_circuit** out) {
+  return guard([&] {
+    need(out, "null argument");
+    *out = new ns_circuit{ns::gen_factor_circuit(product, bits)};
+  });
+}
+
 int ns_gen_qsat(uint64_t seed, uint32_t n, uint32_t gates, uint32_t last, ns_circuit** out) {
   return guard([&] {
     need(out, "null argument");

@@ -378,6 +378,71 @@ Circuit gen_sat_circuit(uint64_t seed, uint32_t n, uint32_t gates, uint32_t wbit
   return lb.c;
 }
 
+Circuit gen_factor_circuit(uint64_t N, uint32_t bits) {
+  if (bits < 2 || 2 * bits > 64 || (2 * bits < 64 && (N >> (2 * bits)) != 0))
+    throw std::runtime_error("bad factoring generator parameters");
+  using Lit = std::pair<uint32_t, bool>;
+  LitBuilder lb(2 * bits);
+  auto op = [&](int o, Lit x, Lit y) -> Lit { return {lb.gate(o, x.first, x.second, y.first, y.second), false}; };
+  // partial products a_i & b_j in the column of weight i + j
+  std::vector<std::vector<Lit>> col(2 * bits + 1);
+  for (uint32_t i = 0; i < bits; i++)
+    for (uint32_t j = 0; j < bits; j++) col[i + j].push_back(op(0, {i, false}, {bits + j, false}));
+  // carry-save rounds: every column of three or more takes full adders (sum in the column,
+  // carry in the next) until each column holds at most two values
+  for (bool more = true; more;) {
+    more = false;
+    std::vector<std::vector<Lit>> nc(2 * bits + 1);
+    for (uint32_t w = 0; w < 2 * bits; w++) {
+      auto& v = col[w];
+      size_t k = 0;
+      for (; k + 3 <= v.size(); k += 3) {
+        const Lit t = op(2, v[k], v[k + 1]);
+        nc[w].push_back(op(2, t, v[k + 2]));
+        nc[w + 1].push_back(op(1, op(0, v[k], v[k + 1]), op(0, t, v[k + 2])));
+      }
+      for (; k < v.size(); k++) nc[w].push_back(v[k]);
+    }
+    col.swap(nc);
+    for (uint32_t w = 0; w < 2 * bits; w++) more |= col[w].size() > 2;
+  }
+  // the two rows added with a ripple carry; the product has 2 * bits bits
+  std::vector<Lit> prod(2 * bits);
+  bool have_c = false;
+  Lit carry{0, false};
+  for (uint32_t w = 0; w < 2 * bits; w++) {
+    std::vector<Lit> v = col[w];
+    if (have_c) v.push_back(carry);
+    if (v.empty()) {  // a column with nothing in it (weight 2 * bits - 1 for 2 bits): zero
+      prod[w] = op(2, {0, false}, {0, false});
+      have_c = false;
+    } else if (v.size() == 1) {
+      prod[w] = v[0];
+      have_c = false;
+    } else if (v.size() == 2) {
+      prod[w] = op(2, v[0], v[1]);
+      carry = op(0, v[0], v[1]);
+      have_c = true;
+    } else {
+      const Lit t = op(2, v[0], v[1]);
+      prod[w] = op(2, t, v[2]);
+      carry = op(1, op(0, v[0], v[1]), op(0, t, v[2]));
+      have_c = true;
+    }
+  }
+  // [a * b == N] and a != 1 and b != 1 (an input of ones is not a factorisation)
+  std::vector<Lit> lits;
+  for (uint32_t w = 0; w < 2 * bits; w++) lits.push_back({prod[w].first, prod[w].second ^ (((N >> w) & 1) == 0)});
+  for (uint32_t f = 0; f < 2; f++) {
+    Lit any{f * bits, true};  // a != 1: ~a_0 or a_1 or ... or a_{bits-1}
+    for (uint32_t i = 1; i < bits; i++) any = op(1, any, {f * bits + i, false});
+    lits.push_back(any);
+  }
+  auto r = and_tree(lb, lits);
+  lb.output(r.first, r.second);
+  return lb.c;
+}
+
 Circuit gen_qsat_circuit(uint64_t seed, uint32_t n, uint32_t gates, uint32_t last) {
   if (n == 0 || n > 64 || last < 3 || last > gates) throw std::runtime_error("bad QSAT generator parameters");
   Rng rng(seed);

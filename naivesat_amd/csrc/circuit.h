@@ -106,6 +106,11 @@ std::vector<uint32_t> random_block(LitBuilder& lb, Rng& rng, const std::vector<u
 // single output = [W(x) == W(k)] for a key k drawn from the seed.
 Circuit gen_sat_circuit(uint64_t seed, uint32_t n, uint32_t gates, uint32_t wbits,
                         uint64_t* key);
+// Factoring workload (the kind of circuit the reference's published naivesat runs search,
+// benchmarks.txt): inputs a = x0..x(bits-1) and b = x(bits)..x(2 bits - 1), single output =
+// [a * b == N] and a != 1 and b != 1, over the full 2 bits-bit product (carry-save rounds of
+// full adders, then a ripple carry).
+Circuit gen_factor_circuit(uint64_t N, uint32_t bits);
 // QSAT workload: n inputs, `gates` random gates, output = XOR of 3 distinct gates
 // drawn from the last `last` gates.
 Circuit gen_qsat_circuit(uint64_t seed, uint32_t n, uint32_t gates, uint32_t last);

@@ -10,8 +10,8 @@ Register contract of every placeholder ("; NS_BODY <name>"):
                          the six lane bits
   outputs  v0 .. v[OUT-1] circuit outputs for the current word
   scratch  v[POOL_LO..POOL_HI], s[40..101], scc (clobbered); the SAT kernels, whose own code
-           keeps to s0..s13, also give the body s[14..19] and s[22..39], and the r256 / r512
-           ones vcc (with it clobbered the r64 / r128 ones' own code needs one VGPR more)
+           keeps to s0..s13, also give the body s[14..19] and s[22..39], and vcc (their r64 /
+           r128 pools end one register lower: with vcc clobbered their own code takes one more)
 Register classes bound the VGPR budget (and with it the occupancy).
 
 Usage: python gen_templates.py OUT.hip
@@ -29,6 +29,9 @@ SGPR_POOL = list(range(40, 102))
 # two registers lower, so that the kernels stay within their class (64 / 128 VGPRs)
 QSATB_POOL_CUT = 2
 SAT_SGPR_POOL = list(range(14, 20)) + list(range(22, 102))  # s20, s21 carry the word index
+# the SAT bodies may write vcc (group exits); with vcc clobbered the r64 / r128 SAT kernels'
+# own code takes one VGPR more, so their bodies' pool ends one register lower
+SAT_VCC_POOL_CUT = 1
 
 
 def clobbers(lo, hi, agprs=False, sgprs=SGPR_POOL, vcc=False):
@@ -37,7 +40,7 @@ def clobbers(lo, hi, agprs=False, sgprs=SGPR_POOL, vcc=False):
         regs += [f'"a{i}"' for i in range(256)]
     regs += [f'"s{i}"' for i in sgprs]
     regs.append('"scc"')
-    if vcc:  # the group exits of the r256 / r512 SAT bodies compare into vcc
+    if vcc:  # the group exits of the SAT bodies compare into vcc
         regs.append('"vcc"')
     return ", ".join(regs)
 
@@ -113,7 +116,7 @@ extern "C" __global__ void __launch_bounds__(256) {name}(u64 w0, u64 wend, u64* 
     const u32 wlo = (u32)wu, whi = (u32)(wu >> 32);
     // words wu + lane at or past wend are not counted: lane >= rem (uniform)
     const u64 rem = wend > wu ? wend - wu : 0ull;
-{body1(name, 1, 8, pool_hi, SAT_SGPR_POOL, vcc=vb >= 256)}    o &= valid;
+{body1(name, 1, 8, pool_hi if vb >= 256 else pool_hi - SAT_VCC_POOL_CUT, SAT_SGPR_POOL, vcc=True)}    o &= valid;
     if ((u64)lane >= rem) o = 0u;
     cnt += __popc(o);
     if (o != 0u && brel == ~0u) brel = (((j << 6) + lane) << 5) | (u32)__builtin_ctz(o);

@@ -50,6 +50,23 @@ def test_naivesat_group_exits_and_full_evaluation(tmp_path):
         assert f"Found Input: {lowest:026b}" in out, extra
 
 
+# The reference's published runs (benchmarks.txt:3-19) print this for prime_69690421.circuit
+# (54 inputs); that file is not in the reference, so the same search is rebuilt here as
+# [a * b == 69,690,421], a != 1, b != 1 over a = inputs 0..26, b = inputs 27..53.
+REFERENCE_FACTOR_ANSWER = "000000000000000110111010101000000000000100110011100001"
+
+
+@pytest.mark.parametrize("extra", [[], ["-X"]])
+def test_naivesat_prints_the_reference_factoring_answer(tmp_path, extra):
+    c = ns.Circuit.gen_factor(69690421, 27)
+    f = tmp_path / "prime_69690421.circuit"
+    f.write_text(c.text())
+    out = run("naivesat", "-e", "32", "-t", "opencl:0", *extra, f)
+    assert f"Found Input: {REFERENCE_FACTOR_ANSWER}" in out, out[-3:]
+    x = int(REFERENCE_FACTOR_ANSWER, 2)
+    assert (x & ((1 << 27) - 1)) * (x >> 27) == 69690421
+
+
 def test_naivesat_small_and_unsatisfiable(tmp_path):
     f = tmp_path / "small.circuit"
     # x0 & ~x1 & x2: lowest satisfying assignment is 0b101

@@ -208,6 +208,16 @@ def test_group_exits_config1_golden(ctx):
     assert (r["count"], r["lowest"]) == (g["count"], g["lowest"])
 
 
+def test_group_exits_with_a_negated_output(ctx):
+    # ~(conjunction of the literals): almost every assignment satisfies; no group exits
+    c, _ = ns.Circuit.gen_sat(2, 26, 40000, 48)
+    d = c.arrays()
+    neg = ns.Circuit.from_arrays(d["n"], d["in0"], d["in1"], d["func"], d["out_wire"], 1 - d["out_neg"])
+    want = oracle.OracleCircuit.from_dict(neg.arrays()).sat(threads=8)
+    r = ctx.sat_solve(neg, group_exits=True)
+    assert (r["count"], r["lowest"]) == want
+
+
 @pytest.mark.parametrize("groups", ["", "3"])
 def test_group_exits_keep_answers(ctx, monkeypatch, groups):
     """Group exits against the oracle: a 40,000-gate circuit whose output takes groups

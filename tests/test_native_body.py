@@ -246,6 +246,17 @@ def test_config1_body_with_group_exits():
     assert traces[3] == [".Lns_done"] and traces[4] == [".Lns_done"]  # solutions: every group
 
 
+def test_no_group_exits_for_a_negated_output():
+    # the output ~(conjunction) is 1 where the conjunction is 0: no exit may write 0 there
+    c, _ = ns.Circuit.gen_sat(2, 26, 40000, 48)
+    d = c.arrays()
+    assert "s_cbranch_vccz" in c.native_body(7)[0]
+    neg = ns.Circuit.from_arrays(d["n"], d["in0"], d["in1"], d["func"], d["out_wire"], 1 - d["out_neg"])
+    body, cls, _ = neg.native_body(7)
+    assert "s_cbranch_vccz" not in body
+    check(neg, 7, 0, [64 * 4321])
+
+
 def test_groups_when_one_group_fits_no_class():
     # 3,204 cone gates with ~800 live values fit no register class as one group (the
     # compiler path would take them); as output groups they fit the AGPR class
