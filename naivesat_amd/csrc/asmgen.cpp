@@ -872,6 +872,111 @@ std::vector<std::vector<size_t>> refine_groups(std::vector<std::vector<size_t>> 
 // after it. wgroup gets the group (1..K) of every new wire, so the schedule can keep groups
 // apart and only the accumulator (and kept values) live from one group into the next.
 // Returns false when the output has fewer than K literals.
+// Literal order of an exit-first group: starting empty, the literal whose cone adds the fewest
+// gates to the group's cone so far, `count` times. union_gates[i] = the cone after i + 1.
+std::vector<size_t> cheapest_union_order(const std::vector<std::vector<uint8_t>>& cone, size_t count,
+                                         std::vector<uint32_t>* union_gates = nullptr) {
+  const size_t G = cone.empty() ? 0 : cone[0].size();
+  std::vector<uint8_t> u(G, 0), used(cone.size(), 0);
+  std::vector<size_t> order;
+  uint32_t have = 0;
+  while (order.size() < count && order.size() < cone.size()) {
+    size_t pick = SIZE_MAX;
+    uint32_t pbest = 0;
+    for (size_t i = 0; i < cone.size(); i++) {
+      if (used[i]) continue;
+      uint32_t extra = 0;
+      for (size_t g = 0; g < G; g++) extra += cone[i][g] & (u[g] ^ 1);
+      if (pick == SIZE_MAX || extra < pbest) {
+        pick = i;
+        pbest = extra;
+      }
+    }
+    used[pick] = 1;
+    order.push_back(pick);
+    for (size_t g = 0; g < G; g++) u[g] |= cone[pick][g];
+    have += pbest;
+    if (union_gates) union_gates->push_back(have);
+  }
+  return order;
+}
+
+std::vector<std::vector<uint8_t>> literal_cones(const Circuit& c, const std::vector<std::pair<uint32_t, bool>>& lits) {
+  const uint32_t G = (uint32_t)c.gates.size();
+  std::vector<std::vector<uint8_t>> cone(lits.size());
+  for (size_t i = 0; i < lits.size(); i++) {
+    cone[i].assign(G, 0);
+    std::vector<uint32_t> st{lits[i].first};
+    while (!st.empty()) {
+      const uint32_t w = st.back();
+      st.pop_back();
+      if (w < c.n || cone[i][w - c.n]) continue;
+      cone[i][w - c.n] = 1;
+      st.push_back(c.gates[w - c.n].i0);
+      st.push_back(c.gates[w - c.n].i1);
+    }
+  }
+  return cone;
+}
+
+// The chance that a wave of the SAT search keeps an assignment standing after a first group
+// of the first L literals of `order`, for every L (survive[L]), measured on `samples` waves at
+// random places of the 2^n space: each wave's 2,048 assignments (words wu .. wu + 63, wu a
+// multiple of 64) evaluated on the host 64 at a time, bit-sliced.
+std::vector<double> sampled_survival(const Circuit& c, const std::vector<std::pair<uint32_t, bool>>& lits,
+                                     const std::vector<size_t>& order, uint32_t samples, uint64_t seed) {
+  const uint32_t n = c.n;
+  std::vector<uint8_t> need(c.wires(), 0);
+  for (const auto& l : lits) need[l.first] = 1;
+  for (size_t j = c.gates.size(); j-- > 0;)
+    if (need[n + j]) need[c.gates[j].i0] = need[c.gates[j].i1] = 1;
+  std::vector<uint32_t> cone_gates;
+  for (size_t j = 0; j < c.gates.size(); j++)
+    if (need[n + j]) cone_gates.push_back((uint32_t)j);
+  static const uint64_t kPat[6] = {0xAAAAAAAAAAAAAAAAull, 0xCCCCCCCCCCCCCCCCull, 0xF0F0F0F0F0F0F0F0ull,
+                                   0xFF00FF00FF00FF00ull, 0xFFFF0000FFFF0000ull, 0xFFFFFFFF00000000ull};
+  const uint64_t words = n >= 64 ? (1ull << 59) : (1ull << (n - 5));
+  uint64_t st = seed;
+  auto next = [&]() {  // SplitMix64
+    uint64_t z = (st += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+  };
+  std::vector<uint64_t> v(c.wires());
+  std::vector<uint32_t> alive(order.size() + 1, 0);
+  for (uint32_t s = 0; s < samples; s++) {
+    const uint64_t wu = (next() % words) & ~63ull;
+    std::vector<uint8_t> hit(order.size() + 1, 0);
+    for (uint32_t j = 0; j < 32; j++) {  // words wu + 2j, wu + 2j + 1: assignments base .. base + 63
+      const uint64_t base = (wu + 2 * j) << 5;
+      for (uint32_t k = 0; k < n; k++) v[k] = k < 6 ? kPat[k] : (((base >> k) & 1) ? ~0ull : 0ull);
+      for (uint32_t g : cone_gates) {
+        const uint64_t a = v[c.gates[g].i0], b = v[c.gates[g].i1];
+        uint64_t r;
+        switch (c.gates[g].func) {
+          case FUNC_AND: r = a & b; break;
+          case FUNC_NOR: r = ~(a | b); break;
+          case FUNC_NIMPL: r = a & ~b; break;
+          default: r = a ^ b; break;
+        }
+        v[n + g] = r;
+      }
+      uint64_t acc = ~0ull;
+      hit[0] = 1;
+      for (size_t L = 1; L <= order.size(); L++) {
+        const auto& l = lits[order[L - 1]];
+        acc &= l.second ? ~v[l.first] : v[l.first];
+        if (acc) hit[L] = 1;
+      }
+    }
+    for (size_t L = 0; L <= order.size(); L++) alive[L] += hit[L];
+  }
+  std::vector<double> out(order.size() + 1);
+  for (size_t L = 0; L <= order.size(); L++) out[L] = (double)alive[L] / samples;
+  return out;
+}
+
 bool grouped_circuit(const Circuit& c, uint32_t out_k, uint32_t K, Circuit& out, std::vector<uint32_t>& wgroup,
                      uint32_t refine_cap_pct, uint32_t pin = 0, const std::vector<uint8_t>* uni_in = nullptr,
                      uint32_t pin_u = 0, std::vector<std::pair<uint32_t, bool>>* accs = nullptr,
@@ -911,25 +1016,9 @@ bool grouped_circuit(const Circuit& c, uint32_t out_k, uint32_t K, Circuit& out,
     // Exit-first (for group exits): the first group takes the first_lits literals whose cones
     // add the fewest gates to it, one at a time; later groups run only in the waves where
     // the first one left some assignment standing, so their size matters little.
-    std::vector<uint8_t> u(G, 0);
-    std::vector<size_t> grp;
-    while (grp.size() < first_lits) {
-      size_t pick = SIZE_MAX;
-      uint32_t pbest = 0;
-      for (size_t i = 0; i < lits.size(); i++) {
-        if (used[i]) continue;
-        uint32_t extra = 0;
-        for (uint32_t g = 0; g < G; g++) extra += cone[i][g] & (u[g] ^ 1);
-        if (pick == SIZE_MAX || extra < pbest) {
-          pick = i;
-          pbest = extra;
-        }
-      }
-      grp.push_back(pick);
-      used[pick] = 1;
-      left--;
-      for (uint32_t g = 0; g < G; g++) u[g] |= cone[pick][g];
-    }
+    std::vector<size_t> grp = cheapest_union_order(cone, first_lits);
+    for (size_t i : grp) used[i] = 1;
+    left -= grp.size();
     groups.push_back(std::move(grp));
     rest_groups = K - 1;
   }
@@ -1112,6 +1201,13 @@ bool grouped_circuit(const Circuit& c, uint32_t out_k, uint32_t K, Circuit& out,
 // take 2,296 -> 2,253 issue slots per word, 4.96 -> 4.83 ms per launch (more of them no
 // longer fit the SGPRs and go to lane slots, whose v_readlane costs more than recomputing)
 constexpr uint32_t kKeptUniform = 16;
+
+// Weight of the issue slots after a group exit: a wave that keeps an assignment standing runs
+// on while the waves around it leave, so at less than two waves per SIMD. Fitted on the
+// 36-input, 20,000-gate search: 13 first literals took 22.2 ms, 16 took 19.7 ms, where equal
+// weights ranked 13 first; weighted 2, the 54-input factoring search's best size (20 of 56:
+// 164 ms, against 296 ms with 16 and 471 ms with 13) is ranked first too.
+constexpr double kSurvivorWeight = 2.0;
 
 const char* kind_name(NativeKind k) {
   switch (k) {
@@ -1333,14 +1429,44 @@ NativeKernel build_native(const Circuit& c, NativeKind kind, const std::vector<u
       (unfit || c.n >= 34)) {
     // Group exits: the first group takes the literals whose cones are cheapest together, so
     // that most waves leave after it. Expected issue slots per word = the slots before the
-    // first exit + the chance that a 2,048-assignment word survives the first group's L
-    // literals (1 - (1 - 2^-L)^2048, each literal taken as an even coin) x the slots after.
+    // first exit + the chance that a wave keeps an assignment standing after the first group
+    // x the slots after (weighted: kSurvivorWeight). That chance is measured on sampled
+    // waves (sampled_survival): the literals of a real circuit are far from independent coins
+    // (the low product bits of a multiplier fix the low bits of one factor given the other).
+    // First-group sizes are ranked by first-group cone gates + weighted chance x all cone
+    // gates, and the three sizes around the best are allocated with 2 to 6 groups.
     struct Try {
       uint32_t L, K;
     };
     std::vector<Try> tries;
-    for (uint32_t L : {12u, 14u, 16u, 18u, 20u, 24u})
-      for (uint32_t K : {2u, 3u, 4u, 5u}) tries.push_back({L, K});
+    std::vector<double> survive;
+    {
+      const auto lits = and_literals(c, c.outputs.at(outs[0]).wire);
+      if (lits.size() >= 3) {
+        const auto lc = literal_cones(c, lits);
+        std::vector<uint32_t> ug;
+        const auto order = cheapest_union_order(lc, std::min<size_t>(lits.size() - 1, 48), &ug);
+        survive = sampled_survival(c, lits, order, 512, 0x5eedull ^ c.gates.size());
+        std::vector<uint8_t> all(c.gates.size(), 0);
+        for (const auto& m : lc)
+          for (size_t g = 0; g < m.size(); g++) all[g] |= m[g];
+        double total = 0;
+        for (uint8_t b : all) total += b;
+        size_t bestL = 1;
+        double bp = 0;
+        for (size_t L = 1; L <= order.size(); L++) {
+          const double proxy = ug[L - 1] + kSurvivorWeight * survive[L] * total;
+          if (L == 1 || proxy < bp) {
+            bestL = L;
+            bp = proxy;
+          }
+        }
+        for (size_t L : {bestL > 2 ? bestL - 2 : 1, bestL, bestL + 2})
+          if (L >= 1 && L <= order.size())
+            for (uint32_t K : {2u, 3u, 4u, 5u, 6u})
+              if (L + K - 1 <= lits.size()) tries.push_back({(uint32_t)L, K});
+      }
+    }
     std::vector<Attempt> ats(tries.size());
     std::vector<std::exception_ptr> err(tries.size());
     std::vector<std::thread> th;
@@ -1371,8 +1497,7 @@ NativeKernel build_native(const Circuit& c, NativeKind kind, const std::vector<u
       uint32_t first = 0;
       for (size_t pos = 0; (pos = at.body.find('\t', pos)) < cut; pos++)
         if (at.body.compare(pos, 7, "\ts_nop ") != 0) first++;
-      const double survive = 1.0 - std::pow(1.0 - std::ldexp(1.0, -(int)tries[q].L), 2048.0);
-      const double cost = first + survive * ((double)at.slots() - first);
+      const double cost = first + kSurvivorWeight * survive[tries[q].L] * ((double)at.slots() - first);
       if ((pick < 0 && !best.vgpr_class) || cost < best_cost) {
         pick = (int)q;
         best_cost = cost;
