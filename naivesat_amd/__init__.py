@@ -200,7 +200,7 @@ def exported_symbols() -> list:
     """Names of the C-ABI entry points this binding declares."""
     return ["ns_last_error", "ns_version", "ns_circuit_parse", "ns_quant_circuit_parse",
             "ns_circuit_from_arrays", "ns_circuit_info", "ns_circuit_to_arrays", "ns_circuit_format",
-            "ns_circuit_eval", "ns_circuit_cone", "ns_circuit_free", "ns_gen_sat", "ns_gen_qsat",
+            "ns_circuit_eval", "ns_circuit_cone", "ns_circuit_free", "ns_gen_sat", "ns_gen_qsat", "ns_gen_factor",
             "ns_gen_satostop", "ns_ctx_create", "ns_ctx_destroy", "ns_sat_solve", "ns_qsat_solve",
             "ns_satostop_solve", "ns_table_build", "ns_satostop_pass_times", "ns_native_body", "ns_serve", "ns_native_encode",
             "ns_native_code", "ns_satostop_solve_wide", "ns_satostop_hashmap_solve", "ns_hashmap_step",
