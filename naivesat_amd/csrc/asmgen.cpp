@@ -1442,7 +1442,9 @@ NativeKernel build_native(const Circuit& c, NativeKind kind, const std::vector<u
     std::vector<double> survive;
     {
       const auto lits = and_literals(c, c.outputs.at(outs[0]).wire);
-      if (lits.size() >= 3) {
+      // the same bounds as grouped_circuit's: the cone marks take L G bytes, the order ~L^2 G steps
+      const uint64_t nl = lits.size(), ng = c.gates.size();
+      if (nl >= 3 && nl * ng <= (1ull << 27) && nl * nl * ng <= (1ull << 32)) {
         const auto lc = literal_cones(c, lits);
         std::vector<uint32_t> ug;
         const auto order = cheapest_union_order(lc, std::min<size_t>(lits.size() - 1, 48), &ug);
