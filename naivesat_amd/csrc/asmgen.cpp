@@ -22,6 +22,8 @@ namespace ns {
 namespace {
 
 const uint32_t kWordMasks[5] = {0xAAAAAAAAu, 0xCCCCCCCCu, 0xF0F0F0F0u, 0xFF00FF00u, 0xFFFF0000u};
+// future uses of a new value whose other VGPR sources steer the bank of its register
+constexpr size_t kBankLookahead = 8;
 // labels of the group exits (one body per module or code object region: unique there)
 const char* const kExitLabel = ".Lns_exit";
 const char* const kDoneLabel = ".Lns_done";
@@ -394,12 +396,13 @@ class Allocator {
   int alloc_v(uint32_t owner) {
     if (vfree_[0].empty() && vfree_[1].empty() && vfree_[2].empty() && vfree_[3].empty()) spill_one();
     int pen[4] = {0, 0, 0, 0};
-    if (cur_ < end_ && p_.ops[cur_].lut)
+    // a temporary copy (no owner) is a source of the current op; a destination is not
+    if (owner == UINT32_MAX && cur_ < end_ && p_.ops[cur_].lut)
       for (uint32_t s : p_.ops[cur_].src)
-        if (s != owner && bank_of(s) >= 0) pen[bank_of(s)] += 8;
+        if (bank_of(s) >= 0) pen[bank_of(s)] += 8;
     if (owner != UINT32_MAX) {
       const auto& u = uses_[owner];
-      for (size_t k = ptr_[owner], m = 0; k < u.size() && m < 3; k++, m++) {
+      for (size_t k = ptr_[owner], m = 0; k < u.size() && m < kBankLookahead; k++, m++) {
         if (u[k] >= end_ || u[k] == cur_) continue;
         const LutOp& op = p_.ops[u[k]];
         if (!op.lut) continue;
