@@ -95,8 +95,69 @@ uint32_t vop3(uint32_t op, uint32_t vdst, uint32_t s0, uint32_t s1, uint32_t s2,
 
 }  // namespace
 
+namespace {
+
+// "v<N>" or "s<N>" at p (after spaces), then spaces and `end_ch` (',' or the end): the 9-bit
+// VALU source code (SGPR N: N, VGPR N: 256 + N); false when the text is anything else
+bool fast_reg(const char*& p, const char* e, char end_ch, uint32_t* code, bool vgpr_only) {
+  while (p < e && (*p == ' ' || *p == '\t')) p++;
+  if (p >= e || (*p != 'v' && *p != 's') || (vgpr_only && *p != 'v')) return false;
+  const bool v = *p == 'v';
+  p++;
+  uint32_t n = 0;
+  const char* d = p;
+  while (p < e && *p >= '0' && *p <= '9' && p - d < 4) n = n * 10 + (uint32_t)(*p++ - '0');
+  if (p == d || (p < e && *p >= '0' && *p <= '9')) return false;
+  if (v ? n > 255 : n > 101) return false;
+  while (p < e && (*p == ' ' || *p == '\t')) p++;
+  if (end_ch) {
+    if (p >= e || *p != end_ch) return false;
+    p++;
+  }
+  *code = v ? 256 + n : n;
+  return true;
+}
+
+// "v_bitop3_b32 vD, a, b, c bitop3:T" with register operands, the bulk of every body, parsed
+// without building strings; false sends the line to the general parser (and its errors)
+bool fast_bitop3(const char* b, const char* e, std::vector<uint32_t>& w) {
+  static const char kMn[] = "v_bitop3_b32";
+  const size_t mn = sizeof(kMn) - 1;
+  if ((size_t)(e - b) <= mn || std::memcmp(b, kMn, mn) != 0 || (b[mn] != ' ' && b[mn] != '\t')) return false;
+  const char* p = b + mn;
+  uint32_t d, s0, s1, s2;
+  if (!fast_reg(p, e, ',', &d, true) || !fast_reg(p, e, ',', &s0, false) || !fast_reg(p, e, ',', &s1, false) ||
+      !fast_reg(p, e, 0, &s2, false))
+    return false;
+  static const char kMod[] = "bitop3:";
+  const size_t ml = sizeof(kMod) - 1;
+  if ((size_t)(e - p) <= ml || std::memcmp(p, kMod, ml) != 0) return false;
+  p += ml;
+  uint32_t tt = 0;
+  if (e - p > 2 && p[0] == '0' && (p[1] == 'x' || p[1] == 'X')) {
+    p += 2;
+    const char* h = p;
+    for (; p < e; p++) {
+      const char c = *p;
+      const int dig = c >= '0' && c <= '9' ? c - '0' : c >= 'a' && c <= 'f' ? c - 'a' + 10 : c >= 'A' && c <= 'F' ? c - 'A' + 10 : -1;
+      if (dig < 0 || tt > 0xFF) break;
+      tt = tt * 16 + (uint32_t)dig;
+    }
+    if (p == h) return false;
+  } else {
+    return false;
+  }
+  while (p < e && (*p == ' ' || *p == '\t' || *p == '\r')) p++;
+  if (p != e || tt > 255) return false;
+  uint32_t w1;
+  w.push_back(vop3(0x234, d - 256, s0, s1, s2, &w1, tt));
+  w.push_back(w1);
+  return true;
+}
+
+}  // namespace
+
 bool encode_body(const std::string& body, std::vector<uint32_t>& w, std::string* err) {
-  std::istringstream in(body);
   std::string line;
   auto fail = [&](const std::string& m) {
     if (err) *err = m + ": " + line;
@@ -106,7 +167,16 @@ bool encode_body(const std::string& body, std::vector<uint32_t>& w, std::string*
   // holds the signed distance in words from the instruction after it
   std::map<std::string, size_t> labels;
   std::vector<std::pair<size_t, std::string>> fixups;
-  while (std::getline(in, line)) {
+  w.reserve(w.size() + body.size() / 12);
+  for (size_t pos = 0; pos < body.size();) {
+    size_t eol = body.find('\n', pos);
+    if (eol == std::string::npos) eol = body.size();
+    const char* lb = body.data() + pos;
+    const char* le = body.data() + eol;
+    pos = eol + 1;
+    while (lb < le && (*lb == ' ' || *lb == '\t')) lb++;
+    if (lb < le && fast_bitop3(lb, le, w)) continue;
+    line.assign(lb, le);
     size_t a = line.find_first_not_of(" \t");
     if (a == std::string::npos || line[a] == ';') continue;
     std::string t = line.substr(a);

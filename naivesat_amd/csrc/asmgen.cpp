@@ -14,8 +14,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <exception>
+#include <set>
 #include <sstream>
 #include <thread>
+#include <tuple>
 
 namespace ns {
 
@@ -44,10 +46,57 @@ struct Loc {
 // wave-uniform values can be parked in lanes of the highest pool VGPRs (64 slots each)
 
 std::string hex(uint32_t v) {
-  std::ostringstream o;
-  o << "0x" << std::hex << v;
-  return o.str();
+  char buf[12];
+  char* e = buf + sizeof(buf);
+  char* p = e;
+  do {
+    *--p = "0123456789abcdef"[v & 15u];
+    v >>= 4;
+  } while (v);
+  *--p = 'x';
+  *--p = '0';
+  return std::string(p, e);
 }
+
+// Text of a body as it is built: appends without the stream machinery (a body is thousands
+// of short lines and is generated for every circuit served)
+class BodyText {
+ public:
+  BodyText() { s_.reserve(1 << 16); }
+  BodyText& operator<<(const char* x) {
+    s_ += x;
+    return *this;
+  }
+  BodyText& operator<<(const std::string& x) {
+    s_ += x;
+    return *this;
+  }
+  BodyText& operator<<(char c) {
+    s_ += c;
+    return *this;
+  }
+  BodyText& operator<<(long long v) {
+    char buf[24];
+    char* e = buf + sizeof(buf);
+    char* p = e;
+    const bool neg = v < 0;
+    unsigned long long u = neg ? 0ull - (unsigned long long)v : (unsigned long long)v;
+    do {
+      *--p = (char)('0' + u % 10);
+      u /= 10;
+    } while (u);
+    if (neg) *--p = '-';
+    s_.append(p, e);
+    return *this;
+  }
+  BodyText& operator<<(int v) { return *this << (long long)v; }
+  BodyText& operator<<(unsigned v) { return *this << (long long)v; }
+  BodyText& operator<<(unsigned long v) { return *this << (long long)v; }
+  const std::string& str() const { return s_; }
+
+ private:
+  std::string s_;
+};
 
 // Linear allocation in evaluation order. VGPRs come from the pool; when it is exhausted
 // (and the template provides AGPRs) the resident value with the furthest next use is
@@ -466,7 +515,7 @@ class Allocator {
   std::vector<uint32_t> outreg_;
   const std::vector<uint32_t>& outw_;
   const std::vector<uint8_t>& outneg_;
-  std::ostringstream out_;
+  BodyText out_;
   uint32_t cur_ = 0, end_ = 0;
   uint32_t spills_ = 0, reloads_ = 0;
   std::vector<std::pair<uint32_t, bool>> exits_;
@@ -536,41 +585,62 @@ LutProgram pressure_order(const LutProgram& p, const std::vector<uint32_t>& out_
       }
     }
   };
-  std::vector<uint32_t> ready;
+  // The ready lane ops in an ordered set keyed (group, 1 - freed, index), freed = the op's
+  // distinct lane sources it is the last use of: the set's first element is the op the
+  // greedy rule takes. A source's remaining uses dropping to one bumps that last consumer.
+  std::vector<std::vector<uint32_t>> users(wires);
+  for (uint32_t i = 0; i < m; i++) {
+    const LutOp& op = p.ops[i];
+    for (int k = 0; k < nsrc(op); k++) {
+      const uint32_t s = op.src[k];
+      bool dup = false;
+      for (int t = 0; t < k; t++) dup |= op.src[t] == s;
+      if (!dup && prod[s] >= 0 && !p.uniform[s]) users[s].push_back(i);
+    }
+  }
+  auto group_of = [&](uint32_t i) { return wgroup ? (*wgroup)[p.ops[i].dst] : 0u; };
+  std::vector<int> freed(m, 0);
+  std::vector<uint8_t> in_ready(m, 0);
+  std::set<std::tuple<uint32_t, int, uint32_t>> ready;
+  auto make_ready = [&](uint32_t i) {
+    const LutOp& op = p.ops[i];
+    int f = 0;
+    for (int k = 0; k < nsrc(op); k++) {
+      const uint32_t s = op.src[k];
+      bool dup = false;
+      for (int t = 0; t < k; t++) dup |= op.src[t] == s;
+      if (!dup && prod[s] >= 0 && !p.uniform[s] && rem[s] == 1) f++;
+    }
+    freed[i] = f;
+    in_ready[i] = 1;
+    ready.insert({group_of(i), 1 - f, i});
+  };
+  // after an emitted op: lane sources left with one use make that user's choice cheaper
+  auto on_emitted = [&](uint32_t i) {
+    const LutOp& op = p.ops[i];
+    for (int k = 0; k < nsrc(op); k++) {
+      const uint32_t s = op.src[k];
+      bool dup = false;
+      for (int t = 0; t < k; t++) dup |= op.src[t] == s;
+      if (dup || prod[s] < 0 || p.uniform[s] || rem[s] != 1) continue;
+      for (uint32_t u : users[s])
+        if (!done[u] && in_ready[u]) {
+          ready.erase({group_of(u), 1 - freed[u], u});
+          freed[u]++;
+          ready.insert({group_of(u), 1 - freed[u], u});
+        }
+    }
+  };
   for (uint32_t i = 0; i < m; i++)
-    if (!p.uniform[p.ops[i].dst] && npred[i] == 0) ready.push_back(i);
+    if (!p.uniform[p.ops[i].dst] && npred[i] == 0) make_ready(i);
   while (!ready.empty()) {
-    size_t bi = 0;
-    int bcost = 0;
-    uint32_t gmin = 0;
-    if (wgroup) {
-      gmin = ~0u;
-      for (uint32_t r : ready) gmin = std::min(gmin, (*wgroup)[p.ops[r].dst]);
-    }
-    bool have = false;
-    for (size_t r = 0; r < ready.size(); r++) {
-      const LutOp& op = p.ops[ready[r]];
-      if (wgroup && (*wgroup)[op.dst] != gmin) continue;
-      int freed = 0;
-      for (int k = 0; k < nsrc(op); k++) {
-        uint32_t s = op.src[k];
-        bool dup = false;
-        for (int t = 0; t < k; t++) dup |= op.src[t] == s;
-        if (!dup && prod[s] >= 0 && !p.uniform[s] && rem[s] == 1) freed++;
-      }
-      int cost = 1 - freed;
-      if (!have || cost < bcost || (cost == bcost && ready[r] < ready[bi])) {
-        bi = r;
-        bcost = cost;
-        have = true;
-      }
-    }
-    uint32_t i = ready[bi];
-    ready[bi] = ready.back();
-    ready.pop_back();
+    const uint32_t i = std::get<2>(*ready.begin());
+    ready.erase(ready.begin());
+    in_ready[i] = 0;
     emit(i);
+    on_emitted(i);
     for (uint32_t j : succ[i])
-      if (--npred[j] == 0) ready.push_back(j);
+      if (--npred[j] == 0) make_ready(j);
   }
   for (uint32_t i = 0; i < m; i++)
     if (!done[i]) emit(i);
@@ -1338,7 +1408,8 @@ NativeKernel build_native(const Circuit& c, NativeKind kind, const std::vector<u
   };
   // LUT mapping, schedule and the smallest register class that holds the program of cc
   auto attempt = [&](const Circuit& cc, const std::vector<uint32_t>& outs_cc, const std::vector<uint32_t>* wgroup,
-                     const std::vector<std::pair<uint32_t, bool>>* accs = nullptr) {
+                     const std::vector<std::pair<uint32_t, bool>>* accs = nullptr,
+                     const std::vector<uint32_t>* order_in = nullptr) {
     Attempt at;
     std::vector<uint32_t> out_wires;
     std::vector<uint8_t> out_neg;
@@ -1346,8 +1417,7 @@ NativeKernel build_native(const Circuit& c, NativeKind kind, const std::vector<u
       out_wires.push_back(cc.outputs.at(k).wire);
       out_neg.push_back(cc.outputs.at(k).neg ? 1 : 0);
     }
-    auto cone_cc = output_cone(cc, outs_cc);
-    auto order_cc = schedule_gates(cc, cone_cc, outs_cc);
+    const std::vector<uint32_t> order_cc = order_in ? *order_in : schedule_gates(cc, output_cone(cc, outs_cc), outs_cc);
     Circuit cx = cc;  // with the scalar gates of split_uniform_pairs appended
     // group exits test the groups' running conjunctions: those are materialised values. Only
     // for an output that is the conjunction itself: a negated one is 1 where it is 0.
@@ -1408,7 +1478,7 @@ NativeKernel build_native(const Circuit& c, NativeKind kind, const std::vector<u
     for (size_t pos = 0; (pos = at.body.find("\tv_readlane", pos)) != std::string::npos; pos += 3) at.lane_reads++;
     return at;
   };
-  Attempt best = attempt(c, outs, nullptr);
+  Attempt best = attempt(c, outs, nullptr, nullptr, &order);  // the schedule computed above
   uint32_t groups = 1;
   // One wave per SIMD (the AGPR class) issues fewer VALU ops per cycle than two or more. A
   // single output that is a conjunction can be evaluated as K groups of its literals, one
