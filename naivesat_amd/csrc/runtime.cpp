@@ -38,6 +38,43 @@ JitModule::~JitModule() {
   if (mod) (void)hipModuleUnload(mod);
 }
 
+// Cache key of a kernel: 128 bits over the text the code depends on (kernel name and source,
+// assembly or body), as 32 hex digits, so that a cache entry does not keep its text (a body
+// runs to hundreds of KB). Two independent 64-bit lanes, 8 bytes a step.
+static std::string text_digest(std::initializer_list<const std::string*> parts) {
+  uint64_t h1 = 0x243f6a8885a308d3ull, h2 = 0x13198a2e03707344ull, len = 0;
+  auto rotl = [](uint64_t x, int r) { return (x << r) | (x >> (64 - r)); };
+  auto step = [&](uint64_t w) {
+    h1 = rotl(h1 ^ (w * 0x87c37b91114253d5ull), 31) * 0x4cf5ad432745937full;
+    h2 = rotl(h2 + (w * 0x9e3779b97f4a7c15ull), 29) * 0xc2b2ae3d27d4eb4full + 0x52dce729u;
+  };
+  for (const std::string* p : parts) {
+    const char* s = p->data();
+    size_t n = p->size(), i = 0;
+    for (; i + 8 <= n; i += 8) {
+      uint64_t w;
+      std::memcpy(&w, s + i, 8);
+      step(w);
+    }
+    uint64_t tail = 0;
+    std::memcpy(&tail, s + i, n - i);
+    step(tail ^ ((uint64_t)(n - i) << 56));
+    len += n;
+    step(len);  // part boundary
+  }
+  auto fin = [](uint64_t x) {
+    x ^= x >> 33;
+    x *= 0xff51afd7ed558ccdull;
+    x ^= x >> 33;
+    x *= 0xc4ceb9fe1a85ec53ull;
+    return x ^ (x >> 33);
+  };
+  char buf[33];
+  std::snprintf(buf, sizeof buf, "%016llx%016llx", (unsigned long long)fin(h1 ^ len),
+                (unsigned long long)fin(h2 + len));
+  return buf;
+}
+
 std::shared_ptr<JitModule> Jit::lookup(const std::string& key, double* compile_ms) {
   std::lock_guard<std::mutex> lk(mu_);
   auto it = cache_.find(key);
@@ -48,7 +85,8 @@ std::shared_ptr<JitModule> Jit::lookup(const std::string& key, double* compile_m
 }
 
 std::shared_ptr<JitModule> Jit::get(const std::string& src, const char* kernel, double* compile_ms) {
-  std::string key = std::string(kernel) + '\n' + src;
+  const std::string kname(kernel), tag("rtc");
+  std::string key = text_digest({&tag, &kname, &src});
   if (auto hit = lookup(key, compile_ms)) return hit;
   auto t0 = std::chrono::steady_clock::now();
   hiprtcProgram prog;
@@ -110,7 +148,8 @@ std::shared_ptr<JitModule> Jit::insert(const std::string& key, std::shared_ptr<J
 }
 
 std::shared_ptr<JitModule> Jit::get_asm(const std::string& asm_text, const std::string& kernel, double* compile_ms) {
-  std::string key = "asm:" + kernel + '\n' + asm_text;
+  const std::string tag("asm");
+  std::string key = text_digest({&tag, &kernel, &asm_text});
   if (auto hit = lookup(key, compile_ms)) return hit;
   auto t0 = std::chrono::steady_clock::now();
   std::vector<char> code = assemble_native(asm_text);
@@ -150,7 +189,8 @@ void Jit::put_built(const std::string& fp, const BuiltKernel& k) {
 
 std::shared_ptr<JitModule> Jit::get_native(const std::string& kernel, const std::string& body,
                                            const std::string& asm_text, double* compile_ms) {
-  std::string key = "nat:" + kernel + '\n' + body;
+  const std::string tag("nat");
+  std::string key = text_digest({&tag, &kernel, &body});
   if (auto hit = lookup(key, compile_ms)) return hit;
   auto t0 = std::chrono::steady_clock::now();
   std::vector<char> code = patch_native(kernel, body);
