@@ -29,9 +29,9 @@ constexpr size_t kBankLookahead = 8;
 // labels of the group exits (one body per module or code object region: unique there)
 const char* const kExitLabel = ".Lns_exit";
 const char* const kDoneLabel = ".Lns_done";
-// SGPRs of the body: s[40..101]; the SAT templates, whose own code keeps to s0..s13, also
-// give it s[14..19] and s[22..39] (s20 and s21 carry the word index)
-const uint32_t kSgprLoSat = 14, kSgprLo = 40, kSgprLoBatch = 48, kSgprHi = 101;
+// SGPRs of the body: s[40..101]; the SAT templates, whose own code keeps to s0..s2 across
+// the body, also give it s[3..19] and s[22..39] (s20 and s21 carry the word index)
+const uint32_t kSgprLoSat = 3, kSgprLo = 40, kSgprLoBatch = 48, kSgprHi = 101;
 // state tiers of the walk templates (templates/gen_templates.py WALK_TIERS)
 const uint32_t kWalkTiers[5] = {16, 24, 32, 40, 48};
 

@@ -10,8 +10,9 @@ Register contract of every placeholder ("; NS_BODY <name>"):
                          the six lane bits
   outputs  v0 .. v[OUT-1] circuit outputs for the current word
   scratch  v[POOL_LO..POOL_HI], s[40..101], scc (clobbered); the SAT kernels, whose own code
-           keeps to s0..s13, also give the body s[14..19] and s[22..39], and vcc (their r64 /
-           r128 pools end one register lower: with vcc clobbered their own code takes one more)
+           keeps to s0..s2 across the body, also give it s[3..19] and s[22..39], and vcc (their
+           r64 / r128 pools end one register lower: with vcc clobbered their own code takes one
+           more)
 Register classes bound the VGPR budget (and with it the occupancy).
 
 Usage: python gen_templates.py OUT.hip
@@ -28,7 +29,7 @@ SGPR_POOL = list(range(40, 102))
 # the QSAT-B templates' own code needs two VGPRs more than the others: their bodies' pool ends
 # two registers lower, so that the kernels stay within their class (64 / 128 VGPRs)
 QSATB_POOL_CUT = 2
-SAT_SGPR_POOL = list(range(14, 20)) + list(range(22, 102))  # s20, s21 carry the word index
+SAT_SGPR_POOL = list(range(3, 20)) + list(range(22, 102))  # s20, s21 carry the word index
 # the SAT bodies may write vcc (group exits); with vcc clobbered the r64 / r128 SAT kernels'
 # own code takes one VGPR more, so their bodies' pool ends one register lower
 SAT_VCC_POOL_CUT = 1
@@ -75,6 +76,13 @@ __device__ inline void ns_wave_stats(u64 cnt, u64 best, u64* stats) {
   }
 }
 
+// a wave-uniform value copied into a VGPR (the compiler would keep it in an SGPR)
+__device__ inline u32 ns_to_vgpr(u32 x) {
+  u32 r;
+  asm volatile("v_mov_b32 %0, %1" : "=v"(r) : "s"(x));
+  return r;
+}
+
 __device__ inline u32 ns_red(u32 x, u32 sh, u32 all) { return all ? (x & (x >> sh)) : (x | (x >> sh)); }
 
 // Exclusive prefix sum over the wave's lanes of r (0..63) and, in *total, the wave's sum: one
@@ -105,23 +113,43 @@ def body1(name, lm, lo, hi, sgprs=SGPR_POOL, vcc=False):
 def sat_kernel(cls, vb, pool_hi):
     name = f"ns_nat_sat_{cls}"
     return f'''
-// SAT: res[0] += satisfying assignments, res[1] = min(lowest); w = U + lane + (j << 6)
+// SAT: res[0] += satisfying assignments, res[1] = min(lowest); w = U + lane + (j << 6).
+// Across the body the loop keeps two scalars (its counter and the valid mask) besides the
+// word index in s20/s21, so that the body has the other SGPRs: the word limit is a per-lane
+// bound on j, and the values used after the loop wait in lanes of one VGPR.
 extern "C" __global__ void __launch_bounds__(256) {name}(u64 w0, u64 wend, u64* res, u32 wpl, u32 valid) {{
   NS_PROLOGUE
   u32 cnt = 0u;
   u32 brel = ~0u;  // lowest hit as ((word - U) << 5) | bit: one VGPR across the body
+  // words U + (j << 6) + lane at or past wend are not counted: j < jlim for this lane
+  const u64 first = U + lane;
+  const u64 left = wend > first ? wend - first : 0ull;
+  const u32 jlim = left >= ((u64)wpl << 6) ? wpl : (u32)((left + 63ull) >> 6);
+  u32 jlimrel = jlim << 11, jrel = lane << 5;  // (j << 11) + (lane << 5) against jlim << 11
+  // the loop counter and the valid mask as scalars of their own (a copy through a VGPR: a
+  // part of the kernel-argument registers would keep all of them live across the body)
+  const u32 vs = __builtin_amdgcn_readfirstlane(ns_to_vgpr(valid));
+  // U and res in lanes 0..3 of one VGPR across the loop, read back after it
+  u32 keep = 0u;
+  asm volatile("v_writelane_b32 %0, %1, 0\\n\\tv_writelane_b32 %0, %2, 1\\n\\t"
+               "v_writelane_b32 %0, %3, 2\\n\\tv_writelane_b32 %0, %4, 3"
+               : "+v"(keep) : "s"((u32)U), "s"((u32)(U >> 32)), "s"((u32)(u64)res), "s"((u32)((u64)res >> 32)));
+  u32 wlo = (u32)U, whi = (u32)(U >> 32);
 #pragma unroll 1
-  for (u32 j = 0; j < wpl; j++) {{
-    const u64 wu = U + ((u64)j << 6);
-    const u32 wlo = (u32)wu, whi = (u32)(wu >> 32);
-    // words wu + lane at or past wend are not counted: lane >= rem (uniform)
-    const u64 rem = wend > wu ? wend - wu : 0ull;
-{body1(name, 1, 8, pool_hi if vb >= 256 else pool_hi - SAT_VCC_POOL_CUT, SAT_SGPR_POOL, vcc=True)}    o &= valid;
-    if ((u64)lane >= rem) o = 0u;
+  for (u32 jj = __builtin_amdgcn_readfirstlane(ns_to_vgpr(wpl)); jj != 0u; jj--) {{
+{body1(name, 1, 8, pool_hi if vb >= 256 else pool_hi - SAT_VCC_POOL_CUT, SAT_SGPR_POOL, vcc=True)}    o &= vs;
+    if (jrel >= jlimrel) o = 0u;
     cnt += __popc(o);
-    if (o != 0u && brel == ~0u) brel = (((j << 6) + lane) << 5) | (u32)__builtin_ctz(o);
+    if (o != 0u && brel == ~0u) brel = jrel | (u32)__builtin_ctz(o);
+    jrel += 2048u;
+    const u64 wn = (((u64)whi << 32) | wlo) + 64ull;
+    wlo = (u32)wn;
+    whi = (u32)(wn >> 32);
   }}
-  ns_wave_stats(cnt, brel == ~0u ? ~0ull : (U << 5) + brel, res);
+  // (readlane returns int: each half goes through u32, so that the low half is not sign-extended)
+  const u64 Ub = ((u64)(u32)__builtin_amdgcn_readlane(keep, 1u) << 32) | (u32)__builtin_amdgcn_readlane(keep, 0u);
+  u64* resb = (u64*)(((u64)(u32)__builtin_amdgcn_readlane(keep, 3u) << 32) | (u32)__builtin_amdgcn_readlane(keep, 2u));
+  ns_wave_stats(cnt, brel == ~0u ? ~0ull : (Ub << 5) + brel, resb);
 }}
 '''
 
