@@ -35,6 +35,11 @@ bool parse_int(const std::string& t, int64_t* v) {
 }
 
 bool parse_operand(const std::string& t, Operand* op) {
+  if (t == "vcc_lo" || t == "vcc_hi") {  // SGPRs 106 and 107
+    op->kind = Operand::SGPR;
+    op->value = t == "vcc_lo" ? 106 : 107;
+    return true;
+  }
   if (t.size() >= 2 && (t[0] == 's' || t[0] == 'v' || t[0] == 'a') && t[1] >= '0' && t[1] <= '9') {
     int64_t r;
     if (!parse_int(t.substr(1), &r)) return false;

@@ -114,7 +114,7 @@ class Allocator {
             uint32_t lm_base, uint32_t pool_lo, uint32_t pool_hi, uint32_t agprs, uint32_t slot_regs,
             const std::vector<uint32_t>& out_wires, const std::vector<uint8_t>& out_neg, uint32_t sgpr_lo,
             const std::vector<int>* in_regs = nullptr, const std::vector<uint32_t>* out_regs = nullptr,
-            const std::vector<std::pair<uint32_t, bool>>* exits = nullptr)
+            const std::vector<std::pair<uint32_t, bool>>* exits = nullptr, bool vcc_regs = false)
       : c_(c), p_(p), vbit_(vbit), ls_(lane_shift), lm_(lm_base), pool_lo_(pool_lo), pool_hi_(pool_hi),
         sgpr_lo_(sgpr_lo),
         loc_(c.wires()), uses_(c.wires()), ptr_(c.wires(), 0), outw_(out_wires), outneg_(out_neg) {
@@ -124,8 +124,12 @@ class Allocator {
     pool_hi_ = pool_hi - slot_regs;
     for (int r = (int)pool_hi_; r >= (int)pool_lo; r--) vfree_[r & 3].push_back(r);
     for (int r = (int)(64 * slot_regs) - 1; r >= 0; r--) slotfree_.push_back(r);
+    // vcc_lo / vcc_hi (SGPRs 106, 107) as two more, taken last, when the template gives vcc
+    // and the body tests no exits
+    if (vcc_regs) sregs_ = {107, 106};
     for (int r = (int)kSgprHi; r >= (int)sgpr_lo_; r--)
-      if (r != 20 && r != 21) sfree_.push_back(r);
+      if (r != 20 && r != 21) sregs_.push_back(r);
+    sfree_ = sregs_;
     for (int r = (int)agprs - 1; r >= 0; r--) afree_.push_back(r);
     vowner_.assign(512, UINT32_MAX);
     end_ = (uint32_t)p.ops.size();
@@ -173,7 +177,7 @@ class Allocator {
         pinned_.clear();
         set_s(op.dst, d);
         static const char* names[4] = {"s_and_b32", "s_nor_b32", "s_andn2_b32", "s_xor_b32"};
-        out_ << "\t" << names[op.fn] << " s" << d << ", " << a << ", " << b << "\n";
+        out_ << "\t" << names[op.fn] << " " << sname(d) << ", " << a << ", " << b << "\n";
         if (uses_[op.dst].empty()) kill(op.dst);
         continue;
       }
@@ -283,6 +287,12 @@ class Allocator {
  private:
   bool is_pool_v(int r) const { return r >= (int)pool_lo_ && r <= (int)pool_hi_; }
 
+  static std::string sname(int r) {
+    if (r == 106) return "vcc_lo";
+    if (r == 107) return "vcc_hi";
+    return "s" + std::to_string(r);
+  }
+
   // an output may be computed straight into register r when r holds no value still needed
   bool output_reg_free(int r) const {
     const uint32_t w = vowner_[(size_t)r];
@@ -293,30 +303,30 @@ class Allocator {
   // parked in a VGPR lane
   std::string sreg(uint32_t w) {
     Loc& l = loc_[w];
-    if (l.s >= 0) return "s" + std::to_string(l.s);
+    if (l.s >= 0) return sname(l.s);
     if (l.sl >= 0) {
       int d = alloc_s();
       reload_slot(w, d);
       out_ << "\ts_nop 1\n";  // two wait states between the SGPR write and its read
-      return "s" + std::to_string(d);
+      return sname(d);
     }
     if (w >= c_.n) throw std::runtime_error("internal: uniform value used before definition");
     uint32_t vb = vbit_[w];
     int d = alloc_s();
     if (vb < 5) {
-      out_ << "\ts_mov_b32 s" << d << ", " << hex(kWordMasks[vb]) << "\n";
+      out_ << "\ts_mov_b32 " << sname(d) << ", " << hex(kWordMasks[vb]) << "\n";
     } else {
       uint32_t wb = vb - 5;  // bit of the word index, taken from the uniform part s20/s21
-      out_ << "\ts_bfe_i32 s" << d << ", " << (wb < 32 ? "s20" : "s21") << ", " << hex((1u << 16) | (wb & 31u))
+      out_ << "\ts_bfe_i32 " << sname(d) << ", " << (wb < 32 ? "s20" : "s21") << ", " << hex((1u << 16) | (wb & 31u))
            << "\n";
     }
     set_s(w, d);
-    return "s" + std::to_string(d);
+    return sname(d);
   }
 
   void reload_slot(uint32_t w, int d) {
     Loc& l = loc_[w];
-    out_ << "\tv_readlane_b32 s" << d << ", v" << (slot_base_ + l.sl / 64) << ", " << (l.sl % 64) << "\n";
+    out_ << "\tv_readlane_b32 " << sname(d) << ", v" << (slot_base_ + l.sl / 64) << ", " << (l.sl % 64) << "\n";
     set_s(w, d);
     reloads_++;
   }
@@ -397,8 +407,7 @@ class Allocator {
   void evict_sgpr() {
     uint32_t best = UINT32_MAX;
     uint64_t best_use = 0;
-    for (uint32_t r = sgpr_lo_; r <= kSgprHi; r++) {
-      if (r == 20 || r == 21) continue;
+    for (int r : sregs_) {
       const uint32_t w = sowner_[r];
       if (w == UINT32_MAX || loc_[w].dead) continue;
       bool pin = false;
@@ -422,7 +431,7 @@ class Allocator {
       if (slotfree_.empty()) throw NativeUnfit("out of SGPRs and lane slots for the native body");
       l.sl = slotfree_.back();
       slotfree_.pop_back();
-      out_ << "\tv_writelane_b32 v" << (slot_base_ + l.sl / 64) << ", s" << l.s << ", " << (l.sl % 64) << "\n";
+      out_ << "\tv_writelane_b32 v" << (slot_base_ + l.sl / 64) << ", " << sname(l.s) << ", " << (l.sl % 64) << "\n";
       spills_++;
     }
     drop_s(best);
@@ -507,6 +516,7 @@ class Allocator {
   std::vector<std::vector<uint32_t>> uses_;
   std::vector<uint32_t> ptr_;
   std::vector<int> vfree_[4], sfree_, afree_, slotfree_;
+  std::vector<int> sregs_;  // the body's SGPRs
   int slot_base_ = 0;
   std::vector<uint32_t> vowner_;
   std::vector<uint32_t> sowner_ = std::vector<uint32_t>(128, UINT32_MAX);
@@ -1460,7 +1470,7 @@ NativeKernel build_native(const Circuit& c, NativeKind kind, const std::vector<u
       try {
         Allocator a(cx, p, vbit, lane_shift, lm, pool_lo, cl.hi, cl.agprs, cl.slots, out_wires, out_neg,
                     kind == NativeKind::SAT ? kSgprLoSat : kind == NativeKind::BATCH ? kSgprLoBatch : kSgprLo,
-                    nullptr, nullptr, exits ? accs : nullptr);
+                    nullptr, nullptr, exits ? accs : nullptr, kind == NativeKind::SAT && !exits);
         at.body = a.run();
         at.vgpr_class = cl.budget;
         at.spills = a.spills();

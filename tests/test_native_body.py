@@ -22,15 +22,21 @@ def run_body(body, lm_base, wu, vregs=None, trace=None):
         for i in range(6):
             v[lm_base + i] = np.where((LANES >> i) & 1, np.uint32(M32), np.uint32(0)).astype(np.uint32)
 
+    def sidx(tok):  # SGPR number: sN, or vcc_lo / vcc_hi (106, 107)
+        return {"vcc_lo": 106, "vcc_hi": 107}.get(tok) or int(tok[1:])
+
+    def is_s(tok):
+        return tok.startswith("s") or tok in ("vcc_lo", "vcc_hi")
+
     def val(tok):
+        if is_s(tok):
+            return np.full(64, s[sidx(tok)], np.uint32)
         if tok.startswith("v"):
             return v[int(tok[1:])]
-        if tok.startswith("s"):
-            return np.full(64, s[int(tok[1:])], np.uint32)
         return np.full(64, int(tok, 0) & M32, np.uint32)  # constants (-1: all ones)
 
     def sval(tok):
-        return s[int(tok[1:])] if tok.startswith("s") else int(tok, 0) & M32
+        return s[sidx(tok)] if is_s(tok) else int(tok, 0) & M32
 
     lines = [ln.strip() for ln in body.splitlines()]
     lines = [ln for ln in lines if ln and not ln.startswith(";")]
@@ -58,16 +64,16 @@ def run_body(body, lm_base, wu, vregs=None, trace=None):
             if trace is not None:
                 trace.setdefault("branches", []).append(args[0])
         elif op == "s_mov_b32":
-            s[int(args[0][1:])] = sval(args[1])
+            s[sidx(args[0])] = sval(args[1])
         elif op == "s_bfe_i32":
             src, ctl = sval(args[1]), sval(args[2])
             off, width = ctl & 31, (ctl >> 16) & 0x7F
             assert width == 1
-            s[int(args[0][1:])] = M32 if (src >> off) & 1 else 0
+            s[sidx(args[0])] = M32 if (src >> off) & 1 else 0
         elif op in ("s_and_b32", "s_nor_b32", "s_andn2_b32", "s_xor_b32"):
             a, b = sval(args[1]), sval(args[2])
             r = {"s_and_b32": a & b, "s_nor_b32": ~(a | b), "s_andn2_b32": a & ~b, "s_xor_b32": a ^ b}[op]
-            s[int(args[0][1:])] = r & M32
+            s[sidx(args[0])] = r & M32
         elif op == "v_bitop3_b32":
             src2, tt = args[3].split()
             tt = int(tt.split(":")[1], 0)
@@ -91,7 +97,7 @@ def run_body(body, lm_base, wu, vregs=None, trace=None):
             v[reg] = v[reg].copy()
             v[reg][int(args[2])] = sval(args[1])
         elif op == "v_readlane_b32":
-            s[int(args[0][1:])] = int(v[int(args[1][1:])][int(args[2])])
+            s[sidx(args[0])] = int(v[int(args[1][1:])][int(args[2])])
         elif op == "s_nop":
             pass
         else:
