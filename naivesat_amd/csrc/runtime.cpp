@@ -132,7 +132,34 @@ std::shared_ptr<JitModule> Jit::get(const std::string& src, const char* kernel, 
   return insert(key, m);
 }
 
+void Jit::drop(std::shared_ptr<JitModule> m, std::vector<std::shared_ptr<JitModule>>& release) {
+  if (!m) return;
+  // only the last reference is parked (the module cache and the built kernels both hold one;
+  // dropping another holder's does not unload)
+  if (parking_ > 0 && m.use_count() == 1 && parked_.size() < kParkedMax) parked_.push_back(std::move(m));
+  else release.push_back(std::move(m));
+}
+
+void Jit::park(bool on) {
+  std::lock_guard<std::mutex> lk(mu_);
+  parking_ += on ? 1 : -1;
+}
+
+size_t Jit::reap(size_t n) {
+  std::vector<std::shared_ptr<JitModule>> release;
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    const size_t k = n == 0 || n > parked_.size() ? parked_.size() : n;
+    release.assign(std::make_move_iterator(parked_.end() - (long)k), std::make_move_iterator(parked_.end()));
+    parked_.resize(parked_.size() - k);
+  }
+  return release.size();  // released here, unlocked (a module another holder still uses stays loaded)
+}
+
 std::shared_ptr<JitModule> Jit::insert(const std::string& key, std::shared_ptr<JitModule> m) {
+  // dropped modules are released after the lock (declared before it, the vector is destroyed
+  // after it): an unload takes ~0.1-0.6 ms, and the lanes of a server share this cache
+  std::vector<std::shared_ptr<JitModule>> release;
   std::lock_guard<std::mutex> lk(mu_);
   auto it = cache_.find(key);
   if (it != cache_.end()) return it->second;  // compiled concurrently by another thread: keep the first
@@ -140,6 +167,7 @@ std::shared_ptr<JitModule> Jit::insert(const std::string& key, std::shared_ptr<J
     auto victim = cache_.begin();
     for (auto jt = cache_.begin(); jt != cache_.end(); ++jt)
       if (jt->second->last_use < victim->second->last_use) victim = jt;
+    drop(std::move(victim->second), release);
     cache_.erase(victim);
   }
   m->last_use = ++clock_;
@@ -177,11 +205,19 @@ bool Jit::find_built(const std::string& fp, BuiltKernel* out) {
 }
 
 void Jit::put_built(const std::string& fp, const BuiltKernel& k) {
+  std::vector<std::shared_ptr<JitModule>> release;  // see insert
   std::lock_guard<std::mutex> lk(mu_);
-  if (built_.size() >= kCapacity && !built_.count(fp)) {
+  auto it = built_.find(fp);
+  if (it != built_.end()) {
+    drop(std::move(it->second.first.mod), release);  // the replaced entry's reference
+    it->second = {k, ++clock_};
+    return;
+  }
+  if (built_.size() >= kCapacity) {
     auto victim = built_.begin();
-    for (auto it = built_.begin(); it != built_.end(); ++it)
-      if (it->second.second < victim->second.second) victim = it;
+    for (auto jt = built_.begin(); jt != built_.end(); ++jt)
+      if (jt->second.second < victim->second.second) victim = jt;
+    drop(std::move(victim->second.first.mod), release);
     built_.erase(victim);
   }
   built_[fp] = {k, ++clock_};

@@ -55,11 +55,15 @@ struct BuiltKernel {
 
 // Compiles generated sources once per context; keyed by a 128-bit digest of the source
 // text. Holds at most kCapacity modules: the least recently used is dropped (a solve still
-// running it keeps it alive through its reference). The capacity is large because dropping
-// modules is dear on this runtime: freeing a module's code memory costs about 0.5 ms, paid at
-// the unload or by the loads after it (a stream of distinct circuits with one unload per
-// load measured 0.8 ms per load against 0.25 ms without unloads). A module holds tens of KB
-// of device memory.
+// running it keeps it alive through its reference). Module loads and unloads on this runtime
+// set the sizes: a load costs ~0.2 ms plus ~0.55 us per module loaded in the process (4,000
+// distinct SAT solves in a row: 2.84 s with 256 modules kept, 3.51 s with 512, 6.90 s with
+// 4,096), and an unload while other threads load and launch slows them (8 serving lanes on
+// 4,096 distinct requests: ~1,500 solves/s with unloads during the run, ~3,100 without).
+// So the cache is small, and while a server is open (park()) the modules it drops are
+// parked instead of unloaded: its lanes unload them when they find no request waiting
+// (reap), and past kParkedMax parked modules a drop unloads at once.
+// A module holds tens of KB of device memory.
 class Jit {
  public:
   // HIP source compiled with hiprtc
@@ -74,16 +78,25 @@ class Jit {
   // the same circuit skips code generation as well as compilation
   bool find_built(const std::string& fp, BuiltKernel* out);
   void put_built(const std::string& fp, const BuiltKernel& k);
+  // an open server parks dropped modules (calls nest: one per open server)
+  void park(bool on);
+  // unloads up to n parked modules (all when n is 0); returns how many it unloaded
+  size_t reap(size_t n);
   ~Jit();
 
  private:
   std::shared_ptr<JitModule> lookup(const std::string& key, double* compile_ms);
   std::shared_ptr<JitModule> insert(const std::string& key, std::shared_ptr<JitModule> m);
-  static constexpr size_t kCapacity = 4096;
+  // under mu_: a dropped module reference, parked or handed back to be released unlocked
+  void drop(std::shared_ptr<JitModule> m, std::vector<std::shared_ptr<JitModule>>& release);
+  static constexpr size_t kCapacity = 256;
+  static constexpr size_t kParkedMax = 4096;
   std::mutex mu_;
   uint64_t clock_ = 0;
+  int parking_ = 0;
   std::unordered_map<std::string, std::shared_ptr<JitModule>> cache_;
   std::unordered_map<std::string, std::pair<BuiltKernel, uint64_t>> built_;  // value, last use
+  std::vector<std::shared_ptr<JitModule>> parked_;
 };
 
 // Assembles and links a gfx950 assembly module in-process; returns the code object.

@@ -99,6 +99,8 @@ Server::Server(Context& ctx, const ns_serve_opts& o, uint64_t budget_bytes) : ct
   arena_ = std::make_unique<Arena>(budget_bytes);
   for (uint32_t w = 0; w < workers; w++) lanes_.push_back(std::make_unique<Lane>(ctx.device, ctx.jit));
   if (o.batch_max > 1) batcher_ = std::make_unique<Batcher>(ctx, o, false);
+  ctx.jit.park(true);  // modules the cache drops wait for idle lanes (Jit::reap)
+  parking_ = true;
   t0_ = clk::now();
   for (uint32_t w = 0; w < workers; w++) threads_.emplace_back([this, w] { work(w); });
 }
@@ -182,6 +184,11 @@ void Server::close() {
   batcher_.reset();
   lanes_.clear();
   arena_.reset();
+  if (parking_) {
+    parking_ = false;
+    ctx_.jit.park(false);
+    ctx_.jit.reap(0);
+  }
 }
 
 // Under mu_. Admission in submission order: a request that needs arena memory and cannot
@@ -279,6 +286,15 @@ void Server::work(uint32_t wid) {
   std::vector<Pending> got;
   while (true) {
     uint64_t off = 0;
+    // the whole server idle (no request waiting, no lane at work): unload parked modules, a
+    // few at a time (an unload while other lanes load and launch slows them)
+    while (true) {
+      {
+        std::lock_guard<std::mutex> lk(mu_);
+        if (!queue_.empty() || running_ != 0 || closing_) break;
+      }
+      if (!ctx_.jit.reap(4)) break;
+    }
     {
       std::unique_lock<std::mutex> lk(mu_);
       cv_work_.wait(lk, [&] { return take(got, &off) || (closing_ && queue_.empty()); });

@@ -83,6 +83,7 @@ class Server {
   uint64_t next_ticket_ = 1;
   uint64_t running_ = 0;
   bool closing_ = false;
+  bool parking_ = false;  // the context's kernel cache parks dropped modules while this is open
   std::chrono::steady_clock::time_point t0_;
 };
 

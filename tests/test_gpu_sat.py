@@ -139,6 +139,21 @@ def test_graph_replay_matches_eager(ctx):
         assert (ro["count"], ro["lowest"]) == (ocount, olowest), i
 
 
+def test_more_circuits_than_the_kernel_cache(ctx):
+    # 300 distinct circuits against a cache of 256 modules: every answer is the oracle's, and
+    # the first circuit solved again after its kernel was dropped is built and answered anew
+    cs = [ns.Circuit.gen_sat(7000 + i, 16, 60, 16)[0] for i in range(300)]
+    want = [oracle.OracleCircuit.from_dict(c.arrays()).sat(threads=8) for c in cs]
+    for i, (c, (cnt, low)) in enumerate(zip(cs, want)):
+        r = ctx.sat_solve(c)
+        assert r["count"] == cnt, i
+        if cnt:
+            assert r["lowest"] == low, i
+    r = ctx.sat_solve(cs[0])
+    assert r["compile_ms"] > 0.0
+    assert r["count"] == want[0][0]
+
+
 def test_dense_solutions_count_exactly(ctx):
     # most waves of these searches hit: their counts spread over the device's count slots
     # must add up exactly, on the eager, tuning and graph-replay paths alike
