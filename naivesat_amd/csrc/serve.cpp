@@ -286,14 +286,15 @@ void Server::work(uint32_t wid) {
   std::vector<Pending> got;
   while (true) {
     uint64_t off = 0;
-    // the whole server idle (no request waiting, no lane at work): unload parked modules, a
-    // few at a time (an unload while other lanes load and launch slows them)
+    // the whole server idle (no request waiting, no lane at work): unload parked modules, one
+    // at a time, so that a request arriving meanwhile waits for one unload at most (an unload
+    // while other lanes load and launch slows them)
     while (true) {
       {
         std::lock_guard<std::mutex> lk(mu_);
         if (!queue_.empty() || running_ != 0 || closing_) break;
       }
-      if (!ctx_.jit.reap(4)) break;
+      if (!ctx_.jit.reap(1)) break;
     }
     {
       std::unique_lock<std::mutex> lk(mu_);
