@@ -29,9 +29,11 @@ constexpr size_t kBankLookahead = 8;
 // labels of the group exits (one body per module or code object region: unique there)
 const char* const kExitLabel = ".Lns_exit";
 const char* const kDoneLabel = ".Lns_done";
-// SGPRs of the body: s[40..101]; the SAT templates, whose own code keeps to s0..s2 across
-// the body, also give it s[3..19] and s[22..39] (s20 and s21 carry the word index)
-const uint32_t kSgprLoSat = 3, kSgprLo = 40, kSgprLoBatch = 48, kSgprHi = 101;
+// SGPRs of the body: s[40..101]; the SAT templates, whose own code keeps only s2 (its loop
+// counter) across the body, also give it s0, s1, s[3..19] and s[22..39] (s20 and s21 carry
+// the word index)
+const uint32_t kSgprLoSat = 0, kSgprLo = 40, kSgprLoBatch = 48, kSgprHi = 101;
+const uint32_t kSatLoopSgpr = 2;
 // state tiers of the walk templates (templates/gen_templates.py WALK_TIERS)
 const uint32_t kWalkTiers[5] = {16, 24, 32, 40, 48};
 
@@ -128,7 +130,7 @@ class Allocator {
     // and the body tests no exits
     if (vcc_regs) sregs_ = {107, 106};
     for (int r = (int)kSgprHi; r >= (int)sgpr_lo_; r--)
-      if (r != 20 && r != 21) sregs_.push_back(r);
+      if (r != 20 && r != 21 && !(sgpr_lo_ == kSgprLoSat && r == (int)kSatLoopSgpr)) sregs_.push_back(r);
     sfree_ = sregs_;
     for (int r = (int)agprs - 1; r >= 0; r--) afree_.push_back(r);
     vowner_.assign(512, UINT32_MAX);

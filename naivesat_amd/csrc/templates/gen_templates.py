@@ -10,7 +10,7 @@ Register contract of every placeholder ("; NS_BODY <name>"):
                          the six lane bits
   outputs  v0 .. v[OUT-1] circuit outputs for the current word
   scratch  v[POOL_LO..POOL_HI], s[40..101], scc (clobbered); the SAT kernels, whose own code
-           keeps to s0..s2 across the body, also give it s[3..19] and s[22..39], and vcc (their
+           keeps to s2 across the body, also give it s0, s1, s[3..19] and s[22..39], and vcc (their
            r64 / r128 pools end one register lower: with vcc clobbered their own code takes one
            more)
 Register classes bound the VGPR budget (and with it the occupancy).
@@ -29,7 +29,7 @@ SGPR_POOL = list(range(40, 102))
 # the QSAT-B templates' own code needs two VGPRs more than the others: their bodies' pool ends
 # two registers lower, so that the kernels stay within their class (64 / 128 VGPRs)
 QSATB_POOL_CUT = 2
-SAT_SGPR_POOL = list(range(3, 20)) + list(range(22, 102))  # s20, s21 carry the word index
+SAT_SGPR_POOL = [0, 1] + list(range(3, 20)) + list(range(22, 102))  # s20, s21: the word index
 # the SAT bodies may write vcc (group exits); with vcc clobbered the r64 / r128 SAT kernels'
 # own code takes one VGPR more, so their bodies' pool ends one register lower
 SAT_VCC_POOL_CUT = 1
@@ -114,9 +114,9 @@ def sat_kernel(cls, vb, pool_hi):
     name = f"ns_nat_sat_{cls}"
     return f'''
 // SAT: res[0] += satisfying assignments, res[1] = min(lowest); w = U + lane + (j << 6).
-// Across the body the loop keeps two scalars (its counter and the valid mask) besides the
-// word index in s20/s21, so that the body has the other SGPRs: the word limit is a per-lane
-// bound on j, and the values used after the loop wait in lanes of one VGPR.
+// Across the body the loop keeps one scalar (its counter, in s2) besides the word index in
+// s20/s21, so that the body has the other SGPRs: the word limit is a per-lane bound on j, the
+// valid mask a VGPR, and the values used after the loop wait in lanes of one VGPR.
 extern "C" __global__ void __launch_bounds__(256) {name}(u64 w0, u64 wend, u64* res, u32 wpl, u32 valid) {{
   NS_PROLOGUE
   u32 cnt = 0u;
@@ -126,9 +126,9 @@ extern "C" __global__ void __launch_bounds__(256) {name}(u64 w0, u64 wend, u64* 
   const u64 left = wend > first ? wend - first : 0ull;
   const u32 jlim = left >= ((u64)wpl << 6) ? wpl : (u32)((left + 63ull) >> 6);
   u32 jlimrel = jlim << 11, jrel = lane << 5;  // (j << 11) + (lane << 5) against jlim << 11
-  // the loop counter and the valid mask as scalars of their own (a copy through a VGPR: a
-  // part of the kernel-argument registers would keep all of them live across the body)
-  const u32 vs = __builtin_amdgcn_readfirstlane(ns_to_vgpr(valid));
+  // the valid mask in a VGPR, and the loop counter as a scalar of its own (a copy through a
+  // VGPR: a part of the kernel-argument registers would keep all of them live across the body)
+  const u32 vvalid = ns_to_vgpr(valid);
   // U and res in lanes 0..3 of one VGPR across the loop, read back after it
   u32 keep = 0u;
   asm volatile("v_writelane_b32 %0, %1, 0\\n\\tv_writelane_b32 %0, %2, 1\\n\\t"
@@ -137,7 +137,7 @@ extern "C" __global__ void __launch_bounds__(256) {name}(u64 w0, u64 wend, u64* 
   u32 wlo = (u32)U, whi = (u32)(U >> 32);
 #pragma unroll 1
   for (u32 jj = __builtin_amdgcn_readfirstlane(ns_to_vgpr(wpl)); jj != 0u; jj--) {{
-{body1(name, 1, 8, pool_hi if vb >= 256 else pool_hi - SAT_VCC_POOL_CUT, SAT_SGPR_POOL, vcc=True)}    o &= vs;
+{body1(name, 1, 8, pool_hi if vb >= 256 else pool_hi - SAT_VCC_POOL_CUT, SAT_SGPR_POOL, vcc=True)}    o &= vvalid;
     if (jrel >= jlimrel) o = 0u;
     cnt += __popc(o);
     if (o != 0u && brel == ~0u) brel = jrel | (u32)__builtin_ctz(o);
