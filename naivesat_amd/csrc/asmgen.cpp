@@ -806,12 +806,12 @@ std::vector<std::pair<uint32_t, bool>> and_literals(const Circuit& c, uint32_t w
 }
 
 // The local search of grouped_circuit: groups of literal indices, clist[i] = the cone gates
-// of literal i (of G gates). One descent from the given groups, then two rounds of eight
+// of literal i (of G gates). One descent from the given groups, then `rounds` rounds of eight
 // perturbed descents from the best so far (seeded, run on threads of their own); the lowest
 // total is kept (the first on ties), so the result does not depend on timing.
 std::vector<std::vector<size_t>> refine_groups(std::vector<std::vector<size_t>> groups,
                                                const std::vector<std::vector<uint32_t>>& clist, uint32_t G,
-                                               uint32_t cap_pct) {
+                                               uint32_t cap_pct, int rounds = 2) {
   const size_t ng = groups.size();
   uint32_t cap = 0;
   {
@@ -924,8 +924,8 @@ std::vector<std::vector<size_t>> refine_groups(std::vector<std::vector<size_t>> 
     return t;
   };
   uint64_t best_total = descend(groups, 0, 0);
-  constexpr int kRounds = 2, kRestarts = 8;
-  for (int round = 0; round < kRounds; round++) {
+  constexpr int kRestarts = 8;
+  for (int round = 0; round < rounds; round++) {
     std::vector<std::vector<std::vector<size_t>>> tries(kRestarts, groups);
     std::vector<uint64_t> totals(kRestarts, 0);
     std::vector<std::thread> th;
@@ -1157,7 +1157,9 @@ bool grouped_circuit(const Circuit& c, uint32_t out_k, uint32_t K, Circuit& out,
         if (cone[i][g]) clist[i].push_back(g);
     if (first_lits) {  // the first group stays as chosen
       std::vector<std::vector<size_t>> tail(groups.begin() + 1, groups.end());
-      if (tail.size() >= 2) tail = refine_groups(tail, clist, G, refine_cap_pct);
+      // the later groups run only where a wave survives the first: one descent, no perturbed
+      // rounds (54-input factoring search: code generation 0.37 -> 0.20 s, the same body)
+      if (tail.size() >= 2) tail = refine_groups(tail, clist, G, refine_cap_pct, 0);
       tail.insert(tail.begin(), groups[0]);
       groups = std::move(tail);
     } else {
